@@ -1,0 +1,331 @@
+"""ctypes binding of the MI355X move-delta engine (include/bmatch.h, include/bm_host.h).
+
+The engine is the C ABI in ``brain-match_amd/lib/libbmatch.so``; this module
+is thin plumbing over it for the test suite, the benchmark and scripting.
+Arrays cross the boundary as contiguous numpy int32 buffers.  There is no
+fallback path: if the shared library is missing or a call fails, an
+exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libbmatch.so")
+BIN_DIR = os.path.join(_PKG, "bin")
+
+BM_ROW_FIRST_POSITIVE = 0
+BM_ROW_ARGMAX = 1
+BM_ROT3_ROTATIONS = 0
+BM_ROT3_ALL = 1
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+class BMError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where}: error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libbmatch.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} not built: run `make -C brain-match_amd`")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "bm_strerror": (C.c_char_p, [C.c_int]),
+        "bm_last_error": (C.c_char_p, []),
+        "bm_abi_version": (C.c_int, []),
+        "bm_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+        "bm_destroy": (C.c_int, [C.c_void_p]),
+        "bm_upload_graphs": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p, _i32p, C.c_int64,
+                                       _i32p, _i32p, _i32p, C.c_int64]),
+        "bm_num_vertices": (C.c_int32, [C.c_void_p]),
+        "bm_device_bytes": (C.c_int64, [C.c_void_p]),
+        "bm_set_mapping": (C.c_int, [C.c_void_p, _i32p]),
+        "bm_get_mapping": (C.c_int, [C.c_void_p, _i32p]),
+        "bm_score": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+        "bm_eval_swaps": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
+        "bm_eval_swap_row": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int,
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "bm_sweep_all_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]),
+        "bm_eval_rot3": (C.c_int, [C.c_void_p, _i32p, C.c_int64, C.c_int, _i32p, _i32p]),
+        "bm_eval_permset": (C.c_int, [C.c_void_p, C.c_int32, _i32p, C.c_int64, _i32p, _i32p]),
+        "bm_perm_from_rank": (C.c_int, [C.c_int32, C.c_int32, _i32p]),
+        "bm_eval_swaps_hillclimb": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
+        "bm_apply_swap": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+        "bm_apply_rot3": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+        "bm_apply_perm": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p]),
+        "bm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+        "bm_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "bm_reset_timing": (C.c_int, [C.c_void_p]),
+        # host layer
+        "bm_gen_powerlaw": (C.c_int, [C.c_uint64, C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_void_p]),
+        "bm_gen_chunglu": (C.c_int, [C.c_uint64, C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                     C.c_void_p]),
+        "bm_instance_free": (None, [C.c_void_p]),
+        "bm_gen_triples_2hop": (C.c_int, [C.c_uint64, C.c_void_p, C.c_int64, _i32p]),
+        "bm_gen_sets": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int64, _i32p]),
+        "bm_drive_first_improvement": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+        "bm_drive_row_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_int64, C.c_void_p]),
+        "bm_drive_global_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Every entry point of the C ABI that this binding declares."""
+    lib()
+    return sorted(n for n in dir(_lib) if n.startswith("bm_"))
+
+
+def _check(rc: int, where: str):
+    if rc != 0:
+        L = lib()
+        raise BMError(rc, where, L.bm_last_error().decode(errors="replace") or L.bm_strerror(rc).decode())
+
+
+def _i32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+class _Instance(C.Structure):
+    _fields_ = [("n", C.c_int32), ("e1", C.c_int64), ("e2", C.c_int64),
+                ("s1", C.POINTER(C.c_int32)), ("d1", C.POINTER(C.c_int32)), ("w1", C.POINTER(C.c_int32)),
+                ("s2", C.POINTER(C.c_int32)), ("d2", C.POINTER(C.c_int32)), ("w2", C.POINTER(C.c_int32)),
+                ("pi", C.POINTER(C.c_int32)), ("p0", C.POINTER(C.c_int32))]
+
+
+class _DriveStats(C.Structure):
+    _fields_ = [("sweeps", C.c_int64), ("rows", C.c_int64), ("accepted", C.c_int64), ("evals_rows", C.c_int64),
+                ("initial_score", C.c_int64), ("final_score", C.c_int64), ("mismatches", C.c_int64)]
+
+
+class Instance:
+    """A seeded synthetic graph pair (numpy copies of a bm_instance)."""
+
+    def __init__(self, n, g1, g2, pi, p0):
+        self.n = int(n)
+        self.g1 = tuple(_i32(a) for a in g1)  # (src, dst, w)
+        self.g2 = tuple(_i32(a) for a in g2)
+        self.pi = _i32(pi)
+        self.p0 = _i32(p0)
+
+    @classmethod
+    def _from_c(cls, inst: _Instance):
+        n = inst.n
+
+        def arr(p, m):
+            return np.ctypeslib.as_array(p, shape=(m,)).copy() if m > 0 else np.zeros(0, np.int32)
+
+        return cls(n, (arr(inst.s1, inst.e1), arr(inst.d1, inst.e1), arr(inst.w1, inst.e1)),
+                   (arr(inst.s2, inst.e2), arr(inst.d2, inst.e2), arr(inst.w2, inst.e2)),
+                   arr(inst.pi, n + 1), arr(inst.p0, n + 1))
+
+    @classmethod
+    def powerlaw(cls, seed, n, m, del_frac=0.10, shuffle_frac=1.0):
+        """Power-law out-degree pair (the plain CPU-runnable refinement workload)."""
+        L = lib()
+        inst = _Instance()
+        _check(L.bm_gen_powerlaw(seed, n, m, del_frac, shuffle_frac, C.byref(inst)), "bm_gen_powerlaw")
+        try:
+            return cls._from_c(inst)
+        finally:
+            L.bm_instance_free(C.byref(inst))
+
+    @classmethod
+    def chunglu(cls, seed, n, m, rewire_frac=0.15, reweight_frac=0.15, shuffle_frac=0.30):
+        """Chung-Lu pair with log-normal weights (the full-size sweep workload)."""
+        L = lib()
+        inst = _Instance()
+        _check(L.bm_gen_chunglu(seed, n, m, rewire_frac, reweight_frac, shuffle_frac, C.byref(inst)),
+               "bm_gen_chunglu")
+        try:
+            return cls._from_c(inst)
+        finally:
+            L.bm_instance_free(C.byref(inst))
+
+    def _as_c(self):
+        inst = _Instance()
+        inst.n = self.n
+        inst.e1 = len(self.g1[0])
+        inst.e2 = len(self.g2[0])
+        keep = []
+        for name, a in (("s1", self.g1[0]), ("d1", self.g1[1]), ("w1", self.g1[2]),
+                        ("s2", self.g2[0]), ("d2", self.g2[1]), ("w2", self.g2[2]),
+                        ("pi", self.pi), ("p0", self.p0)):
+            keep.append(a)
+            setattr(inst, name, a.ctypes.data_as(C.POINTER(C.c_int32)))
+        return inst, keep
+
+    def triples_2hop(self, seed, count):
+        out = np.zeros(3 * count, np.int32)
+        inst, keep = self._as_c()
+        _check(lib().bm_gen_triples_2hop(seed, C.byref(inst), count, out), "bm_gen_triples_2hop")
+        return out.reshape(count, 3)
+
+
+def gen_sets(seed, n, k, count):
+    out = np.zeros(k * count, np.int32)
+    _check(lib().bm_gen_sets(seed, n, k, count, out), "bm_gen_sets")
+    return out.reshape(count, k)
+
+
+def perm_from_rank(k, rank):
+    out = np.zeros(k, np.int32)
+    _check(lib().bm_perm_from_rank(k, rank, out), "bm_perm_from_rank")
+    return out
+
+
+class Engine:
+    """One device context holding one graph pair and its mapping."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        _check(self._L.bm_create(device, C.byref(h)), "bm_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def n(self):
+        return int(self._L.bm_num_vertices(self._h))
+
+    @property
+    def device_bytes(self):
+        return int(self._L.bm_device_bytes(self._h))
+
+    def upload(self, n, g1, g2):
+        s1, d1, w1 = (_i32(a) for a in g1)
+        s2, d2, w2 = (_i32(a) for a in g2)
+        _check(self._L.bm_upload_graphs(self._h, n, s1, d1, w1, len(s1), s2, d2, w2, len(s2)), "bm_upload_graphs")
+
+    def set_mapping(self, mapping):
+        _check(self._L.bm_set_mapping(self._h, _i32(mapping)), "bm_set_mapping")
+
+    def mapping(self):
+        out = np.zeros(self.n + 1, np.int32)
+        _check(self._L.bm_get_mapping(self._h, out), "bm_get_mapping")
+        return out
+
+    def score(self):
+        s = C.c_int64()
+        _check(self._L.bm_score(self._h, C.byref(s)), "bm_score")
+        return s.value
+
+    def eval_swaps(self, a, b):
+        a, b = _i32(a), _i32(b)
+        out = np.zeros(len(a), np.int32)
+        _check(self._L.bm_eval_swaps(self._h, a, b, len(a), out), "bm_eval_swaps")
+        return out
+
+    def eval_swaps_hillclimb(self, a, b):
+        a, b = _i32(a), _i32(b)
+        out = np.zeros(len(a), np.int32)
+        _check(self._L.bm_eval_swaps_hillclimb(self._h, a, b, len(a), out), "bm_eval_swaps_hillclimb")
+        return out
+
+    def eval_swap_row(self, a, lo, hi, mode=BM_ROW_FIRST_POSITIVE):
+        b, d = C.c_int32(), C.c_int32()
+        _check(self._L.bm_eval_swap_row(self._h, a, lo, hi, mode, C.byref(b), C.byref(d)), "bm_eval_swap_row")
+        return b.value, d.value
+
+    def sweep_all_pairs(self, rows=False, matrix=False):
+        a, b, d = C.c_int32(), C.c_int32(), C.c_int32()
+        n1 = self.n + 1
+        rb = np.zeros(n1, np.int32) if rows else None
+        rd = np.zeros(n1, np.int32) if rows else None
+        dm = np.zeros((n1, n1), np.int32) if matrix else None
+        ptr = lambda x: x.ctypes.data_as(C.c_void_p) if x is not None else None
+        _check(self._L.bm_sweep_all_pairs(self._h, C.byref(a), C.byref(b), C.byref(d), ptr(rb), ptr(rd), ptr(dm)),
+               "bm_sweep_all_pairs")
+        return {"best": (a.value, b.value, d.value), "row_best_b": rb, "row_best_delta": rd, "matrix": dm}
+
+    def eval_rot3(self, triples, mode=BM_ROT3_ROTATIONS):
+        t = _i32(triples).reshape(-1)
+        cnt = len(t) // 3
+        d = np.zeros(cnt, np.int32)
+        p = np.zeros(cnt, np.int32)
+        _check(self._L.bm_eval_rot3(self._h, t, cnt, mode, d, p), "bm_eval_rot3")
+        return d, p
+
+    def eval_permset(self, sets):
+        s = _i32(sets)
+        k = s.shape[1]
+        cnt = s.shape[0]
+        d = np.zeros(cnt, np.int32)
+        r = np.zeros(cnt, np.int32)
+        _check(self._L.bm_eval_permset(self._h, k, s.reshape(-1), cnt, d, r), "bm_eval_permset")
+        return d, r
+
+    def apply_swap(self, a, b):
+        _check(self._L.bm_apply_swap(self._h, a, b), "bm_apply_swap")
+
+    def apply_rot3(self, m1, m2, m3, p):
+        _check(self._L.bm_apply_rot3(self._h, m1, m2, m3, p), "bm_apply_rot3")
+
+    def apply_perm(self, sset, perm):
+        s, p = _i32(sset), _i32(perm)
+        _check(self._L.bm_apply_perm(self._h, len(s), s, p), "bm_apply_perm")
+
+    # --- timing
+    def set_timing(self, on=True):
+        _check(self._L.bm_set_timing(self._h, 1 if on else 0), "bm_set_timing")
+
+    def reset_timing(self):
+        _check(self._L.bm_reset_timing(self._h), "bm_reset_timing")
+
+    def kernel_time(self, name):
+        n, ms = C.c_int64(), C.c_double()
+        _check(self._L.bm_kernel_time(self._h, name.encode(), C.byref(n), C.byref(ms)), "bm_kernel_time")
+        return n.value, ms.value
+
+    # --- drivers
+    def _drive(self, fn, *args, acc_cap):
+        acc = np.zeros(3 * max(acc_cap, 1), np.int32)
+        st = _DriveStats()
+        _check(fn(self._h, *args, acc.ctypes.data_as(C.c_void_p), acc_cap, C.byref(st)), fn.__name__)
+        k = min(st.accepted, acc_cap)
+        return acc[: 3 * k].reshape(k, 3), {f: getattr(st, f) for f, _ in _DriveStats._fields_}
+
+    def first_improvement(self, rows_per_sweep=0, max_sweeps=0, verify=True, acc_cap=1 << 20):
+        return self._drive(self._L.bm_drive_first_improvement, rows_per_sweep, max_sweeps, int(verify), None, None,
+                           acc_cap=acc_cap)
+
+    def row_best(self, max_sweeps=0, verify=True, acc_cap=1 << 20):
+        return self._drive(self._L.bm_drive_row_best, max_sweeps, int(verify), None, None, acc_cap=acc_cap)
+
+    def global_best(self, max_steps=0, verify=True, acc_cap=1 << 20):
+        return self._drive(self._L.bm_drive_global_best, max_steps, int(verify), acc_cap=acc_cap)

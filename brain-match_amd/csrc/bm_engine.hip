@@ -1,0 +1,672 @@
+// Context, upload, mapping, score and move application of the MI355X
+// move-delta engine.  See bm_internal.h for the device data layout.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bm_internal.h"
+
+namespace bm {
+
+static thread_local char g_err[512];
+
+int set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return set_error(e == hipErrorOutOfMemory ? BM_ENOMEM : BM_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int launch_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return BM_OK;
+}
+
+int dev_alloc(bm_ctx *c, void **p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    c->dev_bytes += (int64_t)bytes;
+    return BM_OK;
+}
+
+void dev_free(bm_ctx *c, void **p, size_t bytes) {
+    if (*p) {
+        (void)hipFree(*p);
+        c->dev_bytes -= (int64_t)(bytes ? bytes : 16);
+    }
+    *p = nullptr;
+}
+
+int ensure_stage(bm_ctx *c, size_t bytes) {
+    if (c->stage_bytes >= bytes) return BM_OK;
+    if (c->stage) {
+        (void)hipFree(c->stage);
+        c->dev_bytes -= (int64_t)c->stage_bytes;
+        c->stage = nullptr;
+        c->stage_bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    int rc = dev_alloc(c, &c->stage, want);
+    if (rc) return rc;
+    c->stage_bytes = want;
+    return BM_OK;
+}
+
+static int kernel_index(bm_ctx *c, const char *name) {
+    for (size_t i = 0; i < c->kstats.size(); i++)
+        if (c->kstats[i].name == name) return (int)i;
+    c->kstats.push_back(KernelStat{name, 0, 0.0});
+    return (int)c->kstats.size() - 1;
+}
+
+static hipEvent_t take_event(bm_ctx *c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int timer_begin(bm_ctx *c, const char *name) {
+    if (!c->timing) return -1;
+    TimerRec r;
+    r.kernel = kernel_index(c, name);
+    r.start = take_event(c);
+    r.stop = take_event(c);
+    if (!r.start || !r.stop) return -1;
+    (void)hipEventRecord(r.start, c->stream);
+    c->pending.push_back(r);
+    return (int)c->pending.size() - 1;
+}
+
+void timer_end(bm_ctx *c, int token) {
+    if (token < 0 || !c->timing) return;
+    (void)hipEventRecord(c->pending[token].stop, c->stream);
+}
+
+static int resolve_timers(bm_ctx *c) {
+    if (c->pending.empty()) return BM_OK;
+    BM_HIP(hipStreamSynchronize(c->stream));
+    for (auto &r : c->pending) {
+        float ms = 0.f;
+        BM_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+        c->kstats[r.kernel].launches++;
+        c->kstats[r.kernel].ms += ms;
+        c->ev_pool.push_back(r.start);
+        c->ev_pool.push_back(r.stop);
+    }
+    c->pending.clear();
+    return BM_OK;
+}
+
+}  // namespace bm
+
+using namespace bm;
+
+// ---------------------------------------------------------------- kernels
+
+// W2[u][v] = w for the de-duplicated G2 lines (last line per pair wins).
+__global__ void k_scatter_w2(int16_t *__restrict__ W2, int ld, const int32_t *__restrict__ tri, int64_t m) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int u = tri[3 * i], v = tri[3 * i + 1], w = tri[3 * i + 2];
+    W2[(size_t)u * ld + v] = (int16_t)w;
+}
+
+// P[x][y] = W2[p(x)][p(y)], padding columns zero.  grid (ld/256, N1)
+__global__ void k_build_p(int16_t *__restrict__ P, const int16_t *__restrict__ W2, const int32_t *__restrict__ map,
+                          int N1, int ld) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x;
+    int x = blockIdx.y;
+    if (y >= ld) return;
+    int16_t v = 0;
+    if (y < N1) v = W2[(size_t)map[x] * ld + map[y]];
+    P[(size_t)x * ld + y] = v;
+}
+
+// PT[x][y] = P[y][x] for x < N1, y < ld (P rows >= N1 read as 0).
+// 64x64 tiles staged in LDS; block (64, 4).
+__global__ void k_transpose(int16_t *__restrict__ PT, const int16_t *__restrict__ P, int N1, int ld) {
+    __shared__ int16_t tile[64][66];
+    int bx = blockIdx.x * 64;  // P column block == PT row block
+    int by = blockIdx.y * 64;  // P row block == PT column block
+    int tx = threadIdx.x, ty = threadIdx.y;
+    for (int i = ty; i < 64; i += 4) {
+        int y = by + i, x = bx + tx;
+        tile[i][tx] = (y < N1 && x < ld) ? P[(size_t)y * ld + x] : (int16_t)0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        int x = bx + i, y = by + tx;
+        if (x < N1 && y < ld) PT[(size_t)x * ld + y] = tile[tx][i];
+    }
+}
+
+// diag[v] = sum_{e in comb(v)} min(w_e, Q[r_e][v]); one wave per vertex.
+__global__ __launch_bounds__(256) void k_diag(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
+                                              const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
+                                              const int16_t *__restrict__ Q, int N1, int ld) {
+    int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (v >= N1) return;
+    int s = 0;
+    for (int e = cptr[v] + lane; e < cptr[v + 1]; e += 64) s += min(cw[e], (int)Q[(size_t)cr[e] * ld + v]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) diag[v] = s;
+}
+
+// S_opt = sum over G1 lines (x->y, w) of min(w, P[x][y]).
+__global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ out, const int32_t *__restrict__ es,
+                                               const int32_t *__restrict__ ed, const int32_t *__restrict__ ew,
+                                               int64_t m, const int16_t *__restrict__ P, int ld) {
+    __shared__ long long part[4];
+    long long s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        s += min(ew[i], (int)P[(size_t)es[i] * ld + ed[i]]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = part[0] + part[1] + part[2] + part[3];
+        atomicAdd(out, (unsigned long long)t);
+    }
+}
+
+struct SetPerm {
+    int k;
+    int set[8];
+    int src[8];  // new row/col set[i] <- old row/col set[src[i]]
+};
+
+// Row part of a set permutation on both P and PT: thread per column.
+__global__ void k_perm_rows(int16_t *__restrict__ Q, int N1, int ld, SetPerm sp) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld) return;
+    for (int half = 0; half < 2; half++) {
+        int16_t *M = Q + (size_t)half * N1 * ld;
+        int16_t old[8];
+        for (int i = 0; i < sp.k; i++) old[i] = M[(size_t)sp.set[sp.src[i]] * ld + j];
+        for (int i = 0; i < sp.k; i++) M[(size_t)sp.set[i] * ld + j] = old[i];
+    }
+}
+
+// Column part: thread per row.
+__global__ void k_perm_cols(int16_t *__restrict__ Q, int N1, int ld, SetPerm sp) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 2 * N1) return;
+    int16_t *row = Q + (size_t)x * ld;
+    int16_t old[8];
+    for (int i = 0; i < sp.k; i++) old[i] = row[sp.set[sp.src[i]]];
+    for (int i = 0; i < sp.k; i++) row[sp.set[i]] = old[i];
+}
+
+__global__ void k_perm_map(int32_t *__restrict__ map, SetPerm sp) {
+    if (threadIdx.x != 0) return;
+    int old[8];
+    for (int i = 0; i < sp.k; i++) old[i] = map[sp.set[sp.src[i]]];
+    for (int i = 0; i < sp.k; i++) map[sp.set[i]] = old[i];
+}
+
+// ---------------------------------------------------------------- helpers
+
+namespace bm {
+
+int ensure_diag(bm_ctx *c) {
+    if (c->diag_valid) return BM_OK;
+    int blocks = (c->N1 + 3) / 4;
+    BM_LAUNCH(c, "diag", hipLaunchKernelGGL(k_diag, dim3(blocks), dim3(256), 0, c->stream, c->diag, c->cptr, c->cr,
+                                            c->cw, c->Q, c->N1, c->ld));
+    c->diag_valid = true;
+    return BM_OK;
+}
+
+int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of) {
+    SetPerm sp;
+    memset(&sp, 0, sizeof sp);
+    sp.k = k;
+    for (int i = 0; i < k; i++) {
+        sp.set[i] = set[i];
+        sp.src[i] = src_of[i];
+    }
+    BM_LAUNCH(c, "perm_rows",
+              hipLaunchKernelGGL(k_perm_rows, dim3((c->ld + 255) / 256), dim3(256), 0, c->stream, c->Q, c->N1, c->ld, sp));
+    BM_LAUNCH(c, "perm_cols", hipLaunchKernelGGL(k_perm_cols, dim3((2 * c->N1 + 255) / 256), dim3(256), 0, c->stream,
+                                                 c->Q, c->N1, c->ld, sp));
+    BM_LAUNCH(c, "perm_map", hipLaunchKernelGGL(k_perm_map, dim3(1), dim3(64), 0, c->stream, c->map, sp));
+    int32_t old[8];
+    for (int i = 0; i < k; i++) old[i] = c->hmap[set[src_of[i]]];
+    for (int i = 0; i < k; i++) c->hmap[set[i]] = old[i];
+    c->diag_valid = false;
+    return BM_OK;
+}
+
+}  // namespace bm
+
+template <typename T>
+static int upload(bm_ctx *c, T **dst, const std::vector<T> &src) {
+    int rc = dev_alloc(c, (void **)dst, src.size() * sizeof(T));
+    if (rc) return rc;
+    if (!src.empty()) BM_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return BM_OK;
+}
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" {
+
+const char *bm_strerror(int code) {
+    switch (code) {
+        case BM_OK: return "ok";
+        case BM_EINVAL: return "invalid argument";
+        case BM_ENOMEM: return "out of memory";
+        case BM_EHIP: return "HIP runtime error";
+        case BM_ESTATE: return "graphs or mapping not uploaded";
+        case BM_ENODEV: return "no such HIP device";
+        case BM_ERANGE: return "value out of range";
+        case BM_EIO: return "I/O error";
+        default: return "unknown error";
+    }
+}
+
+const char *bm_last_error(void) { return bm::g_err; }
+
+int bm_abi_version(void) { return BM_ABI_VERSION; }
+
+int bm_create(int device, bm_ctx **out) {
+    if (!out) return set_error(BM_EINVAL, "bm_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return set_error(BM_ENODEV, "no HIP device (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return set_error(BM_ENODEV, "device %d not in [0,%d)", device, ndev);
+    BM_HIP(hipSetDevice(device));
+    bm_ctx *c = new (std::nothrow) bm_ctx();
+    if (!c) return set_error(BM_ENOMEM, "bm_create: host allocation");
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    *out = c;
+    return BM_OK;
+}
+
+static void free_graph_buffers(bm_ctx *c) {
+    size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
+    dev_free(c, (void **)&c->W2, N1 * ld * 2);
+    dev_free(c, (void **)&c->Q, 2 * N1 * ld * 2);
+    dev_free(c, (void **)&c->map, N1 * 4);
+    dev_free(c, (void **)&c->cptr, (N1 + 1) * 4);
+    dev_free(c, (void **)&c->cr, (size_t)c->ncomb * 4);
+    dev_free(c, (void **)&c->cw, (size_t)c->ncomb * 4);
+    dev_free(c, (void **)&c->cdw, (size_t)c->ncomb * 4);
+    dev_free(c, (void **)&c->ddptr, (N1 + 1) * 4);
+    dev_free(c, (void **)&c->ddmid, N1 * 4);
+    dev_free(c, (void **)&c->ddr, (size_t)c->ndd * 4);
+    dev_free(c, (void **)&c->ddw, (size_t)c->ndd * 4);
+    dev_free(c, (void **)&c->esrc, (size_t)c->e1 * 4);
+    dev_free(c, (void **)&c->edst, (size_t)c->e1 * 4);
+    dev_free(c, (void **)&c->ew, (size_t)c->e1 * 4);
+    dev_free(c, (void **)&c->edw, (size_t)c->e1 * 4);
+    dev_free(c, (void **)&c->diag, N1 * 4);
+    dev_free(c, (void **)&c->X, N1 * (size_t)c->ldx * 4);
+    dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
+    c->graphs = c->mapped = c->diag_valid = false;
+}
+
+int bm_destroy(bm_ctx *c) {
+    if (!c) return BM_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_graph_buffers(c);
+    if (c->stage) (void)hipFree(c->stage);
+    for (auto &r : c->pending) {
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BM_OK;
+}
+
+int32_t bm_num_vertices(const bm_ctx *c) { return c ? c->n : 0; }
+int64_t bm_device_bytes(const bm_ctx *c) { return c ? c->dev_bytes : 0; }
+
+int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1, const int32_t *w1, int64_t e1,
+                     const int32_t *s2, const int32_t *d2, const int32_t *w2, int64_t e2) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    if (n < 1 || n > 65534) return set_error(BM_ERANGE, "n=%d outside [1, 65534]", n);
+    if (e1 < 0 || e2 < 0 || (e1 && (!s1 || !d1 || !w1)) || (e2 && (!s2 || !d2 || !w2)))
+        return set_error(BM_EINVAL, "bad edge arrays");
+    if (2 * e1 >= (int64_t)INT32_MAX) return set_error(BM_ERANGE, "too many G1 edges (%lld)", (long long)e1);
+    for (int64_t i = 0; i < e1; i++) {
+        if (s1[i] < 0 || s1[i] > n || d1[i] < 0 || d1[i] > n)
+            return set_error(BM_EINVAL, "G1 line %lld: vertex outside [0,%d]", (long long)i, n);
+        if (w1[i] < -32768) return set_error(BM_ERANGE, "G1 line %lld: weight %d < -32768", (long long)i, w1[i]);
+    }
+    for (int64_t i = 0; i < e2; i++)
+        if (s2[i] < 0 || s2[i] > n || d2[i] < 0 || d2[i] > n)
+            return set_error(BM_EINVAL, "G2 line %lld: vertex outside [0,%d]", (long long)i, n);
+    BM_HIP(hipSetDevice(c->device));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    free_graph_buffers(c);
+
+    const int N1 = n + 1;
+    c->n = n;
+    c->N1 = N1;
+    c->ld = (N1 + 63) / 64 * 64;
+    c->e1 = e1;
+
+    // ---- G1 comb lists: out entries (file order) then in entries (file order)
+    std::vector<int32_t> cnt_out(N1, 0), cnt_in(N1, 0);
+    for (int64_t i = 0; i < e1; i++) {
+        cnt_out[s1[i]]++;
+        cnt_in[d1[i]]++;
+    }
+    std::vector<int32_t> cptr(N1 + 1, 0), pos_out(N1), pos_in(N1);
+    for (int v = 0; v < N1; v++) cptr[v + 1] = cptr[v] + cnt_out[v] + cnt_in[v];
+    for (int v = 0; v < N1; v++) {
+        pos_out[v] = cptr[v];
+        pos_in[v] = cptr[v] + cnt_out[v];
+    }
+    const int64_t nc = 2 * e1;
+    std::vector<int32_t> cr(nc), cw(nc), cdw(nc, 0), nb(nc);
+    for (int64_t i = 0; i < e1; i++) {
+        int s = s1[i], d = d1[i];
+        int po = pos_out[s]++;
+        cr[po] = N1 + d;
+        cw[po] = w1[i];
+        nb[po] = d;
+        int pi = pos_in[d]++;
+        cr[pi] = s;
+        cw[pi] = w1[i];
+        nb[pi] = s;
+    }
+    // owner of the dense G1 cell: the last line of each ordered pair
+    std::vector<int32_t> stamp(N1, -1);
+    for (int v = 0; v < N1; v++) {
+        int mid = cptr[v] + cnt_out[v];
+        for (int e = mid - 1; e >= cptr[v]; e--)
+            if (stamp[nb[e]] != 2 * v) {
+                stamp[nb[e]] = 2 * v;
+                cdw[e] = (int16_t)cw[e];
+            }
+        for (int e = cptr[v + 1] - 1; e >= mid; e--)
+            if (stamp[nb[e]] != 2 * v + 1) {
+                stamp[nb[e]] = 2 * v + 1;
+                cdw[e] = (int16_t)cw[e];
+            }
+    }
+    // flat out-edge list in CSR order (score, all-pairs corrections)
+    std::vector<int32_t> es(e1), ed(e1), ewv(e1), edwv(e1);
+    {
+        int64_t k = 0;
+        for (int v = 0; v < N1; v++)
+            for (int e = cptr[v]; e < cptr[v] + cnt_out[v]; e++) {
+                es[k] = v;
+                ed[k] = nb[e];
+                ewv[k] = cw[e];
+                edwv[k] = cdw[e];
+                k++;
+            }
+    }
+    // de-duplicated view (last weight wins), out part then in part per vertex
+    std::vector<int32_t> ddptr(N1 + 1, 0), ddmid(N1, 0), ddr, ddw;
+    ddr.reserve(nc);
+    ddw.reserve(nc);
+    std::fill(stamp.begin(), stamp.end(), -1);
+    for (int v = 0; v < N1; v++) {
+        ddptr[v] = (int32_t)ddr.size();
+        int mid = cptr[v] + cnt_out[v];
+        // keep the last occurrence of each pair (list order is immaterial to the sums)
+        for (int e = mid - 1; e >= cptr[v]; e--)
+            if (stamp[nb[e]] != 2 * v) {
+                stamp[nb[e]] = 2 * v;
+                ddr.push_back(cr[e]);
+                ddw.push_back(cw[e]);
+            }
+        ddmid[v] = (int32_t)ddr.size();
+        for (int e = cptr[v + 1] - 1; e >= mid; e--)
+            if (stamp[nb[e]] != 2 * v + 1) {
+                stamp[nb[e]] = 2 * v + 1;
+                ddr.push_back(cr[e]);
+                ddw.push_back(cw[e]);
+            }
+    }
+    ddptr[N1] = (int32_t)ddr.size();
+    c->ncomb = nc;
+    c->ndd = (int64_t)ddr.size();
+
+    // weight facts used by the packed fast paths
+    bool nonneg = true;
+    int32_t maxw = 0;
+    for (int64_t i = 0; i < e1; i++) {
+        if (w1[i] < 0) nonneg = false;
+        maxw = std::max(maxw, w1[i]);
+    }
+
+    // ---- G2: last line per ordered pair
+    std::vector<int32_t> tri;
+    {
+        std::vector<int32_t> p2(N1 + 1, 0);
+        for (int64_t i = 0; i < e2; i++) p2[s2[i] + 1]++;
+        for (int v = 0; v < N1; v++) p2[v + 1] += p2[v];
+        std::vector<int32_t> pos(p2.begin(), p2.end() - 1), od(e2), ow(e2);
+        for (int64_t i = 0; i < e2; i++) {
+            int p = pos[s2[i]]++;
+            od[p] = d2[i];
+            ow[p] = w2[i];
+        }
+        std::fill(stamp.begin(), stamp.end(), -1);
+        tri.reserve(3 * (size_t)e2);
+        for (int u = 0; u < N1; u++)
+            for (int e = p2[u + 1] - 1; e >= p2[u]; e--)
+                if (stamp[od[e]] != u) {
+                    stamp[od[e]] = u;
+                    tri.push_back(u);
+                    tri.push_back(od[e]);
+                    tri.push_back(ow[e]);
+                    if ((int16_t)ow[e] < 0) nonneg = false;
+                }
+    }
+    c->nonneg = nonneg;
+    c->max_w1 = maxw;
+
+    // ---- device buffers
+    size_t ld = (size_t)c->ld;
+    int rc;
+    if ((rc = dev_alloc(c, (void **)&c->W2, (size_t)N1 * ld * 2))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->Q, 2 * (size_t)N1 * ld * 2))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->map, (size_t)N1 * 4))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->diag, (size_t)N1 * 4))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->keys, ((size_t)N1 + 8) * 8))) goto fail;
+    if ((rc = upload(c, &c->cptr, cptr))) goto fail;
+    if ((rc = upload(c, &c->cr, cr))) goto fail;
+    if ((rc = upload(c, &c->cw, cw))) goto fail;
+    if ((rc = upload(c, &c->cdw, cdw))) goto fail;
+    if ((rc = upload(c, &c->ddptr, ddptr))) goto fail;
+    if ((rc = upload(c, &c->ddmid, ddmid))) goto fail;
+    if ((rc = upload(c, &c->ddr, ddr))) goto fail;
+    if ((rc = upload(c, &c->ddw, ddw))) goto fail;
+    if ((rc = upload(c, &c->esrc, es))) goto fail;
+    if ((rc = upload(c, &c->edst, ed))) goto fail;
+    if ((rc = upload(c, &c->ew, ewv))) goto fail;
+    if ((rc = upload(c, &c->edw, edwv))) goto fail;
+    {
+        hipError_t e = hipMemsetAsync(c->W2, 0, (size_t)N1 * ld * 2, c->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "memset W2"); goto fail; }
+        e = hipMemsetAsync(c->Q, 0, 2 * (size_t)N1 * ld * 2, c->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "memset Q"); goto fail; }
+        int64_t m2 = (int64_t)tri.size() / 3;
+        if (m2 > 0) {
+            if ((rc = ensure_stage(c, tri.size() * 4))) goto fail;
+            e = hipMemcpyAsync(c->stage, tri.data(), tri.size() * 4, hipMemcpyHostToDevice, c->stream);
+            if (e != hipSuccess) { rc = hip_fail(e, "copy G2"); goto fail; }
+            hipLaunchKernelGGL(k_scatter_w2, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, c->stream, c->W2,
+                               c->ld, (const int32_t *)c->stage, m2);
+            if ((rc = launch_check("scatter_w2"))) goto fail;
+        }
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "upload sync"); goto fail; }
+    }
+    c->hmap.assign(N1, 0);
+    c->graphs = true;
+    c->mapped = false;
+    return BM_OK;
+fail:
+    free_graph_buffers(c);
+    return rc;
+}
+
+int bm_set_mapping(bm_ctx *c, const int32_t *mapping) {
+    if (!c || !mapping) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->graphs) return set_error(BM_ESTATE, "graphs not uploaded");
+    for (int v = 0; v < c->N1; v++)
+        if (mapping[v] < 0 || mapping[v] > c->n)
+            return set_error(BM_EINVAL, "mapping[%d]=%d outside [0,%d]", v, mapping[v], c->n);
+    BM_HIP(hipSetDevice(c->device));
+    c->hmap.assign(mapping, mapping + c->N1);
+    BM_HIP(hipMemcpyAsync(c->map, c->hmap.data(), (size_t)c->N1 * 4, hipMemcpyHostToDevice, c->stream));
+    int16_t *P = c->Q, *PT = c->Q + (size_t)c->N1 * c->ld;
+    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_build_p, dim3(c->ld / 256 + 1, c->N1), dim3(256), 0, c->stream, P,
+                                               c->W2, c->map, c->N1, c->ld));
+    BM_LAUNCH(c, "transpose", hipLaunchKernelGGL(k_transpose, dim3((c->N1 + 63) / 64, c->ld / 64), dim3(64, 4), 0,
+                                                 c->stream, PT, P, c->N1, c->ld));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    c->mapped = true;
+    c->diag_valid = false;
+    return BM_OK;
+}
+
+int bm_get_mapping(bm_ctx *c, int32_t *mapping) {
+    if (!c || !mapping) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    memcpy(mapping, c->hmap.data(), (size_t)c->N1 * 4);
+    return BM_OK;
+}
+
+int bm_score(bm_ctx *c, int64_t *score) {
+    if (!c || !score) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    BM_HIP(hipSetDevice(c->device));
+    unsigned long long *out = c->keys + c->N1 + 2;
+    BM_HIP(hipMemsetAsync(out, 0, 8, c->stream));
+    int64_t m = c->e1;
+    int blocks = (int)std::min<int64_t>((m + 255) / 256, 4096);
+    if (blocks < 1) blocks = 1;
+    BM_LAUNCH(c, "score", hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, c->stream, out, c->esrc, c->edst,
+                                             c->ew, m, c->Q, c->ld));
+    unsigned long long h = 0;
+    BM_HIP(hipMemcpyAsync(&h, out, 8, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    *score = (int64_t)h;
+    return BM_OK;
+}
+
+static int check_vertex(const bm_ctx *c, int32_t v) { return v >= 0 && v <= c->n; }
+
+int bm_apply_swap(bm_ctx *c, int32_t a, int32_t b) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (!check_vertex(c, a) || !check_vertex(c, b) || a == b)
+        return set_error(BM_EINVAL, "apply_swap(%d,%d): bad vertices", a, b);
+    BM_HIP(hipSetDevice(c->device));
+    int32_t set[2] = {a, b}, src[2] = {1, 0};
+    int rc = permute_rows_cols(c, 2, set, src);
+    if (rc) return rc;
+    BM_HIP(hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+// triple.cu:903-934: new (p(m1),p(m2),p(m3)) = table[p] applied to (f1,f2,f3)
+int bm_apply_rot3(bm_ctx *c, int32_t m1, int32_t m2, int32_t m3, int32_t p) {
+    static const int tab[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (!check_vertex(c, m1) || !check_vertex(c, m2) || !check_vertex(c, m3) || m1 == m2 || m1 == m3 || m2 == m3 ||
+        p < 0 || p > 5)
+        return set_error(BM_EINVAL, "apply_rot3: bad arguments");
+    if (p == 0) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    int32_t set[3] = {m1, m2, m3}, src[3] = {tab[p][0], tab[p][1], tab[p][2]};
+    int rc = permute_rows_cols(c, 3, set, src);
+    if (rc) return rc;
+    BM_HIP(hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+int bm_apply_perm(bm_ctx *c, int32_t k, const int32_t *set, const int32_t *perm) {
+    if (!c || !set || !perm) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (k < 1 || k > 8) return set_error(BM_EINVAL, "k=%d outside [1,8]", k);
+    int seen = 0;
+    for (int i = 0; i < k; i++) {
+        if (!check_vertex(c, set[i])) return set_error(BM_EINVAL, "set[%d]=%d bad vertex", i, set[i]);
+        for (int j = 0; j < i; j++)
+            if (set[j] == set[i]) return set_error(BM_EINVAL, "set has duplicate vertex %d", set[i]);
+        if (perm[i] < 0 || perm[i] >= k || (seen & (1 << perm[i])))
+            return set_error(BM_EINVAL, "perm is not a permutation of 0..%d", k - 1);
+        seen |= 1 << perm[i];
+    }
+    BM_HIP(hipSetDevice(c->device));
+    int rc = permute_rows_cols(c, k, set, perm);
+    if (rc) return rc;
+    BM_HIP(hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+int bm_set_timing(bm_ctx *c, int enable) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    if (!enable) {
+        int rc = resolve_timers(c);
+        if (rc) return rc;
+    }
+    c->timing = enable != 0;
+    return BM_OK;
+}
+
+int bm_kernel_time(bm_ctx *c, const char *kernel, int64_t *launches, double *total_ms) {
+    if (!c || !kernel) return set_error(BM_EINVAL, "NULL argument");
+    int rc = resolve_timers(c);
+    if (rc) return rc;
+    for (auto &k : c->kstats)
+        if (k.name == kernel) {
+            if (launches) *launches = k.launches;
+            if (total_ms) *total_ms = k.ms;
+            return BM_OK;
+        }
+    if (launches) *launches = 0;
+    if (total_ms) *total_ms = 0.0;
+    return BM_OK;
+}
+
+int bm_reset_timing(bm_ctx *c) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    int rc = resolve_timers(c);
+    if (rc) return rc;
+    for (auto &k : c->kstats) {
+        k.launches = 0;
+        k.ms = 0.0;
+    }
+    return BM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// Internal declarations of the MI355X move-delta engine (gfx950, wave64).
+//
+// Device-resident state of one context (one graph pair):
+//   W2   int16 [(n+1) x ld]      G2 dense adjacency, last line per pair wins
+//   Q    int16 [2(n+1) x ld]     rows 0..n     : P[x][y]  = W2[p(x)][p(y)]
+//                                rows n+1..2n+1: PT[x][y] = P[y][x]
+//   comb lists, one per vertex v (out entries of v, then in entries of v):
+//        r  = n+1+d for an out entry v->d  (row of PT: Q[r][b] = P[b][d])
+//        r  = s     for an in entry  s->v  (row of P : Q[r][b] = P[s][b])
+//        w  = the CSV weight, dw = (short)weight of the line that owns the
+//        dense G1 cell for that ordered pair (0 for earlier duplicates)
+//   diag int32 [n+1]             X(v,v), X(u,v) = sum_{e in comb(u)} min(w_e, Q[r_e][v])
+// With these, Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b) + Corr(a,b),
+// where Corr collects the skipped a<->b list entries and the two dense cells.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "bmatch.h"
+
+namespace bm {
+
+constexpr int kWave = 64;
+
+struct TimerRec {
+    int kernel;  // index into kernel name table
+    hipEvent_t start, stop;
+};
+
+struct KernelStat {
+    std::string name;
+    int64_t launches = 0;
+    double ms = 0.0;
+};
+
+}  // namespace bm
+
+struct bm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // sizes
+    int32_t n = 0;    // largest vertex id
+    int32_t N1 = 0;   // n + 1
+    int32_t ld = 0;   // row stride (elements) of W2 and Q, multiple of 64
+    int64_t e1 = 0;   // G1 CSV lines
+    int64_t ncomb = 0;
+    int64_t ndd = 0;
+    bool graphs = false;
+    bool mapped = false;
+    bool diag_valid = false;
+    bool nonneg = false;  // all weights >= 0 (packed-accumulate fast paths)
+    int32_t max_w1 = 0;
+
+    // device buffers
+    int16_t *W2 = nullptr;
+    int16_t *Q = nullptr;
+    int32_t *map = nullptr;
+    int32_t *cptr = nullptr, *cr = nullptr, *cw = nullptr, *cdw = nullptr;
+    int32_t *ddptr = nullptr, *ddmid = nullptr, *ddr = nullptr, *ddw = nullptr;
+    int32_t *esrc = nullptr, *edst = nullptr, *ew = nullptr, *edw = nullptr;
+    int32_t *diag = nullptr;
+    int32_t *X = nullptr;  // all-pairs gain matrix (lazy)
+    int64_t ldx = 0;
+    unsigned long long *keys = nullptr;  // N1 + 8 reduction slots
+    void *stage = nullptr;               // move-list staging (lazy, grows)
+    size_t stage_bytes = 0;
+    int64_t dev_bytes = 0;
+
+    // host mirror of the mapping
+    std::vector<int32_t> hmap;
+
+    // kernel timing
+    bool timing = false;
+    std::vector<bm::KernelStat> kstats;
+    std::vector<bm::TimerRec> pending;
+    std::vector<hipEvent_t> ev_pool;
+};
+
+namespace bm {
+
+int set_error(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+#define BM_HIP(call)                                        \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return bm::hip_fail(e_, #call); \
+    } while (0)
+
+int dev_alloc(bm_ctx *c, void **p, size_t bytes);
+void dev_free(bm_ctx *c, void **p, size_t bytes);
+int ensure_stage(bm_ctx *c, size_t bytes);
+
+// kernel timing helpers (no-ops unless c->timing)
+int timer_begin(bm_ctx *c, const char *name);
+void timer_end(bm_ctx *c, int token);
+
+int ensure_diag(bm_ctx *c);
+int launch_check(const char *what);
+
+// device-side row/column permutation of P and PT for a vertex set
+int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of);
+
+}  // namespace bm
+
+// Launch helper: enqueues `launch` on c->stream, timed under `name`.
+#define BM_LAUNCH(c, name, ...)                        \
+    do {                                               \
+        int tok_ = bm::timer_begin((c), (name));       \
+        __VA_ARGS__;                                   \
+        bm::timer_end((c), tok_);                      \
+        int rc_ = bm::launch_check(name);              \
+        if (rc_) return rc_;                           \
+    } while (0)

@@ -1,0 +1,363 @@
+// Kernels for the 3-vertex, k-set and hill-climb move kinds (gfx950).
+//
+//   k_rot3     : Delta_G3 of triple.cu:85-214 for a triple list, 16 lanes per
+//                triple.  Each neighbour entry of m_k loads the three values
+//                Q[r][m_0..2] once and feeds all five permutations.
+//   k_permset  : Delta_P of graph-alignment-permset-parallel.c:304-339 for
+//                every permutation of a k-set, one workgroup per set: an
+//                LDS table C[i][j] (vertex i takes j's image) plus the
+//                intra-set lines (counted twice, as the reference does), then
+//                k! evaluations in next_permutation order, strict-> arg-max.
+//   k_hillclimb: Delta_A of refine-matching-hillclimb.c:244-296.
+#include <limits.h>
+
+#include <algorithm>
+
+#include "bm_internal.h"
+
+using namespace bm;
+
+__device__ __forceinline__ int qv(const int16_t *__restrict__ Q, int row, int ld, int col) {
+    return (int)Q[(size_t)row * (size_t)ld + col];
+}
+
+// permutation table of triple.cu:85-92: new image of m_k = old image of m_tab[p][k]
+__constant__ int kG3Tab[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+__global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int64_t count, int mode,
+                                              int32_t *__restrict__ best_d, int32_t *__restrict__ best_p,
+                                              const int32_t *__restrict__ ddptr, const int32_t *__restrict__ ddr,
+                                              const int32_t *__restrict__ ddw, const int16_t *__restrict__ Q, int N1,
+                                              int ld) {
+    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int gl = threadIdx.x & 15;
+    int s[6] = {0, 0, 0, 0, 0, 0};
+    int m[3] = {0, 0, 0};
+    if (t < count) {
+        m[0] = T[3 * t];
+        m[1] = T[3 * t + 1];
+        m[2] = T[3 * t + 2];
+        for (int k = 0; k < 3; k++) {
+            const int v = m[k];
+            for (int e = ddptr[v] + gl; e < ddptr[v + 1]; e += 16) {
+                const int r = ddr[e];
+                const int w = (int)(int16_t)ddw[e];  // the dense G1 cell
+                if (w <= 0) continue;
+                const bool out = r >= N1;
+                const int nb = out ? r - N1 : r;
+                if (nb == 0) continue;  // the dense scan runs over i = 1..n
+                int j = nb == m[0] ? 0 : nb == m[1] ? 1 : nb == m[2] ? 2 : -1;
+                if (j >= 0) {
+                    // intra-triple cell m_k -> m_j (:164-207), from the out entry only
+                    if (out && j != k) {
+                        const int old = min(w, qv(Q, m[k], ld, m[j]));
+#pragma unroll
+                        for (int p = 1; p < 6; p++)
+                            s[p] += min(w, qv(Q, m[kG3Tab[p][k]], ld, m[kG3Tab[p][j]])) - old;
+                    }
+                    continue;
+                }
+                const int x0 = min(w, qv(Q, r, ld, m[0]));
+                const int x1 = min(w, qv(Q, r, ld, m[1]));
+                const int x2 = min(w, qv(Q, r, ld, m[2]));
+                const int xs[3] = {x0, x1, x2};
+                const int old = xs[k];
+#pragma unroll
+                for (int p = 1; p < 6; p++) s[p] += xs[kG3Tab[p][k]] - old;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 1; p < 6; p++) {
+        s[p] += __shfl_xor(s[p], 8, 16);
+        s[p] += __shfl_xor(s[p], 4, 16);
+        s[p] += __shfl_xor(s[p], 2, 16);
+        s[p] += __shfl_xor(s[p], 1, 16);
+    }
+    if (t < count && gl == 0) {
+        int bd = 0, bp = 0;
+        for (int p = 1; p < 6; p++) {
+            if (mode == BM_ROT3_ROTATIONS && p != 3 && p != 4) continue;
+            if (s[p] > bd) {
+                bd = s[p];
+                bp = p;
+            }
+        }
+        best_d[t] = bd;
+        best_p[t] = bp;
+    }
+}
+
+constexpr int kMaxIntra = 256;
+
+__device__ __forceinline__ bool next_perm(int *a, int k) {
+    int i = k - 2;
+    while (i >= 0 && a[i] >= a[i + 1]) i--;
+    if (i < 0) return false;
+    int j = k - 1;
+    while (a[j] <= a[i]) j--;
+    int t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+    for (int x = i + 1, y = k - 1; x < y; x++, y--) {
+        t = a[x];
+        a[x] = a[y];
+        a[y] = t;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ sets, int64_t count, int k,
+                                                 int32_t *__restrict__ best_d, int32_t *__restrict__ best_rank,
+                                                 const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
+                                                 const int32_t *__restrict__ cw, const int16_t *__restrict__ Q, int N1,
+                                                 int ld, int *__restrict__ overflow) {
+    __shared__ int S[8];
+    __shared__ int C[8][8];
+    __shared__ int PS[8][8];
+    __shared__ int ni;
+    __shared__ int iw[kMaxIntra];
+    __shared__ unsigned char ix[kMaxIntra], il[kMaxIntra];
+    __shared__ int lbase;
+    __shared__ unsigned long long red[4];
+    const int64_t set = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid < k) S[tid] = sets[set * k + tid];
+    if (tid < 64) C[tid >> 3][tid & 7] = 0;
+    if (tid == 0) {
+        ni = 0;
+        lbase = 0;
+    }
+    __syncthreads();
+    if (tid < k * k) PS[tid / k][tid % k] = qv(Q, S[tid / k], ld, S[tid % k]);
+    // table C over all entries of the k lists
+    for (int i = 0; i < k; i++) {
+        const int v = S[i];
+        for (int e = cptr[v] + tid; e < cptr[v + 1]; e += 256) {
+            const int r = cr[e], w = cw[e];
+            const bool out = r >= N1;
+            const int nb = out ? r - N1 : r;
+            int l = -1;
+            for (int q = 0; q < k; q++)
+                if (S[q] == nb) l = q;
+            if (l >= 0) {
+                // intra line: both of its list entries change with sigma;
+                // keep it once (from the out entry), doubled when evaluated
+                if (out) {
+                    const int slot = atomicAdd(&ni, 1);
+                    if (slot < kMaxIntra) {
+                        iw[slot] = w;
+                        ix[slot] = (unsigned char)i;
+                        il[slot] = (unsigned char)l;
+                    }
+                }
+                continue;
+            }
+            const int old = min(w, qv(Q, r, ld, v));
+            for (int j = 0; j < k; j++)
+                if (j != i) atomicAdd(&C[i][j], min(w, qv(Q, r, ld, S[j])) - old);
+        }
+    }
+    __syncthreads();
+    const int nint = min(ni, kMaxIntra);
+    if (ni > kMaxIntra && tid == 0) atomicOr(overflow, 1);
+    if (tid < nint) atomicAdd(&lbase, -2 * min(iw[tid], PS[ix[tid]][il[tid]]));
+    __syncthreads();
+    // k! permutations, contiguous rank ranges per thread
+    int total = 1;
+    for (int i = 2; i <= k; i++) total *= i;
+    const int per = (total + 255) / 256;
+    const int r0 = tid * per, r1 = min(total, r0 + per);
+    int bestd = 0, bestr = -1;
+    if (r0 < r1) {
+        int perm[8], pool[8];
+        for (int i = 0; i < k; i++) pool[i] = i;
+        int rem = r0, left = k;
+        for (int i = 0; i < k; i++) {
+            int f = 1;
+            for (int q = 2; q <= k - 1 - i; q++) f *= q;
+            const int idx = rem / f;
+            rem -= idx * f;
+            perm[i] = pool[idx];
+            for (int q = idx; q < left - 1; q++) pool[q] = pool[q + 1];
+            left--;
+        }
+        for (int rank = r0; rank < r1; rank++) {
+            int d = lbase;
+            for (int i = 0; i < k; i++) d += C[i][perm[i]];
+            for (int q = 0; q < nint; q++) d += 2 * min(iw[q], PS[perm[ix[q]]][perm[il[q]]]);
+            if (d > bestd) {
+                bestd = d;
+                bestr = rank;
+            }
+            if (rank + 1 < r1) next_perm(perm, k);
+        }
+    }
+    // block arg-max: larger delta, then smaller rank
+    unsigned long long key = 0;
+    if (bestr >= 0)
+        key = ((unsigned long long)((unsigned)bestd ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (unsigned)bestr);
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long t2 = __shfl_xor(key, o);
+        key = t2 > key ? t2 : key;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = key;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long kk = max(max(red[0], red[1]), max(red[2], red[3]));
+        if (kk) {
+            best_d[set] = (int32_t)((uint32_t)(kk >> 32) ^ 0x80000000u);
+            best_rank[set] = (int32_t)(0xFFFFFFFFu - (uint32_t)(kk & 0xFFFFFFFFu));
+        } else {
+            best_d[set] = 0;
+            best_rank[set] = -1;
+        }
+    }
+}
+
+// Delta_A: out entries of the de-duplicated G1 only, a<->b skipped.
+__global__ __launch_bounds__(256) void k_hillclimb(const int32_t *__restrict__ A, const int32_t *__restrict__ B,
+                                                   int64_t count, int32_t *__restrict__ out,
+                                                   const int32_t *__restrict__ ddptr, const int32_t *__restrict__ ddmid,
+                                                   const int32_t *__restrict__ ddr, const int32_t *__restrict__ ddw,
+                                                   const int16_t *__restrict__ Q, int N1, int ld) {
+    const int64_t pair = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int gl = threadIdx.x & 15;
+    int sum = 0;
+    if (pair < count) {
+        const int a = A[pair], b = B[pair];
+        for (int side = 0; side < 2; side++) {
+            const int x = side ? b : a, y = side ? a : b;
+            for (int e = ddptr[x] + gl; e < ddmid[x]; e += 16) {
+                const int t = ddr[e] - N1, w = ddw[e];
+                if (t == y) continue;
+                const int oldv = min(w, qv(Q, x, ld, t));
+                const int newv = min(w, t == x ? qv(Q, y, ld, y) : qv(Q, y, ld, t));
+                sum += newv - oldv;
+            }
+        }
+    }
+    sum += __shfl_xor(sum, 8, 16);
+    sum += __shfl_xor(sum, 4, 16);
+    sum += __shfl_xor(sum, 2, 16);
+    sum += __shfl_xor(sum, 1, 16);
+    if (pair < count && gl == 0) out[pair] = sum;
+}
+
+// ------------------------------------------------------------ C ABI
+
+extern "C" int bm_eval_rot3(bm_ctx *c, const int32_t *triples, int64_t count, int mode, int32_t *best_delta,
+                            int32_t *best_perm) {
+    if (!c || (count > 0 && (!triples || !best_delta || !best_perm))) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (mode != BM_ROT3_ROTATIONS && mode != BM_ROT3_ALL) return set_error(BM_EINVAL, "bad mode %d", mode);
+    if (count < 0) return set_error(BM_EINVAL, "count < 0");
+    for (int64_t i = 0; i < count; i++) {
+        const int32_t *t = triples + 3 * i;
+        for (int j = 0; j < 3; j++)
+            if (t[j] < 1 || t[j] > c->n) return set_error(BM_EINVAL, "triple %lld: vertex %d", (long long)i, t[j]);
+        if (t[0] == t[1] || t[0] == t[2] || t[1] == t[2])
+            return set_error(BM_EINVAL, "triple %lld has repeated vertices", (long long)i);
+    }
+    if (count == 0) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    const int64_t chunk = 1 << 22;
+    int rc = ensure_stage(c, (size_t)chunk * 20);
+    if (rc) return rc;
+    int32_t *dT = (int32_t *)c->stage, *dD = dT + 3 * chunk, *dP = dD + chunk;
+    for (int64_t off = 0; off < count; off += chunk) {
+        const int64_t m = std::min(chunk, count - off);
+        BM_HIP(hipMemcpyAsync(dT, triples + 3 * off, m * 12, hipMemcpyHostToDevice, c->stream));
+        BM_LAUNCH(c, "rot3", hipLaunchKernelGGL(k_rot3, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0,
+                                                c->stream, dT, m, mode, dD, dP, c->ddptr, c->ddr, c->ddw, c->Q, c->N1,
+                                                c->ld));
+        BM_HIP(hipMemcpyAsync(best_delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipMemcpyAsync(best_perm + off, dP, m * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipStreamSynchronize(c->stream));
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_eval_permset(bm_ctx *c, int32_t k, const int32_t *sets, int64_t count, int32_t *best_delta,
+                               int32_t *best_rank) {
+    if (!c || (count > 0 && (!sets || !best_delta || !best_rank))) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (k < 2 || k > 8) return set_error(BM_EINVAL, "k=%d outside [2,8]", k);
+    if (count < 0) return set_error(BM_EINVAL, "count < 0");
+    for (int64_t i = 0; i < count; i++) {
+        const int32_t *s = sets + (size_t)k * i;
+        for (int j = 0; j < k; j++) {
+            if (s[j] < 1 || s[j] > c->n) return set_error(BM_EINVAL, "set %lld: vertex %d", (long long)i, s[j]);
+            for (int q = 0; q < j; q++)
+                if (s[q] == s[j]) return set_error(BM_EINVAL, "set %lld repeats vertex %d", (long long)i, s[j]);
+        }
+    }
+    if (count == 0) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    const int64_t chunk = 1 << 20;
+    int rc = ensure_stage(c, (size_t)chunk * (k + 2) * 4 + 64);
+    if (rc) return rc;
+    int32_t *dS = (int32_t *)c->stage, *dD = dS + (size_t)k * chunk, *dR = dD + chunk;
+    int *dOv = dR + chunk;
+    BM_HIP(hipMemsetAsync(dOv, 0, 4, c->stream));
+    for (int64_t off = 0; off < count; off += chunk) {
+        const int64_t m = std::min(chunk, count - off);
+        BM_HIP(hipMemcpyAsync(dS, sets + (size_t)k * off, (size_t)m * k * 4, hipMemcpyHostToDevice, c->stream));
+        BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset, dim3((unsigned)m), dim3(256), 0, c->stream, dS, m, k, dD,
+                                                   dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv));
+        BM_HIP(hipMemcpyAsync(best_delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipMemcpyAsync(best_rank + off, dR, m * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    int ov = 0;
+    BM_HIP(hipMemcpyAsync(&ov, dOv, 4, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    if (ov) return set_error(BM_ERANGE, "a set has more than %d intra-set G1 lines", kMaxIntra);
+    return BM_OK;
+}
+
+extern "C" int bm_perm_from_rank(int32_t k, int32_t rank, int32_t *perm) {
+    if (k < 1 || k > 8 || !perm) return set_error(BM_EINVAL, "bad k or NULL perm");
+    int total = 1;
+    for (int i = 2; i <= k; i++) total *= i;
+    if (rank < 0 || rank >= total) return set_error(BM_EINVAL, "rank %d outside [0,%d)", rank, total);
+    int pool[8];
+    for (int i = 0; i < k; i++) pool[i] = i;
+    int left = k, rem = rank;
+    for (int i = 0; i < k; i++) {
+        int f = 1;
+        for (int q = 2; q <= k - 1 - i; q++) f *= q;
+        const int idx = rem / f;
+        rem -= idx * f;
+        perm[i] = pool[idx];
+        for (int q = idx; q < left - 1; q++) pool[q] = pool[q + 1];
+        left--;
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_eval_swaps_hillclimb(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count,
+                                       int32_t *delta) {
+    if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (count < 0) return set_error(BM_EINVAL, "count < 0");
+    for (int64_t i = 0; i < count; i++)
+        if (a[i] < 0 || a[i] > c->n || b[i] < 0 || b[i] > c->n || a[i] == b[i])
+            return set_error(BM_EINVAL, "pair %lld: (%d,%d) invalid", (long long)i, a[i], b[i]);
+    if (count == 0) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    const int64_t chunk = 1 << 22;
+    int rc = ensure_stage(c, (size_t)chunk * 12);
+    if (rc) return rc;
+    int32_t *dA = (int32_t *)c->stage, *dB = dA + chunk, *dD = dB + chunk;
+    for (int64_t off = 0; off < count; off += chunk) {
+        const int64_t m = std::min(chunk, count - off);
+        BM_HIP(hipMemcpyAsync(dA, a + off, m * 4, hipMemcpyHostToDevice, c->stream));
+        BM_HIP(hipMemcpyAsync(dB, b + off, m * 4, hipMemcpyHostToDevice, c->stream));
+        BM_LAUNCH(c, "hillclimb", hipLaunchKernelGGL(k_hillclimb, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0,
+                                                     c->stream, dA, dB, m, dD, c->ddptr, c->ddmid, c->ddr, c->ddw,
+                                                     c->Q, c->N1, c->ld));
+        BM_HIP(hipMemcpyAsync(delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipStreamSynchronize(c->stream));
+    }
+    return BM_OK;
+}

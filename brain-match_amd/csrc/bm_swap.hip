@@ -1,0 +1,467 @@
+// Delta_B pair-swap kernels (graph-alignment-optimized.c:321-408) for gfx950.
+//
+//   k_eval_pairs : a move list, 16 lanes per pair walking the four neighbour
+//                  lists exactly as the reference loop does.
+//   k_row_eval   : one row a against a range of b (first-positive or argmax):
+//                  push part X(a,b) streamed coalesced over b, pull part
+//                  X(b,a) from column a of P staged in LDS, list corrections
+//                  for b adjacent to a, block reduction, one atomic per block.
+//   k_gain + k_corr + k_combine : the all-pairs sweep.  X(a,b) for every a, b
+//                  (sparse x dense min-plus product over the comb lists), then
+//                  Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b) + Corr(a,b)
+//                  on 64x64 tiles with row-best and global arg-max reductions.
+#include <limits.h>
+
+#include <algorithm>
+
+#include "bm_internal.h"
+
+using namespace bm;
+
+__device__ __forceinline__ int q16(const int16_t *__restrict__ Q, size_t row, int ld, int col) {
+    return (int)Q[row * (size_t)ld + col];
+}
+
+// ------------------------------------------------------------ move list
+
+// Delta_B for pairs (A[i], B[i]).  16-lane groups, 4 pairs per wave.
+__global__ __launch_bounds__(256) void k_eval_pairs(const int32_t *__restrict__ A, const int32_t *__restrict__ B,
+                                                    int64_t count, int32_t *__restrict__ out,
+                                                    const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
+                                                    const int32_t *__restrict__ cw, const int32_t *__restrict__ cdw,
+                                                    const int16_t *__restrict__ Q, int N1, int ld) {
+    const int64_t pair = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int gl = threadIdx.x & 15;
+    int sum = 0;
+    int a = 0, b = 0;
+    if (pair < count) {
+        a = A[pair];
+        b = B[pair];
+        const int pab = q16(Q, a, ld, b), pba = q16(Q, b, ld, a);
+        // lists of a: entries toward b are skipped, the owner line of a->b
+        // carries the dense-cell term (:392-397)
+        for (int e = cptr[a] + gl; e < cptr[a + 1]; e += 16) {
+            const int r = cr[e], w = cw[e];
+            const int nb = r >= N1 ? r - N1 : r;
+            if (nb == b) {
+                const int dw = cdw[e];
+                if (r >= N1 && dw > 0) sum += min(dw, pba) - min(dw, pab);
+            } else {
+                sum += min(w, q16(Q, r, ld, b)) - min(w, q16(Q, r, ld, a));
+            }
+        }
+        for (int e = cptr[b] + gl; e < cptr[b + 1]; e += 16) {
+            const int r = cr[e], w = cw[e];
+            const int nb = r >= N1 ? r - N1 : r;
+            if (nb == a) {
+                const int dw = cdw[e];
+                if (r >= N1 && dw > 0) sum += min(dw, pab) - min(dw, pba);  // :400-405
+            } else {
+                sum += min(w, q16(Q, r, ld, a)) - min(w, q16(Q, r, ld, b));
+            }
+        }
+    }
+    sum += __shfl_xor(sum, 8, 16);
+    sum += __shfl_xor(sum, 4, 16);
+    sum += __shfl_xor(sum, 2, 16);
+    sum += __shfl_xor(sum, 1, 16);
+    if (pair < count && gl == 0) out[pair] = sum;
+}
+
+// ------------------------------------------------------------ one row
+
+constexpr int kRowCh = 128;  // b columns per block
+
+__device__ __forceinline__ unsigned long long key_first(int b, int d) {
+    return ((unsigned long long)(unsigned)b << 32) | (unsigned)d;
+}
+__device__ __forceinline__ unsigned long long key_max(int b, int d) {
+    return ((unsigned long long)((unsigned)d ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (unsigned)b);
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long k) {
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long t = __shfl_xor(k, o);
+        k = t < k ? t : k;
+    }
+    return k;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long t = __shfl_xor(k, o);
+        k = t > k ? t : k;
+    }
+    return k;
+}
+
+// Correction for b adjacent to a, from one entry of comb(a):
+//   -K_e  (the skipped list entries of the edge, counted in X)
+//   + dense-cell term when this line owns W1[x][y] (:390-405)
+__device__ __forceinline__ int pair_corr(bool out, int w, int dw, int paa, int pbb, int pab, int pba) {
+    int v;
+    if (out) {  // a -> b
+        v = 2 * min(w, pab) - min(w, paa) - min(w, pbb);
+        if (dw > 0) v += min(dw, pba) - min(dw, pab);
+    } else {  // b -> a
+        v = 2 * min(w, pba) - min(w, pbb) - min(w, paa);
+        if (dw > 0) v += min(dw, pab) - min(dw, pba);
+    }
+    return v;
+}
+
+template <bool kLdsCol>
+__global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mode, int N1, int ld,
+                                                  const int16_t *__restrict__ Q, const int32_t *__restrict__ cptr,
+                                                  const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
+                                                  const int32_t *__restrict__ cdw, const int32_t *__restrict__ diag,
+                                                  unsigned long long *__restrict__ result) {
+    extern __shared__ int16_t colA[];  // colA[r] = Q[r][a], r < 2*N1
+    __shared__ int push[256];
+    __shared__ int pull[kRowCh];
+    __shared__ int corr[kRowCh];
+    __shared__ unsigned long long red[4];
+
+    const int tid = threadIdx.x;
+    const int b0 = lo + blockIdx.x * kRowCh;
+    const int nbk = min(kRowCh, hi + 1 - b0);
+    const int16_t *prow_a = Q + (size_t)a * ld;          // P[a][.]  == Q[N1+d][a]
+    const int16_t *pcol_a = Q + (size_t)(N1 + a) * ld;   // PT[a][.] == Q[s][a]
+    if (kLdsCol) {
+        for (int i = tid; i < N1; i += 256) {
+            colA[i] = pcol_a[i];
+            colA[N1 + i] = prow_a[i];
+        }
+    }
+    if (tid < kRowCh) {
+        pull[tid] = 0;
+        corr[tid] = 0;
+    }
+    __syncthreads();
+
+    const int ca0 = cptr[a], ca1 = cptr[a + 1];
+    // push: X(a,b) = sum_{e in comb(a)} min(w, Q[r][b]); two list halves
+    {
+        const int col = tid & (kRowCh - 1), split = tid / kRowCh;
+        int acc = 0;
+        if (col < nbk) {
+            const int b = b0 + col;
+            for (int e = ca0 + split; e < ca1; e += 2) acc += min(cw[e], q16(Q, cr[e], ld, b));
+        }
+        push[tid] = acc;
+    }
+    // pull: X(b,a) = sum_{e in comb(b)} min(w, Q[r][a]); 16 lanes per b
+    {
+        const int g = tid >> 4, gl = tid & 15;
+        for (int bb = g; bb < nbk; bb += 16) {
+            const int v = b0 + bb;
+            int s = 0;
+            for (int e = cptr[v] + gl; e < cptr[v + 1]; e += 16) {
+                const int r = cr[e];
+                int q;
+                if (kLdsCol) q = colA[r];
+                else q = r < N1 ? (int)pcol_a[r] : (int)prow_a[r - N1];
+                s += min(cw[e], q);
+            }
+            s += __shfl_xor(s, 8, 16);
+            s += __shfl_xor(s, 4, 16);
+            s += __shfl_xor(s, 2, 16);
+            s += __shfl_xor(s, 1, 16);
+            if (gl == 0) pull[bb] = s;
+        }
+    }
+    // corrections for b adjacent to a
+    {
+        const int paa = prow_a[a];
+        for (int e = ca0 + tid; e < ca1; e += 256) {
+            const int r = cr[e];
+            const bool out = r >= N1;
+            const int nb = out ? r - N1 : r;
+            if (nb == a || nb < b0 || nb >= b0 + nbk) continue;
+            const int pbb = q16(Q, nb, ld, nb), pab = prow_a[nb], pba = q16(Q, nb, ld, a);
+            atomicAdd(&corr[nb - b0], pair_corr(out, cw[e], cdw[e], paa, pbb, pab, pba));
+        }
+    }
+    __syncthreads();
+
+    unsigned long long key = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
+    if (tid < nbk) {
+        const int b = b0 + tid;
+        if (b != a) {
+            const int d = push[tid] + push[tid + kRowCh] + pull[tid] - diag[a] - diag[b] + corr[tid];
+            if (mode == BM_ROW_FIRST_POSITIVE) {
+                if (d > 0) key = key_first(b, d);
+            } else {
+                key = key_max(b, d);
+            }
+        }
+    }
+    key = mode == BM_ROW_FIRST_POSITIVE ? wave_min_u64(key) : wave_max_u64(key);
+    if ((tid & 63) == 0) red[tid >> 6] = key;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long k = red[0];
+        for (int i = 1; i < 4; i++) k = mode == BM_ROW_FIRST_POSITIVE ? min(k, red[i]) : max(k, red[i]);
+        if (mode == BM_ROW_FIRST_POSITIVE) {
+            if (k != ~0ull) atomicMin(result, k);
+        } else if (k != 0ull) {
+            atomicMax(result, k);
+        }
+    }
+}
+
+// ------------------------------------------------------------ all pairs
+
+// X[a][col..col+7] = sum_{e in comb(a)} min(w_e, Q[r_e][col..col+7]).
+// grid (ceil(ncols/2048), N1), 8 columns per thread, 16-byte loads.
+__global__ __launch_bounds__(256) void k_gain(const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
+                                              const int32_t *__restrict__ cw, const int16_t *__restrict__ Q, int ld,
+                                              int32_t *__restrict__ X, int64_t ldx, int ncols) {
+    const int a = blockIdx.y;
+    const int col = (blockIdx.x * 256 + threadIdx.x) * 8;
+    if (col >= ncols) return;
+    int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int beg = cptr[a], end = cptr[a + 1];
+    const int16_t *base = Q + col;
+    int e = beg;
+    for (; e + 4 <= end; e += 4) {
+        int r[4], w[4];
+        int4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            r[u] = cr[e + u];
+            w[u] = cw[e + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = *reinterpret_cast<const int4 *>(base + (size_t)r[u] * ld);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int x[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                acc[2 * j] += min(w[u], (int)(int16_t)(x[j] & 0xffff));
+                acc[2 * j + 1] += min(w[u], x[j] >> 16);
+            }
+        }
+    }
+    for (; e < end; e++) {
+        const int r = cr[e], w = cw[e];
+        const int4 v = *reinterpret_cast<const int4 *>(base + (size_t)r * ld);
+        const int x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            acc[2 * j] += min(w, (int)(int16_t)(x[j] & 0xffff));
+            acc[2 * j + 1] += min(w, x[j] >> 16);
+        }
+    }
+    int4 *dst = reinterpret_cast<int4 *>(X + (size_t)a * ldx + col);
+    dst[0] = make_int4(acc[0], acc[1], acc[2], acc[3]);
+    dst[1] = make_int4(acc[4], acc[5], acc[6], acc[7]);
+}
+
+// Corr(a,b) added into X[min(a,b)][max(a,b)], one thread per G1 line.
+__global__ __launch_bounds__(256) void k_corr(const int32_t *__restrict__ es, const int32_t *__restrict__ ed,
+                                              const int32_t *__restrict__ ew, const int32_t *__restrict__ edw,
+                                              int64_t m, const int16_t *__restrict__ Q, int ld,
+                                              int32_t *__restrict__ X, int64_t ldx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int x = es[i], y = ed[i];
+    if (x == y) return;
+    const int pxx = q16(Q, x, ld, x), pyy = q16(Q, y, ld, y), pxy = q16(Q, x, ld, y), pyx = q16(Q, y, ld, x);
+    // the line x->y seen from the pair (a=x, b=y) as an out entry of a
+    const int v = pair_corr(true, ew[i], edw[i], pxx, pyy, pxy, pyx);
+    const int lo = min(x, y), hi = max(x, y);
+    atomicAdd(&X[(size_t)lo * ldx + hi], v);
+}
+
+// Delta tiles over (I <= J) 64x64 blocks: row-best and global arg-max.
+__global__ __launch_bounds__(256) void k_combine(const int32_t *__restrict__ X, int64_t ldx,
+                                                 const int32_t *__restrict__ diag, int n,
+                                                 unsigned long long *__restrict__ rowkey,
+                                                 unsigned long long *__restrict__ gkey,
+                                                 int32_t *__restrict__ dmat) {
+    const int I = blockIdx.y, J = blockIdx.x;
+    if (I > J) return;
+    __shared__ int tT[64][65];
+    __shared__ int dA[64], dB[64];
+    __shared__ unsigned long long red[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int a0 = I * 64, bcol0 = J * 64;
+    const int N1 = n + 1;
+    for (int i = wv; i < 64; i += 4) {
+        const int row = bcol0 + i, colx = a0 + lane;
+        tT[i][lane] = (row < N1 && colx < N1) ? X[(size_t)row * ldx + colx] : 0;
+    }
+    if (threadIdx.x < 64) {
+        dA[lane] = (a0 + lane < N1) ? diag[a0 + lane] : 0;
+    } else if (threadIdx.x < 128) {
+        dB[lane] = (bcol0 + lane < N1) ? diag[bcol0 + lane] : 0;
+    }
+    __syncthreads();
+    const int b = bcol0 + lane;
+    unsigned long long gbest = 0;
+    for (int i = wv; i < 64; i += 4) {
+        const int a = a0 + i;
+        const bool valid = a >= 1 && a < b && b <= n;
+        unsigned long long k = 0;
+        if (valid) {
+            const int d = X[(size_t)a * ldx + b] + tT[lane][i] - dA[i] - dB[lane];
+            k = ((unsigned long long)((unsigned)d ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (unsigned)b);
+            const unsigned long long g =
+                ((unsigned long long)((unsigned)d ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (((unsigned)a << 16) | (unsigned)b));
+            gbest = g > gbest ? g : gbest;
+            if (dmat) {
+                dmat[(size_t)a * N1 + b] = d;
+                dmat[(size_t)b * N1 + a] = d;
+            }
+        }
+        k = wave_max_u64(k);
+        if (lane == 0 && k) atomicMax(&rowkey[a], k);
+    }
+    gbest = wave_max_u64(gbest);
+    if (lane == 0) red[wv] = gbest;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long k = max(max(red[0], red[1]), max(red[2], red[3]));
+        if (k) atomicMax(gkey, k);
+    }
+}
+
+// ------------------------------------------------------------ C ABI
+
+static int check_pairs(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count) {
+    for (int64_t i = 0; i < count; i++)
+        if (a[i] < 0 || a[i] > c->n || b[i] < 0 || b[i] > c->n || a[i] == b[i])
+            return set_error(BM_EINVAL, "pair %lld: (%d,%d) invalid", (long long)i, a[i], b[i]);
+    return BM_OK;
+}
+
+extern "C" int bm_eval_swaps(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta) {
+    if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (count < 0) return set_error(BM_EINVAL, "count < 0");
+    if (count == 0) return BM_OK;
+    int rc = check_pairs(c, a, b, count);
+    if (rc) return rc;
+    BM_HIP(hipSetDevice(c->device));
+    const int64_t chunk = 1 << 22;
+    if ((rc = ensure_stage(c, (size_t)chunk * 12))) return rc;
+    int32_t *dA = (int32_t *)c->stage, *dB = dA + chunk, *dD = dB + chunk;
+    for (int64_t off = 0; off < count; off += chunk) {
+        const int64_t m = std::min(chunk, count - off);
+        BM_HIP(hipMemcpyAsync(dA, a + off, m * 4, hipMemcpyHostToDevice, c->stream));
+        BM_HIP(hipMemcpyAsync(dB, b + off, m * 4, hipMemcpyHostToDevice, c->stream));
+        const unsigned blocks = (unsigned)((m * 16 + 255) / 256);
+        BM_LAUNCH(c, "eval_pairs", hipLaunchKernelGGL(k_eval_pairs, dim3(blocks), dim3(256), 0, c->stream, dA, dB, m,
+                                                      dD, c->cptr, c->cr, c->cw, c->cdw, c->Q, c->N1, c->ld));
+        BM_HIP(hipMemcpyAsync(delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipStreamSynchronize(c->stream));
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi, int mode, int32_t *best_b,
+                                int32_t *best_delta) {
+    if (!c || !best_b || !best_delta) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (mode != BM_ROW_FIRST_POSITIVE && mode != BM_ROW_ARGMAX) return set_error(BM_EINVAL, "bad mode %d", mode);
+    if (a < 0 || a > c->n) return set_error(BM_EINVAL, "row %d outside [0,%d]", a, c->n);
+    if (b_lo < 0) b_lo = 0;
+    if (b_hi > c->n) b_hi = c->n;
+    *best_b = 0;
+    *best_delta = mode == BM_ROW_ARGMAX ? INT32_MIN : 0;
+    if (b_lo > b_hi) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    int rc = ensure_diag(c);
+    if (rc) return rc;
+    unsigned long long *res = c->keys + c->N1 + 1;
+    BM_HIP(hipMemsetAsync(res, mode == BM_ROW_FIRST_POSITIVE ? 0xFF : 0x00, 8, c->stream));
+    const int blocks = (b_hi - b_lo + kRowCh) / kRowCh;
+    const size_t lds = (size_t)2 * c->N1 * sizeof(int16_t);
+    if (lds <= 96 * 1024) {
+        BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<true>, dim3(blocks), dim3(256), lds, c->stream, a, b_lo,
+                                                    b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw, c->cdw,
+                                                    c->diag, res));
+    } else {
+        BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<false>, dim3(blocks), dim3(256), 0, c->stream, a, b_lo,
+                                                    b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw, c->cdw,
+                                                    c->diag, res));
+    }
+    unsigned long long k = 0;
+    BM_HIP(hipMemcpyAsync(&k, res, 8, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    if (mode == BM_ROW_FIRST_POSITIVE) {
+        if (k != ~0ull) {
+            *best_b = (int32_t)(k >> 32);
+            *best_delta = (int32_t)(uint32_t)(k & 0xFFFFFFFFu);
+        }
+    } else if (k != 0ull) {
+        *best_delta = (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+        *best_b = (int32_t)(0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFu));
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
+                                  int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    BM_HIP(hipSetDevice(c->device));
+    const int N1 = c->N1;
+    int rc;
+    if (!c->X) {
+        c->ldx = c->ld;
+        if ((rc = dev_alloc(c, (void **)&c->X, (size_t)N1 * c->ldx * 4))) return rc;
+    }
+    if ((rc = ensure_diag(c))) return rc;
+    int32_t *dmat = nullptr;
+    if (delta_matrix) {
+        if ((rc = ensure_stage(c, (size_t)N1 * N1 * 4))) return rc;
+        dmat = (int32_t *)c->stage;
+        BM_HIP(hipMemsetAsync(dmat, 0, (size_t)N1 * N1 * 4, c->stream));
+    }
+    BM_HIP(hipMemsetAsync(c->keys, 0, ((size_t)N1 + 8) * 8, c->stream));
+    const int ncols = c->ld;
+    BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain, dim3((ncols + 2047) / 2048, N1), dim3(256), 0, c->stream, c->cptr,
+                                            c->cr, c->cw, c->Q, c->ld, c->X, c->ldx, ncols));
+    if (c->e1 > 0)
+        BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->e1 + 255) / 256)), dim3(256), 0, c->stream,
+                                                c->esrc, c->edst, c->ew, c->edw, c->e1, c->Q, c->ld, c->X, c->ldx));
+    const int T = (N1 + 63) / 64;
+    unsigned long long *gkey = c->keys + N1;
+    BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, T), dim3(256), 0, c->stream, c->X, c->ldx, c->diag,
+                                               c->n, c->keys, gkey, dmat));
+    std::vector<unsigned long long> keys;
+    unsigned long long g = 0;
+    if (row_best_b || row_best_delta) {
+        keys.resize(N1);
+        BM_HIP(hipMemcpyAsync(keys.data(), c->keys, (size_t)N1 * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    BM_HIP(hipMemcpyAsync(&g, gkey, 8, hipMemcpyDeviceToHost, c->stream));
+    if (delta_matrix)
+        BM_HIP(hipMemcpyAsync(delta_matrix, dmat, (size_t)N1 * N1 * 4, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    for (int v = 0; v < (int)keys.size(); v++) {
+        int32_t bb = 0, dd = INT32_MIN;
+        if (keys[v]) {
+            dd = (int32_t)((uint32_t)(keys[v] >> 32) ^ 0x80000000u);
+            bb = (int32_t)(0xFFFFFFFFu - (uint32_t)(keys[v] & 0xFFFFFFFFu));
+        }
+        if (row_best_b) row_best_b[v] = bb;
+        if (row_best_delta) row_best_delta[v] = dd;
+    }
+    int32_t ba = 0, bb = 0, bd = 0;
+    if (g) {
+        const int32_t d = (int32_t)((uint32_t)(g >> 32) ^ 0x80000000u);
+        const uint32_t ab = 0xFFFFFFFFu - (uint32_t)(g & 0xFFFFFFFFu);
+        if (d > 0) {
+            bd = d;
+            ba = (int32_t)(ab >> 16);
+            bb = (int32_t)(ab & 0xFFFF);
+        }
+    }
+    if (best_a) *best_a = ba;
+    if (best_b) *best_b = bb;
+    if (best_delta) *best_delta = bd;
+    return BM_OK;
+}

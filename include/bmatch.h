@@ -1,0 +1,159 @@
+/*
+ * bmatch.h -- C ABI of the MI355X move-delta engine for brain-match's
+ * weighted directed graph alignment.
+ *
+ * Score of a mapping p (G1 vertex -> G2 vertex, 0 = unmapped):
+ *     S(p) = sum over G1 edge-list entries (u,v,w) of min(w, W2[p(u)][p(v)])
+ * where W2 is G2's dense `short` adjacency (last CSV line for a pair wins).
+ * A move is accepted by the callers when its integer delta is > 0.
+ *
+ * The reference has no library or FFI: its deltas are file-local C functions
+ * called from each program's inner loop.  Each entry point below names the
+ * reference routine (zhihuidu/brain-match, src/file:line) whose results it
+ * reproduces bit for bit, and the call site it replaces.
+ *
+ * Conventions
+ *   - Vertex ids are 0..n (n = largest id); 0 is the reference's "unmapped"
+ *     sentinel, drivers sweep 1..n.  mapping[] has n+1 entries.
+ *   - All pointers are host pointers; the library copies inputs and owns all
+ *     device memory.  Outputs go to caller-provided buffers.
+ *   - Every entry point returns BM_OK (0) or a negative BM_E* code; none
+ *     calls exit().  bm_last_error() gives a message for the calling thread.
+ *   - A context is driven by one host thread at a time.  All device work of
+ *     a context runs on its own HIP stream; calls return after that work is
+ *     complete.
+ */
+#ifndef BMATCH_H
+#define BMATCH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BM_ABI_VERSION 1
+
+enum {
+    BM_OK = 0,
+    BM_EINVAL = -1,  /* bad argument (vertex id, k, mode, size) */
+    BM_ENOMEM = -2,  /* device or host allocation failed / over budget */
+    BM_EHIP = -3,    /* a HIP runtime call failed */
+    BM_ESTATE = -4,  /* graphs or mapping not uploaded yet */
+    BM_ENODEV = -5,  /* no such HIP device */
+    BM_ERANGE = -6,  /* value outside what the engine represents */
+    BM_EIO = -7      /* file I/O or parse failure (host helpers) */
+};
+
+typedef struct bm_ctx bm_ctx;
+
+const char *bm_strerror(int code);
+const char *bm_last_error(void);
+int bm_abi_version(void);
+
+/* Context on HIP device `device`. */
+int bm_create(int device, bm_ctx **out);
+int bm_destroy(bm_ctx *ctx);
+
+/*
+ * Upload G1 and G2 as edge lists in CSV line order (duplicates allowed).
+ * Replaces load_graph_from_csv + add_edge of graph-alignment-optimized.c:
+ * 183-204, 635-679 (and the identical loaders of the permset and
+ * parallel programs): G1 keeps every line in per-vertex out/in lists, the
+ * dense G1 cell and all of G2 take the last line for a pair, truncated to
+ * `short` as the reference's adj_matrix does.  Vertex ids must lie in
+ * [0, n]; G1 weights must be >= -32768.  Device footprint is about
+ * 6*(n+1)^2 bytes + 40*|E1| bytes (+4*(n+1)^2 for bm_sweep_all_pairs).
+ */
+int bm_upload_graphs(bm_ctx *ctx, int32_t n,
+                     const int32_t *g1_src, const int32_t *g1_dst, const int32_t *g1_w, int64_t e1,
+                     const int32_t *g2_src, const int32_t *g2_dst, const int32_t *g2_w, int64_t e2);
+
+int32_t bm_num_vertices(const bm_ctx *ctx);
+/* bytes of device memory held by the context */
+int64_t bm_device_bytes(const bm_ctx *ctx);
+
+/* Set / read the mapping (n+1 entries, values in [0, n]).  set_mapping
+ * rebuilds the device-resident permuted W2 (P[x][y] = W2[p(x)][p(y)]) and
+ * its transpose. */
+int bm_set_mapping(bm_ctx *ctx, const int32_t *mapping);
+int bm_get_mapping(bm_ctx *ctx, int32_t *mapping);
+
+/* S_opt of calculate_alignment_score, graph-alignment-optimized.c:292-307
+ * (called after every accepted swap at :571 and in every dump at :485). */
+int bm_score(bm_ctx *ctx, int64_t *score);
+
+/* Delta_B of calculate_swap_delta, graph-alignment-optimized.c:321-408
+ * (call sites :554, alignment-omp.c:613,
+ * graph-alignment-optimized-parallel.c:557), for count pairs (a[i], b[i]),
+ * a != b, both in [0, n]. */
+int bm_eval_swaps(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta);
+
+/* One row of the sweep: Delta_B(a, b) for b in [b_lo, b_hi], b != a.
+ *   BM_ROW_FIRST_POSITIVE: smallest b with Delta_B > 0 -- the m2 loop of
+ *     graph-alignment-optimized.c:542-579; *best_b = 0 if none.
+ *   BM_ROW_ARGMAX: largest Delta_B, lowest b on ties -- the m2 loop of
+ *     alignment-omp.c:601-622 at one thread; *best_b = 0 if range empty. */
+#define BM_ROW_FIRST_POSITIVE 0
+#define BM_ROW_ARGMAX 1
+int bm_eval_swap_row(bm_ctx *ctx, int32_t a, int32_t b_lo, int32_t b_hi, int mode,
+                     int32_t *best_b, int32_t *best_delta);
+
+/* All-pairs sweep over 1 <= a < b <= n with the current mapping: the
+ * launch + host arg-max of graph-re-alignment-gpu.cu:837-868 (lexicographic
+ * pair order, strict >, so the lowest (a,b) wins ties; *best_delta = 0 and
+ * *best_a = *best_b = 0 when no pair is positive), computed with Delta_B.
+ * Optional outputs (NULL to skip): per-row best b > a and its delta
+ * (row_best_b[a] = 0, row_best_delta[a] = INT32_MIN when a has no b > a),
+ * and the full symmetric delta matrix ((n+1)^2 int32, zero diagonal and
+ * zero row/column 0) for small n. */
+int bm_sweep_all_pairs(bm_ctx *ctx, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
+                       int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix);
+
+/* Delta_G3 of calculateThreeNodeSwapKernel, triple.cu:85-214, per triple
+ * (m1,m2,m3) of distinct vertices in [1, n]: best over the permutations
+ * (strict >, from 0) and its index p (0 = none positive).
+ *   BM_ROT3_ROTATIONS: p in {3, 4}, the two 3-cycles (f2,f3,f1), (f3,f1,f2)
+ *   BM_ROT3_ALL: p in 1..5, as triple.cu */
+#define BM_ROT3_ROTATIONS 0
+#define BM_ROT3_ALL 1
+int bm_eval_rot3(bm_ctx *ctx, const int32_t *triples, int64_t count, int mode,
+                 int32_t *best_delta, int32_t *best_perm);
+
+/* Delta_P of calculate_permutation_delta,
+ * graph-alignment-permset-parallel.c:304-339, evaluated for every one of
+ * the k! permutations of each k-set (2 <= k <= 8, distinct vertices in
+ * [1, n]) in next_permutation order (:342-363, :563-571); per set the best
+ * delta (strict >, from 0; :665-674) and the lexicographic rank of its
+ * permutation (-1 when none is positive).  bm_perm_from_rank decodes it. */
+int bm_eval_permset(bm_ctx *ctx, int32_t k, const int32_t *sets, int64_t count,
+                    int32_t *best_delta, int32_t *best_rank);
+int bm_perm_from_rank(int32_t k, int32_t rank, int32_t *perm);
+
+/* Delta_A of refine-matching-hillclimb.c:244-296 (out-edges of a and b
+ * only, de-duplicated G1, edge a<->b skipped) for count pairs; call site
+ * :320.  Unmapped vertices (mapping 0) contribute 0. */
+int bm_eval_swaps_hillclimb(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta);
+
+/* Moves.  Update the mapping and patch the permuted W2 rows/columns on the
+ * device.  apply_swap: graph-alignment-optimized.c:561-562;
+ * apply_rot3: apply_permutation, triple.cu:903-934 (p in 1..5);
+ * apply_perm: apply_best_permutation,
+ * graph-alignment-permset-parallel.c:508-518 (new p(set[i]) =
+ * old p(set[perm[i]])). */
+int bm_apply_swap(bm_ctx *ctx, int32_t a, int32_t b);
+int bm_apply_rot3(bm_ctx *ctx, int32_t m1, int32_t m2, int32_t m3, int32_t p);
+int bm_apply_perm(bm_ctx *ctx, int32_t k, const int32_t *set, const int32_t *perm);
+
+/* Kernel timing.  When enabled, the engine brackets its kernels with HIP
+ * events on the context stream; bm_kernel_time reports the launch count
+ * and summed device milliseconds of the named kernel since the last reset.
+ * No counterpart in the reference (it logs wall-clock seconds only). */
+int bm_set_timing(bm_ctx *ctx, int enable);
+int bm_kernel_time(bm_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+int bm_reset_timing(bm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BMATCH_H */

@@ -1,0 +1,642 @@
+/*
+ * bm_oracle.c -- CPU restatement of brain-match's move-delta and score
+ * arithmetic.  TEST INFRASTRUCTURE ONLY (the checker, never the product):
+ * loaded by the test suite, the smoke check and the benchmark's CPU-baseline
+ * leg.  Each function cites the reference routine (zhihuidu/brain-match,
+ * src/...) that it restates.
+ *
+ * Pinning (what checks each function, beyond retyping the formula):
+ *   orc_score_opt / orc_score_ref : score-matching built from source
+ *       (golden scores), invariants S(id; G, G) = sum of weights.
+ *   orc_delta_b : Delta_B = exact rescore difference on self-loop-free,
+ *       duplicate-free graphs (brute force, every pair, tiny n); the
+ *       accepted-move log of graph-alignment-optimized built from source.
+ *   orc_first_improvement : the same log (rows, score, improvements every
+ *       20 rows) and the converged mapping file.
+ *   orc_delta_g3 : exact rescore difference for self-loop-free triples.
+ *   orc_delta_p : Delta_P = exact + sum over intra-set edges (brute force).
+ *   orc_delta_a : before/after restatement checked against the exact
+ *       out-edge-only score difference (brute force).
+ */
+#include "bm_oracle.h"
+
+#include <limits.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OMIN(a, b) ((a) < (b) ? (a) : (b))
+
+struct orc_graphs {
+    int n;
+    int64_t e1, e2;
+    /* G1 per-vertex lists in file order (EdgeMap edges / reverse_edges) */
+    int64_t *out_ptr, *in_ptr;
+    int32_t *out_dst, *out_w, *in_src, *in_w;
+    /* dense short matrices, last write wins (add_edge :185) */
+    int16_t *w1d, *w2d;
+    int16_t **w2rows; /* row pointers, the reference's short** layout */
+    /* de-duplicated views with full int weights (score-matching add_edge
+     * :339-358 / hill-climb insert_edge :101-125), rows sorted by target */
+    int64_t *g1dd_ptr, *g2dd_ptr;
+    int32_t *g1dd_dst, *g1dd_w, *g2dd_dst, *g2dd_w;
+};
+
+int orc_n(const orc_graphs *g) { return g->n; }
+
+static void *xcalloc(size_t n, size_t sz) {
+    void *p = calloc(n ? n : 1, sz);
+    if (!p) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return p;
+}
+
+/* stable counting sort of edges by key[] into CSR */
+static void csr_stable(int n, int64_t m, const int32_t *key, const int32_t *other, const int32_t *w,
+                       int64_t **ptr_out, int32_t **oth_out, int32_t **w_out) {
+    int64_t *ptr = xcalloc((size_t)n + 2, sizeof(int64_t));
+    int32_t *oth = xcalloc((size_t)m, sizeof(int32_t));
+    int32_t *ww = xcalloc((size_t)m, sizeof(int32_t));
+    for (int64_t i = 0; i < m; i++) ptr[key[i] + 1]++;
+    for (int v = 0; v <= n; v++) ptr[v + 1] += ptr[v];
+    int64_t *pos = xcalloc((size_t)n + 1, sizeof(int64_t));
+    for (int v = 0; v <= n; v++) pos[v] = ptr[v];
+    for (int64_t i = 0; i < m; i++) {
+        int64_t p = pos[key[i]]++;
+        oth[p] = other[i];
+        ww[p] = w[i];
+    }
+    free(pos);
+    *ptr_out = ptr;
+    *oth_out = oth;
+    *w_out = ww;
+}
+
+/* de-duplicate a stable CSR (last occurrence wins), rows sorted by target */
+typedef struct { int32_t d, w; } dw_t;
+static int cmp_dw(const void *x, const void *y) {
+    const dw_t *a = x, *b = y;
+    return (a->d > b->d) - (a->d < b->d);
+}
+static void dedup_csr(int n, const int64_t *ptr, const int32_t *dst, const int32_t *w,
+                      int64_t **dptr, int32_t **ddst, int32_t **dw) {
+    int64_t m = ptr[n + 1];
+    int64_t *np = xcalloc((size_t)n + 2, sizeof(int64_t));
+    int32_t *nd = xcalloc((size_t)m, sizeof(int32_t));
+    int32_t *nw = xcalloc((size_t)m, sizeof(int32_t));
+    int32_t *last = xcalloc((size_t)n + 1, sizeof(int32_t)); /* last position+1 of dst in row */
+    int64_t k = 0;
+    for (int v = 0; v <= n; v++) {
+        np[v] = k;
+        for (int64_t i = ptr[v]; i < ptr[v + 1]; i++) last[dst[i]] = (int32_t)(i - ptr[v] + 1);
+        int64_t start = k;
+        for (int64_t i = ptr[v]; i < ptr[v + 1]; i++) {
+            if (last[dst[i]] == (int32_t)(i - ptr[v] + 1)) { /* last occurrence */
+                nd[k] = dst[i];
+                nw[k] = w[i];
+                k++;
+            }
+        }
+        for (int64_t i = ptr[v]; i < ptr[v + 1]; i++) last[dst[i]] = 0;
+        dw_t *tmp = xcalloc((size_t)(k - start), sizeof(dw_t));
+        for (int64_t i = start; i < k; i++) tmp[i - start] = (dw_t){nd[i], nw[i]};
+        qsort(tmp, (size_t)(k - start), sizeof(dw_t), cmp_dw);
+        for (int64_t i = start; i < k; i++) {
+            nd[i] = tmp[i - start].d;
+            nw[i] = tmp[i - start].w;
+        }
+        free(tmp);
+    }
+    np[n + 1] = k;
+    free(last);
+    *dptr = np;
+    *ddst = nd;
+    *dw = nw;
+}
+
+orc_graphs *orc_build(int n, const int32_t *s1, const int32_t *d1, const int32_t *w1, int64_t e1,
+                      const int32_t *s2, const int32_t *d2, const int32_t *w2, int64_t e2) {
+    for (int64_t i = 0; i < e1; i++)
+        if (s1[i] < 0 || s1[i] > n || d1[i] < 0 || d1[i] > n) return NULL;
+    for (int64_t i = 0; i < e2; i++)
+        if (s2[i] < 0 || s2[i] > n || d2[i] < 0 || d2[i] > n) return NULL;
+    orc_graphs *g = xcalloc(1, sizeof(*g));
+    g->n = n;
+    g->e1 = e1;
+    g->e2 = e2;
+    csr_stable(n, e1, s1, d1, w1, &g->out_ptr, &g->out_dst, &g->out_w);
+    csr_stable(n, e1, d1, s1, w1, &g->in_ptr, &g->in_src, &g->in_w);
+    size_t N1 = (size_t)n + 1;
+    g->w1d = xcalloc(N1 * N1, sizeof(int16_t));
+    g->w2d = xcalloc(N1 * N1, sizeof(int16_t));
+    for (int64_t i = 0; i < e1; i++) g->w1d[(size_t)s1[i] * N1 + d1[i]] = (int16_t)w1[i];
+    for (int64_t i = 0; i < e2; i++) g->w2d[(size_t)s2[i] * N1 + d2[i]] = (int16_t)w2[i];
+    g->w2rows = xcalloc(N1, sizeof(int16_t *));
+    for (size_t r = 0; r < N1; r++) g->w2rows[r] = g->w2d + r * N1;
+    dedup_csr(n, g->out_ptr, g->out_dst, g->out_w, &g->g1dd_ptr, &g->g1dd_dst, &g->g1dd_w);
+    int64_t *p2;
+    int32_t *o2, *ww2;
+    csr_stable(n, e2, s2, d2, w2, &p2, &o2, &ww2);
+    dedup_csr(n, p2, o2, ww2, &g->g2dd_ptr, &g->g2dd_dst, &g->g2dd_w);
+    free(p2);
+    free(o2);
+    free(ww2);
+    return g;
+}
+
+void orc_free(orc_graphs *g) {
+    if (!g) return;
+    free(g->out_ptr); free(g->in_ptr); free(g->out_dst); free(g->out_w); free(g->in_src); free(g->in_w);
+    free(g->w1d); free(g->w2d); free(g->w2rows);
+    free(g->g1dd_ptr); free(g->g1dd_dst); free(g->g1dd_w);
+    free(g->g2dd_ptr); free(g->g2dd_dst); free(g->g2dd_w);
+    free(g);
+}
+
+#define W1(g, x, y) ((int)(g)->w1d[(size_t)(x) * ((size_t)(g)->n + 1) + (size_t)(y)])
+#define W2(g, x, y) ((int)(g)->w2d[(size_t)(x) * ((size_t)(g)->n + 1) + (size_t)(y)])
+
+/* graph-alignment-optimized.c:292-307 */
+int64_t orc_score_opt(const orc_graphs *g, const int32_t *map) {
+    int64_t s = 0;
+    for (int u = 0; u <= g->n; u++)
+        for (int64_t i = g->out_ptr[u]; i < g->out_ptr[u + 1]; i++) {
+            int v = g->out_dst[i];
+            s += OMIN(g->out_w[i], W2(g, map[u], map[v]));
+        }
+    return s;
+}
+
+/* find_edge in the de-duplicated G2 (score-matching.c:327-337) */
+static int g2_lookup(const orc_graphs *g, int u, int v, int *w) {
+    if (u < 0 || u > g->n || v < 0 || v > g->n) return 0;
+    int64_t lo = g->g2dd_ptr[u], hi = g->g2dd_ptr[u + 1];
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (g->g2dd_dst[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < g->g2dd_ptr[u + 1] && g->g2dd_dst[lo] == v) {
+        *w = g->g2dd_w[lo];
+        return 1;
+    }
+    return 0;
+}
+
+/* score-matching.c:558-598 */
+int64_t orc_score_ref(const orc_graphs *g, const int32_t *map) {
+    int64_t s = 0;
+    for (int u = 0; u <= g->n; u++) {
+        for (int64_t i = g->g1dd_ptr[u]; i < g->g1dd_ptr[u + 1]; i++) {
+            int v = g->g1dd_dst[i];
+            int su = map[u], tv = map[v];
+            if (su < 0 || tv < 0) continue;
+            int w2;
+            if (g2_lookup(g, su, tv, &w2)) s += OMIN(g->g1dd_w[i], w2);
+        }
+    }
+    return s;
+}
+
+/* graph-alignment-optimized.c:321-408 */
+int32_t orc_delta_b(const orc_graphs *g, const int32_t *map, int a, int b) {
+    int f1 = map[a], f2 = map[b];
+    int delta = 0;
+    for (int64_t i = g->out_ptr[a]; i < g->out_ptr[a + 1]; i++) {
+        int d = g->out_dst[i];
+        if (d == b) continue;
+        int w = g->out_w[i], df = map[d];
+        delta += OMIN(w, W2(g, f2, df)) - OMIN(w, W2(g, f1, df));
+    }
+    for (int64_t i = g->in_ptr[a]; i < g->in_ptr[a + 1]; i++) {
+        int s = g->in_src[i];
+        if (s == b) continue;
+        int w = g->in_w[i], sf = map[s];
+        delta += OMIN(w, W2(g, sf, f2)) - OMIN(w, W2(g, sf, f1));
+    }
+    for (int64_t i = g->out_ptr[b]; i < g->out_ptr[b + 1]; i++) {
+        int d = g->out_dst[i];
+        if (d == a) continue;
+        int w = g->out_w[i], df = map[d];
+        delta += OMIN(w, W2(g, f1, df)) - OMIN(w, W2(g, f2, df));
+    }
+    for (int64_t i = g->in_ptr[b]; i < g->in_ptr[b + 1]; i++) {
+        int s = g->in_src[i];
+        if (s == a) continue;
+        int w = g->in_w[i], sf = map[s];
+        delta += OMIN(w, W2(g, sf, f1)) - OMIN(w, W2(g, sf, f2));
+    }
+    int m12 = W1(g, a, b);
+    if (m12 > 0) delta += OMIN(m12, W2(g, f2, f1)) - OMIN(m12, W2(g, f1, f2));
+    int m21 = W1(g, b, a);
+    if (m21 > 0) delta += OMIN(m21, W2(g, f1, f2)) - OMIN(m21, W2(g, f2, f1));
+    return delta;
+}
+
+/* graph-re-alignment-gpu.cu:60-128 (dense scan; self-loops never visited) */
+int32_t orc_delta_g2(const orc_graphs *g, const int32_t *map, int a, int b) {
+    int f1 = map[a], f2 = map[b];
+    int delta = 0;
+    for (int i = 1; i <= g->n; i++) {
+        if (i == a || i == b) continue;
+        int w;
+        if ((w = W1(g, a, i)) > 0) delta += OMIN(w, W2(g, f2, map[i])) - OMIN(w, W2(g, f1, map[i]));
+        if ((w = W1(g, i, a)) > 0) delta += OMIN(w, W2(g, map[i], f2)) - OMIN(w, W2(g, map[i], f1));
+        if ((w = W1(g, b, i)) > 0) delta += OMIN(w, W2(g, f1, map[i])) - OMIN(w, W2(g, f2, map[i]));
+        if ((w = W1(g, i, b)) > 0) delta += OMIN(w, W2(g, map[i], f1)) - OMIN(w, W2(g, map[i], f2));
+    }
+    int m12 = W1(g, a, b);
+    if (m12 > 0) delta += OMIN(m12, W2(g, f2, f1)) - OMIN(m12, W2(g, f1, f2));
+    int m21 = W1(g, b, a);
+    if (m21 > 0) delta += OMIN(m21, W2(g, f1, f2)) - OMIN(m21, W2(g, f2, f1));
+    return delta;
+}
+
+/* permutation table of triple.cu:85-92 */
+static void g3_new(int p, int f1, int f2, int f3, int *n1, int *n2, int *n3) {
+    static const int tab[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    int f[3] = {f1, f2, f3};
+    *n1 = f[tab[p][0]];
+    *n2 = f[tab[p][1]];
+    *n3 = f[tab[p][2]];
+}
+
+/* triple.cu:98-208 */
+int32_t orc_delta_g3(const orc_graphs *g, const int32_t *map, int m1, int m2, int m3, int p) {
+    int m[3] = {m1, m2, m3};
+    int f[3] = {map[m1], map[m2], map[m3]};
+    int nf[3];
+    g3_new(p, f[0], f[1], f[2], &nf[0], &nf[1], &nf[2]);
+    int delta = 0;
+    for (int k = 0; k < 3; k++) {
+        for (int i = 1; i <= g->n; i++) {
+            if (i == m1 || i == m2 || i == m3) continue;
+            int w;
+            if ((w = W1(g, m[k], i)) > 0)
+                delta += OMIN(w, W2(g, nf[k], map[i])) - OMIN(w, W2(g, f[k], map[i]));
+            if ((w = W1(g, i, m[k])) > 0)
+                delta += OMIN(w, W2(g, map[i], nf[k])) - OMIN(w, W2(g, map[i], f[k]));
+        }
+    }
+    /* the six intra-triple cells, :164-207 */
+    static const int pairs[6][2] = {{0, 1}, {1, 0}, {1, 2}, {2, 1}, {2, 0}, {0, 2}};
+    for (int q = 0; q < 6; q++) {
+        int x = pairs[q][0], y = pairs[q][1];
+        int w = W1(g, m[x], m[y]);
+        if (w > 0) delta += OMIN(w, W2(g, nf[x], nf[y])) - OMIN(w, W2(g, f[x], f[y]));
+    }
+    return delta;
+}
+
+int32_t orc_best_g3(const orc_graphs *g, const int32_t *map, int m1, int m2, int m3, int mode, int *best_p) {
+    int best = 0, bp = 0;
+    for (int p = 1; p < 6; p++) {
+        if (mode == 0 && p != 3 && p != 4) continue;
+        int d = orc_delta_g3(g, map, m1, m2, m3, p);
+        if (d > best) {
+            best = d;
+            bp = p;
+        }
+    }
+    *best_p = bp;
+    return best;
+}
+
+/* triple.cu:903-934 */
+void orc_apply_g3(int32_t *map, int m1, int m2, int m3, int p) {
+    int n1, n2, n3;
+    g3_new(p, map[m1], map[m2], map[m3], &n1, &n2, &n3);
+    map[m1] = n1;
+    map[m2] = n2;
+    map[m3] = n3;
+}
+
+/* graph-alignment-permset-parallel.c:304-339 */
+int32_t orc_delta_p(const orc_graphs *g, const int32_t *map, const int32_t *set, const int32_t *perm, int k) {
+    /* new_mapping = copy of current with the set permuted (:307-313); only
+     * the set's entries differ, so look them up in a small table. */
+    int32_t newf[8];
+    for (int i = 0; i < k; i++) newf[i] = map[set[perm[i]]];
+    int delta = 0;
+    for (int i = 0; i < k; i++) {
+        int v = set[i];
+        for (int64_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; j++) {
+            int w = g->out_dst[j], wt = g->out_w[j];
+            int nv = newf[i], nw = map[w];
+            for (int q = 0; q < k; q++)
+                if (set[q] == w) nw = newf[q];
+            delta -= OMIN(wt, W2(g, map[v], map[w]));
+            delta += OMIN(wt, W2(g, nv, nw));
+        }
+        for (int64_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; j++) {
+            int u = g->in_src[j], wt = g->in_w[j];
+            int nv = newf[i], nu = map[u];
+            for (int q = 0; q < k; q++)
+                if (set[q] == u) nu = newf[q];
+            delta -= OMIN(wt, W2(g, map[u], map[v]));
+            delta += OMIN(wt, W2(g, nu, nv));
+        }
+    }
+    return delta;
+}
+
+/* graph-alignment-permset-parallel.c:342-363 */
+int orc_next_permutation(int32_t *arr, int n) {
+    int i = n - 2;
+    while (i >= 0 && arr[i] >= arr[i + 1]) i--;
+    if (i < 0) return 0;
+    int j = n - 1;
+    while (arr[j] <= arr[i]) j--;
+    int32_t t = arr[i];
+    arr[i] = arr[j];
+    arr[j] = t;
+    for (int x = i + 1, y = n - 1; x < y; x++, y--) {
+        t = arr[x];
+        arr[x] = arr[y];
+        arr[y] = t;
+    }
+    return 1;
+}
+
+/* Lehmer-code decode: the rank-th permutation of 0..k-1 in lexicographic
+ * order, which is the order next_permutation visits from the identity. */
+void orc_unrank_perm(int k, int64_t rank, int32_t *perm) {
+    int32_t pool[8];
+    int64_t fact[9] = {1};
+    for (int i = 1; i <= 8; i++) fact[i] = fact[i - 1] * i;
+    for (int i = 0; i < k; i++) pool[i] = i;
+    int left = k;
+    for (int i = 0; i < k; i++) {
+        int64_t f = fact[k - 1 - i];
+        int idx = (int)(rank / f);
+        rank %= f;
+        perm[i] = pool[idx];
+        for (int j = idx; j < left - 1; j++) pool[j] = pool[j + 1];
+        left--;
+    }
+}
+
+/* evaluate all k! permutations in next_permutation order, strict > from 0
+ * (graph-alignment-permset-parallel.c:547-573, :665-674) */
+int32_t orc_best_perm(const orc_graphs *g, const int32_t *map, const int32_t *set, int k, int64_t *rank) {
+    int32_t perm[8];
+    for (int i = 0; i < k; i++) perm[i] = i;
+    int32_t best = 0;
+    int64_t br = -1, r = 0;
+    do {
+        int32_t d = orc_delta_p(g, map, set, perm, k);
+        if (d > best) {
+            best = d;
+            br = r;
+        }
+        r++;
+    } while (orc_next_permutation(perm, k));
+    *rank = br;
+    return best;
+}
+
+/* :508-518 */
+void orc_apply_perm(int32_t *map, const int32_t *set, const int32_t *perm, int k) {
+    int32_t tmp[8];
+    for (int i = 0; i < k; i++) tmp[i] = map[set[perm[i]]];
+    for (int i = 0; i < k; i++) map[set[i]] = tmp[i];
+}
+
+/* calculate_edge_alignment, refine-matching-hillclimb.c:210-220 */
+static int align_a(const orc_graphs *g, int w1, int f, int h) {
+    int w2;
+    if (f <= 0 || h <= 0) return 0;
+    if (!g2_lookup(g, f, h, &w2)) return 0;
+    return OMIN(w1, w2);
+}
+
+/* refine-matching-hillclimb.c:244-296 */
+int32_t orc_delta_a(const orc_graphs *g, const int32_t *map_in, int a, int b) {
+    int32_t fa = map_in[a], fb = map_in[b];
+    int before = 0, after = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        int acc = 0;
+        for (int side = 0; side < 2; side++) {
+            int x = side ? b : a, other = side ? a : b;
+            for (int64_t i = g->g1dd_ptr[x]; i < g->g1dd_ptr[x + 1]; i++) {
+                int t = g->g1dd_dst[i];
+                if (t == other) continue;
+                int mx = (x == a) ? (pass ? fb : fa) : (pass ? fa : fb);
+                int mt = map_in[t];
+                if (t == a) mt = pass ? fb : fa;
+                else if (t == b) mt = pass ? fa : fb;
+                acc += align_a(g, g->g1dd_w[i], mx, mt);
+            }
+        }
+        if (pass == 0) before = acc;
+        else after = acc;
+    }
+    return after - before;
+}
+
+int64_t orc_delta_swap_exact(const orc_graphs *g, const int32_t *map, int a, int b) {
+    int32_t *m2 = xcalloc((size_t)g->n + 1, sizeof(int32_t));
+    memcpy(m2, map, ((size_t)g->n + 1) * sizeof(int32_t));
+    int64_t s0 = orc_score_opt(g, m2);
+    int32_t t = m2[a];
+    m2[a] = m2[b];
+    m2[b] = t;
+    int64_t s1 = orc_score_opt(g, m2);
+    free(m2);
+    return s1 - s0;
+}
+
+/* graph-alignment-optimized.c:495-581 */
+int orc_first_improvement(const orc_graphs *g, int32_t *map, int rows_per_sweep, int max_sweeps,
+                          int32_t *acc, int64_t acc_cap, int64_t *n_acc,
+                          int64_t *logv, int64_t log_cap, int64_t *n_log, int log_every,
+                          int64_t *final_score) {
+    int n = g->n;
+    if (rows_per_sweep < n) rows_per_sweep = n;
+    int64_t score = orc_score_opt(g, map);
+    int64_t rows = 0, improvements = 0, na = 0, nl = 0;
+    int sweeps = 0;
+    for (;;) {
+        if (max_sweeps > 0 && sweeps >= max_sweeps) break;
+        int64_t before = improvements;
+        for (int m1 = 1; m1 <= rows_per_sweep; m1++) {
+            rows++;
+            if (log_every > 0 && rows % log_every == 0) {
+                if (logv && nl < log_cap) {
+                    logv[3 * nl] = rows;
+                    logv[3 * nl + 1] = score;
+                    logv[3 * nl + 2] = improvements;
+                }
+                nl++;
+            }
+            if (m1 > n) continue; /* padding rows never accept (weights >= 0) */
+            for (int m2 = 1; m2 <= n; m2++) {
+                if (m1 == m2) continue;
+                int d = orc_delta_b(g, map, m1, m2);
+                if (d > 0) {
+                    int32_t t = map[m1];
+                    map[m1] = map[m2];
+                    map[m2] = t;
+                    score += d;
+                    improvements++;
+                    if (acc && na < acc_cap) {
+                        acc[3 * na] = m1;
+                        acc[3 * na + 1] = m2;
+                        acc[3 * na + 2] = d;
+                    }
+                    na++;
+                    int64_t verify = orc_score_opt(g, map);
+                    if (verify != score) score = verify; /* :570-576 */
+                }
+            }
+        }
+        sweeps++;
+        if (improvements == before) break;
+    }
+    if (n_acc) *n_acc = na;
+    if (n_log) *n_log = nl;
+    if (final_score) *final_score = score;
+    return sweeps;
+}
+
+/* alignment-omp.c:598-658 at one thread */
+int64_t orc_row_best_pass(const orc_graphs *g, int32_t *map, int32_t *acc, int64_t acc_cap) {
+    int n = g->n;
+    int64_t na = 0;
+    for (int m1 = 1; m1 <= n; m1++) {
+        int maxdelta = -1, maxv = -1;
+        for (int m2 = m1 + 1; m2 <= n; m2++) {
+            int d = orc_delta_b(g, map, m1, m2);
+            if (d > maxdelta) {
+                maxdelta = d;
+                maxv = m2;
+            }
+        }
+        if (maxdelta > 0) {
+            int32_t t = map[m1];
+            map[m1] = map[maxv];
+            map[maxv] = t;
+            if (acc && na < acc_cap) {
+                acc[3 * na] = m1;
+                acc[3 * na + 1] = maxv;
+                acc[3 * na + 2] = maxdelta;
+            }
+            na++;
+        }
+    }
+    return na;
+}
+
+/* graph-re-alignment-gpu.cu:859-868 arg-max over the lexicographic pair list */
+int32_t orc_all_pairs(const orc_graphs *g, const int32_t *map, int32_t *best_a, int32_t *best_b,
+                      int32_t *row_best_b, int32_t *row_best_d, int32_t *delta_matrix) {
+    int n = g->n;
+    int32_t best = 0, ba = 0, bb = 0;
+    size_t N1 = (size_t)n + 1;
+    if (delta_matrix) memset(delta_matrix, 0, N1 * N1 * sizeof(int32_t));
+    for (int a = 0; a <= n; a++) {
+        if (row_best_b) row_best_b[a] = 0;
+        if (row_best_d) row_best_d[a] = INT32_MIN;
+    }
+    for (int a = 1; a <= n; a++) {
+        int32_t rb = INT32_MIN, rbb = 0;
+        for (int b = a + 1; b <= n; b++) {
+            int32_t d = orc_delta_b(g, map, a, b);
+            if (delta_matrix) {
+                delta_matrix[(size_t)a * N1 + b] = d;
+                delta_matrix[(size_t)b * N1 + a] = d;
+            }
+            if (d > rb) {
+                rb = d;
+                rbb = b;
+            }
+            if (d > best) {
+                best = d;
+                ba = a;
+                bb = b;
+            }
+        }
+        if (row_best_b) row_best_b[a] = rbb;
+        if (row_best_d) row_best_d[a] = rb;
+    }
+    if (best_a) *best_a = ba;
+    if (best_b) *best_b = bb;
+    return best;
+}
+
+/* ---- reference-layout OpenMP row-best (CPU baseline timing) ---------- */
+
+/* calculate_swap_delta exactly as graph-alignment-optimized.c:321-408,
+ * reading W2 through short** row pointers as the reference does. */
+static int ref_layout_delta(const orc_graphs *g, int16_t *const *adj_f, const int32_t *mapping, int m1, int m2) {
+    int f1 = mapping[m1], f2 = mapping[m2];
+    int delta = 0;
+    const int64_t *op = g->out_ptr, *ip = g->in_ptr;
+    for (int64_t i = op[m1]; i < op[m1 + 1]; i++) {
+        int d = g->out_dst[i];
+        if (d == m2) continue;
+        int w = g->out_w[i], df = mapping[d];
+        delta += OMIN(w, adj_f[f2][df]) - OMIN(w, adj_f[f1][df]);
+    }
+    for (int64_t i = ip[m1]; i < ip[m1 + 1]; i++) {
+        int s = g->in_src[i];
+        if (s == m2) continue;
+        int w = g->in_w[i], sf = mapping[s];
+        delta += OMIN(w, adj_f[sf][f2]) - OMIN(w, adj_f[sf][f1]);
+    }
+    for (int64_t i = op[m2]; i < op[m2 + 1]; i++) {
+        int d = g->out_dst[i];
+        if (d == m1) continue;
+        int w = g->out_w[i], df = mapping[d];
+        delta += OMIN(w, adj_f[f1][df]) - OMIN(w, adj_f[f2][df]);
+    }
+    for (int64_t i = ip[m2]; i < ip[m2 + 1]; i++) {
+        int s = g->in_src[i];
+        if (s == m1) continue;
+        int w = g->in_w[i], sf = mapping[s];
+        delta += OMIN(w, adj_f[sf][f1]) - OMIN(w, adj_f[sf][f2]);
+    }
+    int m12 = W1(g, m1, m2);
+    if (m12 > 0) delta += OMIN(m12, adj_f[f2][f1]) - OMIN(m12, adj_f[f1][f2]);
+    int m21 = W1(g, m2, m1);
+    if (m21 > 0) delta += OMIN(m21, adj_f[f1][f2]) - OMIN(m21, adj_f[f2][f1]);
+    return delta;
+}
+
+/* alignment-omp.c:598-622: one parallel-for per row over m2 > m1 with a
+ * critical-section arg-max per evaluation. */
+int64_t orc_omp_rowbest_rows(const orc_graphs *g, const int32_t *map, int row_lo, int row_hi, int threads,
+                             int64_t *checksum) {
+    int n = g->n;
+    int64_t evals = 0, cs = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    int16_t *const *adj_f = g->w2rows;
+    for (int m1 = row_lo; m1 < row_hi && m1 <= n; m1++) {
+        if (m1 < 1) continue;
+        int maxdelta = -1, maxv = -1;
+#pragma omp parallel for shared(maxdelta, maxv) schedule(dynamic)
+        for (int m2 = m1 + 1; m2 <= n; m2++) {
+            int delta = ref_layout_delta(g, adj_f, map, m1, m2);
+#pragma omp critical
+            {
+                if (delta > maxdelta) {
+                    maxdelta = delta;
+                    maxv = m2;
+                }
+            }
+        }
+        evals += n - m1;
+        cs += (int64_t)maxdelta * 1000003 + maxv;
+    }
+    if (checksum) *checksum = cs;
+    return evals;
+}

@@ -1,0 +1,124 @@
+/*
+ * bm_oracle.h -- CPU restatement of brain-match's move-delta and score
+ * arithmetic.  TEST INFRASTRUCTURE ONLY: this code is the checker that the
+ * HIP path is compared against.  Only the test suite, the smoke check and the
+ * CPU-baseline leg of the benchmark may load it; the product library never
+ * links or calls it.
+ *
+ * Every function restates one reference routine, cited by file:line into
+ * zhihuidu/brain-match (src/...).  The data layout deliberately follows the
+ * reference: per-vertex out/in lists in file order (the reference's EdgeMap,
+ * graph-alignment-optimized.c:41-53, 183-204) and dense `short` adjacency
+ * matrices of (n+1)^2 cells where the last CSV line for a pair wins
+ * (graph-alignment-optimized.c:185).
+ *
+ * Vertex ids are 0..n; index 0 is the "unmapped" sentinel of the reference
+ * (mapping[] is calloc'ed, W2 row/column 0 stay zero unless an input names
+ * vertex 0).
+ */
+#ifndef BM_ORACLE_H
+#define BM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_graphs orc_graphs;
+
+/* Build from edge lists in file order.  n = largest vertex id. */
+orc_graphs *orc_build(int n,
+                      const int32_t *s1, const int32_t *d1, const int32_t *w1, int64_t e1,
+                      const int32_t *s2, const int32_t *d2, const int32_t *w2, int64_t e2);
+void orc_free(orc_graphs *g);
+int orc_n(const orc_graphs *g);
+
+/* S_opt: graph-alignment-optimized.c:292-307 (duplicates in G1 counted,
+ * W2 through the dense short matrix). */
+int64_t orc_score_opt(const orc_graphs *g, const int32_t *map);
+
+/* S_ref: score-matching.c:558-598 with the loaders :339-358 (G1 and G2
+ * de-duplicated, last weight wins, full int weights, absent G2 edge -> 0).
+ * map[x] < 0 means "not in the matching file". */
+int64_t orc_score_ref(const orc_graphs *g, const int32_t *map);
+
+/* Delta_B: graph-alignment-optimized.c:321-408. */
+int32_t orc_delta_b(const orc_graphs *g, const int32_t *map, int a, int b);
+
+/* Delta_G2: graph-re-alignment-gpu.cu:60-128 / quick.cu:81-130 (dense scan
+ * over i not in {a,b}; self-loops omitted). */
+int32_t orc_delta_g2(const orc_graphs *g, const int32_t *map, int a, int b);
+
+/* Delta_G3 for permutation p in 1..5 of triple (m1,m2,m3):
+ * triple.cu:85-208.  p=3 -> (f2,f3,f1), p=4 -> (f3,f1,f2) are the rotations. */
+int32_t orc_delta_g3(const orc_graphs *g, const int32_t *map, int m1, int m2, int m3, int p);
+/* Best over p (strict >, from 0; triple.cu:94-214).  mode 0: p in {3,4};
+ * mode 1: p in 1..5.  Returns best delta, *best_p = 0 when none positive. */
+int32_t orc_best_g3(const orc_graphs *g, const int32_t *map, int m1, int m2, int m3, int mode, int *best_p);
+/* apply_permutation, triple.cu:903-934 */
+void orc_apply_g3(int32_t *map, int m1, int m2, int m3, int p);
+
+/* Delta_P: graph-alignment-permset-parallel.c:304-339 (intra-set edges are
+ * counted twice, exactly as written). */
+int32_t orc_delta_p(const orc_graphs *g, const int32_t *map, const int32_t *set, const int32_t *perm, int k);
+/* next_permutation, graph-alignment-permset-parallel.c:342-363 */
+int orc_next_permutation(int32_t *arr, int n);
+/* rank -> permutation in next_permutation order from the identity
+ * (generate_permutation_samples :563-571). */
+void orc_unrank_perm(int k, int64_t rank, int32_t *perm);
+/* Best permutation of one set: strict > from 0 in lexicographic order
+ * (:665-674).  Returns delta; *rank = -1 when none positive. */
+int32_t orc_best_perm(const orc_graphs *g, const int32_t *map, const int32_t *set, int k, int64_t *rank);
+/* apply_best_permutation :508-518 */
+void orc_apply_perm(int32_t *map, const int32_t *set, const int32_t *perm, int k);
+
+/* Delta_A: refine-matching-hillclimb.c:244-296 over the hash-table
+ * adjacency (:101-137: de-duplicated, last weight wins; absent edge -> 0).
+ * map[x] <= 0 is unmapped (contributes 0, as calculate_total_score skips it). */
+int32_t orc_delta_a(const orc_graphs *g, const int32_t *map, int a, int b);
+
+/* Exact change of S_opt for swapping a,b (full rescore, brute force). */
+int64_t orc_delta_swap_exact(const orc_graphs *g, const int32_t *map, int a, int b);
+
+/* --- drivers ------------------------------------------------------------ */
+
+/* First-improvement sweep of graph-alignment-optimized.c:513-581.  Rows
+ * m1 = 1..rows_per_sweep, m2 = 1..rows_per_sweep (m2 != m1); a swap is
+ * applied as soon as Delta_B > 0 and the current score is replaced by a full
+ * rescore when they disagree (:570-576).  rows_per_sweep >= n emulates the
+ * reference's fixed NUM_NODES padding (rows above n never accept when all
+ * weights are >= 0).  Stops after a sweep without an accepted swap or after
+ * max_sweeps.  Records accepted moves (a,b,delta) into acc[3*i] (up to
+ * acc_cap) and a progress sample (rows processed, score, improvements) every
+ * log_every rows into logv[3*j] (up to log_cap), like :519-534.
+ * Returns the number of sweeps run; *n_acc / *n_log get the counts. */
+int orc_first_improvement(const orc_graphs *g, int32_t *map, int rows_per_sweep, int max_sweeps,
+                          int32_t *acc, int64_t acc_cap, int64_t *n_acc,
+                          int64_t *logv, int64_t log_cap, int64_t *n_log, int log_every,
+                          int64_t *final_score);
+
+/* Row-best pass of alignment-omp.c:598-658 at one thread: for m1 = 1..n,
+ * arg-max over m2 > m1 (strict >, init -1, lowest m2 wins), apply if > 0.
+ * Returns number of applied swaps; acc as above. */
+int64_t orc_row_best_pass(const orc_graphs *g, int32_t *map, int32_t *acc, int64_t acc_cap);
+
+/* All-pairs arg-max of graph-re-alignment-gpu.cu:859-868 over pairs
+ * a < b in lexicographic order (strict >, from 0).  Uses Delta_B.  Fills
+ * row_best_b/row_best_d (index a, for b > a; strict > from INT_MIN so the
+ * lowest b wins ties) when non-NULL.  Returns best delta (0 if none > 0). */
+int32_t orc_all_pairs(const orc_graphs *g, const int32_t *map, int32_t *best_a, int32_t *best_b,
+                      int32_t *row_best_b, int32_t *row_best_d, int32_t *delta_matrix);
+
+/* --- the reference OpenMP row-best loop, reference data layout ----------- */
+/* Times rows [row_lo, row_hi) of alignment-omp.c:598-622 (Delta_B of
+ * :321-408 over EdgeMap lists and `short` rows, `#pragma omp parallel for
+ * schedule(dynamic)` over m2 > m1, critical arg-max) with the mapping held
+ * fixed (no apply), and returns the number of Delta evaluations done.
+ * threads <= 0 uses the OpenMP default. */
+int64_t orc_omp_rowbest_rows(const orc_graphs *g, const int32_t *map, int row_lo, int row_hi, int threads,
+                             int64_t *checksum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

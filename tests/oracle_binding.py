@@ -1,0 +1,176 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the test suite.
+
+The oracle is the checker: a C restatement of the reference's delta and
+score arithmetic.  Only tests, the smoke check and the benchmark's
+CPU-baseline leg load it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_L = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "liboracle.so"], check=True)
+
+
+def lib():
+    global _L
+    if _L is not None:
+        return _L
+    if not os.path.exists(ORACLE_LIB):
+        build()
+    L = C.CDLL(ORACLE_LIB)
+    vp = C.c_void_p
+    sig = {
+        "orc_build": (vp, [C.c_int, _i32p, _i32p, _i32p, C.c_int64, _i32p, _i32p, _i32p, C.c_int64]),
+        "orc_free": (None, [vp]),
+        "orc_score_opt": (C.c_int64, [vp, _i32p]),
+        "orc_score_ref": (C.c_int64, [vp, _i32p]),
+        "orc_delta_b": (C.c_int32, [vp, _i32p, C.c_int, C.c_int]),
+        "orc_delta_g2": (C.c_int32, [vp, _i32p, C.c_int, C.c_int]),
+        "orc_delta_g3": (C.c_int32, [vp, _i32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "orc_best_g3": (C.c_int32, [vp, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+        "orc_apply_g3": (None, [_i32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "orc_delta_p": (C.c_int32, [vp, _i32p, _i32p, _i32p, C.c_int]),
+        "orc_next_permutation": (C.c_int, [_i32p, C.c_int]),
+        "orc_unrank_perm": (None, [C.c_int, C.c_int64, _i32p]),
+        "orc_best_perm": (C.c_int32, [vp, _i32p, _i32p, C.c_int, C.POINTER(C.c_int64)]),
+        "orc_apply_perm": (None, [_i32p, _i32p, _i32p, C.c_int]),
+        "orc_delta_a": (C.c_int32, [vp, _i32p, C.c_int, C.c_int]),
+        "orc_delta_swap_exact": (C.c_int64, [vp, _i32p, C.c_int, C.c_int]),
+        "orc_first_improvement": (C.c_int, [vp, _i32p, C.c_int, C.c_int, _i32p, C.c_int64, C.POINTER(C.c_int64),
+                                            _i64p, C.c_int64, C.POINTER(C.c_int64), C.c_int,
+                                            C.POINTER(C.c_int64)]),
+        "orc_row_best_pass": (C.c_int64, [vp, _i32p, _i32p, C.c_int64]),
+        "orc_all_pairs": (C.c_int32, [vp, _i32p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp, vp, vp]),
+        "orc_omp_rowbest_rows": (C.c_int64, [vp, _i32p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _L = L
+    return L
+
+
+def _i32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+class Oracle:
+    def __init__(self, n, g1, g2):
+        self.n = int(n)
+        s1, d1, w1 = (_i32(a) for a in g1)
+        s2, d2, w2 = (_i32(a) for a in g2)
+        self._h = lib().orc_build(self.n, s1, d1, w1, len(s1), s2, d2, w2, len(s2))
+        if not self._h:
+            raise ValueError("orc_build rejected the graphs (vertex id outside [0, n])")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_free(self._h)
+            self._h = None
+
+    def score_opt(self, m):
+        return lib().orc_score_opt(self._h, _i32(m))
+
+    def score_ref(self, m):
+        return lib().orc_score_ref(self._h, _i32(m))
+
+    def delta_b(self, m, a, b):
+        return lib().orc_delta_b(self._h, _i32(m), a, b)
+
+    def delta_g2(self, m, a, b):
+        return lib().orc_delta_g2(self._h, _i32(m), a, b)
+
+    def delta_g3(self, m, m1, m2, m3, p):
+        return lib().orc_delta_g3(self._h, _i32(m), m1, m2, m3, p)
+
+    def best_g3(self, m, m1, m2, m3, mode):
+        bp = C.c_int()
+        d = lib().orc_best_g3(self._h, _i32(m), m1, m2, m3, mode, C.byref(bp))
+        return d, bp.value
+
+    def delta_p(self, m, sset, perm):
+        return lib().orc_delta_p(self._h, _i32(m), _i32(sset), _i32(perm), len(sset))
+
+    def best_perm(self, m, sset):
+        r = C.c_int64()
+        d = lib().orc_best_perm(self._h, _i32(m), _i32(sset), len(sset), C.byref(r))
+        return d, r.value
+
+    def delta_a(self, m, a, b):
+        return lib().orc_delta_a(self._h, _i32(m), a, b)
+
+    def delta_exact(self, m, a, b):
+        return lib().orc_delta_swap_exact(self._h, _i32(m), a, b)
+
+    def first_improvement(self, m, rows_per_sweep=0, max_sweeps=0, log_every=20, acc_cap=1 << 20,
+                          log_cap=1 << 20):
+        m = _i32(m).copy()
+        acc = np.zeros(3 * acc_cap, np.int32)
+        logv = np.zeros(3 * log_cap, np.int64)
+        na, nl, fs = C.c_int64(), C.c_int64(), C.c_int64()
+        sweeps = lib().orc_first_improvement(self._h, m, rows_per_sweep, max_sweeps, acc, acc_cap, C.byref(na),
+                                             logv, log_cap, C.byref(nl), log_every, C.byref(fs))
+        return {"mapping": m, "accepted": acc[: 3 * min(na.value, acc_cap)].reshape(-1, 3),
+                "log": logv[: 3 * min(nl.value, log_cap)].reshape(-1, 3), "sweeps": sweeps,
+                "final_score": fs.value}
+
+    def row_best_pass(self, m, acc_cap=1 << 20):
+        m = _i32(m).copy()
+        acc = np.zeros(3 * acc_cap, np.int32)
+        k = lib().orc_row_best_pass(self._h, m, acc, acc_cap)
+        return m, acc[: 3 * min(k, acc_cap)].reshape(-1, 3)
+
+    def all_pairs(self, m, matrix=False):
+        n1 = self.n + 1
+        ba, bb = C.c_int32(), C.c_int32()
+        rb = np.zeros(n1, np.int32)
+        rd = np.zeros(n1, np.int32)
+        dm = np.zeros((n1, n1), np.int32) if matrix else None
+        d = lib().orc_all_pairs(self._h, _i32(m), C.byref(ba), C.byref(bb), rb.ctypes.data_as(C.c_void_p),
+                                rd.ctypes.data_as(C.c_void_p),
+                                dm.ctypes.data_as(C.c_void_p) if dm is not None else None)
+        return {"best": (ba.value, bb.value, d), "row_best_b": rb, "row_best_delta": rd, "matrix": dm}
+
+    def omp_rowbest_rows(self, m, lo, hi, threads=0):
+        cs = C.c_int64()
+        ev = lib().orc_omp_rowbest_rows(self._h, _i32(m), lo, hi, threads, C.byref(cs))
+        return ev, cs.value
+
+
+def unrank_perm(k, rank):
+    out = np.zeros(k, np.int32)
+    lib().orc_unrank_perm(k, rank, out)
+    return out
+
+
+def next_permutation(arr):
+    a = _i32(arr).copy()
+    ok = lib().orc_next_permutation(a, len(a))
+    return bool(ok), a
+
+
+def apply_g3(m, m1, m2, m3, p):
+    m = _i32(m).copy()
+    lib().orc_apply_g3(m, m1, m2, m3, p)
+    return m
+
+
+def apply_perm(m, sset, perm):
+    m = _i32(m).copy()
+    lib().orc_apply_perm(m, _i32(sset), _i32(perm), len(sset))
+    return m

@@ -1,0 +1,100 @@
+"""3-vertex (Delta_G3), k-set (Delta_P) and hill-climb (Delta_A) parity on the GPU."""
+import numpy as np
+import pytest
+
+import bmatch
+import oracle_binding as ob
+from conftest import random_mapping, random_multigraph
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(seed, n=40, m=400, dup=0.1, loops=0.05, unm=0.0):
+    g1 = random_multigraph(seed, n, m, dup, loops, 40)
+    g2 = random_multigraph(seed + 1, n, m, dup, loops, 40)
+    mp = random_mapping(seed + 2, n, unm)
+    e = bmatch.Engine(0)
+    e.upload(n, g1, g2)
+    e.set_mapping(mp)
+    return n, g1, g2, mp, e, ob.Oracle(n, g1, g2)
+
+
+@pytest.mark.parametrize("mode", [bmatch.BM_ROT3_ROTATIONS, bmatch.BM_ROT3_ALL])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_rot3_matches_oracle(mode, seed):
+    n, g1, g2, mp, e, o = setup(seed, unm=0.1)
+    rng = np.random.default_rng(seed)
+    tri = np.array([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(600)], np.int32)
+    with e:
+        d, p = e.eval_rot3(tri, mode)
+    want = [o.best_g3(mp, *map(int, t), mode) for t in tri]
+    np.testing.assert_array_equal(d, [w[0] for w in want])
+    np.testing.assert_array_equal(p, [w[1] for w in want])
+
+
+def test_rot3_each_permutation():
+    # every p of triple.cu:85-92 through the ALL mode: the best over p in 1..5
+    # must equal the max of the oracle's individual deltas
+    n, g1, g2, mp, e, o = setup(9, n=20, m=300)
+    rng = np.random.default_rng(9)
+    tri = np.array([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(300)], np.int32)
+    with e:
+        d, p = e.eval_rot3(tri, bmatch.BM_ROT3_ALL)
+    for t, dd, pp in zip(tri, d, p):
+        vals = [o.delta_g3(mp, *map(int, t), q) for q in range(1, 6)]
+        best = max([0] + vals)
+        assert dd == best
+        if best > 0:
+            assert pp == 1 + vals.index(best)
+
+
+def test_apply_rot3():
+    n, g1, g2, mp, e, o = setup(4)
+    cur = mp.copy()
+    with e:
+        for p in range(1, 6):
+            e.apply_rot3(3, 7, 11, p)
+            cur = ob.apply_g3(cur, 3, 7, 11, p)
+            np.testing.assert_array_equal(e.mapping(), cur)
+            assert e.score() == o.score_opt(cur)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 8])
+def test_permset_matches_oracle(k):
+    n, g1, g2, mp, e, o = setup(10 + k, n=30, m=350)
+    sets = bmatch.gen_sets(100 + k, n, k, 12 if k == 8 else 60)
+    with e:
+        d, r = e.eval_permset(sets)
+    for s, dd, rr in zip(sets, d, r):
+        wd, wr = o.best_perm(mp, s)
+        assert (dd, rr) == (wd, wr), (s, dd, rr, wd, wr)
+
+
+def test_permset_rank_decoding_and_apply():
+    n, g1, g2, mp, e, o = setup(31, n=30, m=350)
+    sets = bmatch.gen_sets(7, n, 4, 40)
+    cur = mp.copy()
+    with e:
+        d, r = e.eval_permset(sets)
+        for s, dd, rr in zip(sets, d, r):
+            if rr < 0:
+                continue
+            perm = bmatch.perm_from_rank(4, int(rr))
+            np.testing.assert_array_equal(perm, ob.unrank_perm(4, int(rr)))
+            assert o.delta_p(cur, s, perm) == dd
+            e.apply_perm(s, perm)
+            cur = ob.apply_perm(cur, s, perm)
+            np.testing.assert_array_equal(e.mapping(), cur)
+            assert e.score() == o.score_opt(cur)
+            break
+
+
+def test_hillclimb_delta_matches_oracle():
+    # Delta_A: de-duplicated G1 out-lists; G2 weights in [1, 32767] so the
+    # dense short cell and the hash-table lookup agree
+    n, g1, g2, mp, e, o = setup(41, n=35, m=400, unm=0.15)
+    a, b = np.nonzero(~np.eye(n + 1, dtype=bool))
+    with e:
+        got = e.eval_swaps_hillclimb(a, b)
+    want = [o.delta_a(mp, int(x), int(y)) for x, y in zip(a, b)]
+    np.testing.assert_array_equal(got, want)

@@ -1,0 +1,158 @@
+"""Delta_B parity on the GPU: every swap kernel against the oracle.
+
+Bar: bit-exact integers (graph-alignment-optimized.c:321-408), including
+self-loops, repeated CSV lines (dense cell = last line), unmapped vertices
+(mapping 0), non-bijective mappings and vertex 0.
+"""
+import numpy as np
+import pytest
+
+import bmatch
+import oracle_binding as ob
+from conftest import random_mapping, random_multigraph
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (name, n, m, dup, loops, unmapped, bijective, zero_vertex)
+    ("simple", 40, 300, 0.0, 0.0, 0.0, True, False),
+    ("dups_loops", 50, 500, 0.2, 0.05, 0.0, True, False),
+    ("unmapped", 45, 400, 0.1, 0.05, 0.2, True, False),
+    ("nonbijective", 30, 250, 0.1, 0.05, 0.1, False, False),
+    ("vertex0", 35, 300, 0.1, 0.05, 0.0, True, True),
+    ("dense", 24, 400, 0.3, 0.1, 0.0, True, False),
+]
+
+
+def make_case(case, seed=7):
+    name, n, m, dup, loops, unm, bij, zero = case
+    g1 = random_multigraph(seed, n, m, dup, loops, 30, zero)
+    g2 = random_multigraph(seed + 1, n, m, dup, loops, 30, zero)
+    mp = random_mapping(seed + 2, n, unm, bij)
+    if zero:
+        mp[0] = 3  # the reference lets a mapping line name vertex 0
+    return n, g1, g2, mp
+
+
+def engine_for(n, g1, g2, mp):
+    e = bmatch.Engine(0)
+    e.upload(n, g1, g2)
+    e.set_mapping(mp)
+    return e
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_score_and_pairs(case):
+    n, g1, g2, mp = make_case(case)
+    o = ob.Oracle(n, g1, g2)
+    with engine_for(n, g1, g2, mp) as e:
+        assert e.score() == o.score_opt(mp)
+        a, b = np.nonzero(~np.eye(n + 1, dtype=bool))
+        got = e.eval_swaps(a, b)
+        want = np.array([o.delta_b(mp, x, y) for x, y in zip(a, b)], np.int32)
+        np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_all_pairs_matrix(case):
+    n, g1, g2, mp = make_case(case)
+    o = ob.Oracle(n, g1, g2)
+    ref = o.all_pairs(mp, matrix=True)
+    with engine_for(n, g1, g2, mp) as e:
+        got = e.sweep_all_pairs(rows=True, matrix=True)
+    np.testing.assert_array_equal(got["matrix"], ref["matrix"])
+    assert got["best"] == ref["best"]
+    np.testing.assert_array_equal(got["row_best_b"][1:n], ref["row_best_b"][1:n])
+    np.testing.assert_array_equal(got["row_best_delta"][1:n], ref["row_best_delta"][1:n])
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_row_kernel(case):
+    n, g1, g2, mp = make_case(case)
+    o = ob.Oracle(n, g1, g2)
+    rng = np.random.default_rng(3)
+    with engine_for(n, g1, g2, mp) as e:
+        for a in range(0, n + 1):
+            lo = int(rng.integers(0, n + 1))
+            hi = int(rng.integers(lo, n + 1))
+            ds = {b: o.delta_b(mp, a, b) for b in range(lo, hi + 1) if b != a}
+            pos = [b for b in sorted(ds) if ds[b] > 0]
+            want_fp = (pos[0], ds[pos[0]]) if pos else (0, 0)
+            assert e.eval_swap_row(a, lo, hi, bmatch.BM_ROW_FIRST_POSITIVE) == want_fp, (a, lo, hi)
+            if ds:
+                bb = max(ds, key=lambda b: (ds[b], -b))
+                want_am = (bb, ds[bb])
+            else:
+                want_am = (0, np.iinfo(np.int32).min)
+            assert e.eval_swap_row(a, lo, hi, bmatch.BM_ROW_ARGMAX) == want_am, (a, lo, hi)
+
+
+def test_apply_swap_patches_state():
+    n, g1, g2, mp = make_case(CASES[1])
+    o = ob.Oracle(n, g1, g2)
+    rng = np.random.default_rng(11)
+    cur = mp.copy()
+    with engine_for(n, g1, g2, mp) as e:
+        for _ in range(25):
+            a, b = rng.choice(np.arange(1, n + 1), 2, replace=False)
+            e.apply_swap(int(a), int(b))
+            cur[a], cur[b] = cur[b], cur[a]
+            np.testing.assert_array_equal(e.mapping(), cur)
+            assert e.score() == o.score_opt(cur)
+            x, y = rng.integers(1, n + 1, 64), rng.integers(1, n + 1, 64)
+            keep = x != y
+            got = e.eval_swaps(x[keep], y[keep])
+            want = [o.delta_b(cur, int(p), int(q)) for p, q in zip(x[keep], y[keep])]
+            np.testing.assert_array_equal(got, want)
+
+
+def test_first_improvement_trace_matches_oracle():
+    inst = bmatch.Instance.powerlaw(5, 120, 1500)
+    o = ob.Oracle(inst.n, inst.g1, inst.g2)
+    ref = o.first_improvement(inst.p0)
+    with engine_for(inst.n, inst.g1, inst.g2, inst.p0) as e:
+        acc, st = e.first_improvement(verify=True)
+        assert st["mismatches"] == 0
+        np.testing.assert_array_equal(acc, ref["accepted"])
+        np.testing.assert_array_equal(e.mapping(), ref["mapping"])
+        assert st["final_score"] == ref["final_score"] == o.score_opt(ref["mapping"])
+        assert st["sweeps"] == ref["sweeps"]
+
+
+def test_first_improvement_with_loops_and_dups():
+    n, g1, g2, mp = make_case(CASES[1], seed=21)
+    o = ob.Oracle(n, g1, g2)
+    ref = o.first_improvement(mp, max_sweeps=3)
+    with engine_for(n, g1, g2, mp) as e:
+        acc, st = e.first_improvement(max_sweeps=3, verify=True)
+        np.testing.assert_array_equal(acc, ref["accepted"])
+        np.testing.assert_array_equal(e.mapping(), ref["mapping"])
+        assert st["final_score"] == ref["final_score"]
+
+
+def test_row_best_pass_matches_oracle():
+    inst = bmatch.Instance.powerlaw(6, 150, 2000)
+    o = ob.Oracle(inst.n, inst.g1, inst.g2)
+    m_ref, acc_ref = o.row_best_pass(inst.p0)
+    with engine_for(inst.n, inst.g1, inst.g2, inst.p0) as e:
+        acc, st = e.row_best(max_sweeps=1)
+        np.testing.assert_array_equal(acc, acc_ref)
+        np.testing.assert_array_equal(e.mapping(), m_ref)
+
+
+def test_global_best_steps_match_oracle():
+    inst = bmatch.Instance.powerlaw(8, 80, 900)
+    o = ob.Oracle(inst.n, inst.g1, inst.g2)
+    cur = inst.p0.copy()
+    want = []
+    for _ in range(15):
+        r = o.all_pairs(cur)
+        a, b, d = r["best"]
+        if d <= 0:
+            break
+        want.append((a, b, d))
+        cur[a], cur[b] = cur[b], cur[a]
+    with engine_for(inst.n, inst.g1, inst.g2, inst.p0) as e:
+        acc, st = e.global_best(max_steps=15)
+        np.testing.assert_array_equal(acc[: len(want)], np.array(want, np.int32).reshape(-1, 3))
+        np.testing.assert_array_equal(e.mapping(), cur)
