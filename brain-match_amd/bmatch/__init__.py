@@ -75,6 +75,13 @@ def lib():
         "bm_instance_free": (None, [C.c_void_p]),
         "bm_gen_triples_2hop": (C.c_int, [C.c_uint64, C.c_void_p, C.c_int64, _i32p]),
         "bm_gen_sets": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int64, _i32p]),
+        "bm_edges_read_csv": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p]),
+        "bm_edges_free": (None, [C.c_void_p]),
+        "bm_pairs_read_optimized": (C.c_int, [C.c_char_p, C.c_void_p]),
+        "bm_pairs_read_permset": (C.c_int, [C.c_char_p, C.c_void_p]),
+        "bm_pairs_read_plain": (C.c_int, [C.c_char_p, C.c_void_p]),
+        "bm_pairs_free": (None, [C.c_void_p]),
+        "bm_mapping_write_mf": (C.c_int, [C.c_char_p, _i32p, C.c_int32]),
         "bm_drive_first_improvement": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
         "bm_drive_row_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -179,6 +186,50 @@ class Instance:
         inst, keep = self._as_c()
         _check(lib().bm_gen_triples_2hop(seed, C.byref(inst), count, out), "bm_gen_triples_2hop")
         return out.reshape(count, 3)
+
+
+class _Edges(C.Structure):
+    _fields_ = [("src", C.POINTER(C.c_int32)), ("dst", C.POINTER(C.c_int32)), ("w", C.POINTER(C.c_int32)),
+                ("m", C.c_int64), ("cap", C.c_int64), ("max_id", C.c_int32), ("malformed", C.c_int64)]
+
+
+class _Pairs(C.Structure):
+    _fields_ = [("male", C.POINTER(C.c_int32)), ("female", C.POINTER(C.c_int32)), ("m", C.c_int64),
+                ("max_id", C.c_int32)]
+
+
+CSV_SILENT, CSV_LOG_ERROR, CSV_WARN_LINE = 0, 1, 2
+
+
+def read_edges_csv(path, report=CSV_SILENT):
+    """bm_edges_read_csv: (src, dst, w, max_id, malformed) in file order."""
+    L = lib()
+    e = _Edges()
+    _check(L.bm_edges_read_csv(os.fsencode(path), report, C.byref(e)), "bm_edges_read_csv")
+    try:
+        arr = lambda p: np.ctypeslib.as_array(p, shape=(e.m,)).copy() if e.m else np.zeros(0, np.int32)
+        return arr(e.src), arr(e.dst), arr(e.w), e.max_id, e.malformed
+    finally:
+        L.bm_edges_free(C.byref(e))
+
+
+def read_pairs(path, fmt="optimized"):
+    """Mapping-file pairs as the named reference loader parses them."""
+    L = lib()
+    p = _Pairs()
+    fn = {"optimized": L.bm_pairs_read_optimized, "permset": L.bm_pairs_read_permset,
+          "plain": L.bm_pairs_read_plain}[fmt]
+    _check(fn(os.fsencode(path), C.byref(p)), fn.__name__)
+    try:
+        arr = lambda q: np.ctypeslib.as_array(q, shape=(p.m,)).copy() if p.m else np.zeros(0, np.int32)
+        return arr(p.male), arr(p.female)
+    finally:
+        L.bm_pairs_free(C.byref(p))
+
+
+def write_mapping_mf(path, mapping):
+    m = _i32(mapping)
+    _check(lib().bm_mapping_write_mf(os.fsencode(path), m, len(m) - 1), "bm_mapping_write_mf")
 
 
 def gen_sets(seed, n, k, count):

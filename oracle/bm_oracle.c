@@ -635,7 +635,8 @@ int64_t orc_omp_rowbest_rows(const orc_graphs *g, const int32_t *map, int row_lo
             }
         }
         evals += n - m1;
-        cs += (int64_t)maxdelta * 1000003 + maxv;
+        cs += maxdelta; /* maxv depends on thread arrival order on ties */
+        (void)maxv;
     }
     if (checksum) *checksum = cs;
     return evals;
