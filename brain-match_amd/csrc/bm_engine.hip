@@ -317,6 +317,8 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->cr, (size_t)c->ncomb * 4);
     dev_free(c, (void **)&c->cw, (size_t)c->ncomb * 4);
     dev_free(c, (void **)&c->cdw, (size_t)c->ncomb * 4);
+    dev_free(c, (void **)&c->crw, (size_t)c->ncomb * 8);
+    dev_free(c, (void **)&c->rorder, N1 * 4);
     dev_free(c, (void **)&c->ddptr, (N1 + 1) * 4);
     dev_free(c, (void **)&c->ddmid, N1 * 4);
     dev_free(c, (void **)&c->ddr, (size_t)c->ndd * 4);
@@ -329,6 +331,7 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->X, N1 * (size_t)c->ldx * 4);
     dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
     c->graphs = c->mapped = c->diag_valid = false;
+    c->gain_cols = 0;
 }
 
 int bm_destroy(bm_ctx *c) {
@@ -502,6 +505,18 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = upload(c, &c->cr, cr))) goto fail;
     if ((rc = upload(c, &c->cw, cw))) goto fail;
     if ((rc = upload(c, &c->cdw, cdw))) goto fail;
+    {
+        std::vector<int32_t> rw(2 * (size_t)nc), order(N1);
+        for (int64_t i = 0; i < nc; i++) {
+            rw[2 * i] = cr[i];
+            rw[2 * i + 1] = cw[i];
+        }
+        for (int v = 0; v < N1; v++) order[v] = v;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return cptr[x + 1] - cptr[x] > cptr[y + 1] - cptr[y]; });
+        if ((rc = upload(c, &c->crw, rw))) goto fail;
+        if ((rc = upload(c, &c->rorder, order))) goto fail;
+    }
     if ((rc = upload(c, &c->ddptr, ddptr))) goto fail;
     if ((rc = upload(c, &c->ddmid, ddmid))) goto fail;
     if ((rc = upload(c, &c->ddr, ddr))) goto fail;

@@ -61,11 +61,15 @@ struct bm_ctx {
     int16_t *Q = nullptr;
     int32_t *map = nullptr;
     int32_t *cptr = nullptr, *cr = nullptr, *cw = nullptr, *cdw = nullptr;
+    int32_t *crw = nullptr;     // (r, w) pairs of the comb lists, 8 bytes per entry
+    int32_t *rorder = nullptr;  // rows in decreasing comb-list length
     int32_t *ddptr = nullptr, *ddmid = nullptr, *ddr = nullptr, *ddw = nullptr;
     int32_t *esrc = nullptr, *edst = nullptr, *ew = nullptr, *edw = nullptr;
     int32_t *diag = nullptr;
     int32_t *X = nullptr;  // all-pairs gain matrix (lazy)
     int64_t ldx = 0;
+    int gain_cols = 0;        // all-pairs slab width per lane (1, 2, 4), 0 = not tuned yet
+    float gain_ms[3] = {0, 0, 0};  // autotuner timings of widths 1, 2, 4
     unsigned long long *keys = nullptr;  // N1 + 8 reduction slots
     void *stage = nullptr;               // move-list staging (lazy, grows)
     size_t stage_bytes = 0;

@@ -11,6 +11,7 @@
 //                  Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b) + Corr(a,b)
 //                  on 64x64 tiles with row-best and global arg-max reductions.
 #include <limits.h>
+#include <stdlib.h>
 
 #include <algorithm>
 
@@ -211,51 +212,219 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
 
 // ------------------------------------------------------------ all pairs
 
-// X[a][col..col+7] = sum_{e in comb(a)} min(w_e, Q[r_e][col..col+7]).
-// grid (ceil(ncols/2048), N1), 8 columns per thread, 16-byte loads.
-__global__ __launch_bounds__(256) void k_gain(const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
-                                              const int32_t *__restrict__ cw, const int16_t *__restrict__ Q, int ld,
-                                              int32_t *__restrict__ X, int64_t ldx, int ncols) {
-    const int a = blockIdx.y;
-    const int col = (blockIdx.x * 256 + threadIdx.x) * 8;
-    if (col >= ncols) return;
-    int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int beg = cptr[a], end = cptr[a + 1];
-    const int16_t *base = Q + col;
-    int e = beg;
-    for (; e + 4 <= end; e += 4) {
-        int r[4], w[4];
-        int4 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            r[u] = cr[e + u];
-            w[u] = cw[e + u];
+// X[a][c..c+W-1] = sum_{e in comb(a)} min(w_e, Q[r_e][c..c+W-1]) for every row a.
+//
+// Work items are (W-column slab of Q, block of 16 rows), W = 64 * kCols.
+// Each wave owns one row at a time, so its list entries are wave-uniform and
+// come in as scalar loads (8 entries per s_load); each lane covers kCols
+// columns, so one entry costs one coalesced 128*kCols-byte vector load per
+// wave.  Workgroups are handed to the 8 XCDs round-robin, so items are
+// numbered per XCD: XCD x walks its own contiguous range of slabs, all row
+// blocks of one slab before the next, and keeps that slab (2(n+1) rows x
+// 2W bytes) in its L2 / the MALL while every row reads it.  The slab width
+// trades L2 residency (narrow) against load instructions (wide); the
+// autotuner in bm_sweep_all_pairs picks kCols per graph.  With non-negative
+// weights the mins of column pairs are summed as packed uint16
+// (v_pk_min_i16 / v_pk_add_u16) and widened to int32 every `flush` entries,
+// chosen so that flush * max(w) < 65536.
+typedef short bm_s2 __attribute__((ext_vector_type(2)));
+typedef unsigned short bm_u2 __attribute__((ext_vector_type(2)));
+typedef int bm_i4 __attribute__((ext_vector_type(4)));
+typedef int bm_i2 __attribute__((ext_vector_type(2)));
+
+constexpr int kRowsWave = 4;   // rows per wave per work item
+constexpr int kRowsWG = 16;    // rows per work item
+
+template <int kCols>
+struct GainLoad;
+template <>
+struct GainLoad<1> {
+    typedef int16_t T;
+};
+template <>
+struct GainLoad<2> {
+    typedef int T;
+};
+template <>
+struct GainLoad<4> {
+    typedef int2 T;
+};
+
+__device__ __forceinline__ bm_u2 pk_min(int x, bm_s2 wp) {
+    return __builtin_bit_cast(bm_u2, __builtin_elementwise_min(__builtin_bit_cast(bm_s2, x), wp));
+}
+
+constexpr int kGainUnroll = 16;  // list entries (vector loads) in flight per wave
+
+template <int kCols, bool kPacked>
+__global__ __launch_bounds__(256) void k_gain(const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                              const int16_t *__restrict__ Q, int ld, int32_t *__restrict__ X,
+                                              int64_t ldx, int N1, int nslab, int nrb, int per_xcd, int flush) {
+    typedef typename GainLoad<kCols>::T LoadT;
+    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    if (k >= per_xcd) return;
+    const int t = xcd * per_xcd + k;
+    if (t >= nslab * nrb) return;
+    const int slab = t / nrb, rb = t - slab * nrb;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = slab * 64 * kCols + (threadIdx.x & 63) * kCols;
+    const bool live = col < ld;
+    const unsigned boff = live ? (unsigned)col * 2u : 0u;  // byte offset of this lane in a Q row
+    const char *qbase = reinterpret_cast<const char *>(Q);
+    const size_t row_bytes = (size_t)ld * 2;
+    // row r's columns for this lane: scalar row base + 32-bit lane offset
+    auto load = [&](int r) { return *reinterpret_cast<const LoadT *>(qbase + (size_t)r * row_bytes + boff); };
+    int s[4];
+    bm_u2 acc[2];
+    auto consume = [&](LoadT v, int w) {
+        if constexpr (kCols == 1) {
+            s[0] += min(w, (int)v);
+        } else if constexpr (kCols == 2) {
+            if (kPacked) {
+                const bm_s2 wp = {(short)w, (short)w};
+                acc[0] += pk_min(v, wp);
+            } else {
+                s[0] += min(w, (int)(int16_t)(v & 0xffff));
+                s[1] += min(w, v >> 16);
+            }
+        } else {
+            if (kPacked) {
+                const bm_s2 wp = {(short)w, (short)w};
+                acc[0] += pk_min(v.x, wp);
+                acc[1] += pk_min(v.y, wp);
+            } else {
+                s[0] += min(w, (int)(int16_t)(v.x & 0xffff));
+                s[1] += min(w, v.x >> 16);
+                s[2] += min(w, (int)(int16_t)(v.y & 0xffff));
+                s[3] += min(w, v.y >> 16);
+            }
         }
+    };
+    auto widen = [&]() {
+        if (kPacked && kCols >= 2) {
+            s[0] += acc[0].x;
+            s[1] += acc[0].y;
+            s[2] += acc[1].x;
+            s[3] += acc[1].y;
+            acc[0] = bm_u2{0, 0};
+            acc[1] = bm_u2{0, 0};
+        }
+    };
+    for (int i = 0; i < kRowsWave; i++) {
+        const int a = rb * kRowsWG + wave * kRowsWave + i;
+        if (a >= N1) break;
+        const int beg = cptr[a], end = cptr[a + 1];
+        s[0] = s[1] = s[2] = s[3] = 0;
+        acc[0] = bm_u2{0, 0};
+        acc[1] = bm_u2{0, 0};
+        // flush is a multiple of kGainUnroll, so widening falls on block edges
+        int since = 0;
+        int e = beg;
+        if (e + kGainUnroll <= end) {
+            int2 rw[kGainUnroll];
 #pragma unroll
-        for (int u = 0; u < 4; u++) v[u] = *reinterpret_cast<const int4 *>(base + (size_t)r[u] * ld);
+            for (int u = 0; u < kGainUnroll; u++) rw[u] = crw[e + u];
+            for (;;) {
+                LoadT v[kGainUnroll];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int x[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+                for (int u = 0; u < kGainUnroll; u++) v[u] = load(rw[u].x);
+                int w[kGainUnroll];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                acc[2 * j] += min(w[u], (int)(int16_t)(x[j] & 0xffff));
-                acc[2 * j + 1] += min(w[u], x[j] >> 16);
+                for (int u = 0; u < kGainUnroll; u++) w[u] = rw[u].y;
+                e += kGainUnroll;
+                const bool more = e + kGainUnroll <= end;
+                if (more) {  // next block's entries while this block's loads are in flight
+#pragma unroll
+                    for (int u = 0; u < kGainUnroll; u++) rw[u] = crw[e + u];
+                }
+#pragma unroll
+                for (int u = 0; u < kGainUnroll; u++) consume(v[u], w[u]);
+                since += kGainUnroll;
+                if (kPacked && since >= flush) {
+                    widen();
+                    since = 0;
+                }
+                if (!more) break;
+            }
+        }
+        for (; e < end; e++) {
+            const int2 rw = crw[e];
+            consume(load(rw.x), rw.y);
+        }
+        widen();
+        if (live) {
+            int32_t *dst = X + (size_t)a * ldx + col;
+            if constexpr (kCols == 4) {
+                const bm_i4 out = {s[0], s[1], s[2], s[3]};
+                __builtin_nontemporal_store(out, reinterpret_cast<bm_i4 *>(dst));
+            } else if constexpr (kCols == 2) {
+                const bm_i2 out = {s[0], s[1]};
+                __builtin_nontemporal_store(out, reinterpret_cast<bm_i2 *>(dst));
+            } else {
+                __builtin_nontemporal_store(s[0], dst);
             }
         }
     }
-    for (; e < end; e++) {
-        const int r = cr[e], w = cw[e];
-        const int4 v = *reinterpret_cast<const int4 *>(base + (size_t)r * ld);
-        const int x[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            acc[2 * j] += min(w, (int)(int16_t)(x[j] & 0xffff));
-            acc[2 * j + 1] += min(w, x[j] >> 16);
+}
+
+template <int kCols>
+static int launch_gain(bm_ctx *c, int N1) {
+    const int nslab = (c->ld + 64 * kCols - 1) / (64 * kCols), nrb = (N1 + kRowsWG - 1) / kRowsWG;
+    const int per_xcd = (int)(((int64_t)nslab * nrb + 7) / 8);
+    // packed uint16 partial sums need 0 <= min(w, P) and flush * max(w) < 65536
+    // (the tail after the last full block adds < kGainUnroll more entries)
+    const int flush = c->max_w1 > 0 ? ((65535 / c->max_w1) - kGainUnroll) & ~(kGainUnroll - 1) : 1 << 20;
+    if (kCols > 1 && c->nonneg && flush >= kGainUnroll) {
+        BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain<kCols, true>), dim3(8 * per_xcd), dim3(256), 0, c->stream,
+                                                c->cptr, (const int2 *)c->crw, c->Q, c->ld, c->X, c->ldx, N1, nslab,
+                                                nrb, per_xcd, flush));
+    } else {
+        BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain<kCols, false>), dim3(8 * per_xcd), dim3(256), 0, c->stream,
+                                                c->cptr, (const int2 *)c->crw, c->Q, c->ld, c->X, c->ldx, N1, nslab,
+                                                nrb, per_xcd, flush));
+    }
+    return BM_OK;
+}
+
+static int launch_gain_cols(bm_ctx *c, int cols) {
+    return cols == 1 ? launch_gain<1>(c, c->N1) : cols == 2 ? launch_gain<2>(c, c->N1) : launch_gain<4>(c, c->N1);
+}
+
+// Slab-width autotuner: on the first sweep of a graph pair, time one launch
+// of each width with HIP events on the context stream and keep the fastest
+// (all widths produce identical X).  BM_GAIN_COLS = 1, 2 or 4 forces one.
+static int tune_gain(bm_ctx *c) {
+    if (c->gain_cols) return BM_OK;
+    const char *force = getenv("BM_GAIN_COLS");
+    if (force && (atoi(force) == 1 || atoi(force) == 2 || atoi(force) == 4)) {
+        c->gain_cols = atoi(force);
+        return BM_OK;
+    }
+    hipEvent_t ev[4];
+    for (auto &e : ev) BM_HIP(hipEventCreate(&e));
+    const int widths[3] = {1, 2, 4};
+    float best = 1e30f;
+    int best_w = 4;
+    const bool was = c->timing;
+    c->timing = false;
+    for (int i = 0; i < 3; i++) {
+        BM_HIP(hipEventRecord(ev[0], c->stream));
+        int rc = launch_gain_cols(c, widths[i]);
+        if (rc) return rc;
+        BM_HIP(hipEventRecord(ev[1], c->stream));
+        BM_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        BM_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        c->gain_ms[i] = ms;
+        if (ms < best) {
+            best = ms;
+            best_w = widths[i];
         }
     }
-    int4 *dst = reinterpret_cast<int4 *>(X + (size_t)a * ldx + col);
-    dst[0] = make_int4(acc[0], acc[1], acc[2], acc[3]);
-    dst[1] = make_int4(acc[4], acc[5], acc[6], acc[7]);
+    c->timing = was;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    c->gain_cols = best_w;
+    return BM_OK;
 }
 
 // Corr(a,b) added into X[min(a,b)][max(a,b)], one thread per G1 line.
@@ -421,9 +590,8 @@ extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, i
         BM_HIP(hipMemsetAsync(dmat, 0, (size_t)N1 * N1 * 4, c->stream));
     }
     BM_HIP(hipMemsetAsync(c->keys, 0, ((size_t)N1 + 8) * 8, c->stream));
-    const int ncols = c->ld;
-    BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain, dim3((ncols + 2047) / 2048, N1), dim3(256), 0, c->stream, c->cptr,
-                                            c->cr, c->cw, c->Q, c->ld, c->X, c->ldx, ncols));
+    if ((rc = tune_gain(c))) return rc;
+    if ((rc = launch_gain_cols(c, c->gain_cols))) return rc;
     if (c->e1 > 0)
         BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->e1 + 255) / 256)), dim3(256), 0, c->stream,
                                                 c->esrc, c->edst, c->ew, c->edw, c->e1, c->Q, c->ld, c->X, c->ldx));

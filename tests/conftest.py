@@ -13,7 +13,7 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu")
 
 
-def random_multigraph(seed, n, m, dup_frac=0.0, selfloop_frac=0.0, wmax=30, zero_vertex=False):
+def random_multigraph(seed, n, m, dup_frac=0.0, selfloop_frac=0.0, wmax=30, zero_vertex=False, negative=False):
     """Edge list in 'CSV order' with optional repeated pairs and self-loops.
 
     Repeated pairs get fresh weights, so the dense cell (last line wins)
@@ -27,13 +27,13 @@ def random_multigraph(seed, n, m, dup_frac=0.0, selfloop_frac=0.0, wmax=30, zero
     dst = np.where(loops, src, dst)
     same = (src == dst) & ~loops
     dst = np.where(same, (dst % n) + 1, dst)
-    w = rng.integers(1, wmax + 1, m)
+    w = rng.integers(-wmax if negative else 1, wmax + 1, m)
     ndup = int(dup_frac * m)
     if ndup:
         idx = rng.integers(0, m, ndup)
         src = np.concatenate([src, src[idx]])
         dst = np.concatenate([dst, dst[idx]])
-        w = np.concatenate([w, rng.integers(1, wmax + 1, ndup)])
+        w = np.concatenate([w, rng.integers(-wmax if negative else 1, wmax + 1, ndup)])
         perm = rng.permutation(len(src))
         src, dst, w = src[perm], dst[perm], w[perm]
     return (src.astype(np.int32), dst.astype(np.int32), w.astype(np.int32))

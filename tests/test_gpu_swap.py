@@ -21,13 +21,16 @@ CASES = [
     ("nonbijective", 30, 250, 0.1, 0.05, 0.1, False, False),
     ("vertex0", 35, 300, 0.1, 0.05, 0.0, True, True),
     ("dense", 24, 400, 0.3, 0.1, 0.0, True, False),
+    ("negative", 30, 300, 0.1, 0.05, 0.1, True, False),  # weights < 0: unpacked int32 path
+    ("wide", 40, 300, 0.1, 0.05, 0.0, True, False),      # weights up to 30000: unpacked int32 path
 ]
 
 
 def make_case(case, seed=7):
     name, n, m, dup, loops, unm, bij, zero = case
-    g1 = random_multigraph(seed, n, m, dup, loops, 30, zero)
-    g2 = random_multigraph(seed + 1, n, m, dup, loops, 30, zero)
+    wmax = 30000 if name == "wide" else 30
+    g1 = random_multigraph(seed, n, m, dup, loops, wmax, zero, negative=name == "negative")
+    g2 = random_multigraph(seed + 1, n, m, dup, loops, wmax, zero, negative=name == "negative")
     mp = random_mapping(seed + 2, n, unm, bij)
     if zero:
         mp[0] = 3  # the reference lets a mapping line name vertex 0
@@ -156,3 +159,32 @@ def test_global_best_steps_match_oracle():
         acc, st = e.global_best(max_steps=15)
         np.testing.assert_array_equal(acc[: len(want)], np.array(want, np.int32).reshape(-1, 3))
         np.testing.assert_array_equal(e.mapping(), cur)
+
+
+@pytest.mark.parametrize("cols", [1, 2, 4])
+@pytest.mark.parametrize("case", [CASES[1], CASES[6], CASES[7]], ids=["dups_loops", "negative", "wide"])
+def test_all_pairs_every_slab_width(case, cols, monkeypatch):
+    # the autotuner may pick any width; each one must give the oracle's matrix
+    monkeypatch.setenv("BM_GAIN_COLS", str(cols))
+    n, g1, g2, mp = make_case(case, seed=17)
+    ref = ob.Oracle(n, g1, g2).all_pairs(mp, matrix=True)
+    with engine_for(n, g1, g2, mp) as e:
+        got = e.sweep_all_pairs(rows=True, matrix=True)
+    np.testing.assert_array_equal(got["matrix"], ref["matrix"])
+    assert got["best"] == ref["best"]
+
+
+@pytest.mark.parametrize("cols", [2, 4])
+def test_all_pairs_long_lists_packed_flush(cols, monkeypatch):
+    # weights up to 4000 make the packed uint16 sums widen every 16 entries;
+    # lists of ~200 entries cross that interval many times
+    monkeypatch.setenv("BM_GAIN_COLS", str(cols))
+    n = 60
+    g1 = random_multigraph(77, n, 6000, 0.05, 0.02, 4000)
+    g2 = random_multigraph(78, n, 6000, 0.05, 0.02, 4000)
+    mp = random_mapping(79, n)
+    ref = ob.Oracle(n, g1, g2).all_pairs(mp, matrix=True)
+    with engine_for(n, g1, g2, mp) as e:
+        got = e.sweep_all_pairs(rows=True, matrix=True)
+    np.testing.assert_array_equal(got["matrix"], ref["matrix"])
+    assert got["best"] == ref["best"]
