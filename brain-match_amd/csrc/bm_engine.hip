@@ -216,11 +216,15 @@ __global__ void k_perm_cols(int16_t *__restrict__ Q, int N1, int ld, SetPerm sp)
     for (int i = 0; i < sp.k; i++) row[sp.set[i]] = old[i];
 }
 
-__global__ void k_perm_map(int32_t *__restrict__ map, SetPerm sp) {
+__global__ void k_perm_map(int32_t *__restrict__ map, int32_t *__restrict__ pinv, int with_pinv, SetPerm sp) {
     if (threadIdx.x != 0) return;
     int old[8];
     for (int i = 0; i < sp.k; i++) old[i] = map[sp.set[sp.src[i]]];
     for (int i = 0; i < sp.k; i++) map[sp.set[i]] = old[i];
+    // the images are permuted among the set, so p stays injective
+    if (with_pinv)
+        for (int i = 0; i < sp.k; i++)
+            if (old[i] != 0) pinv[old[i]] = sp.set[i];
 }
 
 // ---------------------------------------------------------------- helpers
@@ -248,11 +252,15 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
               hipLaunchKernelGGL(k_perm_rows, dim3((c->ld + 255) / 256), dim3(256), 0, c->stream, c->Q, c->N1, c->ld, sp));
     BM_LAUNCH(c, "perm_cols", hipLaunchKernelGGL(k_perm_cols, dim3((2 * c->N1 + 255) / 256), dim3(256), 0, c->stream,
                                                  c->Q, c->N1, c->ld, sp));
-    BM_LAUNCH(c, "perm_map", hipLaunchKernelGGL(k_perm_map, dim3(1), dim3(64), 0, c->stream, c->map, sp));
+    BM_LAUNCH(c, "perm_map", hipLaunchKernelGGL(k_perm_map, dim3(1), dim3(64), 0, c->stream, c->map, c->pinv, c->sparse_ok ? 1 : 0, sp));
     int32_t old[8];
     for (int i = 0; i < k; i++) old[i] = c->hmap[set[src_of[i]]];
     for (int i = 0; i < k; i++) c->hmap[set[i]] = old[i];
+    if (c->sparse_ok)
+        for (int i = 0; i < k; i++)
+            if (old[i] != 0) c->hpinv[old[i]] = set[i];
     c->diag_valid = false;
+    c->g2y_dirty = true;
     return BM_OK;
 }
 
@@ -328,10 +336,19 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->ew, (size_t)c->e1 * 4);
     dev_free(c, (void **)&c->edw, (size_t)c->e1 * 4);
     dev_free(c, (void **)&c->diag, N1 * 4);
+    dev_free(c, (void **)&c->g2optr, (N1 + 1) * 4);
+    dev_free(c, (void **)&c->g2iptr, (N1 + 1) * 4);
+    dev_free(c, (void **)&c->g2o, (size_t)c->ng2 * 4);
+    dev_free(c, (void **)&c->g2i, (size_t)c->ng2 * 4);
+    dev_free(c, (void **)&c->pinv, N1 * 4);
+    dev_free(c, (void **)&c->g2oy, (size_t)c->ng2 * 4);
+    dev_free(c, (void **)&c->g2iy, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->X, N1 * (size_t)c->ldx * 4);
     dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
     c->graphs = c->mapped = c->diag_valid = false;
     c->gain_cols = 0;
+    c->sweep_kind = 0;
+    c->sparse_ok = false;
 }
 
 int bm_destroy(bm_ctx *c) {
@@ -493,6 +510,33 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     c->nonneg = nonneg;
     c->max_w1 = maxw;
 
+    // ---- G2 out- and in-lists of the de-duplicated lines, packed (vertex << 16 | weight)
+    std::vector<int32_t> g2optr(N1 + 1, 0), g2iptr(N1 + 1, 0);
+    std::vector<uint32_t> g2o, g2i;
+    {
+        const int64_t m2 = (int64_t)tri.size() / 3;
+        c->g2_vertex0 = false;
+        for (int64_t i = 0; i < m2; i++) {
+            g2optr[tri[3 * i] + 1]++;
+            g2iptr[tri[3 * i + 1] + 1]++;
+            if (tri[3 * i] == 0 || tri[3 * i + 1] == 0) c->g2_vertex0 = true;
+        }
+        for (int v = 0; v < N1; v++) {
+            g2optr[v + 1] += g2optr[v];
+            g2iptr[v + 1] += g2iptr[v];
+        }
+        std::vector<int32_t> po(g2optr.begin(), g2optr.end() - 1), pi(g2iptr.begin(), g2iptr.end() - 1);
+        g2o.resize(m2);
+        g2i.resize(m2);
+        for (int64_t i = 0; i < m2; i++) {
+            const uint32_t u = (uint32_t)tri[3 * i], v = (uint32_t)tri[3 * i + 1];
+            const uint32_t wv = (uint16_t)(int16_t)tri[3 * i + 2];
+            g2o[po[u]++] = (v << 16) | wv;
+            g2i[pi[v]++] = (u << 16) | wv;
+        }
+        c->ng2 = m2;
+    }
+
     // ---- device buffers
     size_t ld = (size_t)c->ld;
     int rc;
@@ -525,6 +569,13 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = upload(c, &c->edst, ed))) goto fail;
     if ((rc = upload(c, &c->ew, ewv))) goto fail;
     if ((rc = upload(c, &c->edw, edwv))) goto fail;
+    if ((rc = upload(c, &c->g2optr, g2optr))) goto fail;
+    if ((rc = upload(c, &c->g2iptr, g2iptr))) goto fail;
+    if ((rc = upload(c, &c->g2o, g2o))) goto fail;
+    if ((rc = upload(c, &c->g2i, g2i))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->pinv, (size_t)N1 * 4))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->g2oy, (size_t)c->ng2 * 4))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->g2iy, (size_t)c->ng2 * 4))) goto fail;
     {
         hipError_t e = hipMemsetAsync(c->W2, 0, (size_t)N1 * ld * 2, c->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "memset W2"); goto fail; }
@@ -560,6 +611,19 @@ int bm_set_mapping(bm_ctx *c, const int32_t *mapping) {
     BM_HIP(hipSetDevice(c->device));
     c->hmap.assign(mapping, mapping + c->N1);
     BM_HIP(hipMemcpyAsync(c->map, c->hmap.data(), (size_t)c->N1 * 4, hipMemcpyHostToDevice, c->stream));
+    // inverse of the mapping on mapped vertices; the sparse all-pairs kernel
+    // needs it to be a function (p injective where p != 0) and W2 row/column 0 empty
+    c->hpinv.assign(c->N1, -1);
+    c->sparse_ok = !c->g2_vertex0;
+    for (int v = 0; v < c->N1 && c->sparse_ok; v++) {
+        const int g = mapping[v];
+        if (g == 0) continue;
+        if (c->hpinv[g] >= 0) c->sparse_ok = false;
+        else c->hpinv[g] = v;
+    }
+    if (c->sparse_ok)
+        BM_HIP(hipMemcpyAsync(c->pinv, c->hpinv.data(), (size_t)c->N1 * 4, hipMemcpyHostToDevice, c->stream));
+    c->g2y_dirty = true;
     int16_t *P = c->Q, *PT = c->Q + (size_t)c->N1 * c->ld;
     BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_build_p, dim3(c->ld / 256 + 1, c->N1), dim3(256), 0, c->stream, P,
                                                c->W2, c->map, c->N1, c->ld));

@@ -65,6 +65,17 @@ struct bm_ctx {
     int32_t *rorder = nullptr;  // rows in decreasing comb-list length
     int32_t *ddptr = nullptr, *ddmid = nullptr, *ddr = nullptr, *ddw = nullptr;
     int32_t *esrc = nullptr, *edst = nullptr, *ew = nullptr, *edw = nullptr;
+    // G2 de-duplicated (last line wins) out- and in-lists, packed (vertex << 16 | (uint16)weight)
+    int32_t *g2optr = nullptr, *g2iptr = nullptr;
+    uint32_t *g2o = nullptr, *g2i = nullptr;
+    int64_t ng2 = 0;
+    bool g2_vertex0 = false;  // a G2 line names vertex 0
+    int32_t *pinv = nullptr;  // pinv[g] = the vertex mapped to g (-1 if none); valid when sparse_ok
+    uint32_t *g2oy = nullptr, *g2iy = nullptr;  // the G2 lists with targets as P coordinates (p^-1), 0xFFFF = none
+    bool g2y_dirty = true;
+    bool sparse_ok = false;   // p injective on mapped vertices and !g2_vertex0
+    int sweep_kind = 0;       // all-pairs gain kernel: 0 = not tuned, 1 = dense, 2 = sparse
+    float sparse_ms = 0.f;
     int32_t *diag = nullptr;
     int32_t *X = nullptr;  // all-pairs gain matrix (lazy)
     int64_t ldx = 0;
@@ -77,6 +88,7 @@ struct bm_ctx {
 
     // host mirror of the mapping
     std::vector<int32_t> hmap;
+    std::vector<int32_t> hpinv;
 
     // kernel timing
     bool timing = false;

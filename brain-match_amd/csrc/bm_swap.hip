@@ -12,6 +12,7 @@
 //                  on 64x64 tiles with row-best and global arg-max reductions.
 #include <limits.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 
@@ -398,6 +399,7 @@ static int tune_gain(bm_ctx *c) {
     const char *force = getenv("BM_GAIN_COLS");
     if (force && (atoi(force) == 1 || atoi(force) == 2 || atoi(force) == 4)) {
         c->gain_cols = atoi(force);
+        if (!getenv("BM_SWEEP_KERNEL")) c->sweep_kind = 1;  // a forced width means the dense kernel
         return BM_OK;
     }
     hipEvent_t ev[4];
@@ -424,6 +426,170 @@ static int tune_gain(bm_ctx *c) {
     c->timing = was;
     for (auto &e : ev) (void)hipEventDestroy(e);
     c->gain_cols = best_w;
+    return BM_OK;
+}
+
+// Sparse form of the same row: P = W2 permuted is as sparse as G2, so
+//   X(a,b) = C_a + sum over comb entries e of a and non-zero cells Q[r_e][b]
+//            of (min(w_e, Q[r_e][b]) - min(w_e, 0)),   C_a = sum_e min(w_e, 0).
+// The non-zero cells of row s of P are the G2 out-list of p(s) and those of
+// row d of P^T the G2 in-list of p(d), with targets taken back through p^-1
+// (the lists are kept translated, k_g2_to_p).  One 1024-thread workgroup per
+// row a accumulates into an (n+1)-int LDS row.  The row's comb entries are
+// taken in chunks; per chunk the G2 lists they select are laid end to end
+// (prefix sums in LDS) and the 16 waves split that flat index space evenly,
+// lanes of a wave reading consecutive list entries (coalesced 4-byte
+// (target << 16 | weight) words) and ds_add-ing the contribution.  Work is the
+// sum over G1 lines of the G2 degree instead of 2E * n.  Exact when p is
+// injective on mapped vertices and no G2 line names vertex 0 (W2 row/column 0
+// empty); the caller checks that.
+constexpr int kSparseChunk = 512;
+
+__global__ void k_g2_to_p(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
+                          const int32_t *__restrict__ pinv, int64_t m) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t gv = src[i];
+    const int y = pinv[gv >> 16];
+    dst[i] = ((uint32_t)(y < 0 ? 0xFFFF : y) << 16) | (gv & 0xFFFFu);
+}
+
+__global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                      const int32_t *__restrict__ map,
+                                                      const int32_t *__restrict__ g2optr,
+                                                      const uint32_t *__restrict__ g2oy,
+                                                      const int32_t *__restrict__ g2iptr,
+                                                      const uint32_t *__restrict__ g2iy, int32_t *__restrict__ X,
+                                                      int64_t ldx, int N1) {
+    extern __shared__ int acc[];
+    __shared__ int pref[kSparseChunk + 1];
+    __shared__ int lst[kSparseChunk];
+    __shared__ int wts[kSparseChunk];
+    __shared__ int wsum[16];
+    __shared__ int cneg;
+    const int a = blockIdx.x, tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    for (int i = tid; i < N1; i += 1024) acc[i] = 0;
+    if (tid == 0) cneg = 0;
+    __syncthreads();
+    const int beg = cptr[a], end = cptr[a + 1];
+    for (int c0 = beg; c0 < end; c0 += kSparseChunk) {
+        const int m = min(kSparseChunk, end - c0);
+        // per entry: which G2 list (start, length) and its weight
+        int len = 0;
+        if (tid < m) {
+            const int2 rw = crw[c0 + tid];
+            const bool out = rw.x >= N1;  // row of P^T (out entry a->d) or of P (in entry s->a)
+            const int g = map[out ? rw.x - N1 : rw.x];
+            const int lo = out ? g2iptr[g] : g2optr[g];
+            len = (out ? g2iptr[g + 1] : g2optr[g + 1]) - lo;
+            lst[tid] = out ? (lo | (int)0x80000000) : lo;
+            wts[tid] = rw.y;
+            if (rw.y < 0) atomicAdd(&cneg, rw.y);
+        }
+        // exclusive prefix sum of the list lengths (first 512 threads)
+        int x = len;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (tid < 16) {
+            int v = wsum[tid];
+            for (int o = 1; o < 16; o <<= 1) {
+                const int y = __shfl_up(v, o, 16);
+                if (tid >= o) v += y;
+            }
+            wsum[tid] = v;
+        }
+        __syncthreads();
+        if (tid < m) pref[tid + 1] = x + (wave ? wsum[wave - 1] : 0);
+        if (tid == 0) pref[0] = 0;
+        __syncthreads();
+        const int total = pref[m];
+        // the 16 waves split [0, total) into contiguous spans
+        const int span = (total + 15) / 16;
+        const int w0 = wave * span, w1 = min(total, w0 + span);
+        if (w0 < w1) {
+            int lo_i = 0, hi_i = m - 1;  // entry holding flat index w0
+            while (lo_i < hi_i) {
+                const int mid = (lo_i + hi_i + 1) >> 1;
+                if (pref[mid] <= w0) lo_i = mid;
+                else hi_i = mid - 1;
+            }
+            int i = lo_i;
+            for (int idx = w0 + lane; idx < w1; idx += 64) {
+                while (pref[i + 1] <= idx) i++;
+                const int L = lst[i];
+                const uint32_t *list = L < 0 ? g2iy : g2oy;
+                const uint32_t yv = list[(L & 0x7fffffff) + idx - pref[i]];
+                const int y = yv >> 16;
+                if (y != 0xFFFF) {
+                    const int w = wts[i];
+                    atomicAdd(&acc[y], min(w, (int)(int16_t)(yv & 0xffff)) - min(w, 0));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int ca = cneg;
+    int32_t *row = X + (size_t)a * ldx;
+    for (int i = tid; i < ldx; i += 1024) __builtin_nontemporal_store(i < N1 ? acc[i] + ca : 0, row + i);
+}
+
+static int launch_gain_sparse(bm_ctx *c) {
+    if (c->g2y_dirty && c->ng2 > 0) {
+        const unsigned blocks = (unsigned)((c->ng2 + 255) / 256);
+        BM_LAUNCH(c, "g2_to_p", hipLaunchKernelGGL(k_g2_to_p, dim3(blocks), dim3(256), 0, c->stream, c->g2oy, c->g2o,
+                                                   c->pinv, c->ng2));
+        BM_LAUNCH(c, "g2_to_p", hipLaunchKernelGGL(k_g2_to_p, dim3(blocks), dim3(256), 0, c->stream, c->g2iy, c->g2i,
+                                                   c->pinv, c->ng2));
+    }
+    c->g2y_dirty = false;
+    const size_t lds = (size_t)c->N1 * 4;
+    BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse, dim3(c->N1), dim3(1024), lds, c->stream, c->cptr,
+                                            (const int2 *)c->crw, c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy,
+                                            c->X, c->ldx, c->N1));
+    return BM_OK;
+}
+
+// The largest LDS row the sparse kernel may use (bytes).
+constexpr size_t kSparseLdsMax = 144 * 1024;
+
+// Chooses the all-pairs gain kernel on the first sweep of a graph pair: the
+// sparse kernel when its preconditions hold, timed against the best dense
+// slab width, the faster one kept (both produce identical X).
+// BM_SWEEP_KERNEL = dense | sparse forces one.
+static int choose_gain(bm_ctx *c) {
+    int rc = tune_gain(c);
+    if (rc) return rc;
+    const bool can_sparse = c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
+    if (!can_sparse) return BM_OK;
+    if (c->sweep_kind) return BM_OK;
+    const char *force = getenv("BM_SWEEP_KERNEL");
+    if (force && !strcmp(force, "dense")) {
+        c->sweep_kind = 1;
+        return BM_OK;
+    }
+    if (force && !strcmp(force, "sparse")) {
+        c->sweep_kind = 2;
+        return BM_OK;
+    }
+    hipEvent_t ev[2];
+    for (auto &e : ev) BM_HIP(hipEventCreate(&e));
+    const bool was = c->timing;
+    c->timing = false;
+    BM_HIP(hipEventRecord(ev[0], c->stream));
+    if ((rc = launch_gain_sparse(c))) return rc;
+    BM_HIP(hipEventRecord(ev[1], c->stream));
+    BM_HIP(hipEventSynchronize(ev[1]));
+    c->timing = was;
+    BM_HIP(hipEventElapsedTime(&c->sparse_ms, ev[0], ev[1]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    const float dense_ms = c->gain_ms[c->gain_cols == 1 ? 0 : c->gain_cols == 2 ? 1 : 2];
+    c->sweep_kind = (dense_ms > 0.f && dense_ms < c->sparse_ms) ? 1 : 2;
     return BM_OK;
 }
 
@@ -590,8 +756,10 @@ extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, i
         BM_HIP(hipMemsetAsync(dmat, 0, (size_t)N1 * N1 * 4, c->stream));
     }
     BM_HIP(hipMemsetAsync(c->keys, 0, ((size_t)N1 + 8) * 8, c->stream));
-    if ((rc = tune_gain(c))) return rc;
-    if ((rc = launch_gain_cols(c, c->gain_cols))) return rc;
+    if ((rc = choose_gain(c))) return rc;
+    if (c->sweep_kind == 2 && c->sparse_ok && (size_t)N1 * 4 <= kSparseLdsMax) rc = launch_gain_sparse(c);
+    else rc = launch_gain_cols(c, c->gain_cols);
+    if (rc) return rc;
     if (c->e1 > 0)
         BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->e1 + 255) / 256)), dim3(256), 0, c->stream,
                                                 c->esrc, c->edst, c->ew, c->edw, c->e1, c->Q, c->ld, c->X, c->ldx));

@@ -188,3 +188,41 @@ def test_all_pairs_long_lists_packed_flush(cols, monkeypatch):
         got = e.sweep_all_pairs(rows=True, matrix=True)
     np.testing.assert_array_equal(got["matrix"], ref["matrix"])
     assert got["best"] == ref["best"]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_all_pairs_sparse_kernel(case, monkeypatch):
+    # the sparse gain kernel (taken when p is injective on mapped vertices and
+    # G2 never names vertex 0; the other cases fall back to the dense kernel)
+    monkeypatch.setenv("BM_SWEEP_KERNEL", "sparse")
+    n, g1, g2, mp = make_case(case, seed=23)
+    ref = ob.Oracle(n, g1, g2).all_pairs(mp, matrix=True)
+    with engine_for(n, g1, g2, mp) as e:
+        got = e.sweep_all_pairs(rows=True, matrix=True)
+    np.testing.assert_array_equal(got["matrix"], ref["matrix"])
+    assert got["best"] == ref["best"]
+    np.testing.assert_array_equal(got["row_best_b"][1:n], ref["row_best_b"][1:n])
+
+
+@pytest.mark.parametrize("kernel", ["sparse", "dense"])
+def test_global_best_after_moves_each_kernel(kernel, monkeypatch):
+    # p^-1 must follow every applied swap, rotation and set permutation
+    monkeypatch.setenv("BM_SWEEP_KERNEL", kernel)
+    inst = bmatch.Instance.powerlaw(31, 90, 1100)
+    o = ob.Oracle(inst.n, inst.g1, inst.g2)
+    cur = inst.p0.copy()
+    with engine_for(inst.n, inst.g1, inst.g2, inst.p0) as e:
+        e.apply_rot3(4, 9, 17, 3)
+        cur = ob.apply_g3(cur, 4, 9, 17, 3)
+        e.apply_perm([5, 6, 7, 8], [2, 0, 3, 1])
+        cur = ob.apply_perm(cur, [5, 6, 7, 8], [2, 0, 3, 1])
+        for _ in range(10):
+            got = e.sweep_all_pairs(rows=True)
+            ref = o.all_pairs(cur)
+            assert got["best"] == ref["best"]
+            np.testing.assert_array_equal(got["row_best_delta"][1:inst.n], ref["row_best_delta"][1:inst.n])
+            a, b, d = ref["best"]
+            if d <= 0:
+                break
+            e.apply_swap(a, b)
+            cur[a], cur[b] = cur[b], cur[a]
