@@ -24,6 +24,14 @@ __device__ __forceinline__ int q16(const int16_t *__restrict__ Q, size_t row, in
     return (int)Q[row * (size_t)ld + col];
 }
 
+// The all-pairs gain matrix X is stored tile-major: 64x64 tiles of int32,
+// tile (I, J) contiguous, T tiles per tile row (T = ldx / 64).  Rows of a
+// tile are 256 contiguous bytes, so row-wise writers stay coalesced and the
+// combine reads a tile and its mirror as two contiguous 16 KB blocks.
+__device__ __forceinline__ size_t xidx(int a, int b, int T) {
+    return ((size_t)(a >> 6) * T + (size_t)(b >> 6)) * 4096 + (size_t)((a & 63) << 6) + (size_t)(b & 63);
+}
+
 // ------------------------------------------------------------ move list
 
 // Delta_B for pairs (A[i], B[i]).  16-lane groups, 4 pairs per wave.
@@ -354,7 +362,7 @@ __global__ __launch_bounds__(256) void k_gain(const int32_t *__restrict__ cptr, 
         }
         widen();
         if (live) {
-            int32_t *dst = X + (size_t)a * ldx + col;
+            int32_t *dst = X + xidx(a, col, (int)(ldx >> 6));
             if constexpr (kCols == 4) {
                 const bm_i4 out = {s[0], s[1], s[2], s[3]};
                 __builtin_nontemporal_store(out, reinterpret_cast<bm_i4 *>(dst));
@@ -455,12 +463,14 @@ __global__ void k_g2_to_p(uint32_t *__restrict__ dst, const uint32_t *__restrict
 }
 
 __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                      const int32_t *__restrict__ cdw,
                                                       const int32_t *__restrict__ map,
                                                       const int32_t *__restrict__ g2optr,
                                                       const uint32_t *__restrict__ g2oy,
                                                       const int32_t *__restrict__ g2iptr,
-                                                      const uint32_t *__restrict__ g2iy, int32_t *__restrict__ X,
-                                                      int64_t ldx, int N1) {
+                                                      const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
+                                                      int ld, int32_t *__restrict__ X, int64_t ldx,
+                                                      int32_t *__restrict__ diag, int N1) {
     extern __shared__ int acc[];
     __shared__ int pref[kSparseChunk + 1];
     __shared__ int lst[kSparseChunk];
@@ -518,24 +528,61 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
                 if (pref[mid] <= w0) lo_i = mid;
                 else hi_i = mid - 1;
             }
-            int i = lo_i;
-            for (int idx = w0 + lane; idx < w1; idx += 64) {
-                while (pref[i + 1] <= idx) i++;
-                const int L = lst[i];
-                const uint32_t *list = L < 0 ? g2iy : g2oy;
-                const uint32_t yv = list[(L & 0x7fffffff) + idx - pref[i]];
+            // four list entries in flight per lane: indices idx, idx+64, idx+128, idx+192
+            int i0 = lo_i;
+            int idx = w0 + lane;
+            for (; idx + 192 < w1; idx += 256) {
+                int ii[4];
+                uint32_t yv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    while (pref[i0 + 1] <= idx + 64 * u) i0++;
+                    ii[u] = i0;
+                    const int L = lst[i0];
+                    yv[u] = (L < 0 ? g2iy : g2oy)[(L & 0x7fffffff) + idx + 64 * u - pref[i0]];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int y = yv[u] >> 16;
+                    if (y != 0xFFFF) {
+                        const int w = wts[ii[u]];
+                        atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - min(w, 0));
+                    }
+                }
+            }
+            for (; idx < w1; idx += 64) {
+                while (pref[i0 + 1] <= idx) i0++;
+                const int L = lst[i0];
+                const uint32_t yv = (L < 0 ? g2iy : g2oy)[(L & 0x7fffffff) + idx - pref[i0]];
                 const int y = yv >> 16;
                 if (y != 0xFFFF) {
-                    const int w = wts[i];
+                    const int w = wts[i0];
                     atomicAdd(&acc[y], min(w, (int)(int16_t)(yv & 0xffff)) - min(w, 0));
                 }
             }
         }
         __syncthreads();
     }
+    // X(a,a) before corrections; then Corr(a,b) for b > a from a's own entries
+    // (each G1 line between a and b is in comb(a) exactly once)
+    if (tid == 0) diag[a] = acc[a] + cneg;
+    __syncthreads();
+    {
+        const int paa = (int)Q[(size_t)a * ld + a];
+        for (int e = beg + tid; e < end; e += 1024) {
+            const int2 rw = crw[e];
+            const bool out = rw.x >= N1;
+            const int b = out ? rw.x - N1 : rw.x;
+            if (b <= a) continue;
+            const int pbb = (int)Q[(size_t)b * ld + b], pab = (int)Q[(size_t)a * ld + b],
+                      pba = (int)Q[(size_t)b * ld + a];
+            atomicAdd(&acc[b], pair_corr(out, rw.y, cdw[e], paa, pbb, pab, pba));
+        }
+    }
+    __syncthreads();
     const int ca = cneg;
-    int32_t *row = X + (size_t)a * ldx;
-    for (int i = tid; i < ldx; i += 1024) __builtin_nontemporal_store(i < N1 ? acc[i] + ca : 0, row + i);
+    const int T = (int)(ldx >> 6);
+    for (int i = tid; i < ldx; i += 1024) __builtin_nontemporal_store(i < N1 ? acc[i] + ca : 0, X + xidx(a, i, T));
 }
 
 static int launch_gain_sparse(bm_ctx *c) {
@@ -550,8 +597,9 @@ static int launch_gain_sparse(bm_ctx *c) {
     const size_t lds = (size_t)c->N1 * 4;
     BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse, dim3(c->N1), dim3(1024), lds, c->stream, c->cptr,
-                                            (const int2 *)c->crw, c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy,
-                                            c->X, c->ldx, c->N1));
+                                            (const int2 *)c->crw, c->cdw, c->map, c->g2optr, c->g2oy, c->g2iptr,
+                                            c->g2iy, c->Q, c->ld, c->X, c->ldx, c->diag, c->N1));
+    c->diag_valid = true;
     return BM_OK;
 }
 
@@ -606,59 +654,82 @@ __global__ __launch_bounds__(256) void k_corr(const int32_t *__restrict__ es, co
     // the line x->y seen from the pair (a=x, b=y) as an out entry of a
     const int v = pair_corr(true, ew[i], edw[i], pxx, pyy, pxy, pyx);
     const int lo = min(x, y), hi = max(x, y);
-    atomicAdd(&X[(size_t)lo * ldx + hi], v);
+    atomicAdd(&X[xidx(lo, hi, (int)(ldx >> 6))], v);
 }
 
-// Delta tiles over (I <= J) 64x64 blocks: row-best and global arg-max.
+// Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b) for 1 <= a < b <= n, with
+// Corr(a,b) already folded into X[a][b].  A block takes one 64-row tile I and
+// a chunk of kCombChunk column tiles J >= I.  Each tile pair (I,J), (J,I) is
+// two contiguous 16 KB blocks of the tile-major X, staged in LDS; lane l of
+// every wave owns row a = 64I + l and wave w the 16 columns 16w..16w+15 of
+// the tile, so each lane keeps its row's best (delta, -b) in a register
+// across the whole chunk.  At the end the 4 waves' bests of a row meet in
+// LDS: one 64-bit atomic max per row per block, and one for the block's
+// global best (lexicographic (a, b) tie-break in the key).
+constexpr int kCombChunk = 8;
+
 __global__ __launch_bounds__(256) void k_combine(const int32_t *__restrict__ X, int64_t ldx,
                                                  const int32_t *__restrict__ diag, int n,
                                                  unsigned long long *__restrict__ rowkey,
                                                  unsigned long long *__restrict__ gkey,
                                                  int32_t *__restrict__ dmat) {
-    const int I = blockIdx.y, J = blockIdx.x;
-    if (I > J) return;
-    __shared__ int tT[64][65];
+    const int I = blockIdx.x;
+    const int T = (int)(ldx >> 6), Tn = (n + 64) / 64;
+    const int J0 = I + blockIdx.y * kCombChunk, J1 = min(Tn, J0 + kCombChunk);
+    if (J0 >= J1) return;
+    __shared__ int XA[64][65], XB[64][65];
     __shared__ int dA[64], dB[64];
-    __shared__ unsigned long long red[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int a0 = I * 64, bcol0 = J * 64;
+    __shared__ unsigned long long best[4][64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int a0 = I * 64, a = a0 + lane;
     const int N1 = n + 1;
-    for (int i = wv; i < 64; i += 4) {
-        const int row = bcol0 + i, colx = a0 + lane;
-        tT[i][lane] = (row < N1 && colx < N1) ? X[(size_t)row * ldx + colx] : 0;
-    }
-    if (threadIdx.x < 64) {
-        dA[lane] = (a0 + lane < N1) ? diag[a0 + lane] : 0;
-    } else if (threadIdx.x < 128) {
-        dB[lane] = (bcol0 + lane < N1) ? diag[bcol0 + lane] : 0;
-    }
-    __syncthreads();
-    const int b = bcol0 + lane;
-    unsigned long long gbest = 0;
-    for (int i = wv; i < 64; i += 4) {
-        const int a = a0 + i;
-        const bool valid = a >= 1 && a < b && b <= n;
-        unsigned long long k = 0;
-        if (valid) {
-            const int d = X[(size_t)a * ldx + b] + tT[lane][i] - dA[i] - dB[lane];
-            k = ((unsigned long long)((unsigned)d ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (unsigned)b);
-            const unsigned long long g =
-                ((unsigned long long)((unsigned)d ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (((unsigned)a << 16) | (unsigned)b));
-            gbest = g > gbest ? g : gbest;
-            if (dmat) {
-                dmat[(size_t)a * N1 + b] = d;
-                dmat[(size_t)b * N1 + a] = d;
+    if (tid < 64) dA[lane] = (a <= n) ? diag[a] : 0;
+    unsigned long long kb = 0;
+    for (int J = J0; J < J1; J++) {
+        const int b0 = J * 64;
+        const int4 *ta = reinterpret_cast<const int4 *>(X + ((size_t)I * T + J) * 4096);
+        const int4 *tb = reinterpret_cast<const int4 *>(X + ((size_t)J * T + I) * 4096);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int q = tid + 256 * k;  // int4 index in the 1024-int4 tile
+            const int4 va = ta[q], vb = tb[q];
+            const int r = q >> 4, c = (q & 15) * 4;
+            XA[r][c] = va.x; XA[r][c + 1] = va.y; XA[r][c + 2] = va.z; XA[r][c + 3] = va.w;
+            XB[r][c] = vb.x; XB[r][c + 1] = vb.y; XB[r][c + 2] = vb.z; XB[r][c + 3] = vb.w;
+        }
+        if (tid < 64) dB[lane] = (b0 + lane <= n) ? diag[b0 + lane] : 0;
+        __syncthreads();
+        if (a >= 1 && a <= n) {
+            const int da = dA[lane];
+#pragma unroll 4
+            for (int j = 0; j < 16; j++) {
+                const int bl = wv * 16 + j, b = b0 + bl;
+                if (a < b && b <= n) {
+                    const int d = XA[lane][bl] + XB[bl][lane] - da - dB[bl];
+                    const unsigned long long k =
+                        ((unsigned long long)((unsigned)d ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (unsigned)b);
+                    kb = k > kb ? k : kb;
+                    if (dmat) {
+                        dmat[(size_t)a * N1 + b] = d;
+                        dmat[(size_t)b * N1 + a] = d;
+                    }
+                }
             }
         }
-        k = wave_max_u64(k);
-        if (lane == 0 && k) atomicMax(&rowkey[a], k);
+        __syncthreads();
     }
-    gbest = wave_max_u64(gbest);
-    if (lane == 0) red[wv] = gbest;
+    best[wv][lane] = kb;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long k = max(max(red[0], red[1]), max(red[2], red[3]));
-        if (k) atomicMax(gkey, k);
+    if (tid < 64) {
+        unsigned long long k = max(max(best[0][lane], best[1][lane]), max(best[2][lane], best[3][lane]));
+        unsigned long long g = 0;
+        if (k && a <= n) {
+            atomicMax(&rowkey[a], k);
+            const unsigned bb = 0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFu);
+            g = (k & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - (((unsigned)a << 16) | bb));
+        }
+        g = wave_max_u64(g);
+        if (lane == 0 && g) atomicMax(gkey, g);
     }
 }
 
@@ -746,27 +817,32 @@ extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, i
     int rc;
     if (!c->X) {
         c->ldx = c->ld;
-        if ((rc = dev_alloc(c, (void **)&c->X, (size_t)N1 * c->ldx * 4))) return rc;
+        // tile-major: whole 64-row tiles, the last one partly unused
+        if ((rc = dev_alloc(c, (void **)&c->X, (size_t)((N1 + 63) / 64 * 64) * c->ldx * 4))) return rc;
     }
-    if ((rc = ensure_diag(c))) return rc;
     int32_t *dmat = nullptr;
     if (delta_matrix) {
         if ((rc = ensure_stage(c, (size_t)N1 * N1 * 4))) return rc;
         dmat = (int32_t *)c->stage;
         BM_HIP(hipMemsetAsync(dmat, 0, (size_t)N1 * N1 * 4, c->stream));
     }
-    BM_HIP(hipMemsetAsync(c->keys, 0, ((size_t)N1 + 8) * 8, c->stream));
+    BM_HIP(hipMemsetAsync(c->keys, 0, ((size_t)N1 + 1) * 8, c->stream));
     if ((rc = choose_gain(c))) return rc;
-    if (c->sweep_kind == 2 && c->sparse_ok && (size_t)N1 * 4 <= kSparseLdsMax) rc = launch_gain_sparse(c);
-    else rc = launch_gain_cols(c, c->gain_cols);
-    if (rc) return rc;
-    if (c->e1 > 0)
-        BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->e1 + 255) / 256)), dim3(256), 0, c->stream,
-                                                c->esrc, c->edst, c->ew, c->edw, c->e1, c->Q, c->ld, c->X, c->ldx));
-    const int T = (N1 + 63) / 64;
+    const bool sparse = c->sweep_kind == 2 && c->sparse_ok && (size_t)N1 * 4 <= kSparseLdsMax;
+    if (sparse) {
+        if ((rc = launch_gain_sparse(c))) return rc;  // also writes X(a,a) and folds Corr in
+    } else {
+        if ((rc = ensure_diag(c))) return rc;
+        if ((rc = launch_gain_cols(c, c->gain_cols))) return rc;
+        if (c->e1 > 0)
+            BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->e1 + 255) / 256)), dim3(256), 0,
+                                                    c->stream, c->esrc, c->edst, c->ew, c->edw, c->e1, c->Q, c->ld,
+                                                    c->X, c->ldx));
+    }
     unsigned long long *gkey = c->keys + N1;
-    BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, T), dim3(256), 0, c->stream, c->X, c->ldx, c->diag,
-                                               c->n, c->keys, gkey, dmat));
+    const int T = (N1 + 63) / 64;
+    BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, (T + kCombChunk - 1) / kCombChunk), dim3(256), 0,
+                                               c->stream, c->X, c->ldx, c->diag, c->n, c->keys, gkey, dmat));
     std::vector<unsigned long long> keys;
     unsigned long long g = 0;
     if (row_best_b || row_best_delta) {
