@@ -452,6 +452,7 @@ static int tune_gain(bm_ctx *c) {
 // injective on mapped vertices and no G2 line names vertex 0 (W2 row/column 0
 // empty); the caller checks that.
 constexpr int kSparseChunk = 512;
+constexpr int kSparseUnroll = 8;  // list entries in flight per lane
 
 __global__ void k_g2_to_p(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
                           const int32_t *__restrict__ pinv, int64_t m) {
@@ -528,21 +529,24 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
                 if (pref[mid] <= w0) lo_i = mid;
                 else hi_i = mid - 1;
             }
-            // four list entries in flight per lane: indices idx, idx+64, idx+128, idx+192
+            // kSparseUnroll list entries in flight per lane: indices idx + 64u
             int i0 = lo_i;
             int idx = w0 + lane;
-            for (; idx + 192 < w1; idx += 256) {
-                int ii[4];
-                uint32_t yv[4];
+            for (; idx + 64 * (kSparseUnroll - 1) < w1; idx += 64 * kSparseUnroll) {
+                int ii[kSparseUnroll];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < kSparseUnroll; u++) {
                     while (pref[i0 + 1] <= idx + 64 * u) i0++;
                     ii[u] = i0;
-                    const int L = lst[i0];
-                    yv[u] = (L < 0 ? g2iy : g2oy)[(L & 0x7fffffff) + idx + 64 * u - pref[i0]];
+                }
+                uint32_t yv[kSparseUnroll];
+#pragma unroll
+                for (int u = 0; u < kSparseUnroll; u++) {
+                    const int L = lst[ii[u]];
+                    yv[u] = (L < 0 ? g2iy : g2oy)[(L & 0x7fffffff) + idx + 64 * u - pref[ii[u]]];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < kSparseUnroll; u++) {
                     const int y = yv[u] >> 16;
                     if (y != 0xFFFF) {
                         const int w = wts[ii[u]];
