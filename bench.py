@@ -107,6 +107,8 @@ def main():
     t1 = time.perf_counter()
     eng.set_timing(False)
     launches, gain_ms = eng.kernel_time("gain")
+    kind, cols = eng.sweep_config()
+    kname = "k_gain_sparse" if kind == 2 else f"k_gain<{cols}>"
     ms_step = (t1 - t0) * 1000.0 / args.steps
     gain_avg_ms = gain_ms / max(launches, 1)
     alg = algorithmic_bytes(n, two_e)
@@ -114,7 +116,8 @@ def main():
     traffic = None
     if os.path.exists(PMC_SUMMARY):
         with open(PMC_SUMMARY) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            pmc = json.load(f)
+        traffic = pmc.get("hbm_bytes_per_launch") if pmc.get("kernel") == kname else None
     out = {
         "metric": METRIC,
         "value": pairs / (ms_step / 1000.0),
@@ -135,7 +138,7 @@ def main():
         "best_move": {"a": best[0], "b": best[1], "delta": best[2]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_gain", "kernel_avg_ms": gain_avg_ms, "algorithmic_bytes_per_launch": alg},
+                     "kernel": kname, "kernel_avg_ms": gain_avg_ms, "algorithmic_bytes_per_launch": alg},
     }
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s)", file=sys.stderr, flush=True)

@@ -65,6 +65,7 @@ def lib():
         "bm_apply_swap": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
         "bm_apply_rot3": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
         "bm_apply_perm": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p]),
+        "bm_sweep_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "bm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
         "bm_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "bm_reset_timing": (C.c_int, [C.c_void_p]),
@@ -350,6 +351,12 @@ class Engine:
     def apply_perm(self, sset, perm):
         s, p = _i32(sset), _i32(perm)
         _check(self._L.bm_apply_perm(self._h, len(s), s, p), "bm_apply_perm")
+
+    def sweep_config(self):
+        """(kernel, dense_cols): 1 = dense slab kernel, 2 = sparse kernel, 0 = not chosen yet."""
+        k, cols = C.c_int(), C.c_int()
+        _check(self._L.bm_sweep_config(self._h, C.byref(k), C.byref(cols)), "bm_sweep_config")
+        return k.value, cols.value
 
     # --- timing
     def set_timing(self, on=True):

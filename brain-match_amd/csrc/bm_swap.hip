@@ -881,3 +881,11 @@ extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, i
     if (best_delta) *best_delta = bd;
     return BM_OK;
 }
+
+extern "C" int bm_sweep_config(bm_ctx *c, int *kernel, int *dense_cols) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    const bool sparse = c->sweep_kind == 2 && c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
+    if (kernel) *kernel = c->gain_cols == 0 ? 0 : sparse ? 2 : 1;
+    if (dense_cols) *dense_cols = c->gain_cols;
+    return BM_OK;
+}

@@ -145,6 +145,12 @@ int bm_apply_swap(bm_ctx *ctx, int32_t a, int32_t b);
 int bm_apply_rot3(bm_ctx *ctx, int32_t m1, int32_t m2, int32_t m3, int32_t p);
 int bm_apply_perm(bm_ctx *ctx, int32_t k, const int32_t *set, const int32_t *perm);
 
+/* Which all-pairs gain kernel bm_sweep_all_pairs runs for this graph pair,
+ * as chosen on its first call (0 before that): *kernel = 1 dense slab kernel
+ * (with *dense_cols columns per lane) or 2 sparse kernel.  No counterpart in
+ * the reference. */
+int bm_sweep_config(bm_ctx *ctx, int *kernel, int *dense_cols);
+
 /* Kernel timing.  When enabled, the engine brackets its kernels with HIP
  * events on the context stream; bm_kernel_time reports the launch count
  * and summed device milliseconds of the named kernel since the last reset.
