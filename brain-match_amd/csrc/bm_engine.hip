@@ -188,6 +188,41 @@ __global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ 
     }
 }
 
+// X(v,v) for the vertices whose diagonal a move on `set` changed: the set
+// itself and every G1 neighbour of it (the Q rows/columns that moved are
+// exactly those of the set).  One block per set vertex; its waves take the
+// neighbours one at a time and each recomputes diag[v] over comb(v).
+struct TouchSet {
+    int k;
+    int v[8];
+};
+
+__device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restrict__ cptr,
+                                       const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
+                                       const int16_t *__restrict__ Q, int ld) {
+    int s = 0;
+    for (int e = cptr[v] + lane; e < cptr[v + 1]; e += 64) s += min(cw[e], (int)Q[(size_t)cr[e] * ld + v]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    return s;
+}
+
+__global__ __launch_bounds__(512) void k_diag_touch(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
+                                                    const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
+                                                    const int16_t *__restrict__ Q, int N1, int ld, TouchSet ts) {
+    const int s = ts.v[blockIdx.x];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave == 0) {
+        const int d = diag_of(s, lane, cptr, cr, cw, Q, ld);
+        if (lane == 0) diag[s] = d;
+    }
+    for (int e = cptr[s] + wave; e < cptr[s + 1]; e += 8) {
+        const int r = cr[e];
+        const int v = r >= N1 ? r - N1 : r;
+        const int d = diag_of(v, lane, cptr, cr, cw, Q, ld);
+        if (lane == 0) diag[v] = d;  // a vertex reached twice gets the same value
+    }
+}
+
 struct SetPerm {
     int k;
     int set[8];
@@ -240,6 +275,16 @@ int ensure_diag(bm_ctx *c) {
     return BM_OK;
 }
 
+int touch_diag(bm_ctx *c, int k, const int32_t *set) {
+    TouchSet ts;
+    memset(&ts, 0, sizeof ts);
+    ts.k = k;
+    for (int i = 0; i < k; i++) ts.v[i] = set[i];
+    BM_LAUNCH(c, "diag_touch", hipLaunchKernelGGL(k_diag_touch, dim3(k), dim3(512), 0, c->stream, c->diag, c->cptr,
+                                                  c->cr, c->cw, c->Q, c->N1, c->ld, ts));
+    return BM_OK;
+}
+
 int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of) {
     SetPerm sp;
     memset(&sp, 0, sizeof sp);
@@ -259,8 +304,11 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
     if (c->sparse_ok)
         for (int i = 0; i < k; i++)
             if (old[i] != 0) c->hpinv[old[i]] = set[i];
-    c->diag_valid = false;
     c->g2y_dirty = true;
+    if (c->diag_valid) {  // keep X(v,v) current where it changed
+        int rc = touch_diag(c, k, set);
+        if (rc) return rc;
+    }
     return BM_OK;
 }
 
@@ -326,7 +374,9 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->cw, (size_t)c->ncomb * 4);
     dev_free(c, (void **)&c->cdw, (size_t)c->ncomb * 4);
     dev_free(c, (void **)&c->crw, (size_t)c->ncomb * 8);
-    dev_free(c, (void **)&c->rorder, N1 * 4);
+    dev_free(c, (void **)&c->cmid, N1 * 4);
+    dev_free(c, (void **)&c->cneg, N1 * 4);
+    dev_free(c, (void **)&c->rowacc, N1 * 4);
     dev_free(c, (void **)&c->ddptr, (N1 + 1) * 4);
     dev_free(c, (void **)&c->ddmid, N1 * 4);
     dev_free(c, (void **)&c->ddr, (size_t)c->ndd * 4);
@@ -550,16 +600,21 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = upload(c, &c->cw, cw))) goto fail;
     if ((rc = upload(c, &c->cdw, cdw))) goto fail;
     {
-        std::vector<int32_t> rw(2 * (size_t)nc), order(N1);
+        std::vector<int32_t> rw(2 * (size_t)nc);
         for (int64_t i = 0; i < nc; i++) {
             rw[2 * i] = cr[i];
             rw[2 * i + 1] = cw[i];
         }
-        for (int v = 0; v < N1; v++) order[v] = v;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int x, int y) { return cptr[x + 1] - cptr[x] > cptr[y + 1] - cptr[y]; });
         if ((rc = upload(c, &c->crw, rw))) goto fail;
-        if ((rc = upload(c, &c->rorder, order))) goto fail;
+        std::vector<int32_t> mid(N1), neg(N1, 0);
+        for (int v = 0; v < N1; v++) {
+            mid[v] = cptr[v] + cnt_out[v];
+            for (int e = cptr[v]; e < cptr[v + 1]; e++) neg[v] += std::min(cw[e], 0);
+        }
+        if ((rc = upload(c, &c->cmid, mid))) goto fail;
+        if ((rc = upload(c, &c->cneg, neg))) goto fail;
+        std::vector<int32_t> zeros(N1, 0);
+        if ((rc = upload(c, &c->rowacc, zeros))) goto fail;
     }
     if ((rc = upload(c, &c->ddptr, ddptr))) goto fail;
     if ((rc = upload(c, &c->ddmid, ddmid))) goto fail;
@@ -594,6 +649,12 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         if (e != hipSuccess) { rc = hip_fail(e, "upload sync"); goto fail; }
     }
     c->hmap.assign(N1, 0);
+    c->hcdeg.resize(N1);
+    c->hg2deg.resize(N1);
+    for (int v = 0; v < N1; v++) {
+        c->hcdeg[v] = cptr[v + 1] - cptr[v];
+        c->hg2deg[v] = (g2optr[v + 1] - g2optr[v]) + (g2iptr[v + 1] - g2iptr[v]);
+    }
     c->graphs = true;
     c->mapped = false;
     return BM_OK;

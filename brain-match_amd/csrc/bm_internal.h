@@ -62,7 +62,9 @@ struct bm_ctx {
     int32_t *map = nullptr;
     int32_t *cptr = nullptr, *cr = nullptr, *cw = nullptr, *cdw = nullptr;
     int32_t *crw = nullptr;     // (r, w) pairs of the comb lists, 8 bytes per entry
-    int32_t *rorder = nullptr;  // rows in decreasing comb-list length
+    int32_t *cmid = nullptr;    // cmid[v]: first in entry of comb(v)
+    int32_t *cneg = nullptr;    // cneg[v] = sum over comb(v) of min(w, 0)
+    int32_t *rowacc = nullptr;  // (n+1)-int accumulator of the sparse row kernel, kept zeroed
     int32_t *ddptr = nullptr, *ddmid = nullptr, *ddr = nullptr, *ddw = nullptr;
     int32_t *esrc = nullptr, *edst = nullptr, *ew = nullptr, *edw = nullptr;
     // G2 de-duplicated (last line wins) out- and in-lists, packed (vertex << 16 | (uint16)weight)
@@ -89,6 +91,8 @@ struct bm_ctx {
     // host mirror of the mapping
     std::vector<int32_t> hmap;
     std::vector<int32_t> hpinv;
+    std::vector<int32_t> hcdeg;   // comb-list length per vertex
+    std::vector<int32_t> hg2deg;  // G2 in- plus out-degree per G2 vertex (de-duplicated)
 
     // kernel timing
     bool timing = false;
@@ -117,6 +121,8 @@ int timer_begin(bm_ctx *c, const char *name);
 void timer_end(bm_ctx *c, int token);
 
 int ensure_diag(bm_ctx *c);
+// after a move on `set`: recompute X(v,v) for the set and its G1 neighbours
+int touch_diag(bm_ctx *c, int k, const int32_t *set);
 int launch_check(const char *what);
 
 // device-side row/column permutation of P and PT for a vertex set

@@ -219,6 +219,114 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
     }
 }
 
+// Sparse form of one row (taken when p is injective on mapped vertices and
+// no G2 line names vertex 0, the preconditions of k_gain_sparse):
+//   X(a,b) = C_a + sum over comb entries e of a, non-zero Q[r_e][b]:
+//            min(w_e, Q[r_e][b]) - min(w_e, 0)
+//   X(b,a) = C_b + sum over non-zero Q[r][a], comb entries (r, w) of b:
+//            min(w, Q[r][a]) - min(w, 0)
+// The non-zeros of column a of Q are the G2 neighbours of p(a) taken back
+// through p^-1: Q[s][a] != 0 for s = p^-1(g), g an in-neighbour of p(a), and
+// Q[n+1+d][a] != 0 for d = p^-1(g), g an out-neighbour; the comb entries
+// reading row s are the G1 out-lines s->b, those reading row n+1+d the G1
+// in-lines b->d.  k_row_expand gives each such list head one wave (comb(a)
+// entries, then in- and out-neighbours of p(a)) and adds the contributions
+// for b in [lo, hi] into a device row buffer with atomics; its block 0 also
+// adds Corr(a,b) for adjacent b and seeds the result key.  k_row_final forms
+// Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b), reduces it to the first
+// positive or arg-max key, and zeroes the buffer for the next call.
+__global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int mode, int N1,
+                                                    const int32_t *__restrict__ cptr,
+                                                    const int32_t *__restrict__ cmid, const int2 *__restrict__ crw,
+                                                    const int32_t *__restrict__ cdw, const int32_t *__restrict__ map,
+                                                    const int32_t *__restrict__ g2optr,
+                                                    const uint32_t *__restrict__ g2oy,
+                                                    const int32_t *__restrict__ g2iptr,
+                                                    const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
+                                                    int ld, int *__restrict__ acc,
+                                                    unsigned long long *__restrict__ result) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int ca0 = cptr[a], dega = cptr[a + 1] - ca0;
+    const int pa = map[a];
+    const int i0 = g2iptr[pa], nin = g2iptr[pa + 1] - i0;
+    const int o0 = g2optr[pa], nout = g2optr[pa + 1] - o0;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) *result = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
+        const int paa = (int)Q[(size_t)a * ld + a];
+        for (int e = ca0 + threadIdx.x; e < ca0 + dega; e += 256) {
+            const int2 rw = crw[e];
+            const bool out = rw.x >= N1;
+            const int b = out ? rw.x - N1 : rw.x;
+            if (b == a || b < lo || b > hi) continue;
+            const int pbb = (int)Q[(size_t)b * ld + b], pab = (int)Q[(size_t)a * ld + b],
+                      pba = (int)Q[(size_t)b * ld + a];
+            atomicAdd(&acc[b], pair_corr(out, rw.y, cdw[e], paa, pbb, pab, pba));
+        }
+        return;
+    }
+    const int h = (blockIdx.x - 1) * 4 + wave;
+    if (h < dega) {  // X(a, b): comb entry of a, its G2 list
+        const int2 rw = crw[ca0 + h];
+        const bool out = rw.x >= N1;
+        const int g = map[out ? rw.x - N1 : rw.x];
+        const uint32_t *L = out ? g2iy : g2oy;
+        const int l0 = out ? g2iptr[g] : g2optr[g], l1 = out ? g2iptr[g + 1] : g2optr[g + 1];
+        const int w = rw.y, wneg = min(w, 0);
+        for (int j = l0 + lane; j < l1; j += 64) {
+            const uint32_t yv = L[j];
+            const int y = yv >> 16;
+            if (y != 0xFFFF && y >= lo && y <= hi) atomicAdd(&acc[y], min(w, (int)(int16_t)(yv & 0xffff)) - wneg);
+        }
+    } else if (h < dega + nin + nout) {  // X(b, a): a G2 neighbour of p(a), the G1 lines of its preimage
+        const bool in = h < dega + nin;
+        const uint32_t yv = in ? g2iy[i0 + h - dega] : g2oy[o0 + h - dega - nin];
+        const int v = yv >> 16;
+        if (v == 0xFFFF) return;
+        const int val = (int)(int16_t)(yv & 0xffff);
+        // in-neighbour: rows s = v of P, read by the out-lines v->b (in entries of b);
+        // out-neighbour: rows n+1+v of P^T, read by the in-lines b->v (out entries of b)
+        const int l0 = in ? cptr[v] : cmid[v], l1 = in ? cmid[v] : cptr[v + 1];
+        for (int e = l0 + lane; e < l1; e += 64) {
+            const int2 rw = crw[e];
+            const int b = in ? rw.x - N1 : rw.x;
+            if (b >= lo && b <= hi) atomicAdd(&acc[b], min(rw.y, val) - min(rw.y, 0));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_row_final(int a, int lo, int hi, int mode, int *__restrict__ acc,
+                                                   const int32_t *__restrict__ cneg,
+                                                   const int32_t *__restrict__ diag,
+                                                   unsigned long long *__restrict__ result) {
+    __shared__ unsigned long long red[4];
+    const int b = lo + blockIdx.x * 256 + threadIdx.x;
+    unsigned long long key = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
+    if (b <= hi) {
+        const int x = acc[b];
+        acc[b] = 0;
+        if (b != a) {
+            const int d = x + cneg[a] + cneg[b] - diag[a] - diag[b];
+            if (mode == BM_ROW_FIRST_POSITIVE) {
+                if (d > 0) key = key_first(b, d);
+            } else {
+                key = key_max(b, d);
+            }
+        }
+    }
+    key = mode == BM_ROW_FIRST_POSITIVE ? wave_min_u64(key) : wave_max_u64(key);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long k = red[0];
+        for (int i = 1; i < 4; i++) k = mode == BM_ROW_FIRST_POSITIVE ? min(k, red[i]) : max(k, red[i]);
+        if (mode == BM_ROW_FIRST_POSITIVE) {
+            if (k != ~0ull) atomicMin(result, k);
+        } else if (k != 0ull) {
+            atomicMax(result, k);
+        }
+    }
+}
+
 // ------------------------------------------------------------ all pairs
 
 // X[a][c..c+W-1] = sum_{e in comb(a)} min(w_e, Q[r_e][c..c+W-1]) for every row a.
@@ -463,6 +571,19 @@ __global__ void k_g2_to_p(uint32_t *__restrict__ dst, const uint32_t *__restrict
     dst[i] = ((uint32_t)(y < 0 ? 0xFFFF : y) << 16) | (gv & 0xFFFFu);
 }
 
+// G2 lists with targets in P coordinates, rebuilt after the mapping changed
+static int refresh_g2y(bm_ctx *c) {
+    if (c->ng2 > 0) {
+        const unsigned blocks = (unsigned)((c->ng2 + 255) / 256);
+        BM_LAUNCH(c, "g2_to_p", hipLaunchKernelGGL(k_g2_to_p, dim3(blocks), dim3(256), 0, c->stream, c->g2oy, c->g2o,
+                                                   c->pinv, c->ng2));
+        BM_LAUNCH(c, "g2_to_p", hipLaunchKernelGGL(k_g2_to_p, dim3(blocks), dim3(256), 0, c->stream, c->g2iy, c->g2i,
+                                                   c->pinv, c->ng2));
+    }
+    c->g2y_dirty = false;
+    return BM_OK;
+}
+
 __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
                                                       const int32_t *__restrict__ cdw,
                                                       const int32_t *__restrict__ map,
@@ -590,14 +711,10 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
 }
 
 static int launch_gain_sparse(bm_ctx *c) {
-    if (c->g2y_dirty && c->ng2 > 0) {
-        const unsigned blocks = (unsigned)((c->ng2 + 255) / 256);
-        BM_LAUNCH(c, "g2_to_p", hipLaunchKernelGGL(k_g2_to_p, dim3(blocks), dim3(256), 0, c->stream, c->g2oy, c->g2o,
-                                                   c->pinv, c->ng2));
-        BM_LAUNCH(c, "g2_to_p", hipLaunchKernelGGL(k_g2_to_p, dim3(blocks), dim3(256), 0, c->stream, c->g2iy, c->g2i,
-                                                   c->pinv, c->ng2));
+    if (c->g2y_dirty) {
+        int rc = refresh_g2y(c);
+        if (rc) return rc;
     }
-    c->g2y_dirty = false;
     const size_t lds = (size_t)c->N1 * 4;
     BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse, dim3(c->N1), dim3(1024), lds, c->stream, c->cptr,
@@ -785,17 +902,31 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
     int rc = ensure_diag(c);
     if (rc) return rc;
     unsigned long long *res = c->keys + c->N1 + 1;
-    BM_HIP(hipMemsetAsync(res, mode == BM_ROW_FIRST_POSITIVE ? 0xFF : 0x00, 8, c->stream));
-    const int blocks = (b_hi - b_lo + kRowCh) / kRowCh;
-    const size_t lds = (size_t)2 * c->N1 * sizeof(int16_t);
-    if (lds <= 96 * 1024) {
-        BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<true>, dim3(blocks), dim3(256), lds, c->stream, a, b_lo,
-                                                    b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw, c->cdw,
-                                                    c->diag, res));
+    if (c->sparse_ok) {
+        if (c->g2y_dirty) {
+            if ((rc = refresh_g2y(c))) return rc;
+        }
+        const int pa = c->hmap[a];
+        const int heads = c->hcdeg[a] + c->hg2deg[pa];
+        BM_LAUNCH(c, "row_expand", hipLaunchKernelGGL(k_row_expand, dim3(1 + (heads + 3) / 4), dim3(256), 0, c->stream,
+                                                      a, b_lo, b_hi, mode, c->N1, c->cptr, c->cmid,
+                                                      (const int2 *)c->crw, c->cdw, c->map, c->g2optr, c->g2oy,
+                                                      c->g2iptr, c->g2iy, c->Q, c->ld, c->rowacc, res));
+        BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((b_hi - b_lo + 256) / 256), dim3(256), 0,
+                                                     c->stream, a, b_lo, b_hi, mode, c->rowacc, c->cneg, c->diag, res));
     } else {
-        BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<false>, dim3(blocks), dim3(256), 0, c->stream, a, b_lo,
-                                                    b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw, c->cdw,
-                                                    c->diag, res));
+        BM_HIP(hipMemsetAsync(res, mode == BM_ROW_FIRST_POSITIVE ? 0xFF : 0x00, 8, c->stream));
+        const int blocks = (b_hi - b_lo + kRowCh) / kRowCh;
+        const size_t lds = (size_t)2 * c->N1 * sizeof(int16_t);
+        if (lds <= 96 * 1024) {
+            BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<true>, dim3(blocks), dim3(256), lds, c->stream, a,
+                                                        b_lo, b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw,
+                                                        c->cdw, c->diag, res));
+        } else {
+            BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<false>, dim3(blocks), dim3(256), 0, c->stream, a,
+                                                        b_lo, b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw,
+                                                        c->cdw, c->diag, res));
+        }
     }
     unsigned long long k = 0;
     BM_HIP(hipMemcpyAsync(&k, res, 8, hipMemcpyDeviceToHost, c->stream));

@@ -226,3 +226,32 @@ def test_global_best_after_moves_each_kernel(kernel, monkeypatch):
                 break
             e.apply_swap(a, b)
             cur[a], cur[b] = cur[b], cur[a]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_row_kernel_after_moves(case):
+    # the row kernels (sparse where p is injective, dense otherwise) and the
+    # incrementally maintained X(v,v) after swaps, rotations and set moves
+    n, g1, g2, mp = make_case(case, seed=29)
+    o = ob.Oracle(n, g1, g2)
+    rng = np.random.default_rng(5)
+    cur = mp.copy()
+    with engine_for(n, g1, g2, mp) as e:
+        e.eval_swap_row(1, 1, n)  # X(v,v) valid from here on
+        for step in range(12):
+            a, b, c3 = (int(x) for x in rng.choice(np.arange(1, n + 1), 3, replace=False))
+            if step % 3 == 0:
+                e.apply_swap(a, b)
+                cur[a], cur[b] = cur[b], cur[a]
+            elif step % 3 == 1:
+                e.apply_rot3(a, b, c3, 4)
+                cur = ob.apply_g3(cur, a, b, c3, 4)
+            else:
+                e.apply_perm([a, b, c3], [1, 2, 0])
+                cur = ob.apply_perm(cur, [a, b, c3], [1, 2, 0])
+            row = int(rng.integers(1, n + 1))
+            ds = {bb: o.delta_b(cur, row, bb) for bb in range(1, n + 1) if bb != row}
+            pos = [bb for bb in sorted(ds) if ds[bb] > 0]
+            assert e.eval_swap_row(row, 1, n) == ((pos[0], ds[pos[0]]) if pos else (0, 0))
+            bb = max(ds, key=lambda x: (ds[x], -x))
+            assert e.eval_swap_row(row, 1, n, bmatch.BM_ROW_ARGMAX) == (bb, ds[bb])
