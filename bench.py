@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB")
+    ap.add_argument("--workload", default="allpairs",
+                    choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "serve"],
+                    help="allpairs (default, the headline); the others measure the other configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sweep would shard rows "
@@ -86,6 +89,12 @@ def main():
         return
 
     import bmatch
+
+    if args.workload != "allpairs":
+        from bmatch import bench_workloads
+
+        print(json.dumps(bench_workloads.run(args.workload, args)))
+        return
 
     print(f"[bench] generating n={N} E={E} (seed {SEED})", file=sys.stderr, flush=True)
     inst = bmatch.Instance.chunglu(SEED, N, E)

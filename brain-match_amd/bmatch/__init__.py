@@ -66,6 +66,10 @@ def lib():
         "bm_apply_rot3": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
         "bm_apply_perm": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p]),
         "bm_sweep_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "bm_moves_set": (C.c_int, [C.c_void_p, C.c_int32, _i32p, C.c_int64]),
+        "bm_moves_eval": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32)]),
+        "bm_moves_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "bm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
         "bm_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "bm_reset_timing": (C.c_int, [C.c_void_p]),
@@ -123,6 +127,9 @@ class _Instance(C.Structure):
 class _DriveStats(C.Structure):
     _fields_ = [("sweeps", C.c_int64), ("rows", C.c_int64), ("accepted", C.c_int64), ("evals_rows", C.c_int64),
                 ("initial_score", C.c_int64), ("final_score", C.c_int64), ("mismatches", C.c_int64)]
+
+
+_ROW_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_int64)
 
 
 class Instance:
@@ -352,6 +359,24 @@ class Engine:
         s, p = _i32(sset), _i32(perm)
         _check(self._L.bm_apply_perm(self._h, len(s), s, p), "bm_apply_perm")
 
+    def moves_set(self, k, verts):
+        """Upload a resident candidate list: k=2 pairs, k=3 triples, 4..8 sets."""
+        v = _i32(verts).reshape(-1)
+        _check(self._L.bm_moves_set(self._h, k, v, len(v) // k), "bm_moves_set")
+
+    def moves_eval(self, mode=0):
+        """Score the resident list on the device: (best index or -1, delta, aux)."""
+        i, d, x = C.c_int64(), C.c_int32(), C.c_int32()
+        _check(self._L.bm_moves_eval(self._h, mode, C.byref(i), C.byref(d), C.byref(x)), "bm_moves_eval")
+        return i.value, d.value, x.value
+
+    def moves_results(self, count):
+        d = np.zeros(count, np.int32)
+        x = np.zeros(count, np.int32)
+        _check(self._L.bm_moves_results(self._h, d.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p)),
+               "bm_moves_results")
+        return d, x
+
     def sweep_config(self):
         """(kernel, dense_cols): 1 = dense slab kernel, 2 = sparse kernel, 0 = not chosen yet."""
         k, cols = C.c_int(), C.c_int()
@@ -378,9 +403,13 @@ class Engine:
         k = min(st.accepted, acc_cap)
         return acc[: 3 * k].reshape(k, 3), {f: getattr(st, f) for f, _ in _DriveStats._fields_}
 
-    def first_improvement(self, rows_per_sweep=0, max_sweeps=0, verify=True, acc_cap=1 << 20):
-        return self._drive(self._L.bm_drive_first_improvement, rows_per_sweep, max_sweeps, int(verify), None, None,
-                           acc_cap=acc_cap)
+    def first_improvement(self, rows_per_sweep=0, max_sweeps=0, verify=True, acc_cap=1 << 20, max_rows=0):
+        """bm_drive_first_improvement; max_rows > 0 stops after that many rows (a row hook)."""
+        hook = None
+        if max_rows > 0:
+            hook = _ROW_HOOK(lambda user, rows, row, score, imp: 1 if rows > max_rows else 0)
+        return self._drive(self._L.bm_drive_first_improvement, rows_per_sweep, max_sweeps, int(verify),
+                           C.cast(hook, C.c_void_p) if hook else None, None, acc_cap=acc_cap)
 
     def row_best(self, max_sweeps=0, verify=True, acc_cap=1 << 20):
         return self._drive(self._L.bm_drive_row_best, max_sweeps, int(verify), None, None, acc_cap=acc_cap)

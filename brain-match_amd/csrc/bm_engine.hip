@@ -395,6 +395,7 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->g2iy, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->X, (N1 + 63) / 64 * 64 * (size_t)c->ldx * 4);
     dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
+    free_moves(c);
     c->graphs = c->mapped = c->diag_valid = false;
     c->gain_cols = 0;
     c->sweep_kind = 0;

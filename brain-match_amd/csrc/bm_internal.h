@@ -88,6 +88,12 @@ struct bm_ctx {
     size_t stage_bytes = 0;
     int64_t dev_bytes = 0;
 
+    // device-resident candidate list (bm_moves_*)
+    int32_t mv_k = 0;
+    int64_t mv_count = 0, mv_cap = 0;
+    int32_t *mv_verts = nullptr, *mv_delta = nullptr, *mv_aux = nullptr;
+    int *mv_flag = nullptr;
+
     // host mirror of the mapping
     std::vector<int32_t> hmap;
     std::vector<int32_t> hpinv;
@@ -124,6 +130,12 @@ int ensure_diag(bm_ctx *c);
 // after a move on `set`: recompute X(v,v) for the set and its G1 neighbours
 int touch_diag(bm_ctx *c, int k, const int32_t *set);
 int launch_check(const char *what);
+
+// launchers shared across translation units (device pointers, context stream)
+int launch_eval_pairs(bm_ctx *c, const int32_t *dA, const int32_t *dB, int64_t m, int32_t *dD);
+int launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP);
+int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv);
+int free_moves(bm_ctx *c);
 
 // device-side row/column permutation of P and PT for a vertex set
 int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of);

@@ -244,6 +244,22 @@ __global__ __launch_bounds__(256) void k_hillclimb(const int32_t *__restrict__ A
     if (pair < count && gl == 0) out[pair] = sum;
 }
 
+// ------------------------------------------------------------ launchers
+
+int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP) {
+    if (m <= 0) return BM_OK;
+    BM_LAUNCH(c, "rot3", hipLaunchKernelGGL(k_rot3, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0, c->stream,
+                                            dT, m, mode, dD, dP, c->ddptr, c->ddr, c->ddw, c->Q, c->N1, c->ld));
+    return BM_OK;
+}
+
+int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv) {
+    if (m <= 0) return BM_OK;
+    BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset, dim3((unsigned)m), dim3(256), 0, c->stream, dS, m, k, dD, dR,
+                                               c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv));
+    return BM_OK;
+}
+
 // ------------------------------------------------------------ C ABI
 
 extern "C" int bm_eval_rot3(bm_ctx *c, const int32_t *triples, int64_t count, int mode, int32_t *best_delta,
@@ -268,9 +284,7 @@ extern "C" int bm_eval_rot3(bm_ctx *c, const int32_t *triples, int64_t count, in
     for (int64_t off = 0; off < count; off += chunk) {
         const int64_t m = std::min(chunk, count - off);
         BM_HIP(hipMemcpyAsync(dT, triples + 3 * off, m * 12, hipMemcpyHostToDevice, c->stream));
-        BM_LAUNCH(c, "rot3", hipLaunchKernelGGL(k_rot3, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0,
-                                                c->stream, dT, m, mode, dD, dP, c->ddptr, c->ddr, c->ddw, c->Q, c->N1,
-                                                c->ld));
+        if ((rc = launch_rot3(c, dT, m, mode, dD, dP))) return rc;
         BM_HIP(hipMemcpyAsync(best_delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipMemcpyAsync(best_perm + off, dP, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipStreamSynchronize(c->stream));
@@ -303,8 +317,7 @@ extern "C" int bm_eval_permset(bm_ctx *c, int32_t k, const int32_t *sets, int64_
     for (int64_t off = 0; off < count; off += chunk) {
         const int64_t m = std::min(chunk, count - off);
         BM_HIP(hipMemcpyAsync(dS, sets + (size_t)k * off, (size_t)m * k * 4, hipMemcpyHostToDevice, c->stream));
-        BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset, dim3((unsigned)m), dim3(256), 0, c->stream, dS, m, k, dD,
-                                                   dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv));
+        if ((rc = launch_permset(c, k, dS, m, dD, dR, dOv))) return rc;
         BM_HIP(hipMemcpyAsync(best_delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipMemcpyAsync(best_rank + off, dR, m * 4, hipMemcpyDeviceToHost, c->stream));
     }

@@ -856,6 +856,14 @@ __global__ __launch_bounds__(256) void k_combine(const int32_t *__restrict__ X, 
 
 // ------------------------------------------------------------ C ABI
 
+int bm::launch_eval_pairs(bm_ctx *c, const int32_t *dA, const int32_t *dB, int64_t m, int32_t *dD) {
+    if (m <= 0) return BM_OK;
+    const unsigned blocks = (unsigned)((m * 16 + 255) / 256);
+    BM_LAUNCH(c, "eval_pairs", hipLaunchKernelGGL(k_eval_pairs, dim3(blocks), dim3(256), 0, c->stream, dA, dB, m, dD,
+                                                  c->cptr, c->cr, c->cw, c->cdw, c->Q, c->N1, c->ld));
+    return BM_OK;
+}
+
 static int check_pairs(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count) {
     for (int64_t i = 0; i < count; i++)
         if (a[i] < 0 || a[i] > c->n || b[i] < 0 || b[i] > c->n || a[i] == b[i])
@@ -878,9 +886,7 @@ extern "C" int bm_eval_swaps(bm_ctx *c, const int32_t *a, const int32_t *b, int6
         const int64_t m = std::min(chunk, count - off);
         BM_HIP(hipMemcpyAsync(dA, a + off, m * 4, hipMemcpyHostToDevice, c->stream));
         BM_HIP(hipMemcpyAsync(dB, b + off, m * 4, hipMemcpyHostToDevice, c->stream));
-        const unsigned blocks = (unsigned)((m * 16 + 255) / 256);
-        BM_LAUNCH(c, "eval_pairs", hipLaunchKernelGGL(k_eval_pairs, dim3(blocks), dim3(256), 0, c->stream, dA, dB, m,
-                                                      dD, c->cptr, c->cr, c->cw, c->cdw, c->Q, c->N1, c->ld));
+        if ((rc = launch_eval_pairs(c, dA, dB, m, dD))) return rc;
         BM_HIP(hipMemcpyAsync(delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipStreamSynchronize(c->stream));
     }

@@ -145,6 +145,22 @@ int bm_apply_swap(bm_ctx *ctx, int32_t a, int32_t b);
 int bm_apply_rot3(bm_ctx *ctx, int32_t m1, int32_t m2, int32_t m3, int32_t p);
 int bm_apply_perm(bm_ctx *ctx, int32_t k, const int32_t *set, const int32_t *perm);
 
+/* Device-resident candidate list.  bm_moves_set copies `count` moves of
+ * size k to the device once: k = 2 pairs (a, b) scored with Delta_B, k = 3
+ * triples scored with Delta_G3, 4 <= k <= 8 sets scored with Delta_P over
+ * all k! permutations (the move kinds of bm_eval_swaps, bm_eval_rot3 and
+ * bm_eval_permset).  bm_moves_eval scores the whole list with the current
+ * mapping on the device and reduces it there: the best move (largest delta,
+ * lowest index on ties, only if > 0; *best_index = -1 otherwise), its delta
+ * and aux (the permutation index p of triple.cu:85-92 for k = 3, selected by
+ * `mode` as in bm_eval_rot3; the lexicographic rank for k >= 4; 0 for pairs).
+ * bm_moves_results copies the per-move deltas / aux back.  This is the
+ * launch + host arg-max of triple.cu:1082-1126 and quick.cu:1175-1217 with
+ * the list and the reduction kept on the GPU. */
+int bm_moves_set(bm_ctx *ctx, int32_t k, const int32_t *verts, int64_t count);
+int bm_moves_eval(bm_ctx *ctx, int mode, int64_t *best_index, int32_t *best_delta, int32_t *best_aux);
+int bm_moves_results(bm_ctx *ctx, int32_t *delta, int32_t *aux);
+
 /* Which all-pairs gain kernel bm_sweep_all_pairs runs for this graph pair,
  * as chosen on its first call (0 before that): *kernel = 1 dense slab kernel
  * (with *dense_cols columns per lane) or 2 sparse kernel.  No counterpart in

@@ -98,3 +98,40 @@ def test_hillclimb_delta_matches_oracle():
         got = e.eval_swaps_hillclimb(a, b)
     want = [o.delta_a(mp, int(x), int(y)) for x, y in zip(a, b)]
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 8])
+def test_resident_moves_match_host_lists(k):
+    n, g1, g2, mp, e, o = setup(50 + k, n=40, m=450, unm=0.1)
+    rng = np.random.default_rng(k)
+    cnt = 30 if k == 8 else 700
+    moves = np.array([rng.choice(np.arange(1, n + 1), k, replace=False) for _ in range(cnt)], np.int32)
+    with e:
+        e.moves_set(k, moves)
+        for mode in ([bmatch.BM_ROT3_ROTATIONS, bmatch.BM_ROT3_ALL] if k == 3 else [0]):
+            bi, bd, bx = e.moves_eval(mode)
+            d, x = e.moves_results(cnt)
+            if k == 2:
+                want_d = e.eval_swaps(moves[:, 0], moves[:, 1])
+                want_x = np.zeros(cnt, np.int32)
+            elif k == 3:
+                want_d, want_x = e.eval_rot3(moves, mode)
+            else:
+                want_d, want_x = e.eval_permset(moves)
+            np.testing.assert_array_equal(d, want_d)
+            if k >= 3:
+                np.testing.assert_array_equal(x, want_x)
+            if (want_d > 0).any():
+                i = int(np.argmax(want_d))  # first index of the maximum
+                assert (bi, bd, bx) == (i, int(want_d[i]), int(want_x[i]))
+            else:
+                assert (bi, bd, bx) == (-1, 0, 0)
+        # the list stays resident across a move
+        a, b = int(moves[0][0]), int(moves[0][1])
+        e.apply_swap(a, b)
+        mp2 = mp.copy()
+        mp2[a], mp2[b] = mp2[b], mp2[a]
+        e.moves_eval(bmatch.BM_ROT3_ALL if k == 3 else 0)
+        d2, _ = e.moves_results(cnt)
+        if k == 2:
+            np.testing.assert_array_equal(d2, [o.delta_b(mp2, int(p), int(q)) for p, q in moves])
