@@ -190,11 +190,13 @@ __global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ 
 
 // X(v,v) for the vertices whose diagonal a move on `set` changed: the set
 // itself and every G1 neighbour of it (the Q rows/columns that moved are
-// exactly those of the set).  One block per set vertex; its waves take the
-// neighbours one at a time and each recomputes diag[v] over comb(v).
+// exactly those of the set).  One wave per touched vertex: wave w of the grid
+// maps through the prefix `off` to set vertex i and its entry w - off[i]
+// (entry -1 is the set vertex itself); each recomputes diag[v] over comb(v).
 struct TouchSet {
     int k;
     int v[8];
+    int off[9];  // prefix of (1 + list length) over the set
 };
 
 __device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restrict__ cptr,
@@ -206,21 +208,21 @@ __device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restric
     return s;
 }
 
-__global__ __launch_bounds__(512) void k_diag_touch(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
+__global__ __launch_bounds__(256) void k_diag_touch(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
                                                     const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
                                                     const int16_t *__restrict__ Q, int N1, int ld, TouchSet ts) {
-    const int s = ts.v[blockIdx.x];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (wave == 0) {
-        const int d = diag_of(s, lane, cptr, cr, cw, Q, ld);
-        if (lane == 0) diag[s] = d;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= ts.off[ts.k]) return;
+    int i = 0;
+    while (w >= ts.off[i + 1]) i++;
+    const int s = ts.v[i], j = w - ts.off[i] - 1;
+    int v = s;
+    if (j >= 0) {
+        const int r = cr[cptr[s] + j];
+        v = r >= N1 ? r - N1 : r;
     }
-    for (int e = cptr[s] + wave; e < cptr[s + 1]; e += 8) {
-        const int r = cr[e];
-        const int v = r >= N1 ? r - N1 : r;
-        const int d = diag_of(v, lane, cptr, cr, cw, Q, ld);
-        if (lane == 0) diag[v] = d;  // a vertex reached twice gets the same value
-    }
+    const int d = diag_of(v, lane, cptr, cr, cw, Q, ld);
+    if (lane == 0) diag[v] = d;  // a vertex reached twice gets the same value
 }
 
 struct SetPerm {
@@ -266,6 +268,31 @@ __global__ void k_perm_map(int32_t *__restrict__ map, int32_t *__restrict__ pinv
 
 namespace bm {
 
+int launch_rebuild_p(bm_ctx *c) {
+    int16_t *P = c->Q, *PT = c->Q + (size_t)c->N1 * c->ld;
+    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_build_p, dim3(c->ld / 256 + 1, c->N1), dim3(256), 0, c->stream, P,
+                                               c->W2, c->map, c->N1, c->ld));
+    BM_LAUNCH(c, "transpose", hipLaunchKernelGGL(k_transpose, dim3((c->N1 + 63) / 64, c->ld / 64), dim3(64, 4), 0,
+                                                 c->stream, PT, P, c->N1, c->ld));
+    return BM_OK;
+}
+
+int launch_diag(bm_ctx *c) {
+    BM_LAUNCH(c, "diag", hipLaunchKernelGGL(k_diag, dim3((c->N1 + 3) / 4), dim3(256), 0, c->stream, c->diag, c->cptr,
+                                            c->cr, c->cw, c->Q, c->N1, c->ld));
+    return BM_OK;
+}
+
+// S_opt into *out (which must hold 0 beforehand); no host synchronisation
+int launch_score(bm_ctx *c, unsigned long long *out) {
+    int64_t m = c->e1;
+    int blocks = (int)std::min<int64_t>((m + 255) / 256, 4096);
+    if (blocks < 1) blocks = 1;
+    BM_LAUNCH(c, "score", hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, c->stream, out, c->esrc, c->edst,
+                                             c->ew, m, c->Q, c->ld));
+    return BM_OK;
+}
+
 int ensure_diag(bm_ctx *c) {
     if (c->diag_valid) return BM_OK;
     int blocks = (c->N1 + 3) / 4;
@@ -279,9 +306,13 @@ int touch_diag(bm_ctx *c, int k, const int32_t *set) {
     TouchSet ts;
     memset(&ts, 0, sizeof ts);
     ts.k = k;
-    for (int i = 0; i < k; i++) ts.v[i] = set[i];
-    BM_LAUNCH(c, "diag_touch", hipLaunchKernelGGL(k_diag_touch, dim3(k), dim3(512), 0, c->stream, c->diag, c->cptr,
-                                                  c->cr, c->cw, c->Q, c->N1, c->ld, ts));
+    for (int i = 0; i < k; i++) {
+        ts.v[i] = set[i];
+        ts.off[i + 1] = ts.off[i] + 1 + c->hcdeg[set[i]];
+    }
+    const int waves = ts.off[k];
+    BM_LAUNCH(c, "diag_touch", hipLaunchKernelGGL(k_diag_touch, dim3((waves + 3) / 4), dim3(256), 0, c->stream,
+                                                  c->diag, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, ts));
     return BM_OK;
 }
 

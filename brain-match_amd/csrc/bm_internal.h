@@ -88,6 +88,9 @@ struct bm_ctx {
     size_t stage_bytes = 0;
     int64_t dev_bytes = 0;
 
+    // batched refinement (bm_batch_*)
+    struct Batch *batch = nullptr;
+
     // device-resident candidate list (bm_moves_*)
     int32_t mv_k = 0;
     int64_t mv_count = 0, mv_cap = 0;
@@ -136,6 +139,13 @@ int launch_eval_pairs(bm_ctx *c, const int32_t *dA, const int32_t *dB, int64_t m
 int launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP);
 int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv);
 int free_moves(bm_ctx *c);
+int free_batch(bm_ctx *c);
+int launch_rebuild_p(bm_ctx *c);        // P and P^T from W2 and the device mapping
+int launch_diag(bm_ctx *c);             // X(v,v) for every v
+int launch_score(bm_ctx *c, unsigned long long *out);
+int launch_refresh_g2y(bm_ctx *c);      // G2 lists through p^-1
+int sweep_prepare(bm_ctx *c);           // allocations and kernel choice of the all-pairs sweep
+int sweep_enqueue(bm_ctx *c, int32_t *dmat);  // row-best keys into keys[0..n], global best into keys[n+1]
 
 // device-side row/column permutation of P and PT for a vertex set
 int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of);

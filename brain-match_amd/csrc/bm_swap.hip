@@ -572,7 +572,7 @@ __global__ void k_g2_to_p(uint32_t *__restrict__ dst, const uint32_t *__restrict
 }
 
 // G2 lists with targets in P coordinates, rebuilt after the mapping changed
-static int refresh_g2y(bm_ctx *c) {
+int bm::launch_refresh_g2y(bm_ctx *c) {
     if (c->ng2 > 0) {
         const unsigned blocks = (unsigned)((c->ng2 + 255) / 256);
         BM_LAUNCH(c, "g2_to_p", hipLaunchKernelGGL(k_g2_to_p, dim3(blocks), dim3(256), 0, c->stream, c->g2oy, c->g2o,
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
 
 static int launch_gain_sparse(bm_ctx *c) {
     if (c->g2y_dirty) {
-        int rc = refresh_g2y(c);
+        int rc = launch_refresh_g2y(c);
         if (rc) return rc;
     }
     const size_t lds = (size_t)c->N1 * 4;
@@ -910,7 +910,7 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
     unsigned long long *res = c->keys + c->N1 + 1;
     if (c->sparse_ok) {
         if (c->g2y_dirty) {
-            if ((rc = refresh_g2y(c))) return rc;
+            if ((rc = launch_refresh_g2y(c))) return rc;
         }
         const int pa = c->hmap[a];
         const int heads = c->hcdeg[a] + c->hg2deg[pa];
@@ -949,6 +949,50 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
     return BM_OK;
 }
 
+__global__ void k_fill_keys(unsigned long long *__restrict__ p, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0ull;
+}
+
+int bm::sweep_prepare(bm_ctx *c) {
+    int rc;
+    if (!c->X) {
+        c->ldx = c->ld;
+        // tile-major: whole 64-row tiles, the last one partly unused
+        if ((rc = dev_alloc(c, (void **)&c->X, (size_t)((c->N1 + 63) / 64 * 64) * c->ldx * 4))) return rc;
+    }
+    if ((rc = choose_gain(c))) return rc;
+    if (c->g2y_dirty && c->sparse_ok) {
+        if ((rc = launch_refresh_g2y(c))) return rc;
+    }
+    return BM_OK;
+}
+
+// Graph-capturable: kernels only (the choice of gain kernel and all
+// allocations are made by sweep_prepare).
+int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat) {
+    const int N1 = c->N1;
+    int rc;
+    BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill_keys, dim3((N1 + 256) / 256), dim3(256), 0, c->stream, c->keys,
+                                            (int64_t)N1 + 1));
+    const bool sparse = c->sweep_kind == 2 && c->sparse_ok && (size_t)N1 * 4 <= kSparseLdsMax;
+    if (sparse) {
+        if ((rc = launch_gain_sparse(c))) return rc;  // also writes X(a,a) and folds Corr in
+    } else {
+        if ((rc = launch_diag(c))) return rc;
+        if ((rc = launch_gain_cols(c, c->gain_cols))) return rc;
+        if (c->e1 > 0)
+            BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->e1 + 255) / 256)), dim3(256), 0,
+                                                    c->stream, c->esrc, c->edst, c->ew, c->edw, c->e1, c->Q, c->ld,
+                                                    c->X, c->ldx));
+    }
+    c->diag_valid = true;
+    const int T = (N1 + 63) / 64;
+    BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, (T + kCombChunk - 1) / kCombChunk), dim3(256), 0,
+                                               c->stream, c->X, c->ldx, c->diag, c->n, c->keys, c->keys + N1, dmat));
+    return BM_OK;
+}
+
 extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
                                   int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix) {
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");
@@ -956,34 +1000,15 @@ extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, i
     BM_HIP(hipSetDevice(c->device));
     const int N1 = c->N1;
     int rc;
-    if (!c->X) {
-        c->ldx = c->ld;
-        // tile-major: whole 64-row tiles, the last one partly unused
-        if ((rc = dev_alloc(c, (void **)&c->X, (size_t)((N1 + 63) / 64 * 64) * c->ldx * 4))) return rc;
-    }
     int32_t *dmat = nullptr;
     if (delta_matrix) {
         if ((rc = ensure_stage(c, (size_t)N1 * N1 * 4))) return rc;
         dmat = (int32_t *)c->stage;
         BM_HIP(hipMemsetAsync(dmat, 0, (size_t)N1 * N1 * 4, c->stream));
     }
-    BM_HIP(hipMemsetAsync(c->keys, 0, ((size_t)N1 + 1) * 8, c->stream));
-    if ((rc = choose_gain(c))) return rc;
-    const bool sparse = c->sweep_kind == 2 && c->sparse_ok && (size_t)N1 * 4 <= kSparseLdsMax;
-    if (sparse) {
-        if ((rc = launch_gain_sparse(c))) return rc;  // also writes X(a,a) and folds Corr in
-    } else {
-        if ((rc = ensure_diag(c))) return rc;
-        if ((rc = launch_gain_cols(c, c->gain_cols))) return rc;
-        if (c->e1 > 0)
-            BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->e1 + 255) / 256)), dim3(256), 0,
-                                                    c->stream, c->esrc, c->edst, c->ew, c->edw, c->e1, c->Q, c->ld,
-                                                    c->X, c->ldx));
-    }
+    if ((rc = sweep_prepare(c))) return rc;
+    if ((rc = sweep_enqueue(c, dmat))) return rc;
     unsigned long long *gkey = c->keys + N1;
-    const int T = (N1 + 63) / 64;
-    BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, (T + kCombChunk - 1) / kCombChunk), dim3(256), 0,
-                                               c->stream, c->X, c->ldx, c->diag, c->n, c->keys, gkey, dmat));
     std::vector<unsigned long long> keys;
     unsigned long long g = 0;
     if (row_best_b || row_best_delta) {
