@@ -70,6 +70,12 @@ def lib():
         "bm_moves_eval": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                     C.POINTER(C.c_int32)]),
         "bm_moves_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+        "bm_refine_configure": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "bm_refine_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_void_p]),
+        "bm_refine_history": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+        "bm_refine_log": (C.c_int, [C.c_void_p, C.c_int32]),
+        "bm_refine_moves": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.c_int64]),
+        "bm_refine_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "bm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
         "bm_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "bm_reset_timing": (C.c_int, [C.c_void_p]),
@@ -130,6 +136,28 @@ class _DriveStats(C.Structure):
 
 
 _ROW_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_int64)
+
+
+class RefineParams(C.Structure):
+    """bm_refine_params of include/bmatch.h."""
+    _fields_ = [("pairs", C.c_int32), ("triples", C.c_int32), ("sets", C.c_int32), ("set_k", C.c_int32),
+                ("rounds", C.c_int32), ("steps_per_check", C.c_int32), ("seed", C.c_uint64)]
+
+
+class _RefineStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("moves", C.c_int64), ("pair_moves", C.c_int64),
+                ("triple_moves", C.c_int64), ("set_moves", C.c_int64), ("initial_score", C.c_int64),
+                ("final_score", C.c_int64), ("idle_steps", C.c_int64), ("bad_step", C.c_int64),
+                ("converged", C.c_int32), ("verified", C.c_int32)]
+
+
+class _RefineStep(C.Structure):
+    _fields_ = [("score", C.c_int64), ("gain", C.c_int64), ("moves", C.c_int32), ("ok", C.c_int32)]
+
+
+def refine_params(pairs=1, triples=0, sets=0, set_k=4, rounds=2, steps_per_check=4, seed=1):
+    return RefineParams(int(pairs), int(triples), int(sets), int(set_k), int(rounds), int(steps_per_check),
+                        int(seed))
 
 
 class Instance:
@@ -382,6 +410,40 @@ class Engine:
         k, cols = C.c_int(), C.c_int()
         _check(self._L.bm_sweep_config(self._h, C.byref(k), C.byref(cols)), "bm_sweep_config")
         return k.value, cols.value
+
+    # --- batched refinement
+    def refine_configure(self, params=None, **kw):
+        """bm_refine_configure with a RefineParams (or the keyword fields of refine_params)."""
+        prm = params if params is not None else refine_params(**kw)
+        _check(self._L.bm_refine_configure(self._h, C.byref(prm)), "bm_refine_configure")
+
+    def refine_run(self, max_steps=1 << 30, move_budget=0, patience=2):
+        st = _RefineStats()
+        _check(self._L.bm_refine_run(self._h, max_steps, move_budget, patience, C.byref(st)), "bm_refine_run")
+        return {f: getattr(st, f) for f, _ in _RefineStats._fields_}
+
+    def refine_history(self, first, count):
+        out = (_RefineStep * max(count, 1))()
+        _check(self._L.bm_refine_history(self._h, first, count, out), "bm_refine_history")
+        return [(h.score, h.gain, h.moves, h.ok) for h in out[:count]]
+
+    def refine_log(self, capacity):
+        _check(self._L.bm_refine_log(self._h, capacity), "bm_refine_log")
+
+    def refine_moves(self, capacity=1 << 20):
+        """Applied-move records of the last run: int32 array (count, 20) =
+        step, kind, k, delta, vertices[8], new images[8]."""
+        cnt = C.c_int64()
+        _check(self._L.bm_refine_moves(self._h, C.byref(cnt), None, 0), "bm_refine_moves")
+        m = min(cnt.value, capacity)
+        rec = np.zeros((max(m, 1), 20), np.int32)
+        _check(self._L.bm_refine_moves(self._h, C.byref(cnt), rec.ctypes.data_as(C.c_void_p), m), "bm_refine_moves")
+        return rec[:m], cnt.value
+
+    def refine_config(self):
+        a, b = C.c_int32(), C.c_int32()
+        _check(self._L.bm_refine_config(self._h, C.byref(a), C.byref(b)), "bm_refine_config")
+        return a.value, b.value
 
     # --- timing
     def set_timing(self, on=True):

@@ -426,7 +426,9 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->g2iy, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->X, (N1 + 63) / 64 * 64 * (size_t)c->ldx * 4);
     dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
+    dev_free(c, (void **)&c->vsimple, N1);
     free_moves(c);
+    free_batch(c);
     c->graphs = c->mapped = c->diag_valid = false;
     c->gain_cols = 0;
     c->sweep_kind = 0;
@@ -660,6 +662,16 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = upload(c, &c->g2iptr, g2iptr))) goto fail;
     if ((rc = upload(c, &c->g2o, g2o))) goto fail;
     if ((rc = upload(c, &c->g2i, g2i))) goto fail;
+    {
+        // vertices on which every delta kind equals the exact score change:
+        // no self-loop line and no repeated (u,v) line touches them
+        std::vector<uint8_t> simple(N1, 1);
+        for (int64_t i = 0; i < e1; i++)
+            if (s1[i] == d1[i]) simple[s1[i]] = 0;
+        for (int v = 0; v < N1; v++)
+            if (ddptr[v + 1] - ddptr[v] != cptr[v + 1] - cptr[v]) simple[v] = 0;
+        if ((rc = upload(c, &c->vsimple, simple))) goto fail;
+    }
     if ((rc = dev_alloc(c, (void **)&c->pinv, (size_t)N1 * 4))) goto fail;
     if ((rc = dev_alloc(c, (void **)&c->g2oy, (size_t)c->ng2 * 4))) goto fail;
     if ((rc = dev_alloc(c, (void **)&c->g2iy, (size_t)c->ng2 * 4))) goto fail;

@@ -83,13 +83,17 @@ struct bm_ctx {
     int64_t ldx = 0;
     int gain_cols = 0;        // all-pairs slab width per lane (1, 2, 4), 0 = not tuned yet
     float gain_ms[3] = {0, 0, 0};  // autotuner timings of widths 1, 2, 4
+    bool sparse_attr = false;      // dynamic-LDS attribute of the sparse gain kernel set
+    int rot3_lanes = 16;           // lanes per triple of the 3-vertex kernel (8, 16, 32)
+    int permset_threads = 256;     // workgroup size of the k-set kernel (64, 256)
+    uint8_t *vsimple = nullptr;    // 1: no self-loop and no repeated line touches the vertex
     unsigned long long *keys = nullptr;  // N1 + 8 reduction slots
     void *stage = nullptr;               // move-list staging (lazy, grows)
     size_t stage_bytes = 0;
     int64_t dev_bytes = 0;
 
     // batched refinement (bm_batch_*)
-    struct Batch *batch = nullptr;
+    struct bm_batch_state *batch = nullptr;
 
     // device-resident candidate list (bm_moves_*)
     int32_t mv_k = 0;
@@ -137,7 +141,8 @@ int launch_check(const char *what);
 // launchers shared across translation units (device pointers, context stream)
 int launch_eval_pairs(bm_ctx *c, const int32_t *dA, const int32_t *dB, int64_t m, int32_t *dD);
 int launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP);
-int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv);
+// mult = 2: Delta_P as the reference (intra-set lines twice); 1: the exact score change
+int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult = 2);
 int free_moves(bm_ctx *c);
 int free_batch(bm_ctx *c);
 int launch_rebuild_p(bm_ctx *c);        // P and P^T from W2 and the device mapping
@@ -146,6 +151,15 @@ int launch_score(bm_ctx *c, unsigned long long *out);
 int launch_refresh_g2y(bm_ctx *c);      // G2 lists through p^-1
 int sweep_prepare(bm_ctx *c);           // allocations and kernel choice of the all-pairs sweep
 int sweep_enqueue(bm_ctx *c, int32_t *dmat);  // row-best keys into keys[0..n], global best into keys[n+1]
+
+// batched refinement, split for the serving loop: start a run, enqueue a chunk
+// of steps (returns at once), poll / wait for it, read the counters, finish
+int refine_configure(bm_ctx *c, const bm_refine_params *p);
+int refine_start(bm_ctx *c);
+int refine_enqueue(bm_ctx *c, int steps);
+int refine_poll(bm_ctx *c, bool wait);
+void refine_stats(bm_ctx *c, bm_refine_stats *out);
+int refine_finish(bm_ctx *c);
 
 // device-side row/column permutation of P and PT for a vertex set
 int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of);

@@ -1,12 +1,14 @@
 // Kernels for the 3-vertex, k-set and hill-climb move kinds (gfx950).
 //
-//   k_rot3     : Delta_G3 of triple.cu:85-214 for a triple list, 16 lanes per
-//                triple.  Each neighbour entry of m_k loads the three values
-//                Q[r][m_0..2] once and feeds all five permutations.
+//   k_rot3     : Delta_G3 of triple.cu:85-214 for a triple list, kLanes
+//                (8, 16 or 32) lanes per triple.  Each neighbour entry of m_k
+//                loads the three values Q[r][m_0..2] once and feeds all five
+//                permutations.
 //   k_permset  : Delta_P of graph-alignment-permset-parallel.c:304-339 for
-//                every permutation of a k-set, one workgroup per set: an
-//                LDS table C[i][j] (vertex i takes j's image) plus the
-//                intra-set lines (counted twice, as the reference does), then
+//                every permutation of a k-set, one workgroup (64 or 256
+//                threads) per set: an LDS table C[i][j] (vertex i takes j's
+//                image) plus the intra-set lines (counted twice, as the
+//                reference does, or once for the exact score change), then
 //                k! evaluations in next_permutation order, strict-> arg-max.
 //   k_hillclimb: Delta_A of refine-matching-hillclimb.c:244-296.
 #include <limits.h>
@@ -24,13 +26,14 @@ __device__ __forceinline__ int qv(const int16_t *__restrict__ Q, int row, int ld
 // permutation table of triple.cu:85-92: new image of m_k = old image of m_tab[p][k]
 __constant__ int kG3Tab[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
+template <int kLanes>
 __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int64_t count, int mode,
                                               int32_t *__restrict__ best_d, int32_t *__restrict__ best_p,
                                               const int32_t *__restrict__ ddptr, const int32_t *__restrict__ ddr,
                                               const int32_t *__restrict__ ddw, const int16_t *__restrict__ Q, int N1,
                                               int ld) {
-    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-    const int gl = threadIdx.x & 15;
+    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+    const int gl = threadIdx.x & (kLanes - 1);
     int s[6] = {0, 0, 0, 0, 0, 0};
     int m[3] = {0, 0, 0};
     if (t < count) {
@@ -39,7 +42,7 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
         m[2] = T[3 * t + 2];
         for (int k = 0; k < 3; k++) {
             const int v = m[k];
-            for (int e = ddptr[v] + gl; e < ddptr[v + 1]; e += 16) {
+            for (int e = ddptr[v] + gl; e < ddptr[v + 1]; e += kLanes) {
                 const int r = ddr[e];
                 const int w = (int)(int16_t)ddw[e];  // the dense G1 cell
                 if (w <= 0) continue;
@@ -68,12 +71,9 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
         }
     }
 #pragma unroll
-    for (int p = 1; p < 6; p++) {
-        s[p] += __shfl_xor(s[p], 8, 16);
-        s[p] += __shfl_xor(s[p], 4, 16);
-        s[p] += __shfl_xor(s[p], 2, 16);
-        s[p] += __shfl_xor(s[p], 1, 16);
-    }
+    for (int p = 1; p < 6; p++)
+#pragma unroll
+        for (int o = kLanes / 2; o > 0; o >>= 1) s[p] += __shfl_xor(s[p], o, kLanes);
     if (t < count && gl == 0) {
         int bd = 0, bp = 0;
         for (int p = 1; p < 6; p++) {
@@ -107,11 +107,13 @@ __device__ __forceinline__ bool next_perm(int *a, int k) {
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ sets, int64_t count, int k,
-                                                 int32_t *__restrict__ best_d, int32_t *__restrict__ best_rank,
-                                                 const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
-                                                 const int32_t *__restrict__ cw, const int16_t *__restrict__ Q, int N1,
-                                                 int ld, int *__restrict__ overflow) {
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict__ sets, int64_t count, int k,
+                                                      int32_t *__restrict__ best_d, int32_t *__restrict__ best_rank,
+                                                      const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
+                                                      const int32_t *__restrict__ cw, const int16_t *__restrict__ Q,
+                                                      int N1, int ld, int *__restrict__ overflow, int mult) {
+    constexpr int kWaves = kThreads / 64;
     __shared__ int S[8];
     __shared__ int C[8][8];
     __shared__ int PS[8][8];
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ set
     __shared__ int iw[kMaxIntra];
     __shared__ unsigned char ix[kMaxIntra], il[kMaxIntra];
     __shared__ int lbase;
-    __shared__ unsigned long long red[4];
+    __shared__ unsigned long long red[kWaves];
     const int64_t set = blockIdx.x;
     const int tid = threadIdx.x;
     if (tid < k) S[tid] = sets[set * k + tid];
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ set
     // table C over all entries of the k lists
     for (int i = 0; i < k; i++) {
         const int v = S[i];
-        for (int e = cptr[v] + tid; e < cptr[v + 1]; e += 256) {
+        for (int e = cptr[v] + tid; e < cptr[v + 1]; e += kThreads) {
             const int r = cr[e], w = cw[e];
             const bool out = r >= N1;
             const int nb = out ? r - N1 : r;
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ set
                 if (S[q] == nb) l = q;
             if (l >= 0) {
                 // intra line: both of its list entries change with sigma;
-                // keep it once (from the out entry), doubled when evaluated
+                // keep it once (from the out entry), weighted by mult when evaluated
                 if (out) {
                     const int slot = atomicAdd(&ni, 1);
                     if (slot < kMaxIntra) {
@@ -161,12 +163,12 @@ __global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ set
     __syncthreads();
     const int nint = min(ni, kMaxIntra);
     if (ni > kMaxIntra && tid == 0) atomicOr(overflow, 1);
-    if (tid < nint) atomicAdd(&lbase, -2 * min(iw[tid], PS[ix[tid]][il[tid]]));
+    for (int q = tid; q < nint; q += kThreads) atomicAdd(&lbase, -mult * min(iw[q], PS[ix[q]][il[q]]));
     __syncthreads();
     // k! permutations, contiguous rank ranges per thread
     int total = 1;
     for (int i = 2; i <= k; i++) total *= i;
-    const int per = (total + 255) / 256;
+    const int per = (total + kThreads - 1) / kThreads;
     const int r0 = tid * per, r1 = min(total, r0 + per);
     int bestd = 0, bestr = -1;
     if (r0 < r1) {
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ set
         for (int rank = r0; rank < r1; rank++) {
             int d = lbase;
             for (int i = 0; i < k; i++) d += C[i][perm[i]];
-            for (int q = 0; q < nint; q++) d += 2 * min(iw[q], PS[perm[ix[q]]][perm[il[q]]]);
+            for (int q = 0; q < nint; q++) d += mult * min(iw[q], PS[perm[ix[q]]][perm[il[q]]]);
             if (d > bestd) {
                 bestd = d;
                 bestr = rank;
@@ -201,10 +203,14 @@ __global__ __launch_bounds__(256) void k_permset(const int32_t *__restrict__ set
         unsigned long long t2 = __shfl_xor(key, o);
         key = t2 > key ? t2 : key;
     }
-    if ((tid & 63) == 0) red[tid >> 6] = key;
-    __syncthreads();
+    if (kWaves > 1) {
+        if ((tid & 63) == 0) red[tid >> 6] = key;
+        __syncthreads();
+        if (tid == 0)
+            for (int q = 1; q < kWaves; q++) key = max(key, red[q]);
+    }
     if (tid == 0) {
-        unsigned long long kk = max(max(red[0], red[1]), max(red[2], red[3]));
+        const unsigned long long kk = key;
         if (kk) {
             best_d[set] = (int32_t)((uint32_t)(kk >> 32) ^ 0x80000000u);
             best_rank[set] = (int32_t)(0xFFFFFFFFu - (uint32_t)(kk & 0xFFFFFFFFu));
@@ -248,15 +254,29 @@ __global__ __launch_bounds__(256) void k_hillclimb(const int32_t *__restrict__ A
 
 int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP) {
     if (m <= 0) return BM_OK;
-    BM_LAUNCH(c, "rot3", hipLaunchKernelGGL(k_rot3, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0, c->stream,
-                                            dT, m, mode, dD, dP, c->ddptr, c->ddr, c->ddw, c->Q, c->N1, c->ld));
+    const int lanes = c->rot3_lanes;
+    const unsigned blocks = (unsigned)((m * lanes + 255) / 256);
+#define BM_ROT3(L)                                                                                                 \
+    BM_LAUNCH(c, "rot3", hipLaunchKernelGGL(k_rot3<L>, dim3(blocks), dim3(256), 0, c->stream, dT, m, mode, dD, dP, \
+                                            c->ddptr, c->ddr, c->ddw, c->Q, c->N1, c->ld))
+    if (lanes == 8) BM_ROT3(8);
+    else if (lanes == 32) BM_ROT3(32);
+    else if (lanes == 16) BM_ROT3(16);
+    else return set_error(BM_EINVAL, "rot3: %d lanes per triple", lanes);
+#undef BM_ROT3
     return BM_OK;
 }
 
-int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv) {
+int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult) {
     if (m <= 0) return BM_OK;
-    BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset, dim3((unsigned)m), dim3(256), 0, c->stream, dS, m, k, dD, dR,
-                                               c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv));
+    if (c->permset_threads == 64)
+        BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset<64>, dim3((unsigned)m), dim3(64), 0, c->stream, dS, m, k,
+                                                   dD, dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv, mult));
+    else if (c->permset_threads == 256)
+        BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset<256>, dim3((unsigned)m), dim3(256), 0, c->stream, dS, m,
+                                                   k, dD, dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv, mult));
+    else
+        return set_error(BM_EINVAL, "permset: %d threads per set", c->permset_threads);
     return BM_OK;
 }
 

@@ -1,0 +1,832 @@
+// Batched refinement: many exact improving moves per step, on the device.
+//
+// One step (all kernels on the context stream, captured once in a hipGraph
+// and replayed):
+//   1. candidates: the row-best swap of every row a (b > a) from the
+//      all-pairs Delta_B sweep; `triples` 3-vertex moves drawn from 2-hop
+//      G1 neighbourhoods, scored with Delta_G3 over p = 1..5; `sets` k-sets
+//      grown along G1 lines, scored over all k! permutations with the intra
+//      lines counted once (the exact score change);
+//   2. keys: an improving candidate whose vertices all lie in [1, n] and are
+//      "simple" (no self-loop, no repeated line touches them; there every
+//      delta kind equals the exact change of S_opt) gets the priority key
+//      delta << 32 | ~index;
+//   3. selection, `rounds` times: each live candidate claims its vertices
+//      (atomicMax of its key), then wins if no vertex of its closed G1
+//      neighbourhood N[V] was claimed by a larger key; winners block N[V],
+//      which retires every candidate that touches it.  Two winners are never
+//      equal or G1-adjacent, so the selected deltas add up exactly;
+//   4. apply: every winner permutes the images of its vertices (and p^-1);
+//      P and P^T are rebuilt from W2 and the new mapping, the G2 lists
+//      re-expressed through p^-1;
+//   5. verify: S_opt is recomputed from scratch on the device and must equal
+//      the previous score plus the sum of the applied deltas (and rise when a
+//      move was applied); the first failing step is recorded.
+// No host round trip inside a step.  Selection and results do not depend on
+// timing: the autotuned choices (gain kernel, lanes per triple, threads per
+// set) only pick among variants that compute identical integers.
+//
+// Reference analogues: the per-step evaluate / host arg-max / apply loops of
+// triple.cu:1082-1126, quick.cu:1175-1217 and
+// graph-alignment-permset-parallel.c:651-730, which apply one move per step.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "bm_internal.h"
+
+using namespace bm;
+
+namespace {
+
+constexpr int kHist = 4096;  // per-step history ring
+
+struct RefineState {
+    unsigned long long step;   // steps completed in this run
+    long long score;           // exact S_opt after the last step
+    long long moves;           // moves applied in this run
+    long long kind_moves[3];   // of which pairs, triples, sets
+    long long gain_total;      // sum of applied deltas in this run
+    unsigned long long gain;   // this step
+    unsigned int applied;      // this step
+    unsigned int nwin;         // winner-list length this step
+    unsigned int bad_step;     // 1 + first step whose exact check failed (0: none)
+    unsigned int log_count;
+    unsigned long long score_slot;
+    long long idle;            // consecutive steps without a move
+    long long initial_score;
+};
+
+struct MoveRec {
+    int32_t step, kind, k, delta;
+    int32_t v[8];
+    int32_t img[8];  // images after the move
+};
+
+// what the candidate index space holds: [0, N1) rows, then triples, then sets
+struct CandView {
+    const unsigned long long *rowkeys;
+    const int32_t *tri;
+    const int32_t *sets;
+    const int32_t *tri_d, *tri_p, *set_d, *set_r;
+    int N1, n, T, S, k, pairs;
+};
+
+__constant__ int kTab3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ unsigned long long rnd(unsigned long long seed, unsigned long long step, long long i,
+                                                  int j) {
+    return mix64(seed ^ mix64(step * 0x100000001B3ull + 0x51ED27ull) ^ mix64((unsigned long long)i * 16 + j));
+}
+
+__device__ __forceinline__ int nbr(const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr, int N1, int x,
+                                   unsigned long long h) {
+    const int deg = cptr[x + 1] - cptr[x];
+    if (deg == 0) return -1;
+    const int r = cr[cptr[x] + (int)(h % (unsigned long long)deg)];
+    return r >= N1 ? r - N1 : r;
+}
+
+// vertices, delta and kind of candidate i; k = 0 when the slot is empty
+__device__ __forceinline__ int cand_get(const CandView &cv, long long i, int *v, int &delta, int &kind) {
+    if (i < cv.N1) {
+        kind = 0;
+        if (!cv.pairs || i == 0) return 0;
+        const unsigned long long rk = cv.rowkeys[i];
+        if (!rk) return 0;
+        delta = (int)((unsigned)(rk >> 32) ^ 0x80000000u);
+        v[0] = (int)i;
+        v[1] = (int)(0xFFFFFFFFu - (unsigned)rk);
+        return 2;
+    }
+    i -= cv.N1;
+    if (i < cv.T) {
+        kind = 1;
+        delta = cv.tri_d[i];
+        v[0] = cv.tri[3 * i];
+        v[1] = cv.tri[3 * i + 1];
+        v[2] = cv.tri[3 * i + 2];
+        return 3;
+    }
+    i -= cv.T;
+    kind = 2;
+    delta = cv.set_d[i];
+    for (int q = 0; q < cv.k; q++) v[q] = cv.sets[(long long)cv.k * i + q];
+    return cv.k;
+}
+
+__global__ void k_step_begin(RefineState *st) {
+    if (threadIdx.x == 0) {
+        st->gain = 0;
+        st->applied = 0;
+        st->nwin = 0;
+        st->score_slot = 0;
+    }
+}
+
+__global__ void k_fill32(uint32_t *__restrict__ p, long long count, uint32_t v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+// 3-vertex candidates: u uniform, v a G1 neighbour of u, w a neighbour of v or u
+__global__ void k_gen_triples(int32_t *__restrict__ tri, int T, const RefineState *__restrict__ st,
+                              unsigned long long seed, const int32_t *__restrict__ cptr,
+                              const int32_t *__restrict__ cr, int N1, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    const unsigned long long step = st->step;
+    const unsigned long long h0 = rnd(seed, step, i, 0), h1 = rnd(seed, step, i, 1), h2 = rnd(seed, step, i, 2);
+    const int u = 1 + (int)(h0 % (unsigned long long)n);
+    int v = nbr(cptr, cr, N1, u, h1);
+    if (v < 0) v = 1 + (int)(h1 % (unsigned long long)n);
+    const int x = (h2 & 1) ? v : u;
+    int w = nbr(cptr, cr, N1, x, h2 >> 1);
+    if (w < 0) w = 1 + (int)((h2 >> 1) % (unsigned long long)n);
+    tri[3 * i] = u;
+    tri[3 * i + 1] = v;
+    tri[3 * i + 2] = w;
+}
+
+// k-set candidates grown along G1 lines from a uniform seed vertex
+__global__ void k_gen_sets(int32_t *__restrict__ sets, int S, int k, const RefineState *__restrict__ st,
+                           unsigned long long seed, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
+                           int N1, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    const unsigned long long step = st->step;
+    int s[8];
+    int j = 0, draw = 0;
+    s[j++] = 1 + (int)(rnd(seed ^ 0xA5A5ull, step, i, draw++) % (unsigned long long)n);
+    for (int tries = 0; j < k && tries < 6 * k; tries++) {
+        const unsigned long long h = rnd(seed ^ 0xA5A5ull, step, i, draw++);
+        const int m = s[(int)(h % (unsigned long long)j)];
+        int c = tries < 4 * k ? nbr(cptr, cr, N1, m, h >> 8) : -1;
+        if (c < 0) c = 1 + (int)((h >> 8) % (unsigned long long)n);
+        bool fresh = c >= 1 && c <= n;
+        for (int q = 0; q < j && fresh; q++) fresh = s[q] != c;
+        if (fresh) s[j++] = c;
+    }
+    for (; j < k; j++) s[j] = s[0];  // left invalid: repeated vertex
+    for (int q = 0; q < k; q++) sets[(long long)k * i + q] = s[q];
+}
+
+__global__ void k_cand_keys(unsigned long long *__restrict__ ckey, long long M, CandView cv,
+                            const uint8_t *__restrict__ vsimple) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    int v[8], delta = 0, kind = 0;
+    const int k = cand_get(cv, i, v, delta, kind);
+    bool ok = k > 0 && delta > 0;
+    for (int q = 0; q < k && ok; q++) {
+        ok = v[q] >= 1 && v[q] <= cv.n && vsimple[v[q]];
+        for (int r = 0; r < q && ok; r++) ok = v[r] != v[q];
+    }
+    ckey[i] = ok ? ((unsigned long long)(unsigned)delta << 32) | (0xFFFFFFFFu - (unsigned)i) : 0ull;
+}
+
+__global__ void k_claim(unsigned long long *__restrict__ ckey, long long M, CandView cv,
+                        unsigned long long *__restrict__ vmax, const uint32_t *__restrict__ blocked) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const unsigned long long key = ckey[i];
+    if (!key) return;
+    int v[8], delta, kind;
+    const int k = cand_get(cv, i, v, delta, kind);
+    for (int q = 0; q < k; q++)
+        if (blocked[v[q]]) {
+            ckey[i] = 0;  // touches the closed neighbourhood of an earlier winner
+            return;
+        }
+    for (int q = 0; q < k; q++) atomicMax(&vmax[v[q]], key);
+}
+
+// one wave per candidate: wins when no vertex of N[V] carries a larger claim
+__global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__ ckey, long long M, CandView cv,
+                                                const unsigned long long *__restrict__ vmax,
+                                                uint32_t *__restrict__ blocked, int32_t *__restrict__ win,
+                                                RefineState *__restrict__ st, const int32_t *__restrict__ cptr,
+                                                const int32_t *__restrict__ cr, int N1) {
+    const long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= M) return;
+    const unsigned long long key = ckey[i];
+    if (!key) return;
+    int v[8], delta, kind;
+    const int k = cand_get(cv, i, v, delta, kind);
+    bool lose = false;
+    for (int q = 0; q < k; q++) {
+        const int x = v[q];
+        if (lane == 0 && vmax[x] > key) lose = true;
+        for (int e = cptr[x] + lane; e < cptr[x + 1]; e += 64) {
+            const int r = cr[e];
+            const int y = r >= N1 ? r - N1 : r;
+            if (vmax[y] > key) lose = true;
+        }
+    }
+    if (__ballot(lose)) return;
+    for (int q = 0; q < k; q++) {
+        const int x = v[q];
+        if (lane == 0) blocked[x] = 1;
+        for (int e = cptr[x] + lane; e < cptr[x + 1]; e += 64) {
+            const int r = cr[e];
+            blocked[r >= N1 ? r - N1 : r] = 1;
+        }
+    }
+    if (lane == 0) {
+        ckey[i] = 0;
+        win[atomicAdd(&st->nwin, 1u)] = (int32_t)i;
+    }
+}
+
+__device__ __forceinline__ void unrank(int k, int rank, int *perm) {
+    int pool[8];
+    for (int q = 0; q < k; q++) pool[q] = q;
+    int left = k;
+    for (int q = 0; q < k; q++) {
+        int f = 1;
+        for (int t = 2; t <= k - 1 - q; t++) f *= t;
+        const int idx = rank / f;
+        rank -= idx * f;
+        perm[q] = pool[idx];
+        for (int t = idx; t < left - 1; t++) pool[t] = pool[t + 1];
+        left--;
+    }
+}
+
+// thread per winner: new p(v[q]) = old p(v[src[q]]); the sets are disjoint
+__global__ void k_apply(const int32_t *__restrict__ win, int max_win, CandView cv, int32_t *__restrict__ map,
+                        int32_t *__restrict__ pinv, int with_pinv, RefineState *__restrict__ st,
+                        MoveRec *__restrict__ log, int log_cap) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= max_win || s >= (int)st->nwin) return;
+    const long long i = win[s];
+    int v[8], src[8], old[8], delta = 0, kind = 0;
+    const int k = cand_get(cv, i, v, delta, kind);
+    if (kind == 0) {
+        src[0] = 1;
+        src[1] = 0;
+    } else if (kind == 1) {
+        const int p = cv.tri_p[i - cv.N1];
+        for (int q = 0; q < 3; q++) src[q] = kTab3[p][q];
+    } else {
+        unrank(k, cv.set_r[i - cv.N1 - cv.T], src);
+    }
+    for (int q = 0; q < k; q++) old[q] = map[v[src[q]]];
+    for (int q = 0; q < k; q++) {
+        map[v[q]] = old[q];
+        if (with_pinv && old[q] != 0) pinv[old[q]] = v[q];
+    }
+    atomicAdd(&st->gain, (unsigned long long)(unsigned)delta);
+    atomicAdd(&st->applied, 1u);
+    atomicAdd((unsigned long long *)&st->kind_moves[kind], 1ull);
+    if (log_cap > 0) {
+        const unsigned slot = atomicAdd(&st->log_count, 1u);
+        if (slot < (unsigned)log_cap) {
+            MoveRec &r = log[slot];
+            r.step = (int32_t)st->step;
+            r.kind = kind;
+            r.k = k;
+            r.delta = delta;
+            for (int q = 0; q < 8; q++) {
+                r.v[q] = q < k ? v[q] : 0;
+                r.img[q] = q < k ? old[q] : 0;
+            }
+        }
+    }
+}
+
+__global__ void k_step_end(RefineState *st, bm_refine_step *hist) {
+    if (threadIdx.x != 0) return;
+    const long long now = (long long)st->score_slot;
+    const long long gain = (long long)st->gain;
+    const bool ok = now == st->score + gain && (st->applied == 0 || now > st->score);
+    bm_refine_step &h = hist[st->step % kHist];
+    h.score = now;
+    h.gain = gain;
+    h.moves = (int32_t)st->applied;
+    h.ok = ok ? 1 : 0;
+    if (!ok && !st->bad_step) st->bad_step = (unsigned)(st->step + 1);
+    st->score = now;
+    st->moves += st->applied;
+    st->gain_total += gain;
+    st->idle = st->applied ? 0 : st->idle + 1;
+    st->step++;
+}
+
+__global__ void k_state_init(RefineState *st) {
+    if (threadIdx.x != 0) return;
+    const long long s = (long long)st->score_slot;
+    memset(st, 0, sizeof(RefineState));
+    st->score = s;
+    st->initial_score = s;
+}
+
+// process-wide choices of the autotuner, per class of instance
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, int> g_tune;
+
+int log2i(long long x) {
+    int r = 0;
+    while ((1ll << r) < x) r++;
+    return r;
+}
+
+}  // namespace
+
+struct bm_batch_state {
+    bm_refine_params prm{};
+    bool configured = false;
+    bool captured_sparse = false;
+    int32_t *tri = nullptr, *tri_d = nullptr, *tri_p = nullptr;
+    int32_t *sets = nullptr, *set_d = nullptr, *set_r = nullptr;
+    int *ovf = nullptr;
+    unsigned long long *ckey = nullptr, *vmax = nullptr;
+    uint32_t *blocked = nullptr;
+    int32_t *win = nullptr;
+    RefineState *st = nullptr;
+    RefineState *hst = nullptr;  // pinned copy
+    bm_refine_step *hist = nullptr;
+    MoveRec *log = nullptr;
+    int log_cap = 0;
+    long long M = 0;
+    size_t bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool in_flight = false;
+    int64_t run_steps = 0;
+};
+
+static CandView cand_view(bm_ctx *c) {
+    bm_batch_state *b = c->batch;
+    CandView cv;
+    cv.rowkeys = c->keys;
+    cv.tri = b->tri;
+    cv.sets = b->sets;
+    cv.tri_d = b->tri_d;
+    cv.tri_p = b->tri_p;
+    cv.set_d = b->set_d;
+    cv.set_r = b->set_r;
+    cv.N1 = c->N1;
+    cv.n = c->n;
+    cv.T = b->prm.triples;
+    cv.S = b->prm.sets;
+    cv.k = b->prm.set_k;
+    cv.pairs = b->prm.pairs;
+    return cv;
+}
+
+static void drop_graph(bm_batch_state *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int bm::free_batch(bm_ctx *c) {
+    bm_batch_state *b = c->batch;
+    if (!b) return BM_OK;
+    drop_graph(b);
+    auto fr = [&](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
+    fr(b->ckey), fr(b->vmax), fr(b->blocked), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
+    if (b->hst) (void)hipHostFree(b->hst);
+    if (b->done) (void)hipEventDestroy(b->done);
+    c->dev_bytes -= (int64_t)b->bytes;
+    delete b;
+    c->batch = nullptr;
+    return BM_OK;
+}
+
+static int balloc(bm_ctx *c, void **p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(BM_ENOMEM, "device allocation of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+    }
+    c->batch->bytes += bytes;
+    c->dev_bytes += (int64_t)bytes;
+    return BM_OK;
+}
+
+// Everything one step enqueues; capturable (kernels only).
+static int enqueue_step(bm_ctx *c) {
+    bm_batch_state *b = c->batch;
+    const bm_refine_params &p = b->prm;
+    const int N1 = c->N1;
+    const CandView cv = cand_view(c);
+    int rc;
+    BM_LAUNCH(c, "refine_begin", hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, c->stream, b->st));
+    const long long vm_words = (long long)p.rounds * N1 * 2;
+    BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((unsigned)((vm_words + 255) / 256)), dim3(256), 0,
+                                            c->stream, (uint32_t *)b->vmax, vm_words, 0u));
+    BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((N1 + 255) / 256), dim3(256), 0, c->stream, b->blocked,
+                                            (long long)N1, 0u));
+    if (p.pairs) {
+        if ((rc = sweep_enqueue(c, nullptr))) return rc;
+    }
+    if (p.triples > 0) {
+        BM_LAUNCH(c, "gen_triples",
+                  hipLaunchKernelGGL(k_gen_triples, dim3((p.triples + 255) / 256), dim3(256), 0, c->stream, b->tri,
+                                     p.triples, b->st, (unsigned long long)p.seed, c->cptr, c->cr, N1, c->n));
+        if ((rc = launch_rot3(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d, b->tri_p))) return rc;
+    }
+    if (p.sets > 0) {
+        BM_LAUNCH(c, "gen_sets",
+                  hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets, p.sets,
+                                     p.set_k, b->st, (unsigned long long)p.seed, c->cptr, c->cr, N1, c->n));
+        if ((rc = launch_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r, b->ovf, 1))) return rc;
+    }
+    const unsigned mb = (unsigned)((b->M + 255) / 256);
+    BM_LAUNCH(c, "cand_keys",
+              hipLaunchKernelGGL(k_cand_keys, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, c->vsimple));
+    for (int r = 0; r < p.rounds; r++) {
+        unsigned long long *vm = b->vmax + (size_t)r * N1;
+        BM_LAUNCH(c, "claim", hipLaunchKernelGGL(k_claim, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, vm,
+                                                 b->blocked));
+        BM_LAUNCH(c, "decide",
+                  hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->M + 3) / 4)), dim3(256), 0, c->stream, b->ckey,
+                                     b->M, cv, vm, b->blocked, b->win, b->st, c->cptr, c->cr, N1));
+    }
+    const int max_win = N1 / 2 + 1;
+    BM_LAUNCH(c, "apply", hipLaunchKernelGGL(k_apply, dim3((max_win + 255) / 256), dim3(256), 0, c->stream, b->win,
+                                             max_win, cv, c->map, c->pinv, c->sparse_ok ? 1 : 0, b->st, b->log,
+                                             b->log_cap));
+    if ((rc = launch_rebuild_p(c))) return rc;
+    if (c->sparse_ok) {
+        if ((rc = launch_refresh_g2y(c))) return rc;
+    }
+    if ((rc = launch_score(c, &b->st->score_slot))) return rc;
+    BM_LAUNCH(c, "refine_end", hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->stream, b->st, b->hist));
+    return BM_OK;
+}
+
+static int capture(bm_ctx *c) {
+    bm_batch_state *b = c->batch;
+    drop_graph(b);
+    const bool timing = c->timing, dirty = c->g2y_dirty;
+    c->timing = false;     // no event records inside the graph
+    c->g2y_dirty = false;  // each step ends with the refresh; refine_start covers the first
+    BM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = timing;
+    c->g2y_dirty = dirty;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(b);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    b->captured_sparse = c->sparse_ok;
+    return BM_OK;
+}
+
+// time `reps` launches of the current variant (eager, events on the stream)
+template <class F>
+static int time_variant(bm_ctx *c, F launch, float *ms) {
+    hipEvent_t e0, e1;
+    BM_HIP(hipEventCreate(&e0));
+    BM_HIP(hipEventCreate(&e1));
+    int rc = launch();  // warm
+    if (!rc) {
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < 3 && !rc; r++) rc = launch();
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(ms, e0, e1);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+static int env_int(const char *name, int dflt) {
+    const char *s = getenv(name);
+    return s && *s ? atoi(s) : dflt;
+}
+
+// Picks lanes per triple and threads per set: timed on this instance's own
+// generated candidates, cached per class (kind, log2 n, log2 mean list length, k).
+static int autotune(bm_ctx *c) {
+    bm_batch_state *b = c->batch;
+    const bm_refine_params &p = b->prm;
+    const int dl = log2i(std::max<long long>(1, c->ncomb / std::max(1, c->n)));
+    const int nl = log2i(c->n);
+    int rc;
+    if (p.triples > 0) {
+        const int forced = env_int("BM_ROT3_LANES", 0);
+        auto key = std::make_tuple(1, nl, dl, 3);
+        int best = forced;
+        if (!best) {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) best = it->second;
+        }
+        if (!best) {
+            BM_LAUNCH(c, "gen_triples",
+                      hipLaunchKernelGGL(k_gen_triples, dim3((p.triples + 255) / 256), dim3(256), 0, c->stream,
+                                         b->tri, p.triples, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1,
+                                         c->n));
+            float bms = 1e30f;
+            for (int lanes : {8, 16, 32}) {
+                c->rot3_lanes = lanes;
+                float ms = 0;
+                if ((rc = time_variant(c, [&] { return launch_rot3(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d,
+                                                                   b->tri_p); }, &ms)))
+                    return rc;
+                if (ms < bms) {
+                    bms = ms;
+                    best = lanes;
+                }
+            }
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            g_tune[key] = best;
+        }
+        c->rot3_lanes = best;
+    }
+    if (p.sets > 0) {
+        const int forced = env_int("BM_PERMSET_THREADS", 0);
+        auto key = std::make_tuple(2, nl, dl, p.set_k);
+        int best = forced;
+        if (!best) {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) best = it->second;
+        }
+        if (!best) {
+            BM_LAUNCH(c, "gen_sets",
+                      hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets,
+                                         p.sets, p.set_k, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1,
+                                         c->n));
+            float bms = 1e30f;
+            for (int th : {64, 256}) {
+                c->permset_threads = th;
+                float ms = 0;
+                if ((rc = time_variant(c, [&] { return launch_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r,
+                                                                      b->ovf, 1); }, &ms)))
+                    return rc;
+                if (ms < bms) {
+                    bms = ms;
+                    best = th;
+                }
+            }
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            g_tune[key] = best;
+        }
+        c->permset_threads = best;
+    }
+    BM_HIP(hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+namespace bm {
+
+int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
+    if (!c || !prm) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    bm_refine_params p = *prm;
+    if (p.triples < 0 || p.sets < 0 || p.triples > (1 << 24) || p.sets > (1 << 20))
+        return set_error(BM_EINVAL, "triples %d / sets %d outside [0, 2^24] / [0, 2^20]", p.triples, p.sets);
+    if (p.sets > 0 && (p.set_k < 2 || p.set_k > 8)) return set_error(BM_EINVAL, "set_k=%d outside [2,8]", p.set_k);
+    if (p.rounds < 1 || p.rounds > 8) return set_error(BM_EINVAL, "rounds=%d outside [1,8]", p.rounds);
+    if (p.steps_per_check < 1) return set_error(BM_EINVAL, "steps_per_check=%d < 1", p.steps_per_check);
+    if (!p.pairs && !p.triples && !p.sets) return set_error(BM_EINVAL, "no move kind enabled");
+    if (c->n < 3) return set_error(BM_ERANGE, "n=%d too small for batched refinement", c->n);
+    if (p.sets == 0) p.set_k = 0;
+    BM_HIP(hipSetDevice(c->device));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    free_batch(c);
+    c->batch = new bm_batch_state();
+    bm_batch_state *b = c->batch;
+    b->prm = p;
+    const int N1 = c->N1;
+    b->M = (long long)N1 + p.triples + p.sets;
+    int rc;
+    auto fail = [&](int r) {
+        free_batch(c);
+        return r;
+    };
+    if ((rc = balloc(c, (void **)&b->tri, (size_t)p.triples * 12))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->tri_d, (size_t)p.triples * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->tri_p, (size_t)p.triples * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->sets, (size_t)p.sets * 4 * std::max(p.set_k, 1)))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->set_d, (size_t)p.sets * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->set_r, (size_t)p.sets * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->ovf, 64))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->ckey, (size_t)b->M * 8))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->vmax, (size_t)p.rounds * N1 * 8))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->blocked, (size_t)N1 * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->win, (size_t)(N1 / 2 + 1) * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->st, sizeof(RefineState)))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->hist, sizeof(bm_refine_step) * kHist))) return fail(rc);
+    if (hipHostMalloc((void **)&b->hst, sizeof(RefineState), hipHostMallocDefault) != hipSuccess)
+        return fail(set_error(BM_ENOMEM, "pinned host allocation failed"));
+    if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
+        return fail(set_error(BM_EHIP, "hipEventCreate failed"));
+    if (hipMemsetAsync(b->st, 0, sizeof(RefineState), c->stream) != hipSuccess ||
+        hipMemsetAsync(b->ovf, 0, 64, c->stream) != hipSuccess)
+        return fail(set_error(BM_EHIP, "hipMemsetAsync failed"));
+    if (p.pairs && (rc = sweep_prepare(c))) return fail(rc);
+    if (c->sparse_ok && (rc = launch_refresh_g2y(c))) return fail(rc);
+    if ((rc = autotune(c))) return fail(rc);
+    if (p.pairs && c->sweep_kind == 2 && c->sparse_ok && !c->sparse_attr) {
+        // first sparse launch outside the capture sets the kernel's LDS attribute
+        if ((rc = sweep_enqueue(c, nullptr))) return fail(rc);
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(BM_EHIP, "configure sync failed"));
+    if ((rc = capture(c))) return fail(rc);
+    b->configured = true;
+    return BM_OK;
+}
+
+int refine_set_log(bm_ctx *c, int32_t cap) {
+    bm_batch_state *b = c->batch;
+    if (b->log) {
+        (void)hipFree(b->log);
+        b->bytes -= (size_t)b->log_cap * sizeof(MoveRec);
+        c->dev_bytes -= (int64_t)b->log_cap * (int64_t)sizeof(MoveRec);
+        b->log = nullptr;
+    }
+    b->log_cap = 0;
+    int rc;
+    if (cap > 0) {
+        if ((rc = balloc(c, (void **)&b->log, (size_t)cap * sizeof(MoveRec)))) return rc;
+        b->log_cap = cap;
+    }
+    return capture(c);  // the log pointer is a kernel argument
+}
+
+// New run: exact score of the current mapping, counters zeroed.
+int refine_start(bm_ctx *c) {
+    bm_batch_state *b = c->batch;
+    if (b->captured_sparse != c->sparse_ok) {  // the mapping changed kind since the capture
+        int rc = capture(c);
+        if (rc) return rc;
+    }
+    if (c->sparse_ok && c->g2y_dirty) {
+        int rc = launch_refresh_g2y(c);
+        if (rc) return rc;
+    }
+    BM_HIP(hipMemsetAsync(&b->st->score_slot, 0, 8, c->stream));
+    int rc = launch_score(c, &b->st->score_slot);
+    if (rc) return rc;
+    BM_LAUNCH(c, "refine_init", hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, c->stream, b->st));
+    b->run_steps = 0;
+    b->in_flight = false;
+    return BM_OK;
+}
+
+// Enqueue `steps` replays and an async copy of the state; returns at once.
+int refine_enqueue(bm_ctx *c, int steps) {
+    bm_batch_state *b = c->batch;
+    for (int s = 0; s < steps; s++) BM_HIP(hipGraphLaunch(b->exec, c->stream));
+    BM_HIP(hipMemcpyAsync(b->hst, b->st, sizeof(RefineState), hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipEventRecord(b->done, c->stream));
+    b->in_flight = true;
+    b->run_steps += steps;
+    return BM_OK;
+}
+
+// 1 when the last enqueued chunk finished, 0 while it runs, < 0 on error
+int refine_poll(bm_ctx *c, bool wait) {
+    bm_batch_state *b = c->batch;
+    if (!b->in_flight) return 1;
+    hipError_t e = wait ? hipEventSynchronize(b->done) : hipEventQuery(b->done);
+    if (e == hipErrorNotReady) return 0;
+    if (e != hipSuccess) return hip_fail(e, "refine chunk");
+    b->in_flight = false;
+    return 1;
+}
+
+void refine_stats(bm_ctx *c, bm_refine_stats *out) {
+    const RefineState &s = *c->batch->hst;
+    memset(out, 0, sizeof *out);
+    out->steps = (int64_t)s.step;
+    out->moves = s.moves;
+    out->pair_moves = s.kind_moves[0];
+    out->triple_moves = s.kind_moves[1];
+    out->set_moves = s.kind_moves[2];
+    out->initial_score = s.initial_score;
+    out->final_score = s.score;
+    out->idle_steps = s.idle;
+    out->verified = s.bad_step == 0 ? 1 : 0;
+    out->bad_step = s.bad_step ? (int64_t)s.bad_step - 1 : -1;
+}
+
+// Copies the mapping back after a run (the device holds the truth).
+int refine_finish(bm_ctx *c) {
+    BM_HIP(hipMemcpyAsync(c->hmap.data(), c->map, (size_t)c->N1 * 4, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    if (c->sparse_ok) {
+        std::fill(c->hpinv.begin(), c->hpinv.end(), -1);
+        for (int v = 0; v < c->N1; v++)
+            if (c->hmap[v]) c->hpinv[c->hmap[v]] = v;
+    }
+    c->diag_valid = false;
+    c->g2y_dirty = !c->sparse_ok;
+    return BM_OK;
+}
+
+}  // namespace bm
+
+extern "C" int bm_refine_configure(bm_ctx *c, const bm_refine_params *p) { return refine_configure(c, p); }
+
+extern "C" int bm_refine_run(bm_ctx *c, int64_t max_steps, int64_t move_budget, int32_t patience,
+                             bm_refine_stats *stats) {
+    if (!c || !stats) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->batch || !c->batch->configured) return set_error(BM_ESTATE, "bm_refine_configure not called");
+    if (max_steps < 0 || patience < 1) return set_error(BM_EINVAL, "max_steps %lld / patience %d", (long long)max_steps,
+                                                        patience);
+    BM_HIP(hipSetDevice(c->device));
+    bm_batch_state *b = c->batch;
+    int rc = refine_start(c);
+    if (rc) return rc;
+    bm_refine_stats s;
+    memset(&s, 0, sizeof s);
+    int64_t done = 0;
+    for (;;) {
+        const int chunk = (int)std::min<int64_t>(b->prm.steps_per_check, max_steps - done);
+        if ((rc = refine_enqueue(c, chunk))) return rc;
+        if ((rc = refine_poll(c, true)) < 0) return rc;
+        done += chunk;
+        refine_stats(c, &s);
+        if (!s.verified) break;
+        if (done >= max_steps) break;
+        if (move_budget > 0 && s.moves >= move_budget) break;
+        if (s.idle_steps >= patience) {
+            s.converged = 1;
+            break;
+        }
+    }
+    if ((rc = refine_finish(c))) return rc;
+    *stats = s;
+    if (!s.verified)
+        return set_error(BM_ERANGE, "exact score check failed at step %lld", (long long)s.bad_step);
+    return BM_OK;
+}
+
+extern "C" int bm_refine_history(bm_ctx *c, int64_t first, int64_t count, bm_refine_step *out) {
+    if (!c || (count > 0 && !out)) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->batch || !c->batch->configured) return set_error(BM_ESTATE, "bm_refine_configure not called");
+    const int64_t steps = (int64_t)c->batch->hst->step;
+    if (first < 0 || count < 0 || first + count > steps || (steps - first) > kHist)
+        return set_error(BM_ERANGE, "steps [%lld, %lld) not held (last %d of %lld kept)", (long long)first,
+                         (long long)(first + count), kHist, (long long)steps);
+    std::vector<bm_refine_step> h(kHist);
+    BM_HIP(hipSetDevice(c->device));
+    BM_HIP(hipMemcpy(h.data(), c->batch->hist, sizeof(bm_refine_step) * kHist, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < count; i++) out[i] = h[(first + i) % kHist];
+    return BM_OK;
+}
+
+extern "C" int bm_refine_log(bm_ctx *c, int32_t capacity) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    if (!c->batch || !c->batch->configured) return set_error(BM_ESTATE, "bm_refine_configure not called");
+    if (capacity < 0) return set_error(BM_EINVAL, "capacity %d", capacity);
+    BM_HIP(hipSetDevice(c->device));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    return refine_set_log(c, capacity);
+}
+
+extern "C" int bm_refine_moves(bm_ctx *c, int64_t *count, int32_t *recs, int64_t capacity) {
+    if (!c || !count) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->batch || !c->batch->configured) return set_error(BM_ESTATE, "bm_refine_configure not called");
+    bm_batch_state *b = c->batch;
+    const int64_t held = std::min<int64_t>(b->hst->log_count, b->log_cap);
+    *count = b->hst->log_count;
+    if (recs && capacity > 0 && held > 0) {
+        BM_HIP(hipSetDevice(c->device));
+        BM_HIP(hipMemcpy(recs, b->log, (size_t)std::min(held, capacity) * sizeof(MoveRec), hipMemcpyDeviceToHost));
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_refine_config(bm_ctx *c, int32_t *rot3_lanes, int32_t *permset_threads) {
+    if (!c || !rot3_lanes || !permset_threads) return set_error(BM_EINVAL, "NULL argument");
+    *rot3_lanes = c->rot3_lanes;
+    *permset_threads = c->permset_threads;
+    return BM_OK;
+}

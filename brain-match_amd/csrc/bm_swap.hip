@@ -710,22 +710,26 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
     for (int i = tid; i < ldx; i += 1024) __builtin_nontemporal_store(i < N1 ? acc[i] + ca : 0, X + xidx(a, i, T));
 }
 
+// The largest LDS row the sparse kernel may use (bytes).
+constexpr size_t kSparseLdsMax = 144 * 1024;
+
 static int launch_gain_sparse(bm_ctx *c) {
     if (c->g2y_dirty) {
         int rc = launch_refresh_g2y(c);
         if (rc) return rc;
     }
     const size_t lds = (size_t)c->N1 * 4;
-    BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (!c->sparse_attr) {  // once per context, so that captured launches make no attribute calls
+        BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSparseLdsMax));
+        c->sparse_attr = true;
+    }
     BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse, dim3(c->N1), dim3(1024), lds, c->stream, c->cptr,
                                             (const int2 *)c->crw, c->cdw, c->map, c->g2optr, c->g2oy, c->g2iptr,
                                             c->g2iy, c->Q, c->ld, c->X, c->ldx, c->diag, c->N1));
     c->diag_valid = true;
     return BM_OK;
 }
-
-// The largest LDS row the sparse kernel may use (bytes).
-constexpr size_t kSparseLdsMax = 144 * 1024;
 
 // Chooses the all-pairs gain kernel on the first sweep of a graph pair: the
 // sparse kernel when its preconditions hold, timed against the best dense

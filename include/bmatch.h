@@ -161,6 +161,71 @@ int bm_moves_set(bm_ctx *ctx, int32_t k, const int32_t *verts, int64_t count);
 int bm_moves_eval(bm_ctx *ctx, int mode, int64_t *best_index, int32_t *best_delta, int32_t *best_aux);
 int bm_moves_results(bm_ctx *ctx, int32_t *delta, int32_t *aux);
 
+/*
+ * Batched refinement.  Each step runs on the device without a host round
+ * trip and is captured once in a hipGraph: candidates of all three move
+ * kinds are scored with the current mapping (the row-best swap of every row
+ * of the all-pairs Delta_B sweep, `triples` 3-vertex moves drawn from 2-hop
+ * G1 neighbourhoods and scored with Delta_G3 over p = 1..5, `sets` k-sets
+ * grown along G1 lines and scored over all k! permutations with intra-set
+ * lines counted once); a conflict-free subset of the improving ones is
+ * selected (largest delta first, lowest candidate index on ties; no two
+ * selected moves share a vertex or a G1 line, so their deltas add up
+ * exactly); they are applied, P / P^T are rebuilt, and S_opt is recomputed
+ * from scratch and checked to equal the previous score plus the applied
+ * deltas.  Only vertices without self-loops or repeated lines move, where
+ * every delta kind is the exact score change.  Results are deterministic for
+ * a given instance, mapping and parameters.
+ *
+ * The reference applies one move per evaluation pass (host arg-max of
+ * triple.cu:1082-1126, quick.cu:1175-1217, permset :651-730); this mode has
+ * no counterpart there.
+ */
+typedef struct bm_refine_params {
+    int32_t pairs;           /* 1: row-best swaps of the all-pairs sweep are candidates */
+    int32_t triples;         /* 3-vertex candidates generated per step, 0..2^24 */
+    int32_t sets;            /* k-set candidates generated per step, 0..2^20 */
+    int32_t set_k;           /* k of the sets, 2..8 */
+    int32_t rounds;          /* selection rounds per step, 1..8 */
+    int32_t steps_per_check; /* steps enqueued between host checks of the stop rules */
+    uint64_t seed;           /* candidate generator seed */
+} bm_refine_params;
+
+typedef struct bm_refine_stats {
+    int64_t steps, moves;                         /* in this run */
+    int64_t pair_moves, triple_moves, set_moves;
+    int64_t initial_score, final_score;           /* exact S_opt */
+    int64_t idle_steps;                           /* trailing steps without a move */
+    int64_t bad_step;                             /* first step failing the exact check, -1 if none */
+    int32_t converged;                            /* stopped by `patience` */
+    int32_t verified;                             /* every step passed the exact check */
+} bm_refine_stats;
+
+typedef struct bm_refine_step {
+    int64_t score; /* exact S_opt after the step */
+    int64_t gain;  /* sum of the applied deltas */
+    int32_t moves; /* moves applied */
+    int32_t ok;    /* score == previous score + gain, and rose if moves > 0 */
+} bm_refine_step;
+
+/* Allocates the step's buffers, autotunes kernel variants (results do not
+ * depend on the choice) and captures the step graph.  Needs a mapping. */
+int bm_refine_configure(bm_ctx *ctx, const bm_refine_params *params);
+/* Runs steps from the current mapping until max_steps, until `moves` reaches
+ * move_budget (> 0), or until `patience` consecutive steps applied nothing;
+ * the rules are checked every steps_per_check steps.  The mapping (and
+ * bm_get_mapping) then holds the result.  BM_ERANGE if an exact check failed. */
+int bm_refine_run(bm_ctx *ctx, int64_t max_steps, int64_t move_budget, int32_t patience, bm_refine_stats *stats);
+/* Per-step records of the last run, steps [first, first+count) (the last 4096 are kept). */
+int bm_refine_history(bm_ctx *ctx, int64_t first, int64_t count, bm_refine_step *out);
+/* Keep a log of up to `capacity` applied moves per run (0: off).  Each record
+ * is 20 int32: step, kind (0 pair, 1 triple, 2 set), k, delta, vertices[8],
+ * their new images[8].  bm_refine_moves copies min(count, capacity) records. */
+int bm_refine_log(bm_ctx *ctx, int32_t capacity);
+int bm_refine_moves(bm_ctx *ctx, int64_t *count, int32_t *records, int64_t capacity);
+/* Autotuned variants in use: lanes per triple (8/16/32), threads per set (64/256). */
+int bm_refine_config(bm_ctx *ctx, int32_t *rot3_lanes, int32_t *permset_threads);
+
 /* Which all-pairs gain kernel bm_sweep_all_pairs runs for this graph pair,
  * as chosen on its first call (0 before that): *kernel = 1 dense slab kernel
  * (with *dense_cols columns per lane) or 2 sparse kernel.  No counterpart in

@@ -1,0 +1,221 @@
+"""Batched refinement on the GPU: every applied step is checked on the CPU.
+
+For each step of a run the log of applied moves is replayed on the oracle:
+the moves of one step share no vertex and no G1 line, each move's logged
+delta equals the oracle's exact score change when it is applied alone to the
+step's starting mapping (and, for swaps, Delta_B of
+graph-alignment-optimized.c:321-408 and the arg-max of its row over b > a),
+and the step's moves together raise S_opt by exactly their sum, which is the
+score the device recorded for that step.
+"""
+import numpy as np
+import pytest
+
+import bmatch
+import oracle_binding as ob
+from conftest import random_mapping, random_multigraph
+
+pytestmark = pytest.mark.gpu
+
+
+def closed_nbrs(n, g1):
+    nb = [{v} for v in range(n + 1)]
+    for s, d in zip(g1[0].tolist(), g1[1].tolist()):
+        nb[s].add(d)
+        nb[d].add(s)
+    return nb
+
+
+def simple_vertices(n, g1):
+    ok = np.ones(n + 1, bool)
+    seen = set()
+    for s, d in zip(g1[0].tolist(), g1[1].tolist()):
+        if s == d:
+            ok[s] = False
+        if (s, d) in seen:
+            ok[s] = ok[d] = False
+        seen.add((s, d))
+    return ok
+
+
+def sorted_log(recs):
+    order = np.lexsort((recs[:, 4], recs[:, 0]))
+    return recs[order]
+
+
+def replay(n, g1, o, mp0, recs, stats, hist, check_rows=True):
+    """Check the whole run against the oracle; returns the final mapping."""
+    nb = closed_nbrs(n, g1)
+    simple = simple_vertices(n, g1)
+    cur = mp0.copy()
+    score = o.score_opt(cur)
+    assert score == stats["initial_score"]
+    kinds = [0, 0, 0]
+    for step in range(stats["steps"]):
+        ms = recs[recs[:, 0] == step]
+        used = set()
+        nxt = cur.copy()
+        gain = 0
+        for r in ms:
+            kind, k, delta = int(r[1]), int(r[2]), int(r[3])
+            vs = [int(x) for x in r[4:4 + k]]
+            img = [int(x) for x in r[12:12 + k]]
+            kinds[kind] += 1
+            assert delta > 0
+            assert all(1 <= v <= n and simple[v] for v in vs)
+            assert sorted(img) == sorted(int(cur[v]) for v in vs)
+            closed = set().union(*(nb[v] for v in vs))
+            assert not (closed & used), f"step {step}: move {vs} conflicts with an earlier one"
+            used.update(vs)
+            alone = cur.copy()
+            alone[vs] = img
+            assert o.score_opt(alone) - score == delta, (step, kind, vs)
+            if kind == 0:
+                a, b = vs
+                assert a < b and o.delta_b(cur, a, b) == delta
+                if check_rows:
+                    row = [o.delta_b(cur, a, x) for x in range(a + 1, n + 1)]
+                    assert max(row) == delta and row.index(delta) == b - a - 1
+            nxt[vs] = img
+            gain += delta
+        new = o.score_opt(nxt)
+        assert new == score + gain
+        h = hist[step]
+        assert h == (new, gain, len(ms), 1), (step, h)
+        cur, score = nxt, new
+    assert score == stats["final_score"]
+    assert kinds == [stats["pair_moves"], stats["triple_moves"], stats["set_moves"]]
+    return cur
+
+
+def run(e, mp0, params, log_cap=1 << 16, **kw):
+    e.set_mapping(mp0)
+    e.refine_configure(params)
+    e.refine_log(log_cap)
+    st = e.refine_run(**kw)
+    recs, cnt = e.refine_moves()
+    assert cnt == len(recs) == st["moves"]
+    hist = e.refine_history(0, st["steps"])
+    return st, sorted_log(recs), hist
+
+
+def instance(seed, n, m):
+    inst = bmatch.Instance.powerlaw(seed, n, m)
+    return inst, ob.Oracle(inst.n, inst.g1, inst.g2)
+
+
+def test_pairs_only_converges_exactly():
+    inst, o = instance(11, 150, 1500)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        st, recs, hist = run(e, inst.p0, bmatch.refine_params(pairs=1, rounds=2, steps_per_check=1), patience=1)
+        assert st["verified"] == 1 and st["converged"] == 1 and st["moves"] > 0
+        fin = replay(inst.n, inst.g1, o, inst.p0, recs, st, hist)
+        np.testing.assert_array_equal(e.mapping(), fin)
+        assert e.score() == st["final_score"] == o.score_opt(fin)
+        # converged: no improving swap is left (the generator makes every vertex simple)
+        assert e.sweep_all_pairs()["best"][2] <= 0
+
+
+@pytest.mark.parametrize("k", [3, 4, 8])
+def test_all_kinds_exact(k):
+    inst, o = instance(20 + k, 200, 2200)
+    prm = bmatch.refine_params(pairs=1, triples=3000, sets=60 if k == 8 else 400, set_k=k, rounds=3,
+                               steps_per_check=2, seed=5 + k)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        st, recs, hist = run(e, inst.p0, prm, max_steps=12, patience=4)
+        assert st["verified"] == 1 and st["moves"] > 0
+        assert st["triple_moves"] > 0 and st["set_moves"] > 0
+        fin = replay(inst.n, inst.g1, o, inst.p0, recs, st, hist, check_rows=False)
+        np.testing.assert_array_equal(e.mapping(), fin)
+
+
+def test_nonsimple_vertices_never_move():
+    n = 60
+    g1 = random_multigraph(3, n, 700, dup_frac=0.05, selfloop_frac=0.05)
+    g2 = random_multigraph(4, n, 700, dup_frac=0.05, selfloop_frac=0.05)
+    mp = random_mapping(5, n, 0.1)
+    o = ob.Oracle(n, g1, g2)
+    with bmatch.Engine(0) as e:
+        e.upload(n, g1, g2)
+        prm = bmatch.refine_params(pairs=1, triples=2000, sets=200, set_k=4, rounds=2, steps_per_check=3, seed=9)
+        st, recs, hist = run(e, mp, prm, max_steps=9, patience=3)
+        assert st["verified"] == 1
+        fin = replay(n, g1, o, mp, recs, st, hist, check_rows=False)
+        np.testing.assert_array_equal(e.mapping(), fin)
+
+
+def test_deterministic_and_variant_independent(monkeypatch):
+    inst, o = instance(31, 180, 2000)
+    prm = bmatch.refine_params(pairs=1, triples=2000, sets=300, set_k=5, rounds=2, steps_per_check=2, seed=77)
+    results = []
+    for lanes, threads, kernel in [(None, None, None), (8, 64, "dense"), (32, 256, "sparse"), (16, 64, "sparse")]:
+        for name, val in (("BM_ROT3_LANES", lanes), ("BM_PERMSET_THREADS", threads), ("BM_SWEEP_KERNEL", kernel)):
+            if val is None:
+                monkeypatch.delenv(name, raising=False)
+            else:
+                monkeypatch.setenv(name, str(val))
+        with bmatch.Engine(0) as e:
+            e.upload(inst.n, inst.g1, inst.g2)
+            st, recs, hist = run(e, inst.p0, prm, max_steps=8, patience=8)
+            if lanes:
+                assert e.refine_config() == (lanes, threads)
+                assert e.sweep_config()[0] == (1 if kernel == "dense" else 2)
+            results.append((st["final_score"], st["moves"], recs.tobytes(), e.mapping().tobytes()))
+    assert all(r == results[0] for r in results[1:])
+
+
+def test_runs_continue_from_the_current_mapping():
+    inst, o = instance(41, 160, 1700)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        e.refine_configure(bmatch.refine_params(pairs=1, triples=1000, rounds=2, steps_per_check=1, seed=3))
+        a = e.refine_run(max_steps=3, patience=5)
+        mid = e.mapping()
+        assert a["final_score"] == o.score_opt(mid)
+        b = e.refine_run(max_steps=3, patience=5)
+        assert a["steps"] == b["steps"] == 3
+        assert b["initial_score"] == a["final_score"]
+        assert e.score() == b["final_score"] == o.score_opt(e.mapping())
+        # a host move between runs is picked up by the next run
+        e.apply_swap(1, 2)
+        moved = e.mapping()
+        c = e.refine_run(max_steps=1, patience=5)
+        assert c["initial_score"] == o.score_opt(moved)
+        assert e.score() == c["final_score"] == o.score_opt(e.mapping())
+        # and so is a new mapping
+        e.set_mapping(inst.p0)
+        d = e.refine_run(max_steps=1, patience=5)
+        assert d["initial_score"] == o.score_opt(inst.p0)
+
+
+def test_move_budget_stops_at_check():
+    inst, o = instance(51, 200, 2400)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        e.refine_configure(bmatch.refine_params(pairs=1, rounds=2, steps_per_check=1))
+        st = e.refine_run(move_budget=5, patience=3)
+        assert st["moves"] >= 5
+        hist = e.refine_history(0, st["steps"])
+        assert sum(h[2] for h in hist[:-1]) < 5  # stopped at the first check past the budget
+        assert st["final_score"] == o.score_opt(e.mapping())
+
+
+def test_refine_errors():
+    inst, o = instance(61, 50, 400)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        with pytest.raises(bmatch.BMError):
+            e.refine_configure(bmatch.refine_params())  # no mapping yet
+        e.set_mapping(inst.p0)
+        with pytest.raises(bmatch.BMError):
+            e.refine_run()  # not configured
+        with pytest.raises(bmatch.BMError):
+            e.refine_configure(bmatch.refine_params(pairs=0))
+        with pytest.raises(bmatch.BMError):
+            e.refine_configure(bmatch.refine_params(sets=10, set_k=9))
+        with pytest.raises(bmatch.BMError):
+            e.refine_configure(bmatch.refine_params(rounds=0))
