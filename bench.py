@@ -76,8 +76,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB")
+    ap.add_argument("--window", type=float, default=0.0, help="seconds per leg of the batched comparison")
     ap.add_argument("--workload", default="allpairs",
-                    choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "serve"],
+                    choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "batched-c1",
+                             "batched-permset", "serve"],
                     help="allpairs (default, the headline); the others measure the other configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -17,6 +17,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libbmatch.so")
 BIN_DIR = os.path.join(_PKG, "bin")
 
+BM_OK, BM_EINVAL, BM_ENOMEM, BM_EHIP, BM_ESTATE, BM_ENODEV, BM_ERANGE, BM_EIO = 0, -1, -2, -3, -4, -5, -6, -7
 BM_ROW_FIRST_POSITIVE = 0
 BM_ROW_ARGMAX = 1
 BM_ROT3_ROTATIONS = 0
@@ -76,6 +77,12 @@ def lib():
         "bm_refine_log": (C.c_int, [C.c_void_p, C.c_int32]),
         "bm_refine_moves": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.c_int64]),
         "bm_refine_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "bm_job_footprint": (C.c_int64, [C.c_int32, C.c_int64, C.c_int64, C.c_void_p]),
+        "bm_server_create": (C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+        "bm_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
+        "bm_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+        "bm_server_stats_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "bm_server_destroy": (C.c_int, [C.c_void_p]),
         "bm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
         "bm_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "bm_reset_timing": (C.c_int, [C.c_void_p]),
@@ -141,7 +148,8 @@ _ROW_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.
 class RefineParams(C.Structure):
     """bm_refine_params of include/bmatch.h."""
     _fields_ = [("pairs", C.c_int32), ("triples", C.c_int32), ("sets", C.c_int32), ("set_k", C.c_int32),
-                ("rounds", C.c_int32), ("steps_per_check", C.c_int32), ("seed", C.c_uint64)]
+                ("set_gen", C.c_int32), ("rounds", C.c_int32), ("steps_per_check", C.c_int32),
+                ("seed", C.c_uint64)]
 
 
 class _RefineStats(C.Structure):
@@ -155,9 +163,32 @@ class _RefineStep(C.Structure):
     _fields_ = [("score", C.c_int64), ("gain", C.c_int64), ("moves", C.c_int32), ("ok", C.c_int32)]
 
 
-def refine_params(pairs=1, triples=0, sets=0, set_k=4, rounds=2, steps_per_check=4, seed=1):
-    return RefineParams(int(pairs), int(triples), int(sets), int(set_k), int(rounds), int(steps_per_check),
-                        int(seed))
+class _ServerConfig(C.Structure):
+    _fields_ = [("max_active", C.c_int32), ("prep_threads", C.c_int32), ("hbm_budget", C.c_int64)]
+
+
+class _JobSpec(C.Structure):
+    _fields_ = [("n", C.c_int32), ("g1_src", C.c_void_p), ("g1_dst", C.c_void_p), ("g1_w", C.c_void_p),
+                ("e1", C.c_int64), ("g2_src", C.c_void_p), ("g2_dst", C.c_void_p), ("g2_w", C.c_void_p),
+                ("e2", C.c_int64), ("mapping", C.c_void_p), ("move_budget", C.c_int64), ("max_steps", C.c_int64),
+                ("patience", C.c_int32), ("params", RefineParams)]
+
+
+class _JobResult(C.Structure):
+    _fields_ = [("job_id", C.c_int64), ("status", C.c_int32), ("n", C.c_int32), ("initial_score", C.c_int64),
+                ("final_score", C.c_int64), ("steps", C.c_int64), ("moves", C.c_int64),
+                ("device_bytes", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double), ("error", C.c_char * 160)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("submitted", "done", "failed", "queued", "active", "peak_active",
+                                         "reserved_bytes", "peak_reserved_bytes", "hbm_budget")]
+
+
+def refine_params(pairs=1, triples=0, sets=0, set_k=4, set_gen=0, rounds=2, steps_per_check=4, seed=1):
+    return RefineParams(int(pairs), int(triples), int(sets), int(set_k), int(set_gen), int(rounds),
+                        int(steps_per_check), int(seed))
 
 
 class Instance:
@@ -465,11 +496,14 @@ class Engine:
         k = min(st.accepted, acc_cap)
         return acc[: 3 * k].reshape(k, 3), {f: getattr(st, f) for f, _ in _DriveStats._fields_}
 
-    def first_improvement(self, rows_per_sweep=0, max_sweeps=0, verify=True, acc_cap=1 << 20, max_rows=0):
-        """bm_drive_first_improvement; max_rows > 0 stops after that many rows (a row hook)."""
+    def first_improvement(self, rows_per_sweep=0, max_sweeps=0, verify=True, acc_cap=1 << 20, max_rows=0,
+                          stop=None):
+        """bm_drive_first_improvement; max_rows > 0 stops after that many rows, stop(rows) -> True
+        stops after the current row (both through the row hook)."""
         hook = None
-        if max_rows > 0:
-            hook = _ROW_HOOK(lambda user, rows, row, score, imp: 1 if rows > max_rows else 0)
+        if max_rows > 0 or stop is not None:
+            hook = _ROW_HOOK(lambda user, rows, row, score, imp:
+                             1 if (max_rows > 0 and rows > max_rows) or (stop is not None and stop(rows)) else 0)
         return self._drive(self._L.bm_drive_first_improvement, rows_per_sweep, max_sweeps, int(verify),
                            C.cast(hook, C.c_void_p) if hook else None, None, acc_cap=acc_cap)
 
@@ -478,3 +512,64 @@ class Engine:
 
     def global_best(self, max_steps=0, verify=True, acc_cap=1 << 20):
         return self._drive(self._L.bm_drive_global_best, max_steps, int(verify), acc_cap=acc_cap)
+
+
+def job_footprint(n, e1, e2, params):
+    """Device bytes the server reserves for a job of this shape."""
+    return lib().bm_job_footprint(n, e1, e2, C.byref(params))
+
+
+class Server:
+    """bm_server: refinement jobs admitted against an HBM budget and run
+    concurrently on one device (include/bmatch.h, serving loop)."""
+
+    def __init__(self, device=0, max_active=8, prep_threads=4, hbm_budget=0):
+        self._L = lib()
+        h = C.c_void_p()
+        cfg = _ServerConfig(max_active, prep_threads, hbm_budget)
+        _check(self._L.bm_server_create(device, C.byref(cfg), C.byref(h)), "bm_server_create")
+        self._h = h
+
+    def submit(self, n, g1, g2, mapping, params, move_budget=0, max_steps=0, patience=2):
+        """Queue a job (the arrays are copied before this returns); returns its id."""
+        a = [_i32(x) for x in (*g1, *g2, mapping)]
+        if len(a[6]) != n + 1:
+            raise ValueError("mapping needs n+1 entries")
+        ptr = [x.ctypes.data_as(C.c_void_p) for x in a]
+        spec = _JobSpec(n, ptr[0], ptr[1], ptr[2], len(a[0]), ptr[3], ptr[4], ptr[5], len(a[3]), ptr[6],
+                        move_budget, max_steps, patience, params)
+        jid = C.c_int64()
+        _check(self._L.bm_server_submit(self._h, C.byref(spec), C.byref(jid)), "bm_server_submit")
+        return jid.value
+
+    def wait(self, job_id, n=None):
+        """Block until the job is done: (result dict, final mapping or None)."""
+        res = _JobResult()
+        mp = np.zeros(n + 1, np.int32) if n is not None else None
+        _check(self._L.bm_server_wait(self._h, job_id, C.byref(res),
+                                      mp.ctypes.data_as(C.c_void_p) if mp is not None else None), "bm_server_wait")
+        out = {f: getattr(res, f) for f, _ in _JobResult._fields_}
+        out["error"] = out["error"].decode(errors="replace")
+        return out, (mp if out["status"] == 0 else None)
+
+    def stats(self):
+        st = _ServerStats()
+        _check(self._L.bm_server_stats_get(self._h, C.byref(st)), "bm_server_stats_get")
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            self._L.bm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

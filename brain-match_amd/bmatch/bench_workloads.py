@@ -7,6 +7,12 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
   plain-c2  the n=18,524 / 3.0M pair, first-improvement over the first rows
   rot3      the n=1,000 pair, 2^24 two-hop triples per launch, both rotations
   permset   the n=1,000 pair, 4,096 8-vertex sets x 40,320 permutations
+  batched   the n=18,524 / 3.0M pair: exact score gain per second of the
+            batched refinement against plain first-improvement, both from the
+            same start over the same wall-clock window
+  batched-c1       the same comparison on the n=1,000 pair, both to convergence
+  batched-permset  the n=1,000 pair, 4,096 uniform 8-vertex sets per step,
+            every improving non-conflicting set applied, exact check per step
 """
 import time
 
@@ -107,5 +113,115 @@ def permset(args):
     return out
 
 
+def _refine_window(e, seconds, chunk=8, max_steps=1 << 30):
+    """Refinement runs of `chunk` steps until `seconds` elapsed or a run applies nothing."""
+    s0 = e.score()
+    steps = moves = 0
+    kinds = [0, 0, 0]
+    trace = []
+    t0 = time.perf_counter()
+    while True:
+        st = e.refine_run(max_steps=chunk, patience=chunk)
+        el = time.perf_counter() - t0
+        steps += st["steps"]
+        moves += st["moves"]
+        kinds = [kinds[0] + st["pair_moves"], kinds[1] + st["triple_moves"], kinds[2] + st["set_moves"]]
+        assert st["verified"] == 1
+        trace.append((round(el, 4), st["final_score"]))
+        if st["moves"] == 0 or el >= seconds or steps >= max_steps:
+            break
+    return {"seconds": el, "steps": steps, "moves": moves, "pair_moves": kinds[0], "triple_moves": kinds[1],
+            "set_moves": kinds[2], "initial_score": s0, "final_score": trace[-1][1],
+            "gain_per_s": (trace[-1][1] - s0) / el, "trace": trace[:: max(1, len(trace) // 12)]}
+
+
+def _plain_window(e, seconds):
+    s0 = e.score()
+    deadline = time.perf_counter() + seconds
+    t0 = time.perf_counter()
+    acc, st = e.first_improvement(verify=False, stop=lambda rows: time.perf_counter() > deadline)
+    el = time.perf_counter() - t0
+    return {"seconds": el, "rows": int(st["rows"]), "moves": int(st["accepted"]), "initial_score": s0,
+            "final_score": int(st["final_score"]), "gain_per_s": (int(st["final_score"]) - s0) / el}
+
+
+def _batched_compare(inst, prm, seconds, label, chunk):
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        t = time.perf_counter()
+        e.refine_configure(prm)
+        setup = time.perf_counter() - t
+        b = _refine_window(e, seconds, chunk=chunk)
+        assert e.score() == b["final_score"]
+        lanes, threads = e.refine_config()
+        kind, cols = e.sweep_config()
+        e.set_mapping(inst.p0)
+        p = _plain_window(e, max(b["seconds"], 1.0) if seconds > 0 else 0)
+    ratio = b["gain_per_s"] / p["gain_per_s"] if p["gain_per_s"] > 0 else None
+    out = _base("batched refinement: exact score gain per second vs plain first-improvement", b["gain_per_s"],
+                "score/s", b["steps"], b["seconds"] * 1000 / max(b["steps"], 1), label)
+    out.update({"vs_plain": ratio, "batched": b, "plain": p, "setup_seconds": setup,
+                "autotuned": {"rot3_lanes": lanes, "permset_threads": threads,
+                              "gain_kernel": "sparse" if kind == 2 else f"dense/{cols}"},
+                "params": {f: getattr(prm, f) for f, _ in bmatch.RefineParams._fields_}})
+    return out
+
+
+def batched(args):
+    inst = bmatch.Instance.chunglu(2, 18524, 3_000_000)
+    prm = bmatch.refine_params(pairs=1, triples=1 << 16, sets=1024, set_k=4, rounds=3, steps_per_check=8, seed=1)
+    secs = getattr(args, "window", 0) or 10.0
+    return _batched_compare(inst, prm, secs, {"workload": "Chung-Lu pair n=18524 E=3.0M, 30% shuffled start, "
+                                                          f"{secs:.0f} s window, batched vs plain", "n": inst.n},
+                            chunk=8)
+
+
+def batched_c1(args):
+    inst = bmatch.Instance.powerlaw(1, 1000, 20000)
+    prm = bmatch.refine_params(pairs=1, triples=1 << 14, sets=512, set_k=4, rounds=3, steps_per_check=8, seed=1)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        e.refine_configure(prm)
+        b = _refine_window(e, 60.0, chunk=16)
+        e.set_mapping(inst.p0)
+        t = time.perf_counter()
+        acc, st = e.first_improvement(verify=False)
+        el = time.perf_counter() - t
+    p = {"seconds": el, "moves": int(st["accepted"]), "initial_score": b["initial_score"],
+         "final_score": int(st["final_score"]), "gain_per_s": (int(st["final_score"]) - b["initial_score"]) / el}
+    out = _base("batched refinement: exact score gain per second vs plain first-improvement", b["gain_per_s"],
+                "score/s", b["steps"], b["seconds"] * 1000 / max(b["steps"], 1),
+                {"workload": "power-law pair n=1000 E=20000, random start, both to convergence", "n": inst.n})
+    out.update({"vs_plain": b["gain_per_s"] / p["gain_per_s"], "batched": b, "plain": p})
+    return out
+
+
+def batched_permset(args):
+    inst = bmatch.Instance.powerlaw(1, 1000, 20000)
+    prm = bmatch.refine_params(pairs=0, triples=0, sets=4096, set_k=8, set_gen=1, rounds=3, steps_per_check=4,
+                               seed=1)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        e.refine_configure(prm)
+        steps = max(args.steps, 8)
+        t = time.perf_counter()
+        st = e.refine_run(max_steps=steps, patience=steps + 1)
+        el = time.perf_counter() - t
+        hist = e.refine_history(0, st["steps"])
+        lanes, threads = e.refine_config()
+    assert st["verified"] == 1
+    out = _base("batched 8-set permutation evaluations per second (exact step check)",
+                4096 * 40320 * st["steps"] / el, "permutation evals/s", st["steps"], el * 1000 / st["steps"],
+                {"workload": "power-law pair n=1000 E=20000, 4096 uniform 8-vertex sets x 40320 permutations per "
+                             "step, non-conflicting improving sets applied", "n": inst.n})
+    out.update({"sets_applied": st["set_moves"], "score_gain": st["final_score"] - st["initial_score"],
+                "per_step_moves": [h[2] for h in hist][:32], "permset_threads": threads})
+    return out
+
+
 def run(name, args):
-    return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset}[name](args)
+    return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset, "batched": batched,
+            "batched-c1": batched_c1, "batched-permset": batched_permset}[name](args)

@@ -31,9 +31,18 @@ int launch_check(const char *what) {
     return BM_OK;
 }
 
+std::mutex &sync_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+
 int dev_alloc(bm_ctx *c, void **p, size_t bytes) {
     if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e;
+    {
+        std::lock_guard<std::mutex> g(sync_mutex());
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         return hip_fail(e, "hipMalloc");
@@ -44,6 +53,7 @@ int dev_alloc(bm_ctx *c, void **p, size_t bytes) {
 
 void dev_free(bm_ctx *c, void **p, size_t bytes) {
     if (*p) {
+        std::lock_guard<std::mutex> g(sync_mutex());
         (void)hipFree(*p);
         c->dev_bytes -= (int64_t)(bytes ? bytes : 16);
     }
@@ -53,6 +63,7 @@ void dev_free(bm_ctx *c, void **p, size_t bytes) {
 int ensure_stage(bm_ctx *c, size_t bytes) {
     if (c->stage_bytes >= bytes) return BM_OK;
     if (c->stage) {
+        std::lock_guard<std::mutex> g(sync_mutex());
         (void)hipFree(c->stage);
         c->dev_bytes -= (int64_t)c->stage_bytes;
         c->stage = nullptr;
@@ -349,7 +360,10 @@ template <typename T>
 static int upload(bm_ctx *c, T **dst, const std::vector<T> &src) {
     int rc = dev_alloc(c, (void **)dst, src.size() * sizeof(T));
     if (rc) return rc;
-    if (!src.empty()) BM_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!src.empty()) {
+        std::lock_guard<std::mutex> g(sync_mutex());
+        BM_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
     return BM_OK;
 }
 
@@ -440,6 +454,7 @@ int bm_destroy(bm_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph_buffers(c);
+    std::lock_guard<std::mutex> g(sync_mutex());
     if (c->stage) (void)hipFree(c->stage);
     for (auto &r : c->pending) {
         (void)hipEventDestroy(r.start);

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -124,6 +125,12 @@ int hip_fail(hipError_t e, const char *what);
         hipError_t e_ = (call);                             \
         if (e_ != hipSuccess) return bm::hip_fail(e_, #call); \
     } while (0)
+
+// Held across a stream capture and around the engine's allocation, free and
+// synchronous-copy calls (and stream / graph destruction): made from another
+// thread in HIP's default capture mode, such a call would invalidate a
+// capture in progress.
+std::mutex &sync_mutex();
 
 int dev_alloc(bm_ctx *c, void **p, size_t bytes);
 void dev_free(bm_ctx *c, void **p, size_t bytes);

@@ -60,6 +60,7 @@ struct RefineState {
     unsigned long long score_slot;
     long long idle;            // consecutive steps without a move
     long long initial_score;
+    unsigned long long gen_step;  // steps since bm_refine_configure: seeds the candidate generators
 };
 
 struct MoveRec {
@@ -147,7 +148,7 @@ __global__ void k_gen_triples(int32_t *__restrict__ tri, int T, const RefineStat
                               const int32_t *__restrict__ cr, int N1, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= T) return;
-    const unsigned long long step = st->step;
+    const unsigned long long step = st->gen_step;
     const unsigned long long h0 = rnd(seed, step, i, 0), h1 = rnd(seed, step, i, 1), h2 = rnd(seed, step, i, 2);
     const int u = 1 + (int)(h0 % (unsigned long long)n);
     int v = nbr(cptr, cr, N1, u, h1);
@@ -160,20 +161,21 @@ __global__ void k_gen_triples(int32_t *__restrict__ tri, int T, const RefineStat
     tri[3 * i + 2] = w;
 }
 
-// k-set candidates grown along G1 lines from a uniform seed vertex
-__global__ void k_gen_sets(int32_t *__restrict__ sets, int S, int k, const RefineState *__restrict__ st,
+// k-set candidates from a uniform seed vertex, grown along G1 lines
+// (uniform = 0) or drawn uniformly (uniform = 1)
+__global__ void k_gen_sets(int32_t *__restrict__ sets, int S, int k, int uniform, const RefineState *__restrict__ st,
                            unsigned long long seed, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
                            int N1, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S) return;
-    const unsigned long long step = st->step;
+    const unsigned long long step = st->gen_step;
     int s[8];
     int j = 0, draw = 0;
     s[j++] = 1 + (int)(rnd(seed ^ 0xA5A5ull, step, i, draw++) % (unsigned long long)n);
     for (int tries = 0; j < k && tries < 6 * k; tries++) {
         const unsigned long long h = rnd(seed ^ 0xA5A5ull, step, i, draw++);
         const int m = s[(int)(h % (unsigned long long)j)];
-        int c = tries < 4 * k ? nbr(cptr, cr, N1, m, h >> 8) : -1;
+        int c = !uniform && tries < 4 * k ? nbr(cptr, cr, N1, m, h >> 8) : -1;
         if (c < 0) c = 1 + (int)((h >> 8) % (unsigned long long)n);
         bool fresh = c >= 1 && c <= n;
         for (int q = 0; q < j && fresh; q++) fresh = s[q] != c;
@@ -324,14 +326,17 @@ __global__ void k_step_end(RefineState *st, bm_refine_step *hist) {
     st->gain_total += gain;
     st->idle = st->applied ? 0 : st->idle + 1;
     st->step++;
+    st->gen_step++;
 }
 
 __global__ void k_state_init(RefineState *st) {
     if (threadIdx.x != 0) return;
     const long long s = (long long)st->score_slot;
+    const unsigned long long g = st->gen_step;
     memset(st, 0, sizeof(RefineState));
     st->score = s;
     st->initial_score = s;
+    st->gen_step = g;
 }
 
 // process-wide choices of the autotuner, per class of instance
@@ -390,6 +395,7 @@ static CandView cand_view(bm_ctx *c) {
 }
 
 static void drop_graph(bm_batch_state *b) {
+    std::lock_guard<std::mutex> g(sync_mutex());
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -400,6 +406,7 @@ int bm::free_batch(bm_ctx *c) {
     bm_batch_state *b = c->batch;
     if (!b) return BM_OK;
     drop_graph(b);
+    std::lock_guard<std::mutex> g(sync_mutex());
     auto fr = [&](void *p) {
         if (p) (void)hipFree(p);
     };
@@ -415,7 +422,11 @@ int bm::free_batch(bm_ctx *c) {
 
 static int balloc(bm_ctx *c, void **p, size_t bytes) {
     if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e;
+    {
+        std::lock_guard<std::mutex> g(sync_mutex());
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         return set_error(BM_ENOMEM, "device allocation of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
@@ -450,7 +461,7 @@ static int enqueue_step(bm_ctx *c) {
     if (p.sets > 0) {
         BM_LAUNCH(c, "gen_sets",
                   hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets, p.sets,
-                                     p.set_k, b->st, (unsigned long long)p.seed, c->cptr, c->cr, N1, c->n));
+                                     p.set_k, p.set_gen, b->st, (unsigned long long)p.seed, c->cptr, c->cr, N1, c->n));
         if ((rc = launch_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r, b->ovf, 1))) return rc;
     }
     const unsigned mb = (unsigned)((b->M + 255) / 256);
@@ -483,16 +494,24 @@ static int capture(bm_ctx *c) {
     const bool timing = c->timing, dirty = c->g2y_dirty;
     c->timing = false;     // no event records inside the graph
     c->g2y_dirty = false;  // each step ends with the refresh; refine_start covers the first
-    BM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(c);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    hipError_t e;
+    int rc;
+    {
+        std::lock_guard<std::mutex> lk(sync_mutex());
+        e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            c->timing = timing;
+            c->g2y_dirty = dirty;
+            return hip_fail(e, "hipStreamBeginCapture");
+        }
+        rc = enqueue_step(c);
+        e = hipStreamEndCapture(c->stream, &g);
+        if (rc && g) (void)hipGraphDestroy(g);
+    }
     c->timing = timing;
     c->g2y_dirty = dirty;
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
+    if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     b->graph = g;
     e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
@@ -579,8 +598,8 @@ static int autotune(bm_ctx *c) {
         if (!best) {
             BM_LAUNCH(c, "gen_sets",
                       hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets,
-                                         p.sets, p.set_k, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1,
-                                         c->n));
+                                         p.sets, p.set_k, p.set_gen, b->st, (unsigned long long)p.seed, c->cptr,
+                                         c->cr, c->N1, c->n));
             float bms = 1e30f;
             for (int th : {64, 256}) {
                 c->permset_threads = th;
@@ -611,6 +630,7 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if (p.triples < 0 || p.sets < 0 || p.triples > (1 << 24) || p.sets > (1 << 20))
         return set_error(BM_EINVAL, "triples %d / sets %d outside [0, 2^24] / [0, 2^20]", p.triples, p.sets);
     if (p.sets > 0 && (p.set_k < 2 || p.set_k > 8)) return set_error(BM_EINVAL, "set_k=%d outside [2,8]", p.set_k);
+    if (p.set_gen != 0 && p.set_gen != 1) return set_error(BM_EINVAL, "set_gen=%d not 0 or 1", p.set_gen);
     if (p.rounds < 1 || p.rounds > 8) return set_error(BM_EINVAL, "rounds=%d outside [1,8]", p.rounds);
     if (p.steps_per_check < 1) return set_error(BM_EINVAL, "steps_per_check=%d < 1", p.steps_per_check);
     if (!p.pairs && !p.triples && !p.sets) return set_error(BM_EINVAL, "no move kind enabled");
@@ -642,8 +662,12 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->win, (size_t)(N1 / 2 + 1) * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->st, sizeof(RefineState)))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->hist, sizeof(bm_refine_step) * kHist))) return fail(rc);
-    if (hipHostMalloc((void **)&b->hst, sizeof(RefineState), hipHostMallocDefault) != hipSuccess)
-        return fail(set_error(BM_ENOMEM, "pinned host allocation failed"));
+    hipError_t he;
+    {
+        std::lock_guard<std::mutex> g(sync_mutex());
+        he = hipHostMalloc((void **)&b->hst, sizeof(RefineState), hipHostMallocDefault);
+    }
+    if (he != hipSuccess) return fail(set_error(BM_ENOMEM, "pinned host allocation failed"));
     if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
         return fail(set_error(BM_EHIP, "hipEventCreate failed"));
     if (hipMemsetAsync(b->st, 0, sizeof(RefineState), c->stream) != hipSuccess ||
@@ -665,6 +689,7 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
 int refine_set_log(bm_ctx *c, int32_t cap) {
     bm_batch_state *b = c->batch;
     if (b->log) {
+        std::lock_guard<std::mutex> g(sync_mutex());
         (void)hipFree(b->log);
         b->bytes -= (size_t)b->log_cap * sizeof(MoveRec);
         c->dev_bytes -= (int64_t)b->log_cap * (int64_t)sizeof(MoveRec);

@@ -186,6 +186,7 @@ typedef struct bm_refine_params {
     int32_t triples;         /* 3-vertex candidates generated per step, 0..2^24 */
     int32_t sets;            /* k-set candidates generated per step, 0..2^20 */
     int32_t set_k;           /* k of the sets, 2..8 */
+    int32_t set_gen;         /* 0: sets grown along G1 lines from a random vertex; 1: uniform random sets */
     int32_t rounds;          /* selection rounds per step, 1..8 */
     int32_t steps_per_check; /* steps enqueued between host checks of the stop rules */
     uint64_t seed;           /* candidate generator seed */
@@ -225,6 +226,68 @@ int bm_refine_log(bm_ctx *ctx, int32_t capacity);
 int bm_refine_moves(bm_ctx *ctx, int64_t *count, int32_t *records, int64_t capacity);
 /* Autotuned variants in use: lanes per triple (8/16/32), threads per set (64/256). */
 int bm_refine_config(bm_ctx *ctx, int32_t *rot3_lanes, int32_t *permset_threads);
+
+/*
+ * Serving loop.  A server owns one device and takes refinement jobs (graph
+ * pair, initial mapping, move budget, refinement parameters) as they arrive:
+ * bm_server_submit copies the job and returns at once.  Preparation threads
+ * admit queued jobs in arrival order while the job's device footprint fits
+ * the HBM budget and an active slot is free (upload, mapping, step-graph
+ * capture); the scheduler thread keeps every admitted job's step chunks
+ * enqueued on the job's own stream, so jobs of any size overlap on the
+ * device, and retires a job when its stop rule holds (move budget, patience,
+ * step limit), freeing its slot and memory for the next.  Each job's result
+ * is what bm_refine_run gives for it alone: the stop rules are checked at the
+ * same step counts whatever else runs.  No counterpart in the reference.
+ */
+typedef struct bm_server bm_server;
+
+typedef struct bm_server_config {
+    int32_t max_active;     /* jobs on the device at once (>= 1) */
+    int32_t prep_threads;   /* admission threads (>= 1) */
+    int64_t hbm_budget;     /* bytes; 0 = the device's free memory at creation */
+} bm_server_config;
+
+typedef struct bm_job_spec {
+    int32_t n;
+    const int32_t *g1_src, *g1_dst, *g1_w;
+    int64_t e1;
+    const int32_t *g2_src, *g2_dst, *g2_w;
+    int64_t e2;
+    const int32_t *mapping;  /* n+1 entries */
+    int64_t move_budget;     /* stop once this many moves were applied (0: no limit) */
+    int64_t max_steps;       /* 0: no limit */
+    int32_t patience;        /* stop after this many steps without a move (>= 1) */
+    bm_refine_params params;
+} bm_job_spec;
+
+typedef struct bm_job_result {
+    int64_t job_id;
+    int32_t status;          /* BM_OK or the error code of the failing call */
+    int32_t n;
+    int64_t initial_score, final_score, steps, moves;
+    int64_t device_bytes;    /* footprint while active */
+    double t_submit, t_admit, t_start, t_done;  /* seconds since bm_server_create */
+    char error[160];         /* message of the failing call ("" when status is BM_OK) */
+} bm_job_result;
+
+typedef struct bm_server_stats {
+    int64_t submitted, done, failed, queued, active;
+    int64_t peak_active;
+    int64_t reserved_bytes, peak_reserved_bytes, hbm_budget;
+} bm_server_stats;
+
+/* Upper bound of the device bytes a job of this shape holds while active. */
+int64_t bm_job_footprint(int32_t n, int64_t e1, int64_t e2, const bm_refine_params *params);
+int bm_server_create(int device, const bm_server_config *config, bm_server **out);
+/* BM_ENOMEM when the job alone exceeds the budget. */
+int bm_server_submit(bm_server *srv, const bm_job_spec *job, int64_t *job_id);
+/* Blocks until the job is done; mapping (n+1 entries) may be NULL.  A job's
+ * result can be collected once. */
+int bm_server_wait(bm_server *srv, int64_t job_id, bm_job_result *result, int32_t *mapping);
+int bm_server_stats_get(bm_server *srv, bm_server_stats *stats);
+/* Waits for every submitted job, then frees the server. */
+int bm_server_destroy(bm_server *srv);
 
 /* Which all-pairs gain kernel bm_sweep_all_pairs runs for this graph pair,
  * as chosen on its first call (0 before that): *kernel = 1 dense slab kernel
