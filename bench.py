@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB")
     ap.add_argument("--window", type=float, default=0.0, help="seconds per leg of the batched comparison")
+    ap.add_argument("--jobs", type=int, default=0, help="jobs of the serve workload")
     ap.add_argument("--workload", default="allpairs",
                     choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "batched-c1",
                              "batched-permset", "serve"],

@@ -13,6 +13,10 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
   batched-c1       the same comparison on the n=1,000 pair, both to convergence
   batched-permset  the n=1,000 pair, 4,096 uniform 8-vertex sets per step,
             every improving non-conflicting set applied, exact check per step
+  serve     Poisson arrivals (20 jobs/s) of power-law pairs with n uniform in
+            500..20,000 and 160 lines per vertex, 30% shuffled starts, each
+            refined by the batched mode to convergence (budget 10^6 moves) on
+            the serving loop; jobs/s and p50/p99 time-to-result
 """
 import time
 
@@ -170,7 +174,7 @@ def _batched_compare(inst, prm, seconds, label, chunk):
 
 def batched(args):
     inst = bmatch.Instance.chunglu(2, 18524, 3_000_000)
-    prm = bmatch.refine_params(pairs=1, triples=1 << 16, sets=1024, set_k=4, rounds=3, steps_per_check=8, seed=1)
+    prm = bmatch.refine_params(pairs=1, triples=4096, sets=1024, set_k=4, rounds=3, steps_per_check=8, seed=1)
     secs = getattr(args, "window", 0) or 10.0
     return _batched_compare(inst, prm, secs, {"workload": "Chung-Lu pair n=18524 E=3.0M, 30% shuffled start, "
                                                           f"{secs:.0f} s window, batched vs plain", "n": inst.n},
@@ -222,6 +226,62 @@ def batched_permset(args):
     return out
 
 
+def serve(args):
+    import concurrent.futures as cf
+    import threading
+
+    jobs = getattr(args, "jobs", 0) or 40
+    rate = 20.0
+    rng = np.random.default_rng(2024)
+    ns = rng.integers(500, 20001, jobs)
+    gaps = rng.exponential(1.0 / rate, jobs)
+    prm = bmatch.refine_params(pairs=1, triples=4096, sets=0, rounds=3, steps_per_check=8, seed=11)
+    t = time.perf_counter()
+    with cf.ThreadPoolExecutor(8) as ex:
+        insts = list(ex.map(lambda i: bmatch.Instance.powerlaw(500 + i, int(ns[i]), 160 * int(ns[i]),
+                                                               shuffle_frac=0.3), range(jobs)))
+    gen_s = time.perf_counter() - t
+    with bmatch.Server(max_active=8, prep_threads=4, hbm_budget=0) as srv:
+        ids = [None] * jobs
+
+        def client():
+            t0 = time.perf_counter()
+            due = 0.0
+            for i in range(jobs):
+                due += gaps[i]
+                d = t0 + due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = srv.submit(insts[i].n, insts[i].g1, insts[i].g2, insts[i].p0, prm, move_budget=1_000_000,
+                                    patience=8)
+
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        res = [srv.wait(ids[i])[0] for i in range(jobs)]
+        st = srv.stats()
+    bad = [r for r in res if r["status"] != 0]
+    assert not bad, bad[0]["error"]
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in res])
+    svc = np.array([r["t_done"] - r["t_start"] for r in res])
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    gain = sum(r["final_score"] - r["initial_score"] for r in res)
+    out = _base("served refinement: jobs/s with p50/p99 time-to-result", jobs / span, "jobs/s", jobs,
+                span * 1000 / jobs,
+                {"workload": "Poisson arrivals 20 jobs/s, power-law pairs n~U[500,20000], 160 lines/vertex, 30% "
+                             "shuffled start, batched refinement to convergence (budget 1e6 moves)",
+                 "jobs": jobs, "arrival_rate": rate})
+    out.update({"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                "mean_queue_s": float(wait.mean()), "mean_service_s": float(svc.mean()),
+                "score_gain_per_s": gain / span, "moves": int(sum(r["moves"] for r in res)),
+                "steps": int(sum(r["steps"] for r in res)), "peak_active": st["peak_active"],
+                "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "hbm_budget_gb": st["hbm_budget"] / 1e9,
+                "generation_s": gen_s, "params": {f: getattr(prm, f) for f, _ in bmatch.RefineParams._fields_},
+                "per_job": [[r["n"], round(r["t_done"] - r["t_submit"], 3), r["steps"], r["moves"]] for r in res]})
+    return out
+
+
 def run(name, args):
     return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset, "batched": batched,
-            "batched-c1": batched_c1, "batched-permset": batched_permset}[name](args)
+            "batched-c1": batched_c1, "batched-permset": batched_permset, "serve": serve}[name](args)

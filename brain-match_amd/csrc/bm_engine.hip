@@ -441,6 +441,10 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->X, (N1 + 63) / 64 * 64 * (size_t)c->ldx * 4);
     dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
     dev_free(c, (void **)&c->vsimple, N1);
+    dev_free(c, (void **)&c->plan, (size_t)c->nplan * 16);
+    dev_free(c, (void **)&c->split_rows, (size_t)c->nsplit * 4);
+    c->nplan = 0;
+    c->nsplit = 0;
     free_moves(c);
     free_batch(c);
     c->graphs = c->mapped = c->diag_valid = false;
@@ -707,6 +711,9 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "upload sync"); goto fail; }
     }
+    c->g2_maxlen = 0;
+    for (int v = 0; v < N1; v++)
+        c->g2_maxlen = std::max(c->g2_maxlen, std::max(g2optr[v + 1] - g2optr[v], g2iptr[v + 1] - g2iptr[v]));
     c->hmap.assign(N1, 0);
     c->hcdeg.resize(N1);
     c->hg2deg.resize(N1);

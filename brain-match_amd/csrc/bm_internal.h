@@ -85,6 +85,12 @@ struct bm_ctx {
     int gain_cols = 0;        // all-pairs slab width per lane (1, 2, 4), 0 = not tuned yet
     float gain_ms[3] = {0, 0, 0};  // autotuner timings of widths 1, 2, 4
     bool sparse_attr = false;      // dynamic-LDS attribute of the sparse gain kernel set
+    int4 *plan = nullptr;          // sparse-kernel row plan: (row, entry lo, entry hi, flags)
+    int64_t nplan = 0;
+    int32_t *split_rows = nullptr; // rows the plan splits over several blocks
+    int nsplit = 0;
+    int dense_len = 1 << 30;       // sparse kernel: G2 lists longer than this stream their Q row
+    int g2_maxlen = 0;             // longest de-duplicated G2 out- or in-list
     int rot3_lanes = 16;           // lanes per triple of the 3-vertex kernel (8, 16, 32)
     int permset_threads = 256;     // workgroup size of the k-set kernel (64, 256)
     uint8_t *vsimple = nullptr;    // 1: no self-loop and no repeated line touches the vertex

@@ -15,6 +15,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "bm_internal.h"
 
@@ -561,6 +564,10 @@ static int tune_gain(bm_ctx *c) {
 // empty); the caller checks that.
 constexpr int kSparseChunk = 512;
 constexpr int kSparseUnroll = 8;  // list entries in flight per lane
+constexpr int kSplitEntries = 2048;  // comb entries per block of the sparse kernel, at most
+// Long G2 lists streamed as whole Q rows, per chunk of entries.  Kept small:
+// two blocks per CU need the static LDS plus 4 (n+1) bytes under 80 KiB.
+constexpr int kSparseHubs = 32;
 
 __global__ void k_g2_to_p(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
                           const int32_t *__restrict__ pinv, int64_t m) {
@@ -584,30 +591,44 @@ int bm::launch_refresh_g2y(bm_ctx *c) {
     return BM_OK;
 }
 
-__global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+// One block per work item (a, lo, hi, flags) of the row plan: entries
+// [lo, hi) of comb(a).  A row with one item (flags & kRowWhole) is finished
+// here: X(a,a) into diag, the corrections, the row stored.  The items of a
+// split row add their partial rows into X (zeroed beforehand) with atomics;
+// the first of them (flags & kRowFirst) adds cneg and the corrections, and
+// X(a,a) becomes diag[a] afterwards (k_diag_from_x).
+constexpr int kRowFirst = 1, kRowWhole = 2;
+
+template <bool kHubs>
+__global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ plan, const int32_t *__restrict__ cptr,
+                                                      const int2 *__restrict__ crw,
                                                       const int32_t *__restrict__ cdw,
+                                                      const int32_t *__restrict__ cnegv,
                                                       const int32_t *__restrict__ map,
                                                       const int32_t *__restrict__ g2optr,
                                                       const uint32_t *__restrict__ g2oy,
                                                       const int32_t *__restrict__ g2iptr,
                                                       const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
                                                       int ld, int32_t *__restrict__ X, int64_t ldx,
-                                                      int32_t *__restrict__ diag, int N1) {
-    extern __shared__ int acc[];
+                                                      int32_t *__restrict__ diag, int N1, int dense_len) {
+    extern __shared__ __align__(16) int acc[];
     __shared__ int pref[kSparseChunk + 1];
     __shared__ int lst[kSparseChunk];
     __shared__ int wts[kSparseChunk];
+    __shared__ int hubr[kHubs ? kSparseHubs : 1], hubw[kHubs ? kSparseHubs : 1];
     __shared__ int wsum[16];
-    __shared__ int cneg;
-    const int a = blockIdx.x, tid = threadIdx.x;
+    __shared__ int nhub;
+    const int4 item = plan[blockIdx.x];
+    const int a = item.x, tid = threadIdx.x, flags = item.w;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     for (int i = tid; i < N1; i += 1024) acc[i] = 0;
-    if (tid == 0) cneg = 0;
-    __syncthreads();
-    const int beg = cptr[a], end = cptr[a + 1];
+    const int beg = item.y, end = item.z;
     for (int c0 = beg; c0 < end; c0 += kSparseChunk) {
         const int m = min(kSparseChunk, end - c0);
-        // per entry: which G2 list (start, length) and its weight
+        if (tid == 0) nhub = 0;
+        __syncthreads();
+        // per entry: which G2 list (start, length) and its weight; an entry
+        // whose list is longer than dense_len streams its Q row instead
         int len = 0;
         if (tid < m) {
             const int2 rw = crw[c0 + tid];
@@ -617,7 +638,14 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
             len = (out ? g2iptr[g + 1] : g2optr[g + 1]) - lo;
             lst[tid] = out ? (lo | (int)0x80000000) : lo;
             wts[tid] = rw.y;
-            if (rw.y < 0) atomicAdd(&cneg, rw.y);
+            if (kHubs && len > dense_len) {
+                const int h = atomicAdd(&nhub, 1);
+                if (h < kSparseHubs) {  // beyond the cap the list is walked like the others
+                    hubr[h] = rw.x;
+                    hubw[h] = rw.y;
+                    len = 0;
+                }
+            }
         }
         // exclusive prefix sum of the list lengths (first 512 threads)
         int x = len;
@@ -687,14 +715,43 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
             }
         }
         __syncthreads();
+        // long lists: whole Q rows instead, up to four folded per pass; each
+        // lane owns an adjacent pair of columns (64-bit LDS accesses)
+        const int nh = kHubs ? min(nhub, kSparseHubs) : 0;
+        for (int h0 = 0; h0 < nh; h0 += 4) {
+            const int g = min(4, nh - h0);
+            const int16_t *row[4];
+            int w[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                row[u] = Q + (size_t)hubr[h0 + min(u, g - 1)] * ld;
+                w[u] = u < g ? hubw[h0 + u] : 0;
+            }
+            for (int y = tid * 2; y < N1; y += 2048) {
+                int2 v = *(int2 *)&acc[y];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (u < g) {
+                        const bm_s2 q = *(const bm_s2 *)(row[u] + y);
+                        const int base = -min(w[u], 0);
+                        v.x += min(w[u], (int)q[0]) + base;
+                        v.y += min(w[u], (int)q[1]) + base;
+                    }
+                }
+                *(int2 *)&acc[y] = v;  // acc has N1 + 1 slots: the spare one absorbs y + 1 = N1
+            }
+        }
+        __syncthreads();
     }
+    __syncthreads();  // also orders the zeroing of acc when the item has no entries
     // X(a,a) before corrections; then Corr(a,b) for b > a from a's own entries
     // (each G1 line between a and b is in comb(a) exactly once)
-    if (tid == 0) diag[a] = acc[a] + cneg;
+    const int ca = cnegv[a];
+    if ((flags & kRowWhole) && tid == 0) diag[a] = acc[a] + ca;
     __syncthreads();
-    {
+    if (flags & kRowFirst) {
         const int paa = (int)Q[(size_t)a * ld + a];
-        for (int e = beg + tid; e < end; e += 1024) {
+        for (int e = cptr[a] + tid; e < cptr[a + 1]; e += 1024) {
             const int2 rw = crw[e];
             const bool out = rw.x >= N1;
             const int b = out ? rw.x - N1 : rw.x;
@@ -705,28 +762,106 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int32_t *__restrict_
         }
     }
     __syncthreads();
-    const int ca = cneg;
     const int T = (int)(ldx >> 6);
-    for (int i = tid; i < ldx; i += 1024) __builtin_nontemporal_store(i < N1 ? acc[i] + ca : 0, X + xidx(a, i, T));
+    if (flags & kRowWhole) {
+        for (int i = tid; i < ldx; i += 1024) __builtin_nontemporal_store(i < N1 ? acc[i] + ca : 0, X + xidx(a, i, T));
+    } else {
+        const int add = (flags & kRowFirst) ? ca : 0;
+        for (int i = tid; i < N1; i += 1024) {
+            const int v = acc[i] + add;
+            if (v) atomicAdd(X + xidx(a, i, T), v);
+        }
+    }
 }
 
 // The largest LDS row the sparse kernel may use (bytes).
-constexpr size_t kSparseLdsMax = 144 * 1024;
+constexpr size_t kSparseLdsMax = 144 * 1024 - 32;
+
+// rows split over several plan items start from zero
+__global__ void k_zero_rows(int32_t *__restrict__ X, int64_t ldx, const int32_t *__restrict__ rows) {
+    const int a = rows[blockIdx.x];
+    const int T = (int)(ldx >> 6);
+    for (int i = threadIdx.x; i < ldx; i += blockDim.x) X[xidx(a, i, T)] = 0;
+}
+
+__global__ void k_diag_from_x(const int32_t *__restrict__ X, int64_t ldx, const int32_t *__restrict__ rows, int count,
+                              int32_t *__restrict__ diag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int a = rows[i];
+    diag[a] = X[xidx(a, a, (int)(ldx >> 6))];
+}
+
+// The row plan of the sparse kernel: rows with more than kSplitEntries comb
+// entries become ceil(deg / kSplitEntries) items; items ordered by entry count,
+// largest first, so that the longest blocks start first.
+static int build_row_plan(bm_ctx *c) {
+    if (c->plan) return BM_OK;
+    int split = kSplitEntries;
+    if (const char *e = getenv("BM_SPARSE_SPLIT")) split = std::max(4, atoi(e));
+    // G2 lists longer than N1 / 2 stream their Q row densely (BM_SPARSE_DENSE_LEN overrides)
+    c->dense_len = std::max(64, c->N1 / 2);
+    if (const char *e = getenv("BM_SPARSE_DENSE_LEN")) c->dense_len = std::max(0, atoi(e));
+    std::vector<int4> items;
+    std::vector<int32_t> rows;
+    std::vector<int32_t> cp(c->N1 + 1, 0);
+    for (int v = 0; v < c->N1; v++) cp[v + 1] = cp[v] + c->hcdeg[v];
+    for (int a = 0; a < c->N1; a++) {
+        const int deg = c->hcdeg[a];
+        if (deg <= split) {
+            items.push_back(make_int4(a, cp[a], cp[a + 1], kRowFirst | kRowWhole));
+            continue;
+        }
+        rows.push_back(a);
+        for (int lo = cp[a]; lo < cp[a + 1]; lo += split)
+            items.push_back(make_int4(a, lo, std::min(cp[a + 1], lo + split), lo == cp[a] ? kRowFirst : 0));
+    }
+    std::stable_sort(items.begin(), items.end(), [](const int4 &x, const int4 &y) { return x.z - x.y > y.z - y.y; });
+    int rc;
+    if ((rc = dev_alloc(c, (void **)&c->plan, items.size() * sizeof(int4)))) return rc;
+    c->nplan = (int64_t)items.size();
+    BM_HIP(hipMemcpy(c->plan, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    c->nsplit = (int)rows.size();
+    if (!rows.empty()) {
+        if ((rc = dev_alloc(c, (void **)&c->split_rows, rows.size() * 4))) return rc;
+        BM_HIP(hipMemcpy(c->split_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+    }
+    return BM_OK;
+}
 
 static int launch_gain_sparse(bm_ctx *c) {
     if (c->g2y_dirty) {
         int rc = launch_refresh_g2y(c);
         if (rc) return rc;
     }
-    const size_t lds = (size_t)c->N1 * 4;
+    const size_t lds = ((size_t)c->N1 + 8) * 4;  // a spare slot past N1 for the paired column writes
+    int rc = build_row_plan(c);
+    if (rc) return rc;
     if (!c->sparse_attr) {  // once per context, so that captured launches make no attribute calls
-        BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)kSparseLdsMax));
+        BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(kSparseLdsMax + 32)));
+        BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(kSparseLdsMax + 32)));
         c->sparse_attr = true;
     }
-    BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse, dim3(c->N1), dim3(1024), lds, c->stream, c->cptr,
-                                            (const int2 *)c->crw, c->cdw, c->map, c->g2optr, c->g2oy, c->g2iptr,
-                                            c->g2iy, c->Q, c->ld, c->X, c->ldx, c->diag, c->N1));
+    if (c->nsplit > 0)
+        BM_LAUNCH(c, "zero_rows", hipLaunchKernelGGL(k_zero_rows, dim3(c->nsplit), dim3(256), 0, c->stream, c->X,
+                                                     c->ldx, c->split_rows));
+    // the long-list path costs registers and scheduling even when idle: only
+    // the instantiation that needs it carries it
+    if (c->g2_maxlen > c->dense_len)
+        BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<true>, dim3((unsigned)c->nplan), dim3(1024), lds,
+                                                c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
+                                                c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
+                                                c->ldx, c->diag, c->N1, c->dense_len));
+    else
+        BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<false>, dim3((unsigned)c->nplan), dim3(1024), lds,
+                                                c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
+                                                c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
+                                                c->ldx, c->diag, c->N1, c->dense_len));
+    if (c->nsplit > 0)
+        BM_LAUNCH(c, "diag_from_x", hipLaunchKernelGGL(k_diag_from_x, dim3((c->nsplit + 255) / 256), dim3(256), 0,
+                                                       c->stream, c->X, c->ldx, c->split_rows, c->nsplit, c->diag));
     c->diag_valid = true;
     return BM_OK;
 }
@@ -735,11 +870,49 @@ static int launch_gain_sparse(bm_ctx *c) {
 // sparse kernel when its preconditions hold, timed against the best dense
 // slab width, the faster one kept (both produce identical X).
 // BM_SWEEP_KERNEL = dense | sparse forces one.
+// Choices per class of instance (log2 n, log2 mean comb length, sparse
+// possible), shared by every context of the process: later graph pairs of a
+// class skip the timing runs.
+static std::mutex g_gain_mu;
+static std::map<std::tuple<int, int, int>, std::pair<int, int>> g_gain_cache;
+
+static int ilog2(long long x) {
+    int r = 0;
+    while ((1ll << r) < x) r++;
+    return r;
+}
+
+static int choose_gain_timed(bm_ctx *c);
+
 static int choose_gain(bm_ctx *c) {
+    if (c->gain_cols && (c->sweep_kind || !(c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax))) return BM_OK;
+    const bool can_sparse = c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
+    const bool forced = getenv("BM_GAIN_COLS") || getenv("BM_SWEEP_KERNEL");
+    const auto key = std::make_tuple(ilog2(c->N1), ilog2(std::max<long long>(1, c->ncomb / c->N1)), can_sparse ? 1 : 0);
+    if (!forced) {
+        std::lock_guard<std::mutex> g(g_gain_mu);
+        auto it = g_gain_cache.find(key);
+        if (it != g_gain_cache.end()) {
+            c->gain_cols = it->second.second;
+            if (can_sparse) c->sweep_kind = it->second.first;
+            return BM_OK;
+        }
+    }
+    int rc = choose_gain_timed(c);
+    if (rc) return rc;
+    if (!forced) {
+        std::lock_guard<std::mutex> g(g_gain_mu);
+        g_gain_cache[key] = std::make_pair(c->sweep_kind, c->gain_cols);
+    }
+    return BM_OK;
+}
+
+static int choose_gain_timed(bm_ctx *c) {
     int rc = tune_gain(c);
     if (rc) return rc;
     const bool can_sparse = c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
     if (!can_sparse) return BM_OK;
+    if ((rc = build_row_plan(c))) return rc;  // its host copies stay outside the timed launch
     if (c->sweep_kind) return BM_OK;
     const char *force = getenv("BM_SWEEP_KERNEL");
     if (force && !strcmp(force, "dense")) {
@@ -966,6 +1139,7 @@ int bm::sweep_prepare(bm_ctx *c) {
         if ((rc = dev_alloc(c, (void **)&c->X, (size_t)((c->N1 + 63) / 64 * 64) * c->ldx * 4))) return rc;
     }
     if ((rc = choose_gain(c))) return rc;
+    if (c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax && (rc = build_row_plan(c))) return rc;
     if (c->g2y_dirty && c->sparse_ok) {
         if ((rc = launch_refresh_g2y(c))) return rc;
     }

@@ -204,6 +204,42 @@ def test_all_pairs_sparse_kernel(case, monkeypatch):
     np.testing.assert_array_equal(got["row_best_b"][1:n], ref["row_best_b"][1:n])
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_all_pairs_sparse_kernel_split_rows(case, monkeypatch):
+    # rows longer than BM_SPARSE_SPLIT comb entries are spread over several
+    # blocks that add their partial rows into X with atomics
+    monkeypatch.setenv("BM_SWEEP_KERNEL", "sparse")
+    monkeypatch.setenv("BM_SPARSE_SPLIT", "8")
+    monkeypatch.setenv("BM_SPARSE_DENSE_LEN", "6")
+    n, g1, g2, mp = make_case(case, seed=29)
+    ref = ob.Oracle(n, g1, g2).all_pairs(mp, matrix=True)
+    with engine_for(n, g1, g2, mp) as e:
+        for _ in range(2):  # the split rows are zeroed again on every sweep
+            got = e.sweep_all_pairs(rows=True, matrix=True)
+            np.testing.assert_array_equal(got["matrix"], ref["matrix"])
+            assert got["best"] == ref["best"]
+
+
+@pytest.mark.parametrize("dense_len", ["100000", "20", "0"])
+def test_sparse_split_hub_rows_after_moves(dense_len, monkeypatch):
+    # split rows, and G2 lists longer than BM_SPARSE_DENSE_LEN streamed as whole Q rows
+    monkeypatch.setenv("BM_SWEEP_KERNEL", "sparse")
+    monkeypatch.setenv("BM_SPARSE_SPLIT", "100")
+    monkeypatch.setenv("BM_SPARSE_DENSE_LEN", dense_len)
+    inst = bmatch.Instance.powerlaw(37, 400, 9000)  # hubs far above 100 entries
+    o = ob.Oracle(inst.n, inst.g1, inst.g2)
+    cur = inst.p0.copy()
+    with engine_for(inst.n, inst.g1, inst.g2, inst.p0) as e:
+        for _ in range(4):
+            got = e.sweep_all_pairs(rows=True)
+            ref = o.all_pairs(cur)
+            assert got["best"] == ref["best"]
+            np.testing.assert_array_equal(got["row_best_delta"][1:inst.n], ref["row_best_delta"][1:inst.n])
+            a, b, d = ref["best"]
+            e.apply_swap(a, b)
+            cur[a], cur[b] = cur[b], cur[a]
+
+
 @pytest.mark.parametrize("kernel", ["sparse", "dense"])
 def test_global_best_after_moves_each_kernel(kernel, monkeypatch):
     # p^-1 must follow every applied swap, rotation and set permutation
