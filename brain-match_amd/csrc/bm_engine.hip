@@ -132,40 +132,36 @@ using namespace bm;
 
 // ---------------------------------------------------------------- kernels
 
-// W2[u][v] = w for the de-duplicated G2 lines (last line per pair wins).
-__global__ void k_scatter_w2(int16_t *__restrict__ W2, int ld, const int32_t *__restrict__ tri, int64_t m) {
+// W2[u][v] = W2T[v][u] = w for the de-duplicated G2 lines (last line per pair wins).
+__global__ void k_scatter_w2(int16_t *__restrict__ W2, int N1, int ld, const int32_t *__restrict__ tri, int64_t m) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     int u = tri[3 * i], v = tri[3 * i + 1], w = tri[3 * i + 2];
     W2[(size_t)u * ld + v] = (int16_t)w;
+    W2[((size_t)N1 + v) * ld + u] = (int16_t)w;
 }
 
-// P[x][y] = W2[p(x)][p(y)], padding columns zero.  grid (ld/256, N1)
-__global__ void k_build_p(int16_t *__restrict__ P, const int16_t *__restrict__ W2, const int32_t *__restrict__ map,
-                          int N1, int ld) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x;
-    int x = blockIdx.y;
-    if (y >= ld) return;
-    int16_t v = 0;
-    if (y < N1) v = W2[(size_t)map[x] * ld + map[y]];
-    P[(size_t)x * ld + y] = v;
-}
+typedef int16_t bm_h8 __attribute__((ext_vector_type(8)));
 
-// PT[x][y] = P[y][x] for x < N1, y < ld (P rows >= N1 read as 0).
-// 64x64 tiles staged in LDS; block (64, 4).
-__global__ void k_transpose(int16_t *__restrict__ PT, const int16_t *__restrict__ P, int N1, int ld) {
-    __shared__ int16_t tile[64][66];
-    int bx = blockIdx.x * 64;  // P column block == PT row block
-    int by = blockIdx.y * 64;  // P row block == PT column block
-    int tx = threadIdx.x, ty = threadIdx.y;
-    for (int i = ty; i < 64; i += 4) {
-        int y = by + i, x = bx + tx;
-        tile[i][tx] = (y < N1 && x < ld) ? P[(size_t)y * ld + x] : (int16_t)0;
-    }
-    __syncthreads();
-    for (int i = ty; i < 64; i += 4) {
-        int x = bx + i, y = by + tx;
-        if (x < N1 && y < ld) PT[(size_t)x * ld + y] = tile[tx][i];
+// Q = [P; P^T] from [W2; W2^T]: row x of half z is W[z][p(x)][p(y)] over y,
+// zero in the padding columns.  One block per (row, half): the source row is
+// read into one XCD's L2 once; 8 columns per thread, 16-byte stores.
+__global__ __launch_bounds__(256) void k_gather_rows(int16_t *__restrict__ Q, const int16_t *__restrict__ W2,
+                                                     const int32_t *__restrict__ map, int N1, int ld) {
+    const int x = blockIdx.x, z = blockIdx.y;
+    const int16_t *src = W2 + ((size_t)z * N1 + map[x]) * ld;
+    int16_t *dst = Q + ((size_t)z * N1 + x) * ld;
+    for (int y0 = threadIdx.x * 8; y0 < ld; y0 += 2048) {
+        bm_h8 v;
+        if (y0 + 8 <= N1) {
+            const int4 m0 = *(const int4 *)(map + y0), m1 = *(const int4 *)(map + y0 + 4);
+            v[0] = src[m0.x], v[1] = src[m0.y], v[2] = src[m0.z], v[3] = src[m0.w];
+            v[4] = src[m1.x], v[5] = src[m1.y], v[6] = src[m1.z], v[7] = src[m1.w];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = y0 + j < N1 ? src[map[y0 + j]] : (int16_t)0;
+        }
+        __builtin_nontemporal_store(v, (bm_h8 *)(dst + y0));
     }
 }
 
@@ -280,11 +276,8 @@ __global__ void k_perm_map(int32_t *__restrict__ map, int32_t *__restrict__ pinv
 namespace bm {
 
 int launch_rebuild_p(bm_ctx *c) {
-    int16_t *P = c->Q, *PT = c->Q + (size_t)c->N1 * c->ld;
-    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_build_p, dim3(c->ld / 256 + 1, c->N1), dim3(256), 0, c->stream, P,
-                                               c->W2, c->map, c->N1, c->ld));
-    BM_LAUNCH(c, "transpose", hipLaunchKernelGGL(k_transpose, dim3((c->N1 + 63) / 64, c->ld / 64), dim3(64, 4), 0,
-                                                 c->stream, PT, P, c->N1, c->ld));
+    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_gather_rows, dim3(c->N1, 2), dim3(256), 0, c->stream, c->Q, c->W2,
+                                               c->map, c->N1, c->ld));
     return BM_OK;
 }
 
@@ -411,7 +404,7 @@ int bm_create(int device, bm_ctx **out) {
 
 static void free_graph_buffers(bm_ctx *c) {
     size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
-    dev_free(c, (void **)&c->W2, N1 * ld * 2);
+    dev_free(c, (void **)&c->W2, 2 * N1 * ld * 2);
     dev_free(c, (void **)&c->Q, 2 * N1 * ld * 2);
     dev_free(c, (void **)&c->map, N1 * 4);
     dev_free(c, (void **)&c->cptr, (N1 + 1) * 4);
@@ -643,7 +636,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     // ---- device buffers
     size_t ld = (size_t)c->ld;
     int rc;
-    if ((rc = dev_alloc(c, (void **)&c->W2, (size_t)N1 * ld * 2))) goto fail;
+    if ((rc = dev_alloc(c, (void **)&c->W2, 2 * (size_t)N1 * ld * 2))) goto fail;
     if ((rc = dev_alloc(c, (void **)&c->Q, 2 * (size_t)N1 * ld * 2))) goto fail;
     if ((rc = dev_alloc(c, (void **)&c->map, (size_t)N1 * 4))) goto fail;
     if ((rc = dev_alloc(c, (void **)&c->diag, (size_t)N1 * 4))) goto fail;
@@ -695,7 +688,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = dev_alloc(c, (void **)&c->g2oy, (size_t)c->ng2 * 4))) goto fail;
     if ((rc = dev_alloc(c, (void **)&c->g2iy, (size_t)c->ng2 * 4))) goto fail;
     {
-        hipError_t e = hipMemsetAsync(c->W2, 0, (size_t)N1 * ld * 2, c->stream);
+        hipError_t e = hipMemsetAsync(c->W2, 0, 2 * (size_t)N1 * ld * 2, c->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "memset W2"); goto fail; }
         e = hipMemsetAsync(c->Q, 0, 2 * (size_t)N1 * ld * 2, c->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "memset Q"); goto fail; }
@@ -705,7 +698,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
             e = hipMemcpyAsync(c->stage, tri.data(), tri.size() * 4, hipMemcpyHostToDevice, c->stream);
             if (e != hipSuccess) { rc = hip_fail(e, "copy G2"); goto fail; }
             hipLaunchKernelGGL(k_scatter_w2, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, c->stream, c->W2,
-                               c->ld, (const int32_t *)c->stage, m2);
+                               N1, c->ld, (const int32_t *)c->stage, m2);
             if ((rc = launch_check("scatter_w2"))) goto fail;
         }
         e = hipStreamSynchronize(c->stream);
@@ -751,11 +744,8 @@ int bm_set_mapping(bm_ctx *c, const int32_t *mapping) {
     if (c->sparse_ok)
         BM_HIP(hipMemcpyAsync(c->pinv, c->hpinv.data(), (size_t)c->N1 * 4, hipMemcpyHostToDevice, c->stream));
     c->g2y_dirty = true;
-    int16_t *P = c->Q, *PT = c->Q + (size_t)c->N1 * c->ld;
-    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_build_p, dim3(c->ld / 256 + 1, c->N1), dim3(256), 0, c->stream, P,
-                                               c->W2, c->map, c->N1, c->ld));
-    BM_LAUNCH(c, "transpose", hipLaunchKernelGGL(k_transpose, dim3((c->N1 + 63) / 64, c->ld / 64), dim3(64, 4), 0,
-                                                 c->stream, PT, P, c->N1, c->ld));
+    int rc = launch_rebuild_p(c);
+    if (rc) return rc;
     BM_HIP(hipStreamSynchronize(c->stream));
     c->mapped = true;
     c->diag_valid = false;

@@ -1,7 +1,8 @@
 // Internal declarations of the MI355X move-delta engine (gfx950, wave64).
 //
 // Device-resident state of one context (one graph pair):
-//   W2   int16 [(n+1) x ld]      G2 dense adjacency, last line per pair wins
+//   W2   int16 [2(n+1) x ld]     rows 0..n: G2 dense adjacency (last line per
+//                                pair wins); rows n+1..2n+1: its transpose
 //   Q    int16 [2(n+1) x ld]     rows 0..n     : P[x][y]  = W2[p(x)][p(y)]
 //                                rows n+1..2n+1: PT[x][y] = P[y][x]
 //   comb lists, one per vertex v (out entries of v, then in entries of v):

@@ -78,7 +78,7 @@ struct bm_server {
 
 extern "C" int64_t bm_job_footprint(int32_t n, int64_t e1, int64_t e2, const bm_refine_params *p) {
     const int64_t N1 = (int64_t)n + 1, ld = (N1 + 63) / 64 * 64, xrows = (N1 + 63) / 64 * 64;
-    int64_t b = N1 * ld * 2 * 3     // W2, P, P^T
+    int64_t b = N1 * ld * 2 * 4     // W2, W2^T, P, P^T
                 + xrows * ld * 4    // all-pairs gain matrix
                 + 2 * e1 * 36       // comb lists (r, w, dw, (r,w)), de-duplicated view
                 + e1 * 16           // flat out-edge list

@@ -63,7 +63,8 @@ int bm_destroy(bm_ctx *ctx);
  * dense G1 cell and all of G2 take the last line for a pair, truncated to
  * `short` as the reference's adj_matrix does.  Vertex ids must lie in
  * [0, n]; G1 weights must be >= -32768.  Device footprint is about
- * 6*(n+1)^2 bytes + 40*|E1| bytes (+4*(n+1)^2 for bm_sweep_all_pairs).
+ * 8*(n+1)^2 bytes + 72*|E1| + 28*|E2| bytes (+4*(n+1)^2 for the all-pairs
+ * sweep); bm_job_footprint gives the bound the serving loop uses.
  */
 int bm_upload_graphs(bm_ctx *ctx, int32_t n,
                      const int32_t *g1_src, const int32_t *g1_dst, const int32_t *g1_w, int64_t e1,
