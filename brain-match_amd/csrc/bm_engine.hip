@@ -142,26 +142,54 @@ __global__ void k_scatter_w2(int16_t *__restrict__ W2, int N1, int ld, const int
 }
 
 typedef int16_t bm_h8 __attribute__((ext_vector_type(8)));
+// LDS limit of the row-gather kernel (160 KiB per CU on gfx950); one row of
+// ld <= 65536 int16 always fits
+constexpr size_t kGatherLdsMax = 160 * 1024;
 
-// Q = [P; P^T] from [W2; W2^T]: row x of half z is W[z][p(x)][p(y)] over y,
-// zero in the padding columns.  One block per (row, half): the source row is
-// read into one XCD's L2 once; 8 columns per thread, 16-byte stores.
-__global__ __launch_bounds__(256) void k_gather_rows(int16_t *__restrict__ Q, const int16_t *__restrict__ W2,
-                                                     const int32_t *__restrict__ map, int N1, int ld) {
-    const int x = blockIdx.x, z = blockIdx.y;
-    const int16_t *src = W2 + ((size_t)z * N1 + map[x]) * ld;
-    int16_t *dst = Q + ((size_t)z * N1 + x) * ld;
-    for (int y0 = threadIdx.x * 8; y0 < ld; y0 += 2048) {
-        bm_h8 v;
+// Q = [P; P^T] from [W2; W2^T]: P[x][y] = W2[p(x)][p(y)] and
+// P^T[x][y] = W2^T[p(x)][p(y)], zero in the padding columns.  One block per
+// x: the rows p(x) of W2 and W2^T are staged in LDS with 16-byte loads, then
+// both output rows are gathered from LDS through one read of the mapping,
+// 8 columns per thread, 16-byte stores.  (Gathering straight from global
+// memory puts every lane on its own cache line.)
+// Rows wider than fit twice in LDS take one half per block (grid y = 2,
+// `both` = 0): the same code with r1 / d1 aliased to the single half.
+__global__ __launch_bounds__(512) void k_gather_rows(int16_t *__restrict__ Q, const int16_t *__restrict__ W2,
+                                                     const int32_t *__restrict__ map, int N1, int ld, int both) {
+    extern __shared__ __align__(16) int16_t srow[];  // 2 * ld (both) or ld
+    const int x = blockIdx.x, tid = threadIdx.x, z = both ? 0 : blockIdx.y;
+    const int g = map[x];
+    const bm_h8 *s0 = (const bm_h8 *)(W2 + ((size_t)z * N1 + g) * ld);
+    const bm_h8 *s1 = (const bm_h8 *)(W2 + ((size_t)N1 + g) * ld);
+    bm_h8 *l0 = (bm_h8 *)srow, *l1 = (bm_h8 *)(srow + ld);
+    for (int i = tid; i < ld / 8; i += 512) {
+        l0[i] = s0[i];
+        if (both) l1[i] = s1[i];
+    }
+    __syncthreads();
+    const int16_t *r0 = srow, *r1 = both ? srow + ld : srow;
+    int16_t *d0 = Q + ((size_t)z * N1 + x) * ld, *d1 = both ? Q + ((size_t)N1 + x) * ld : d0;
+    for (int y0 = tid * 8; y0 < ld; y0 += 4096) {
+        bm_h8 v0, v1;
         if (y0 + 8 <= N1) {
             const int4 m0 = *(const int4 *)(map + y0), m1 = *(const int4 *)(map + y0 + 4);
-            v[0] = src[m0.x], v[1] = src[m0.y], v[2] = src[m0.z], v[3] = src[m0.w];
-            v[4] = src[m1.x], v[5] = src[m1.y], v[6] = src[m1.z], v[7] = src[m1.w];
+            const int m[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                v0[j] = r0[m[j]];
+                v1[j] = r1[m[j]];
+            }
         } else {
 #pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = y0 + j < N1 ? src[map[y0 + j]] : (int16_t)0;
+            for (int j = 0; j < 8; j++) {
+                const bool in = y0 + j < N1;
+                const int m = in ? map[y0 + j] : 0;
+                v0[j] = in ? r0[m] : (int16_t)0;
+                v1[j] = in ? r1[m] : (int16_t)0;
+            }
         }
-        __builtin_nontemporal_store(v, (bm_h8 *)(dst + y0));
+        __builtin_nontemporal_store(v0, (bm_h8 *)(d0 + y0));
+        if (both) __builtin_nontemporal_store(v1, (bm_h8 *)(d1 + y0));
     }
 }
 
@@ -276,8 +304,16 @@ __global__ void k_perm_map(int32_t *__restrict__ map, int32_t *__restrict__ pinv
 namespace bm {
 
 int launch_rebuild_p(bm_ctx *c) {
-    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_gather_rows, dim3(c->N1, 2), dim3(256), 0, c->stream, c->Q, c->W2,
-                                               c->map, c->N1, c->ld));
+    // both source rows in LDS when they fit (BM_GATHER_SPLIT=1 forces one row per block)
+    const int both = (size_t)c->ld * 4 <= kGatherLdsMax && !getenv("BM_GATHER_SPLIT") ? 1 : 0;
+    const size_t lds = (size_t)c->ld * 2 * (both ? 2 : 1);
+    if (!c->gather_attr) {  // once per context, outside any capture
+        BM_HIP(hipFuncSetAttribute((const void *)k_gather_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kGatherLdsMax));
+        c->gather_attr = true;
+    }
+    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_gather_rows, dim3(c->N1, both ? 1 : 2), dim3(512), lds, c->stream,
+                                               c->Q, c->W2, c->map, c->N1, c->ld, both));
     return BM_OK;
 }
 

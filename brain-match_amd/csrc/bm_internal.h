@@ -86,6 +86,7 @@ struct bm_ctx {
     int gain_cols = 0;        // all-pairs slab width per lane (1, 2, 4), 0 = not tuned yet
     float gain_ms[3] = {0, 0, 0};  // autotuner timings of widths 1, 2, 4
     bool sparse_attr = false;      // dynamic-LDS attribute of the sparse gain kernel set
+    bool gather_attr = false;      // dynamic-LDS attribute of the P / P^T rebuild kernel set
     int4 *plan = nullptr;          // sparse-kernel row plan: (row, entry lo, entry hi, flags)
     int64_t nplan = 0;
     int32_t *split_rows = nullptr; // rows the plan splits over several blocks

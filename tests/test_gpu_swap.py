@@ -56,6 +56,19 @@ def test_score_and_pairs(case):
         np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("case", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_p_rebuild_one_row_per_block(case, monkeypatch):
+    # the P / P^T rebuild for rows too wide to stage twice in LDS
+    monkeypatch.setenv("BM_GATHER_SPLIT", "1")
+    n, g1, g2, mp = make_case(case, seed=41)
+    o = ob.Oracle(n, g1, g2)
+    with engine_for(n, g1, g2, mp) as e:
+        assert e.score() == o.score_opt(mp)
+        ref = o.all_pairs(mp, matrix=True)
+        got = e.sweep_all_pairs(matrix=True)
+        np.testing.assert_array_equal(got["matrix"], ref["matrix"])
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_all_pairs_matrix(case):
     n, g1, g2, mp = make_case(case)
