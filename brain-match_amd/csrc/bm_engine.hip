@@ -223,16 +223,41 @@ __global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ 
     }
 }
 
-// X(v,v) for the vertices whose diagonal a move on `set` changed: the set
-// itself and every G1 neighbour of it (the Q rows/columns that moved are
-// exactly those of the set).  One wave per touched vertex: wave w of the grid
-// maps through the prefix `off` to set vertex i and its entry w - off[i]
-// (entry -1 is the set vertex itself); each recomputes diag[v] over comb(v).
+// X(v,v) after a move on a vertex set S (new p(S[i]) = old p(S[src[i]])).
+// For v outside S only the entries of comb(v) that are G1 lines to S change,
+// so they are patched from comb(S), before the permutation, with old and
+// new cells both read from the old P (the new row of S[i] is the old row of
+// S[src[i]]):
+//   line s -> v:  diag[v] += min(w, P[S[src]][v]) - min(w, P[s][v])
+//   line v -> s:  diag[v] += min(w, P[v][S[src]]) - min(w, P[v][s])
+// The vertices of S are recomputed over their whole lists afterwards.
 struct TouchSet {
     int k;
     int v[8];
-    int off[9];  // prefix of (1 + list length) over the set
+    int src[8];  // the old row / column that S[i] takes
+    int off[9];  // prefix of the comb-list lengths over the set
 };
+
+__global__ __launch_bounds__(256) void k_diag_delta(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
+                                                    const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
+                                                    const int16_t *__restrict__ Q, int N1, int ld, TouchSet ts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ts.off[ts.k]) return;
+    int i = 0;
+    while (t >= ts.off[i + 1]) i++;
+    const int s = ts.v[i], s2 = ts.v[ts.src[i]];
+    if (s2 == s) return;
+    const int e = cptr[s] + (t - ts.off[i]);
+    const int r = cr[e], w = cw[e];
+    const bool out = r >= N1;
+    const int v = out ? r - N1 : r;
+    for (int q = 0; q < ts.k; q++)
+        if (ts.v[q] == v) return;  // a line inside S: v is recomputed in full
+    const int o = out ? (int)Q[(size_t)s * ld + v] : (int)Q[(size_t)v * ld + s];
+    const int nw = out ? (int)Q[(size_t)s2 * ld + v] : (int)Q[(size_t)v * ld + s2];
+    const int d = min(w, nw) - min(w, o);
+    if (d) atomicAdd(&diag[v], d);
+}
 
 __device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restrict__ cptr,
                                        const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
@@ -243,21 +268,15 @@ __device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restric
     return s;
 }
 
-__global__ __launch_bounds__(256) void k_diag_touch(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
-                                                    const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
-                                                    const int16_t *__restrict__ Q, int N1, int ld, TouchSet ts) {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (w >= ts.off[ts.k]) return;
-    int i = 0;
-    while (w >= ts.off[i + 1]) i++;
-    const int s = ts.v[i], j = w - ts.off[i] - 1;
-    int v = s;
-    if (j >= 0) {
-        const int r = cr[cptr[s] + j];
-        v = r >= N1 ? r - N1 : r;
-    }
+// one wave per set vertex: its whole list, after the permutation
+__global__ __launch_bounds__(512) void k_diag_set(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
+                                                  const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
+                                                  const int16_t *__restrict__ Q, int ld, TouchSet ts) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (w >= ts.k) return;
+    const int v = ts.v[w];
     const int d = diag_of(v, lane, cptr, cr, cw, Q, ld);
-    if (lane == 0) diag[v] = d;  // a vertex reached twice gets the same value
+    if (lane == 0) diag[v] = d;
 }
 
 struct SetPerm {
@@ -342,21 +361,27 @@ int ensure_diag(bm_ctx *c) {
     return BM_OK;
 }
 
-int touch_diag(bm_ctx *c, int k, const int32_t *set) {
-    TouchSet ts;
+static void touch_set(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of, TouchSet &ts) {
     memset(&ts, 0, sizeof ts);
     ts.k = k;
     for (int i = 0; i < k; i++) {
         ts.v[i] = set[i];
-        ts.off[i + 1] = ts.off[i] + 1 + c->hcdeg[set[i]];
+        ts.src[i] = src_of[i];
+        ts.off[i + 1] = ts.off[i] + c->hcdeg[set[i]];
     }
-    const int waves = ts.off[k];
-    BM_LAUNCH(c, "diag_touch", hipLaunchKernelGGL(k_diag_touch, dim3((waves + 3) / 4), dim3(256), 0, c->stream,
-                                                  c->diag, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, ts));
-    return BM_OK;
 }
 
 int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of) {
+    TouchSet ts;
+    const bool diag = c->diag_valid;  // keep X(v,v) current where it changes
+    if (diag) {
+        touch_set(c, k, set, src_of, ts);
+        const int m = ts.off[k];
+        if (m > 0)
+            BM_LAUNCH(c, "diag_delta", hipLaunchKernelGGL(k_diag_delta, dim3((m + 255) / 256), dim3(256), 0,
+                                                          c->stream, c->diag, c->cptr, c->cr, c->cw, c->Q, c->N1,
+                                                          c->ld, ts));
+    }
     SetPerm sp;
     memset(&sp, 0, sizeof sp);
     sp.k = k;
@@ -376,10 +401,9 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
         for (int i = 0; i < k; i++)
             if (old[i] != 0) c->hpinv[old[i]] = set[i];
     c->g2y_dirty = true;
-    if (c->diag_valid) {  // keep X(v,v) current where it changed
-        int rc = touch_diag(c, k, set);
-        if (rc) return rc;
-    }
+    if (diag)
+        BM_LAUNCH(c, "diag_set", hipLaunchKernelGGL(k_diag_set, dim3(1), dim3(64 * k), 0, c->stream, c->diag,
+                                                    c->cptr, c->cr, c->cw, c->Q, c->ld, ts));
     return BM_OK;
 }
 

@@ -149,8 +149,6 @@ int timer_begin(bm_ctx *c, const char *name);
 void timer_end(bm_ctx *c, int token);
 
 int ensure_diag(bm_ctx *c);
-// after a move on `set`: recompute X(v,v) for the set and its G1 neighbours
-int touch_diag(bm_ctx *c, int k, const int32_t *set);
 int launch_check(const char *what);
 
 // launchers shared across translation units (device pointers, context stream)
