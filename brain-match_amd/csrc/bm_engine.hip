@@ -279,6 +279,39 @@ __global__ __launch_bounds__(512) void k_diag_set(int32_t *__restrict__ diag, co
     if (lane == 0) diag[v] = d;
 }
 
+// The translated G2 lists (target as p^-1(target), 0xFFFF when unmapped)
+// after a move: only entries whose target is one of the moved images change;
+// they are the G2 lines at those images, found through the cross positions.
+struct ImageSet {
+    int k;
+    int g[8];
+    int off[9];  // prefix of (in-degree + out-degree) over the images
+};
+
+__global__ __launch_bounds__(256) void k_g2y_patch(uint32_t *__restrict__ g2oy, uint32_t *__restrict__ g2iy,
+                                                   const uint32_t *__restrict__ g2o, const uint32_t *__restrict__ g2i,
+                                                   const int32_t *__restrict__ g2optr,
+                                                   const int32_t *__restrict__ g2iptr,
+                                                   const int32_t *__restrict__ g2o_at_in,
+                                                   const int32_t *__restrict__ g2i_at_out,
+                                                   const int32_t *__restrict__ pinv, ImageSet is) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= is.off[is.k]) return;
+    int i = 0;
+    while (t >= is.off[i + 1]) i++;
+    const int g = is.g[i], l = t - is.off[i];
+    const int y = pinv[g];
+    const uint32_t hi = (uint32_t)(y < 0 ? 0xFFFF : y) << 16;
+    const int nin = g2iptr[g + 1] - g2iptr[g];
+    if (l < nin) {  // line u -> g: entry of u's out-list
+        const int j = g2o_at_in[g2iptr[g] + l];
+        g2oy[j] = hi | (g2o[j] & 0xFFFFu);
+    } else {        // line g -> v: entry of v's in-list
+        const int j = g2i_at_out[g2optr[g] + l - nin];
+        g2iy[j] = hi | (g2i[j] & 0xFFFFu);
+    }
+}
+
 struct SetPerm {
     int k;
     int set[8];
@@ -400,7 +433,25 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
     if (c->sparse_ok)
         for (int i = 0; i < k; i++)
             if (old[i] != 0) c->hpinv[old[i]] = set[i];
-    c->g2y_dirty = true;
+    if (c->sparse_ok && !c->g2y_dirty) {
+        // keep the translated G2 lists current: the lines at the moved images
+        ImageSet is;
+        memset(&is, 0, sizeof is);
+        for (int i = 0; i < k; i++) {
+            const int g = old[i];
+            if (g == 0) continue;
+            is.g[is.k] = g;
+            is.off[is.k + 1] = is.off[is.k] + (c->hg2iptr[g + 1] - c->hg2iptr[g]) + (c->hg2optr[g + 1] - c->hg2optr[g]);
+            is.k++;
+        }
+        const int m = is.off[is.k];
+        if (m > 0)
+            BM_LAUNCH(c, "g2y_patch", hipLaunchKernelGGL(k_g2y_patch, dim3((m + 255) / 256), dim3(256), 0, c->stream,
+                                                         c->g2oy, c->g2iy, c->g2o, c->g2i, c->g2optr, c->g2iptr,
+                                                         c->g2o_at_in, c->g2i_at_out, c->pinv, is));
+    } else {
+        c->g2y_dirty = true;
+    }
     if (diag)
         BM_LAUNCH(c, "diag_set", hipLaunchKernelGGL(k_diag_set, dim3(1), dim3(64 * k), 0, c->stream, c->diag,
                                                     c->cptr, c->cr, c->cw, c->Q, c->ld, ts));
@@ -488,6 +539,8 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->g2iptr, (N1 + 1) * 4);
     dev_free(c, (void **)&c->g2o, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->g2i, (size_t)c->ng2 * 4);
+    dev_free(c, (void **)&c->g2o_at_in, (size_t)c->ng2 * 4);
+    dev_free(c, (void **)&c->g2i_at_out, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->pinv, N1 * 4);
     dev_free(c, (void **)&c->g2oy, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->g2iy, (size_t)c->ng2 * 4);
@@ -669,6 +722,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     // ---- G2 out- and in-lists of the de-duplicated lines, packed (vertex << 16 | weight)
     std::vector<int32_t> g2optr(N1 + 1, 0), g2iptr(N1 + 1, 0);
     std::vector<uint32_t> g2o, g2i;
+    std::vector<int32_t> g2o_at_in, g2i_at_out;
     {
         const int64_t m2 = (int64_t)tri.size() / 3;
         c->g2_vertex0 = false;
@@ -684,11 +738,16 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         std::vector<int32_t> po(g2optr.begin(), g2optr.end() - 1), pi(g2iptr.begin(), g2iptr.end() - 1);
         g2o.resize(m2);
         g2i.resize(m2);
+        g2o_at_in.resize(m2);
+        g2i_at_out.resize(m2);
         for (int64_t i = 0; i < m2; i++) {
             const uint32_t u = (uint32_t)tri[3 * i], v = (uint32_t)tri[3 * i + 1];
             const uint32_t wv = (uint16_t)(int16_t)tri[3 * i + 2];
-            g2o[po[u]++] = (v << 16) | wv;
-            g2i[pi[v]++] = (u << 16) | wv;
+            const int jo = po[u]++, ji = pi[v]++;
+            g2o[jo] = (v << 16) | wv;
+            g2i[ji] = (u << 16) | wv;
+            g2o_at_in[ji] = jo;  // the same line in the other list
+            g2i_at_out[jo] = ji;
         }
         c->ng2 = m2;
     }
@@ -734,6 +793,8 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = upload(c, &c->g2iptr, g2iptr))) goto fail;
     if ((rc = upload(c, &c->g2o, g2o))) goto fail;
     if ((rc = upload(c, &c->g2i, g2i))) goto fail;
+    if ((rc = upload(c, &c->g2o_at_in, g2o_at_in))) goto fail;
+    if ((rc = upload(c, &c->g2i_at_out, g2i_at_out))) goto fail;
     {
         // vertices on which every delta kind equals the exact score change:
         // no self-loop line and no repeated (u,v) line touches them
@@ -767,6 +828,8 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     c->g2_maxlen = 0;
     for (int v = 0; v < N1; v++)
         c->g2_maxlen = std::max(c->g2_maxlen, std::max(g2optr[v + 1] - g2optr[v], g2iptr[v + 1] - g2iptr[v]));
+    c->hg2optr = g2optr;
+    c->hg2iptr = g2iptr;
     c->hmap.assign(N1, 0);
     c->hcdeg.resize(N1);
     c->hg2deg.resize(N1);

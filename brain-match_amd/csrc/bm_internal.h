@@ -76,6 +76,8 @@ struct bm_ctx {
     bool g2_vertex0 = false;  // a G2 line names vertex 0
     int32_t *pinv = nullptr;  // pinv[g] = the vertex mapped to g (-1 if none); valid when sparse_ok
     uint32_t *g2oy = nullptr, *g2iy = nullptr;  // the G2 lists with targets as P coordinates (p^-1), 0xFFFF = none
+    int32_t *g2o_at_in = nullptr;   // per in-list entry (line u->g): its index in the out-lists
+    int32_t *g2i_at_out = nullptr;  // per out-list entry (line g->v): its index in the in-lists
     bool g2y_dirty = true;
     bool sparse_ok = false;   // p injective on mapped vertices and !g2_vertex0
     int sweep_kind = 0;       // all-pairs gain kernel: 0 = not tuned, 1 = dense, 2 = sparse
@@ -115,6 +117,7 @@ struct bm_ctx {
     std::vector<int32_t> hpinv;
     std::vector<int32_t> hcdeg;   // comb-list length per vertex
     std::vector<int32_t> hg2deg;  // G2 in- plus out-degree per G2 vertex (de-duplicated)
+    std::vector<int32_t> hg2optr, hg2iptr;  // host copies of the G2 list offsets
 
     // kernel timing
     bool timing = false;
