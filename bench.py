@@ -7,12 +7,16 @@ synthetic n = 18,524 / 3.0M-edge graph pair, everything resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
                     [--cpu-seconds S | --no-cpu-baseline] [--calibrate-copy MiB]
+                    [--workload NAME] [--window S] [--jobs J]
 
 Prints one JSON line: metric/value (Delta_B evaluations per second), the
-roofline of the dominant kernel (k_gain) with its average launch time
-measured by HIP events on the engine's stream, and the CPU baseline (the
-oracle's restatement of the reference OpenMP row-best loop,
-alignment-omp.c:598-622, timed on this host on a bounded row sample).
+roofline of the dominant kernel (the all-pairs gain kernel the chooser
+picked) with its average launch time measured by HIP events on the engine's
+stream, and the CPU baseline (the oracle's restatement of the reference
+OpenMP row-best loop, alignment-omp.c:598-622, timed on this host on a
+bounded row sample).  --workload selects the other measurements: plain,
+plain-c2, rot3, permset, batched, batched-c1, batched-permset, serve
+(bmatch/bench_workloads.py).
 """
 import argparse
 import json
@@ -150,7 +154,10 @@ def main():
         "best_move": {"a": best[0], "b": best[1], "delta": best[2]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "kernel_avg_ms": gain_avg_ms, "algorithmic_bytes_per_launch": alg},
+                     "kernel": kname, "kernel_avg_ms": gain_avg_ms, "algorithmic_bytes_per_launch": alg,
+                     # the bytes the kernel really moves past L2 (PMC, profiles/), per second of this run
+                     "traffic_gbs": traffic / (gain_avg_ms / 1000.0) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (gain_avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS if traffic else None},
     }
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s)", file=sys.stderr, flush=True)
