@@ -23,6 +23,14 @@ __device__ __forceinline__ int qv(const int16_t *__restrict__ Q, int row, int ld
     return (int)Q[(size_t)row * (size_t)ld + col];
 }
 
+// Q[r][m] read from row m of the other half: Q[n+1+d][m] = P^T[d][m] = P[m][d]
+// and Q[s][m] = P[s][m] = P^T[m][s].  A move kernel reads column m_j for many
+// neighbour rows r; this way those reads stay inside the rows m_j (a few KB
+// per move vertex) instead of touching one cache line per neighbour.
+__device__ __forceinline__ int qcol(const int16_t *__restrict__ Q, int r, int N1, int ld, int m) {
+    return r >= N1 ? (int)Q[(size_t)m * (size_t)ld + (r - N1)] : (int)Q[((size_t)N1 + m) * (size_t)ld + r];
+}
+
 // permutation table of triple.cu:85-92: new image of m_k = old image of m_tab[p][k]
 __constant__ int kG3Tab[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
@@ -60,9 +68,9 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
                     }
                     continue;
                 }
-                const int x0 = min(w, qv(Q, r, ld, m[0]));
-                const int x1 = min(w, qv(Q, r, ld, m[1]));
-                const int x2 = min(w, qv(Q, r, ld, m[2]));
+                const int x0 = min(w, qcol(Q, r, N1, ld, m[0]));
+                const int x1 = min(w, qcol(Q, r, N1, ld, m[1]));
+                const int x2 = min(w, qcol(Q, r, N1, ld, m[2]));
                 const int xs[3] = {x0, x1, x2};
                 const int old = xs[k];
 #pragma unroll
@@ -155,9 +163,9 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
                 }
                 continue;
             }
-            const int old = min(w, qv(Q, r, ld, v));
+            const int old = min(w, qcol(Q, r, N1, ld, v));
             for (int j = 0; j < k; j++)
-                if (j != i) atomicAdd(&C[i][j], min(w, qv(Q, r, ld, S[j])) - old);
+                if (j != i) atomicAdd(&C[i][j], min(w, qcol(Q, r, N1, ld, S[j])) - old);
         }
     }
     __syncthreads();
