@@ -557,6 +557,8 @@ static void free_graph_buffers(bm_ctx *c) {
     c->gain_cols = 0;
     c->sweep_kind = 0;
     c->sparse_ok = false;
+    c->rot3_tuned = false;
+    c->permset_threads_k = 0;
 }
 
 int bm_destroy(bm_ctx *c) {

@@ -97,6 +97,8 @@ struct bm_ctx {
     int g2_maxlen = 0;             // longest de-duplicated G2 out- or in-list
     int rot3_lanes = 16;           // lanes per triple of the 3-vertex kernel (8, 16, 32)
     int permset_threads = 256;     // workgroup size of the k-set kernel (64, 256)
+    bool rot3_tuned = false;       // rot3_lanes chosen by the autotuner for this graph pair
+    int permset_threads_k = 0;     // the k permset_threads was tuned for (0: none)
     uint8_t *vsimple = nullptr;    // 1: no self-loop and no repeated line touches the vertex
     unsigned long long *keys = nullptr;  // N1 + 8 reduction slots
     void *stage = nullptr;               // move-list staging (lazy, grows)
@@ -160,6 +162,9 @@ int launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, 
 // mult = 2: Delta_P as the reference (intra-set lines twice); 1: the exact score change
 int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult = 2);
 int free_moves(bm_ctx *c);
+// autotuner of the move kernels: times the variants on the given device list
+int tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP);
+int tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult);
 int free_batch(bm_ctx *c);
 int launch_rebuild_p(bm_ctx *c);        // P and P^T from W2 and the device mapping
 int launch_diag(bm_ctx *c);             // X(v,v) for every v

@@ -13,7 +13,12 @@
 //   k_hillclimb: Delta_A of refine-matching-hillclimb.c:244-296.
 #include <limits.h>
 
+#include <stdlib.h>
+
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "bm_internal.h"
 
@@ -33,6 +38,9 @@ __device__ __forceinline__ int qcol(const int16_t *__restrict__ Q, int r, int N1
 
 // permutation table of triple.cu:85-92: new image of m_k = old image of m_tab[p][k]
 __constant__ int kG3Tab[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+// the same table as a compile-time constant, for indices the unrolled loops fix
+constexpr int kG3Perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+constexpr int kRot3Unroll = 4;
 
 template <int kLanes>
 __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int64_t count, int mode,
@@ -48,33 +56,57 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
         m[0] = T[3 * t];
         m[1] = T[3 * t + 1];
         m[2] = T[3 * t + 2];
+        // rows m_j of P (out entries) and of P^T (in entries): every column read stays in them
+        const int16_t *rp[3], *rt[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            rp[j] = Q + (size_t)m[j] * ld;
+            rt[j] = Q + ((size_t)N1 + m[j]) * ld;
+        }
+#pragma unroll
         for (int k = 0; k < 3; k++) {
             const int v = m[k];
-            for (int e = ddptr[v] + gl; e < ddptr[v + 1]; e += kLanes) {
-                const int r = ddr[e];
-                const int w = (int)(int16_t)ddw[e];  // the dense G1 cell
-                if (w <= 0) continue;
-                const bool out = r >= N1;
-                const int nb = out ? r - N1 : r;
-                if (nb == 0) continue;  // the dense scan runs over i = 1..n
-                int j = nb == m[0] ? 0 : nb == m[1] ? 1 : nb == m[2] ? 2 : -1;
-                if (j >= 0) {
-                    // intra-triple cell m_k -> m_j (:164-207), from the out entry only
-                    if (out && j != k) {
-                        const int old = min(w, qv(Q, m[k], ld, m[j]));
+            const int e1 = ddptr[v + 1];
+            // kRot3Unroll entries per lane in flight: their list words, then their 3 column reads each
+            for (int eb = ddptr[v] + gl; eb < e1; eb += kLanes * kRot3Unroll) {
+                int r[kRot3Unroll], w[kRot3Unroll], q[kRot3Unroll][3];
 #pragma unroll
-                        for (int p = 1; p < 6; p++)
-                            s[p] += min(w, qv(Q, m[kG3Tab[p][k]], ld, m[kG3Tab[p][j]])) - old;
-                    }
-                    continue;
+                for (int u = 0; u < kRot3Unroll; u++) {
+                    const int e = eb + u * kLanes;
+                    const bool in = e < e1;
+                    r[u] = in ? ddr[e] : N1;  // past the list: reads column 0, contributes nothing
+                    w[u] = in ? (int)(int16_t)ddw[e] : 0;  // the dense G1 cell
                 }
-                const int x0 = min(w, qcol(Q, r, N1, ld, m[0]));
-                const int x1 = min(w, qcol(Q, r, N1, ld, m[1]));
-                const int x2 = min(w, qcol(Q, r, N1, ld, m[2]));
-                const int xs[3] = {x0, x1, x2};
-                const int old = xs[k];
 #pragma unroll
-                for (int p = 1; p < 6; p++) s[p] += xs[kG3Tab[p][k]] - old;
+                for (int u = 0; u < kRot3Unroll; u++) {
+                    const bool out = r[u] >= N1;
+                    const int col = out ? r[u] - N1 : r[u];
+#pragma unroll
+                    for (int j = 0; j < 3; j++) q[u][j] = (int)(out ? rp[j] : rt[j])[col];
+                }
+#pragma unroll
+                for (int u = 0; u < kRot3Unroll; u++) {
+                    const int wu = w[u];
+                    if (wu <= 0) continue;
+                    const bool out = r[u] >= N1;
+                    const int nb = out ? r[u] - N1 : r[u];
+                    if (nb == 0) continue;  // the dense scan runs over i = 1..n
+                    const int j = nb == m[0] ? 0 : nb == m[1] ? 1 : nb == m[2] ? 2 : -1;
+                    if (j >= 0) {
+                        // intra-triple cell m_k -> m_j (:164-207), from the out entry only
+                        if (out && j != k) {
+                            const int old = min(wu, qv(Q, m[k], ld, m[j]));
+#pragma unroll
+                            for (int p = 1; p < 6; p++)
+                                s[p] += min(wu, qv(Q, m[kG3Tab[p][k]], ld, m[kG3Tab[p][j]])) - old;
+                        }
+                        continue;
+                    }
+                    const int xs[3] = {min(wu, q[u][0]), min(wu, q[u][1]), min(wu, q[u][2])};
+                    const int old = xs[k];
+#pragma unroll
+                    for (int p = 1; p < 6; p++) s[p] += xs[kG3Perm[p][k]] - old;
+                }
             }
         }
     }
@@ -97,6 +129,8 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
 }
 
 constexpr int kMaxIntra = 256;
+constexpr int kTuneMinTriples = 4096;  // lists shorter than this use the current variant untimed
+constexpr int kTuneMinSets = 256;
 
 __device__ __forceinline__ bool next_perm(int *a, int k) {
     int i = k - 2;
@@ -288,6 +322,96 @@ int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *
     return BM_OK;
 }
 
+// ------------------------------------------------------------ autotuner
+//
+// Lanes per triple (8 / 16 / 32, i.e. 8, 4 or 2 triples per wavefront) and
+// threads per set (64 / 256) are timed with HIP events on the caller's own
+// candidate list (one warm-up and three timed launches per variant) and the
+// fastest is kept per instance class (log2 n, log2 mean list length, k) for
+// the whole process.  Every variant computes the same integers.  The
+// environment variables BM_ROT3_LANES / BM_PERMSET_THREADS force a variant.
+
+static std::mutex g_mv_mu;
+static std::map<std::tuple<int, int, int, int>, int> g_mv_tune;
+
+static int log2c(long long x) {
+    int r = 0;
+    while ((1ll << r) < x) r++;
+    return r;
+}
+
+template <class F>
+static int time_launches(bm_ctx *c, F launch, float *ms) {
+    hipEvent_t e0, e1;
+    BM_HIP(hipEventCreate(&e0));
+    BM_HIP(hipEventCreate(&e1));
+    const bool was = c->timing;
+    c->timing = false;
+    int rc = launch();
+    if (!rc) {
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < 3 && !rc; r++) rc = launch();
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(ms, e0, e1);
+    }
+    c->timing = was;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+template <class Set, class Launch>
+static int tune_variant(bm_ctx *c, const char *env, std::tuple<int, int, int, int> key, const int *choices, int nch,
+                        Set set, Launch launch) {
+    if (const char *f = getenv(env)) {
+        set(atoi(f));
+        return BM_OK;
+    }
+    {
+        std::lock_guard<std::mutex> g(g_mv_mu);
+        auto it = g_mv_tune.find(key);
+        if (it != g_mv_tune.end()) {
+            set(it->second);
+            return BM_OK;
+        }
+    }
+    float best_ms = 1e30f;
+    int best = choices[0];
+    for (int i = 0; i < nch; i++) {
+        set(choices[i]);
+        float ms = 0.f;
+        int rc = time_launches(c, launch, &ms);
+        if (rc) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = choices[i];
+        }
+    }
+    set(best);
+    std::lock_guard<std::mutex> g(g_mv_mu);
+    g_mv_tune[key] = best;
+    return BM_OK;
+}
+
+int bm::tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP) {
+    static const int ch[3] = {8, 16, 32};
+    const auto key = std::make_tuple(1, log2c(c->n), log2c(std::max<long long>(1, c->ndd / c->N1)), 3);
+    int rc = tune_variant(c, "BM_ROT3_LANES", key, ch, 3, [&](int v) { c->rot3_lanes = v; },
+                          [&] { return launch_rot3(c, dT, m, BM_ROT3_ALL, dD, dP); });
+    if (!rc) c->rot3_tuned = true;
+    return rc;
+}
+
+int bm::tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult) {
+    static const int ch[2] = {64, 256};
+    const auto key = std::make_tuple(2, log2c(c->n), log2c(std::max<long long>(1, c->ncomb / c->N1)), k);
+    int rc = tune_variant(c, "BM_PERMSET_THREADS", key, ch, 2, [&](int v) { c->permset_threads = v; },
+                          [&] { return launch_permset(c, k, dS, m, dD, dR, dOv, mult); });
+    if (!rc) c->permset_threads_k = k;
+    return rc;
+}
+
 // ------------------------------------------------------------ C ABI
 
 extern "C" int bm_eval_rot3(bm_ctx *c, const int32_t *triples, int64_t count, int mode, int32_t *best_delta,
@@ -312,6 +436,7 @@ extern "C" int bm_eval_rot3(bm_ctx *c, const int32_t *triples, int64_t count, in
     for (int64_t off = 0; off < count; off += chunk) {
         const int64_t m = std::min(chunk, count - off);
         BM_HIP(hipMemcpyAsync(dT, triples + 3 * off, m * 12, hipMemcpyHostToDevice, c->stream));
+        if (!c->rot3_tuned && m >= kTuneMinTriples && (rc = tune_rot3(c, dT, m, dD, dP))) return rc;
         if ((rc = launch_rot3(c, dT, m, mode, dD, dP))) return rc;
         BM_HIP(hipMemcpyAsync(best_delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipMemcpyAsync(best_perm + off, dP, m * 4, hipMemcpyDeviceToHost, c->stream));
@@ -345,6 +470,8 @@ extern "C" int bm_eval_permset(bm_ctx *c, int32_t k, const int32_t *sets, int64_
     for (int64_t off = 0; off < count; off += chunk) {
         const int64_t m = std::min(chunk, count - off);
         BM_HIP(hipMemcpyAsync(dS, sets + (size_t)k * off, (size_t)m * k * 4, hipMemcpyHostToDevice, c->stream));
+        if (c->permset_threads_k != k && m >= kTuneMinSets && (rc = tune_permset(c, k, dS, m, dD, dR, dOv, 2)))
+            return rc;
         if ((rc = launch_permset(c, k, dS, m, dD, dR, dOv))) return rc;
         BM_HIP(hipMemcpyAsync(best_delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipMemcpyAsync(best_rank + off, dR, m * 4, hipMemcpyDeviceToHost, c->stream));

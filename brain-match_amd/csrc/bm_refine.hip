@@ -339,16 +339,6 @@ __global__ void k_state_init(RefineState *st) {
     st->gen_step = g;
 }
 
-// process-wide choices of the autotuner, per class of instance
-std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int>, int> g_tune;
-
-int log2i(long long x) {
-    int r = 0;
-    while ((1ll << r) < x) r++;
-    return r;
-}
-
 }  // namespace
 
 struct bm_batch_state {
@@ -523,99 +513,26 @@ static int capture(bm_ctx *c) {
     return BM_OK;
 }
 
-// time `reps` launches of the current variant (eager, events on the stream)
-template <class F>
-static int time_variant(bm_ctx *c, F launch, float *ms) {
-    hipEvent_t e0, e1;
-    BM_HIP(hipEventCreate(&e0));
-    BM_HIP(hipEventCreate(&e1));
-    int rc = launch();  // warm
-    if (!rc) {
-        (void)hipEventRecord(e0, c->stream);
-        for (int r = 0; r < 3 && !rc; r++) rc = launch();
-        (void)hipEventRecord(e1, c->stream);
-        (void)hipEventSynchronize(e1);
-        (void)hipEventElapsedTime(ms, e0, e1);
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return rc;
-}
 
-static int env_int(const char *name, int dflt) {
-    const char *s = getenv(name);
-    return s && *s ? atoi(s) : dflt;
-}
 
-// Picks lanes per triple and threads per set: timed on this instance's own
-// generated candidates, cached per class (kind, log2 n, log2 mean list length, k).
+// Lanes per triple and threads per set, timed on this instance's own
+// generated candidates (the move-kernel autotuner; the choice never changes results).
 static int autotune(bm_ctx *c) {
     bm_batch_state *b = c->batch;
     const bm_refine_params &p = b->prm;
-    const int dl = log2i(std::max<long long>(1, c->ncomb / std::max(1, c->n)));
-    const int nl = log2i(c->n);
     int rc;
     if (p.triples > 0) {
-        const int forced = env_int("BM_ROT3_LANES", 0);
-        auto key = std::make_tuple(1, nl, dl, 3);
-        int best = forced;
-        if (!best) {
-            std::lock_guard<std::mutex> g(g_tune_mu);
-            auto it = g_tune.find(key);
-            if (it != g_tune.end()) best = it->second;
-        }
-        if (!best) {
-            BM_LAUNCH(c, "gen_triples",
-                      hipLaunchKernelGGL(k_gen_triples, dim3((p.triples + 255) / 256), dim3(256), 0, c->stream,
-                                         b->tri, p.triples, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1,
-                                         c->n));
-            float bms = 1e30f;
-            for (int lanes : {8, 16, 32}) {
-                c->rot3_lanes = lanes;
-                float ms = 0;
-                if ((rc = time_variant(c, [&] { return launch_rot3(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d,
-                                                                   b->tri_p); }, &ms)))
-                    return rc;
-                if (ms < bms) {
-                    bms = ms;
-                    best = lanes;
-                }
-            }
-            std::lock_guard<std::mutex> g(g_tune_mu);
-            g_tune[key] = best;
-        }
-        c->rot3_lanes = best;
+        BM_LAUNCH(c, "gen_triples",
+                  hipLaunchKernelGGL(k_gen_triples, dim3((p.triples + 255) / 256), dim3(256), 0, c->stream, b->tri,
+                                     p.triples, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1, c->n));
+        if ((rc = tune_rot3(c, b->tri, p.triples, b->tri_d, b->tri_p))) return rc;
     }
     if (p.sets > 0) {
-        const int forced = env_int("BM_PERMSET_THREADS", 0);
-        auto key = std::make_tuple(2, nl, dl, p.set_k);
-        int best = forced;
-        if (!best) {
-            std::lock_guard<std::mutex> g(g_tune_mu);
-            auto it = g_tune.find(key);
-            if (it != g_tune.end()) best = it->second;
-        }
-        if (!best) {
-            BM_LAUNCH(c, "gen_sets",
-                      hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets,
-                                         p.sets, p.set_k, p.set_gen, b->st, (unsigned long long)p.seed, c->cptr,
-                                         c->cr, c->N1, c->n));
-            float bms = 1e30f;
-            for (int th : {64, 256}) {
-                c->permset_threads = th;
-                float ms = 0;
-                if ((rc = time_variant(c, [&] { return launch_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r,
-                                                                      b->ovf, 1); }, &ms)))
-                    return rc;
-                if (ms < bms) {
-                    bms = ms;
-                    best = th;
-                }
-            }
-            std::lock_guard<std::mutex> g(g_tune_mu);
-            g_tune[key] = best;
-        }
-        c->permset_threads = best;
+        BM_LAUNCH(c, "gen_sets",
+                  hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets, p.sets,
+                                     p.set_k, p.set_gen, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1,
+                                     c->n));
+        if ((rc = tune_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r, b->ovf, 1))) return rc;
     }
     BM_HIP(hipStreamSynchronize(c->stream));
     return BM_OK;
