@@ -275,7 +275,19 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
         const uint32_t *L = out ? g2iy : g2oy;
         const int l0 = out ? g2iptr[g] : g2optr[g], l1 = out ? g2iptr[g + 1] : g2optr[g + 1];
         const int w = rw.y, wneg = min(w, 0);
-        for (int j = l0 + lane; j < l1; j += 64) {
+        int j = l0 + lane;
+        for (; j + 192 < l1; j += 256) {  // four list words in flight per lane
+            uint32_t yv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) yv[u] = L[j + 64 * u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int y = yv[u] >> 16;
+                if (y != 0xFFFF && y >= lo && y <= hi)
+                    atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - wneg);
+            }
+        }
+        for (; j < l1; j += 64) {
             const uint32_t yv = L[j];
             const int y = yv >> 16;
             if (y != 0xFFFF && y >= lo && y <= hi) atomicAdd(&acc[y], min(w, (int)(int16_t)(yv & 0xffff)) - wneg);
@@ -289,7 +301,18 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
         // in-neighbour: rows s = v of P, read by the out-lines v->b (in entries of b);
         // out-neighbour: rows n+1+v of P^T, read by the in-lines b->v (out entries of b)
         const int l0 = in ? cptr[v] : cmid[v], l1 = in ? cmid[v] : cptr[v + 1];
-        for (int e = l0 + lane; e < l1; e += 64) {
+        int e = l0 + lane;
+        for (; e + 192 < l1; e += 256) {
+            int2 rw[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) rw[u] = crw[e + 64 * u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int b = in ? rw[u].x - N1 : rw[u].x;
+                if (b >= lo && b <= hi) atomicAdd(&acc[b], min(rw[u].y, val) - min(rw[u].y, 0));
+            }
+        }
+        for (; e < l1; e += 64) {
             const int2 rw = crw[e];
             const int b = in ? rw.x - N1 : rw.x;
             if (b >= lo && b <= hi) atomicAdd(&acc[b], min(rw.y, val) - min(rw.y, 0));
