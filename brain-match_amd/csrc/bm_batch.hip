@@ -133,11 +133,12 @@ extern "C" int bm_moves_eval(bm_ctx *c, int mode, int64_t *best_index, int32_t *
     if (c->mv_k >= 4) BM_HIP(hipMemsetAsync(c->mv_flag, 0, 4, c->stream));
     int rc = eval_resident(c, mode);
     if (rc) return rc;
-    unsigned long long key = 0;
-    int flag = 0;
-    BM_HIP(hipMemcpyAsync(&key, c->keys + c->N1 + 3, 8, hipMemcpyDeviceToHost, c->stream));
-    if (c->mv_k >= 4) BM_HIP(hipMemcpyAsync(&flag, c->mv_flag, 4, hipMemcpyDeviceToHost, c->stream));
+    c->hres[1] = 0;
+    BM_HIP(hipMemcpyAsync(c->hres, c->keys + c->N1 + 3, 8, hipMemcpyDeviceToHost, c->stream));
+    if (c->mv_k >= 4) BM_HIP(hipMemcpyAsync(c->hres + 1, c->mv_flag, 4, hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipStreamSynchronize(c->stream));
+    const unsigned long long key = c->hres[0];
+    const int flag = (int)(uint32_t)c->hres[1];
     if (flag) return set_error(BM_ERANGE, "a set has too many intra-set G1 lines");
     *best_index = -1;
     *best_delta = 0;

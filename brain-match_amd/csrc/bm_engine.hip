@@ -509,6 +509,15 @@ int bm_create(int device, bm_ctx **out) {
         delete c;
         return hip_fail(e, "hipStreamCreate");
     }
+    {
+        std::lock_guard<std::mutex> g(sync_mutex());
+        e = hipHostMalloc((void **)&c->hres, 8 * sizeof(unsigned long long), hipHostMallocDefault);
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return hip_fail(e, "hipHostMalloc");
+    }
     *out = c;
     return BM_OK;
 }
@@ -573,6 +582,7 @@ int bm_destroy(bm_ctx *c) {
         (void)hipEventDestroy(r.stop);
     }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->hres) (void)hipHostFree(c->hres);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BM_OK;
@@ -895,10 +905,9 @@ int bm_score(bm_ctx *c, int64_t *score) {
     if (blocks < 1) blocks = 1;
     BM_LAUNCH(c, "score", hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, c->stream, out, c->esrc, c->edst,
                                              c->ew, m, c->Q, c->ld));
-    unsigned long long h = 0;
-    BM_HIP(hipMemcpyAsync(&h, out, 8, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipMemcpyAsync(c->hres, out, 8, hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipStreamSynchronize(c->stream));
-    *score = (int64_t)h;
+    *score = (int64_t)c->hres[0];
     return BM_OK;
 }
 

@@ -102,6 +102,7 @@ struct bm_ctx {
     uint8_t *vsimple = nullptr;    // 1: no self-loop and no repeated line touches the vertex
     unsigned long long *keys = nullptr;  // N1 + 8 reduction slots
     void *stage = nullptr;               // move-list staging (lazy, grows)
+    unsigned long long *hres = nullptr;  // pinned host slots for small results (8 x 8 bytes)
     size_t stage_bytes = 0;
     int64_t dev_bytes = 0;
 

@@ -1134,9 +1134,9 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
                                                         c->cdw, c->diag, res));
         }
     }
-    unsigned long long k = 0;
-    BM_HIP(hipMemcpyAsync(&k, res, 8, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipMemcpyAsync(c->hres, res, 8, hipMemcpyDeviceToHost, c->stream));  // pinned: no staging copy
     BM_HIP(hipStreamSynchronize(c->stream));
+    const unsigned long long k = c->hres[0];
     if (mode == BM_ROW_FIRST_POSITIVE) {
         if (k != ~0ull) {
             *best_b = (int32_t)(k >> 32);
