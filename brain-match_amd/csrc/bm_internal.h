@@ -172,7 +172,10 @@ int launch_diag(bm_ctx *c);             // X(v,v) for every v
 int launch_score(bm_ctx *c, unsigned long long *out);
 int launch_refresh_g2y(bm_ctx *c);      // G2 lists through p^-1
 int sweep_prepare(bm_ctx *c);           // allocations and kernel choice of the all-pairs sweep
-int sweep_enqueue(bm_ctx *c, int32_t *dmat);  // row-best keys into keys[0..n], global best into keys[n+1]
+// row-best keys into keys[0..n], global best into keys[n+1]; with run_flag the
+// sparse gain kernels run only when *run_flag != 0 (X stale), the combine always
+int sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag = nullptr);
+bool sweep_is_sparse(const bm_ctx *c);  // the sweep runs the sparse gain kernel
 
 // batched refinement, split for the serving loop: start a run, enqueue a chunk
 // of steps (returns at once), poll / wait for it, read the counters, finish

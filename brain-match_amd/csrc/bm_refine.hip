@@ -38,6 +38,7 @@
 #include <tuple>
 #include <vector>
 
+#include "bm_device.h"
 #include "bm_internal.h"
 
 using namespace bm;
@@ -45,6 +46,7 @@ using namespace bm;
 namespace {
 
 constexpr int kHist = 4096;  // per-step history ring
+constexpr int kXcolBlocks = 512;  // workgroups of the column recompute (grid-stride over U)
 
 struct RefineState {
     unsigned long long step;   // steps completed in this run
@@ -61,6 +63,11 @@ struct RefineState {
     long long idle;            // consecutive steps without a move
     long long initial_score;
     unsigned long long gen_step;  // steps since bm_refine_configure: seeds the candidate generators
+    // incremental update of the all-pairs gain matrix X between steps
+    int xstale;   // 1: X does not match the mapping, the next sweep recomputes it whole
+    int inc;      // this step updates X incrementally (few moved vertices)
+    int nu;       // moved vertices this step
+    int npairs;   // their comb entries (prefix ubase)
 };
 
 struct MoveRec {
@@ -337,6 +344,220 @@ __global__ void k_state_init(RefineState *st) {
     st->score = s;
     st->initial_score = s;
     st->gen_step = g;
+    st->xstale = 1;
+}
+
+// ---------------------------------------------------------------- incremental X
+//
+// X(u,v) = C_u + sum over e in comb(u) of the G2 list of e's neighbour's
+// image, in P coordinates (the sparse form of the all-pairs sweep), plus
+// Corr(u,v) at u < v.  After a step that moved the vertex set U (images
+// permuted within U):
+//   - an entry of comb(u) whose neighbour x is in U now reads the list of
+//     p'(x) instead of p(x): row u gains f over the new list and loses f over
+//     the old one, in the columns outside U (k_xdelta);
+//   - a list target in p(U) now translates to another vertex of U, so the
+//     columns of U are recomputed whole, pulled from the G2 neighbours of
+//     their new images (k_xcol);
+//   - Corr changes on the lines at U: subtracted before the move with the old
+//     P, added after it with the new P (k_corr_inc);
+//   - X(v,v) is read back into diag (k_diag_all).
+// f(w, q) = min(w, q) - min(w, 0).  Costs about 3 |U| / n of a sweep; steps
+// that move more than inc_max vertices mark X stale instead.
+
+__device__ __forceinline__ int block_scan_excl(int v, int *wsum, int &total) {  // 1024 threads
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        int t = wsum[threadIdx.x];
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(t, o, 16);
+            if ((int)threadIdx.x >= o) t += y;
+        }
+        wsum[threadIdx.x] = t;
+    }
+    __syncthreads();
+    const int incl = x + (wave ? wsum[wave - 1] : 0);
+    total = wsum[15];
+    __syncthreads();
+    return incl - v;
+}
+
+// largest i < count with base[i] <= p (base ascending, base[0] = 0)
+__device__ __forceinline__ int find_base(const int32_t *__restrict__ base, int count, int p) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One workgroup: the moved vertices (winners' vertices) with their old
+// images, the U flags, the prefix of their comb-list lengths, the decision.
+__global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ win, CandView cv,
+                                                  const int32_t *__restrict__ map, const int32_t *__restrict__ cptr,
+                                                  int32_t *__restrict__ ulist, int32_t *__restrict__ uold,
+                                                  int32_t *__restrict__ ubase, uint8_t *__restrict__ inU,
+                                                  RefineState *__restrict__ st, int inc_max) {
+    __shared__ int cnt, carry;
+    __shared__ int wsum[16];
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        carry = 0;
+    }
+    __syncthreads();
+    const int nwin = (int)st->nwin;
+    for (int w = threadIdx.x; w < nwin; w += 1024) {
+        int v[8], d, kind;
+        const int k = cand_get(cv, win[w], v, d, kind);
+        const int b = atomicAdd(&cnt, k);
+        for (int q = 0; q < k; q++) {
+            ulist[b + q] = v[q];
+            uold[b + q] = map[v[q]];
+            inU[v[q]] = 1;
+        }
+    }
+    __syncthreads();
+    const int nu = cnt;
+    for (int c0 = 0; c0 < nu; c0 += 1024) {
+        const int i = c0 + threadIdx.x;
+        const int deg = i < nu ? cptr[ulist[i] + 1] - cptr[ulist[i]] : 0;
+        int total;
+        const int ex = block_scan_excl(deg, wsum, total);
+        if (i < nu) ubase[i] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ubase[nu] = carry;
+        st->nu = nu;
+        st->npairs = carry;
+        st->inc = nu <= inc_max ? 1 : 0;
+        if (nu > inc_max) st->xstale = 1;
+    }
+}
+
+
+// Corr of every G1 line at U into X(min, max), times sign; a line between two
+// vertices of U is taken from the smaller one's list only.
+__global__ void k_corr_inc(int sign, const RefineState *__restrict__ st, const int32_t *__restrict__ ulist,
+                           const int32_t *__restrict__ ubase, const uint8_t *__restrict__ inU,
+                           const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                           const int32_t *__restrict__ cdw, const int16_t *__restrict__ Q, int ld, int N1,
+                           int32_t *__restrict__ X, int64_t ldx) {
+    if (!st->inc) return;
+    const int nu = st->nu, total = st->npairs, T = (int)(ldx >> 6);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        const int xi = find_base(ubase, nu, p);
+        const int x = ulist[xi];
+        const int e = cptr[x] + (p - ubase[xi]);
+        const int2 rw = crw[e];
+        const bool out = rw.x >= N1;
+        const int u = out ? rw.x - N1 : rw.x;
+        if (u == x || (inU[u] && u < x)) continue;
+        const int a = min(x, u), b = max(x, u);
+        const bool out_a = x == a ? out : !out;
+        const int corr = pair_corr(out_a, rw.y, cdw[e], (int)Q[(size_t)a * ld + a], (int)Q[(size_t)b * ld + b],
+                                   (int)Q[(size_t)a * ld + b], (int)Q[(size_t)b * ld + a]);
+        if (corr) atomicAdd(X + xidx(a, b, T), sign * corr);
+    }
+}
+
+// one wave per comb entry of U: the mirrored entry of the neighbour's row
+// swaps the list of the old image for that of the new one
+__global__ __launch_bounds__(256) void k_xdelta(const RefineState *__restrict__ st, const int32_t *__restrict__ ulist,
+                                                const int32_t *__restrict__ uold, const int32_t *__restrict__ ubase,
+                                                const uint8_t *__restrict__ inU, const int32_t *__restrict__ map,
+                                                const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                const int32_t *__restrict__ g2optr, const uint32_t *__restrict__ g2oy,
+                                                const int32_t *__restrict__ g2iptr, const uint32_t *__restrict__ g2iy,
+                                                int N1, int32_t *__restrict__ X, int64_t ldx) {
+    if (!st->inc) return;
+    const int nu = st->nu, total = st->npairs, T = (int)(ldx >> 6);
+    const int lane = threadIdx.x & 63;
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    for (int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < total; p += nwaves) {
+        const int xi = find_base(ubase, nu, p);
+        const int x = ulist[xi];
+        const int gold = uold[xi], gnew = map[x];
+        if (gold == gnew) continue;
+        const int2 rw = crw[cptr[x] + (p - ubase[xi])];
+        const bool out = rw.x >= N1;  // x -> u: u reads row x of P (G2 out-lists); u -> x: row x of P^T (in-lists)
+        const int u = out ? rw.x - N1 : rw.x;
+        const int w = rw.y, wneg = min(w, 0);
+        const int32_t *ptr = out ? g2optr : g2iptr;
+        const uint32_t *L = out ? g2oy : g2iy;
+        for (int side = 0; side < 2; side++) {
+            const int g = side ? gold : gnew, sg = side ? -1 : 1;
+            for (int j = ptr[g] + lane; j < ptr[g + 1]; j += 64) {
+                const uint32_t yv = L[j];
+                const int y = yv >> 16;
+                if (y != 0xFFFF && !inU[y])
+                    atomicAdd(X + xidx(u, y, T), sg * (min(w, (int)(int16_t)(yv & 0xffff)) - wneg));
+            }
+        }
+    }
+}
+
+// Column v of X for each v in U, one workgroup per v: C_u for every row u,
+// plus, for each G2 line at the new image p'(v) (G2 in-neighbour g -> p'(v):
+// the G1 lines s -> u of s = p'^-1(g); out-neighbour p'(v) -> g: the lines
+// u -> s), f(w, that line's G2 weight), accumulated in an (n+1)-int LDS
+// column and written once.
+__global__ __launch_bounds__(1024) void k_xcol(const RefineState *__restrict__ st, const int32_t *__restrict__ ulist,
+                                               const int32_t *__restrict__ map, const int32_t *__restrict__ cptr,
+                                               const int32_t *__restrict__ cmid, const int2 *__restrict__ crw,
+                                               const int32_t *__restrict__ cneg, const int32_t *__restrict__ g2optr,
+                                               const uint32_t *__restrict__ g2oy, const int32_t *__restrict__ g2iptr,
+                                               const uint32_t *__restrict__ g2iy, int N1, int32_t *__restrict__ X,
+                                               int64_t ldx) {
+    extern __shared__ __align__(16) int col[];
+    if (!st->inc) return;
+    const int nu = st->nu, T = (int)(ldx >> 6);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int vi = blockIdx.x; vi < nu; vi += gridDim.x) {
+        const int v = ulist[vi], g0 = map[v];
+        for (int u = threadIdx.x; u < N1; u += 1024) col[u] = cneg[u];
+        __syncthreads();
+        const int i0 = g2iptr[g0], nin = g2iptr[g0 + 1] - i0;
+        const int o0 = g2optr[g0], nout = g2optr[g0 + 1] - o0;
+        // one wave per G2 line, lanes over the preimage's G1 lines
+        for (int h = wave; h < nin + nout; h += 16) {
+            const bool in = h < nin;
+            const uint32_t yv = in ? g2iy[i0 + h] : g2oy[o0 + h - nin];
+            const int s = yv >> 16;
+            if (s == 0xFFFF) continue;
+            const int val = (int)(int16_t)(yv & 0xffff);
+            const int l0 = in ? cptr[s] : cmid[s], l1 = in ? cmid[s] : cptr[s + 1];
+            for (int e = l0 + lane; e < l1; e += 64) {
+                const int2 rw = crw[e];
+                atomicAdd(&col[in ? rw.x - N1 : rw.x], min(rw.y, val) - min(rw.y, 0));
+            }
+        }
+        __syncthreads();
+        for (int u = threadIdx.x; u < N1; u += 1024) X[xidx(u, v, T)] = col[u];
+        __syncthreads();
+    }
+}
+
+__global__ void k_diag_all(const RefineState *__restrict__ st, const int32_t *__restrict__ X, int64_t ldx, int N1,
+                           int32_t *__restrict__ diag) {
+    if (!st->inc) return;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N1) diag[v] = X[xidx(v, v, (int)(ldx >> 6))];
+}
+
+__global__ void k_clear_stale(RefineState *st) {
+    if (threadIdx.x == 0) st->xstale = 0;
 }
 
 }  // namespace
@@ -351,6 +572,10 @@ struct bm_batch_state {
     unsigned long long *ckey = nullptr, *vmax = nullptr;
     uint32_t *blocked = nullptr;
     int32_t *win = nullptr;
+    int32_t *ulist = nullptr, *uold = nullptr, *ubase = nullptr;
+    uint8_t *inU = nullptr;
+    bool incremental = false;  // the captured step updates X incrementally
+    int inc_max = 0;
     RefineState *st = nullptr;
     RefineState *hst = nullptr;  // pinned copy
     bm_refine_step *hist = nullptr;
@@ -402,6 +627,7 @@ int bm::free_batch(bm_ctx *c) {
     };
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
     fr(b->ckey), fr(b->vmax), fr(b->blocked), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
+    fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->inU);
     if (b->hst) (void)hipHostFree(b->hst);
     if (b->done) (void)hipEventDestroy(b->done);
     c->dev_bytes -= (int64_t)b->bytes;
@@ -439,8 +665,15 @@ static int enqueue_step(bm_ctx *c) {
                                             c->stream, (uint32_t *)b->vmax, vm_words, 0u));
     BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((N1 + 255) / 256), dim3(256), 0, c->stream, b->blocked,
                                             (long long)N1, 0u));
+    // X is updated in place between steps only with the sparse sweep and swap candidates
+    const bool inc = p.pairs && sweep_is_sparse(c);
+    b->incremental = inc;
+    if (inc)
+        BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((N1 / 4 + 256) / 256), dim3(256), 0, c->stream,
+                                                (uint32_t *)b->inU, (long long)(N1 + 3) / 4, 0u));
     if (p.pairs) {
-        if ((rc = sweep_enqueue(c, nullptr))) return rc;
+        if ((rc = sweep_enqueue(c, nullptr, inc ? &b->st->xstale : nullptr))) return rc;
+        if (inc) BM_LAUNCH(c, "clear_stale", hipLaunchKernelGGL(k_clear_stale, dim3(1), dim3(64), 0, c->stream, b->st));
     }
     if (p.triples > 0) {
         BM_LAUNCH(c, "gen_triples",
@@ -465,6 +698,14 @@ static int enqueue_step(bm_ctx *c) {
                   hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->M + 3) / 4)), dim3(256), 0, c->stream, b->ckey,
                                      b->M, cv, vm, b->blocked, b->win, b->st, c->cptr, c->cr, N1));
     }
+    const unsigned ublocks = 1024;  // grid-stride kernels of the incremental update
+    if (inc) {
+        BM_LAUNCH(c, "collect", hipLaunchKernelGGL(k_collect, dim3(1), dim3(1024), 0, c->stream, b->win, cv, c->map,
+                                                   c->cptr, b->ulist, b->uold, b->ubase, b->inU, b->st, b->inc_max));
+        BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, -1, b->st,
+                                                    b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
+                                                    c->Q, c->ld, N1, c->X, c->ldx));
+    }
     const int max_win = N1 / 2 + 1;
     BM_LAUNCH(c, "apply", hipLaunchKernelGGL(k_apply, dim3((max_win + 255) / 256), dim3(256), 0, c->stream, b->win,
                                              max_win, cv, c->map, c->pinv, c->sparse_ok ? 1 : 0, b->st, b->log,
@@ -472,6 +713,19 @@ static int enqueue_step(bm_ctx *c) {
     if ((rc = launch_rebuild_p(c))) return rc;
     if (c->sparse_ok) {
         if ((rc = launch_refresh_g2y(c))) return rc;
+    }
+    if (inc) {
+        BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta, dim3(ublocks), dim3(256), 0, c->stream, b->st, b->ulist,
+                                                  b->uold, b->ubase, b->inU, c->map, c->cptr, (const int2 *)c->crw,
+                                                  c->g2optr, c->g2oy, c->g2iptr, c->g2iy, N1, c->X, c->ldx));
+        BM_LAUNCH(c, "xcol", hipLaunchKernelGGL(k_xcol, dim3(kXcolBlocks), dim3(1024), (size_t)N1 * 4, c->stream,
+                                                b->st, b->ulist, c->map, c->cptr, c->cmid, (const int2 *)c->crw,
+                                                c->cneg, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, N1, c->X, c->ldx));
+        BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, 1, b->st,
+                                                    b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
+                                                    c->Q, c->ld, N1, c->X, c->ldx));
+        BM_LAUNCH(c, "diag_all", hipLaunchKernelGGL(k_diag_all, dim3((N1 + 255) / 256), dim3(256), 0, c->stream,
+                                                    b->st, c->X, c->ldx, N1, c->diag));
     }
     if ((rc = launch_score(c, &b->st->score_slot))) return rc;
     BM_LAUNCH(c, "refine_end", hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->stream, b->st, b->hist));
@@ -577,6 +831,13 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->vmax, (size_t)p.rounds * N1 * 8))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->blocked, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->win, (size_t)(N1 / 2 + 1) * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->ulist, (size_t)N1 * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->uold, (size_t)N1 * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->ubase, (size_t)(N1 + 1) * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->inU, (size_t)(N1 + 3) / 4 * 4))) return fail(rc);
+    // steps moving at most inc_max vertices update X in place (about 3 |U| / n of a sweep)
+    b->inc_max = std::max(64, c->N1 / 8);
+    if (const char *e = getenv("BM_REFINE_INC_MAX")) b->inc_max = atoi(e);
     if ((rc = balloc(c, (void **)&b->st, sizeof(RefineState)))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->hist, sizeof(bm_refine_step) * kHist))) return fail(rc);
     hipError_t he;
@@ -597,6 +858,9 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
         // first sparse launch outside the capture sets the kernel's LDS attribute
         if ((rc = sweep_enqueue(c, nullptr))) return fail(rc);
     }
+    if (hipFuncSetAttribute((const void *)k_xcol, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) !=
+        hipSuccess)
+        return fail(set_error(BM_EHIP, "hipFuncSetAttribute failed"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(BM_EHIP, "configure sync failed"));
     if ((rc = capture(c))) return fail(rc);
     b->configured = true;

@@ -19,6 +19,7 @@
 #include <mutex>
 #include <tuple>
 
+#include "bm_device.h"
 #include "bm_internal.h"
 
 using namespace bm;
@@ -27,13 +28,6 @@ __device__ __forceinline__ int q16(const int16_t *__restrict__ Q, size_t row, in
     return (int)Q[row * (size_t)ld + col];
 }
 
-// The all-pairs gain matrix X is stored tile-major: 64x64 tiles of int32,
-// tile (I, J) contiguous, T tiles per tile row (T = ldx / 64).  Rows of a
-// tile are 256 contiguous bytes, so row-wise writers stay coalesced and the
-// combine reads a tile and its mirror as two contiguous 16 KB blocks.
-__device__ __forceinline__ size_t xidx(int a, int b, int T) {
-    return ((size_t)(a >> 6) * T + (size_t)(b >> 6)) * 4096 + (size_t)((a & 63) << 6) + (size_t)(b & 63);
-}
 
 // ------------------------------------------------------------ move list
 
@@ -107,20 +101,6 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k)
     return k;
 }
 
-// Correction for b adjacent to a, from one entry of comb(a):
-//   -K_e  (the skipped list entries of the edge, counted in X)
-//   + dense-cell term when this line owns W1[x][y] (:390-405)
-__device__ __forceinline__ int pair_corr(bool out, int w, int dw, int paa, int pbb, int pab, int pba) {
-    int v;
-    if (out) {  // a -> b
-        v = 2 * min(w, pab) - min(w, paa) - min(w, pbb);
-        if (dw > 0) v += min(dw, pba) - min(dw, pab);
-    } else {  // b -> a
-        v = 2 * min(w, pba) - min(w, pbb) - min(w, paa);
-        if (dw > 0) v += min(dw, pab) - min(dw, pba);
-    }
-    return v;
-}
 
 template <bool kLdsCol>
 __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mode, int N1, int ld,
@@ -633,7 +613,9 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                                                       const int32_t *__restrict__ g2iptr,
                                                       const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
                                                       int ld, int32_t *__restrict__ X, int64_t ldx,
-                                                      int32_t *__restrict__ diag, int N1, int dense_len) {
+                                                      int32_t *__restrict__ diag, int N1, int dense_len,
+                                                      const int *__restrict__ run_flag) {
+    if (run_flag && *run_flag == 0) return;  // X already current (the batched step's incremental update)
     extern __shared__ __align__(16) int acc[];
     __shared__ int pref[kSparseChunk + 1];
     __shared__ int lst[kSparseChunk];
@@ -801,14 +783,17 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
 constexpr size_t kSparseLdsMax = 144 * 1024 - 32;
 
 // rows split over several plan items start from zero
-__global__ void k_zero_rows(int32_t *__restrict__ X, int64_t ldx, const int32_t *__restrict__ rows) {
+__global__ void k_zero_rows(int32_t *__restrict__ X, int64_t ldx, const int32_t *__restrict__ rows,
+                            const int *__restrict__ run_flag) {
+    if (run_flag && *run_flag == 0) return;
     const int a = rows[blockIdx.x];
     const int T = (int)(ldx >> 6);
     for (int i = threadIdx.x; i < ldx; i += blockDim.x) X[xidx(a, i, T)] = 0;
 }
 
 __global__ void k_diag_from_x(const int32_t *__restrict__ X, int64_t ldx, const int32_t *__restrict__ rows, int count,
-                              int32_t *__restrict__ diag) {
+                              int32_t *__restrict__ diag, const int *__restrict__ run_flag) {
+    if (run_flag && *run_flag == 0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const int a = rows[i];
@@ -852,7 +837,7 @@ static int build_row_plan(bm_ctx *c) {
     return BM_OK;
 }
 
-static int launch_gain_sparse(bm_ctx *c) {
+static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
     if (c->g2y_dirty) {
         int rc = launch_refresh_g2y(c);
         if (rc) return rc;
@@ -869,22 +854,23 @@ static int launch_gain_sparse(bm_ctx *c) {
     }
     if (c->nsplit > 0)
         BM_LAUNCH(c, "zero_rows", hipLaunchKernelGGL(k_zero_rows, dim3(c->nsplit), dim3(256), 0, c->stream, c->X,
-                                                     c->ldx, c->split_rows));
+                                                     c->ldx, c->split_rows, run_flag));
     // the long-list path costs registers and scheduling even when idle: only
     // the instantiation that needs it carries it
     if (c->g2_maxlen > c->dense_len)
         BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<true>, dim3((unsigned)c->nplan), dim3(1024), lds,
                                                 c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
                                                 c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
-                                                c->ldx, c->diag, c->N1, c->dense_len));
+                                                c->ldx, c->diag, c->N1, c->dense_len, run_flag));
     else
         BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<false>, dim3((unsigned)c->nplan), dim3(1024), lds,
                                                 c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
                                                 c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
-                                                c->ldx, c->diag, c->N1, c->dense_len));
+                                                c->ldx, c->diag, c->N1, c->dense_len, run_flag));
     if (c->nsplit > 0)
         BM_LAUNCH(c, "diag_from_x", hipLaunchKernelGGL(k_diag_from_x, dim3((c->nsplit + 255) / 256), dim3(256), 0,
-                                                       c->stream, c->X, c->ldx, c->split_rows, c->nsplit, c->diag));
+                                                       c->stream, c->X, c->ldx, c->split_rows, c->nsplit, c->diag,
+                                                       run_flag));
     c->diag_valid = true;
     return BM_OK;
 }
@@ -1171,14 +1157,18 @@ int bm::sweep_prepare(bm_ctx *c) {
 
 // Graph-capturable: kernels only (the choice of gain kernel and all
 // allocations are made by sweep_prepare).
-int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat) {
+bool bm::sweep_is_sparse(const bm_ctx *c) {
+    return c->sweep_kind == 2 && c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
+}
+
+int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag) {
     const int N1 = c->N1;
     int rc;
     BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill_keys, dim3((N1 + 256) / 256), dim3(256), 0, c->stream, c->keys,
                                             (int64_t)N1 + 1));
     const bool sparse = c->sweep_kind == 2 && c->sparse_ok && (size_t)N1 * 4 <= kSparseLdsMax;
     if (sparse) {
-        if ((rc = launch_gain_sparse(c))) return rc;  // also writes X(a,a) and folds Corr in
+        if ((rc = launch_gain_sparse(c, run_flag))) return rc;  // also writes X(a,a) and folds Corr in
     } else {
         if ((rc = launch_diag(c))) return rc;
         if ((rc = launch_gain_cols(c, c->gain_cols))) return rc;

@@ -166,6 +166,41 @@ def test_deterministic_and_variant_independent(monkeypatch):
     assert all(r == results[0] for r in results[1:])
 
 
+@pytest.mark.parametrize("k", [4, 8])
+def test_incremental_x_update_matches_full_sweeps(k, monkeypatch):
+    # X kept current between steps (moved rows, columns and corrections only)
+    # must give the same steps as recomputing it whole every step
+    inst, o = instance(60 + k, 220, 2600)
+    prm = bmatch.refine_params(pairs=1, triples=1500, sets=200, set_k=k, rounds=2, steps_per_check=2, seed=19 + k)
+    runs = []
+    for inc_max in ("100000", "0"):
+        monkeypatch.setenv("BM_REFINE_INC_MAX", inc_max)
+        with bmatch.Engine(0) as e:
+            e.upload(inst.n, inst.g1, inst.g2)
+            st, recs, hist = run(e, inst.p0, prm, max_steps=14, patience=6)
+            assert st["verified"] == 1 and st["moves"] > 0
+            fin = replay(inst.n, inst.g1, o, inst.p0, recs, st, hist, check_rows=(inc_max != "0"))
+            np.testing.assert_array_equal(e.mapping(), fin)
+            runs.append((recs.tobytes(), e.mapping().tobytes(), st["final_score"]))
+    assert runs[0] == runs[1]
+
+
+def test_incremental_update_with_loops_dups_and_unmapped(monkeypatch):
+    monkeypatch.setenv("BM_REFINE_INC_MAX", "100000")
+    n = 70
+    g1 = random_multigraph(13, n, 900, dup_frac=0.05, selfloop_frac=0.05)
+    g2 = random_multigraph(14, n, 900, dup_frac=0.05, selfloop_frac=0.05)
+    mp = random_mapping(15, n, 0.1)
+    o = ob.Oracle(n, g1, g2)
+    with bmatch.Engine(0) as e:
+        e.upload(n, g1, g2)
+        prm = bmatch.refine_params(pairs=1, triples=800, sets=100, set_k=4, rounds=2, steps_per_check=3, seed=4)
+        st, recs, hist = run(e, mp, prm, max_steps=12, patience=6)
+        assert st["verified"] == 1
+        fin = replay(n, g1, o, mp, recs, st, hist, check_rows=False)
+        np.testing.assert_array_equal(e.mapping(), fin)
+
+
 def test_runs_continue_from_the_current_mapping():
     inst, o = instance(41, 160, 1700)
     with bmatch.Engine(0) as e:
