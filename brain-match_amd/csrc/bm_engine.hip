@@ -154,42 +154,72 @@ constexpr size_t kGatherLdsMax = 160 * 1024;
 // memory puts every lane on its own cache line.)
 // Rows wider than fit twice in LDS take one half per block (grid y = 2,
 // `both` = 0): the same code with r1 / d1 aliased to the single half.
+// With `rows` (a device list of nrows[0] vertices) only those rows are
+// rebuilt, grid-stride; `flag` (optional) skips the launch when *flag != want.
 __global__ __launch_bounds__(512) void k_gather_rows(int16_t *__restrict__ Q, const int16_t *__restrict__ W2,
-                                                     const int32_t *__restrict__ map, int N1, int ld, int both) {
+                                                     const int32_t *__restrict__ map, int N1, int ld, int both,
+                                                     const int32_t *__restrict__ rows, const int *__restrict__ nrows,
+                                                     const int *__restrict__ flag, int want) {
     extern __shared__ __align__(16) int16_t srow[];  // 2 * ld (both) or ld
-    const int x = blockIdx.x, tid = threadIdx.x, z = both ? 0 : blockIdx.y;
-    const int g = map[x];
-    const bm_h8 *s0 = (const bm_h8 *)(W2 + ((size_t)z * N1 + g) * ld);
-    const bm_h8 *s1 = (const bm_h8 *)(W2 + ((size_t)N1 + g) * ld);
-    bm_h8 *l0 = (bm_h8 *)srow, *l1 = (bm_h8 *)(srow + ld);
-    for (int i = tid; i < ld / 8; i += 512) {
-        l0[i] = s0[i];
-        if (both) l1[i] = s1[i];
-    }
-    __syncthreads();
-    const int16_t *r0 = srow, *r1 = both ? srow + ld : srow;
-    int16_t *d0 = Q + ((size_t)z * N1 + x) * ld, *d1 = both ? Q + ((size_t)N1 + x) * ld : d0;
-    for (int y0 = tid * 8; y0 < ld; y0 += 4096) {
-        bm_h8 v0, v1;
-        if (y0 + 8 <= N1) {
-            const int4 m0 = *(const int4 *)(map + y0), m1 = *(const int4 *)(map + y0 + 4);
-            const int m[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                v0[j] = r0[m[j]];
-                v1[j] = r1[m[j]];
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const bool in = y0 + j < N1;
-                const int m = in ? map[y0 + j] : 0;
-                v0[j] = in ? r0[m] : (int16_t)0;
-                v1[j] = in ? r1[m] : (int16_t)0;
-            }
+    if (flag && *flag != want) return;
+    const int count = rows ? *nrows : N1;
+    for (int bi = blockIdx.x; bi < count; bi += gridDim.x) {
+        const int x = rows ? rows[bi] : bi, tid = threadIdx.x, z = both ? 0 : blockIdx.y;
+        const int g = map[x];
+        const bm_h8 *s0 = (const bm_h8 *)(W2 + ((size_t)z * N1 + g) * ld);
+        const bm_h8 *s1 = (const bm_h8 *)(W2 + ((size_t)N1 + g) * ld);
+        bm_h8 *l0 = (bm_h8 *)srow, *l1 = (bm_h8 *)(srow + ld);
+        for (int i = tid; i < ld / 8; i += 512) {
+            l0[i] = s0[i];
+            if (both) l1[i] = s1[i];
         }
-        __builtin_nontemporal_store(v0, (bm_h8 *)(d0 + y0));
-        if (both) __builtin_nontemporal_store(v1, (bm_h8 *)(d1 + y0));
+        __syncthreads();
+        const int16_t *r0 = srow, *r1 = both ? srow + ld : srow;
+        int16_t *d0 = Q + ((size_t)z * N1 + x) * ld, *d1 = both ? Q + ((size_t)N1 + x) * ld : d0;
+        for (int y0 = tid * 8; y0 < ld; y0 += 4096) {
+            bm_h8 v0, v1;
+            if (y0 + 8 <= N1) {
+                const int4 m0 = *(const int4 *)(map + y0), m1 = *(const int4 *)(map + y0 + 4);
+                const int m[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    v0[j] = r0[m[j]];
+                    v1[j] = r1[m[j]];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const bool in = y0 + j < N1;
+                    const int m = in ? map[y0 + j] : 0;
+                    v0[j] = in ? r0[m] : (int16_t)0;
+                    v1[j] = in ? r1[m] : (int16_t)0;
+                }
+            }
+            __builtin_nontemporal_store(v0, (bm_h8 *)(d0 + y0));
+            if (both) __builtin_nontemporal_store(v1, (bm_h8 *)(d1 + y0));
+        }
+        __syncthreads();  // srow is reused by the next row
+    }
+}
+
+// The columns of a vertex list in every row of P and P^T:
+// P[x][v] = W2[p(x)][p(v)], P^T[x][v] = W2^T[p(x)][p(v)]; one wave per
+// output row, lanes over the list.  Skipped when *flag != want.
+__global__ __launch_bounds__(256) void k_patch_cols(int16_t *__restrict__ Q, const int16_t *__restrict__ W2,
+                                                    const int32_t *__restrict__ map, int N1, int ld,
+                                                    const int32_t *__restrict__ cols, const int *__restrict__ ncols,
+                                                    const int *__restrict__ flag, int want) {
+    if (*flag != want) return;
+    const int n = *ncols, lane = threadIdx.x & 63;
+    const int nwaves = gridDim.x * 4;
+    for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < 2 * N1; r += nwaves) {
+        const int z = r >= N1, x = r - z * N1;
+        const int16_t *src = W2 + ((size_t)z * N1 + map[x]) * ld;
+        int16_t *dst = Q + (size_t)r * ld;
+        for (int i = lane; i < n; i += 64) {
+            const int v = cols[i];
+            dst[v] = src[map[v]];
+        }
     }
 }
 
@@ -355,7 +385,7 @@ __global__ void k_perm_map(int32_t *__restrict__ map, int32_t *__restrict__ pinv
 
 namespace bm {
 
-int launch_rebuild_p(bm_ctx *c) {
+int launch_rebuild_p(bm_ctx *c, const int *patch_flag, const int32_t *verts, const int *nverts) {
     // both source rows in LDS when they fit (BM_GATHER_SPLIT=1 forces one row per block)
     const int both = (size_t)c->ld * 4 <= kGatherLdsMax && !getenv("BM_GATHER_SPLIT") ? 1 : 0;
     const size_t lds = (size_t)c->ld * 2 * (both ? 2 : 1);
@@ -364,8 +394,17 @@ int launch_rebuild_p(bm_ctx *c) {
                                    (int)kGatherLdsMax));
         c->gather_attr = true;
     }
+    // whole matrices (unless *patch_flag says patch)
     BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_gather_rows, dim3(c->N1, both ? 1 : 2), dim3(512), lds, c->stream,
-                                               c->Q, c->W2, c->map, c->N1, c->ld, both));
+                                               c->Q, c->W2, c->map, c->N1, c->ld, both, nullptr, nullptr,
+                                               patch_flag, 0));
+    if (patch_flag) {  // or the rows and columns of the moved vertices only
+        BM_LAUNCH(c, "patch_p", hipLaunchKernelGGL(k_gather_rows, dim3(1024, both ? 1 : 2), dim3(512), lds, c->stream,
+                                                   c->Q, c->W2, c->map, c->N1, c->ld, both, verts, nverts,
+                                                   patch_flag, 1));
+        BM_LAUNCH(c, "patch_p", hipLaunchKernelGGL(k_patch_cols, dim3(2048), dim3(256), 0, c->stream, c->Q, c->W2,
+                                                   c->map, c->N1, c->ld, verts, nverts, patch_flag, 1));
+    }
     return BM_OK;
 }
 

@@ -167,7 +167,10 @@ int free_moves(bm_ctx *c);
 int tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP);
 int tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult);
 int free_batch(bm_ctx *c);
-int launch_rebuild_p(bm_ctx *c);        // P and P^T from W2 and the device mapping
+// P and P^T from W2 and the device mapping; with patch_flag, when *patch_flag
+// is 1 only the rows and columns of verts[0 .. *nverts) are rebuilt
+int launch_rebuild_p(bm_ctx *c, const int *patch_flag = nullptr, const int32_t *verts = nullptr,
+                     const int *nverts = nullptr);
 int launch_diag(bm_ctx *c);             // X(v,v) for every v
 int launch_score(bm_ctx *c, unsigned long long *out);
 int launch_refresh_g2y(bm_ctx *c);      // G2 lists through p^-1

@@ -66,6 +66,7 @@ struct RefineState {
     // incremental update of the all-pairs gain matrix X between steps
     int xstale;   // 1: X does not match the mapping, the next sweep recomputes it whole
     int inc;      // this step updates X incrementally (few moved vertices)
+    int pinc;     // this step patches the moved rows and columns of P / P^T only
     int nu;       // moved vertices this step
     int npairs;   // their comb entries (prefix ubase)
 };
@@ -406,7 +407,7 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
                                                   const int32_t *__restrict__ map, const int32_t *__restrict__ cptr,
                                                   int32_t *__restrict__ ulist, int32_t *__restrict__ uold,
                                                   int32_t *__restrict__ ubase, uint8_t *__restrict__ inU,
-                                                  RefineState *__restrict__ st, int inc_max) {
+                                                  RefineState *__restrict__ st, int inc_max, int pinc_max) {
     __shared__ int cnt, carry;
     __shared__ int wsum[16];
     if (threadIdx.x == 0) {
@@ -442,6 +443,7 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
         st->nu = nu;
         st->npairs = carry;
         st->inc = nu <= inc_max ? 1 : 0;
+        st->pinc = nu <= pinc_max ? 1 : 0;
         if (nu > inc_max) st->xstale = 1;
     }
 }
@@ -576,6 +578,7 @@ struct bm_batch_state {
     uint8_t *inU = nullptr;
     bool incremental = false;  // the captured step updates X incrementally
     int inc_max = 0;
+    int pinc_max = 0;
     RefineState *st = nullptr;
     RefineState *hst = nullptr;  // pinned copy
     bm_refine_step *hist = nullptr;
@@ -701,7 +704,8 @@ static int enqueue_step(bm_ctx *c) {
     const unsigned ublocks = 1024;  // grid-stride kernels of the incremental update
     if (inc) {
         BM_LAUNCH(c, "collect", hipLaunchKernelGGL(k_collect, dim3(1), dim3(1024), 0, c->stream, b->win, cv, c->map,
-                                                   c->cptr, b->ulist, b->uold, b->ubase, b->inU, b->st, b->inc_max));
+                                                   c->cptr, b->ulist, b->uold, b->ubase, b->inU, b->st, b->inc_max,
+                                                   b->pinc_max));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, -1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
                                                     c->Q, c->ld, N1, c->X, c->ldx));
@@ -710,7 +714,7 @@ static int enqueue_step(bm_ctx *c) {
     BM_LAUNCH(c, "apply", hipLaunchKernelGGL(k_apply, dim3((max_win + 255) / 256), dim3(256), 0, c->stream, b->win,
                                              max_win, cv, c->map, c->pinv, c->sparse_ok ? 1 : 0, b->st, b->log,
                                              b->log_cap));
-    if ((rc = launch_rebuild_p(c))) return rc;
+    if ((rc = inc ? launch_rebuild_p(c, &b->st->pinc, b->ulist, &b->st->nu) : launch_rebuild_p(c))) return rc;
     if (c->sparse_ok) {
         if ((rc = launch_refresh_g2y(c))) return rc;
     }
@@ -838,6 +842,9 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     // steps moving at most inc_max vertices update X in place (about 3 |U| / n of a sweep)
     b->inc_max = std::max(64, c->N1 / 8);
     if (const char *e = getenv("BM_REFINE_INC_MAX")) b->inc_max = atoi(e);
+    // and patch P / P^T at the moved rows and columns up to pinc_max vertices
+    b->pinc_max = std::max(32, c->N1 / 64);
+    if (const char *e = getenv("BM_REFINE_PATCH_MAX")) b->pinc_max = atoi(e);
     if ((rc = balloc(c, (void **)&b->st, sizeof(RefineState)))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->hist, sizeof(bm_refine_step) * kHist))) return fail(rc);
     hipError_t he;

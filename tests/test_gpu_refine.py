@@ -173,8 +173,10 @@ def test_incremental_x_update_matches_full_sweeps(k, monkeypatch):
     inst, o = instance(60 + k, 220, 2600)
     prm = bmatch.refine_params(pairs=1, triples=1500, sets=200, set_k=k, rounds=2, steps_per_check=2, seed=19 + k)
     runs = []
-    for inc_max in ("100000", "0"):
+    # X and P / P^T both updated in place; X in place with P rebuilt; both rebuilt
+    for inc_max, patch_max in (("100000", "100000"), ("100000", "0"), ("0", "0")):
         monkeypatch.setenv("BM_REFINE_INC_MAX", inc_max)
+        monkeypatch.setenv("BM_REFINE_PATCH_MAX", patch_max)
         with bmatch.Engine(0) as e:
             e.upload(inst.n, inst.g1, inst.g2)
             st, recs, hist = run(e, inst.p0, prm, max_steps=14, patience=6)
@@ -182,11 +184,12 @@ def test_incremental_x_update_matches_full_sweeps(k, monkeypatch):
             fin = replay(inst.n, inst.g1, o, inst.p0, recs, st, hist, check_rows=(inc_max != "0"))
             np.testing.assert_array_equal(e.mapping(), fin)
             runs.append((recs.tobytes(), e.mapping().tobytes(), st["final_score"]))
-    assert runs[0] == runs[1]
+    assert runs[0] == runs[1] == runs[2]
 
 
 def test_incremental_update_with_loops_dups_and_unmapped(monkeypatch):
     monkeypatch.setenv("BM_REFINE_INC_MAX", "100000")
+    monkeypatch.setenv("BM_REFINE_PATCH_MAX", "100000")
     n = 70
     g1 = random_multigraph(13, n, 900, dup_frac=0.05, selfloop_frac=0.05)
     g2 = random_multigraph(14, n, 900, dup_frac=0.05, selfloop_frac=0.05)
