@@ -174,7 +174,7 @@ def _batched_compare(inst, prm, seconds, label, chunk):
 
 def batched(args):
     inst = bmatch.Instance.chunglu(2, 18524, 3_000_000)
-    prm = bmatch.refine_params(pairs=1, triples=4096, sets=1024, set_k=4, rounds=3, steps_per_check=8, seed=1)
+    prm = bmatch.refine_params(pairs=1, triples=1024, sets=256, set_k=4, rounds=3, steps_per_check=8, seed=1)
     secs = getattr(args, "window", 0) or 10.0
     return _batched_compare(inst, prm, secs, {"workload": "Chung-Lu pair n=18524 E=3.0M, 30% shuffled start, "
                                                           f"{secs:.0f} s window, batched vs plain", "n": inst.n},
@@ -235,7 +235,7 @@ def serve(args):
     rng = np.random.default_rng(2024)
     ns = rng.integers(500, 20001, jobs)
     gaps = rng.exponential(1.0 / rate, jobs)
-    prm = bmatch.refine_params(pairs=1, triples=4096, sets=0, rounds=3, steps_per_check=8, seed=11)
+    prm = bmatch.refine_params(pairs=1, triples=1024, sets=0, rounds=3, steps_per_check=8, seed=11)
     t = time.perf_counter()
     with cf.ThreadPoolExecutor(8) as ex:
         insts = list(ex.map(lambda i: bmatch.Instance.powerlaw(500 + i, int(ns[i]), 160 * int(ns[i]),
