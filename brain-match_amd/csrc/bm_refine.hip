@@ -403,16 +403,24 @@ __device__ __forceinline__ int find_base(const int32_t *__restrict__ base, int c
 
 // One workgroup: the moved vertices (winners' vertices) with their old
 // images, the U flags, the prefix of their comb-list lengths, the decision.
+// The update is taken when U is small (nu <= inc_max) and its list work,
+// estimated as sum over x in U of deg1(x) * 2 deg2(p(x)) + deg2(p(x)) * dbar
+// (list swaps of the rows next to x, the pull of x's column), is at most
+// inc_budget; otherwise X is marked stale for the next sweep.
 __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ win, CandView cv,
                                                   const int32_t *__restrict__ map, const int32_t *__restrict__ cptr,
-                                                  int32_t *__restrict__ ulist, int32_t *__restrict__ uold,
-                                                  int32_t *__restrict__ ubase, uint8_t *__restrict__ inU,
-                                                  RefineState *__restrict__ st, int inc_max, int pinc_max) {
+                                                  const int32_t *__restrict__ g2optr,
+                                                  const int32_t *__restrict__ g2iptr, int32_t *__restrict__ ulist,
+                                                  int32_t *__restrict__ uold, int32_t *__restrict__ ubase,
+                                                  uint8_t *__restrict__ inU, RefineState *__restrict__ st,
+                                                  int inc_max, int pinc_max, long long inc_budget, int dbar) {
     __shared__ int cnt, carry;
     __shared__ int wsum[16];
+    __shared__ unsigned long long work;
     if (threadIdx.x == 0) {
         cnt = 0;
         carry = 0;
+        work = 0;
     }
     __syncthreads();
     const int nwin = (int)st->nwin;
@@ -420,11 +428,16 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
         int v[8], d, kind;
         const int k = cand_get(cv, win[w], v, d, kind);
         const int b = atomicAdd(&cnt, k);
+        unsigned long long wk = 0;
         for (int q = 0; q < k; q++) {
+            const int g = map[v[q]];
             ulist[b + q] = v[q];
-            uold[b + q] = map[v[q]];
+            uold[b + q] = g;
             inU[v[q]] = 1;
+            const unsigned long long d2 = (g2iptr[g + 1] - g2iptr[g]) + (g2optr[g + 1] - g2optr[g]);
+            wk += (unsigned long long)(cptr[v[q] + 1] - cptr[v[q]]) * 2 * d2 + d2 * (unsigned long long)dbar;
         }
+        atomicAdd(&work, wk);
     }
     __syncthreads();
     const int nu = cnt;
@@ -442,9 +455,10 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
         ubase[nu] = carry;
         st->nu = nu;
         st->npairs = carry;
-        st->inc = nu <= inc_max ? 1 : 0;
+        const bool inc = nu <= inc_max && work <= (unsigned long long)inc_budget;
+        st->inc = inc ? 1 : 0;
         st->pinc = nu <= pinc_max ? 1 : 0;
-        if (nu > inc_max) st->xstale = 1;
+        if (!inc) st->xstale = 1;
     }
 }
 
@@ -579,6 +593,7 @@ struct bm_batch_state {
     bool incremental = false;  // the captured step updates X incrementally
     int inc_max = 0;
     int pinc_max = 0;
+    long long inc_budget = 0;  // list work an incremental update may take (k_collect)
     RefineState *st = nullptr;
     RefineState *hst = nullptr;  // pinned copy
     bm_refine_step *hist = nullptr;
@@ -704,8 +719,9 @@ static int enqueue_step(bm_ctx *c) {
     const unsigned ublocks = 1024;  // grid-stride kernels of the incremental update
     if (inc) {
         BM_LAUNCH(c, "collect", hipLaunchKernelGGL(k_collect, dim3(1), dim3(1024), 0, c->stream, b->win, cv, c->map,
-                                                   c->cptr, b->ulist, b->uold, b->ubase, b->inU, b->st, b->inc_max,
-                                                   b->pinc_max));
+                                                   c->cptr, c->g2optr, c->g2iptr, b->ulist, b->uold, b->ubase, b->inU,
+                                                   b->st, b->inc_max, b->pinc_max, b->inc_budget,
+                                                   (int)(c->ncomb / std::max(1, c->N1))));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, -1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
                                                     c->Q, c->ld, N1, c->X, c->ldx));
@@ -842,6 +858,18 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     // steps moving at most inc_max vertices update X in place (about 3 |U| / n of a sweep)
     b->inc_max = std::max(64, c->N1 / 8);
     if (const char *e = getenv("BM_REFINE_INC_MAX")) b->inc_max = atoi(e);
+    // The sweep walks about sum_v deg1(v) deg2(p(v)) list words with LDS atomics; the
+    // update, with global atomics, may take a quarter of that (measured on the served
+    // workload: 1/2 to 1/4 are equally fast, 1/32 falls back to whole sweeps too often;
+    // BM_REFINE_INC_RATIO overrides the 4).
+    {
+        long long wfull = 0;
+        for (int v = 0; v < N1; v++) wfull += (long long)c->hcdeg[v] * c->hg2deg[c->hmap[v]];
+        long long ratio = 4;
+        if (const char *e = getenv("BM_REFINE_INC_RATIO")) ratio = std::max(1, atoi(e));
+        b->inc_budget = wfull / ratio;
+        if (getenv("BM_REFINE_INC_MAX")) b->inc_budget = (long long)1 << 62;  // count rule only
+    }
     // and patch P / P^T at the moved rows and columns up to pinc_max vertices
     b->pinc_max = std::max(32, c->N1 / 64);
     if (const char *e = getenv("BM_REFINE_PATCH_MAX")) b->pinc_max = atoi(e);

@@ -571,6 +571,8 @@ constexpr int kSplitEntries = 2048;  // comb entries per block of the sparse ker
 // Long G2 lists streamed as whole Q rows, per chunk of entries.  Kept small:
 // two blocks per CU need the static LDS plus 4 (n+1) bytes under 80 KiB.
 constexpr int kSparseHubs = 32;
+// the sparse kernel is taken without timing when its list work is below 1/8 of the dense cell work
+constexpr long long kSparseMargin = 8;
 
 __global__ void k_g2_to_p(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
                           const int32_t *__restrict__ pinv, int64_t m) {
@@ -917,9 +919,21 @@ static int choose_gain(bm_ctx *c) {
 }
 
 static int choose_gain_timed(bm_ctx *c) {
+    const bool can_sparse = c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
+    if (can_sparse && !getenv("BM_SWEEP_KERNEL") && !getenv("BM_GAIN_COLS")) {
+        // By the work counts: the sparse kernel walks about sum_v deg1(v) deg2(p(v)) list
+        // words, the dense ones ncomb * (n+1) cells.  A clear margin decides without timing
+        // (timings taken while other contexts load the device can mislead).
+        long long ws = 0;
+        for (int v = 0; v < c->N1; v++) ws += (long long)c->hcdeg[v] * c->hg2deg[c->hmap[v]];
+        if (ws * kSparseMargin < (long long)c->ncomb * c->N1) {
+            c->sweep_kind = 2;
+            c->gain_cols = 4;  // unused while the sparse kernel runs
+            return build_row_plan(c);
+        }
+    }
     int rc = tune_gain(c);
     if (rc) return rc;
-    const bool can_sparse = c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
     if (!can_sparse) return BM_OK;
     if ((rc = build_row_plan(c))) return rc;  // its host copies stay outside the timed launch
     if (c->sweep_kind) return BM_OK;
