@@ -16,8 +16,10 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
   serve     Poisson arrivals (20 jobs/s) of power-law pairs with n uniform in
             500..20,000 and 160 lines per vertex, 30% shuffled starts, each
             refined by the batched mode to convergence (budget 10^6 moves) on
-            the serving loop; jobs/s and p50/p99 time-to-result
+            the serving loop; jobs/s and p50/p99 time-to-result (BM_SERVE_ACTIVE
+            sets the server's active slots, default 8)
 """
+import os
 import time
 
 import numpy as np
@@ -241,7 +243,8 @@ def serve(args):
         insts = list(ex.map(lambda i: bmatch.Instance.powerlaw(500 + i, int(ns[i]), 160 * int(ns[i]),
                                                                shuffle_frac=0.3), range(jobs)))
     gen_s = time.perf_counter() - t
-    with bmatch.Server(max_active=8, prep_threads=4, hbm_budget=0) as srv:
+    active = int(os.environ.get("BM_SERVE_ACTIVE", "8"))
+    with bmatch.Server(max_active=active, prep_threads=4, hbm_budget=0) as srv:
         ids = [None] * jobs
 
         def client():
