@@ -174,9 +174,11 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
     }
     __syncthreads();
     if (tid < k * k) PS[tid / k][tid % k] = qv(Q, S[tid / k], ld, S[tid % k]);
-    // table C over all entries of the k lists
+    // table C over all entries of the k lists: per-thread partial sums of row i
+    // in registers, one wave reduction and one LDS atomic per (wave, j)
     for (int i = 0; i < k; i++) {
         const int v = S[i];
+        int part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int e = cptr[v] + tid; e < cptr[v + 1]; e += kThreads) {
             const int r = cr[e], w = cw[e];
             const bool out = r >= N1;
@@ -198,8 +200,16 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
                 continue;
             }
             const int old = min(w, qcol(Q, r, N1, ld, v));
-            for (int j = 0; j < k; j++)
-                if (j != i) atomicAdd(&C[i][j], min(w, qcol(Q, r, N1, ld, S[j])) - old);
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (j < k && j != i) part[j] += min(w, qcol(Q, r, N1, ld, S[j])) - old;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (j >= k || j == i) continue;
+            int x = part[j];
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+            if ((tid & 63) == 0 && x) atomicAdd(&C[i][j], x);
         }
     }
     __syncthreads();
