@@ -132,23 +132,6 @@ constexpr int kMaxIntra = 256;
 constexpr int kTuneMinTriples = 4096;  // lists shorter than this use the current variant untimed
 constexpr int kTuneMinSets = 256;
 
-__device__ __forceinline__ bool next_perm(int *a, int k) {
-    int i = k - 2;
-    while (i >= 0 && a[i] >= a[i + 1]) i--;
-    if (i < 0) return false;
-    int j = k - 1;
-    while (a[j] <= a[i]) j--;
-    int t = a[i];
-    a[i] = a[j];
-    a[j] = t;
-    for (int x = i + 1, y = k - 1; x < y; x++, y--) {
-        t = a[x];
-        a[x] = a[y];
-        a[y] = t;
-    }
-    return true;
-}
-
 template <int kThreads>
 __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict__ sets, int64_t count, int k,
                                                       int32_t *__restrict__ best_d, int32_t *__restrict__ best_rank,
@@ -236,15 +219,33 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
             for (int q = idx; q < left - 1; q++) pool[q] = pool[q + 1];
             left--;
         }
+        // running sum of C[i][perm[i]]: next_permutation changes only the suffix from
+        // its pivot on (about e positions on average), which is re-read
+        int csum = 0;
+        for (int i = 0; i < k; i++) csum += C[i][perm[i]];
         for (int rank = r0; rank < r1; rank++) {
-            int d = lbase;
-            for (int i = 0; i < k; i++) d += C[i][perm[i]];
+            int d = lbase + csum;
             for (int q = 0; q < nint; q++) d += mult * min(iw[q], PS[perm[ix[q]]][perm[il[q]]]);
             if (d > bestd) {
                 bestd = d;
                 bestr = rank;
             }
-            if (rank + 1 < r1) next_perm(perm, k);
+            if (rank + 1 < r1) {  // next_permutation (:342-363) with the suffix re-summed
+                int pv = k - 2;
+                while (perm[pv] >= perm[pv + 1]) pv--;  // pv >= 0: this is not the last permutation
+                for (int i = pv; i < k; i++) csum -= C[i][perm[i]];
+                int j = k - 1;
+                while (perm[j] <= perm[pv]) j--;
+                int t = perm[pv];
+                perm[pv] = perm[j];
+                perm[j] = t;
+                for (int x = pv + 1, y = k - 1; x < y; x++, y--) {
+                    t = perm[x];
+                    perm[x] = perm[y];
+                    perm[y] = t;
+                }
+                for (int i = pv; i < k; i++) csum += C[i][perm[i]];
+            }
         }
     }
     // block arg-max: larger delta, then smaller rank
