@@ -132,13 +132,16 @@ constexpr int kMaxIntra = 256;
 constexpr int kTuneMinTriples = 4096;  // lists shorter than this use the current variant untimed
 constexpr int kTuneMinSets = 256;
 
-template <int kThreads>
-__global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict__ sets, int64_t count, int k,
+// kFixedK > 0 compiles the kernel for that k (the loops over set positions
+// unroll and index the per-thread permutation statically); 0 takes k at run time.
+template <int kThreads, int kFixedK>
+__global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict__ sets, int64_t count, int k_arg,
                                                       int32_t *__restrict__ best_d, int32_t *__restrict__ best_rank,
                                                       const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
                                                       const int32_t *__restrict__ cw, const int16_t *__restrict__ Q,
                                                       int N1, int ld, int *__restrict__ overflow, int mult) {
     constexpr int kWaves = kThreads / 64;
+    const int k = kFixedK ? kFixedK : k_arg;
     __shared__ int S[8];
     __shared__ int C[8][8];
     __shared__ int PS[8][8];
@@ -322,14 +325,19 @@ int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *
 
 int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult) {
     if (m <= 0) return BM_OK;
-    if (c->permset_threads == 64)
-        BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset<64>, dim3((unsigned)m), dim3(64), 0, c->stream, dS, m, k,
-                                                   dD, dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv, mult));
-    else if (c->permset_threads == 256)
-        BM_LAUNCH(c, "permset", hipLaunchKernelGGL(k_permset<256>, dim3((unsigned)m), dim3(256), 0, c->stream, dS, m,
-                                                   k, dD, dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv, mult));
-    else
+#define BM_PERMSET(T, K)                                                                                        \
+    BM_LAUNCH(c, "permset", hipLaunchKernelGGL((k_permset<T, K>), dim3((unsigned)m), dim3(T), 0, c->stream, dS, m, \
+                                               k, dD, dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv, mult))
+    if (c->permset_threads != 64 && c->permset_threads != 256)
         return set_error(BM_EINVAL, "permset: %d threads per set", c->permset_threads);
+    if (k == 8) {
+        if (c->permset_threads == 64) BM_PERMSET(64, 8);
+        else BM_PERMSET(256, 8);
+    } else {
+        if (c->permset_threads == 64) BM_PERMSET(64, 0);
+        else BM_PERMSET(256, 0);
+    }
+#undef BM_PERMSET
     return BM_OK;
 }
 
