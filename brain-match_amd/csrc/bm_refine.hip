@@ -32,10 +32,7 @@
 #include <string.h>
 
 #include <algorithm>
-#include <chrono>
-#include <map>
 #include <mutex>
-#include <tuple>
 #include <vector>
 
 #include "bm_device.h"

@@ -46,7 +46,6 @@ struct Job {
     bm_job_result res{};
     std::vector<int32_t> out_map;
     bool done = false;
-    bool collected = false;
 };
 
 double now_s(std::chrono::steady_clock::time_point t0) {
