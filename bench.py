@@ -15,7 +15,7 @@ picked) with its average launch time measured by HIP events on the engine's
 stream, and the CPU baseline (the oracle's restatement of the reference
 OpenMP row-best loop, alignment-omp.c:598-622, timed on this host on a
 bounded row sample).  --workload selects the other measurements: plain,
-plain-c2, rot3, permset, batched, batched-c1, batched-permset, serve
+plain-c2, rot3, permset, batched, batched-c1, batched-permset, batched-many, serve
 (bmatch/bench_workloads.py).
 """
 import argparse
@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--jobs", type=int, default=0, help="jobs of the serve workload")
     ap.add_argument("--workload", default="allpairs",
                     choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "batched-c1",
-                             "batched-permset", "serve"],
+                             "batched-permset", "batched-many", "serve"],
                     help="allpairs (default, the headline); the others measure the other configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -74,6 +74,7 @@ def lib():
         "bm_refine_configure": (C.c_int, [C.c_void_p, C.c_void_p]),
         "bm_refine_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_void_p]),
         "bm_refine_history": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+        "bm_refine_run_many": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_void_p]),
         "bm_refine_log": (C.c_int, [C.c_void_p, C.c_int32]),
         "bm_refine_moves": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.c_int64]),
         "bm_refine_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -512,6 +513,15 @@ class Engine:
 
     def global_best(self, max_steps=0, verify=True, acc_cap=1 << 20):
         return self._drive(self._L.bm_drive_global_best, max_steps, int(verify), acc_cap=acc_cap)
+
+
+def refine_run_many(engines, max_steps=1 << 30, move_budget=0, patience=2):
+    """bm_refine_run_many: the configured engines' runs packed into one graph launch per chunk."""
+    L = lib()
+    arr = (C.c_void_p * len(engines))(*[e._h.value for e in engines])
+    st = (_RefineStats * len(engines))()
+    _check(L.bm_refine_run_many(arr, len(engines), max_steps, move_budget, patience, st), "bm_refine_run_many")
+    return [{f: getattr(x, f) for f, _ in _RefineStats._fields_} for x in st]
 
 
 def job_footprint(n, e1, e2, params):

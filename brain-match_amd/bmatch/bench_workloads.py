@@ -13,6 +13,9 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
   batched-c1       the same comparison on the n=1,000 pair, both to convergence
   batched-permset  the n=1,000 pair, 4,096 uniform 8-vertex sets per step,
             every improving non-conflicting set applied, exact check per step
+  batched-many  8 power-law pairs of uneven n (2,000 .. 16,000, 160 lines per
+            vertex) refined to convergence: one after another, then packed into
+            one graph launch per chunk (bm_refine_run_many); same results
   serve     Poisson arrivals (20 jobs/s) of power-law pairs with n uniform in
             500..20,000 and 160 lines per vertex, 30% shuffled starts, each
             refined by the batched mode to convergence (budget 10^6 moves) on
@@ -228,6 +231,58 @@ def batched_permset(args):
     return out
 
 
+def batched_many(args):
+    import concurrent.futures as cf
+
+    ns = [2000 * (i + 1) for i in range(8)]
+    with cf.ThreadPoolExecutor(8) as ex:
+        insts = list(ex.map(lambda i: bmatch.Instance.powerlaw(900 + i, ns[i], 160 * ns[i], shuffle_frac=0.3),
+                            range(len(ns))))
+    prm = bmatch.refine_params(pairs=1, triples=1024, sets=0, rounds=3, steps_per_check=8, seed=11)
+
+    def engines():
+        es = []
+        for inst in insts:
+            e = bmatch.Engine(0)
+            e.upload(inst.n, inst.g1, inst.g2)
+            e.set_mapping(inst.p0)
+            e.refine_configure(prm)
+            es.append(e)
+        return es
+
+    es = engines()
+    t = time.perf_counter()
+    solo = [e.refine_run(patience=8) for e in es]
+    t_solo = time.perf_counter() - t
+    for e in es:
+        e.close()
+    es = engines()
+    t = time.perf_counter()
+    packed = bmatch.refine_run_many(es, patience=8)
+    t_packed = time.perf_counter() - t
+    for e in es:
+        e.close()
+    assert [s["final_score"] for s in solo] == [s["final_score"] for s in packed]
+    # the same jobs through the serving loop: one stream per job, all submitted at once
+    with bmatch.Server(max_active=8, prep_threads=4) as srv:
+        ids = [srv.submit(inst.n, inst.g1, inst.g2, inst.p0, prm, patience=8) for inst in insts]
+        t = time.perf_counter()
+        res = [srv.wait(i)[0] for i in ids]
+        t_last = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        t_gpu = max(r["t_done"] for r in res) - min(r["t_start"] for r in res)
+    assert [r["final_score"] for r in res] == [s["final_score"] for s in packed]
+    gain = sum(s["final_score"] - s["initial_score"] for s in packed)
+    out = _base("packed batched refinement: graph pairs per second", len(ns) / t_packed, "pairs/s",
+                sum(s["steps"] for s in packed), t_packed * 1000 / max(1, max(s["steps"] for s in packed)),
+                {"workload": "8 power-law pairs n=2000..16000 (160 lines/vertex), batched refinement to "
+                             "convergence, one graph launch per chunk for all pairs", "n": ns})
+    out.update({"seconds_packed": t_packed, "seconds_one_after_another": t_solo, "speedup": t_solo / t_packed,
+                "seconds_served_streams": t_gpu, "seconds_served_with_upload": t_last,
+                "score_gain_per_s": gain / t_packed, "steps": [s["steps"] for s in packed],
+                "moves": [s["moves"] for s in packed]})
+    return out
+
+
 def serve(args):
     import concurrent.futures as cf
     import threading
@@ -287,4 +342,5 @@ def serve(args):
 
 def run(name, args):
     return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset, "batched": batched,
-            "batched-c1": batched_c1, "batched-permset": batched_permset, "serve": serve}[name](args)
+            "batched-c1": batched_c1, "batched-permset": batched_permset, "batched-many": batched_many,
+            "serve": serve}[name](args)

@@ -1026,6 +1026,90 @@ extern "C" int bm_refine_run(bm_ctx *c, int64_t max_steps, int64_t move_budget, 
     return BM_OK;
 }
 
+// Several graph pairs in one launch per chunk: a parent graph holds, for each
+// running context, its captured step graph as steps_per_check chained child
+// nodes followed by the copy of its counters; the contexts share no buffers, so
+// the branches are independent.  After each launch every context's stop rules
+// are applied exactly as bm_refine_run applies them, finished contexts leave
+// the pack and the parent graph is rebuilt for the rest.
+extern "C" int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t max_steps, int64_t move_budget,
+                                  int32_t patience, bm_refine_stats *stats) {
+    if (!ctxs || !stats || count < 1) return set_error(BM_EINVAL, "NULL argument or count < 1");
+    if (max_steps < 0 || patience < 1)
+        return set_error(BM_EINVAL, "max_steps %lld / patience %d", (long long)max_steps, patience);
+    for (int i = 0; i < count; i++) {
+        bm_ctx *c = ctxs[i];
+        if (!c || !c->batch || !c->batch->configured)
+            return set_error(BM_ESTATE, "context %d: bm_refine_configure not called", i);
+        if (c->device != ctxs[0]->device) return set_error(BM_EINVAL, "contexts on different devices");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == c) return set_error(BM_EINVAL, "context %d given twice", i);
+    }
+    BM_HIP(hipSetDevice(ctxs[0]->device));
+    int rc;
+    for (int i = 0; i < count; i++) {
+        if ((rc = refine_start(ctxs[i]))) return rc;
+        BM_HIP(hipStreamSynchronize(ctxs[i]->stream));
+        memset(&stats[i], 0, sizeof stats[i]);
+    }
+    std::vector<int> live(count);
+    std::vector<int64_t> done(count, 0);
+    for (int i = 0; i < count; i++) live[i] = i;
+    hipStream_t stream = ctxs[0]->stream;
+    while (!live.empty()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        BM_HIP(hipGraphCreate(&g, 0));
+        std::vector<int> chunk(count, 0);
+        int err = BM_OK;
+        for (int i : live) {
+            bm_batch_state *b = ctxs[i]->batch;
+            chunk[i] = (int)std::min<int64_t>(b->prm.steps_per_check, max_steps - done[i]);
+            hipGraphNode_t prev = nullptr, node = nullptr;
+            for (int s = 0; s < chunk[i] && !err; s++) {
+                if (hipGraphAddChildGraphNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, b->graph) != hipSuccess)
+                    err = set_error(BM_EHIP, "hipGraphAddChildGraphNode failed");
+                prev = node;
+            }
+            if (!err && hipGraphAddMemcpyNode1D(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, b->hst, b->st,
+                                                sizeof(RefineState), hipMemcpyDeviceToHost) != hipSuccess)
+                err = set_error(BM_EHIP, "hipGraphAddMemcpyNode1D failed");
+        }
+        if (!err && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
+            err = set_error(BM_EHIP, "hipGraphInstantiate failed");
+        if (!err && hipGraphLaunch(ge, stream) != hipSuccess) err = set_error(BM_EHIP, "hipGraphLaunch failed");
+        if (!err && hipStreamSynchronize(stream) != hipSuccess) err = set_error(BM_EHIP, "pack failed");
+        {
+            std::lock_guard<std::mutex> lk(sync_mutex());
+            if (ge) (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+        }
+        if (err) return err;
+        std::vector<int> next;
+        for (int i : live) {
+            bm_ctx *c = ctxs[i];
+            done[i] += chunk[i];
+            bm_refine_stats &st = stats[i];
+            refine_stats(c, &st);
+            bool fin = !st.verified || done[i] >= max_steps || (move_budget > 0 && st.moves >= move_budget);
+            if (!fin && st.idle_steps >= patience) {
+                st.converged = 1;
+                fin = true;
+            }
+            if (!fin) {
+                next.push_back(i);
+                continue;
+            }
+            if ((rc = refine_finish(c))) return rc;
+            if (!st.verified)
+                return set_error(BM_ERANGE, "context %d: exact score check failed at step %lld", i,
+                                 (long long)st.bad_step);
+        }
+        live.swap(next);
+    }
+    return BM_OK;
+}
+
 extern "C" int bm_refine_history(bm_ctx *c, int64_t first, int64_t count, bm_refine_step *out) {
     if (!c || (count > 0 && !out)) return set_error(BM_EINVAL, "NULL argument");
     if (!c->batch || !c->batch->configured) return set_error(BM_ESTATE, "bm_refine_configure not called");

@@ -204,6 +204,38 @@ def test_incremental_update_with_loops_dups_and_unmapped(monkeypatch):
         np.testing.assert_array_equal(e.mapping(), fin)
 
 
+def test_packed_runs_equal_solo_runs():
+    # graph pairs of uneven n in one launch per chunk: each ends exactly as alone
+    specs = [(71, 90, 900, 2), (72, 260, 3000, 3), (73, 170, 1800, 2)]
+    prms = [bmatch.refine_params(pairs=1, triples=700, sets=80 * (i % 2), set_k=4, rounds=2, steps_per_check=spc,
+                                 seed=5 + i) for i, (_, _, _, spc) in enumerate(specs)]
+    insts = [bmatch.Instance.powerlaw(sd, n, m) for sd, n, m, _ in specs]
+    solo = []
+    for inst, prm in zip(insts, prms):
+        with bmatch.Engine(0) as e:
+            e.upload(inst.n, inst.g1, inst.g2)
+            e.set_mapping(inst.p0)
+            e.refine_configure(prm)
+            st = e.refine_run(max_steps=40, patience=4)
+            solo.append((st, e.mapping()))
+    engines = []
+    try:
+        for inst, prm in zip(insts, prms):
+            e = bmatch.Engine(0)
+            engines.append(e)
+            e.upload(inst.n, inst.g1, inst.g2)
+            e.set_mapping(inst.p0)
+            e.refine_configure(prm)
+        packed = bmatch.refine_run_many(engines, max_steps=40, patience=4)
+        for inst, e, st, (sst, smap) in zip(insts, engines, packed, solo):
+            assert st == sst
+            np.testing.assert_array_equal(e.mapping(), smap)
+            assert e.score() == st["final_score"] == ob.Oracle(inst.n, inst.g1, inst.g2).score_opt(smap)
+    finally:
+        for e in engines:
+            e.close()
+
+
 def test_runs_continue_from_the_current_mapping():
     inst, o = instance(41, 160, 1700)
     with bmatch.Engine(0) as e:
