@@ -511,11 +511,19 @@ __global__ __launch_bounds__(256) void k_xdelta(const RefineState *__restrict__ 
         const uint32_t *L = out ? g2oy : g2iy;
         for (int side = 0; side < 2; side++) {
             const int g = side ? gold : gnew, sg = side ? -1 : 1;
-            for (int j = ptr[g] + lane; j < ptr[g + 1]; j += 64) {
-                const uint32_t yv = L[j];
-                const int y = yv >> 16;
-                if (y != 0xFFFF && !inU[y])
-                    atomicAdd(X + xidx(u, y, T), sg * (min(w, (int)(int16_t)(yv & 0xffff)) - wneg));
+            const int j1 = ptr[g + 1];
+            for (int j = ptr[g] + lane; j < j1; j += 256) {  // four list words in flight per lane
+                uint32_t yv[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) yv[q] = j + 64 * q < j1 ? L[j + 64 * q] : 0xFFFF0000u;
+                uint8_t in[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) in[q] = (yv[q] >> 16) == 0xFFFF ? 1 : inU[yv[q] >> 16];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (!in[q])
+                        atomicAdd(X + xidx(u, (int)(yv[q] >> 16), T),
+                                  sg * (min(w, (int)(int16_t)(yv[q] & 0xffff)) - wneg));
             }
         }
     }
@@ -551,9 +559,13 @@ __global__ __launch_bounds__(1024) void k_xcol(const RefineState *__restrict__ s
             if (s == 0xFFFF) continue;
             const int val = (int)(int16_t)(yv & 0xffff);
             const int l0 = in ? cptr[s] : cmid[s], l1 = in ? cmid[s] : cptr[s + 1];
-            for (int e = l0 + lane; e < l1; e += 64) {
-                const int2 rw = crw[e];
-                atomicAdd(&col[in ? rw.x - N1 : rw.x], min(rw.y, val) - min(rw.y, 0));
+            for (int e = l0 + lane; e < l1; e += 256) {  // four entries in flight per lane
+                int2 rw[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) rw[q] = e + 64 * q < l1 ? crw[e + 64 * q] : make_int2(-1, 0);
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (rw[q].x >= 0) atomicAdd(&col[in ? rw[q].x - N1 : rw[q].x], min(rw[q].y, val) - min(rw[q].y, 0));
             }
         }
         __syncthreads();
