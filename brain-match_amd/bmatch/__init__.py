@@ -161,7 +161,8 @@ class _RefineStats(C.Structure):
 
 
 class _RefineStep(C.Structure):
-    _fields_ = [("score", C.c_int64), ("gain", C.c_int64), ("moves", C.c_int32), ("ok", C.c_int32)]
+    _fields_ = [("score", C.c_int64), ("gain", C.c_int64), ("moves", C.c_int32), ("ok", C.c_int32),
+                ("moved", C.c_int32), ("full", C.c_int32)]
 
 
 class _ServerConfig(C.Structure):
@@ -457,7 +458,7 @@ class Engine:
     def refine_history(self, first, count):
         out = (_RefineStep * max(count, 1))()
         _check(self._L.bm_refine_history(self._h, first, count, out), "bm_refine_history")
-        return [(h.score, h.gain, h.moves, h.ok) for h in out[:count]]
+        return [(h.score, h.gain, h.moves, h.ok, h.moved, h.full) for h in out[:count]]
 
     def refine_log(self, capacity):
         _check(self._L.bm_refine_log(self._h, capacity), "bm_refine_log")

@@ -125,13 +125,14 @@ def permset(args):
 def _refine_window(e, seconds, chunk=8, max_steps=1 << 30):
     """Refinement runs of `chunk` steps until `seconds` elapsed or a run applies nothing."""
     s0 = e.score()
-    steps = moves = 0
+    steps = moves = full = 0
     kinds = [0, 0, 0]
     trace = []
     t0 = time.perf_counter()
     while True:
         st = e.refine_run(max_steps=chunk, patience=chunk)
         el = time.perf_counter() - t0
+        full += sum(h[5] for h in e.refine_history(0, st["steps"]))
         steps += st["steps"]
         moves += st["moves"]
         kinds = [kinds[0] + st["pair_moves"], kinds[1] + st["triple_moves"], kinds[2] + st["set_moves"]]
@@ -139,7 +140,8 @@ def _refine_window(e, seconds, chunk=8, max_steps=1 << 30):
         trace.append((round(el, 4), st["final_score"]))
         if st["moves"] == 0 or el >= seconds or steps >= max_steps:
             break
-    return {"seconds": el, "steps": steps, "moves": moves, "pair_moves": kinds[0], "triple_moves": kinds[1],
+    return {"seconds": el, "steps": steps, "whole_sweep_steps": full, "moves": moves, "pair_moves": kinds[0],
+            "triple_moves": kinds[1],
             "set_moves": kinds[2], "initial_score": s0, "final_score": trace[-1][1],
             "gain_per_s": (trace[-1][1] - s0) / el, "trace": trace[:: max(1, len(trace) // 12)]}
 

@@ -444,6 +444,7 @@ static void touch_set(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
 }
 
 int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of) {
+    c->map_epoch++;
     TouchSet ts;
     const bool diag = c->diag_valid;  // keep X(v,v) current where it changes
     if (diag) {
@@ -904,6 +905,7 @@ int bm_set_mapping(bm_ctx *c, const int32_t *mapping) {
             return set_error(BM_EINVAL, "mapping[%d]=%d outside [0,%d]", v, mapping[v], c->n);
     BM_HIP(hipSetDevice(c->device));
     c->hmap.assign(mapping, mapping + c->N1);
+    c->map_epoch++;
     BM_HIP(hipMemcpyAsync(c->map, c->hmap.data(), (size_t)c->N1 * 4, hipMemcpyHostToDevice, c->stream));
     // inverse of the mapping on mapped vertices; the sparse all-pairs kernel
     // needs it to be a function (p injective where p != 0) and W2 row/column 0 empty

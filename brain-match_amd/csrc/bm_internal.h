@@ -115,6 +115,9 @@ struct bm_ctx {
     int32_t *mv_verts = nullptr, *mv_delta = nullptr, *mv_aux = nullptr;
     int *mv_flag = nullptr;
 
+    // counts host-side mapping changes (set_mapping, apply_*, the end of a batched run)
+    uint64_t map_epoch = 0;
+
     // host mirror of the mapping
     std::vector<int32_t> hmap;
     std::vector<int32_t> hpinv;

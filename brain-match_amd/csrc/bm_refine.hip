@@ -66,6 +66,7 @@ struct RefineState {
     int pinc;     // this step patches the moved rows and columns of P / P^T only
     int nu;       // moved vertices this step
     int npairs;   // their comb entries (prefix ubase)
+    int full;     // this step began with a whole sweep
 };
 
 struct MoveRec {
@@ -139,6 +140,8 @@ __global__ void k_step_begin(RefineState *st) {
         st->applied = 0;
         st->nwin = 0;
         st->score_slot = 0;
+        st->nu = 0;
+        st->full = 1;  // cleared by k_clear_stale when the step's sweep had nothing to do
     }
 }
 
@@ -315,34 +318,50 @@ __global__ void k_apply(const int32_t *__restrict__ win, int max_win, CandView c
     }
 }
 
+// The single-thread kernels that rewrite the state read its fields through a
+// volatile pointer: that makes the reads vector loads, which stay ordered with
+// this thread's later vector stores to the same words.  A uniform plain load
+// may be issued through the scalar cache, and a vector store that overwrites
+// the word with a value that does not depend on the load can then land before
+// the load has returned.
 __global__ void k_step_end(RefineState *st, bm_refine_step *hist) {
     if (threadIdx.x != 0) return;
-    const long long now = (long long)st->score_slot;
-    const long long gain = (long long)st->gain;
-    const bool ok = now == st->score + gain && (st->applied == 0 || now > st->score);
-    bm_refine_step &h = hist[st->step % kHist];
+    const volatile RefineState *vs = st;
+    const long long now = (long long)vs->score_slot;
+    const long long gain = (long long)vs->gain;
+    const long long prev = vs->score;
+    const unsigned applied = vs->applied;
+    const unsigned long long step = vs->step;
+    const bool ok = now == prev + gain && (applied == 0 || now > prev);
+    bm_refine_step &h = hist[step % kHist];
     h.score = now;
     h.gain = gain;
-    h.moves = (int32_t)st->applied;
+    h.moves = (int32_t)applied;
     h.ok = ok ? 1 : 0;
-    if (!ok && !st->bad_step) st->bad_step = (unsigned)(st->step + 1);
+    h.moved = vs->nu;
+    h.full = vs->full;
+    if (!ok && !vs->bad_step) st->bad_step = (unsigned)(step + 1);
     st->score = now;
-    st->moves += st->applied;
-    st->gain_total += gain;
-    st->idle = st->applied ? 0 : st->idle + 1;
-    st->step++;
-    st->gen_step++;
+    st->moves = vs->moves + applied;
+    st->gain_total = vs->gain_total + gain;
+    st->idle = applied ? 0 : vs->idle + 1;
+    st->step = step + 1;
+    st->gen_step = vs->gen_step + 1;
 }
 
-__global__ void k_state_init(RefineState *st) {
+// keep_x: the mapping is the one the previous run ended with, so X (current
+// unless that run's last step marked it stale) carries over.  Volatile reads
+// as in k_step_end.
+__global__ void k_state_init(RefineState *st, int keep_x) {
     if (threadIdx.x != 0) return;
-    const long long s = (long long)st->score_slot;
-    const unsigned long long g = st->gen_step;
-    memset(st, 0, sizeof(RefineState));
-    st->score = s;
-    st->initial_score = s;
-    st->gen_step = g;
-    st->xstale = 1;
+    const volatile RefineState *vs = st;
+    const long long s = (long long)vs->score_slot;
+    RefineState z = {};
+    z.score = s;
+    z.initial_score = s;
+    z.gen_step = vs->gen_step;
+    z.xstale = keep_x ? vs->xstale : 1;
+    *st = z;
 }
 
 // ---------------------------------------------------------------- incremental X
@@ -582,7 +601,11 @@ __global__ void k_diag_all(const RefineState *__restrict__ st, const int32_t *__
 }
 
 __global__ void k_clear_stale(RefineState *st) {
-    if (threadIdx.x == 0) st->xstale = 0;
+    if (threadIdx.x == 0) {
+        const int stale = ((const volatile RefineState *)st)->xstale;  // as in k_step_end
+        st->full = stale;  // the gain kernels ran whole iff X was stale
+        st->xstale = 0;
+    }
 }
 
 }  // namespace
@@ -600,6 +623,7 @@ struct bm_batch_state {
     int32_t *ulist = nullptr, *uold = nullptr, *ubase = nullptr;
     uint8_t *inU = nullptr;
     bool incremental = false;  // the captured step updates X incrementally
+    uint64_t x_epoch = ~0ull;  // map_epoch at the end of the last run (X then matched the mapping)
     int inc_max = 0;
     int pinc_max = 0;
     long long inc_budget = 0;  // list work an incremental update may take (k_collect)
@@ -943,7 +967,8 @@ int refine_start(bm_ctx *c) {
     BM_HIP(hipMemsetAsync(&b->st->score_slot, 0, 8, c->stream));
     int rc = launch_score(c, &b->st->score_slot);
     if (rc) return rc;
-    BM_LAUNCH(c, "refine_init", hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, c->stream, b->st));
+    const int keep_x = b->x_epoch == c->map_epoch && b->incremental ? 1 : 0;
+    BM_LAUNCH(c, "refine_init", hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, c->stream, b->st, keep_x));
     b->run_steps = 0;
     b->in_flight = false;
     return BM_OK;
@@ -997,6 +1022,8 @@ int refine_finish(bm_ctx *c) {
     }
     c->diag_valid = false;
     c->g2y_dirty = !c->sparse_ok;
+    c->map_epoch++;  // the device moved the mapping; X matches it unless the last step left it stale
+    c->batch->x_epoch = c->map_epoch;
     return BM_OK;
 }
 
@@ -1129,10 +1156,15 @@ extern "C" int bm_refine_history(bm_ctx *c, int64_t first, int64_t count, bm_ref
     if (first < 0 || count < 0 || first + count > steps || (steps - first) > kHist)
         return set_error(BM_ERANGE, "steps [%lld, %lld) not held (last %d of %lld kept)", (long long)first,
                          (long long)(first + count), kHist, (long long)steps);
-    std::vector<bm_refine_step> h(kHist);
     BM_HIP(hipSetDevice(c->device));
-    BM_HIP(hipMemcpy(h.data(), c->batch->hist, sizeof(bm_refine_step) * kHist, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < count; i++) out[i] = h[(first + i) % kHist];
+    // the requested steps occupy at most two runs of the ring
+    for (int64_t done = 0; done < count;) {
+        const int64_t slot = (first + done) % kHist;
+        const int64_t n = std::min<int64_t>(count - done, kHist - slot);
+        BM_HIP(hipMemcpy(out + done, c->batch->hist + slot, sizeof(bm_refine_step) * (size_t)n,
+                         hipMemcpyDeviceToHost));
+        done += n;
+    }
     return BM_OK;
 }
 

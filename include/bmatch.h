@@ -208,6 +208,8 @@ typedef struct bm_refine_step {
     int64_t gain;  /* sum of the applied deltas */
     int32_t moves; /* moves applied */
     int32_t ok;    /* score == previous score + gain, and rose if moves > 0 */
+    int32_t moved; /* vertices whose image changed */
+    int32_t full;  /* 1: the step began with a whole all-pairs sweep; 0: the gain matrix was kept current */
 } bm_refine_step;
 
 /* Allocates the step's buffers, autotunes kernel variants (results do not

@@ -81,7 +81,8 @@ def replay(n, g1, o, mp0, recs, stats, hist, check_rows=True):
         new = o.score_opt(nxt)
         assert new == score + gain
         h = hist[step]
-        assert h == (new, gain, len(ms), 1), (step, h)
+        assert h[:4] == (new, gain, len(ms), 1), (step, h)
+        assert h[4] in (0, sum(int(r[2]) for r in ms))  # moved vertices (counted with the incremental update)
         cur, score = nxt, new
     assert score == stats["final_score"]
     assert kinds == [stats["pair_moves"], stats["triple_moves"], stats["set_moves"]]
@@ -259,6 +260,28 @@ def test_runs_continue_from_the_current_mapping():
         e.set_mapping(inst.p0)
         d = e.refine_run(max_steps=1, patience=5)
         assert d["initial_score"] == o.score_opt(inst.p0)
+
+
+def test_back_to_back_runs_equal_one_run():
+    # a run that starts from the mapping the previous one left keeps its gain matrix
+    inst, o = instance(81, 240, 2800)
+    prm = bmatch.refine_params(pairs=1, triples=600, sets=60, set_k=4, rounds=2, steps_per_check=3, seed=8)
+    ends = []
+    for parts in ((6,), (3, 3), (2, 2, 2)):
+        with bmatch.Engine(0) as e:
+            e.upload(inst.n, inst.g1, inst.g2)
+            e.set_mapping(inst.p0)
+            e.refine_configure(prm)
+            e.refine_log(1 << 14)
+            for k in parts:
+                st = e.refine_run(max_steps=k, patience=10)
+                assert st["verified"] == 1 and st["steps"] == k
+                assert st["final_score"] == o.score_opt(e.mapping())
+            hist = e.refine_history(0, parts[-1])
+            if len(parts) > 1:
+                assert all(h[5] == 0 for h in hist)  # no whole sweep: X carried over
+            ends.append((e.mapping().tobytes(), st["final_score"]))
+    assert ends[0] == ends[1] == ends[2]
 
 
 def test_move_budget_stops_at_check():
