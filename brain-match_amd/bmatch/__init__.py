@@ -22,6 +22,8 @@ BM_ROW_FIRST_POSITIVE = 0
 BM_ROW_ARGMAX = 1
 BM_ROT3_ROTATIONS = 0
 BM_ROT3_ALL = 1
+BM_PERTURB_OMP = 0
+BM_PERTURB_WEIGHTED = 1
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lib = None
@@ -63,6 +65,9 @@ def lib():
         "bm_eval_permset": (C.c_int, [C.c_void_p, C.c_int32, _i32p, C.c_int64, _i32p, _i32p]),
         "bm_perm_from_rank": (C.c_int, [C.c_int32, C.c_int32, _i32p]),
         "bm_eval_swaps_hillclimb": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
+        "bm_eval_rows": (C.c_int, [C.c_void_p, _i32p, C.c_int32, _i32p]),
+        "bm_perturb": (C.c_int, [C.c_void_p, C.c_int32, C.c_int, C.POINTER(C.c_uint64), C.c_void_p]),
+        "bm_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
         "bm_apply_swap": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
         "bm_apply_rot3": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
         "bm_apply_perm": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p]),
@@ -106,6 +111,8 @@ def lib():
         "bm_drive_row_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int64, C.c_void_p]),
         "bm_drive_global_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
+        "bm_drive_row_best_restart": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "bm_drive_quick": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -158,6 +165,26 @@ class _RefineStats(C.Structure):
                 ("triple_moves", C.c_int64), ("set_moves", C.c_int64), ("initial_score", C.c_int64),
                 ("final_score", C.c_int64), ("idle_steps", C.c_int64), ("bad_step", C.c_int64),
                 ("converged", C.c_int32), ("verified", C.c_int32)]
+
+
+class _RestartParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("initial_swaps", C.c_int32), ("restart_swaps", C.c_int32),
+                ("idle_passes", C.c_int32), ("verify", C.c_int32), ("max_passes", C.c_int64)]
+
+
+class _RestartStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("passes", "restarts", "accepted", "evals_rows", "initial_score",
+                                         "final_score", "best_score", "mismatches")]
+
+
+class _QuickParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("max_evals", C.c_int64), ("zero_limit", C.c_int32),
+                ("best_window", C.c_int32), ("gap", C.c_double), ("gap_norm", C.c_double)]
+
+
+class _QuickStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("evals", "perturbations", "moves", "positive_moves", "zero_moves",
+                                         "initial_score", "final_score", "best_score")]
 
 
 class _RefineStep(C.Structure):
@@ -301,6 +328,13 @@ def write_mapping_mf(path, mapping):
     _check(lib().bm_mapping_write_mf(os.fsencode(path), m, len(m) - 1), "bm_mapping_write_mf")
 
 
+def rng_next(state):
+    """bm_rng_next (splitmix64): returns (value, advanced state)."""
+    st = C.c_uint64(state)
+    v = lib().bm_rng_next(C.byref(st))
+    return v, st.value
+
+
 def gen_sets(seed, n, k, count):
     out = np.zeros(k * count, np.int32)
     _check(lib().bm_gen_sets(seed, n, k, count, out), "bm_gen_sets")
@@ -409,6 +443,37 @@ class Engine:
         r = np.zeros(cnt, np.int32)
         _check(self._L.bm_eval_permset(self._h, k, s.reshape(-1), cnt, d, r), "bm_eval_permset")
         return d, r
+
+    def eval_rows(self, rows):
+        """bm_eval_rows: Delta_B(rows[i], b) for b = 0..n as a (len(rows), n+1) array."""
+        r = _i32(rows)
+        out = np.zeros((len(r), self.n + 1), np.int32)
+        _check(self._L.bm_eval_rows(self._h, r, len(r), out.reshape(-1)), "bm_eval_rows")
+        return out
+
+    def perturb(self, k, mode, rng_state):
+        """bm_perturb: returns (the 2k drawn vertices, the advanced generator state)."""
+        st = C.c_uint64(rng_state)
+        sel = np.zeros(2 * k, np.int32)
+        _check(self._L.bm_perturb(self._h, k, mode, C.byref(st), sel.ctypes.data_as(C.c_void_p)), "bm_perturb")
+        return sel, st.value
+
+    def row_best_restart(self, seed=1, initial_swaps=100, restart_swaps=50, idle_passes=2, verify=True,
+                         max_passes=4):
+        prm = _RestartParams(seed, initial_swaps, restart_swaps, idle_passes, 1 if verify else 0, max_passes)
+        best = np.zeros(self.n + 1, np.int32)
+        st = _RestartStats()
+        _check(self._L.bm_drive_row_best_restart(self._h, C.byref(prm), best.ctypes.data_as(C.c_void_p),
+                                                 C.byref(st)), "bm_drive_row_best_restart")
+        return best, {f: getattr(st, f) for f, _ in _RestartStats._fields_}
+
+    def quick(self, seed=1, max_evals=1000, zero_limit=1000, best_window=500, gap=0.002, gap_norm=5154247.0):
+        prm = _QuickParams(seed, max_evals, zero_limit, best_window, gap, gap_norm)
+        best = np.zeros(self.n + 1, np.int32)
+        st = _QuickStats()
+        _check(self._L.bm_drive_quick(self._h, C.byref(prm), best.ctypes.data_as(C.c_void_p), C.byref(st)),
+               "bm_drive_quick")
+        return best, {f: getattr(st, f) for f, _ in _QuickStats._fields_}
 
     def apply_swap(self, a, b):
         _check(self._L.bm_apply_swap(self._h, a, b), "bm_apply_swap")

@@ -564,6 +564,7 @@ int bm_create(int device, bm_ctx **out) {
 
 static void free_graph_buffers(bm_ctx *c) {
     size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
+    c->hvscore.clear();
     dev_free(c, (void **)&c->W2, 2 * N1 * ld * 2);
     dev_free(c, (void **)&c->Q, 2 * N1 * ld * 2);
     dev_free(c, (void **)&c->map, N1 * 4);
@@ -1001,6 +1002,124 @@ int bm_apply_perm(bm_ctx *c, int32_t k, const int32_t *set, const int32_t *perm)
     int rc = permute_rows_cols(c, k, set, perm);
     if (rc) return rc;
     BM_HIP(hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+uint64_t bm_rng_next(uint64_t *state) {
+    uint64_t z = (*state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // extern "C"
+
+// Perturbation weights of quick.cu:766-793: 0.7 x out + 0.3 x in, where out
+// (in) sums the positive dense G1 cells W1[v][j] (W1[j][v]) over j = 1..n in
+// ascending j, in float.  The cells are the de-duplicated lists' weights as
+// int16 (the reference's short matrix, last line wins); they are copied back
+// from the device once and sorted by j so that the float sums round in the
+// reference's order.
+static int vertex_scores(bm_ctx *c) {
+    if (!c->hvscore.empty()) return BM_OK;
+    const int N1 = c->N1;
+    std::vector<int32_t> ptr(N1 + 1), mid(N1), r((size_t)c->ndd), w((size_t)c->ndd);
+    BM_HIP(hipMemcpyAsync(ptr.data(), c->ddptr, (size_t)(N1 + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipMemcpyAsync(mid.data(), c->ddmid, (size_t)N1 * 4, hipMemcpyDeviceToHost, c->stream));
+    if (c->ndd > 0) {
+        BM_HIP(hipMemcpyAsync(r.data(), c->ddr, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipMemcpyAsync(w.data(), c->ddw, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    BM_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> score(N1, 0.f);
+    std::vector<std::pair<int, int>> cells;
+    auto sum_sorted = [&](int e0, int e1, bool out) {
+        cells.clear();
+        for (int e = e0; e < e1; e++) {
+            const int j = out ? r[e] - N1 : r[e];
+            const int cell = (int16_t)w[e];
+            if (j >= 1 && cell > 0) cells.emplace_back(j, cell);
+        }
+        std::sort(cells.begin(), cells.end());
+        float s = 0.f;
+        for (const auto &jc : cells) s += (float)jc.second;
+        return s;
+    };
+    for (int v = 1; v < N1; v++) {
+        const float out_w = sum_sorted(ptr[v], mid[v], true);
+        const float in_w = sum_sorted(mid[v], ptr[v + 1], false);
+        const float a = 0.7f * out_w;  // two statements: no fused multiply-add
+        const float b = 0.3f * in_w;
+        score[v] = a + b;
+    }
+    c->hvscore.swap(score);
+    return BM_OK;
+}
+
+// quick.cu:796-814: r = U[0,1] x (remaining total); the first unused vertex
+// whose running sum reaches r, else the first unused vertex.
+static int draw_weighted(const std::vector<float> &score, const std::vector<uint8_t> &used, int n, float total,
+                         uint64_t *rng) {
+    const float u = (float)(bm_rng_next(rng) >> 40) * (1.0f / 16777216.0f);
+    const float r = u * total;
+    float cum = 0.f;
+    for (int i = 1; i <= n; i++)
+        if (!used[i]) {
+            cum += score[i];
+            if (cum >= r) return i;
+        }
+    for (int i = 1; i <= n; i++)
+        if (!used[i]) return i;
+    return 1;
+}
+
+extern "C" {
+
+int bm_perturb(bm_ctx *c, int32_t k, int mode, uint64_t *rng, int32_t *selected) {
+    if (!c || !rng) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (mode != BM_PERTURB_OMP && mode != BM_PERTURB_WEIGHTED) return set_error(BM_EINVAL, "bad mode %d", mode);
+    const int n = c->n;
+    if (k < 1 || 2 * (int64_t)k > n) return set_error(BM_EINVAL, "k=%d: need 1 <= k and 2k <= n=%d", k, n);
+    BM_HIP(hipSetDevice(c->device));
+    std::vector<int32_t> sel;
+    sel.reserve(2 * (size_t)k);
+    std::vector<uint8_t> used((size_t)n + 1, 0);
+    if (mode == BM_PERTURB_OMP) {  // alignment-omp.c:506-513: rand() % n
+        while ((int)sel.size() < 2 * k) {
+            const int idx = (int)(bm_rng_next(rng) % (uint64_t)n);
+            if (!used[idx]) {
+                used[idx] = 1;
+                sel.push_back(idx);
+            }
+        }
+    } else {
+        int rc = vertex_scores(c);
+        if (rc) return rc;
+        float total = 0.f;  // quick.cu:834-838
+        for (int i = 1; i <= n; i++) total += c->hvscore[i];
+        while ((int)sel.size() < k) {  // :842-850
+            const int idx = draw_weighted(c->hvscore, used, n, total, rng);
+            used[idx] = 1;
+            sel.push_back(idx);
+            total -= c->hvscore[idx];
+        }
+        while ((int)sel.size() < 2 * k) {  // :853-860: rand() % n + 1
+            const int idx = (int)(bm_rng_next(rng) % (uint64_t)n) + 1;
+            if (!used[idx]) {
+                used[idx] = 1;
+                sel.push_back(idx);
+            }
+        }
+    }
+    // disjoint swaps sel[i] <-> sel[i + k]: one row/column patch each, one sync
+    for (int i = 0; i < k; i++) {
+        int32_t set[2] = {sel[i], sel[i + k]}, src[2] = {1, 0};
+        int rc = permute_rows_cols(c, 2, set, src);
+        if (rc) return rc;
+    }
+    BM_HIP(hipStreamSynchronize(c->stream));
+    if (selected) memcpy(selected, sel.data(), sizeof(int32_t) * 2 * (size_t)k);
     return BM_OK;
 }
 

@@ -123,6 +123,7 @@ struct bm_ctx {
     std::vector<int32_t> hpinv;
     std::vector<int32_t> hcdeg;   // comb-list length per vertex
     std::vector<int32_t> hg2deg;  // G2 in- plus out-degree per G2 vertex (de-duplicated)
+    std::vector<float> hvscore;   // perturbation weights per G1 vertex (computed on first use)
     std::vector<int32_t> hg2optr, hg2iptr;  // host copies of the G2 list offsets
 
     // kernel timing

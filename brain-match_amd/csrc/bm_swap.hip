@@ -107,7 +107,8 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
                                                   const int16_t *__restrict__ Q, const int32_t *__restrict__ cptr,
                                                   const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
                                                   const int32_t *__restrict__ cdw, const int32_t *__restrict__ diag,
-                                                  unsigned long long *__restrict__ result) {
+                                                  unsigned long long *__restrict__ result,
+                                                  int32_t *__restrict__ store) {
     extern __shared__ int16_t colA[];  // colA[r] = Q[r][a], r < 2*N1
     __shared__ int push[256];
     __shared__ int pull[kRowCh];
@@ -176,6 +177,13 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
     }
     __syncthreads();
 
+    if (store) {  // whole row: every delta, no reduction
+        if (tid < nbk) {
+            const int b = b0 + tid;
+            store[b] = b == a || b == 0 ? 0 : push[tid] + push[tid + kRowCh] + pull[tid] - diag[a] - diag[b] + corr[tid];
+        }
+        return;
+    }
     unsigned long long key = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
     if (tid < nbk) {
         const int b = b0 + tid;
@@ -303,9 +311,18 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
 __global__ __launch_bounds__(256) void k_row_final(int a, int lo, int hi, int mode, int *__restrict__ acc,
                                                    const int32_t *__restrict__ cneg,
                                                    const int32_t *__restrict__ diag,
-                                                   unsigned long long *__restrict__ result) {
+                                                   unsigned long long *__restrict__ result,
+                                                   int32_t *__restrict__ store) {
     __shared__ unsigned long long red[4];
     const int b = lo + blockIdx.x * 256 + threadIdx.x;
+    if (store) {  // whole row: every delta, no reduction
+        if (b <= hi) {
+            const int x = acc[b];
+            acc[b] = 0;
+            store[b] = b == a || b == 0 ? 0 : x + cneg[a] + cneg[b] - diag[a] - diag[b];
+        }
+        return;
+    }
     unsigned long long key = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
     if (b <= hi) {
         const int x = acc[b];
@@ -1119,7 +1136,8 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
                                                       (const int2 *)c->crw, c->cdw, c->map, c->g2optr, c->g2oy,
                                                       c->g2iptr, c->g2iy, c->Q, c->ld, c->rowacc, res));
         BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((b_hi - b_lo + 256) / 256), dim3(256), 0,
-                                                     c->stream, a, b_lo, b_hi, mode, c->rowacc, c->cneg, c->diag, res));
+                                                     c->stream, a, b_lo, b_hi, mode, c->rowacc, c->cneg, c->diag, res,
+                                                     nullptr));
     } else {
         BM_HIP(hipMemsetAsync(res, mode == BM_ROW_FIRST_POSITIVE ? 0xFF : 0x00, 8, c->stream));
         const int blocks = (b_hi - b_lo + kRowCh) / kRowCh;
@@ -1127,11 +1145,11 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
         if (lds <= 96 * 1024) {
             BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<true>, dim3(blocks), dim3(256), lds, c->stream, a,
                                                         b_lo, b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw,
-                                                        c->cdw, c->diag, res));
+                                                        c->cdw, c->diag, res, nullptr));
         } else {
             BM_LAUNCH(c, "row_eval", hipLaunchKernelGGL(k_row_eval<false>, dim3(blocks), dim3(256), 0, c->stream, a,
                                                         b_lo, b_hi, mode, c->N1, c->ld, c->Q, c->cptr, c->cr, c->cw,
-                                                        c->cdw, c->diag, res));
+                                                        c->cdw, c->diag, res, nullptr));
         }
     }
     BM_HIP(hipMemcpyAsync(c->hres, res, 8, hipMemcpyDeviceToHost, c->stream));  // pinned: no staging copy
@@ -1146,6 +1164,60 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
         *best_delta = (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
         *best_b = (int32_t)(0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFu));
     }
+    return BM_OK;
+}
+
+// Whole rows of the changed vertices (quick.cu:141-250 evaluates exactly
+// the pairs that touch them): the row kernels above with their reduction
+// replaced by a store of every delta, rows one after another on the stream
+// into the staging buffer, then one copy to the caller.
+extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32_t *deltas) {
+    if (!c || (count > 0 && (!rows || !deltas))) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (count < 0 || count > 64) return set_error(BM_EINVAL, "count=%d outside [0,64]", count);
+    for (int i = 0; i < count; i++) {
+        if (rows[i] < 1 || rows[i] > c->n) return set_error(BM_EINVAL, "rows[%d]=%d outside [1,%d]", i, rows[i], c->n);
+        for (int j = 0; j < i; j++)
+            if (rows[j] == rows[i]) return set_error(BM_EINVAL, "row %d repeated", rows[i]);
+    }
+    if (count == 0) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    int rc = ensure_diag(c);
+    if (rc) return rc;
+    const size_t N1 = (size_t)c->N1;
+    if ((rc = ensure_stage(c, (size_t)count * N1 * 4))) return rc;
+    int32_t *out = (int32_t *)c->stage;
+    unsigned long long *res = c->keys + c->N1 + 1;  // written by the expand kernel, not read here
+    if (c->sparse_ok && c->g2y_dirty && (rc = launch_refresh_g2y(c))) return rc;
+    const int n = c->n;
+    for (int i = 0; i < count; i++) {
+        const int a = rows[i];
+        int32_t *row = out + (size_t)i * N1;
+        if (c->sparse_ok) {
+            const int heads = c->hcdeg[a] + c->hg2deg[c->hmap[a]];
+            BM_LAUNCH(c, "row_expand",
+                      hipLaunchKernelGGL(k_row_expand, dim3(1 + (heads + 3) / 4), dim3(256), 0, c->stream, a, 0, n,
+                                         BM_ROW_ARGMAX, c->N1, c->cptr, c->cmid, (const int2 *)c->crw, c->cdw, c->map,
+                                         c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->rowacc, res));
+            BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_final, dim3((n + 256) / 256), dim3(256), 0, c->stream,
+                                                         a, 0, n, BM_ROW_ARGMAX, c->rowacc, c->cneg, c->diag, res,
+                                                         row));
+        } else {
+            const int blocks = (n + kRowCh) / kRowCh;
+            const size_t lds = (size_t)2 * c->N1 * sizeof(int16_t);
+            if (lds <= 96 * 1024) {
+                BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_eval<true>, dim3(blocks), dim3(256), lds, c->stream,
+                                                             a, 0, n, BM_ROW_ARGMAX, c->N1, c->ld, c->Q, c->cptr,
+                                                             c->cr, c->cw, c->cdw, c->diag, res, row));
+            } else {
+                BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_eval<false>, dim3(blocks), dim3(256), 0, c->stream,
+                                                             a, 0, n, BM_ROW_ARGMAX, c->N1, c->ld, c->Q, c->cptr,
+                                                             c->cr, c->cw, c->cdw, c->diag, res, row));
+            }
+        }
+    }
+    BM_HIP(hipMemcpyAsync(deltas, out, (size_t)count * N1 * 4, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
     return BM_OK;
 }
 

@@ -4,6 +4,7 @@
  * delta evaluation moves to the GPU.
  */
 #include <limits.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "bm_host.h"
@@ -127,4 +128,212 @@ int bm_drive_global_best(bm_ctx *ctx, int64_t max_steps, int verify, int32_t *ac
     }
     if (stats) *stats = st;
     return BM_OK;
+}
+
+/* alignment-omp.c:543-678: row-best passes, perturbation at the start and
+ * after an idle pass, the best mapping kept */
+int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *best_mapping,
+                              bm_restart_stats *stats) {
+    if (!ctx || !p || p->max_passes < 1 || p->initial_swaps < 0 || p->restart_swaps < 1 || p->idle_passes < 0)
+        return BM_EINVAL;
+    bm_restart_stats st;
+    memset(&st, 0, sizeof st);
+    const int32_t n = bm_num_vertices(ctx);
+    int32_t *best = malloc(sizeof(int32_t) * ((size_t)n + 1));
+    if (!best) return BM_ENOMEM;
+    uint64_t rng = p->seed;
+    int64_t cur = 0;
+    int rc;
+    if ((rc = bm_get_mapping(ctx, best)) || (rc = bm_score(ctx, &st.initial_score))) goto out;
+    st.best_score = st.initial_score;
+    if (p->initial_swaps > 0 && (rc = bm_perturb(ctx, p->initial_swaps, BM_PERTURB_OMP, &rng, NULL))) goto out;
+    if ((rc = bm_score(ctx, &cur))) goto out;
+    int64_t pass = 0, improvements = 0, last = 0;
+    while (st.passes < p->max_passes) {
+        for (int32_t a = 1; a < n; a++) { /* :571-655; row n has no b > a */
+            int32_t b, d;
+            if ((rc = bm_eval_swap_row(ctx, a, a + 1, n, BM_ROW_ARGMAX, &b, &d))) goto out;
+            st.evals_rows++;
+            if (d <= 0) continue;
+            if ((rc = bm_apply_swap(ctx, a, b))) goto out;
+            cur += d;
+            improvements++;
+            st.accepted++;
+            if (p->verify) { /* :645-652, the rescore is trusted */
+                int64_t s;
+                if ((rc = bm_score(ctx, &s))) goto out;
+                if (s != cur) {
+                    st.mismatches++;
+                    cur = s;
+                }
+            }
+            if (cur > st.best_score) { /* :627-636 */
+                st.best_score = cur;
+                if ((rc = bm_get_mapping(ctx, best))) goto out;
+            }
+        }
+        pass++;
+        st.passes++;
+        if (pass > p->idle_passes && improvements - last == 0) { /* :660-673 */
+            if ((rc = bm_set_mapping(ctx, best))) goto out;
+            if ((rc = bm_perturb(ctx, p->restart_swaps, BM_PERTURB_OMP, &rng, NULL))) goto out;
+            if ((rc = bm_score(ctx, &cur))) goto out;
+            improvements = last = pass = 0;
+            st.restarts++;
+        }
+        if (last < improvements) last = improvements; /* :674-676 */
+    }
+    st.final_score = cur;
+    rc = bm_set_mapping(ctx, best);
+out:
+    if (!rc && best_mapping) memcpy(best_mapping, best, sizeof(int32_t) * ((size_t)n + 1));
+    free(best);
+    if (!rc && stats) *stats = st;
+    return rc;
+}
+
+/* Pairs quick.cu's kernel evaluates after a change (:151-196), in its
+ * numbering: inside the sorted changed set (i < j), then each changed vertex
+ * with every unchanged one ascending.  Scans them like :1172-1191 / :1280-1299:
+ * the first strict maximum above 0, the positive pairs and the zero pairs in
+ * scan order. */
+typedef struct {
+    int32_t *rows;       /* nc * (n + 1) deltas from bm_eval_rows */
+    int32_t *pos, *zero; /* pairs (2 ints each) */
+    int64_t npos, nzero;
+    int32_t best_a, best_b, max_delta;
+    uint8_t *in_set;
+} quick_scan;
+
+static int quick_eval(bm_ctx *ctx, int32_t n, const int32_t *changed, int nc, quick_scan *q) {
+    int rc = bm_eval_rows(ctx, changed, nc, q->rows);
+    if (rc) return rc;
+    const size_t N1 = (size_t)n + 1;
+    q->npos = q->nzero = 0;
+    q->max_delta = 0;
+    q->best_a = q->best_b = 0;
+    for (int i = 0; i < nc; i++) q->in_set[changed[i]] = 1;
+#define QUICK_PAIR(A, B, D)                     \
+    do {                                        \
+        const int32_t d_ = (D);                 \
+        if (d_ > q->max_delta) {                \
+            q->max_delta = d_;                  \
+            q->best_a = (A);                    \
+            q->best_b = (B);                    \
+        }                                       \
+        if (d_ > 0) {                           \
+            q->pos[2 * q->npos] = (A);          \
+            q->pos[2 * q->npos + 1] = (B);      \
+            q->npos++;                          \
+        } else if (d_ == 0) {                   \
+            q->zero[2 * q->nzero] = (A);        \
+            q->zero[2 * q->nzero + 1] = (B);    \
+            q->nzero++;                         \
+        }                                       \
+    } while (0)
+    for (int i = 0; i < nc; i++)
+        for (int j = i + 1; j < nc; j++) QUICK_PAIR(changed[i], changed[j], q->rows[i * N1 + changed[j]]);
+    for (int i = 0; i < nc; i++)
+        for (int32_t v = 1; v <= n; v++)
+            if (!q->in_set[v]) QUICK_PAIR(changed[i], v, q->rows[i * N1 + v]);
+#undef QUICK_PAIR
+    for (int i = 0; i < nc; i++) q->in_set[changed[i]] = 0;
+    return BM_OK;
+}
+
+static void sort_small(int32_t *a, int m) {
+    for (int i = 1; i < m; i++)
+        for (int j = i; j > 0 && a[j - 1] > a[j]; j--) {
+            int32_t t = a[j];
+            a[j] = a[j - 1];
+            a[j - 1] = t;
+        }
+}
+
+/* quick.cu:1137-1324 */
+int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping, bm_quick_stats *stats) {
+    if (!ctx || !p || p->max_evals < 1 || p->zero_limit < 1 || p->best_window < 0 || !(p->gap_norm > 0))
+        return BM_EINVAL;
+    const int32_t n = bm_num_vertices(ctx);
+    if (n < 8) return BM_EINVAL; /* 4 swaps need 8 distinct vertices */
+    bm_quick_stats st;
+    memset(&st, 0, sizeof st);
+    const size_t N1 = (size_t)n + 1;
+    const int64_t max_pairs = 8 * (int64_t)n;
+    quick_scan q;
+    memset(&q, 0, sizeof q);
+    int32_t *best = malloc(sizeof(int32_t) * N1);
+    q.rows = malloc(sizeof(int32_t) * 8 * N1);
+    q.pos = malloc(sizeof(int32_t) * 2 * (size_t)max_pairs);
+    q.zero = malloc(sizeof(int32_t) * 2 * (size_t)max_pairs);
+    q.in_set = calloc(N1, 1);
+    int rc = BM_OK;
+    if (!best || !q.rows || !q.pos || !q.zero || !q.in_set) {
+        rc = BM_ENOMEM;
+        goto out;
+    }
+    uint64_t rng = p->seed;
+    int64_t cur, last = 0, iter = 0;
+    if ((rc = bm_get_mapping(ctx, best)) || (rc = bm_score(ctx, &cur))) goto out;
+    st.initial_score = st.best_score = cur;
+    while (iter < p->max_evals) {
+        /* :1139-1171 perturbation (max_delta == 0 here) */
+        const int k = 3 + (int)(bm_rng_next(&rng) % 2);
+        if (iter - last < p->best_window || (double)(st.best_score - cur) / p->gap_norm > p->gap) {
+            if ((rc = bm_set_mapping(ctx, best))) goto out;
+        }
+        int32_t changed[8];
+        if ((rc = bm_perturb(ctx, k, BM_PERTURB_WEIGHTED, &rng, changed))) goto out;
+        sort_small(changed, 2 * k);
+        if ((rc = bm_score(ctx, &cur))) goto out;
+        st.perturbations++;
+        if ((rc = quick_eval(ctx, n, changed, 2 * k, &q))) goto out;
+        iter++;
+        int zerocheck = 0;
+        /* :1203-1318 */
+        while ((q.max_delta > 0 || q.nzero > 0) && zerocheck < p->zero_limit && iter < p->max_evals) {
+            int32_t m1, m2;
+            if (q.max_delta > 0) {
+                if (bm_rng_next(&rng) % 2 == 0) {
+                    const int64_t pick = (int64_t)(bm_rng_next(&rng) % (uint64_t)q.npos);
+                    m1 = q.pos[2 * pick];
+                    m2 = q.pos[2 * pick + 1];
+                } else {
+                    m1 = q.best_a;
+                    m2 = q.best_b;
+                }
+                zerocheck = 0;
+                st.positive_moves++;
+            } else {
+                const int64_t pick = (int64_t)(bm_rng_next(&rng) % (uint64_t)q.nzero);
+                m1 = q.zero[2 * pick];
+                m2 = q.zero[2 * pick + 1];
+                zerocheck++;
+                st.zero_moves++;
+            }
+            if ((rc = bm_apply_swap(ctx, m1, m2)) || (rc = bm_score(ctx, &cur))) goto out;
+            st.moves++;
+            if (cur > st.best_score) {
+                st.best_score = cur;
+                if ((rc = bm_get_mapping(ctx, best))) goto out;
+                last = iter;
+            }
+            changed[0] = m1 < m2 ? m1 : m2;
+            changed[1] = m1 < m2 ? m2 : m1;
+            if ((rc = quick_eval(ctx, n, changed, 2, &q))) goto out;
+            iter++;
+        }
+    }
+    st.evals = iter;
+    st.final_score = cur;
+    rc = bm_set_mapping(ctx, best);
+out:
+    if (!rc && best_mapping) memcpy(best_mapping, best, sizeof(int32_t) * N1);
+    if (!rc && stats) *stats = st;
+    free(best);
+    free(q.rows);
+    free(q.pos);
+    free(q.zero);
+    free(q.in_set);
+    return rc;
 }

@@ -143,6 +143,68 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
 int bm_drive_global_best(bm_ctx *ctx, int64_t max_steps, int verify, int32_t *acc, int64_t acc_cap,
                          bm_drive_stats *stats);
 
+/* Row-best passes with the stagnation restart of alignment-omp.c:543-678:
+ * the start mapping is perturbed by initial_swaps swaps (:564); after each
+ * pass, when more than idle_passes passes ran since the last restart and
+ * the pass accepted nothing (:660), the best mapping so far is restored and
+ * perturbed by restart_swaps swaps (:669-670).  Perturbations are
+ * bm_perturb(BM_PERTURB_OMP) from a splitmix64 state seeded with seed.  The
+ * score is re-verified after each accept when verify != 0 (:645-652).
+ * Stops after max_passes passes (> 0) and leaves the context on the best
+ * mapping seen (the one the reference writes out); best_mapping (n+1
+ * entries, may be NULL) receives it. */
+typedef struct {
+    uint64_t seed;
+    int32_t initial_swaps; /* 100 in the reference */
+    int32_t restart_swaps; /* 50 */
+    int32_t idle_passes;   /* 2: restart when pass > 2 */
+    int32_t verify;
+    int64_t max_passes;
+} bm_restart_params;
+
+typedef struct {
+    int64_t passes, restarts, accepted, evals_rows;
+    int64_t initial_score; /* of the start mapping, before the first perturbation */
+    int64_t final_score;   /* of the mapping the run ends on, before the best is restored */
+    int64_t best_score;
+    int64_t mismatches;
+} bm_restart_stats;
+
+int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *params, int32_t *best_mapping,
+                              bm_restart_stats *stats);
+
+/* The changed-vertex search of quick.cu:1137-1324 (one process, no
+ * best-file synchronisation): perturb by 3 or 4 swaps
+ * (bm_perturb(BM_PERTURB_WEIGHTED)), starting from the best mapping when the
+ * last improvement is fewer than best_window evaluations old or the current
+ * score trails the best by more than gap x gap_norm (:1141); evaluate every
+ * pair touching a changed vertex (bm_eval_rows); then, while a pair has
+ * Delta > 0 or = 0 and fewer than zero_limit zero moves ran in a row: with
+ * Delta > 0 available take a random positive pair or the best pair (even
+ * odds), else a random zero pair; swap, rescore, keep the best, evaluate the
+ * two changed rows again.  Pairs are enumerated as the reference's kernel
+ * numbers them (pairs inside the changed set, then changed x unchanged
+ * ascending); the best is the first strict maximum.  Random choices come
+ * from a splitmix64 state seeded with seed, in the reference's order of
+ * rand() calls.  Deltas are Delta_B (the exact swap delta) where the
+ * reference's kernel omits the self-loop correction.  Stops after
+ * max_evals evaluations and leaves the context on the best mapping. */
+typedef struct {
+    uint64_t seed;
+    int64_t max_evals;
+    int32_t zero_limit;  /* 1000 */
+    int32_t best_window; /* 500 */
+    double gap;          /* 0.002 */
+    double gap_norm;     /* 5154247, the reference's benchmark score */
+} bm_quick_params;
+
+typedef struct {
+    int64_t evals, perturbations, moves, positive_moves, zero_moves;
+    int64_t initial_score, final_score, best_score;
+} bm_quick_stats;
+
+int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *params, int32_t *best_mapping, bm_quick_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

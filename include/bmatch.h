@@ -100,6 +100,36 @@ int bm_eval_swaps(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count
 int bm_eval_swap_row(bm_ctx *ctx, int32_t a, int32_t b_lo, int32_t b_hi, int mode,
                      int32_t *best_b, int32_t *best_delta);
 
+/* Whole rows: deltas[i * (n + 1) + b] = Delta_B(rows[i], b) for b in
+ * [1, n], b != rows[i]; the entries b = 0 and b = rows[i] are 0.  These are
+ * the pairs calculateSwapDeltasKernelQuick (quick.cu:141-250) evaluates
+ * after a perturbation or an accepted swap: every pair that touches one of
+ * the changed vertices (rows distinct, in [1, n], count <= 64). */
+int bm_eval_rows(bm_ctx *ctx, const int32_t *rows, int32_t count, int32_t *deltas);
+
+/* Perturbation by k disjoint swaps, random_swap_k_vertices:
+ *   BM_PERTURB_OMP      alignment-omp.c:495-541: 2k distinct indices drawn
+ *                       uniformly from [0, n-1] (index 0 can be drawn and n
+ *                       cannot, as there; 0 is the unused vertex slot)
+ *   BM_PERTURB_WEIGHTED quick.cu:816-887: k vertices drawn one at a time
+ *                       without replacement with probability proportional
+ *                       to 0.7 x (sum of positive dense G1 cells of the
+ *                       out-lines) + 0.3 x (the same over in-lines),
+ *                       float arithmetic as quick.cu:766-814, then k more
+ *                       uniformly from [1, n]
+ * then mapping[sel[i]] <-> mapping[sel[i + k]] for i < k.  The draws come
+ * from *rng, a splitmix64 state the call advances, in place of the
+ * reference's time-seeded rand() (bm_rng_next gives the same stream to a
+ * caller).  selected (2k entries, may be NULL) receives the drawn vertices
+ * in draw order.  Requires 1 <= k and 2k <= n. */
+#define BM_PERTURB_OMP 0
+#define BM_PERTURB_WEIGHTED 1
+int bm_perturb(bm_ctx *ctx, int32_t k, int mode, uint64_t *rng, int32_t *selected);
+
+/* splitmix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number
+ * generators", OOPSLA 2014): advances *state and returns the next value. */
+uint64_t bm_rng_next(uint64_t *state);
+
 /* All-pairs sweep over 1 <= a < b <= n with the current mapping: the
  * launch + host arg-max of graph-re-alignment-gpu.cu:837-868 (lexicographic
  * pair order, strict >, so the lowest (a,b) wins ties; *best_delta = 0 and

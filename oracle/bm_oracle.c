@@ -641,3 +641,298 @@ int64_t orc_omp_rowbest_rows(const orc_graphs *g, const int32_t *map, int row_lo
     if (checksum) *checksum = cs;
     return evals;
 }
+
+/* ------------------------------------------------------------------------
+ * Perturbation, restart and quick search (alignment-omp.c:495-541, 543-678;
+ * quick.cu:766-887, 1137-1324).  The reference draws from a time-seeded
+ * rand(); these take a splitmix64 state so that a run is repeatable.
+ */
+uint64_t orc_rng_next(uint64_t *state) {
+    uint64_t z = (*state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* quick.cu:766-793 */
+void orc_vertex_scores(const orc_graphs *g, float *scores) {
+    int n = g->n;
+    scores[0] = 0.f;
+    for (int i = 1; i <= n; i++) {
+        float out_weight = 0.0f, in_weight = 0.0f;
+        for (int j = 1; j <= n; j++)
+            if (W1(g, i, j) > 0) out_weight += W1(g, i, j);
+        for (int j = 1; j <= n; j++)
+            if (W1(g, j, i) > 0) in_weight += W1(g, j, i);
+        float a = 0.7f * out_weight;
+        float b = 0.3f * in_weight;
+        scores[i] = a + b;
+    }
+}
+
+/* quick.cu:796-814 */
+static int orc_select_by_score(const float *scores, const uint8_t *used, int n, float total, uint64_t *rng) {
+    float u = (float)(orc_rng_next(rng) >> 40) * (1.0f / 16777216.0f);
+    float r = u * total;
+    float cumsum = 0.0f;
+    for (int i = 1; i <= n; i++) {
+        if (!used[i]) {
+            cumsum += scores[i];
+            if (cumsum >= r) return i;
+        }
+    }
+    for (int i = 1; i <= n; i++)
+        if (!used[i]) return i;
+    return 1;
+}
+
+int orc_perturb(const orc_graphs *g, int32_t *map, int k, int mode, uint64_t *rng, int32_t *sel) {
+    int n = g->n;
+    if (k < 1 || 2 * k > n) return -1;
+    uint8_t *used = xcalloc((size_t)n + 1, 1);
+    int count = 0;
+    if (mode == 0) {
+        while (count < 2 * k) {
+            int idx = (int)(orc_rng_next(rng) % (uint64_t)n);
+            if (!used[idx]) {
+                sel[count++] = idx;
+                used[idx] = 1;
+            }
+        }
+    } else {
+        float *scores = xcalloc((size_t)n + 1, sizeof(float));
+        orc_vertex_scores(g, scores);
+        float total = 0;
+        for (int i = 1; i <= n; i++) total += scores[i];
+        while (count < k) {
+            int idx = orc_select_by_score(scores, used, n, total, rng);
+            if (!used[idx]) {
+                sel[count++] = idx;
+                used[idx] = 1;
+                total -= scores[idx];
+            }
+        }
+        while (count < 2 * k) {
+            int idx = (int)(orc_rng_next(rng) % (uint64_t)n) + 1;
+            if (!used[idx]) {
+                sel[count++] = idx;
+                used[idx] = 1;
+            }
+        }
+        free(scores);
+    }
+    int32_t orig[16];
+    for (int i = 0; i < 2 * k && i < 16; i++) orig[i] = map[sel[i]];
+    if (2 * k > 16) {
+        int32_t *o = xcalloc((size_t)2 * k, sizeof(int32_t));
+        for (int i = 0; i < 2 * k; i++) o[i] = map[sel[i]];
+        for (int i = 0; i < k; i++) {
+            map[sel[i]] = o[i + k];
+            map[sel[i + k]] = o[i];
+        }
+        free(o);
+    } else {
+        for (int i = 0; i < k; i++) {
+            map[sel[i]] = orig[i + k];
+            map[sel[i + k]] = orig[i];
+        }
+    }
+    free(used);
+    return 0;
+}
+
+/* alignment-omp.c:543-678 at one thread */
+int orc_row_best_restart(const orc_graphs *g, int32_t *map, uint64_t seed, int initial_swaps, int restart_swaps,
+                         int idle_passes, int64_t max_passes, int64_t *out) {
+    int n = g->n;
+    size_t bytes = ((size_t)n + 1) * sizeof(int32_t);
+    int32_t *best = xcalloc((size_t)n + 1, sizeof(int32_t));
+    int32_t *sel = xcalloc((size_t)2 * (initial_swaps > restart_swaps ? initial_swaps : restart_swaps) + 2,
+                           sizeof(int32_t));
+    uint64_t rng = seed;
+    memcpy(best, map, bytes);
+    int64_t initial = orc_score_opt(g, map), best_score = initial;
+    if (initial_swaps > 0 && orc_perturb(g, map, initial_swaps, 0, &rng, sel)) return -1;
+    int64_t cur = orc_score_opt(g, map);
+    int64_t pass = 0, improvements = 0, last = 0, passes = 0, restarts = 0, accepted = 0;
+    while (passes < max_passes) {
+        for (int m1 = 1; m1 <= n; m1++) {
+            int maxdelta = -1, maxv = -1;
+            for (int m2 = m1 + 1; m2 <= n; m2++) {
+                int d = orc_delta_b(g, map, m1, m2);
+                if (d > maxdelta) {
+                    maxdelta = d;
+                    maxv = m2;
+                }
+            }
+            if (maxdelta > 0) {
+                int32_t t = map[m1];
+                map[m1] = map[maxv];
+                map[maxv] = t;
+                cur += maxdelta;
+                improvements++;
+                accepted++;
+                int64_t verify = orc_score_opt(g, map);
+                if (verify != cur) cur = verify;
+                if (cur > best_score) {
+                    best_score = cur;
+                    memcpy(best, map, bytes);
+                }
+            }
+        }
+        pass++;
+        passes++;
+        if (pass > idle_passes && improvements - last == 0) {
+            memcpy(map, best, bytes);
+            if (orc_perturb(g, map, restart_swaps, 0, &rng, sel)) return -1;
+            cur = orc_score_opt(g, map);
+            improvements = last = pass = 0;
+            restarts++;
+        }
+        if (last < improvements) last = improvements;
+    }
+    if (out) {
+        out[0] = passes;
+        out[1] = restarts;
+        out[2] = accepted;
+        out[3] = initial;
+        out[4] = cur;
+        out[5] = best_score;
+    }
+    memcpy(map, best, bytes);
+    free(best);
+    free(sel);
+    return 0;
+}
+
+/* quick.cu:151-196 pair numbering and the scan of :1172-1191 */
+typedef struct {
+    int32_t *pos, *zero;
+    int64_t npos, nzero;
+    int32_t best_a, best_b, max_delta;
+} orc_qscan;
+
+static void orc_qscan_pairs(const orc_graphs *g, const int32_t *map, const int32_t *changed, int nc, orc_qscan *q) {
+    int n = g->n;
+    q->npos = q->nzero = 0;
+    q->max_delta = 0;
+    q->best_a = q->best_b = 0;
+    uint8_t *in_set = xcalloc((size_t)n + 1, 1);
+    for (int i = 0; i < nc; i++) in_set[changed[i]] = 1;
+    for (int pass = 0; pass < 2; pass++)
+        for (int i = 0; i < nc; i++)
+            for (int v = 1; v <= n; v++) {
+                int a = changed[i], b;
+                if (pass == 0) {
+                    /* pairs inside the changed set: i < j in sorted order */
+                    if (!in_set[v] || v <= a) continue;
+                    b = v;
+                } else {
+                    if (in_set[v]) continue;
+                    b = v;
+                }
+                int d = orc_delta_b(g, map, a, b);
+                if (d > q->max_delta) {
+                    q->max_delta = d;
+                    q->best_a = a;
+                    q->best_b = b;
+                }
+                if (d > 0) {
+                    q->pos[2 * q->npos] = a;
+                    q->pos[2 * q->npos + 1] = b;
+                    q->npos++;
+                } else if (d == 0) {
+                    q->zero[2 * q->nzero] = a;
+                    q->zero[2 * q->nzero + 1] = b;
+                    q->nzero++;
+                }
+            }
+    free(in_set);
+}
+
+static void orc_sort_ints(int32_t *a, int m) {
+    for (int i = 1; i < m; i++)
+        for (int j = i; j > 0 && a[j - 1] > a[j]; j--) {
+            int32_t t = a[j];
+            a[j] = a[j - 1];
+            a[j - 1] = t;
+        }
+}
+
+/* quick.cu:1137-1324, one process */
+int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit,
+              int best_window, double gap, double gap_norm, int64_t *out) {
+    int n = g->n;
+    if (n < 8) return -1;
+    size_t bytes = ((size_t)n + 1) * sizeof(int32_t);
+    int32_t *best = xcalloc((size_t)n + 1, sizeof(int32_t));
+    orc_qscan q;
+    q.pos = xcalloc((size_t)16 * n + 2, sizeof(int32_t));
+    q.zero = xcalloc((size_t)16 * n + 2, sizeof(int32_t));
+    uint64_t rng = seed;
+    memcpy(best, map, bytes);
+    int64_t cur = orc_score_opt(g, map), initial = cur, best_score = cur;
+    int64_t last = 0, iter = 0, perturbations = 0, moves = 0, pos_moves = 0, zero_moves = 0;
+    while (iter < max_evals) {
+        int k = 3 + (int)(orc_rng_next(&rng) % 2);
+        if (iter - last < best_window || (double)(best_score - cur) / gap_norm > gap) memcpy(map, best, bytes);
+        int32_t changed[8];
+        orc_perturb(g, map, k, 1, &rng, changed);
+        orc_sort_ints(changed, 2 * k);
+        cur = orc_score_opt(g, map);
+        perturbations++;
+        orc_qscan_pairs(g, map, changed, 2 * k, &q);
+        iter++;
+        int zerocheck = 0;
+        while ((q.max_delta > 0 || q.nzero > 0) && zerocheck < zero_limit && iter < max_evals) {
+            int m1, m2;
+            if (q.max_delta > 0) {
+                if (orc_rng_next(&rng) % 2 == 0) {
+                    int64_t pick = (int64_t)(orc_rng_next(&rng) % (uint64_t)q.npos);
+                    m1 = q.pos[2 * pick];
+                    m2 = q.pos[2 * pick + 1];
+                } else {
+                    m1 = q.best_a;
+                    m2 = q.best_b;
+                }
+                zerocheck = 0;
+                pos_moves++;
+            } else {
+                int64_t pick = (int64_t)(orc_rng_next(&rng) % (uint64_t)q.nzero);
+                m1 = q.zero[2 * pick];
+                m2 = q.zero[2 * pick + 1];
+                zerocheck++;
+                zero_moves++;
+            }
+            int32_t t = map[m1];
+            map[m1] = map[m2];
+            map[m2] = t;
+            cur = orc_score_opt(g, map);
+            moves++;
+            if (cur > best_score) {
+                best_score = cur;
+                memcpy(best, map, bytes);
+                last = iter;
+            }
+            changed[0] = m1 < m2 ? m1 : m2;
+            changed[1] = m1 < m2 ? m2 : m1;
+            orc_qscan_pairs(g, map, changed, 2, &q);
+            iter++;
+        }
+    }
+    if (out) {
+        out[0] = iter;
+        out[1] = perturbations;
+        out[2] = moves;
+        out[3] = pos_moves;
+        out[4] = zero_moves;
+        out[5] = initial;
+        out[6] = cur;
+        out[7] = best_score;
+    }
+    memcpy(map, best, bytes);
+    free(best);
+    free(q.pos);
+    free(q.zero);
+    return 0;
+}

@@ -118,6 +118,32 @@ int32_t orc_all_pairs(const orc_graphs *g, const int32_t *map, int32_t *best_a, 
 int64_t orc_omp_rowbest_rows(const orc_graphs *g, const int32_t *map, int row_lo, int row_hi, int threads,
                              int64_t *checksum);
 
+/* --- perturbation and the searches built on it --------------------------- */
+/* splitmix64 step (the generator the engine's perturbation draws from). */
+uint64_t orc_rng_next(uint64_t *state);
+
+/* quick.cu:766-793 vertex weights into scores[0..n] (scores[0] = 0). */
+void orc_vertex_scores(const orc_graphs *g, float *scores);
+
+/* random_swap_k_vertices: mode 0 = alignment-omp.c:495-541 (2k distinct
+ * indices rand() % n), mode 1 = quick.cu:816-887 (k weighted draws, then k
+ * uniform rand() % n + 1); swaps map[sel[i]] <-> map[sel[i+k]].  rand() is
+ * replaced by orc_rng_next(rng).  sel gets the 2k draws.  Returns 0, or -1
+ * when 2k > n or k < 1. */
+int orc_perturb(const orc_graphs *g, int32_t *map, int k, int mode, uint64_t *rng, int32_t *sel);
+
+/* alignment-omp.c:543-678 row-best passes with restarts, max_passes passes;
+ * map ends on the best mapping; out[0..5] = passes, restarts, accepted,
+ * initial score, final (current) score, best score. */
+int orc_row_best_restart(const orc_graphs *g, int32_t *map, uint64_t seed, int initial_swaps, int restart_swaps,
+                         int idle_passes, int64_t max_passes, int64_t *out);
+
+/* quick.cu:1137-1324 changed-vertex search for max_evals evaluations; map
+ * ends on the best mapping; out[0..7] = evals, perturbations, moves,
+ * positive moves, zero moves, initial, final (current), best score. */
+int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit,
+              int best_window, double gap, double gap_norm, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,12 @@ def lib():
         "orc_row_best_pass": (C.c_int64, [vp, _i32p, _i32p, C.c_int64]),
         "orc_all_pairs": (C.c_int32, [vp, _i32p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp, vp, vp]),
         "orc_omp_rowbest_rows": (C.c_int64, [vp, _i32p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
+        "orc_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
+        "orc_vertex_scores": (None, [vp, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]),
+        "orc_perturb": (C.c_int, [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_uint64), _i32p]),
+        "orc_row_best_restart": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int64, _i64p]),
+        "orc_quick": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double,
+                                _i64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -67,6 +73,13 @@ def lib():
 
 def _i32(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def rng_next(state):
+    """orc_rng_next (splitmix64): returns (value, advanced state)."""
+    st = C.c_uint64(state)
+    v = lib().orc_rng_next(C.byref(st))
+    return v, st.value
 
 
 class Oracle:
@@ -113,6 +126,37 @@ class Oracle:
 
     def delta_a(self, m, a, b):
         return lib().orc_delta_a(self._h, _i32(m), a, b)
+
+    def vertex_scores(self):
+        out = np.zeros(self.n + 1, np.float32)
+        lib().orc_vertex_scores(self._h, out)
+        return out
+
+    def perturb(self, m, k, mode, rng_state):
+        """Returns (perturbed copy of m, the 2k draws, the advanced generator state)."""
+        mm = _i32(m).copy()
+        st = C.c_uint64(rng_state)
+        sel = np.zeros(2 * k, np.int32)
+        if lib().orc_perturb(self._h, mm, k, mode, C.byref(st), sel):
+            raise ValueError("orc_perturb: bad k")
+        return mm, sel, st.value
+
+    def row_best_restart(self, m, seed=1, initial_swaps=100, restart_swaps=50, idle_passes=2, max_passes=4):
+        mm = _i32(m).copy()
+        out = np.zeros(6, np.int64)
+        if lib().orc_row_best_restart(self._h, mm, seed, initial_swaps, restart_swaps, idle_passes, max_passes, out):
+            raise ValueError("orc_row_best_restart failed")
+        keys = ("passes", "restarts", "accepted", "initial_score", "final_score", "best_score")
+        return mm, dict(zip(keys, (int(x) for x in out)))
+
+    def quick(self, m, seed=1, max_evals=1000, zero_limit=1000, best_window=500, gap=0.002, gap_norm=5154247.0):
+        mm = _i32(m).copy()
+        out = np.zeros(8, np.int64)
+        if lib().orc_quick(self._h, mm, seed, max_evals, zero_limit, best_window, gap, gap_norm, out):
+            raise ValueError("orc_quick failed")
+        keys = ("evals", "perturbations", "moves", "positive_moves", "zero_moves", "initial_score", "final_score",
+                "best_score")
+        return mm, dict(zip(keys, (int(x) for x in out)))
 
     def delta_exact(self, m, a, b):
         return lib().orc_delta_swap_exact(self._h, _i32(m), a, b)
