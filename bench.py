@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--jobs", type=int, default=0, help="jobs of the serve workload")
     ap.add_argument("--workload", default="allpairs",
                     choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "batched-c1",
-                             "batched-permset", "batched-many", "serve"],
+                             "batched-permset", "batched-many", "quick", "serve"],
                     help="allpairs (default, the headline); the others measure the other configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -184,7 +184,7 @@ class _QuickParams(C.Structure):
 
 class _QuickStats(C.Structure):
     _fields_ = [(f, C.c_int64) for f in ("evals", "perturbations", "moves", "positive_moves", "zero_moves",
-                                         "initial_score", "final_score", "best_score")]
+                                         "initial_score", "final_score", "best_score", "pairs")]
 
 
 class _RefineStep(C.Structure):

@@ -16,6 +16,10 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
   batched-many  8 power-law pairs of uneven n (2,000 .. 16,000, 160 lines per
             vertex) refined to convergence: one after another, then packed into
             one graph launch per chunk (bm_refine_run_many); same results
+  quick     the n=18,524 / 3.0M pair, 30% shuffled start: the changed-vertex
+            search of quick.cu:1137-1324 (perturb 3-4 pairs, evaluate every
+            pair touching a changed vertex, move, re-evaluate the two moved
+            rows) for a fixed number of evaluations
   serve     Poisson arrivals (20 jobs/s) of power-law pairs with n uniform in
             500..20,000 and 160 lines per vertex, 30% shuffled starts, each
             refined by the batched mode to convergence (budget 10^6 moves) on
@@ -342,7 +346,27 @@ def serve(args):
     return out
 
 
+def quick(args):
+    inst = bmatch.Instance.chunglu(2, 18524, 3_000_000)
+    evals = max(args.steps, 1) * 100
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        s0 = e.score()
+        e.eval_rows([1, 2])  # first-call set-up (diagonal, staging) outside the timing
+        t = time.perf_counter()
+        best, st = e.quick(seed=1, max_evals=evals)
+        el = time.perf_counter() - t
+    out = _base("changed-vertex search evaluations per second at n=18.5k", st["evals"] / el, "evals/s",
+                int(st["evals"]), el * 1000 / st["evals"],
+                {"workload": "Chung-Lu pair n=18524 E=3.0M, 30% shuffled start, quick.cu changed-vertex search, "
+                             f"{evals} evaluations", "n": inst.n})
+    out.update({"seconds": el, "initial_score": s0, "best_score": st["best_score"],
+                "score_gain_per_s": (st["best_score"] - s0) / el, "pairs_per_s": st["pairs"] / el, "stats": st})
+    return out
+
+
 def run(name, args):
     return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset, "batched": batched,
             "batched-c1": batched_c1, "batched-permset": batched_permset, "batched-many": batched_many,
-            "serve": serve}[name](args)
+            "quick": quick, "serve": serve}[name](args)

@@ -200,7 +200,7 @@ out:
 typedef struct {
     int32_t *rows;       /* nc * (n + 1) deltas from bm_eval_rows */
     int32_t *pos, *zero; /* pairs (2 ints each) */
-    int64_t npos, nzero;
+    int64_t npos, nzero, pairs;
     int32_t best_a, best_b, max_delta;
     uint8_t *in_set;
 } quick_scan;
@@ -209,6 +209,7 @@ static int quick_eval(bm_ctx *ctx, int32_t n, const int32_t *changed, int nc, qu
     int rc = bm_eval_rows(ctx, changed, nc, q->rows);
     if (rc) return rc;
     const size_t N1 = (size_t)n + 1;
+    q->pairs += (int64_t)nc * (nc - 1) / 2 + (int64_t)nc * (n - nc);
     q->npos = q->nzero = 0;
     q->max_delta = 0;
     q->best_a = q->best_b = 0;
@@ -325,6 +326,7 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
         }
     }
     st.evals = iter;
+    st.pairs = q.pairs;
     st.final_score = cur;
     rc = bm_set_mapping(ctx, best);
 out:

@@ -201,6 +201,7 @@ typedef struct {
 typedef struct {
     int64_t evals, perturbations, moves, positive_moves, zero_moves;
     int64_t initial_score, final_score, best_score;
+    int64_t pairs; /* pairs evaluated over all evaluations */
 } bm_quick_stats;
 
 int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *params, int32_t *best_mapping, bm_quick_stats *stats);

@@ -808,7 +808,7 @@ int orc_row_best_restart(const orc_graphs *g, int32_t *map, uint64_t seed, int i
 /* quick.cu:151-196 pair numbering and the scan of :1172-1191 */
 typedef struct {
     int32_t *pos, *zero;
-    int64_t npos, nzero;
+    int64_t npos, nzero, pairs;
     int32_t best_a, best_b, max_delta;
 } orc_qscan;
 
@@ -817,6 +817,7 @@ static void orc_qscan_pairs(const orc_graphs *g, const int32_t *map, const int32
     q->npos = q->nzero = 0;
     q->max_delta = 0;
     q->best_a = q->best_b = 0;
+    q->pairs += (int64_t)nc * (nc - 1) / 2 + (int64_t)nc * (n - nc); /* quick.cu:154 */
     uint8_t *in_set = xcalloc((size_t)n + 1, 1);
     for (int i = 0; i < nc; i++) in_set[changed[i]] = 1;
     for (int pass = 0; pass < 2; pass++)
@@ -867,6 +868,7 @@ int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_eval
     size_t bytes = ((size_t)n + 1) * sizeof(int32_t);
     int32_t *best = xcalloc((size_t)n + 1, sizeof(int32_t));
     orc_qscan q;
+    q.pairs = 0;
     q.pos = xcalloc((size_t)16 * n + 2, sizeof(int32_t));
     q.zero = xcalloc((size_t)16 * n + 2, sizeof(int32_t));
     uint64_t rng = seed;
@@ -929,6 +931,7 @@ int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_eval
         out[5] = initial;
         out[6] = cur;
         out[7] = best_score;
+        out[8] = q.pairs;
     }
     memcpy(map, best, bytes);
     free(best);

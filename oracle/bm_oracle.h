@@ -139,8 +139,9 @@ int orc_row_best_restart(const orc_graphs *g, int32_t *map, uint64_t seed, int i
                          int idle_passes, int64_t max_passes, int64_t *out);
 
 /* quick.cu:1137-1324 changed-vertex search for max_evals evaluations; map
- * ends on the best mapping; out[0..7] = evals, perturbations, moves,
- * positive moves, zero moves, initial, final (current), best score. */
+ * ends on the best mapping; out[0..8] = evals, perturbations, moves,
+ * positive moves, zero moves, initial, final (current), best score, pairs
+ * evaluated. */
 int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit,
               int best_window, double gap, double gap_norm, int64_t *out);
 

@@ -151,11 +151,11 @@ class Oracle:
 
     def quick(self, m, seed=1, max_evals=1000, zero_limit=1000, best_window=500, gap=0.002, gap_norm=5154247.0):
         mm = _i32(m).copy()
-        out = np.zeros(8, np.int64)
+        out = np.zeros(9, np.int64)
         if lib().orc_quick(self._h, mm, seed, max_evals, zero_limit, best_window, gap, gap_norm, out):
             raise ValueError("orc_quick failed")
         keys = ("evals", "perturbations", "moves", "positive_moves", "zero_moves", "initial_score", "final_score",
-                "best_score")
+                "best_score", "pairs")
         return mm, dict(zip(keys, (int(x) for x in out)))
 
     def delta_exact(self, m, a, b):
