@@ -74,9 +74,8 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
     return BM_OK;
 }
 
-/* alignment-omp.c:598-658 at one thread (the stagnation restart of
- * :660-678 is time-seeded and not reproduced: a pass without an accept ends
- * the run) */
+/* alignment-omp.c:598-658 at one thread; a pass without an accept ends the
+ * run (bm_drive_row_best_restart below adds the restarts of :660-678) */
 int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook hook, void *user, int32_t *acc,
                       int64_t acc_cap, bm_drive_stats *stats) {
     bm_drive_stats st;
