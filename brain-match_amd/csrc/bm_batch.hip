@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "bm_internal.h"
+#include "bm_wave.h"
 
 using namespace bm;
 
@@ -32,10 +33,7 @@ __global__ __launch_bounds__(256) void k_argmax_moves(const int32_t *__restrict_
             best = k > best ? k : best;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long t = __shfl_xor(best, o);
-        best = t > best ? t : best;
-    }
+    best = bm_wave::wave_max_u64(best);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
     __syncthreads();
     if (threadIdx.x == 0) {

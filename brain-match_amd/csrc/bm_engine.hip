@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "bm_internal.h"
+#include "bm_wave.h"
 
 namespace bm {
 
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(256) void k_diag(int32_t *__restrict__ diag, const 
     if (v >= N1) return;
     int s = 0;
     for (int e = cptr[v] + lane; e < cptr[v + 1]; e += 64) s += min(cw[e], (int)Q[(size_t)cr[e] * ld + v]);
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    s = bm_wave::wave_sum(s);
     if (lane == 0) diag[v] = s;
 }
 
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ 
     long long s = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
         s += min(ew[i], (int)P[(size_t)es[i] * ld + ed[i]]);
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    s = bm_wave::wave_sum64(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -294,7 +295,7 @@ __device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restric
                                        const int16_t *__restrict__ Q, int ld) {
     int s = 0;
     for (int e = cptr[v] + lane; e < cptr[v + 1]; e += 64) s += min(cw[e], (int)Q[(size_t)cr[e] * ld + v]);
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    s = bm_wave::wave_sum(s);
     return s;
 }
 

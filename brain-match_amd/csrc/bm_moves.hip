@@ -21,6 +21,7 @@
 #include <tuple>
 
 #include "bm_internal.h"
+#include "bm_wave.h"
 
 using namespace bm;
 
@@ -111,9 +112,7 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
         }
     }
 #pragma unroll
-    for (int p = 1; p < 6; p++)
-#pragma unroll
-        for (int o = kLanes / 2; o > 0; o >>= 1) s[p] += __shfl_xor(s[p], o, kLanes);
+    for (int p = 1; p < 6; p++) s[p] = bm_wave::group_sum<kLanes>(s[p]);
     if (t < count && gl == 0) {
         int bd = 0, bp = 0;
         for (int p = 1; p < 6; p++) {
@@ -194,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
         for (int j = 0; j < 8; j++) {
             if (j >= k || j == i) continue;
             int x = part[j];
-            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+            x = bm_wave::wave_sum(x);
             if ((tid & 63) == 0 && x) atomicAdd(&C[i][j], x);
         }
     }
@@ -255,10 +254,7 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
     unsigned long long key = 0;
     if (bestr >= 0)
         key = ((unsigned long long)((unsigned)bestd ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (unsigned)bestr);
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long t2 = __shfl_xor(key, o);
-        key = t2 > key ? t2 : key;
-    }
+    key = bm_wave::wave_max_u64(key);
     if (kWaves > 1) {
         if ((tid & 63) == 0) red[tid >> 6] = key;
         __syncthreads();
@@ -299,10 +295,7 @@ __global__ __launch_bounds__(256) void k_hillclimb(const int32_t *__restrict__ A
             }
         }
     }
-    sum += __shfl_xor(sum, 8, 16);
-    sum += __shfl_xor(sum, 4, 16);
-    sum += __shfl_xor(sum, 2, 16);
-    sum += __shfl_xor(sum, 1, 16);
+    sum = bm_wave::group_sum<16>(sum);
     if (pair < count && gl == 0) out[pair] = sum;
 }
 

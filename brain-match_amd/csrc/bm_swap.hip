@@ -20,6 +20,7 @@
 #include <tuple>
 
 #include "bm_device.h"
+#include "bm_wave.h"
 #include "bm_internal.h"
 
 using namespace bm;
@@ -68,10 +69,7 @@ __global__ __launch_bounds__(256) void k_eval_pairs(const int32_t *__restrict__ 
             }
         }
     }
-    sum += __shfl_xor(sum, 8, 16);
-    sum += __shfl_xor(sum, 4, 16);
-    sum += __shfl_xor(sum, 2, 16);
-    sum += __shfl_xor(sum, 1, 16);
+    sum = bm_wave::group_sum<16>(sum);
     if (pair < count && gl == 0) out[pair] = sum;
 }
 
@@ -86,20 +84,8 @@ __device__ __forceinline__ unsigned long long key_max(int b, int d) {
     return ((unsigned long long)((unsigned)d ^ 0x80000000u) << 32) | (0xFFFFFFFFu - (unsigned)b);
 }
 
-__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long k) {
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long t = __shfl_xor(k, o);
-        k = t < k ? t : k;
-    }
-    return k;
-}
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long t = __shfl_xor(k, o);
-        k = t > k ? t : k;
-    }
-    return k;
-}
+using bm_wave::wave_max_u64;
+using bm_wave::wave_min_u64;
 
 
 template <bool kLdsCol>
@@ -156,10 +142,7 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
                 else q = r < N1 ? (int)pcol_a[r] : (int)prow_a[r - N1];
                 s += min(cw[e], q);
             }
-            s += __shfl_xor(s, 8, 16);
-            s += __shfl_xor(s, 4, 16);
-            s += __shfl_xor(s, 2, 16);
-            s += __shfl_xor(s, 1, 16);
+            s = bm_wave::group_sum<16>(s);
             if (gl == 0) pull[bb] = s;
         }
     }
