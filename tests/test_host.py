@@ -111,6 +111,22 @@ def test_mapping_readers(tmp_path):
     assert a.tolist() == [5] and b.tolist() == [6]
 
 
+def test_readers_on_empty_and_header_only_files(tmp_path):
+    for text in ("", "Source,Target,Weight\n", "Source,Target,Weight"):
+        p = tmp_path / "g.csv"
+        p.write_text(text)
+        s, d, w, mx, bad = bmatch.read_edges_csv(str(p))
+        assert len(s) == len(d) == len(w) == 0 and mx == -1 and bad == 0
+        for fmt in ("optimized", "permset", "plain"):
+            a, b = bmatch.read_pairs(str(p), fmt)
+            assert len(a) == len(b) == 0
+    # a last line without its newline is read like the others
+    p = tmp_path / "g.csv"
+    p.write_text("h\n1,2,3\n4,5,6")
+    s, d, w, mx, bad = bmatch.read_edges_csv(str(p))
+    assert s.tolist() == [1, 4] and d.tolist() == [2, 5] and w.tolist() == [3, 6] and bad == 0
+
+
 def test_mapping_writer_round_trip(tmp_path):
     p = tmp_path / "out.csv"
     mp = np.array([0, 3, 0, 1, 2], np.int32)
