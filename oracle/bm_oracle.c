@@ -17,6 +17,14 @@
  *   orc_delta_p : Delta_P = exact + sum over intra-set edges (brute force).
  *   orc_delta_a : before/after restatement checked against the exact
  *       out-edge-only score difference (brute force).
+ *   orc_rng_next : the published splitmix64 test vector (seed 1234567).
+ *   orc_vertex_scores : hand-computed cases (last line wins, negative and
+ *       vertex-0 lines, a weight wrapping negative as short, self-loops).
+ *   orc_perturb : uniform draws = first 2k distinct generator values mod n;
+ *       weighted draw frequencies against the weights (20,000 draws).
+ *   orc_row_best_restart : equals plain row-best passes when no restart
+ *       triggers; best score = rescoring of the returned mapping.
+ *   orc_quick : counter identities and rescoring of the returned mapping.
  */
 #include "bm_oracle.h"
 
