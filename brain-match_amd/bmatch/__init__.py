@@ -69,6 +69,7 @@ def lib():
         "bm_perturb": (C.c_int, [C.c_void_p, C.c_int32, C.c_int, C.POINTER(C.c_uint64), C.c_void_p]),
         "bm_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
         "bm_apply_swap": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+        "bm_apply_swap_scored": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64)]),
         "bm_apply_rot3": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
         "bm_apply_perm": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p]),
         "bm_sweep_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -477,6 +478,12 @@ class Engine:
 
     def apply_swap(self, a, b):
         _check(self._L.bm_apply_swap(self._h, a, b), "bm_apply_swap")
+
+    def apply_swap_scored(self, a, b):
+        """bm_apply_swap_scored: applies the swap, returns the exact score change."""
+        ch = C.c_int64()
+        _check(self._L.bm_apply_swap_scored(self._h, a, b, C.byref(ch)), "bm_apply_swap_scored")
+        return ch.value
 
     def apply_rot3(self, m1, m2, m3, p):
         _check(self._L.bm_apply_rot3(self._h, m1, m2, m3, p), "bm_apply_rot3")

@@ -237,6 +237,36 @@ __global__ __launch_bounds__(256) void k_diag(int32_t *__restrict__ diag, const 
     if (lane == 0) diag[v] = s;
 }
 
+// Exact S_opt change of swapping the images of a and b, before the swap: the
+// G1 lines with an end at a or b, old term min(w, P[x][y]) from the current
+// P, new term min(w, W2[p'(x)][p'(y)]) at the swapped images.  Each line
+// once: the out entries of a and b, then their in entries whose source is
+// neither a nor b.  One workgroup; the total goes to *out.
+__global__ __launch_bounds__(256) void k_swap_change(long long *__restrict__ out, int a, int b,
+                                                     const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                     const int32_t *__restrict__ map, const int16_t *__restrict__ Q,
+                                                     const int16_t *__restrict__ W2, int N1, int ld) {
+    __shared__ long long part[4];
+    const int pa = map[a], pb = map[b];
+    long long s = 0;
+    for (int side = 0; side < 2; side++) {
+        const int v = side ? b : a;
+        for (int e = cptr[v] + threadIdx.x; e < cptr[v + 1]; e += 256) {
+            const int2 rw = crw[e];
+            const bool out_entry = rw.x >= N1;  // v -> rw.x - N1, else rw.x -> v
+            const int x = out_entry ? v : rw.x, y = out_entry ? rw.x - N1 : v;
+            if (!out_entry && (x == a || x == b)) continue;  // counted as an out entry of x
+            const int px = x == a ? pb : x == b ? pa : map[x];
+            const int py = y == a ? pb : y == b ? pa : map[y];
+            s += min(rw.y, (int)W2[(size_t)px * ld + py]) - min(rw.y, (int)Q[(size_t)x * ld + y]);
+        }
+    }
+    s = bm_wave::wave_sum64(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = part[0] + part[1] + part[2] + part[3];
+}
+
 // S_opt = sum over G1 lines (x->y, w) of min(w, P[x][y]).
 __global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ out, const int32_t *__restrict__ es,
                                                const int32_t *__restrict__ ed, const int32_t *__restrict__ ew,
@@ -955,6 +985,25 @@ int bm_score(bm_ctx *c, int64_t *score) {
 }
 
 static int check_vertex(const bm_ctx *c, int32_t v) { return v >= 0 && v <= c->n; }
+
+int bm_apply_swap_scored(bm_ctx *c, int32_t a, int32_t b, int64_t *change) {
+    if (!c || !change) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (!check_vertex(c, a) || !check_vertex(c, b) || a == b)
+        return set_error(BM_EINVAL, "apply_swap(%d,%d): bad vertices", a, b);
+    BM_HIP(hipSetDevice(c->device));
+    long long *slot = (long long *)(c->keys + c->N1 + 4);
+    BM_LAUNCH(c, "swap_change", hipLaunchKernelGGL(k_swap_change, dim3(1), dim3(256), 0, c->stream, slot, a, b,
+                                                   c->cptr, (const int2 *)c->crw, c->map, c->Q, c->W2, c->N1,
+                                                   c->ld));
+    int32_t set[2] = {a, b}, src[2] = {1, 0};
+    int rc = permute_rows_cols(c, 2, set, src);
+    if (rc) return rc;
+    BM_HIP(hipMemcpyAsync(c->hres + 2, slot, 8, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    *change = (int64_t)(long long)c->hres[2];
+    return BM_OK;
+}
 
 int bm_apply_swap(bm_ctx *c, int32_t a, int32_t b) {
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");

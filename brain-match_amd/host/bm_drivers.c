@@ -17,20 +17,20 @@ static void record(int32_t *acc, int64_t cap, int64_t i, int32_t a, int32_t b, i
     }
 }
 
-/* apply, advance the score, and verify it like
- * graph-alignment-optimized.c:570-576 (the rescore is trusted) */
+/* apply, advance the score by the delta, and verify it like
+ * graph-alignment-optimized.c:570-576: the verified score (the exact score
+ * change of the swap added to the previous score, which equals the full
+ * rescore there) is trusted */
 static int apply_and_verify(bm_ctx *ctx, int32_t a, int32_t b, int32_t d, int verify, bm_drive_stats *st) {
-    int rc = bm_apply_swap(ctx, a, b);
+    const int64_t before = st->final_score;
+    int64_t change = 0;
+    int rc = verify ? bm_apply_swap_scored(ctx, a, b, &change) : bm_apply_swap(ctx, a, b);
     if (rc) return rc;
     st->final_score += d;
     st->accepted++;
-    if (verify) {
-        int64_t s;
-        if ((rc = bm_score(ctx, &s))) return rc;
-        if (s != st->final_score) {
-            st->mismatches++;
-            st->final_score = s;
-        }
+    if (verify && before + change != st->final_score) {
+        st->mismatches++;
+        st->final_score = before + change;
     }
     return BM_OK;
 }
@@ -154,17 +154,15 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *
             if ((rc = bm_eval_swap_row(ctx, a, a + 1, n, BM_ROW_ARGMAX, &b, &d))) goto out;
             st.evals_rows++;
             if (d <= 0) continue;
-            if ((rc = bm_apply_swap(ctx, a, b))) goto out;
+            int64_t change = 0;
+            if ((rc = p->verify ? bm_apply_swap_scored(ctx, a, b, &change) : bm_apply_swap(ctx, a, b))) goto out;
+            const int64_t verified = cur + change;
             cur += d;
             improvements++;
             st.accepted++;
-            if (p->verify) { /* :645-652, the rescore is trusted */
-                int64_t s;
-                if ((rc = bm_score(ctx, &s))) goto out;
-                if (s != cur) {
-                    st.mismatches++;
-                    cur = s;
-                }
+            if (p->verify && verified != cur) { /* :652-656, the verified score is trusted */
+                st.mismatches++;
+                cur = verified;
             }
             if (cur > st.best_score) { /* :627-636 */
                 st.best_score = cur;
@@ -311,7 +309,9 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
                 zerocheck++;
                 st.zero_moves++;
             }
-            if ((rc = bm_apply_swap(ctx, m1, m2)) || (rc = bm_score(ctx, &cur))) goto out;
+            int64_t change;
+            if ((rc = bm_apply_swap_scored(ctx, m1, m2, &change))) goto out;
+            cur += change; /* :1248, the rescore after the swap */
             st.moves++;
             if (cur > st.best_score) {
                 st.best_score = cur;

@@ -149,7 +149,7 @@ int bm_drive_global_best(bm_ctx *ctx, int64_t max_steps, int verify, int32_t *ac
  * the pass accepted nothing (:660), the best mapping so far is restored and
  * perturbed by restart_swaps swaps (:669-670).  Perturbations are
  * bm_perturb(BM_PERTURB_OMP) from a splitmix64 state seeded with seed.  The
- * score is re-verified after each accept when verify != 0 (:645-652).
+ * score is re-verified after each accept when verify != 0 (:652-656).
  * Stops after max_passes passes (> 0) and leaves the context on the best
  * mapping seen (the one the reference writes out); best_mapping (n+1
  * entries, may be NULL) receives it. */

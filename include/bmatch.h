@@ -173,6 +173,12 @@ int bm_eval_swaps_hillclimb(bm_ctx *ctx, const int32_t *a, const int32_t *b, int
  * graph-alignment-permset-parallel.c:508-518 (new p(set[i]) =
  * old p(set[perm[i]])). */
 int bm_apply_swap(bm_ctx *ctx, int32_t a, int32_t b);
+/* bm_apply_swap that also returns the exact score change S(after) - S(before),
+ * summed over the G1 lines with an end at a or b (each line once): what the
+ * full rescore after an accepted swap (graph-alignment-optimized.c:571,
+ * alignment-omp.c:652, quick.cu:1248) establishes, at the cost of those
+ * lines only. */
+int bm_apply_swap_scored(bm_ctx *ctx, int32_t a, int32_t b, int64_t *change);
 int bm_apply_rot3(bm_ctx *ctx, int32_t m1, int32_t m2, int32_t m3, int32_t p);
 int bm_apply_perm(bm_ctx *ctx, int32_t k, const int32_t *set, const int32_t *perm);
 

@@ -304,3 +304,28 @@ def test_row_kernel_after_moves(case):
             assert e.eval_swap_row(row, 1, n) == ((pos[0], ds[pos[0]]) if pos else (0, 0))
             bb = max(ds, key=lambda x: (ds[x], -x))
             assert e.eval_swap_row(row, 1, n, bmatch.BM_ROW_ARGMAX) == (bb, ds[bb])
+
+
+@pytest.mark.parametrize("case", ["plain", "loops_dups", "vertex0", "nonbijective", "negative"])
+def test_apply_swap_scored_is_the_exact_change(case):
+    n = 70
+    neg = case == "negative"
+    g1 = random_multigraph(81, n, 700, 0.0 if case == "plain" else 0.15, 0.0 if case == "plain" else 0.05,
+                           zero_vertex=case == "vertex0", negative=neg)
+    g2 = random_multigraph(82, n, 700, 0.1, 0.05)
+    mp = random_mapping(83, n, 0.1, bijective=case != "nonbijective")
+    o = ob.Oracle(n, g1, g2)
+    rng = np.random.default_rng(84)
+    cur = mp.copy()
+    with bmatch.Engine(0) as e:
+        e.upload(n, g1, g2)
+        e.set_mapping(mp)
+        s = e.score()
+        for _ in range(40):
+            a, b = (int(x) for x in rng.choice(np.arange(0 if case == "vertex0" else 1, n + 1), 2, replace=False))
+            want = o.delta_exact(cur, a, b)
+            got = e.apply_swap_scored(a, b)
+            cur[a], cur[b] = cur[b], cur[a]
+            assert got == want, (a, b)
+            s += got
+            assert s == e.score() == o.score_opt(cur)
