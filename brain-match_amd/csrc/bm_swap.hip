@@ -203,12 +203,14 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
 // through p^-1: Q[s][a] != 0 for s = p^-1(g), g an in-neighbour of p(a), and
 // Q[n+1+d][a] != 0 for d = p^-1(g), g an out-neighbour; the comb entries
 // reading row s are the G1 out-lines s->b, those reading row n+1+d the G1
-// in-lines b->d.  k_row_expand gives each such list head one wave (comb(a)
-// entries, then in- and out-neighbours of p(a)) and adds the contributions
-// for b in [lo, hi] into a device row buffer with atomics; its block 0 also
-// adds Corr(a,b) for adjacent b and seeds the result key.  k_row_final forms
-// Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b), reduces it to the first
-// positive or arg-max key, and zeroes the buffer for the next call.
+// in-lines b->d.  k_row_expand gives each such list head kWaves waves
+// (comb(a) entries, then in- and out-neighbours of p(a)) and adds the
+// contributions for b in [lo, hi] into a device row buffer with atomics; its
+// block 0 also adds Corr(a,b) for adjacent b and seeds the result key.
+// k_row_final forms Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b),
+// reduces it to the first positive or arg-max key, and zeroes the buffer for
+// the next call.
+template <int kWaves>
 __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int mode, int N1,
                                                     const int32_t *__restrict__ cptr,
                                                     const int32_t *__restrict__ cmid, const int2 *__restrict__ crw,
@@ -219,7 +221,8 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
                                                     const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
                                                     int ld, int *__restrict__ acc,
                                                     unsigned long long *__restrict__ result) {
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    constexpr int kLanes = 64 * kWaves, kHeads = 4 / kWaves;  // lanes per head, heads per block
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes), lane = threadIdx.x % kLanes;
     const int ca0 = cptr[a], dega = cptr[a + 1] - ca0;
     const int pa = map[a];
     const int i0 = g2iptr[pa], nin = g2iptr[pa + 1] - i0;
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
         }
         return;
     }
-    const int h = (blockIdx.x - 1) * 4 + wave;
+    const int h = (blockIdx.x - 1) * kHeads + slot;
     if (h < dega) {  // X(a, b): comb entry of a, its G2 list
         const int2 rw = crw[ca0 + h];
         const bool out = rw.x >= N1;
@@ -247,10 +250,10 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
         const int l0 = out ? g2iptr[g] : g2optr[g], l1 = out ? g2iptr[g + 1] : g2optr[g + 1];
         const int w = rw.y, wneg = min(w, 0);
         int j = l0 + lane;
-        for (; j + 192 < l1; j += 256) {  // four list words in flight per lane
+        for (; j + 3 * kLanes < l1; j += 4 * kLanes) {  // four list words in flight per lane
             uint32_t yv[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) yv[u] = L[j + 64 * u];
+            for (int u = 0; u < 4; u++) yv[u] = L[j + kLanes * u];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int y = yv[u] >> 16;
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
                     atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - wneg);
             }
         }
-        for (; j < l1; j += 64) {
+        for (; j < l1; j += kLanes) {
             const uint32_t yv = L[j];
             const int y = yv >> 16;
             if (y != 0xFFFF && y >= lo && y <= hi) atomicAdd(&acc[y], min(w, (int)(int16_t)(yv & 0xffff)) - wneg);
@@ -273,17 +276,17 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
         // out-neighbour: rows n+1+v of P^T, read by the in-lines b->v (out entries of b)
         const int l0 = in ? cptr[v] : cmid[v], l1 = in ? cmid[v] : cptr[v + 1];
         int e = l0 + lane;
-        for (; e + 192 < l1; e += 256) {
+        for (; e + 3 * kLanes < l1; e += 4 * kLanes) {
             int2 rw[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) rw[u] = crw[e + 64 * u];
+            for (int u = 0; u < 4; u++) rw[u] = crw[e + kLanes * u];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int b = in ? rw[u].x - N1 : rw[u].x;
                 if (b >= lo && b <= hi) atomicAdd(&acc[b], min(rw[u].y, val) - min(rw[u].y, 0));
             }
         }
-        for (; e < l1; e += 64) {
+        for (; e < l1; e += kLanes) {
             const int2 rw = crw[e];
             const int b = in ? rw.x - N1 : rw.x;
             if (b >= lo && b <= hi) atomicAdd(&acc[b], min(rw.y, val) - min(rw.y, 0));
@@ -1093,6 +1096,27 @@ extern "C" int bm_eval_swaps(bm_ctx *c, const int32_t *a, const int32_t *b, int6
     return BM_OK;
 }
 
+// One row's list heads: BM_ROW_WAVES = 1 gives each head one wave (four heads
+// per workgroup), 4 (the default) a whole workgroup, which spreads the long
+// lists of hub neighbours over 256 lanes.
+static int launch_row_expand(bm_ctx *c, int a, int lo, int hi, int mode, unsigned long long *res) {
+    static const int waves = [] {
+        const char *e = getenv("BM_ROW_WAVES");
+        return e && atoi(e) == 1 ? 1 : 4;
+    }();
+    const int heads = c->hcdeg[a] + c->hg2deg[c->hmap[a]];
+    const int per = 4 / waves;
+#define BM_ROW_EXPAND(W)                                                                                         \
+    BM_LAUNCH(c, "row_expand",                                                                                  \
+              hipLaunchKernelGGL(k_row_expand<W>, dim3(1 + (heads + per - 1) / per), dim3(256), 0, c->stream, a, lo, \
+                                 hi, mode, c->N1, c->cptr, c->cmid, (const int2 *)c->crw, c->cdw, c->map, c->g2optr,  \
+                                 c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->rowacc, res))
+    if (waves == 1) BM_ROW_EXPAND(1);
+    else BM_ROW_EXPAND(4);
+#undef BM_ROW_EXPAND
+    return BM_OK;
+}
+
 extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi, int mode, int32_t *best_b,
                                 int32_t *best_delta) {
     if (!c || !best_b || !best_delta) return set_error(BM_EINVAL, "NULL argument");
@@ -1112,12 +1136,7 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
         if (c->g2y_dirty) {
             if ((rc = launch_refresh_g2y(c))) return rc;
         }
-        const int pa = c->hmap[a];
-        const int heads = c->hcdeg[a] + c->hg2deg[pa];
-        BM_LAUNCH(c, "row_expand", hipLaunchKernelGGL(k_row_expand, dim3(1 + (heads + 3) / 4), dim3(256), 0, c->stream,
-                                                      a, b_lo, b_hi, mode, c->N1, c->cptr, c->cmid,
-                                                      (const int2 *)c->crw, c->cdw, c->map, c->g2optr, c->g2oy,
-                                                      c->g2iptr, c->g2iy, c->Q, c->ld, c->rowacc, res));
+        if ((rc = launch_row_expand(c, a, b_lo, b_hi, mode, res))) return rc;
         BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((b_hi - b_lo + 256) / 256), dim3(256), 0,
                                                      c->stream, a, b_lo, b_hi, mode, c->rowacc, c->cneg, c->diag, res,
                                                      nullptr));
@@ -1177,11 +1196,7 @@ extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32
         const int a = rows[i];
         int32_t *row = out + (size_t)i * N1;
         if (c->sparse_ok) {
-            const int heads = c->hcdeg[a] + c->hg2deg[c->hmap[a]];
-            BM_LAUNCH(c, "row_expand",
-                      hipLaunchKernelGGL(k_row_expand, dim3(1 + (heads + 3) / 4), dim3(256), 0, c->stream, a, 0, n,
-                                         BM_ROW_ARGMAX, c->N1, c->cptr, c->cmid, (const int2 *)c->crw, c->cdw, c->map,
-                                         c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->rowacc, res));
+            if ((rc = launch_row_expand(c, a, 0, n, BM_ROW_ARGMAX, res))) return rc;
             BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_final, dim3((n + 256) / 256), dim3(256), 0, c->stream,
                                                          a, 0, n, BM_ROW_ARGMAX, c->rowacc, c->cneg, c->diag, res,
                                                          row));
