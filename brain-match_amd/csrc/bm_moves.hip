@@ -1,7 +1,7 @@
 // Kernels for the 3-vertex, k-set and hill-climb move kinds (gfx950).
 //
 //   k_rot3     : Delta_G3 of triple.cu:85-214 for a triple list, kLanes
-//                (8, 16 or 32) lanes per triple.  Each neighbour entry of m_k
+//                (8, 16, 32 or 64) lanes per triple.  Each neighbour entry of m_k
 //                loads the three values Q[r][m_0..2] once and feeds all five
 //                permutations.
 //   k_permset  : Delta_P of graph-alignment-permset-parallel.c:304-339 for
@@ -311,6 +311,7 @@ int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *
     if (lanes == 8) BM_ROT3(8);
     else if (lanes == 32) BM_ROT3(32);
     else if (lanes == 16) BM_ROT3(16);
+    else if (lanes == 64) BM_ROT3(64);
     else return set_error(BM_EINVAL, "rot3: %d lanes per triple", lanes);
 #undef BM_ROT3
     return BM_OK;
@@ -336,7 +337,7 @@ int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *
 
 // ------------------------------------------------------------ autotuner
 //
-// Lanes per triple (8 / 16 / 32, i.e. 8, 4 or 2 triples per wavefront) and
+// Lanes per triple (8 / 16 / 32 / 64, i.e. 8, 4, 2 or 1 triples per wavefront) and
 // threads per set (64 / 256) are timed with HIP events on the caller's own
 // candidate list (one warm-up and three timed launches per variant) and the
 // fastest is kept per instance class (log2 n, log2 mean list length, k) for
@@ -407,9 +408,9 @@ static int tune_variant(bm_ctx *c, const char *env, std::tuple<int, int, int, in
 }
 
 int bm::tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP) {
-    static const int ch[3] = {8, 16, 32};
+    static const int ch[4] = {8, 16, 32, 64};
     const auto key = std::make_tuple(1, log2c(c->n), log2c(std::max<long long>(1, c->ndd / c->N1)), 3);
-    int rc = tune_variant(c, "BM_ROT3_LANES", key, ch, 3, [&](int v) { c->rot3_lanes = v; },
+    int rc = tune_variant(c, "BM_ROT3_LANES", key, ch, 4, [&](int v) { c->rot3_lanes = v; },
                           [&] { return launch_rot3(c, dT, m, BM_ROT3_ALL, dD, dP); });
     if (!rc) c->rot3_tuned = true;
     return rc;

@@ -269,7 +269,7 @@ int bm_refine_history(bm_ctx *ctx, int64_t first, int64_t count, bm_refine_step 
  * their new images[8].  bm_refine_moves copies min(count, capacity) records. */
 int bm_refine_log(bm_ctx *ctx, int32_t capacity);
 int bm_refine_moves(bm_ctx *ctx, int64_t *count, int32_t *records, int64_t capacity);
-/* Autotuned variants in use: lanes per triple (8/16/32), threads per set (64/256). */
+/* Autotuned variants in use: lanes per triple (8/16/32/64), threads per set (64/256). */
 int bm_refine_config(bm_ctx *ctx, int32_t *rot3_lanes, int32_t *permset_threads);
 
 /*

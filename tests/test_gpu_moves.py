@@ -135,3 +135,18 @@ def test_resident_moves_match_host_lists(k):
         d2, _ = e.moves_results(cnt)
         if k == 2:
             np.testing.assert_array_equal(d2, [o.delta_b(mp2, int(p), int(q)) for p, q in moves])
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+def test_rot3_every_lane_width(lanes, monkeypatch):
+    # 4,096 triples or more go through the autotuner, which the variable overrides
+    monkeypatch.setenv("BM_ROT3_LANES", str(lanes))
+    n, g1, g2, mp, e, o = setup(31, unm=0.05)
+    rng = np.random.default_rng(lanes)
+    tri = np.array([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(4200)], np.int32)
+    with e:
+        d, p = e.eval_rot3(tri, bmatch.BM_ROT3_ALL)
+        assert e.refine_config()[0] == lanes
+    want = [o.best_g3(mp, *map(int, t), bmatch.BM_ROT3_ALL) for t in tri]
+    np.testing.assert_array_equal(d, [w[0] for w in want])
+    np.testing.assert_array_equal(p, [w[1] for w in want])
