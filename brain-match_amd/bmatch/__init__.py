@@ -24,6 +24,8 @@ BM_ROT3_ROTATIONS = 0
 BM_ROT3_ALL = 1
 BM_PERTURB_OMP = 0
 BM_PERTURB_WEIGHTED = 1
+BM_DELTA_LISTS = 0
+BM_DELTA_DENSE = 1
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lib = None
@@ -65,7 +67,8 @@ def lib():
         "bm_eval_permset": (C.c_int, [C.c_void_p, C.c_int32, _i32p, C.c_int64, _i32p, _i32p]),
         "bm_perm_from_rank": (C.c_int, [C.c_int32, C.c_int32, _i32p]),
         "bm_eval_swaps_hillclimb": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
-        "bm_eval_rows": (C.c_int, [C.c_void_p, _i32p, C.c_int32, _i32p]),
+        "bm_eval_rows": (C.c_int, [C.c_void_p, _i32p, C.c_int32, C.c_int32, _i32p]),
+        "bm_eval_swaps_dense": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
         "bm_perturb": (C.c_int, [C.c_void_p, C.c_int32, C.c_int, C.POINTER(C.c_uint64), C.c_void_p]),
         "bm_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
         "bm_apply_swap": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -180,7 +183,8 @@ class _RestartStats(C.Structure):
 
 class _QuickParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("max_evals", C.c_int64), ("zero_limit", C.c_int32),
-                ("best_window", C.c_int32), ("gap", C.c_double), ("gap_norm", C.c_double)]
+                ("best_window", C.c_int32), ("gap", C.c_double), ("gap_norm", C.c_double),
+                ("delta_kind", C.c_int32)]
 
 
 class _QuickStats(C.Structure):
@@ -445,11 +449,19 @@ class Engine:
         _check(self._L.bm_eval_permset(self._h, k, s.reshape(-1), cnt, d, r), "bm_eval_permset")
         return d, r
 
-    def eval_rows(self, rows):
-        """bm_eval_rows: Delta_B(rows[i], b) for b = 0..n as a (len(rows), n+1) array."""
+    def eval_rows(self, rows, kind=BM_DELTA_LISTS):
+        """bm_eval_rows: Delta(rows[i], b) for b = 0..n as a (len(rows), n+1) array; kind BM_DELTA_LISTS
+        gives Delta_B, BM_DELTA_DENSE the dense-matrix delta of quick.cu's kernels."""
         r = _i32(rows)
         out = np.zeros((len(r), self.n + 1), np.int32)
-        _check(self._L.bm_eval_rows(self._h, r, len(r), out.reshape(-1)), "bm_eval_rows")
+        _check(self._L.bm_eval_rows(self._h, r, len(r), kind, out.reshape(-1)), "bm_eval_rows")
+        return out
+
+    def eval_swaps_dense(self, a, b):
+        """bm_eval_swaps_dense: the dense-matrix delta of quick.cu:48-250 for pairs (a[i], b[i])."""
+        a, b = _i32(a), _i32(b)
+        out = np.zeros(len(a), np.int32)
+        _check(self._L.bm_eval_swaps_dense(self._h, a, b, len(a), out), "bm_eval_swaps_dense")
         return out
 
     def perturb(self, k, mode, rng_state):
@@ -468,8 +480,9 @@ class Engine:
                                                  C.byref(st)), "bm_drive_row_best_restart")
         return best, {f: getattr(st, f) for f, _ in _RestartStats._fields_}
 
-    def quick(self, seed=1, max_evals=1000, zero_limit=1000, best_window=500, gap=0.002, gap_norm=5154247.0):
-        prm = _QuickParams(seed, max_evals, zero_limit, best_window, gap, gap_norm)
+    def quick(self, seed=1, max_evals=1000, zero_limit=1000, best_window=500, gap=0.002, gap_norm=5154247.0,
+              kind=BM_DELTA_DENSE):
+        prm = _QuickParams(seed, max_evals, zero_limit, best_window, gap, gap_norm, kind)
         best = np.zeros(self.n + 1, np.int32)
         st = _QuickStats()
         _check(self._L.bm_drive_quick(self._h, C.byref(prm), best.ctypes.data_as(C.c_void_p), C.byref(st)),

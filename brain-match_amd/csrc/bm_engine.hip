@@ -455,6 +455,12 @@ int launch_score(bm_ctx *c, unsigned long long *out) {
     return BM_OK;
 }
 
+int launch_list_diag(bm_ctx *c, int32_t *out, const int32_t *ptr, const int32_t *r, const int32_t *w) {
+    BM_LAUNCH(c, "list_diag", hipLaunchKernelGGL(k_diag, dim3((c->N1 + 3) / 4), dim3(256), 0, c->stream, out, ptr, r,
+                                                 w, c->Q, c->N1, c->ld));
+    return BM_OK;
+}
+
 int ensure_diag(bm_ctx *c) {
     if (c->diag_valid) return BM_OK;
     int blocks = (c->N1 + 3) / 4;
@@ -464,13 +470,14 @@ int ensure_diag(bm_ctx *c) {
     return BM_OK;
 }
 
-static void touch_set(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of, TouchSet &ts) {
+static void touch_set(const std::vector<int32_t> &deg, int k, const int32_t *set, const int32_t *src_of,
+                      TouchSet &ts) {
     memset(&ts, 0, sizeof ts);
     ts.k = k;
     for (int i = 0; i < k; i++) {
         ts.v[i] = set[i];
         ts.src[i] = src_of[i];
-        ts.off[i + 1] = ts.off[i] + c->hcdeg[set[i]];
+        ts.off[i + 1] = ts.off[i] + deg[set[i]];
     }
 }
 
@@ -479,12 +486,22 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
     TouchSet ts;
     const bool diag = c->diag_valid;  // keep X(v,v) current where it changes
     if (diag) {
-        touch_set(c, k, set, src_of, ts);
+        touch_set(c->hcdeg, k, set, src_of, ts);
         const int m = ts.off[k];
         if (m > 0)
             BM_LAUNCH(c, "diag_delta", hipLaunchKernelGGL(k_diag_delta, dim3((m + 255) / 256), dim3(256), 0,
                                                           c->stream, c->diag, c->cptr, c->cr, c->cw, c->Q, c->N1,
                                                           c->ld, ts));
+    }
+    const bool dfdiag = c->dfdiag_valid;  // the same for the dense-form lists
+    TouchSet tsd;
+    if (dfdiag) {
+        touch_set(c->hdfdeg, k, set, src_of, tsd);
+        const int m = tsd.off[k];
+        if (m > 0)
+            BM_LAUNCH(c, "diag_delta", hipLaunchKernelGGL(k_diag_delta, dim3((m + 255) / 256), dim3(256), 0,
+                                                          c->stream, c->dfdiag, c->dfptr, c->dfr, c->dfw, c->Q, c->N1,
+                                                          c->ld, tsd));
     }
     SetPerm sp;
     memset(&sp, 0, sizeof sp);
@@ -526,6 +543,9 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
     if (diag)
         BM_LAUNCH(c, "diag_set", hipLaunchKernelGGL(k_diag_set, dim3(1), dim3(64 * k), 0, c->stream, c->diag,
                                                     c->cptr, c->cr, c->cw, c->Q, c->ld, ts));
+    if (dfdiag)
+        BM_LAUNCH(c, "diag_set", hipLaunchKernelGGL(k_diag_set, dim3(1), dim3(64 * k), 0, c->stream, c->dfdiag,
+                                                    c->dfptr, c->dfr, c->dfw, c->Q, c->ld, tsd));
     return BM_OK;
 }
 
@@ -611,6 +631,16 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->ddmid, N1 * 4);
     dev_free(c, (void **)&c->ddr, (size_t)c->ndd * 4);
     dev_free(c, (void **)&c->ddw, (size_t)c->ndd * 4);
+    dev_free(c, (void **)&c->dfptr, (N1 + 1) * 4);
+    dev_free(c, (void **)&c->dfmid, N1 * 4);
+    dev_free(c, (void **)&c->dfr, (size_t)c->ndf * 4);
+    dev_free(c, (void **)&c->dfw, (size_t)c->ndf * 4);
+    dev_free(c, (void **)&c->dfrw, (size_t)c->ndf * 8);
+    dev_free(c, (void **)&c->dfneg, N1 * 4);
+    dev_free(c, (void **)&c->dfdiag, N1 * 4);
+    c->ndf = 0;
+    c->dfdiag_valid = false;
+    c->hdfdeg.clear();
     dev_free(c, (void **)&c->esrc, (size_t)c->e1 * 4);
     dev_free(c, (void **)&c->edst, (size_t)c->e1 * 4);
     dev_free(c, (void **)&c->ew, (size_t)c->e1 * 4);
@@ -634,7 +664,7 @@ static void free_graph_buffers(bm_ctx *c) {
     c->nsplit = 0;
     free_moves(c);
     free_batch(c);
-    c->graphs = c->mapped = c->diag_valid = false;
+    c->graphs = c->mapped = c->diag_valid = c->dfdiag_valid = false;
     c->gain_cols = 0;
     c->sweep_kind = 0;
     c->sparse_ok = false;
@@ -956,7 +986,7 @@ int bm_set_mapping(bm_ctx *c, const int32_t *mapping) {
     if (rc) return rc;
     BM_HIP(hipStreamSynchronize(c->stream));
     c->mapped = true;
-    c->diag_valid = false;
+    c->diag_valid = c->dfdiag_valid = false;
     return BM_OK;
 }
 
@@ -1103,6 +1133,64 @@ static int vertex_scores(bm_ctx *c) {
         score[v] = a + b;
     }
     c->hvscore.swap(score);
+    return BM_OK;
+}
+
+// Lists of the dense-matrix swap delta of quick.cu:48-250: for every i in
+// 1..n other than a and b it adds the terms of the cells W1[a][i], W1[i][a]
+// (and the same for b) that are positive as short, then the two cells between
+// a and b.  Those are the de-duplicated entries whose int16 cell is > 0,
+// minus self-loops and vertex 0; with each entry owning its cell (dense-cell
+// weight = weight) the list-form kernels compute exactly that delta.
+int bm::ensure_dense_lists(bm_ctx *c) {
+    if (c->dfptr) return BM_OK;
+    const int N1 = c->N1;
+    std::vector<int32_t> ptr(N1 + 1), r((size_t)c->ndd), w((size_t)c->ndd);
+    BM_HIP(hipMemcpyAsync(ptr.data(), c->ddptr, (size_t)(N1 + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    if (c->ndd > 0) {
+        BM_HIP(hipMemcpyAsync(r.data(), c->ddr, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipMemcpyAsync(w.data(), c->ddw, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    BM_HIP(hipStreamSynchronize(c->stream));
+    // out entries (r >= N1) stay ahead of in entries, as in the lists they come from
+    std::vector<int32_t> fptr(N1 + 1, 0), fmid(N1, 0), fr, fw, frw, deg(N1, 0);
+    fr.reserve((size_t)c->ndd);
+    fw.reserve((size_t)c->ndd);
+    frw.reserve(2 * (size_t)c->ndd);
+    for (int v = 0; v < N1; v++) {
+        fptr[v] = fmid[v] = (int32_t)fr.size();
+        if (v == 0) continue;
+        for (int e = ptr[v]; e < ptr[v + 1]; e++) {
+            const int nb = r[e] >= N1 ? r[e] - N1 : r[e];
+            const int cell = (int16_t)w[e];
+            if (cell <= 0 || nb == 0 || nb == v) continue;
+            fr.push_back(r[e]);
+            fw.push_back(cell);
+            frw.push_back(r[e]);
+            frw.push_back(cell);
+            if (r[e] >= N1) fmid[v] = (int32_t)fr.size();
+        }
+        deg[v] = (int32_t)fr.size() - fptr[v];
+    }
+    fptr[N1] = (int32_t)fr.size();
+    int rc;
+    c->ndf = (int64_t)fr.size();
+    const std::vector<int32_t> zeros(N1, 0);
+    if ((rc = upload(c, &c->dfr, fr)) || (rc = upload(c, &c->dfw, fw)) || (rc = upload(c, &c->dfrw, frw)) ||
+        (rc = upload(c, &c->dfmid, fmid)) || (rc = upload(c, &c->dfneg, zeros)) ||
+        (rc = dev_alloc(c, (void **)&c->dfdiag, (size_t)N1 * 4)) || (rc = upload(c, &c->dfptr, fptr))) {
+        dev_free(c, (void **)&c->dfr, (size_t)c->ndf * 4);
+        dev_free(c, (void **)&c->dfw, (size_t)c->ndf * 4);
+        dev_free(c, (void **)&c->dfrw, (size_t)c->ndf * 8);
+        dev_free(c, (void **)&c->dfmid, (size_t)N1 * 4);
+        dev_free(c, (void **)&c->dfneg, (size_t)N1 * 4);
+        dev_free(c, (void **)&c->dfdiag, (size_t)N1 * 4);
+        dev_free(c, (void **)&c->dfptr, (size_t)(N1 + 1) * 4);
+        c->ndf = 0;
+        return rc;
+    }
+    c->hdfdeg.swap(deg);
+    c->dfdiag_valid = false;
     return BM_OK;
 }
 

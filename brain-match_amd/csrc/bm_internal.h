@@ -68,6 +68,14 @@ struct bm_ctx {
     int32_t *cneg = nullptr;    // cneg[v] = sum over comb(v) of min(w, 0)
     int32_t *rowacc = nullptr;  // (n+1)-int accumulator of the sparse row kernel, kept zeroed
     int32_t *ddptr = nullptr, *ddmid = nullptr, *ddr = nullptr, *ddw = nullptr;
+    // dense-form lists (lazy): the de-duplicated entries whose int16 G1 cell is
+    // positive, without self-loops and vertex 0; weights are the int16 cells
+    int32_t *dfptr = nullptr, *dfmid = nullptr, *dfr = nullptr, *dfw = nullptr;
+    int32_t *dfrw = nullptr;    // (r, w) pairs of the same lists
+    int32_t *dfneg = nullptr;   // zeros: the sum of min(w, 0) over lists of positive weights
+    int32_t *dfdiag = nullptr;  // X(v,v) over the dense-form lists
+    bool dfdiag_valid = false;
+    int64_t ndf = 0;
     int32_t *esrc = nullptr, *edst = nullptr, *ew = nullptr, *edw = nullptr;
     // G2 de-duplicated (last line wins) out- and in-lists, packed (vertex << 16 | (uint16)weight)
     int32_t *g2optr = nullptr, *g2iptr = nullptr;
@@ -122,6 +130,7 @@ struct bm_ctx {
     std::vector<int32_t> hmap;
     std::vector<int32_t> hpinv;
     std::vector<int32_t> hcdeg;   // comb-list length per vertex
+    std::vector<int32_t> hdfdeg;  // dense-form list length per vertex (with the lists)
     std::vector<int32_t> hg2deg;  // G2 in- plus out-degree per G2 vertex (de-duplicated)
     std::vector<float> hvscore;   // perturbation weights per G1 vertex (computed on first use)
     std::vector<int32_t> hg2optr, hg2iptr;  // host copies of the G2 list offsets
@@ -159,6 +168,9 @@ int timer_begin(bm_ctx *c, const char *name);
 void timer_end(bm_ctx *c, int token);
 
 int ensure_diag(bm_ctx *c);
+int ensure_dense_lists(bm_ctx *c);
+// out[v] = sum over the list of v of min(w, Q[r][v]), for every v
+int launch_list_diag(bm_ctx *c, int32_t *out, const int32_t *ptr, const int32_t *r, const int32_t *w);
 int launch_check(const char *what);
 
 // launchers shared across translation units (device pointers, context stream)

@@ -1020,7 +1020,7 @@ int refine_finish(bm_ctx *c) {
         for (int v = 0; v < c->N1; v++)
             if (c->hmap[v]) c->hpinv[c->hmap[v]] = v;
     }
-    c->diag_valid = false;
+    c->diag_valid = c->dfdiag_valid = false;
     c->g2y_dirty = !c->sparse_ok;
     c->map_epoch++;  // the device moved the mapping; X matches it unless the last step left it stale
     c->batch->x_epoch = c->map_epoch;

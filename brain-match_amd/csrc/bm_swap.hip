@@ -1096,21 +1096,55 @@ extern "C" int bm_eval_swaps(bm_ctx *c, const int32_t *a, const int32_t *b, int6
     return BM_OK;
 }
 
+// The dense-matrix delta of quick.cu:48-250 through the same pair kernel over
+// the dense-form lists (each entry owns its cell, so the dense-cell weight
+// array is the weight array).
+extern "C" int bm_eval_swaps_dense(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta) {
+    if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (count < 0) return set_error(BM_EINVAL, "count < 0");
+    for (int64_t i = 0; i < count; i++)
+        if (a[i] < 1 || a[i] > c->n || b[i] < 1 || b[i] > c->n || a[i] == b[i])
+            return set_error(BM_EINVAL, "pair %lld: (%d,%d) invalid", (long long)i, a[i], b[i]);
+    if (count == 0) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    int rc = ensure_dense_lists(c);
+    if (rc) return rc;
+    const int64_t chunk = 1 << 22;
+    if ((rc = ensure_stage(c, (size_t)chunk * 12))) return rc;
+    int32_t *dA = (int32_t *)c->stage, *dB = dA + chunk, *dD = dB + chunk;
+    for (int64_t off = 0; off < count; off += chunk) {
+        const int64_t m = std::min(chunk, count - off);
+        BM_HIP(hipMemcpyAsync(dA, a + off, m * 4, hipMemcpyHostToDevice, c->stream));
+        BM_HIP(hipMemcpyAsync(dB, b + off, m * 4, hipMemcpyHostToDevice, c->stream));
+        BM_LAUNCH(c, "eval_pairs_dense",
+                  hipLaunchKernelGGL(k_eval_pairs, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0, c->stream, dA,
+                                     dB, m, dD, c->dfptr, c->dfr, c->dfw, c->dfw, c->Q, c->N1, c->ld));
+        BM_HIP(hipMemcpyAsync(delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
+        BM_HIP(hipStreamSynchronize(c->stream));
+    }
+    return BM_OK;
+}
+
 // One row's list heads: BM_ROW_WAVES = 1 gives each head one wave (four heads
 // per workgroup), 4 (the default) a whole workgroup, which spreads the long
 // lists of hub neighbours over 256 lanes.
-static int launch_row_expand(bm_ctx *c, int a, int lo, int hi, int mode, unsigned long long *res) {
+// dense: the dense-form lists in place of the comb lists.
+static int launch_row_expand(bm_ctx *c, bool dense, int a, int lo, int hi, int mode, unsigned long long *res) {
     static const int waves = [] {
         const char *e = getenv("BM_ROW_WAVES");
         return e && atoi(e) == 1 ? 1 : 4;
     }();
-    const int heads = c->hcdeg[a] + c->hg2deg[c->hmap[a]];
+    const int heads = (dense ? c->hdfdeg[a] : c->hcdeg[a]) + c->hg2deg[c->hmap[a]];
     const int per = 4 / waves;
+    const int32_t *lptr = dense ? c->dfptr : c->cptr, *lmid = dense ? c->dfmid : c->cmid;
+    const int2 *lrw = (const int2 *)(dense ? c->dfrw : c->crw);
+    const int32_t *ldw = dense ? c->dfw : c->cdw;
 #define BM_ROW_EXPAND(W)                                                                                         \
     BM_LAUNCH(c, "row_expand",                                                                                  \
               hipLaunchKernelGGL(k_row_expand<W>, dim3(1 + (heads + per - 1) / per), dim3(256), 0, c->stream, a, lo, \
-                                 hi, mode, c->N1, c->cptr, c->cmid, (const int2 *)c->crw, c->cdw, c->map, c->g2optr,  \
-                                 c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->rowacc, res))
+                                 hi, mode, c->N1, lptr, lmid, lrw, ldw, c->map, c->g2optr, c->g2oy, c->g2iptr,       \
+                                 c->g2iy, c->Q, c->ld, c->rowacc, res))
     if (waves == 1) BM_ROW_EXPAND(1);
     else BM_ROW_EXPAND(4);
 #undef BM_ROW_EXPAND
@@ -1136,7 +1170,7 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
         if (c->g2y_dirty) {
             if ((rc = launch_refresh_g2y(c))) return rc;
         }
-        if ((rc = launch_row_expand(c, a, b_lo, b_hi, mode, res))) return rc;
+        if ((rc = launch_row_expand(c, false, a, b_lo, b_hi, mode, res))) return rc;
         BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((b_hi - b_lo + 256) / 256), dim3(256), 0,
                                                      c->stream, a, b_lo, b_hi, mode, c->rowacc, c->cneg, c->diag, res,
                                                      nullptr));
@@ -1173,10 +1207,11 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
 // the pairs that touch them): the row kernels above with their reduction
 // replaced by a store of every delta, rows one after another on the stream
 // into the staging buffer, then one copy to the caller.
-extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32_t *deltas) {
+extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32_t kind, int32_t *deltas) {
     if (!c || (count > 0 && (!rows || !deltas))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
     if (count < 0 || count > 64) return set_error(BM_EINVAL, "count=%d outside [0,64]", count);
+    if (kind != BM_DELTA_LISTS && kind != BM_DELTA_DENSE) return set_error(BM_EINVAL, "kind=%d unknown", kind);
     for (int i = 0; i < count; i++) {
         if (rows[i] < 1 || rows[i] > c->n) return set_error(BM_EINVAL, "rows[%d]=%d outside [1,%d]", i, rows[i], c->n);
         for (int j = 0; j < i; j++)
@@ -1184,33 +1219,41 @@ extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32
     }
     if (count == 0) return BM_OK;
     BM_HIP(hipSetDevice(c->device));
-    int rc = ensure_diag(c);
+    const bool dense = kind == BM_DELTA_DENSE;
+    int rc = dense ? ensure_dense_lists(c) : ensure_diag(c);
     if (rc) return rc;
     const size_t N1 = (size_t)c->N1;
     if ((rc = ensure_stage(c, (size_t)count * N1 * 4))) return rc;
     int32_t *out = (int32_t *)c->stage;
     unsigned long long *res = c->keys + c->N1 + 1;  // written by the expand kernel, not read here
     if (c->sparse_ok && c->g2y_dirty && (rc = launch_refresh_g2y(c))) return rc;
+    // the dense form: X(v,v) over its own lists, from the current P (kept current by later moves)
+    if (dense && !c->dfdiag_valid) {
+        if ((rc = launch_list_diag(c, c->dfdiag, c->dfptr, c->dfr, c->dfw))) return rc;
+        c->dfdiag_valid = true;
+    }
+    const int32_t *lptr = dense ? c->dfptr : c->cptr, *lr = dense ? c->dfr : c->cr;
+    const int32_t *lw = dense ? c->dfw : c->cw, *ldw = dense ? c->dfw : c->cdw;
+    const int32_t *ldiag = dense ? c->dfdiag : c->diag, *lneg = dense ? c->dfneg : c->cneg;
     const int n = c->n;
     for (int i = 0; i < count; i++) {
         const int a = rows[i];
         int32_t *row = out + (size_t)i * N1;
         if (c->sparse_ok) {
-            if ((rc = launch_row_expand(c, a, 0, n, BM_ROW_ARGMAX, res))) return rc;
+            if ((rc = launch_row_expand(c, dense, a, 0, n, BM_ROW_ARGMAX, res))) return rc;
             BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_final, dim3((n + 256) / 256), dim3(256), 0, c->stream,
-                                                         a, 0, n, BM_ROW_ARGMAX, c->rowacc, c->cneg, c->diag, res,
-                                                         row));
+                                                         a, 0, n, BM_ROW_ARGMAX, c->rowacc, lneg, ldiag, res, row));
         } else {
             const int blocks = (n + kRowCh) / kRowCh;
             const size_t lds = (size_t)2 * c->N1 * sizeof(int16_t);
             if (lds <= 96 * 1024) {
                 BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_eval<true>, dim3(blocks), dim3(256), lds, c->stream,
-                                                             a, 0, n, BM_ROW_ARGMAX, c->N1, c->ld, c->Q, c->cptr,
-                                                             c->cr, c->cw, c->cdw, c->diag, res, row));
+                                                             a, 0, n, BM_ROW_ARGMAX, c->N1, c->ld, c->Q, lptr, lr, lw,
+                                                             ldw, ldiag, res, row));
             } else {
                 BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_eval<false>, dim3(blocks), dim3(256), 0, c->stream,
-                                                             a, 0, n, BM_ROW_ARGMAX, c->N1, c->ld, c->Q, c->cptr,
-                                                             c->cr, c->cw, c->cdw, c->diag, res, row));
+                                                             a, 0, n, BM_ROW_ARGMAX, c->N1, c->ld, c->Q, lptr, lr, lw,
+                                                             ldw, ldiag, res, row));
             }
         }
     }

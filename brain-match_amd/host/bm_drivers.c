@@ -202,8 +202,8 @@ typedef struct {
     uint8_t *in_set;
 } quick_scan;
 
-static int quick_eval(bm_ctx *ctx, int32_t n, const int32_t *changed, int nc, quick_scan *q) {
-    int rc = bm_eval_rows(ctx, changed, nc, q->rows);
+static int quick_eval(bm_ctx *ctx, int32_t n, const int32_t *changed, int nc, int32_t kind, quick_scan *q) {
+    int rc = bm_eval_rows(ctx, changed, nc, kind, q->rows);
     if (rc) return rc;
     const size_t N1 = (size_t)n + 1;
     q->pairs += (int64_t)nc * (nc - 1) / 2 + (int64_t)nc * (n - nc);
@@ -250,7 +250,8 @@ static void sort_small(int32_t *a, int m) {
 
 /* quick.cu:1137-1324 */
 int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping, bm_quick_stats *stats) {
-    if (!ctx || !p || p->max_evals < 1 || p->zero_limit < 1 || p->best_window < 0 || !(p->gap_norm > 0))
+    if (!ctx || !p || p->max_evals < 1 || p->zero_limit < 1 || p->best_window < 0 || !(p->gap_norm > 0) ||
+        (p->delta_kind != BM_DELTA_LISTS && p->delta_kind != BM_DELTA_DENSE))
         return BM_EINVAL;
     const int32_t n = bm_num_vertices(ctx);
     if (n < 8) return BM_EINVAL; /* 4 swaps need 8 distinct vertices */
@@ -285,7 +286,7 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
         sort_small(changed, 2 * k);
         if ((rc = bm_score(ctx, &cur))) goto out;
         st.perturbations++;
-        if ((rc = quick_eval(ctx, n, changed, 2 * k, &q))) goto out;
+        if ((rc = quick_eval(ctx, n, changed, 2 * k, p->delta_kind, &q))) goto out;
         iter++;
         int zerocheck = 0;
         /* :1203-1318 */
@@ -320,7 +321,7 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
             }
             changed[0] = m1 < m2 ? m1 : m2;
             changed[1] = m1 < m2 ? m2 : m1;
-            if ((rc = quick_eval(ctx, n, changed, 2, &q))) goto out;
+            if ((rc = quick_eval(ctx, n, changed, 2, p->delta_kind, &q))) goto out;
             iter++;
         }
     }

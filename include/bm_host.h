@@ -186,9 +186,12 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *params, int3
  * numbers them (pairs inside the changed set, then changed x unchanged
  * ascending); the best is the first strict maximum.  Random choices come
  * from a splitmix64 state seeded with seed, in the reference's order of
- * rand() calls.  Deltas are Delta_B (the exact swap delta) where the
- * reference's kernel omits the self-loop correction.  Stops after
- * max_evals evaluations and leaves the context on the best mapping. */
+ * rand() calls.  delta_kind ranks the pairs: BM_DELTA_DENSE with the
+ * reference kernel's own dense-matrix delta (bm_eval_swaps_dense), or
+ * BM_DELTA_LISTS with Delta_B; the two agree unless G1 has self-loops,
+ * repeated lines, lines at vertex 0 or weights outside [1, 32767].  The
+ * score is S_opt either way, kept by the exact change of every swap.  Stops
+ * after max_evals evaluations and leaves the context on the best mapping. */
 typedef struct {
     uint64_t seed;
     int64_t max_evals;
@@ -196,6 +199,7 @@ typedef struct {
     int32_t best_window; /* 500 */
     double gap;          /* 0.002 */
     double gap_norm;     /* 5154247, the reference's benchmark score */
+    int32_t delta_kind;  /* BM_DELTA_DENSE as quick.cu, or BM_DELTA_LISTS */
 } bm_quick_params;
 
 typedef struct {

@@ -100,12 +100,27 @@ int bm_eval_swaps(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count
 int bm_eval_swap_row(bm_ctx *ctx, int32_t a, int32_t b_lo, int32_t b_hi, int mode,
                      int32_t *best_b, int32_t *best_delta);
 
-/* Whole rows: deltas[i * (n + 1) + b] = Delta_B(rows[i], b) for b in
- * [1, n], b != rows[i]; the entries b = 0 and b = rows[i] are 0.  These are
- * the pairs calculateSwapDeltasKernelQuick (quick.cu:141-250) evaluates
- * after a perturbation or an accepted swap: every pair that touches one of
- * the changed vertices (rows distinct, in [1, n], count <= 64). */
-int bm_eval_rows(bm_ctx *ctx, const int32_t *rows, int32_t count, int32_t *deltas);
+/* The swap delta of the dense-matrix GPU kernels, calculateSwapDeltasKernel
+ * and calculateSwapDeltasKernelQuick (quick.cu:48-250; the same arithmetic
+ * in graph-re-alignment-gpu.cu:60-128): for every i in [1, n] other than a
+ * and b, the terms of the G1 cells W1[a][i], W1[i][a], W1[b][i], W1[i][b]
+ * that are positive as short (last CSV line of a pair wins), then the two
+ * cells between a and b.  It equals Delta_B when G1 has no self-loop, no
+ * repeated line, no line at vertex 0 and every weight in [1, 32767];
+ * otherwise it is that kernel's value, not the score change.  Pairs in
+ * [1, n], a != b. */
+int bm_eval_swaps_dense(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta);
+
+/* Whole rows: deltas[i * (n + 1) + b] = Delta(rows[i], b) for b in [1, n],
+ * b != rows[i]; the entries b = 0 and b = rows[i] are 0.  These are the
+ * pairs calculateSwapDeltasKernelQuick (quick.cu:141-250) evaluates after a
+ * perturbation or an accepted swap: every pair that touches one of the
+ * changed vertices (rows distinct, in [1, n], count <= 64).  kind selects
+ * the delta: BM_DELTA_LISTS = Delta_B of bm_eval_swaps, BM_DELTA_DENSE = the
+ * quick.cu kernel's own delta of bm_eval_swaps_dense. */
+#define BM_DELTA_LISTS 0
+#define BM_DELTA_DENSE 1
+int bm_eval_rows(bm_ctx *ctx, const int32_t *rows, int32_t count, int32_t kind, int32_t *deltas);
 
 /* Perturbation by k disjoint swaps, random_swap_k_vertices:
  *   BM_PERTURB_OMP      alignment-omp.c:495-541: 2k distinct indices drawn

@@ -820,7 +820,8 @@ typedef struct {
     int32_t best_a, best_b, max_delta;
 } orc_qscan;
 
-static void orc_qscan_pairs(const orc_graphs *g, const int32_t *map, const int32_t *changed, int nc, orc_qscan *q) {
+static void orc_qscan_pairs(const orc_graphs *g, const int32_t *map, const int32_t *changed, int nc, int kind,
+                            orc_qscan *q) {
     int n = g->n;
     q->npos = q->nzero = 0;
     q->max_delta = 0;
@@ -840,7 +841,7 @@ static void orc_qscan_pairs(const orc_graphs *g, const int32_t *map, const int32
                     if (in_set[v]) continue;
                     b = v;
                 }
-                int d = orc_delta_b(g, map, a, b);
+                int d = kind ? orc_delta_g2(g, map, a, b) : orc_delta_b(g, map, a, b);
                 if (d > q->max_delta) {
                     q->max_delta = d;
                     q->best_a = a;
@@ -870,7 +871,7 @@ static void orc_sort_ints(int32_t *a, int m) {
 
 /* quick.cu:1137-1324, one process */
 int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit,
-              int best_window, double gap, double gap_norm, int64_t *out) {
+              int best_window, double gap, double gap_norm, int kind, int64_t *out) {
     int n = g->n;
     if (n < 8) return -1;
     size_t bytes = ((size_t)n + 1) * sizeof(int32_t);
@@ -891,7 +892,7 @@ int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_eval
         orc_sort_ints(changed, 2 * k);
         cur = orc_score_opt(g, map);
         perturbations++;
-        orc_qscan_pairs(g, map, changed, 2 * k, &q);
+        orc_qscan_pairs(g, map, changed, 2 * k, kind, &q);
         iter++;
         int zerocheck = 0;
         while ((q.max_delta > 0 || q.nzero > 0) && zerocheck < zero_limit && iter < max_evals) {
@@ -926,7 +927,7 @@ int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_eval
             }
             changed[0] = m1 < m2 ? m1 : m2;
             changed[1] = m1 < m2 ? m2 : m1;
-            orc_qscan_pairs(g, map, changed, 2, &q);
+            orc_qscan_pairs(g, map, changed, 2, kind, &q);
             iter++;
         }
     }

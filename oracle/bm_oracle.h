@@ -139,11 +139,12 @@ int orc_row_best_restart(const orc_graphs *g, int32_t *map, uint64_t seed, int i
                          int idle_passes, int64_t max_passes, int64_t *out);
 
 /* quick.cu:1137-1324 changed-vertex search for max_evals evaluations; map
- * ends on the best mapping; out[0..8] = evals, perturbations, moves,
- * positive moves, zero moves, initial, final (current), best score, pairs
- * evaluated. */
+ * ends on the best mapping; pairs are ranked by Delta_G2 (kind 1, the
+ * reference kernel's delta) or Delta_B (kind 0); out[0..8] = evals,
+ * perturbations, moves, positive moves, zero moves, initial, final (current),
+ * best score, pairs evaluated. */
 int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit,
-              int best_window, double gap, double gap_norm, int64_t *out);
+              int best_window, double gap, double gap_norm, int kind, int64_t *out);
 
 #ifdef __cplusplus
 }

@@ -61,7 +61,7 @@ def lib():
         "orc_perturb": (C.c_int, [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_uint64), _i32p]),
         "orc_row_best_restart": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int64, _i64p]),
         "orc_quick": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double,
-                                _i64p]),
+                                C.c_int, _i64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -149,10 +149,11 @@ class Oracle:
         keys = ("passes", "restarts", "accepted", "initial_score", "final_score", "best_score")
         return mm, dict(zip(keys, (int(x) for x in out)))
 
-    def quick(self, m, seed=1, max_evals=1000, zero_limit=1000, best_window=500, gap=0.002, gap_norm=5154247.0):
+    def quick(self, m, seed=1, max_evals=1000, zero_limit=1000, best_window=500, gap=0.002, gap_norm=5154247.0,
+              kind=1):
         mm = _i32(m).copy()
         out = np.zeros(9, np.int64)
-        if lib().orc_quick(self._h, mm, seed, max_evals, zero_limit, best_window, gap, gap_norm, out):
+        if lib().orc_quick(self._h, mm, seed, max_evals, zero_limit, best_window, gap, gap_norm, kind, out):
             raise ValueError("orc_quick failed")
         keys = ("evals", "perturbations", "moves", "positive_moves", "zero_moves", "initial_score", "final_score",
                 "best_score", "pairs")

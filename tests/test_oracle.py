@@ -123,6 +123,42 @@ def test_delta_g2_drops_self_loops_only():
             assert o.delta_g2(mp, a, b) == want == ob_noloop.delta_b(mp, a, b)
 
 
+def dense_cell_score(n, g1, g2, mp):
+    """Sum over the off-diagonal G1 cells (i, j >= 1) that are positive as short, last line winning, of
+    min(cell, W2[p(i)][p(j)]): the quantity whose change quick.cu's dense kernel computes."""
+    W1 = np.zeros((n + 1, n + 1), np.int64)
+    W1[g1[0], g1[1]] = g1[2]
+    W1 = W1.astype(np.int16).astype(np.int64)
+    W2 = np.zeros((n + 1, n + 1), np.int64)
+    W2[g2[0], g2[1]] = g2[2]
+    W2 = W2.astype(np.int16).astype(np.int64)
+    i, j = np.nonzero(W1 > 0)
+    keep = (i != j) & (i >= 1) & (j >= 1)
+    i, j = i[keep], j[keep]
+    return int(np.minimum(W1[i, j], W2[mp[i], mp[j]]).sum())
+
+
+@pytest.mark.parametrize("case", ["multigraph", "vertex0", "negative", "wide"])
+def test_delta_g2_is_the_dense_cell_score_change(case):
+    # repeated lines, self-loops, lines at vertex 0, weights <= 0 and weights that wrap as short: every
+    # way the dense form departs from Delta_B, against an independent numpy rescore
+    n = 12
+    kw = dict(zero_vertex=case == "vertex0", negative=case == "negative", wmax=40000 if case == "wide" else 30)
+    g1 = random_multigraph(21, n, 90, 0.3, 0.1, **kw)
+    g2 = random_multigraph(22, n, 90, 0.3, 0.1, wmax=kw["wmax"])
+    o = ob.Oracle(n, g1, g2)
+    mp = random_mapping(23, n, 0.1, bijective=False)
+    differs = 0
+    for a in range(1, n + 1):
+        for b in range(1, n + 1):
+            if a == b:
+                continue
+            want = dense_cell_score(n, g1, g2, swapped(mp, a, b)) - dense_cell_score(n, g1, g2, mp)
+            assert o.delta_g2(mp, a, b) == want, (a, b)
+            differs += want != o.delta_b(mp, a, b)
+    assert differs > 0  # the case does separate the two deltas
+
+
 G3 = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
 
 

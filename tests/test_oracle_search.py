@@ -137,14 +137,15 @@ def test_restart_keeps_the_best_and_restarts_when_idle():
     assert sorted(got.tolist()) == sorted(m0.tolist())  # a permutation of the same images (vertex 0 included)
 
 
+@pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("gap_norm", [5154247.0, 50.0])
-def test_quick_invariants(gap_norm):
+def test_quick_invariants(gap_norm, kind):
     n = 40
     g1 = random_multigraph(12, n, 300, 0.1, 0.05)
     g2 = random_multigraph(13, n, 300, 0.1, 0.05)
     o = ob.Oracle(n, g1, g2)
     m0 = random_mapping(14, n)
-    got, st = o.quick(m0, seed=15, max_evals=300, zero_limit=20, best_window=30, gap_norm=gap_norm)
+    got, st = o.quick(m0, seed=15, max_evals=300, zero_limit=20, best_window=30, gap_norm=gap_norm, kind=kind)
     assert st["evals"] == 300
     assert st["moves"] == st["positive_moves"] + st["zero_moves"]
     assert st["moves"] + st["perturbations"] == st["evals"]
@@ -152,3 +153,21 @@ def test_quick_invariants(gap_norm):
     assert st["best_score"] == o.score_opt(got) == np_score(n, g1, g2, got) >= st["initial_score"]
     assert sorted(got[1:].tolist()) == sorted(m0[1:].tolist()) and got[0] == m0[0]
     assert st["positive_moves"] > 0 and st["perturbations"] > 1
+
+
+def test_quick_delta_kinds():
+    # the two rankings coincide where the deltas do (no loops, no repeated lines, positive weights) and
+    # part on a multigraph
+    n = 40
+    kw = dict(seed=5, max_evals=300, zero_limit=20, best_window=30)
+    g1, g2 = simple_graph(30, n, 300), simple_graph(31, n, 300)
+    o = ob.Oracle(n, g1, g2)
+    m0 = random_mapping(32, n)
+    a, sa = o.quick(m0, kind=0, **kw)
+    b, sb = o.quick(m0, kind=1, **kw)
+    assert sa == sb and (a == b).all()
+    g1 = random_multigraph(33, n, 300, 0.3, 0.1)
+    o = ob.Oracle(n, g1, g2)
+    a, sa = o.quick(m0, kind=0, **kw)
+    b, sb = o.quick(m0, kind=1, **kw)
+    assert sa != sb
