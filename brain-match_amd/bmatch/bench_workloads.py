@@ -102,11 +102,15 @@ def rot3(args):
         for _ in range(steps):
             best = e.moves_eval(bmatch.BM_ROT3_ROTATIONS)
         el = (time.perf_counter() - t) / steps
-        n_k, ms_k = e.kernel_time("rot3")
+        kernels = {k: e.kernel_time(k) for k in ("rot3", "rot3_table", "gain", "table_pack")}
+    table = kernels["rot3_table"][0] > 0
+    n_k, ms_k = kernels["rot3_table" if table else "rot3"]
     out = _base("3-rotation deltas per second", 2 * count / el, "rotation evals/s", steps, el * 1000,
                 {"workload": "power-law pair n=1000 E=20000, 2^24 two-hop triples per launch, both rotations",
                  "n": inst.n, "triples": count})
-    out.update({"best": best, "kernel_avg_ms": ms_k / max(n_k, 1)})
+    # the table form rebuilds its all-pairs pass (gain + table_pack) in every launch, inside the timing
+    out.update({"best": best, "form": "table" if table else "direct", "kernel_avg_ms": ms_k / max(n_k, 1),
+                "kernels_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kernels.items() if v[0]}})
     return out
 
 

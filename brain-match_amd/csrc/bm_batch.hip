@@ -104,12 +104,13 @@ extern "C" int bm_moves_set(bm_ctx *c, int32_t k, const int32_t *verts, int64_t 
 static int eval_resident(bm_ctx *c, int mode) {
     const int64_t m = c->mv_count;
     int rc = BM_OK;
-    if (c->mv_k == 3 && !c->rot3_tuned && m >= 4096) rc = tune_rot3(c, c->mv_verts, m, c->mv_delta, c->mv_aux);
-    if (c->mv_k >= 4 && c->permset_threads_k != c->mv_k && m >= 256)
+    int table = 0;
+    if (c->mv_k == 3) rc = choose_rot3_table(c, c->mv_verts, m, c->mv_delta, c->mv_aux, &table);
+    if (!rc && c->mv_k >= 4 && c->permset_threads_k != c->mv_k && m >= 256)
         rc = tune_permset(c, c->mv_k, c->mv_verts, m, c->mv_delta, c->mv_aux, c->mv_flag, 2);
     if (rc) return rc;
     if (c->mv_k == 2) rc = launch_eval_pairs(c, c->mv_verts, c->mv_verts + m, m, c->mv_delta);
-    else if (c->mv_k == 3) rc = launch_rot3(c, c->mv_verts, m, mode, c->mv_delta, c->mv_aux);
+    else if (c->mv_k == 3) rc = launch_rot3_moves(c, c->mv_verts, m, mode, c->mv_delta, c->mv_aux, table, true);
     else rc = launch_permset(c, c->mv_k, c->mv_verts, m, c->mv_delta, c->mv_aux, c->mv_flag);
     if (rc) return rc;
     unsigned long long *key = c->keys + c->N1 + 3;

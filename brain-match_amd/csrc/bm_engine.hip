@@ -638,7 +638,11 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->dfrw, (size_t)c->ndf * 8);
     dev_free(c, (void **)&c->dfneg, N1 * 4);
     dev_free(c, (void **)&c->dfdiag, N1 * 4);
+    dev_free(c, (void **)&c->dfX, (N1 + 63) / 64 * 64 * ld * 4);
+    dev_free(c, (void **)&c->dfW1, N1 * ld * 2);
+    dev_free(c, (void **)&c->dfXP, N1 * ld * 8);
     c->ndf = 0;
+    c->dfmaxw = 0;
     c->dfdiag_valid = false;
     c->hdfdeg.clear();
     dev_free(c, (void **)&c->esrc, (size_t)c->e1 * 4);
@@ -1168,6 +1172,7 @@ int bm::ensure_dense_lists(bm_ctx *c) {
             fw.push_back(cell);
             frw.push_back(r[e]);
             frw.push_back(cell);
+            c->dfmaxw = std::max(c->dfmaxw, cell);
             if (r[e] >= N1) fmid[v] = (int32_t)fr.size();
         }
         deg[v] = (int32_t)fr.size() - fptr[v];

@@ -76,6 +76,12 @@ struct bm_ctx {
     int32_t *dfdiag = nullptr;  // X(v,v) over the dense-form lists
     bool dfdiag_valid = false;
     int64_t ndf = 0;
+    int32_t dfmaxw = 0;         // largest dense-form weight
+    // 3-vertex table form (lazy): X over the dense-form lists for all pairs
+    // (tile-major like X) and the dense-form G1 cells as an int16 matrix [n+1 x ld]
+    int32_t *dfX = nullptr;
+    int16_t *dfW1 = nullptr;
+    int32_t *dfXP = nullptr;    // (Xd, cell << 16 | P) per pair, row-major [n+1 x ld] int2
     int32_t *esrc = nullptr, *edst = nullptr, *ew = nullptr, *edw = nullptr;
     // G2 de-duplicated (last line wins) out- and in-lists, packed (vertex << 16 | (uint16)weight)
     int32_t *g2optr = nullptr, *g2iptr = nullptr;
@@ -171,6 +177,9 @@ int ensure_diag(bm_ctx *c);
 int ensure_dense_lists(bm_ctx *c);
 // out[v] = sum over the list of v of min(w, Q[r][v]), for every v
 int launch_list_diag(bm_ctx *c, int32_t *out, const int32_t *ptr, const int32_t *r, const int32_t *w);
+// X[u][v] = sum over the list of u of min(w, Q[r][v]) for all u, v (tile-major,
+// stride ld), lists of non-negative weights <= max_w given as (r, w) pairs
+int launch_list_gain(bm_ctx *c, const int32_t *ptr, const int32_t *rw, int32_t *X, int max_w);
 int launch_check(const char *what);
 
 // launchers shared across translation units (device pointers, context stream)
@@ -181,6 +190,12 @@ int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, 
 int free_moves(bm_ctx *c);
 // autotuner of the move kernels: times the variants on the given device list
 int tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP);
+// Delta_G3 of a move list outside the batched step: the direct kernel or the
+// table form (choose_rot3_table times both once per instance and size class);
+// prepare = build the table form's all-pairs pass for the current mapping.
+int choose_rot3_table(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP, int *table);
+int launch_rot3_moves(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP, int table,
+                      bool prepare);
 int tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult);
 int free_batch(bm_ctx *c);
 // P and P^T from W2 and the device mapping; with patch_flag, when *patch_flag

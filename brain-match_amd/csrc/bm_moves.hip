@@ -20,6 +20,7 @@
 #include <mutex>
 #include <tuple>
 
+#include "bm_device.h"
 #include "bm_internal.h"
 #include "bm_wave.h"
 
@@ -125,6 +126,102 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
         best_d[t] = bd;
         best_p[t] = bp;
     }
+}
+
+// Table form of the same Delta_G3.  With Xd(u,v) = sum over the dense-form
+// list of u (the entries triple.cu's scan visits: cells > 0 as short, no
+// self-loop, no vertex 0) of min(w, Q[r][v]), moving m_k to the image of
+// m_s(k) changes its list terms by Xd(m_k, m_s(k)) - Xd(m_k, m_k); the scan
+// skips the entries between triple vertices, so their terms are taken out
+// again and the six intra-triple cells (:164-207) added in their place.  All
+// of that needs only the 3x3 blocks of Xd, of P and of the G1 cells at the
+// triple, which one table holds side by side: XP[u][v] = (Xd(u,v),
+// W1[u][v] << 16 | (uint16)P[u][v]), 8 bytes, so one thread per triple makes
+// nine loads instead of three per list entry.
+__global__ __launch_bounds__(256) void k_rot3_table(const int32_t *__restrict__ T, int64_t count, int mode,
+                                                    int32_t *__restrict__ best_d, int32_t *__restrict__ best_p,
+                                                    const int2 *__restrict__ XP, int ld) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int m[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) m[i] = T[3 * t + i];
+    int p[3][3], x[3][3], c[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int2 v = XP[(size_t)m[i] * ld + m[j]];
+            x[i][j] = v.x;
+            p[i][j] = (int)(int16_t)(v.y & 0xffff);  // P[m_i][m_j] = W2[f_i][f_j]
+            c[i][j] = i == j ? 0 : v.y >> 16;         // the G1 cell m_i -> m_j
+        }
+    int bd = 0, bp = 0;
+#pragma unroll
+    for (int q = 1; q < 6; q++) {
+        if (mode == BM_ROT3_ROTATIONS && q != 3 && q != 4) continue;
+        // m_k takes the image of m_s(k), s(k) = kG3Perm[q][k]
+        int d = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) d += x[k][kG3Perm[q][k]] - x[k][k];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int w = c[i][j];
+                if (i == j || w <= 0) continue;
+                const int si = kG3Perm[q][i], sj = kG3Perm[q][j];
+                const int old = min(w, p[i][j]);
+                // the cell's out entry in m_i's list and in entry in m_j's list, as counted in Xd
+                d -= min(w, p[si][j]) - old;
+                d -= min(w, p[i][sj]) - old;
+                d += min(w, p[si][sj]) - old;  // its own term
+            }
+        if (d > bd) {
+            bd = d;
+            bp = q;
+        }
+    }
+    best_d[t] = bd;
+    best_p[t] = bp;
+}
+
+// XP[u][v] for u < N1, v < ld from the tile-major Xd, P (rows 0..n of Q) and
+// the cell matrix; one thread per pair of columns.
+__global__ __launch_bounds__(256) void k_table_pack(int2 *__restrict__ XP, const int32_t *__restrict__ X,
+                                                    const int16_t *__restrict__ Q, const int16_t *__restrict__ W1,
+                                                    int N1, int ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t half = (int64_t)ld / 2;
+    if (i >= (int64_t)N1 * half) return;
+    const int u = (int)(i / half), v = (int)(i - (int64_t)u * half) * 2;
+    const size_t at = (size_t)u * ld + v;
+    const size_t xa = xidx(u, v, ld >> 6);  // v even: v and v + 1 share the tile row
+    const int2 xs = *reinterpret_cast<const int2 *>(X + xa);
+    const int ps = *reinterpret_cast<const int *>(Q + at), ws = *reinterpret_cast<const int *>(W1 + at);
+    int4 out;
+    out.x = xs.x;
+    out.y = (int)(((uint32_t)ws << 16) | ((uint32_t)ps & 0xffffu));
+    out.z = xs.y;
+    out.w = (int)(((uint32_t)ws & 0xffff0000u) | ((uint32_t)ps >> 16));
+    *reinterpret_cast<int4 *>(XP + at) = out;
+}
+
+// dfW1[v][d] = the dense-form cell of the out entry v -> d (0 elsewhere).
+__global__ __launch_bounds__(256) void k_df_cells(int16_t *__restrict__ W1, const int32_t *__restrict__ ptr,
+                                                  const int32_t *__restrict__ mid, const int2 *__restrict__ rw,
+                                                  int N1, int ld) {
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= N1) return;
+    for (int e = ptr[v] + (threadIdx.x & 63); e < mid[v]; e += 64) {
+        const int2 x = rw[e];
+        W1[(size_t)v * ld + (x.x - N1)] = (int16_t)x.y;
+    }
+}
+
+__global__ void k_zero16(int16_t *__restrict__ p, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
 }
 
 constexpr int kMaxIntra = 256;
@@ -317,6 +414,39 @@ int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *
     return BM_OK;
 }
 
+// The table form: Xd for the current mapping (one dense all-pairs pass over
+// the dense-form lists, made for every evaluation call since the mapping may
+// have changed), then one thread per triple.  The G1 cell matrix is built once
+// per graph pair.
+static int rot3_table_prepare(bm_ctx *c) {
+    int rc = ensure_dense_lists(c);
+    if (rc) return rc;
+    const size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
+    if (!c->dfX && (rc = dev_alloc(c, (void **)&c->dfX, (N1 + 63) / 64 * 64 * ld * 4))) return rc;
+    if (!c->dfXP && (rc = dev_alloc(c, (void **)&c->dfXP, N1 * ld * 8))) return rc;
+    if (!c->dfW1) {
+        if ((rc = dev_alloc(c, (void **)&c->dfW1, N1 * ld * 2))) return rc;
+        const int64_t cells = (int64_t)(N1 * ld);
+        BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_zero16, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0,
+                                                    c->stream, c->dfW1, cells));
+        BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_df_cells, dim3((unsigned)((N1 + 3) / 4)), dim3(256), 0,
+                                                    c->stream, c->dfW1, c->dfptr, c->dfmid, (const int2 *)c->dfrw,
+                                                    c->N1, c->ld));
+    }
+    if ((rc = launch_list_gain(c, c->dfptr, c->dfrw, c->dfX, c->dfmaxw))) return rc;
+    const int64_t pairs = (int64_t)N1 * (int64_t)(ld / 2);
+    BM_LAUNCH(c, "table_pack", hipLaunchKernelGGL(k_table_pack, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0,
+                                                  c->stream, (int2 *)c->dfXP, c->dfX, c->Q, c->dfW1, c->N1, c->ld));
+    return BM_OK;
+}
+
+static int launch_rot3_table(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP) {
+    if (m <= 0) return BM_OK;
+    BM_LAUNCH(c, "rot3_table", hipLaunchKernelGGL(k_rot3_table, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+                                                  c->stream, dT, m, mode, dD, dP, (const int2 *)c->dfXP, c->ld));
+    return BM_OK;
+}
+
 int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult) {
     if (m <= 0) return BM_OK;
 #define BM_PERMSET(T, K)                                                                                        \
@@ -416,6 +546,44 @@ int bm::tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t 
     return rc;
 }
 
+// Direct kernel or table form for a triple list of m entries: the table form
+// pays one all-pairs pass per call and then 24 loads per triple, the direct
+// kernel three loads per list entry, so long lists and large batches favour
+// the table.  Both are timed cold (the table with its pass) on the first
+// batch of each size class of an instance class; BM_ROT3_TABLE = 0 / 1
+// forces one.  The table needs (n+1) x ld x 14 bytes, so it is only offered up
+// to kTableMaxBytes.
+constexpr size_t kTableMaxBytes = (size_t)8 << 30;
+
+int bm::choose_rot3_table(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP, int *table) {
+    *table = 0;
+    int rc = BM_OK;
+    if (!c->rot3_tuned && m >= kTuneMinTriples && (rc = tune_rot3(c, dT, m, dD, dP))) return rc;
+    if ((size_t)c->N1 * c->ld * 14 > kTableMaxBytes) return BM_OK;
+    if (m < kTuneMinTriples) {
+        const char *f = getenv("BM_ROT3_TABLE");
+        *table = f && atoi(f) == 1;
+        return BM_OK;
+    }
+    static const int ch[2] = {0, 1};
+    const auto key = std::make_tuple(4, log2c(c->n) * 64 + log2c(m), log2c(std::max<long long>(1, c->ndd / c->N1)), 3);
+    int v = 0;
+    rc = tune_variant(c, "BM_ROT3_TABLE", key, ch, 2, [&](int x) { v = x; }, [&] {
+        if (!v) return launch_rot3(c, dT, m, BM_ROT3_ALL, dD, dP);
+        int r = rot3_table_prepare(c);
+        return r ? r : launch_rot3_table(c, dT, m, BM_ROT3_ALL, dD, dP);
+    });
+    *table = v == 1;
+    return rc;
+}
+
+int bm::launch_rot3_moves(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP, int table,
+                          bool prepare) {
+    if (!table) return launch_rot3(c, dT, m, mode, dD, dP);
+    int rc = prepare ? rot3_table_prepare(c) : BM_OK;
+    return rc ? rc : launch_rot3_table(c, dT, m, mode, dD, dP);
+}
+
 int bm::tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult) {
     static const int ch[2] = {64, 256};
     const auto key = std::make_tuple(2, log2c(c->n), log2c(std::max<long long>(1, c->ncomb / c->N1)), k);
@@ -446,11 +614,13 @@ extern "C" int bm_eval_rot3(bm_ctx *c, const int32_t *triples, int64_t count, in
     int rc = ensure_stage(c, (size_t)chunk * 20);
     if (rc) return rc;
     int32_t *dT = (int32_t *)c->stage, *dD = dT + 3 * chunk, *dP = dD + chunk;
+    int table = 0;
     for (int64_t off = 0; off < count; off += chunk) {
         const int64_t m = std::min(chunk, count - off);
         BM_HIP(hipMemcpyAsync(dT, triples + 3 * off, m * 12, hipMemcpyHostToDevice, c->stream));
-        if (!c->rot3_tuned && m >= kTuneMinTriples && (rc = tune_rot3(c, dT, m, dD, dP))) return rc;
-        if ((rc = launch_rot3(c, dT, m, mode, dD, dP))) return rc;
+        if (off == 0 && (rc = choose_rot3_table(c, dT, m, dD, dP, &table))) return rc;
+        // the mapping is the same for every chunk: the table form's pass runs once
+        if ((rc = launch_rot3_moves(c, dT, m, mode, dD, dP, table, off == 0))) return rc;
         BM_HIP(hipMemcpyAsync(best_delta + off, dD, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipMemcpyAsync(best_perm + off, dP, m * 4, hipMemcpyDeviceToHost, c->stream));
         BM_HIP(hipStreamSynchronize(c->stream));

@@ -493,27 +493,42 @@ __global__ __launch_bounds__(256) void k_gain(const int32_t *__restrict__ cptr, 
     }
 }
 
+// X (tile-major, row stride ld) of the lists (ptr, rw) for rows 0..N1-1;
+// max_w bounds the weights and nonneg says all are >= 0.
 template <int kCols>
-static int launch_gain(bm_ctx *c, int N1) {
+static int launch_gain_on(bm_ctx *c, const int32_t *ptr, const int32_t *rw, int32_t *X, int max_w, bool nonneg) {
+    const int N1 = c->N1;
     const int nslab = (c->ld + 64 * kCols - 1) / (64 * kCols), nrb = (N1 + kRowsWG - 1) / kRowsWG;
     const int per_xcd = (int)(((int64_t)nslab * nrb + 7) / 8);
     // packed uint16 partial sums need 0 <= min(w, P) and flush * max(w) < 65536
     // (the tail after the last full block adds < kGainUnroll more entries)
-    const int flush = c->max_w1 > 0 ? ((65535 / c->max_w1) - kGainUnroll) & ~(kGainUnroll - 1) : 1 << 20;
-    if (kCols > 1 && c->nonneg && flush >= kGainUnroll) {
+    const int flush = max_w > 0 ? ((65535 / max_w) - kGainUnroll) & ~(kGainUnroll - 1) : 1 << 20;
+    if (kCols > 1 && nonneg && flush >= kGainUnroll) {
         BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain<kCols, true>), dim3(8 * per_xcd), dim3(256), 0, c->stream,
-                                                c->cptr, (const int2 *)c->crw, c->Q, c->ld, c->X, c->ldx, N1, nslab,
+                                                ptr, (const int2 *)rw, c->Q, c->ld, X, (int64_t)c->ld, N1, nslab,
                                                 nrb, per_xcd, flush));
     } else {
         BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain<kCols, false>), dim3(8 * per_xcd), dim3(256), 0, c->stream,
-                                                c->cptr, (const int2 *)c->crw, c->Q, c->ld, c->X, c->ldx, N1, nslab,
+                                                ptr, (const int2 *)rw, c->Q, c->ld, X, (int64_t)c->ld, N1, nslab,
                                                 nrb, per_xcd, flush));
     }
     return BM_OK;
 }
 
+template <int kCols>
+static int launch_gain(bm_ctx *c) {
+    return launch_gain_on<kCols>(c, c->cptr, c->crw, c->X, c->max_w1, c->nonneg);
+}
+
 static int launch_gain_cols(bm_ctx *c, int cols) {
-    return cols == 1 ? launch_gain<1>(c, c->N1) : cols == 2 ? launch_gain<2>(c, c->N1) : launch_gain<4>(c, c->N1);
+    return cols == 1 ? launch_gain<1>(c) : cols == 2 ? launch_gain<2>(c) : launch_gain<4>(c);
+}
+
+int bm::launch_list_gain(bm_ctx *c, const int32_t *ptr, const int32_t *rw, int32_t *X, int max_w) {
+    const int cols = c->gain_cols ? c->gain_cols : 2;
+    return cols == 1   ? launch_gain_on<1>(c, ptr, rw, X, max_w, true)
+           : cols == 2 ? launch_gain_on<2>(c, ptr, rw, X, max_w, true)
+                       : launch_gain_on<4>(c, ptr, rw, X, max_w, true);
 }
 
 // Slab-width autotuner: on the first sweep of a graph pair, time one launch

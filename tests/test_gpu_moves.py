@@ -150,3 +150,42 @@ def test_rot3_every_lane_width(lanes, monkeypatch):
     want = [o.best_g3(mp, *map(int, t), bmatch.BM_ROT3_ALL) for t in tri]
     np.testing.assert_array_equal(d, [w[0] for w in want])
     np.testing.assert_array_equal(p, [w[1] for w in want])
+
+
+@pytest.mark.parametrize("case", ["multigraph", "vertex0", "negative", "wide", "unmapped"])
+@pytest.mark.parametrize("table", [0, 1])
+def test_rot3_table_form_matches_oracle(case, table, monkeypatch):
+    # the table form (all-pairs pass over the dense-form lists + 3x3 blocks per triple) and the direct kernel,
+    # each forced, on every G1 feature the dense scan of triple.cu treats specially; both call paths (host
+    # list, resident list), both modes, and again after a move changed the mapping
+    monkeypatch.setenv("BM_ROT3_TABLE", str(table))
+    n = 45
+    kw = dict(zero_vertex=case == "vertex0", negative=case == "negative", wmax=40000 if case == "wide" else 40)
+    g1 = random_multigraph(70, n, 600, 0.15, 0.05, **kw)
+    g2 = random_multigraph(71, n, 600, 0.15, 0.05, wmax=kw["wmax"])
+    mp = random_mapping(72, n, 0.15 if case == "unmapped" else 0.0, bijective=case != "unmapped")
+    o = ob.Oracle(n, g1, g2)
+    rng = np.random.default_rng(73)
+    tri = np.array([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(5000)], np.int32)
+    with bmatch.Engine(0) as e:
+        e.upload(n, g1, g2)
+        e.set_mapping(mp)
+        got = {mode: e.eval_rot3(tri, mode) for mode in (bmatch.BM_ROT3_ROTATIONS, bmatch.BM_ROT3_ALL)}
+        e.moves_set(3, tri[:700])
+        e.moves_eval(bmatch.BM_ROT3_ALL)
+        res = e.moves_results(700)
+        e.apply_swap(int(tri[0][0]), int(tri[0][1]))
+        mp2 = e.mapping()
+        e.moves_eval(bmatch.BM_ROT3_ALL)
+        res2 = e.moves_results(700)
+    for mode, (d, p) in got.items():
+        want = [o.best_g3(mp, *map(int, t), mode) for t in tri]
+        np.testing.assert_array_equal(d, [w[0] for w in want], err_msg=f"mode {mode}")
+        np.testing.assert_array_equal(p, [w[1] for w in want], err_msg=f"mode {mode}")
+    want = [o.best_g3(mp, *map(int, t), bmatch.BM_ROT3_ALL) for t in tri[:700]]
+    np.testing.assert_array_equal(res[0], [w[0] for w in want])
+    np.testing.assert_array_equal(res[1], [w[1] for w in want])
+    want = [o.best_g3(mp2, *map(int, t), bmatch.BM_ROT3_ALL) for t in tri[:700]]
+    np.testing.assert_array_equal(res2[0], [w[0] for w in want])
+    np.testing.assert_array_equal(res2[1], [w[1] for w in want])
+    assert (got[bmatch.BM_ROT3_ALL][0] > 0).any()
