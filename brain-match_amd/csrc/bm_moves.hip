@@ -299,52 +299,80 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
     if (ni > kMaxIntra && tid == 0) atomicOr(overflow, 1);
     for (int q = tid; q < nint; q += kThreads) atomicAdd(&lbase, -mult * min(iw[q], PS[ix[q]][il[q]]));
     __syncthreads();
-    // k! permutations, contiguous rank ranges per thread
+    // k! permutations by lexicographic rank (the order of next_permutation,
+    // :342-363): rank = prefix * s! + suffix, where a prefix fixes the first
+    // k - s positions (s = min(k, 3)) and its s! suffixes are the orderings of
+    // the s values left, themselves in lexicographic order.  A thread takes
+    // whole prefixes (ranks ascending within the thread); a permutation is
+    // kept packed, 3 bits per position, so indexing it costs a shift.
     int total = 1;
     for (int i = 2; i <= k; i++) total *= i;
-    const int per = (total + kThreads - 1) / kThreads;
-    const int r0 = tid * per, r1 = min(total, r0 + per);
+    const int sl = k < 3 ? k : 3, sfact = sl == 3 ? 6 : 2;
+    const int nprefix = total / sfact, plen = k - sl;
     int bestd = 0, bestr = -1;
-    if (r0 < r1) {
-        int perm[8], pool[8];
-        for (int i = 0; i < k; i++) pool[i] = i;
-        int rem = r0, left = k;
-        for (int i = 0; i < k; i++) {
-            int f = 1;
-            for (int q = 2; q <= k - 1 - i; q++) f *= q;
-            const int idx = rem / f;
-            rem -= idx * f;
-            perm[i] = pool[idx];
-            for (int q = idx; q < left - 1; q++) pool[q] = pool[q + 1];
-            left--;
+    for (int t = tid; t < nprefix; t += kThreads) {
+        // Lehmer digits of the prefix: position i takes the d-th smallest unused value,
+        // d = (t / ((k-1-i)! / s!)) mod (k - i)
+        int rem = t, used = 0, pk = 0, psum = 0;
+        int f = nprefix;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (i >= plen) break;
+            f /= k - i;
+            int d = rem / f;
+            rem -= d * f;
+            int v = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (j >= k || ((used >> j) & 1)) continue;
+                if (d == 0) {
+                    v = j;
+                    d = -1;
+                } else if (d > 0) {
+                    d--;
+                }
+            }
+            used |= 1 << v;
+            pk |= v << (3 * i);
+            psum += C[i][v];
         }
-        // running sum of C[i][perm[i]]: next_permutation changes only the suffix from
-        // its pivot on (about e positions on average), which is re-read
-        int csum = 0;
-        for (int i = 0; i < k; i++) csum += C[i][perm[i]];
-        for (int rank = r0; rank < r1; rank++) {
-            int d = lbase + csum;
-            for (int q = 0; q < nint; q++) d += mult * min(iw[q], PS[perm[ix[q]]][perm[il[q]]]);
+        // the s values left, ascending
+        int rest[3] = {0, 0, 0}, nr = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (j < k && !((used >> j) & 1)) {
+                if (nr < 3) rest[nr] = j;
+                nr++;
+            }
+        const int base = t * sfact;
+        auto eval = [&](int u, int sum, int pku) {
+            int d = lbase + psum + sum;
+            for (int q = 0; q < nint; q++)
+                d += mult * min(iw[q], PS[(pku >> (3 * ix[q])) & 7][(pku >> (3 * il[q])) & 7]);
             if (d > bestd) {
                 bestd = d;
-                bestr = rank;
+                bestr = base + u;
             }
-            if (rank + 1 < r1) {  // next_permutation (:342-363) with the suffix re-summed
-                int pv = k - 2;
-                while (perm[pv] >= perm[pv + 1]) pv--;  // pv >= 0: this is not the last permutation
-                for (int i = pv; i < k; i++) csum -= C[i][perm[i]];
-                int j = k - 1;
-                while (perm[j] <= perm[pv]) j--;
-                int t = perm[pv];
-                perm[pv] = perm[j];
-                perm[j] = t;
-                for (int x = pv + 1, y = k - 1; x < y; x++, y--) {
-                    t = perm[x];
-                    perm[x] = perm[y];
-                    perm[y] = t;
-                }
-                for (int i = pv; i < k; i++) csum += C[i][perm[i]];
-            }
+        };
+        const int p0 = plen, p1 = plen + 1;
+        const int a = rest[0], b = rest[1];
+        if (sl == 2) {
+            eval(0, C[p0][a] + C[p1][b], pk | a << (3 * p0) | b << (3 * p1));
+            eval(1, C[p0][b] + C[p1][a], pk | b << (3 * p0) | a << (3 * p1));
+        } else {
+            const int p2 = plen + 2, cc = rest[2];
+            const int a0 = C[p0][a], b0 = C[p0][b], c0 = C[p0][cc];
+            const int a1 = C[p1][a], b1 = C[p1][b], c1 = C[p1][cc];
+            const int a2 = C[p2][a], b2 = C[p2][b], c2 = C[p2][cc];
+            const int sa = a << (3 * p0), sb = b << (3 * p0), sc = cc << (3 * p0);
+            const int ta = a << (3 * p1), tb = b << (3 * p1), tc = cc << (3 * p1);
+            const int ua = a << (3 * p2), ub = b << (3 * p2), uc = cc << (3 * p2);
+            eval(0, a0 + b1 + c2, pk | sa | tb | uc);
+            eval(1, a0 + c1 + b2, pk | sa | tc | ub);
+            eval(2, b0 + a1 + c2, pk | sb | ta | uc);
+            eval(3, b0 + c1 + a2, pk | sb | tc | ua);
+            eval(4, c0 + a1 + b2, pk | sc | ta | ub);
+            eval(5, c0 + b1 + a2, pk | sc | tb | ua);
         }
     }
     // block arg-max: larger delta, then smaller rank
