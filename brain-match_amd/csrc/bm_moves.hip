@@ -575,34 +575,46 @@ int bm::tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t 
 }
 
 // Direct kernel or table form for a triple list of m entries: the table form
-// pays one all-pairs pass per call and then 24 loads per triple, the direct
+// pays one all-pairs pass per call and then 9 loads per triple, the direct
 // kernel three loads per list entry, so long lists and large batches favour
-// the table.  Both are timed cold (the table with its pass) on the first
-// batch of each size class of an instance class; BM_ROT3_TABLE = 0 / 1
-// forces one.  The table needs (n+1) x ld x 14 bytes, so it is only offered up
-// to kTableMaxBytes.
+// the table.  On the first list of each size class of an instance class the
+// direct kernel, the pass and the table kernel are timed (the kernels on the
+// first kTuneSample triples, then scaled to m) and the cheaper total kept;
+// BM_ROT3_TABLE = 0 / 1 forces one.  The table needs (n+1) x ld x 14 bytes,
+// so it is only offered up to kTableMaxBytes.
 constexpr size_t kTableMaxBytes = (size_t)8 << 30;
+constexpr int64_t kTuneSample = 1 << 17;
 
 int bm::choose_rot3_table(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP, int *table) {
     *table = 0;
     int rc = BM_OK;
-    if (!c->rot3_tuned && m >= kTuneMinTriples && (rc = tune_rot3(c, dT, m, dD, dP))) return rc;
+    const int64_t ms = std::min(m, kTuneSample);
+    if (!c->rot3_tuned && m >= kTuneMinTriples && (rc = tune_rot3(c, dT, ms, dD, dP))) return rc;
     if ((size_t)c->N1 * c->ld * 14 > kTableMaxBytes) return BM_OK;
-    if (m < kTuneMinTriples) {
-        const char *f = getenv("BM_ROT3_TABLE");
+    const char *f = getenv("BM_ROT3_TABLE");
+    if (f || m < kTuneMinTriples) {
         *table = f && atoi(f) == 1;
         return BM_OK;
     }
-    static const int ch[2] = {0, 1};
     const auto key = std::make_tuple(4, log2c(c->n) * 64 + log2c(m), log2c(std::max<long long>(1, c->ndd / c->N1)), 3);
-    int v = 0;
-    rc = tune_variant(c, "BM_ROT3_TABLE", key, ch, 2, [&](int x) { v = x; }, [&] {
-        if (!v) return launch_rot3(c, dT, m, BM_ROT3_ALL, dD, dP);
-        int r = rot3_table_prepare(c);
-        return r ? r : launch_rot3_table(c, dT, m, BM_ROT3_ALL, dD, dP);
-    });
-    *table = v == 1;
-    return rc;
+    {
+        std::lock_guard<std::mutex> g(g_mv_mu);
+        auto it = g_mv_tune.find(key);
+        if (it != g_mv_tune.end()) {
+            *table = it->second;
+            return BM_OK;
+        }
+    }
+    float t_dir = 0.f, t_pass = 0.f, t_tab = 0.f;
+    if ((rc = time_launches(c, [&] { return launch_rot3(c, dT, ms, BM_ROT3_ALL, dD, dP); }, &t_dir)) ||
+        (rc = time_launches(c, [&] { return rot3_table_prepare(c); }, &t_pass)) ||
+        (rc = time_launches(c, [&] { return launch_rot3_table(c, dT, ms, BM_ROT3_ALL, dD, dP); }, &t_tab)))
+        return rc;
+    const double scale = (double)m / (double)ms;
+    *table = (double)t_pass + t_tab * scale < t_dir * scale;
+    std::lock_guard<std::mutex> g(g_mv_mu);
+    g_mv_tune[key] = *table;
+    return BM_OK;
 }
 
 int bm::launch_rot3_moves(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP, int table,
