@@ -340,6 +340,7 @@ def serve(args):
     lat = np.array([r["t_done"] - r["t_submit"] for r in res])
     wait = np.array([r["t_admit"] - r["t_submit"] for r in res])
     svc = np.array([r["t_done"] - r["t_start"] for r in res])
+    prep = np.array([r["t_start"] - r["t_admit"] for r in res])
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
     gain = sum(r["final_score"] - r["initial_score"] for r in res)
     out = _base("served refinement: jobs/s with p50/p99 time-to-result", jobs / span, "jobs/s", jobs,
@@ -348,7 +349,8 @@ def serve(args):
                              "shuffled start, batched refinement to convergence (budget 1e6 moves)",
                  "jobs": jobs, "arrival_rate": rate})
     out.update({"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-                "mean_queue_s": float(wait.mean()), "mean_service_s": float(svc.mean()),
+                "mean_queue_s": float(wait.mean()), "mean_prepare_s": float(prep.mean()),
+                "mean_service_s": float(svc.mean()),
                 "score_gain_per_s": gain / span, "moves": int(sum(r["moves"] for r in res)),
                 "steps": int(sum(r["steps"] for r in res)), "peak_active": st["peak_active"],
                 "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "hbm_budget_gb": st["hbm_budget"] / 1e9,
