@@ -43,7 +43,9 @@ using namespace bm;
 namespace {
 
 constexpr int kHist = 4096;  // per-step history ring
-constexpr int kXcolBlocks = 512;  // workgroups of the column recompute (grid-stride over U)
+constexpr int kXcolBlocks = 512;  // workgroups of the column recompute (grid-stride over its chunks)
+constexpr int kXcolLines = 256;   // G2 lines of a column's new image per chunk (at least)
+constexpr int kXcolSpare = 512;   // chunk slots beyond one per moved vertex
 
 struct RefineState {
     unsigned long long step;   // steps completed in this run
@@ -67,6 +69,8 @@ struct RefineState {
     int nu;       // moved vertices this step
     int npairs;   // their comb entries (prefix ubase)
     int full;     // this step began with a whole sweep
+    int xchunks;  // work items of the column recompute this step (prefix cbase)
+    int xlines;   // G2 lines per work item of the column recompute
 };
 
 struct MoveRec {
@@ -548,48 +552,112 @@ __global__ __launch_bounds__(256) void k_xdelta(const RefineState *__restrict__ 
     }
 }
 
-// Column v of X for each v in U, one workgroup per v: C_u for every row u,
-// plus, for each G2 line at the new image p'(v) (G2 in-neighbour g -> p'(v):
-// the G1 lines s -> u of s = p'^-1(g); out-neighbour p'(v) -> g: the lines
-// u -> s), f(w, that line's G2 weight), accumulated in an (n+1)-int LDS
-// column and written once.
+// Column v of X for each v in U: C_u for every row u, plus, for each G2
+// line at the new image p'(v) (G2 in-neighbour g -> p'(v): the G1 lines
+// s -> u of s = p'^-1(g); out-neighbour p'(v) -> g: the lines u -> s),
+// f(w, that line's G2 weight).  The G2 lines of each column are cut into
+// chunks (k_xcol_plan, one workgroup, after the move: at least kXcolLines
+// lines per chunk, more when the chunks would not fit the cap slots of the
+// scratch), so a hub image's column is spread over many workgroups.  k_xcol
+// accumulates a chunk in an (n+1)-int LDS column and stores it to its
+// scratch row; k_xcol_sum adds C_u and the column's chunk rows and writes
+// the column of X once.
+__global__ __launch_bounds__(1024) void k_xcol_plan(RefineState *__restrict__ st, const int32_t *__restrict__ ulist,
+                                                    const int32_t *__restrict__ map,
+                                                    const int32_t *__restrict__ g2optr,
+                                                    const int32_t *__restrict__ g2iptr, int32_t *__restrict__ cbase,
+                                                    int cap) {
+    __shared__ int wsum[16];
+    __shared__ long long all;
+    __shared__ int carry;
+    if (!st->inc) return;
+    const int nu = st->nu;
+    auto lines_of = [&](int i) {
+        const int g = map[ulist[i]];
+        return (g2iptr[g + 1] - g2iptr[g]) + (g2optr[g + 1] - g2optr[g]);
+    };
+    if (threadIdx.x == 0) all = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nu; i += 1024) atomicAdd((unsigned long long *)&all, (unsigned long long)lines_of(i));
+    __syncthreads();
+    // chunks <= total / L + nu <= cap
+    const long long room = max(1, cap - nu);
+    const int L = (int)max((long long)kXcolLines, (all + room - 1) / room);
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nu; c0 += 1024) {
+        const int i = c0 + threadIdx.x;
+        const int chunks = i < nu ? max(1, (lines_of(i) + L - 1) / L) : 0;
+        int total;
+        const int ex = block_scan_excl(chunks, wsum, total);
+        if (i < nu) cbase[i] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cbase[nu] = carry;
+        st->xchunks = carry;
+        st->xlines = L;
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_xcol(const RefineState *__restrict__ st, const int32_t *__restrict__ ulist,
-                                               const int32_t *__restrict__ map, const int32_t *__restrict__ cptr,
-                                               const int32_t *__restrict__ cmid, const int2 *__restrict__ crw,
-                                               const int32_t *__restrict__ cneg, const int32_t *__restrict__ g2optr,
+                                               const int32_t *__restrict__ cbase, const int32_t *__restrict__ map,
+                                               const int32_t *__restrict__ cptr, const int32_t *__restrict__ cmid,
+                                               const int2 *__restrict__ crw, const int32_t *__restrict__ g2optr,
                                                const uint32_t *__restrict__ g2oy, const int32_t *__restrict__ g2iptr,
-                                               const uint32_t *__restrict__ g2iy, int N1, int32_t *__restrict__ X,
-                                               int64_t ldx) {
+                                               const uint32_t *__restrict__ g2iy, int N1,
+                                               int32_t *__restrict__ scratch) {
     extern __shared__ __align__(16) int col[];
     if (!st->inc) return;
-    const int nu = st->nu, T = (int)(ldx >> 6);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int vi = blockIdx.x; vi < nu; vi += gridDim.x) {
-        const int v = ulist[vi], g0 = map[v];
-        for (int u = threadIdx.x; u < N1; u += 1024) col[u] = cneg[u];
+    const int nu = st->nu, items = st->xchunks, L = st->xlines;
+    // 16 lanes per G2 line, so a wave has four lines' list reads in flight
+    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    for (int it = blockIdx.x; it < items; it += gridDim.x) {
+        const int vi = find_base(cbase, nu, it);
+        const int g0 = map[ulist[vi]];
+        for (int u = threadIdx.x; u < N1; u += 1024) col[u] = 0;
         __syncthreads();
         const int i0 = g2iptr[g0], nin = g2iptr[g0 + 1] - i0;
         const int o0 = g2optr[g0], nout = g2optr[g0 + 1] - o0;
-        // one wave per G2 line, lanes over the preimage's G1 lines
-        for (int h = wave; h < nin + nout; h += 16) {
+        const int h0 = (it - cbase[vi]) * L, h1 = min(h0 + L, nin + nout);
+        for (int h = h0 + grp; h < h1; h += 64) {
             const bool in = h < nin;
             const uint32_t yv = in ? g2iy[i0 + h] : g2oy[o0 + h - nin];
             const int s = yv >> 16;
             if (s == 0xFFFF) continue;
             const int val = (int)(int16_t)(yv & 0xffff);
             const int l0 = in ? cptr[s] : cmid[s], l1 = in ? cmid[s] : cptr[s + 1];
-            for (int e = l0 + lane; e < l1; e += 256) {  // four entries in flight per lane
+            for (int e = l0 + gl; e < l1; e += 64) {  // four entries in flight per lane
                 int2 rw[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) rw[q] = e + 64 * q < l1 ? crw[e + 64 * q] : make_int2(-1, 0);
+                for (int q = 0; q < 4; q++) rw[q] = e + 16 * q < l1 ? crw[e + 16 * q] : make_int2(-1, 0);
 #pragma unroll
                 for (int q = 0; q < 4; q++)
                     if (rw[q].x >= 0) atomicAdd(&col[in ? rw[q].x - N1 : rw[q].x], min(rw[q].y, val) - min(rw[q].y, 0));
             }
         }
         __syncthreads();
-        for (int u = threadIdx.x; u < N1; u += 1024) X[xidx(u, v, T)] = col[u];
+        int32_t *row = scratch + (size_t)it * N1;
+        for (int u = threadIdx.x; u < N1; u += 1024) row[u] = col[u];
         __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_xcol_sum(const RefineState *__restrict__ st,
+                                                  const int32_t *__restrict__ ulist, const int32_t *__restrict__ cbase,
+                                                  const int32_t *__restrict__ cneg,
+                                                  const int32_t *__restrict__ scratch, int N1,
+                                                  int32_t *__restrict__ X, int64_t ldx) {
+    if (!st->inc) return;
+    const int nu = st->nu, T = (int)(ldx >> 6);
+    const int64_t total = (int64_t)nu * N1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int vi = (int)(i / N1), u = (int)(i - (int64_t)vi * N1);
+        int x = cneg[u];
+        for (int it = cbase[vi]; it < cbase[vi + 1]; it++) x += scratch[(size_t)it * N1 + u];
+        X[xidx(u, ulist[vi], T)] = x;
     }
 }
 
@@ -620,7 +688,9 @@ struct bm_batch_state {
     unsigned long long *ckey = nullptr, *vmax = nullptr;
     uint32_t *blocked = nullptr;
     int32_t *win = nullptr;
-    int32_t *ulist = nullptr, *uold = nullptr, *ubase = nullptr;
+    int32_t *ulist = nullptr, *uold = nullptr, *ubase = nullptr, *cbase = nullptr;
+    int32_t *xscratch = nullptr;  // chunk rows of the column recompute, xcap x (n+1)
+    int xcap = 0;
     uint8_t *inU = nullptr;
     bool incremental = false;  // the captured step updates X incrementally
     uint64_t x_epoch = ~0ull;  // map_epoch at the end of the last run (X then matched the mapping)
@@ -678,7 +748,7 @@ int bm::free_batch(bm_ctx *c) {
     };
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
     fr(b->ckey), fr(b->vmax), fr(b->blocked), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
-    fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->inU);
+    fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU);
     if (b->hst) (void)hipHostFree(b->hst);
     if (b->done) (void)hipEventDestroy(b->done);
     c->dev_bytes -= (int64_t)b->bytes;
@@ -771,9 +841,14 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta, dim3(ublocks), dim3(256), 0, c->stream, b->st, b->ulist,
                                                   b->uold, b->ubase, b->inU, c->map, c->cptr, (const int2 *)c->crw,
                                                   c->g2optr, c->g2oy, c->g2iptr, c->g2iy, N1, c->X, c->ldx));
+        BM_LAUNCH(c, "xcol_plan", hipLaunchKernelGGL(k_xcol_plan, dim3(1), dim3(1024), 0, c->stream, b->st, b->ulist,
+                                                     c->map, c->g2optr, c->g2iptr, b->cbase, b->xcap));
         BM_LAUNCH(c, "xcol", hipLaunchKernelGGL(k_xcol, dim3(kXcolBlocks), dim3(1024), (size_t)N1 * 4, c->stream,
-                                                b->st, b->ulist, c->map, c->cptr, c->cmid, (const int2 *)c->crw,
-                                                c->cneg, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, N1, c->X, c->ldx));
+                                                b->st, b->ulist, b->cbase, c->map, c->cptr, c->cmid,
+                                                (const int2 *)c->crw, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, N1,
+                                                b->xscratch));
+        BM_LAUNCH(c, "xcol_sum", hipLaunchKernelGGL(k_xcol_sum, dim3(512), dim3(256), 0, c->stream, b->st, b->ulist,
+                                                    b->cbase, c->cneg, b->xscratch, N1, c->X, c->ldx));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, 1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
                                                     c->Q, c->ld, N1, c->X, c->ldx));
@@ -887,10 +962,14 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->ulist, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->uold, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->ubase, (size_t)(N1 + 1) * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->cbase, (size_t)(N1 + 1) * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->inU, (size_t)(N1 + 3) / 4 * 4))) return fail(rc);
     // steps moving at most inc_max vertices update X in place (about 3 |U| / n of a sweep)
     b->inc_max = std::max(64, c->N1 / 8);
     if (const char *e = getenv("BM_REFINE_INC_MAX")) b->inc_max = atoi(e);
+    // chunk rows of the column recompute: one per moved vertex plus kXcolSpare
+    b->xcap = std::min(std::max(b->inc_max, 0), N1) + kXcolSpare;
+    if ((rc = balloc(c, (void **)&b->xscratch, (size_t)b->xcap * N1 * 4))) return fail(rc);
     // The sweep walks about sum_v deg1(v) deg2(p(v)) list words with LDS atomics; the
     // update, with global atomics, may take a quarter of that (measured on the served
     // workload: 1/2 to 1/4 are equally fast, 1/32 falls back to whole sweeps too often;

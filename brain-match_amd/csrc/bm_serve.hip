@@ -82,7 +82,8 @@ extern "C" int64_t bm_job_footprint(int32_t n, int64_t e1, int64_t e2, const bm_
                 + 2 * e1 * 36       // comb lists (r, w, dw, (r,w)), de-duplicated view
                 + e1 * 16           // flat out-edge list
                 + e2 * 28           // G2 lists, translated lists, upload staging
-                + N1 * 64 + (N1 + 8) * 8;
+                + N1 * 64 + (N1 + 8) * 8
+                + (std::max<int64_t>(64, N1 / 8) + 512) * N1 * 4;  // column-recompute scratch of the step
     if (p) {
         const int64_t M = N1 + p->triples + p->sets;
         b += (int64_t)p->triples * 20 + (int64_t)p->sets * (4 * std::max(p->set_k, 1) + 8) + M * 8 +
