@@ -812,6 +812,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
 
     // ---- G2: last line per ordered pair
     std::vector<int32_t> tri;
+    bool g2nonneg = true;
     {
         std::vector<int32_t> p2(N1 + 1, 0);
         for (int64_t i = 0; i < e2; i++) p2[s2[i] + 1]++;
@@ -831,10 +832,11 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
                     tri.push_back(u);
                     tri.push_back(od[e]);
                     tri.push_back(ow[e]);
-                    if ((int16_t)ow[e] < 0) nonneg = false;
+                    if ((int16_t)ow[e] < 0) g2nonneg = false;
                 }
     }
-    c->nonneg = nonneg;
+    c->g2nonneg = g2nonneg;
+    c->nonneg = nonneg && g2nonneg;
     c->max_w1 = maxw;
 
     // ---- G2 out- and in-lists of the de-duplicated lines, packed (vertex << 16 | weight)

@@ -56,6 +56,7 @@ struct bm_ctx {
     bool mapped = false;
     bool diag_valid = false;
     bool nonneg = false;  // all weights >= 0 (packed-accumulate fast paths)
+    bool g2nonneg = false;  // every dense G2 cell >= 0 as int16 (so is every entry of P)
     int32_t max_w1 = 0;
 
     // device buffers
@@ -179,6 +180,7 @@ int ensure_dense_lists(bm_ctx *c);
 int launch_list_diag(bm_ctx *c, int32_t *out, const int32_t *ptr, const int32_t *r, const int32_t *w);
 // X[u][v] = sum over the list of u of min(w, Q[r][v]) for all u, v (tile-major,
 // stride ld), lists of non-negative weights <= max_w given as (r, w) pairs
+// (the packed partial sums also need P >= 0: c->g2nonneg)
 int launch_list_gain(bm_ctx *c, const int32_t *ptr, const int32_t *rw, int32_t *X, int max_w);
 int launch_check(const char *what);
 

@@ -526,9 +526,10 @@ static int launch_gain_cols(bm_ctx *c, int cols) {
 
 int bm::launch_list_gain(bm_ctx *c, const int32_t *ptr, const int32_t *rw, int32_t *X, int max_w) {
     const int cols = c->gain_cols ? c->gain_cols : 2;
-    return cols == 1   ? launch_gain_on<1>(c, ptr, rw, X, max_w, true)
-           : cols == 2 ? launch_gain_on<2>(c, ptr, rw, X, max_w, true)
-                       : launch_gain_on<4>(c, ptr, rw, X, max_w, true);
+    const bool nonneg = c->g2nonneg;  // min(w, P) >= 0 needs P >= 0 as well
+    return cols == 1   ? launch_gain_on<1>(c, ptr, rw, X, max_w, nonneg)
+           : cols == 2 ? launch_gain_on<2>(c, ptr, rw, X, max_w, nonneg)
+                       : launch_gain_on<4>(c, ptr, rw, X, max_w, nonneg);
 }
 
 // Slab-width autotuner: on the first sweep of a graph pair, time one launch

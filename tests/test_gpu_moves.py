@@ -152,7 +152,7 @@ def test_rot3_every_lane_width(lanes, monkeypatch):
     np.testing.assert_array_equal(p, [w[1] for w in want])
 
 
-@pytest.mark.parametrize("case", ["multigraph", "vertex0", "negative", "wide", "unmapped"])
+@pytest.mark.parametrize("case", ["multigraph", "vertex0", "negative", "g2negative", "wide", "unmapped"])
 @pytest.mark.parametrize("table", [0, 1])
 def test_rot3_table_form_matches_oracle(case, table, monkeypatch):
     # the table form (all-pairs pass over the dense-form lists + 3x3 blocks per triple) and the direct kernel,
@@ -162,7 +162,9 @@ def test_rot3_table_form_matches_oracle(case, table, monkeypatch):
     n = 45
     kw = dict(zero_vertex=case == "vertex0", negative=case == "negative", wmax=40000 if case == "wide" else 40)
     g1 = random_multigraph(70, n, 600, 0.15, 0.05, **kw)
-    g2 = random_multigraph(71, n, 600, 0.15, 0.05, wmax=kw["wmax"])
+    # g2negative: small positive G1 weights (the packed partial sums of the all-pairs pass would apply)
+    # against negative G2 cells
+    g2 = random_multigraph(71, n, 600, 0.15, 0.05, wmax=kw["wmax"], negative=case == "g2negative")
     mp = random_mapping(72, n, 0.15 if case == "unmapped" else 0.0, bijective=case != "unmapped")
     o = ob.Oracle(n, g1, g2)
     rng = np.random.default_rng(73)
