@@ -837,6 +837,10 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     }
     c->g2nonneg = g2nonneg;
     c->nonneg = nonneg && g2nonneg;
+    // every line its own dense cell: no self-loop, no repeated pair, no vertex 0, weights in [1, 32767]
+    c->dense_equal = c->ndd == c->ncomb;
+    for (int64_t i = 0; i < e1 && c->dense_equal; i++)
+        if (s1[i] == d1[i] || s1[i] == 0 || d1[i] == 0 || w1[i] < 1 || w1[i] > 32767) c->dense_equal = false;
     c->max_w1 = maxw;
 
     // ---- G2 out- and in-lists of the de-duplicated lines, packed (vertex << 16 | weight)

@@ -57,6 +57,7 @@ struct bm_ctx {
     bool diag_valid = false;
     bool nonneg = false;  // all weights >= 0 (packed-accumulate fast paths)
     bool g2nonneg = false;  // every dense G2 cell >= 0 as int16 (so is every entry of P)
+    bool dense_equal = false;  // every G1 line is its own dense cell: comb lists = dense-form lists
     int32_t max_w1 = 0;
 
     // device buffers
@@ -198,6 +199,10 @@ int tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP)
 int choose_rot3_table(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP, int *table);
 int launch_rot3_moves(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP, int table,
                       bool prepare);
+// the dense-form lists and G1 cell matrix; Delta_G3 from the current X (needs
+// c->dense_equal, the cell matrix and X matching the mapping)
+int ensure_df_cells(bm_ctx *c);
+int launch_rot3_x(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP);
 int tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult);
 int free_batch(bm_ctx *c);
 // P and P^T from W2 and the device mapping; with patch_flag, when *patch_flag

@@ -207,6 +207,71 @@ __global__ __launch_bounds__(256) void k_table_pack(int2 *__restrict__ XP, const
     *reinterpret_cast<int4 *>(XP + at) = out;
 }
 
+// The table form inside the batched step, on the step's own all-pairs gain
+// matrix X (tile-major, kept current by the step).  When every G1 line is
+// its own dense cell (no self-loop, no repeated pair, no line at vertex 0,
+// weights in [1, 32767]), the comb lists are the dense-form lists and X = Xd,
+// except that X[min(a,b)][max(a,b)] also holds Corr(a,b) of the pair sweep
+// (pair_corr over the lines between a and b, each its own dense cell); that
+// is taken out again for the triple's pairs.
+__global__ __launch_bounds__(256) void k_rot3_x(const int32_t *__restrict__ T, int64_t count, int mode,
+                                                int32_t *__restrict__ best_d, int32_t *__restrict__ best_p,
+                                                const int32_t *__restrict__ X, int tiles,
+                                                const int16_t *__restrict__ Q, const int16_t *__restrict__ W1,
+                                                int ld) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int m[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) m[i] = T[3 * t + i];
+    int p[3][3], x[3][3], c[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            p[i][j] = qv(Q, m[i], ld, m[j]);
+            x[i][j] = X[xidx(m[i], m[j], tiles)];
+            c[i][j] = i == j ? 0 : (int)W1[(size_t)m[i] * ld + m[j]];
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (i == j || m[i] > m[j]) continue;
+            // Corr(m_i, m_j) from m_i's entries: the line m_i -> m_j (out) and m_j -> m_i (in)
+            int corr = 0;
+            if (c[i][j] > 0) corr += pair_corr(true, c[i][j], c[i][j], p[i][i], p[j][j], p[i][j], p[j][i]);
+            if (c[j][i] > 0) corr += pair_corr(false, c[j][i], c[j][i], p[i][i], p[j][j], p[i][j], p[j][i]);
+            x[i][j] -= corr;
+        }
+    int bd = 0, bp = 0;
+#pragma unroll
+    for (int q = 1; q < 6; q++) {
+        if (mode == BM_ROT3_ROTATIONS && q != 3 && q != 4) continue;
+        int d = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) d += x[k][kG3Perm[q][k]] - x[k][k];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int w = c[i][j];
+                if (i == j || w <= 0) continue;
+                const int si = kG3Perm[q][i], sj = kG3Perm[q][j];
+                const int old = min(w, p[i][j]);
+                d -= min(w, p[si][j]) - old;
+                d -= min(w, p[i][sj]) - old;
+                d += min(w, p[si][sj]) - old;
+            }
+        if (d > bd) {
+            bd = d;
+            bp = q;
+        }
+    }
+    best_d[t] = bd;
+    best_p[t] = bp;
+}
+
 // dfW1[v][d] = the dense-form cell of the out entry v -> d (0 elsewhere).
 __global__ __launch_bounds__(256) void k_df_cells(int16_t *__restrict__ W1, const int32_t *__restrict__ ptr,
                                                   const int32_t *__restrict__ mid, const int2 *__restrict__ rw,
@@ -446,25 +511,37 @@ int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *
 // the dense-form lists, made for every evaluation call since the mapping may
 // have changed), then one thread per triple.  The G1 cell matrix is built once
 // per graph pair.
-static int rot3_table_prepare(bm_ctx *c) {
+// The dense-form lists and their G1 cell matrix (once per graph pair).
+int bm::ensure_df_cells(bm_ctx *c) {
     int rc = ensure_dense_lists(c);
+    if (rc || c->dfW1) return rc;
+    const size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
+    if ((rc = dev_alloc(c, (void **)&c->dfW1, N1 * ld * 2))) return rc;
+    const int64_t cells = (int64_t)(N1 * ld);
+    BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_zero16, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0,
+                                                c->stream, c->dfW1, cells));
+    BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_df_cells, dim3((unsigned)((N1 + 3) / 4)), dim3(256), 0, c->stream,
+                                                c->dfW1, c->dfptr, c->dfmid, (const int2 *)c->dfrw, c->N1, c->ld));
+    return BM_OK;
+}
+
+static int rot3_table_prepare(bm_ctx *c) {
+    int rc = ensure_df_cells(c);
     if (rc) return rc;
     const size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
     if (!c->dfX && (rc = dev_alloc(c, (void **)&c->dfX, (N1 + 63) / 64 * 64 * ld * 4))) return rc;
     if (!c->dfXP && (rc = dev_alloc(c, (void **)&c->dfXP, N1 * ld * 8))) return rc;
-    if (!c->dfW1) {
-        if ((rc = dev_alloc(c, (void **)&c->dfW1, N1 * ld * 2))) return rc;
-        const int64_t cells = (int64_t)(N1 * ld);
-        BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_zero16, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0,
-                                                    c->stream, c->dfW1, cells));
-        BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_df_cells, dim3((unsigned)((N1 + 3) / 4)), dim3(256), 0,
-                                                    c->stream, c->dfW1, c->dfptr, c->dfmid, (const int2 *)c->dfrw,
-                                                    c->N1, c->ld));
-    }
     if ((rc = launch_list_gain(c, c->dfptr, c->dfrw, c->dfX, c->dfmaxw))) return rc;
     const int64_t pairs = (int64_t)N1 * (int64_t)(ld / 2);
     BM_LAUNCH(c, "table_pack", hipLaunchKernelGGL(k_table_pack, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0,
                                                   c->stream, (int2 *)c->dfXP, c->dfX, c->Q, c->dfW1, c->N1, c->ld));
+    return BM_OK;
+}
+
+int bm::launch_rot3_x(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP) {
+    if (m <= 0) return BM_OK;
+    BM_LAUNCH(c, "rot3_x", hipLaunchKernelGGL(k_rot3_x, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
+                                              dT, m, mode, dD, dP, c->X, (int)(c->ldx >> 6), c->Q, c->dfW1, c->ld));
     return BM_OK;
 }
 

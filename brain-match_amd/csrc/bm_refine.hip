@@ -693,6 +693,7 @@ struct bm_batch_state {
     int xcap = 0;
     uint8_t *inU = nullptr;
     bool incremental = false;  // the captured step updates X incrementally
+    bool rot3_x = false;       // triples scored from the step's X (k_rot3_x) instead of their lists
     uint64_t x_epoch = ~0ull;  // map_epoch at the end of the last run (X then matched the mapping)
     int inc_max = 0;
     int pinc_max = 0;
@@ -800,7 +801,9 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "gen_triples",
                   hipLaunchKernelGGL(k_gen_triples, dim3((p.triples + 255) / 256), dim3(256), 0, c->stream, b->tri,
                                      p.triples, b->st, (unsigned long long)p.seed, c->cptr, c->cr, N1, c->n));
-        if ((rc = launch_rot3(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d, b->tri_p))) return rc;
+        rc = b->rot3_x ? launch_rot3_x(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d, b->tri_p)
+                       : launch_rot3(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d, b->tri_p);
+        if (rc) return rc;
     }
     if (p.sets > 0) {
         BM_LAUNCH(c, "gen_sets",
@@ -903,7 +906,7 @@ static int autotune(bm_ctx *c) {
     bm_batch_state *b = c->batch;
     const bm_refine_params &p = b->prm;
     int rc;
-    if (p.triples > 0) {
+    if (p.triples > 0 && !b->rot3_x) {
         BM_LAUNCH(c, "gen_triples",
                   hipLaunchKernelGGL(k_gen_triples, dim3((p.triples + 255) / 256), dim3(256), 0, c->stream, b->tri,
                                      p.triples, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1, c->n));
@@ -1000,6 +1003,16 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
         return fail(set_error(BM_EHIP, "hipMemsetAsync failed"));
     if (p.pairs && (rc = sweep_prepare(c))) return fail(rc);
     if (c->sparse_ok && (rc = launch_refresh_g2y(c))) return fail(rc);
+    // With pair moves the step keeps X current before the triples are scored; when every G1 line
+    // is its own dense cell, the triples read their deltas from it (BM_REFINE_ROT3=direct: lists)
+    b->rot3_x = false;
+    if (p.triples > 0 && p.pairs && c->dense_equal) {
+        const char *e = getenv("BM_REFINE_ROT3");
+        if (!e || strcmp(e, "direct") != 0) {
+            if ((rc = ensure_df_cells(c))) return fail(rc);
+            b->rot3_x = true;
+        }
+    }
     if ((rc = autotune(c))) return fail(rc);
     if (p.pairs && c->sweep_kind == 2 && c->sparse_ok && !c->sparse_attr) {
         // first sparse launch outside the capture sets the kernel's LDS attribute

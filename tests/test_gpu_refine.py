@@ -151,8 +151,13 @@ def test_deterministic_and_variant_independent(monkeypatch):
     inst, o = instance(31, 180, 2000)
     prm = bmatch.refine_params(pairs=1, triples=2000, sets=300, set_k=5, rounds=2, steps_per_check=2, seed=77)
     results = []
-    for lanes, threads, kernel in [(None, None, None), (8, 64, "dense"), (32, 256, "sparse"), (16, 64, "sparse")]:
-        for name, val in (("BM_ROT3_LANES", lanes), ("BM_PERMSET_THREADS", threads), ("BM_SWEEP_KERNEL", kernel)):
+    # triples from the lists (direct, each lane width) or from the step's X (the default on this pair,
+    # whose lines are all their own dense cells), with either sweep kernel
+    for lanes, threads, kernel, rot3 in [(None, None, None, None), (8, 64, "dense", "direct"),
+                                         (32, 256, "sparse", "direct"), (16, 64, "sparse", "direct"),
+                                         (None, 64, "dense", None), (None, 256, "sparse", None)]:
+        for name, val in (("BM_ROT3_LANES", lanes), ("BM_PERMSET_THREADS", threads), ("BM_SWEEP_KERNEL", kernel),
+                          ("BM_REFINE_ROT3", rot3)):
             if val is None:
                 monkeypatch.delenv(name, raising=False)
             else:
@@ -162,6 +167,7 @@ def test_deterministic_and_variant_independent(monkeypatch):
             st, recs, hist = run(e, inst.p0, prm, max_steps=8, patience=8)
             if lanes:
                 assert e.refine_config() == (lanes, threads)
+            if kernel:
                 assert e.sweep_config()[0] == (1 if kernel == "dense" else 2)
             results.append((st["final_score"], st["moves"], recs.tobytes(), e.mapping().tobytes()))
     assert all(r == results[0] for r in results[1:])
