@@ -432,8 +432,9 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
                                                   const int32_t *__restrict__ g2optr,
                                                   const int32_t *__restrict__ g2iptr, int32_t *__restrict__ ulist,
                                                   int32_t *__restrict__ uold, int32_t *__restrict__ ubase,
-                                                  uint8_t *__restrict__ inU, RefineState *__restrict__ st,
-                                                  int inc_max, int pinc_max, long long inc_budget, int dbar) {
+                                                  uint8_t *__restrict__ inU, int32_t *__restrict__ uoldv,
+                                                  RefineState *__restrict__ st, int inc_max, int pinc_max,
+                                                  long long inc_budget, int dbar) {
     __shared__ int cnt, carry;
     __shared__ int wsum[16];
     __shared__ unsigned long long work;
@@ -453,6 +454,7 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
             const int g = map[v[q]];
             ulist[b + q] = v[q];
             uold[b + q] = g;
+            uoldv[v[q]] = g;
             inU[v[q]] = 1;
             const unsigned long long d2 = (g2iptr[g + 1] - g2iptr[g]) + (g2optr[g + 1] - g2optr[g]);
             wk += (unsigned long long)(cptr[v[q] + 1] - cptr[v[q]]) * 2 * d2 + d2 * (unsigned long long)dbar;
@@ -549,6 +551,81 @@ __global__ __launch_bounds__(256) void k_xdelta(const RefineState *__restrict__ 
                                   sg * (min(w, (int)(int16_t)(yv[q] & 0xffff)) - wneg));
             }
         }
+    }
+}
+
+// The same row updates as k_xdelta, gathered per row: the workgroup of row u
+// finds the entries of comb(u) whose neighbour x moved (an in entry x -> u
+// reads the G2 out-list of x's image, an out entry u -> x its in-list), adds
+// f over the lists at the new image and subtracts it over the old one in an
+// (n+1)-int LDS row (columns outside U only), then adds the row's non-zero
+// entries to X with plain read-modify-writes: one workgroup owns the row, so
+// the scattered global atomics of the per-entry form become coalesced
+// accesses along the row.  Rows without a moved neighbour return after the
+// scan of their list.
+__global__ __launch_bounds__(1024) void k_xdelta_rows(const RefineState *__restrict__ st,
+                                                      const uint8_t *__restrict__ inU,
+                                                      const int32_t *__restrict__ uoldv,
+                                                      const int32_t *__restrict__ map,
+                                                      const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                      const int32_t *__restrict__ g2optr,
+                                                      const uint32_t *__restrict__ g2oy,
+                                                      const int32_t *__restrict__ g2iptr,
+                                                      const uint32_t *__restrict__ g2iy, int N1,
+                                                      int32_t *__restrict__ X, int64_t ldx) {
+    extern __shared__ __align__(16) int row[];
+    __shared__ int any;
+    if (!st->inc) return;
+    const int T = (int)(ldx >> 6);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int u = blockIdx.x; u < N1; u += gridDim.x) {
+        if (threadIdx.x == 0) any = 0;
+        __syncthreads();
+        const int e0 = cptr[u], e1 = cptr[u + 1];
+        for (int e = e0 + threadIdx.x; e < e1; e += 1024) {
+            const int r = crw[e].x;
+            const int x = r >= N1 ? r - N1 : r;
+            if (inU[x] && uoldv[x] != map[x]) any = 1;
+        }
+        __syncthreads();
+        if (!any) continue;  // uniform
+        for (int y = threadIdx.x; y < N1; y += 1024) row[y] = 0;
+        __syncthreads();
+        for (int e = e0 + wave; e < e1; e += 16) {  // one wave per entry
+            const int2 rw = crw[e];
+            const bool out = rw.x >= N1;
+            const int x = out ? rw.x - N1 : rw.x;
+            if (!inU[x]) continue;
+            const int gold = uoldv[x], gnew = map[x];
+            if (gold == gnew) continue;
+            const int w = rw.y, wneg = min(w, 0);
+            const int32_t *ptr = out ? g2iptr : g2optr;
+            const uint32_t *L = out ? g2iy : g2oy;
+            for (int side = 0; side < 2; side++) {
+                const int g = side ? gold : gnew, sg = side ? -1 : 1;
+                const int j1 = ptr[g + 1];
+                for (int j = ptr[g] + lane; j < j1; j += 256) {  // four list words in flight per lane
+                    uint32_t yv[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) yv[q] = j + 64 * q < j1 ? L[j + 64 * q] : 0xFFFF0000u;
+                    uint8_t skip[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) skip[q] = (yv[q] >> 16) == 0xFFFF ? 1 : inU[yv[q] >> 16];
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        if (!skip[q]) atomicAdd(&row[yv[q] >> 16], sg * (min(w, (int)(int16_t)(yv[q] & 0xffff)) - wneg));
+                }
+            }
+        }
+        __syncthreads();
+        for (int y = threadIdx.x; y < N1; y += 1024) {
+            const int d = row[y];
+            if (d) {
+                int32_t *px = X + xidx(u, y, T);
+                *px += d;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -694,6 +771,8 @@ struct bm_batch_state {
     uint8_t *inU = nullptr;
     bool incremental = false;  // the captured step updates X incrementally
     bool rot3_x = false;       // triples scored from the step's X (k_rot3_x) instead of their lists
+    bool xdelta_rows = true;   // row updates gathered per row (k_xdelta_rows) instead of per entry
+    int32_t *uoldv = nullptr;  // image before the step, indexed by moved vertex
     uint64_t x_epoch = ~0ull;  // map_epoch at the end of the last run (X then matched the mapping)
     int inc_max = 0;
     int pinc_max = 0;
@@ -749,7 +828,7 @@ int bm::free_batch(bm_ctx *c) {
     };
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
     fr(b->ckey), fr(b->vmax), fr(b->blocked), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
-    fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU);
+    fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU), fr(b->uoldv);
     if (b->hst) (void)hipHostFree(b->hst);
     if (b->done) (void)hipEventDestroy(b->done);
     c->dev_bytes -= (int64_t)b->bytes;
@@ -826,7 +905,7 @@ static int enqueue_step(bm_ctx *c) {
     if (inc) {
         BM_LAUNCH(c, "collect", hipLaunchKernelGGL(k_collect, dim3(1), dim3(1024), 0, c->stream, b->win, cv, c->map,
                                                    c->cptr, c->g2optr, c->g2iptr, b->ulist, b->uold, b->ubase, b->inU,
-                                                   b->st, b->inc_max, b->pinc_max, b->inc_budget,
+                                                   b->uoldv, b->st, b->inc_max, b->pinc_max, b->inc_budget,
                                                    (int)(c->ncomb / std::max(1, c->N1))));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, -1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
@@ -841,9 +920,16 @@ static int enqueue_step(bm_ctx *c) {
         if ((rc = launch_refresh_g2y(c))) return rc;
     }
     if (inc) {
-        BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta, dim3(ublocks), dim3(256), 0, c->stream, b->st, b->ulist,
-                                                  b->uold, b->ubase, b->inU, c->map, c->cptr, (const int2 *)c->crw,
-                                                  c->g2optr, c->g2oy, c->g2iptr, c->g2iy, N1, c->X, c->ldx));
+        if (b->xdelta_rows)
+            BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta_rows, dim3((unsigned)std::min(N1, 2048)), dim3(1024),
+                                                      (size_t)N1 * 4, c->stream, b->st, b->inU, b->uoldv, c->map,
+                                                      c->cptr, (const int2 *)c->crw, c->g2optr, c->g2oy, c->g2iptr,
+                                                      c->g2iy, N1, c->X, c->ldx));
+        else
+            BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta, dim3(ublocks), dim3(256), 0, c->stream, b->st,
+                                                      b->ulist, b->uold, b->ubase, b->inU, c->map, c->cptr,
+                                                      (const int2 *)c->crw, c->g2optr, c->g2oy, c->g2iptr, c->g2iy,
+                                                      N1, c->X, c->ldx));
         BM_LAUNCH(c, "xcol_plan", hipLaunchKernelGGL(k_xcol_plan, dim3(1), dim3(1024), 0, c->stream, b->st, b->ulist,
                                                      c->map, c->g2optr, c->g2iptr, b->cbase, b->xcap));
         BM_LAUNCH(c, "xcol", hipLaunchKernelGGL(k_xcol, dim3(kXcolBlocks), dim3(1024), (size_t)N1 * 4, c->stream,
@@ -966,6 +1052,7 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->uold, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->ubase, (size_t)(N1 + 1) * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->cbase, (size_t)(N1 + 1) * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->uoldv, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->inU, (size_t)(N1 + 3) / 4 * 4))) return fail(rc);
     // steps moving at most inc_max vertices update X in place (about 3 |U| / n of a sweep)
     b->inc_max = std::max(64, c->N1 / 8);
@@ -1019,8 +1106,12 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
         if ((rc = sweep_enqueue(c, nullptr))) return fail(rc);
     }
     if (hipFuncSetAttribute((const void *)k_xcol, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) !=
-        hipSuccess)
+            hipSuccess ||
+        hipFuncSetAttribute((const void *)k_xdelta_rows, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) !=
+            hipSuccess)
         return fail(set_error(BM_EHIP, "hipFuncSetAttribute failed"));
+    // BM_REFINE_XDELTA=entries: the per-entry form with global atomics
+    b->xdelta_rows = !(getenv("BM_REFINE_XDELTA") && strcmp(getenv("BM_REFINE_XDELTA"), "entries") == 0);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(BM_EHIP, "configure sync failed"));
     if ((rc = capture(c))) return fail(rc);
     b->configured = true;
