@@ -24,7 +24,8 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
             500..20,000 and 160 lines per vertex, 30% shuffled starts, each
             refined by the batched mode to convergence (budget 10^6 moves) on
             the serving loop; jobs/s and p50/p99 time-to-result (BM_SERVE_ACTIVE
-            sets the server's active slots, default 8)
+            sets the server's active slots, default 8; BM_SERVE_PREP its
+            preparation threads, default 4)
 """
 import os
 import time
@@ -316,7 +317,8 @@ def serve(args):
                                                                shuffle_frac=0.3), range(jobs)))
     gen_s = time.perf_counter() - t
     active = int(os.environ.get("BM_SERVE_ACTIVE", "8"))
-    with bmatch.Server(max_active=active, prep_threads=4, hbm_budget=0) as srv:
+    prep = int(os.environ.get("BM_SERVE_PREP", "4"))
+    with bmatch.Server(max_active=active, prep_threads=prep, hbm_budget=0) as srv:
         ids = [None] * jobs
 
         def client():

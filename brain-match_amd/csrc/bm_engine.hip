@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "bm_internal.h"
@@ -722,6 +723,66 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     c->ld = (N1 + 63) / 64 * 64;
     c->e1 = e1;
 
+    // ---- G2 (on a second host thread while this one builds the G1 lists):
+    // last line per ordered pair, then its out- and in-lists packed (vertex << 16 | weight)
+    std::vector<int32_t> tri;
+    bool g2nonneg = true, g2_vertex0 = false;
+    std::vector<int32_t> g2optr(N1 + 1, 0), g2iptr(N1 + 1, 0);
+    std::vector<uint32_t> g2o, g2i;
+    std::vector<int32_t> g2o_at_in, g2i_at_out;
+    int64_t m2 = 0;
+    auto build_g2 = [&]() {
+        std::vector<int32_t> p2(N1 + 1, 0), seen(N1, -1);
+        for (int64_t i = 0; i < e2; i++) p2[s2[i] + 1]++;
+        for (int v = 0; v < N1; v++) p2[v + 1] += p2[v];
+        std::vector<int32_t> pos(p2.begin(), p2.end() - 1), od(e2), ow(e2);
+        for (int64_t i = 0; i < e2; i++) {
+            int p = pos[s2[i]]++;
+            od[p] = d2[i];
+            ow[p] = w2[i];
+        }
+        tri.reserve(3 * (size_t)e2);
+        for (int u = 0; u < N1; u++)
+            for (int e = p2[u + 1] - 1; e >= p2[u]; e--)
+                if (seen[od[e]] != u) {
+                    seen[od[e]] = u;
+                    tri.push_back(u);
+                    tri.push_back(od[e]);
+                    tri.push_back(ow[e]);
+                    if ((int16_t)ow[e] < 0) g2nonneg = false;
+                }
+        m2 = (int64_t)tri.size() / 3;
+        for (int64_t i = 0; i < m2; i++) {
+            g2optr[tri[3 * i] + 1]++;
+            g2iptr[tri[3 * i + 1] + 1]++;
+            if (tri[3 * i] == 0 || tri[3 * i + 1] == 0) g2_vertex0 = true;
+        }
+        for (int v = 0; v < N1; v++) {
+            g2optr[v + 1] += g2optr[v];
+            g2iptr[v + 1] += g2iptr[v];
+        }
+        std::vector<int32_t> po(g2optr.begin(), g2optr.end() - 1), pi(g2iptr.begin(), g2iptr.end() - 1);
+        g2o.resize(m2);
+        g2i.resize(m2);
+        g2o_at_in.resize(m2);
+        g2i_at_out.resize(m2);
+        for (int64_t i = 0; i < m2; i++) {
+            const uint32_t u = (uint32_t)tri[3 * i], v = (uint32_t)tri[3 * i + 1];
+            const uint32_t wv = (uint16_t)(int16_t)tri[3 * i + 2];
+            const int jo = po[u]++, ji = pi[v]++;
+            g2o[jo] = (v << 16) | wv;
+            g2i[ji] = (u << 16) | wv;
+            g2o_at_in[ji] = jo;  // the same line in the other list
+            g2i_at_out[jo] = ji;
+        }
+    };
+    std::thread g2_thread;
+    try {
+        g2_thread = std::thread(build_g2);
+    } catch (...) {
+        build_g2();  // no second thread: build it here
+    }
+
     // ---- G1 comb lists: out entries (file order) then in entries (file order)
     std::vector<int32_t> cnt_out(N1, 0), cnt_in(N1, 0);
     for (int64_t i = 0; i < e1; i++) {
@@ -810,31 +871,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         maxw = std::max(maxw, w1[i]);
     }
 
-    // ---- G2: last line per ordered pair
-    std::vector<int32_t> tri;
-    bool g2nonneg = true;
-    {
-        std::vector<int32_t> p2(N1 + 1, 0);
-        for (int64_t i = 0; i < e2; i++) p2[s2[i] + 1]++;
-        for (int v = 0; v < N1; v++) p2[v + 1] += p2[v];
-        std::vector<int32_t> pos(p2.begin(), p2.end() - 1), od(e2), ow(e2);
-        for (int64_t i = 0; i < e2; i++) {
-            int p = pos[s2[i]]++;
-            od[p] = d2[i];
-            ow[p] = w2[i];
-        }
-        std::fill(stamp.begin(), stamp.end(), -1);
-        tri.reserve(3 * (size_t)e2);
-        for (int u = 0; u < N1; u++)
-            for (int e = p2[u + 1] - 1; e >= p2[u]; e--)
-                if (stamp[od[e]] != u) {
-                    stamp[od[e]] = u;
-                    tri.push_back(u);
-                    tri.push_back(od[e]);
-                    tri.push_back(ow[e]);
-                    if ((int16_t)ow[e] < 0) g2nonneg = false;
-                }
-    }
+    if (g2_thread.joinable()) g2_thread.join();
     c->g2nonneg = g2nonneg;
     c->nonneg = nonneg && g2nonneg;
     // every line its own dense cell: no self-loop, no repeated pair, no vertex 0, weights in [1, 32767]
@@ -842,39 +879,8 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     for (int64_t i = 0; i < e1 && c->dense_equal; i++)
         if (s1[i] == d1[i] || s1[i] == 0 || d1[i] == 0 || w1[i] < 1 || w1[i] > 32767) c->dense_equal = false;
     c->max_w1 = maxw;
-
-    // ---- G2 out- and in-lists of the de-duplicated lines, packed (vertex << 16 | weight)
-    std::vector<int32_t> g2optr(N1 + 1, 0), g2iptr(N1 + 1, 0);
-    std::vector<uint32_t> g2o, g2i;
-    std::vector<int32_t> g2o_at_in, g2i_at_out;
-    {
-        const int64_t m2 = (int64_t)tri.size() / 3;
-        c->g2_vertex0 = false;
-        for (int64_t i = 0; i < m2; i++) {
-            g2optr[tri[3 * i] + 1]++;
-            g2iptr[tri[3 * i + 1] + 1]++;
-            if (tri[3 * i] == 0 || tri[3 * i + 1] == 0) c->g2_vertex0 = true;
-        }
-        for (int v = 0; v < N1; v++) {
-            g2optr[v + 1] += g2optr[v];
-            g2iptr[v + 1] += g2iptr[v];
-        }
-        std::vector<int32_t> po(g2optr.begin(), g2optr.end() - 1), pi(g2iptr.begin(), g2iptr.end() - 1);
-        g2o.resize(m2);
-        g2i.resize(m2);
-        g2o_at_in.resize(m2);
-        g2i_at_out.resize(m2);
-        for (int64_t i = 0; i < m2; i++) {
-            const uint32_t u = (uint32_t)tri[3 * i], v = (uint32_t)tri[3 * i + 1];
-            const uint32_t wv = (uint16_t)(int16_t)tri[3 * i + 2];
-            const int jo = po[u]++, ji = pi[v]++;
-            g2o[jo] = (v << 16) | wv;
-            g2i[ji] = (u << 16) | wv;
-            g2o_at_in[ji] = jo;  // the same line in the other list
-            g2i_at_out[jo] = ji;
-        }
-        c->ng2 = m2;
-    }
+    c->g2_vertex0 = g2_vertex0;
+    c->ng2 = m2;
 
     // ---- device buffers
     size_t ld = (size_t)c->ld;
@@ -937,7 +943,6 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         if (e != hipSuccess) { rc = hip_fail(e, "memset W2"); goto fail; }
         e = hipMemsetAsync(c->Q, 0, 2 * (size_t)N1 * ld * 2, c->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "memset Q"); goto fail; }
-        int64_t m2 = (int64_t)tri.size() / 3;
         if (m2 > 0) {
             if ((rc = ensure_stage(c, tri.size() * 4))) goto fail;
             e = hipMemcpyAsync(c->stage, tri.data(), tri.size() * 4, hipMemcpyHostToDevice, c->stream);
