@@ -63,6 +63,8 @@ def lib():
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "bm_sweep_all_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]),
+        "bm_sweep_all_pairs_dense": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                               C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]),
         "bm_eval_rot3": (C.c_int, [C.c_void_p, _i32p, C.c_int64, C.c_int, _i32p, _i32p]),
         "bm_eval_permset": (C.c_int, [C.c_void_p, C.c_int32, _i32p, C.c_int64, _i32p, _i32p]),
         "bm_perm_from_rank": (C.c_int, [C.c_int32, C.c_int32, _i32p]),
@@ -114,7 +116,8 @@ def lib():
                                                  C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
         "bm_drive_row_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int64, C.c_void_p]),
-        "bm_drive_global_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
+        "bm_drive_global_best": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
+                                           C.c_void_p]),
         "bm_drive_row_best_restart": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "bm_drive_quick": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     }
@@ -421,15 +424,16 @@ class Engine:
         _check(self._L.bm_eval_swap_row(self._h, a, lo, hi, mode, C.byref(b), C.byref(d)), "bm_eval_swap_row")
         return b.value, d.value
 
-    def sweep_all_pairs(self, rows=False, matrix=False):
+    def sweep_all_pairs(self, rows=False, matrix=False, kind=BM_DELTA_LISTS):
+        """bm_sweep_all_pairs (Delta_B) or, with kind=BM_DELTA_DENSE, bm_sweep_all_pairs_dense."""
         a, b, d = C.c_int32(), C.c_int32(), C.c_int32()
         n1 = self.n + 1
         rb = np.zeros(n1, np.int32) if rows else None
         rd = np.zeros(n1, np.int32) if rows else None
         dm = np.zeros((n1, n1), np.int32) if matrix else None
         ptr = lambda x: x.ctypes.data_as(C.c_void_p) if x is not None else None
-        _check(self._L.bm_sweep_all_pairs(self._h, C.byref(a), C.byref(b), C.byref(d), ptr(rb), ptr(rd), ptr(dm)),
-               "bm_sweep_all_pairs")
+        fn = self._L.bm_sweep_all_pairs_dense if kind == BM_DELTA_DENSE else self._L.bm_sweep_all_pairs
+        _check(fn(self._h, C.byref(a), C.byref(b), C.byref(d), ptr(rb), ptr(rd), ptr(dm)), fn.__name__)
         return {"best": (a.value, b.value, d.value), "row_best_b": rb, "row_best_delta": rd, "matrix": dm}
 
     def eval_rot3(self, triples, mode=BM_ROT3_ROTATIONS):
@@ -597,8 +601,8 @@ class Engine:
     def row_best(self, max_sweeps=0, verify=True, acc_cap=1 << 20):
         return self._drive(self._L.bm_drive_row_best, max_sweeps, int(verify), None, None, acc_cap=acc_cap)
 
-    def global_best(self, max_steps=0, verify=True, acc_cap=1 << 20):
-        return self._drive(self._L.bm_drive_global_best, max_steps, int(verify), acc_cap=acc_cap)
+    def global_best(self, max_steps=0, verify=True, acc_cap=1 << 20, kind=BM_DELTA_LISTS):
+        return self._drive(self._L.bm_drive_global_best, kind, max_steps, int(verify), acc_cap=acc_cap)
 
 
 def refine_run_many(engines, max_steps=1 << 30, move_budget=0, patience=2):

@@ -639,6 +639,10 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->dfrw, (size_t)c->ndf * 8);
     dev_free(c, (void **)&c->dfneg, N1 * 4);
     dev_free(c, (void **)&c->dfdiag, N1 * 4);
+    dev_free(c, (void **)&c->dfes, (size_t)c->ndfout * 4);
+    dev_free(c, (void **)&c->dfed, (size_t)c->ndfout * 4);
+    dev_free(c, (void **)&c->dfew, (size_t)c->ndfout * 4);
+    c->ndfout = 0;
     dev_free(c, (void **)&c->dfX, (N1 + 63) / 64 * 64 * ld * 4);
     dev_free(c, (void **)&c->dfW1, N1 * ld * 2);
     dev_free(c, (void **)&c->dfXP, N1 * ld * 8);
@@ -1168,7 +1172,7 @@ int bm::ensure_dense_lists(bm_ctx *c) {
     }
     BM_HIP(hipStreamSynchronize(c->stream));
     // out entries (r >= N1) stay ahead of in entries, as in the lists they come from
-    std::vector<int32_t> fptr(N1 + 1, 0), fmid(N1, 0), fr, fw, frw, deg(N1, 0);
+    std::vector<int32_t> fptr(N1 + 1, 0), fmid(N1, 0), fr, fw, frw, deg(N1, 0), es, ed, ew;
     fr.reserve((size_t)c->ndd);
     fw.reserve((size_t)c->ndd);
     frw.reserve(2 * (size_t)c->ndd);
@@ -1184,17 +1188,28 @@ int bm::ensure_dense_lists(bm_ctx *c) {
             frw.push_back(r[e]);
             frw.push_back(cell);
             c->dfmaxw = std::max(c->dfmaxw, cell);
-            if (r[e] >= N1) fmid[v] = (int32_t)fr.size();
+            if (r[e] >= N1) {
+                fmid[v] = (int32_t)fr.size();
+                es.push_back(v);
+                ed.push_back(nb);
+                ew.push_back(cell);
+            }
         }
         deg[v] = (int32_t)fr.size() - fptr[v];
     }
     fptr[N1] = (int32_t)fr.size();
     int rc;
     c->ndf = (int64_t)fr.size();
+    c->ndfout = (int64_t)es.size();
     const std::vector<int32_t> zeros(N1, 0);
     if ((rc = upload(c, &c->dfr, fr)) || (rc = upload(c, &c->dfw, fw)) || (rc = upload(c, &c->dfrw, frw)) ||
         (rc = upload(c, &c->dfmid, fmid)) || (rc = upload(c, &c->dfneg, zeros)) ||
+        (rc = upload(c, &c->dfes, es)) || (rc = upload(c, &c->dfed, ed)) || (rc = upload(c, &c->dfew, ew)) ||
         (rc = dev_alloc(c, (void **)&c->dfdiag, (size_t)N1 * 4)) || (rc = upload(c, &c->dfptr, fptr))) {
+        dev_free(c, (void **)&c->dfes, (size_t)c->ndfout * 4);
+        dev_free(c, (void **)&c->dfed, (size_t)c->ndfout * 4);
+        dev_free(c, (void **)&c->dfew, (size_t)c->ndfout * 4);
+        c->ndfout = 0;
         dev_free(c, (void **)&c->dfr, (size_t)c->ndf * 4);
         dev_free(c, (void **)&c->dfw, (size_t)c->ndf * 4);
         dev_free(c, (void **)&c->dfrw, (size_t)c->ndf * 8);

@@ -76,6 +76,8 @@ struct bm_ctx {
     int32_t *dfrw = nullptr;    // (r, w) pairs of the same lists
     int32_t *dfneg = nullptr;   // zeros: the sum of min(w, 0) over lists of positive weights
     int32_t *dfdiag = nullptr;  // X(v,v) over the dense-form lists
+    int32_t *dfes = nullptr, *dfed = nullptr, *dfew = nullptr;  // their out entries as flat lines (src, dst, cell)
+    int64_t ndfout = 0;
     bool dfdiag_valid = false;
     int64_t ndf = 0;
     int32_t dfmaxw = 0;         // largest dense-form weight

@@ -1327,6 +1327,51 @@ int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag) {
     return BM_OK;
 }
 
+static int sweep_results(bm_ctx *c, int32_t *dmat, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
+                         int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix);
+
+// The dense-matrix delta of calculateSwapDeltasKernel (quick.cu:48-138,
+// graph-re-alignment-gpu.cu:60-128) for every pair: Xd over the dense-form
+// lists with the dense all-pairs kernel, their X(v,v), Corr of their lines
+// (each its own dense cell) folded in, then the same combine and arg-max.
+static int sweep_dense_enqueue(bm_ctx *c, int32_t *dmat) {
+    const int N1 = c->N1;
+    int rc = ensure_dense_lists(c);
+    if (rc) return rc;
+    if (!c->dfX && (rc = dev_alloc(c, (void **)&c->dfX, (size_t)(N1 + 63) / 64 * 64 * c->ld * 4))) return rc;
+    BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill_keys, dim3((N1 + 256) / 256), dim3(256), 0, c->stream, c->keys,
+                                            (int64_t)N1 + 1));
+    if ((rc = launch_list_diag(c, c->dfdiag, c->dfptr, c->dfr, c->dfw))) return rc;
+    c->dfdiag_valid = true;
+    if ((rc = launch_list_gain(c, c->dfptr, c->dfrw, c->dfX, c->dfmaxw))) return rc;
+    if (c->ndfout > 0)
+        BM_LAUNCH(c, "corr", hipLaunchKernelGGL(k_corr, dim3((unsigned)((c->ndfout + 255) / 256)), dim3(256), 0,
+                                                c->stream, c->dfes, c->dfed, c->dfew, c->dfew, c->ndfout, c->Q,
+                                                c->ld, c->dfX, (int64_t)c->ld));
+    const int T = (N1 + 63) / 64;
+    BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, (T + kCombChunk - 1) / kCombChunk), dim3(256), 0,
+                                               c->stream, c->dfX, (int64_t)c->ld, c->dfdiag, c->n, c->keys,
+                                               c->keys + N1, dmat));
+    return BM_OK;
+}
+
+extern "C" int bm_sweep_all_pairs_dense(bm_ctx *c, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
+                                        int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    BM_HIP(hipSetDevice(c->device));
+    const int N1 = c->N1;
+    int rc;
+    int32_t *dmat = nullptr;
+    if (delta_matrix) {
+        if ((rc = ensure_stage(c, (size_t)N1 * N1 * 4))) return rc;
+        dmat = (int32_t *)c->stage;
+        BM_HIP(hipMemsetAsync(dmat, 0, (size_t)N1 * N1 * 4, c->stream));
+    }
+    if ((rc = sweep_dense_enqueue(c, dmat))) return rc;
+    return sweep_results(c, dmat, best_a, best_b, best_delta, row_best_b, row_best_delta, delta_matrix);
+}
+
 extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
                                   int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix) {
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");
@@ -1342,6 +1387,13 @@ extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, i
     }
     if ((rc = sweep_prepare(c))) return rc;
     if ((rc = sweep_enqueue(c, dmat))) return rc;
+    return sweep_results(c, dmat, best_a, best_b, best_delta, row_best_b, row_best_delta, delta_matrix);
+}
+
+// Row-best and global keys (and the matrix) of the last combine back to the host.
+static int sweep_results(bm_ctx *c, int32_t *dmat, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
+                         int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix) {
+    const int N1 = c->N1;
     unsigned long long *gkey = c->keys + N1;
     std::vector<unsigned long long> keys;
     unsigned long long g = 0;

@@ -110,8 +110,9 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
 }
 
 /* graph-re-alignment-gpu.cu:830-891 without the time-seeded restart */
-int bm_drive_global_best(bm_ctx *ctx, int64_t max_steps, int verify, int32_t *acc, int64_t acc_cap,
+int bm_drive_global_best(bm_ctx *ctx, int32_t kind, int64_t max_steps, int verify, int32_t *acc, int64_t acc_cap,
                          bm_drive_stats *stats) {
+    if (kind != BM_DELTA_LISTS && kind != BM_DELTA_DENSE) return BM_EINVAL;
     bm_drive_stats st;
     memset(&st, 0, sizeof st);
     int rc = bm_score(ctx, &st.initial_score);
@@ -119,7 +120,9 @@ int bm_drive_global_best(bm_ctx *ctx, int64_t max_steps, int verify, int32_t *ac
     st.final_score = st.initial_score;
     while (max_steps <= 0 || st.sweeps < max_steps) {
         int32_t a, b, d;
-        if ((rc = bm_sweep_all_pairs(ctx, &a, &b, &d, NULL, NULL, NULL))) return rc;
+        rc = kind == BM_DELTA_DENSE ? bm_sweep_all_pairs_dense(ctx, &a, &b, &d, NULL, NULL, NULL)
+                                    : bm_sweep_all_pairs(ctx, &a, &b, &d, NULL, NULL, NULL);
+        if (rc) return rc;
         st.sweeps++;
         if (d <= 0) break;
         record(acc, acc_cap, st.accepted, a, b, d);

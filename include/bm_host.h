@@ -139,8 +139,13 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
                       int64_t acc_cap, bm_drive_stats *stats);
 
 /* Global best-swap steps of graph-re-alignment-gpu.cu:830-891 (all-pairs
- * arg-max, apply, repeat) until no positive pair or max_steps. */
-int bm_drive_global_best(bm_ctx *ctx, int64_t max_steps, int verify, int32_t *acc, int64_t acc_cap,
+ * arg-max, apply, repeat) until no positive pair or max_steps.  kind
+ * BM_DELTA_DENSE ranks by that program's own kernel delta
+ * (bm_sweep_all_pairs_dense) and, as it does (:874), adds the delta to the
+ * score; BM_DELTA_LISTS ranks by Delta_B.  With verify the score follows the
+ * exact change of every swap instead, and a delta that differs from it is
+ * counted in stats->mismatches. */
+int bm_drive_global_best(bm_ctx *ctx, int32_t kind, int64_t max_steps, int verify, int32_t *acc, int64_t acc_cap,
                          bm_drive_stats *stats);
 
 /* Row-best passes with the stagnation restart of alignment-omp.c:543-678:

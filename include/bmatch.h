@@ -156,6 +156,14 @@ uint64_t bm_rng_next(uint64_t *state);
 int bm_sweep_all_pairs(bm_ctx *ctx, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
                        int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix);
 
+/* bm_sweep_all_pairs with the delta the reference's GPU kernel computes,
+ * calculateSwapDeltasKernel (graph-re-alignment-gpu.cu:60-128,
+ * quick.cu:48-138): the dense-matrix form of bm_eval_swaps_dense for every
+ * pair, same outputs and tie-break.  Runs the dense all-pairs kernel over the
+ * dense-form lists (n+1 x ld int32 of extra device memory). */
+int bm_sweep_all_pairs_dense(bm_ctx *ctx, int32_t *best_a, int32_t *best_b, int32_t *best_delta,
+                             int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix);
+
 /* Delta_G3 of calculateThreeNodeSwapKernel, triple.cu:85-214, per triple
  * (m1,m2,m3) of distinct vertices in [1, n]: best over the permutations
  * (strict >, from 0) and its index p (0 = none positive).

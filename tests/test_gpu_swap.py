@@ -329,3 +329,70 @@ def test_apply_swap_scored_is_the_exact_change(case):
             assert got == want, (a, b)
             s += got
             assert s == e.score() == o.score_opt(cur)
+
+
+def dense_all_pairs(o, n, mp):
+    """calculateSwapDeltasKernel's matrix, row bests (b > a) and global best from the oracle's Delta_G2."""
+    dm = np.zeros((n + 1, n + 1), np.int32)
+    for a in range(1, n + 1):
+        for b in range(a + 1, n + 1):
+            dm[a, b] = dm[b, a] = o.delta_g2(mp, a, b)
+    rb = np.zeros(n + 1, np.int32)
+    rd = np.full(n + 1, np.iinfo(np.int32).min, np.int32)
+    best = (0, 0, 0)
+    for a in range(1, n):
+        row = dm[a, a + 1:]
+        j = int(np.argmax(row))
+        rb[a], rd[a] = a + 1 + j, row[j]
+        if row[j] > best[2]:
+            best = (a, a + 1 + j, int(row[j]))
+    return dm, rb, rd, best
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_all_pairs_dense_matches_oracle(case):
+    n, g1, g2, mp = make_case(case, seed=23)
+    o = ob.Oracle(n, g1, g2)
+    dm, rb, rd, best = dense_all_pairs(o, n, mp)
+    with engine_for(n, g1, g2, mp) as e:
+        got = e.sweep_all_pairs(rows=True, matrix=True, kind=bmatch.BM_DELTA_DENSE)
+        lists = e.sweep_all_pairs(rows=True, matrix=True)
+    np.testing.assert_array_equal(got["matrix"], dm)
+    np.testing.assert_array_equal(got["row_best_b"][1:n], rb[1:n])
+    np.testing.assert_array_equal(got["row_best_delta"][1:n], rd[1:n])
+    assert got["best"] == best
+    if case[0] in ("dups_loops", "vertex0", "dense", "negative"):
+        assert (got["matrix"] != lists["matrix"]).any()  # the case separates the two deltas
+
+
+@pytest.mark.parametrize("verify", [False, True])
+def test_global_best_dense_steps_match_oracle(verify):
+    n, g1, g2, mp = make_case(CASES[1], seed=29)  # repeated lines and self-loops
+    o = ob.Oracle(n, g1, g2)
+    cur = mp.copy()
+    want = []
+    for _ in range(12):
+        _, _, _, (a, b, d) = dense_all_pairs(o, n, cur)
+        if d <= 0:
+            break
+        want.append((a, b, d))
+        cur[a], cur[b] = cur[b], cur[a]
+    with engine_for(n, g1, g2, mp) as e:
+        acc, st = e.global_best(max_steps=12, verify=verify, kind=bmatch.BM_DELTA_DENSE)
+        np.testing.assert_array_equal(acc, np.array(want, np.int32).reshape(-1, 3))
+        np.testing.assert_array_equal(e.mapping(), cur)
+        exact = o.score_opt(cur)
+    if verify:  # the exact score, and every step whose kernel delta was not the score change counted
+        starts = [apply_swaps(mp, want[:i]) for i in range(len(want))]
+        changes = [o.delta_exact(m, a, b) for m, (a, b, _) in zip(starts, want)]
+        assert st["final_score"] == exact == st["initial_score"] + sum(changes)
+        assert st["mismatches"] == sum(d != ch for (_, _, d), ch in zip(want, changes))
+    else:  # the reference's own bookkeeping: score += delta
+        assert st["final_score"] == st["initial_score"] + sum(d for _, _, d in want)
+
+
+def apply_swaps(mp, swaps):
+    m = mp.copy()
+    for a, b, _ in swaps:
+        m[a], m[b] = m[b], m[a]
+    return m
