@@ -1368,7 +1368,13 @@ extern "C" int bm_sweep_all_pairs_dense(bm_ctx *c, int32_t *best_a, int32_t *bes
         dmat = (int32_t *)c->stage;
         BM_HIP(hipMemsetAsync(dmat, 0, (size_t)N1 * N1 * 4, c->stream));
     }
-    if ((rc = sweep_dense_enqueue(c, dmat))) return rc;
+    // when every G1 line is its own dense cell the comb lists are the dense-form lists and the two
+    // deltas are one: the pair sweep's own (sparse) kernels serve
+    if (c->dense_equal) {
+        if ((rc = sweep_prepare(c)) || (rc = sweep_enqueue(c, dmat))) return rc;
+    } else if ((rc = sweep_dense_enqueue(c, dmat))) {
+        return rc;
+    }
     return sweep_results(c, dmat, best_a, best_b, best_delta, row_best_b, row_best_delta, delta_matrix);
 }
 
