@@ -25,7 +25,7 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
             refined by the batched mode to convergence (budget 10^6 moves) on
             the serving loop; jobs/s and p50/p99 time-to-result (BM_SERVE_ACTIVE
             sets the server's active slots, default 8; BM_SERVE_PREP its
-            preparation threads, default 4)
+            preparation threads, default 8)
 """
 import os
 import time
@@ -317,7 +317,7 @@ def serve(args):
                                                                shuffle_frac=0.3), range(jobs)))
     gen_s = time.perf_counter() - t
     active = int(os.environ.get("BM_SERVE_ACTIVE", "8"))
-    prep = int(os.environ.get("BM_SERVE_PREP", "4"))
+    prep = int(os.environ.get("BM_SERVE_PREP", "8"))
     with bmatch.Server(max_active=active, prep_threads=prep, hbm_budget=0) as srv:
         ids = [None] * jobs
 

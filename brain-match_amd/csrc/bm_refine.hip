@@ -46,6 +46,7 @@ constexpr int kHist = 4096;  // per-step history ring
 constexpr int kXcolBlocks = 512;  // workgroups of the column recompute (grid-stride over its chunks)
 constexpr int kXcolLines = 256;   // G2 lines of a column's new image per chunk (at least)
 constexpr int kXcolSpare = 512;   // chunk slots beyond one per moved vertex
+constexpr int kXdeltaQual = 256;  // entries with a moved neighbour a row update keeps in LDS
 
 struct RefineState {
     unsigned long long step;   // steps completed in this run
@@ -574,24 +575,32 @@ __global__ __launch_bounds__(1024) void k_xdelta_rows(const RefineState *__restr
                                                       const uint32_t *__restrict__ g2iy, int N1,
                                                       int32_t *__restrict__ X, int64_t ldx) {
     extern __shared__ __align__(16) int row[];
-    __shared__ int any;
+    __shared__ int nq;
+    __shared__ int qual[kXdeltaQual];  // the row's entries with a moved neighbour
     if (!st->inc) return;
     const int T = (int)(ldx >> 6);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int u = blockIdx.x; u < N1; u += gridDim.x) {
-        if (threadIdx.x == 0) any = 0;
+        if (threadIdx.x == 0) nq = 0;
         __syncthreads();
         const int e0 = cptr[u], e1 = cptr[u + 1];
         for (int e = e0 + threadIdx.x; e < e1; e += 1024) {
             const int r = crw[e].x;
             const int x = r >= N1 ? r - N1 : r;
-            if (inU[x] && uoldv[x] != map[x]) any = 1;
+            if (inU[x] && uoldv[x] != map[x]) {
+                const int slot = atomicAdd(&nq, 1);
+                if (slot < kXdeltaQual) qual[slot] = e;
+            }
         }
         __syncthreads();
-        if (!any) continue;  // uniform
+        const int nqual = nq;
+        if (nqual == 0) continue;  // uniform
         for (int y = threadIdx.x; y < N1; y += 1024) row[y] = 0;
         __syncthreads();
-        for (int e = e0 + wave; e < e1; e += 16) {  // one wave per entry
+        // the recorded entries, or (more than kXdeltaQual of them) the whole list again
+        const bool listed = nqual <= kXdeltaQual;
+        for (int i = wave; i < (listed ? nqual : e1 - e0); i += 16) {  // one wave per entry
+            const int e = listed ? qual[i] : e0 + i;
             const int2 rw = crw[e];
             const bool out = rw.x >= N1;
             const int x = out ? rw.x - N1 : rw.x;
