@@ -377,10 +377,11 @@ __global__ void k_state_init(RefineState *st, int keep_x) {
 // permuted within U):
 //   - an entry of comb(u) whose neighbour x is in U now reads the list of
 //     p'(x) instead of p(x): row u gains f over the new list and loses f over
-//     the old one, in the columns outside U (k_xdelta);
+//     the old one, in the columns outside U (k_xdelta_rows, per row; or
+//     k_xdelta, per entry);
 //   - a list target in p(U) now translates to another vertex of U, so the
 //     columns of U are recomputed whole, pulled from the G2 neighbours of
-//     their new images (k_xcol);
+//     their new images (k_xcol_plan, k_xcol, k_xcol_sum);
 //   - Corr changes on the lines at U: subtracted before the move with the old
 //     P, added after it with the new P (k_corr_inc);
 //   - X(v,v) is read back into diag (k_diag_all).
