@@ -97,6 +97,13 @@ def test_eval_rows_errors():
             with pytest.raises(bmatch.BMError):
                 e.eval_rows(bad)
         assert e.eval_rows([]).shape == (0, n + 1)
+        for kind in (-1, 2):  # neither delta form
+            with pytest.raises(bmatch.BMError):
+                e.eval_rows([1], kind)
+            with pytest.raises(bmatch.BMError):
+                e.quick(max_evals=5, kind=kind)
+            with pytest.raises(bmatch.BMError):
+                e.global_best(max_steps=1, kind=kind)
 
 
 @pytest.mark.parametrize("mode", [bmatch.BM_PERTURB_OMP, bmatch.BM_PERTURB_WEIGHTED])
