@@ -639,8 +639,7 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
     if (run_flag && *run_flag == 0) return;  // X already current (the batched step's incremental update)
     extern __shared__ __align__(16) int acc[];
     __shared__ int pref[kSparseChunk + 1];
-    __shared__ int lst[kSparseChunk];
-    __shared__ int wts[kSparseChunk];
+    __shared__ int2 lw[kSparseChunk];  // per entry: list start (bit 31: in-list) and weight
     __shared__ int hubr[kHubs ? kSparseHubs : 1], hubw[kHubs ? kSparseHubs : 1];
     __shared__ int wsum[16];
     __shared__ int nhub;
@@ -662,8 +661,7 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
             const int g = map[out ? rw.x - N1 : rw.x];
             const int lo = out ? g2iptr[g] : g2optr[g];
             len = (out ? g2iptr[g + 1] : g2optr[g + 1]) - lo;
-            lst[tid] = out ? (lo | (int)0x80000000) : lo;
-            wts[tid] = rw.y;
+            lw[tid] = make_int2(out ? (lo | (int)0x80000000) : lo, rw.y);
             if (kHubs && len > dense_len) {
                 const int h = atomicAdd(&nhub, 1);
                 if (h < kSparseHubs) {  // beyond the cap the list is walked like the others
@@ -704,40 +702,50 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                 if (pref[mid] <= w0) lo_i = mid;
                 else hi_i = mid - 1;
             }
-            // kSparseUnroll list entries in flight per lane: indices idx + 64u
-            int i0 = lo_i;
+            // kSparseUnroll list entries in flight per lane: indices idx + 64u.  The lane's
+            // current entry i0 and its flat range [s0, s1) stay in registers; pref is read
+            // only when the lane moves on to the next entry.
+            int i0 = lo_i, s0 = pref[i0], s1 = pref[i0 + 1];
             int idx = w0 + lane;
             for (; idx + 64 * (kSparseUnroll - 1) < w1; idx += 64 * kSparseUnroll) {
-                int ii[kSparseUnroll];
+                int ii[kSparseUnroll], st[kSparseUnroll];
 #pragma unroll
                 for (int u = 0; u < kSparseUnroll; u++) {
-                    while (pref[i0 + 1] <= idx + 64 * u) i0++;
+                    while (s1 <= idx + 64 * u) {
+                        i0++;
+                        s0 = s1;
+                        s1 = pref[i0 + 1];
+                    }
                     ii[u] = i0;
+                    st[u] = s0;
                 }
                 uint32_t yv[kSparseUnroll];
+                int wu[kSparseUnroll];
 #pragma unroll
                 for (int u = 0; u < kSparseUnroll; u++) {
-                    const int L = lst[ii[u]];
-                    yv[u] = (L < 0 ? g2iy : g2oy)[(L & 0x7fffffff) + idx + 64 * u - pref[ii[u]]];
+                    const int2 L = lw[ii[u]];
+                    wu[u] = L.y;
+                    yv[u] = (L.x < 0 ? g2iy : g2oy)[(L.x & 0x7fffffff) + idx + 64 * u - st[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < kSparseUnroll; u++) {
                     const int y = yv[u] >> 16;
                     if (y != 0xFFFF) {
-                        const int w = wts[ii[u]];
+                        const int w = wu[u];
                         atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - min(w, 0));
                     }
                 }
             }
             for (; idx < w1; idx += 64) {
-                while (pref[i0 + 1] <= idx) i0++;
-                const int L = lst[i0];
-                const uint32_t yv = (L < 0 ? g2iy : g2oy)[(L & 0x7fffffff) + idx - pref[i0]];
-                const int y = yv >> 16;
-                if (y != 0xFFFF) {
-                    const int w = wts[i0];
-                    atomicAdd(&acc[y], min(w, (int)(int16_t)(yv & 0xffff)) - min(w, 0));
+                while (s1 <= idx) {
+                    i0++;
+                    s0 = s1;
+                    s1 = pref[i0 + 1];
                 }
+                const int2 L = lw[i0];
+                const uint32_t yv = (L.x < 0 ? g2iy : g2oy)[(L.x & 0x7fffffff) + idx - s0];
+                const int y = yv >> 16;
+                if (y != 0xFFFF) atomicAdd(&acc[y], min(L.y, (int)(int16_t)(yv & 0xffff)) - min(L.y, 0));
             }
         }
         __syncthreads();
