@@ -662,8 +662,8 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->g2o_at_in, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->g2i_at_out, (size_t)c->ng2 * 4);
     dev_free(c, (void **)&c->pinv, N1 * 4);
-    dev_free(c, (void **)&c->g2oy, (size_t)c->ng2 * 4);
-    dev_free(c, (void **)&c->g2iy, (size_t)c->ng2 * 4);
+    dev_free(c, (void **)&c->g2oy, (size_t)c->ng2 * 8);  // g2iy lives in the same allocation
+    c->g2iy = nullptr;
     dev_free(c, (void **)&c->X, (N1 + 63) / 64 * 64 * (size_t)c->ldx * 4);
     dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
     dev_free(c, (void **)&c->vsimple, N1);
@@ -940,8 +940,9 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         if ((rc = upload(c, &c->vsimple, simple))) goto fail;
     }
     if ((rc = dev_alloc(c, (void **)&c->pinv, (size_t)N1 * 4))) goto fail;
-    if ((rc = dev_alloc(c, (void **)&c->g2oy, (size_t)c->ng2 * 4))) goto fail;
-    if ((rc = dev_alloc(c, (void **)&c->g2iy, (size_t)c->ng2 * 4))) goto fail;
+    // the translated out- and in-lists back to back: g2iy = g2oy + ng2
+    if ((rc = dev_alloc(c, (void **)&c->g2oy, (size_t)c->ng2 * 8))) goto fail;
+    c->g2iy = c->g2oy + c->ng2;
     {
         hipError_t e = hipMemsetAsync(c->W2, 0, 2 * (size_t)N1 * ld * 2, c->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "memset W2"); goto fail; }

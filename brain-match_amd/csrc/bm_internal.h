@@ -93,7 +93,8 @@ struct bm_ctx {
     int64_t ng2 = 0;
     bool g2_vertex0 = false;  // a G2 line names vertex 0
     int32_t *pinv = nullptr;  // pinv[g] = the vertex mapped to g (-1 if none); valid when sparse_ok
-    uint32_t *g2oy = nullptr, *g2iy = nullptr;  // the G2 lists with targets as P coordinates (p^-1), 0xFFFF = none
+    // the G2 lists with targets as P coordinates (p^-1), 0xFFFF = none; one allocation, g2iy = g2oy + ng2
+    uint32_t *g2oy = nullptr, *g2iy = nullptr;
     int32_t *g2o_at_in = nullptr;   // per in-list entry (line u->g): its index in the out-lists
     int32_t *g2i_at_out = nullptr;  // per out-list entry (line g->v): its index in the in-lists
     bool g2y_dirty = true;

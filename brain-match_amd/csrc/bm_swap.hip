@@ -635,11 +635,11 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                                                       const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
                                                       int ld, int32_t *__restrict__ X, int64_t ldx,
                                                       int32_t *__restrict__ diag, int N1, int dense_len,
-                                                      const int *__restrict__ run_flag) {
+                                                      int ng2, const int *__restrict__ run_flag) {
     if (run_flag && *run_flag == 0) return;  // X already current (the batched step's incremental update)
     extern __shared__ __align__(16) int acc[];
     __shared__ int pref[kSparseChunk + 1];
-    __shared__ int2 lw[kSparseChunk];  // per entry: list start (bit 31: in-list) and weight
+    __shared__ int2 lw[kSparseChunk];  // per entry: list start in g2oy (the in-lists follow) and weight
     __shared__ int hubr[kHubs ? kSparseHubs : 1], hubw[kHubs ? kSparseHubs : 1];
     __shared__ int wsum[16];
     __shared__ int nhub;
@@ -661,7 +661,7 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
             const int g = map[out ? rw.x - N1 : rw.x];
             const int lo = out ? g2iptr[g] : g2optr[g];
             len = (out ? g2iptr[g + 1] : g2optr[g + 1]) - lo;
-            lw[tid] = make_int2(out ? (lo | (int)0x80000000) : lo, rw.y);
+            lw[tid] = make_int2(out ? ng2 + lo : lo, rw.y);  // into g2oy, or g2iy = g2oy + ng2
             if (kHubs && len > dense_len) {
                 const int h = atomicAdd(&nhub, 1);
                 if (h < kSparseHubs) {  // beyond the cap the list is walked like the others
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                 for (int u = 0; u < kSparseUnroll; u++) {
                     const int2 L = lw[ii[u]];
                     wu[u] = L.y;
-                    yv[u] = (L.x < 0 ? g2iy : g2oy)[(L.x & 0x7fffffff) + idx + 64 * u - st[u]];
+                    yv[u] = g2oy[L.x + idx + 64 * u - st[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < kSparseUnroll; u++) {
@@ -743,7 +743,7 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                     s1 = pref[i0 + 1];
                 }
                 const int2 L = lw[i0];
-                const uint32_t yv = (L.x < 0 ? g2iy : g2oy)[(L.x & 0x7fffffff) + idx - s0];
+                const uint32_t yv = g2oy[L.x + idx - s0];
                 const int y = yv >> 16;
                 if (y != 0xFFFF) atomicAdd(&acc[y], min(L.y, (int)(int16_t)(yv & 0xffff)) - min(L.y, 0));
             }
@@ -890,12 +890,12 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
         BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<true>, dim3((unsigned)c->nplan), dim3(1024), lds,
                                                 c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
                                                 c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
-                                                c->ldx, c->diag, c->N1, c->dense_len, run_flag));
+                                                c->ldx, c->diag, c->N1, c->dense_len, (int)c->ng2, run_flag));
     else
         BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<false>, dim3((unsigned)c->nplan), dim3(1024), lds,
                                                 c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
                                                 c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
-                                                c->ldx, c->diag, c->N1, c->dense_len, run_flag));
+                                                c->ldx, c->diag, c->N1, c->dense_len, (int)c->ng2, run_flag));
     if (c->nsplit > 0)
         BM_LAUNCH(c, "diag_from_x", hipLaunchKernelGGL(k_diag_from_x, dim3((c->nsplit + 255) / 256), dim3(256), 0,
                                                        c->stream, c->X, c->ldx, c->split_rows, c->nsplit, c->diag,
