@@ -877,6 +877,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
 
     if (g2_thread.joinable()) g2_thread.join();
     c->g2nonneg = g2nonneg;
+    c->g1nonneg = nonneg;
     c->nonneg = nonneg && g2nonneg;
     // every line its own dense cell: no self-loop, no repeated pair, no vertex 0, weights in [1, 32767]
     c->dense_equal = c->ndd == c->ncomb;

@@ -57,6 +57,7 @@ struct bm_ctx {
     bool diag_valid = false;
     bool nonneg = false;  // all weights >= 0 (packed-accumulate fast paths)
     bool g2nonneg = false;  // every dense G2 cell >= 0 as int16 (so is every entry of P)
+    bool g1nonneg = false;  // every G1 weight >= 0
     bool dense_equal = false;  // every G1 line is its own dense cell: comb lists = dense-form lists
     int32_t max_w1 = 0;
 

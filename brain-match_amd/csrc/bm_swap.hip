@@ -623,7 +623,7 @@ int bm::launch_refresh_g2y(bm_ctx *c) {
 // X(a,a) becomes diag[a] afterwards (k_diag_from_x).
 constexpr int kRowFirst = 1, kRowWhole = 2;
 
-template <bool kHubs>
+template <bool kHubs, bool kNeg>
 __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ plan, const int32_t *__restrict__ cptr,
                                                       const int2 *__restrict__ crw,
                                                       const int32_t *__restrict__ cdw,
@@ -732,7 +732,8 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                     const int y = yv[u] >> 16;
                     if (y != 0xFFFF) {
                         const int w = wu[u];
-                        atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - min(w, 0));
+                        // f(w, q) = min(w, q) - min(w, 0); kNeg: some G1 weight is negative
+                        atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - (kNeg ? min(w, 0) : 0));
                     }
                 }
             }
@@ -745,7 +746,8 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                 const int2 L = lw[i0];
                 const uint32_t yv = g2oy[L.x + idx - s0];
                 const int y = yv >> 16;
-                if (y != 0xFFFF) atomicAdd(&acc[y], min(L.y, (int)(int16_t)(yv & 0xffff)) - min(L.y, 0));
+                if (y != 0xFFFF)
+                    atomicAdd(&acc[y], min(L.y, (int)(int16_t)(yv & 0xffff)) - (kNeg ? min(L.y, 0) : 0));
             }
         }
         __syncthreads();
@@ -875,27 +877,28 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
     int rc = build_row_plan(c);
     if (rc) return rc;
     if (!c->sparse_attr) {  // once per context, so that captured launches make no attribute calls
-        BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(kSparseLdsMax + 32)));
-        BM_HIP(hipFuncSetAttribute((const void *)k_gain_sparse<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(kSparseLdsMax + 32)));
+        const void *fns[4] = {(const void *)k_gain_sparse<true, true>, (const void *)k_gain_sparse<true, false>,
+                              (const void *)k_gain_sparse<false, true>, (const void *)k_gain_sparse<false, false>};
+        for (const void *f : fns)
+            BM_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kSparseLdsMax + 32)));
         c->sparse_attr = true;
     }
     if (c->nsplit > 0)
         BM_LAUNCH(c, "zero_rows", hipLaunchKernelGGL(k_zero_rows, dim3(c->nsplit), dim3(256), 0, c->stream, c->X,
                                                      c->ldx, c->split_rows, run_flag));
     // the long-list path costs registers and scheduling even when idle: only
-    // the instantiation that needs it carries it
-    if (c->g2_maxlen > c->dense_len)
-        BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<true>, dim3((unsigned)c->nplan), dim3(1024), lds,
-                                                c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
-                                                c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
-                                                c->ldx, c->diag, c->N1, c->dense_len, (int)c->ng2, run_flag));
-    else
-        BM_LAUNCH(c, "gain", hipLaunchKernelGGL(k_gain_sparse<false>, dim3((unsigned)c->nplan), dim3(1024), lds,
-                                                c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg,
-                                                c->map, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X,
-                                                c->ldx, c->diag, c->N1, c->dense_len, (int)c->ng2, run_flag));
+    // the instantiation that needs it carries it; likewise the min(w, 0) term
+#define BM_GAIN_SPARSE(H, NEG)                                                                                      \
+    BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain_sparse<H, NEG>), dim3((unsigned)c->nplan), dim3(1024), lds,      \
+                                            c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg, c->map, \
+                                            c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X, c->ldx, c->diag, \
+                                            c->N1, c->dense_len, (int)c->ng2, run_flag))
+    const bool hubs = c->g2_maxlen > c->dense_len, neg = !c->g1nonneg;
+    if (hubs && neg) BM_GAIN_SPARSE(true, true);
+    else if (hubs) BM_GAIN_SPARSE(true, false);
+    else if (neg) BM_GAIN_SPARSE(false, true);
+    else BM_GAIN_SPARSE(false, false);
+#undef BM_GAIN_SPARSE
     if (c->nsplit > 0)
         BM_LAUNCH(c, "diag_from_x", hipLaunchKernelGGL(k_diag_from_x, dim3((c->nsplit + 255) / 256), dim3(256), 0,
                                                        c->stream, c->X, c->ldx, c->split_rows, c->nsplit, c->diag,
