@@ -706,27 +706,24 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
             // current entry i0 and its flat range [s0, s1) stay in registers; pref is read
             // only when the lane moves on to the next entry.
             int i0 = lo_i, s0 = pref[i0], s1 = pref[i0 + 1];
+            int2 cur = lw[i0];  // the current entry's list start and weight
             int idx = w0 + lane;
             for (; idx + 64 * (kSparseUnroll - 1) < w1; idx += 64 * kSparseUnroll) {
-                int ii[kSparseUnroll], st[kSparseUnroll];
+                int at[kSparseUnroll], wu[kSparseUnroll];
 #pragma unroll
                 for (int u = 0; u < kSparseUnroll; u++) {
                     while (s1 <= idx + 64 * u) {
                         i0++;
                         s0 = s1;
                         s1 = pref[i0 + 1];
+                        cur = lw[i0];
                     }
-                    ii[u] = i0;
-                    st[u] = s0;
+                    at[u] = cur.x - s0;  // word idx + 64u of the flat range is g2oy[at + idx + 64u]
+                    wu[u] = cur.y;
                 }
                 uint32_t yv[kSparseUnroll];
-                int wu[kSparseUnroll];
 #pragma unroll
-                for (int u = 0; u < kSparseUnroll; u++) {
-                    const int2 L = lw[ii[u]];
-                    wu[u] = L.y;
-                    yv[u] = g2oy[L.x + idx + 64 * u - st[u]];
-                }
+                for (int u = 0; u < kSparseUnroll; u++) yv[u] = g2oy[at[u] + idx + 64 * u];
 #pragma unroll
                 for (int u = 0; u < kSparseUnroll; u++) {
                     const int y = yv[u] >> 16;
@@ -742,12 +739,12 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                     i0++;
                     s0 = s1;
                     s1 = pref[i0 + 1];
+                    cur = lw[i0];
                 }
-                const int2 L = lw[i0];
-                const uint32_t yv = g2oy[L.x + idx - s0];
+                const uint32_t yv = g2oy[cur.x + idx - s0];
                 const int y = yv >> 16;
                 if (y != 0xFFFF)
-                    atomicAdd(&acc[y], min(L.y, (int)(int16_t)(yv & 0xffff)) - (kNeg ? min(L.y, 0) : 0));
+                    atomicAdd(&acc[y], min(cur.y, (int)(int16_t)(yv & 0xffff)) - (kNeg ? min(cur.y, 0) : 0));
             }
         }
         __syncthreads();
