@@ -128,34 +128,15 @@ __global__ __launch_bounds__(256) void k_rot3(const int32_t *__restrict__ T, int
     }
 }
 
-// Table form of the same Delta_G3.  With Xd(u,v) = sum over the dense-form
-// list of u (the entries triple.cu's scan visits: cells > 0 as short, no
-// self-loop, no vertex 0) of min(w, Q[r][v]), moving m_k to the image of
-// m_s(k) changes its list terms by Xd(m_k, m_s(k)) - Xd(m_k, m_k); the scan
-// skips the entries between triple vertices, so their terms are taken out
-// again and the six intra-triple cells (:164-207) added in their place.  All
-// of that needs only the 3x3 blocks of Xd, of P and of the G1 cells at the
-// triple, which one table holds side by side: XP[u][v] = (Xd(u,v),
-// W1[u][v] << 16 | (uint16)P[u][v]), 8 bytes, so one thread per triple makes
-// nine loads instead of three per list entry.
-__global__ __launch_bounds__(256) void k_rot3_table(const int32_t *__restrict__ T, int64_t count, int mode,
-                                                    int32_t *__restrict__ best_d, int32_t *__restrict__ best_p,
-                                                    const int2 *__restrict__ XP, int ld) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    int m[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) m[i] = T[3 * t + i];
-    int p[3][3], x[3][3], c[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int2 v = XP[(size_t)m[i] * ld + m[j]];
-            x[i][j] = v.x;
-            p[i][j] = (int)(int16_t)(v.y & 0xffff);  // P[m_i][m_j] = W2[f_i][f_j]
-            c[i][j] = i == j ? 0 : v.y >> 16;         // the G1 cell m_i -> m_j
-        }
+// Delta_G3 of every permutation q of a triple from its 3x3 blocks: x = Xd
+// (the dense-form list sums, x[i][j] = Xd(m_i, m_j)), p = P at the triple,
+// c = the G1 cells between triple vertices (0 on the diagonal and where
+// absent).  Moving m_k to the image of m_s(k) changes its list terms by
+// x[k][s(k)] - x[k][k]; the entries between triple vertices, which Xd
+// counted, are taken out and the intra-triple cells (triple.cu:164-207)
+// added instead.  Best strict > from 0 over the allowed q (:94-214).
+__device__ __forceinline__ void rot3_score(const int (&x)[3][3], const int (&p)[3][3], const int (&c)[3][3],
+                                           int mode, int32_t &best_d, int32_t &best_p) {
     int bd = 0, bp = 0;
 #pragma unroll
     for (int q = 1; q < 6; q++) {
@@ -182,8 +163,36 @@ __global__ __launch_bounds__(256) void k_rot3_table(const int32_t *__restrict__ 
             bp = q;
         }
     }
-    best_d[t] = bd;
-    best_p[t] = bp;
+    best_d = bd;
+    best_p = bp;
+}
+
+// Table form of the same Delta_G3, Xd(u,v) being the sum over the dense-form
+// list of u (the entries triple.cu's scan visits: cells > 0 as short, no
+// self-loop, no vertex 0) of min(w, Q[r][v]).  rot3_score needs only the 3x3
+// blocks of Xd, of P and of the G1 cells at the triple, which one table holds
+// side by side: XP[u][v] = (Xd(u,v), W1[u][v] << 16 | (uint16)P[u][v]),
+// 8 bytes, so one thread per triple makes nine loads instead of three per
+// list entry.
+__global__ __launch_bounds__(256) void k_rot3_table(const int32_t *__restrict__ T, int64_t count, int mode,
+                                                    int32_t *__restrict__ best_d, int32_t *__restrict__ best_p,
+                                                    const int2 *__restrict__ XP, int ld) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int m[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) m[i] = T[3 * t + i];
+    int p[3][3], x[3][3], c[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int2 v = XP[(size_t)m[i] * ld + m[j]];
+            x[i][j] = v.x;
+            p[i][j] = (int)(int16_t)(v.y & 0xffff);  // P[m_i][m_j] = W2[f_i][f_j]
+            c[i][j] = i == j ? 0 : v.y >> 16;         // the G1 cell m_i -> m_j
+        }
+    rot3_score(x, p, c, mode, best_d[t], best_p[t]);
 }
 
 // XP[u][v] for u < N1, v < ld from the tile-major Xd, P (rows 0..n of Q) and
@@ -244,32 +253,7 @@ __global__ __launch_bounds__(256) void k_rot3_x(const int32_t *__restrict__ T, i
             if (c[j][i] > 0) corr += pair_corr(false, c[j][i], c[j][i], p[i][i], p[j][j], p[i][j], p[j][i]);
             x[i][j] -= corr;
         }
-    int bd = 0, bp = 0;
-#pragma unroll
-    for (int q = 1; q < 6; q++) {
-        if (mode == BM_ROT3_ROTATIONS && q != 3 && q != 4) continue;
-        int d = 0;
-#pragma unroll
-        for (int k = 0; k < 3; k++) d += x[k][kG3Perm[q][k]] - x[k][k];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int w = c[i][j];
-                if (i == j || w <= 0) continue;
-                const int si = kG3Perm[q][i], sj = kG3Perm[q][j];
-                const int old = min(w, p[i][j]);
-                d -= min(w, p[si][j]) - old;
-                d -= min(w, p[i][sj]) - old;
-                d += min(w, p[si][sj]) - old;
-            }
-        if (d > bd) {
-            bd = d;
-            bp = q;
-        }
-    }
-    best_d[t] = bd;
-    best_p[t] = bp;
+    rot3_score(x, p, c, mode, best_d[t], best_p[t]);
 }
 
 // dfW1[v][d] = the dense-form cell of the out entry v -> d (0 elsewhere).
