@@ -346,33 +346,10 @@ __global__ __launch_bounds__(512) void k_diag_set(int32_t *__restrict__ diag, co
 // they are the G2 lines at those images, found through the cross positions.
 struct ImageSet {
     int k;
-    int g[8];
+    int g[8];    // the moved images
+    int y[8];    // their preimages after the move
     int off[9];  // prefix of (in-degree + out-degree) over the images
 };
-
-__global__ __launch_bounds__(256) void k_g2y_patch(uint32_t *__restrict__ g2oy, uint32_t *__restrict__ g2iy,
-                                                   const uint32_t *__restrict__ g2o, const uint32_t *__restrict__ g2i,
-                                                   const int32_t *__restrict__ g2optr,
-                                                   const int32_t *__restrict__ g2iptr,
-                                                   const int32_t *__restrict__ g2o_at_in,
-                                                   const int32_t *__restrict__ g2i_at_out,
-                                                   const int32_t *__restrict__ pinv, ImageSet is) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= is.off[is.k]) return;
-    int i = 0;
-    while (t >= is.off[i + 1]) i++;
-    const int g = is.g[i], l = t - is.off[i];
-    const int y = pinv[g];
-    const uint32_t hi = (uint32_t)(y < 0 ? 0xFFFF : y) << 16;
-    const int nin = g2iptr[g + 1] - g2iptr[g];
-    if (l < nin) {  // line u -> g: entry of u's out-list
-        const int j = g2o_at_in[g2iptr[g] + l];
-        g2oy[j] = hi | (g2o[j] & 0xFFFFu);
-    } else {        // line g -> v: entry of v's in-list
-        const int j = g2i_at_out[g2optr[g] + l - nin];
-        g2iy[j] = hi | (g2i[j] & 0xFFFFu);
-    }
-}
 
 struct SetPerm {
     int k;
@@ -380,37 +357,86 @@ struct SetPerm {
     int src[8];  // new row/col set[i] <- old row/col set[src[i]]
 };
 
-// Row part of a set permutation on both P and PT: thread per column.
-__global__ void k_perm_rows(int16_t *__restrict__ Q, int N1, int ld, SetPerm sp) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ld) return;
-    for (int half = 0; half < 2; half++) {
-        int16_t *M = Q + (size_t)half * N1 * ld;
-        int16_t old[8];
-        for (int i = 0; i < sp.k; i++) old[i] = M[(size_t)sp.set[sp.src[i]] * ld + j];
-        for (int i = 0; i < sp.k; i++) M[(size_t)sp.set[i] * ld + j] = old[i];
+// One move on P and P^T (rows and columns of the set permuted), the mapping
+// and the translated G2 lists, in one launch.  Thread ranges:
+//   [0, ld)              column j outside the set: the set's rows at j, both halves
+//   [ld, ld + 2(n+1))    row x outside the set: the set's columns in row x
+//   next npatch          the translated entries of the G2 lines at the moved images
+//                        (target -> its preimage after the move, is.y)
+//   the last one         the set x set block of both halves, the mapping and p^-1
+// The four parts write disjoint elements and read only elements no other part
+// writes, so they need no ordering among themselves.
+__global__ __launch_bounds__(256) void k_apply_move(int16_t *__restrict__ Q, int N1, int ld, SetPerm sp,
+                                                    int32_t *__restrict__ map, int32_t *__restrict__ pinv,
+                                                    int with_pinv, ImageSet is, int npatch,
+                                                    uint32_t *__restrict__ g2oy, uint32_t *__restrict__ g2iy,
+                                                    const uint32_t *__restrict__ g2o, const uint32_t *__restrict__ g2i,
+                                                    const int32_t *__restrict__ g2optr,
+                                                    const int32_t *__restrict__ g2iptr,
+                                                    const int32_t *__restrict__ g2o_at_in,
+                                                    const int32_t *__restrict__ g2i_at_out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = sp.k;
+    auto in_set = [&](int v) {
+        bool hit = false;
+        for (int i = 0; i < k; i++) hit |= sp.set[i] == v;
+        return hit;
+    };
+    if (t < ld) {  // rows part
+        const int j = (int)t;
+        if (in_set(j)) return;
+        for (int half = 0; half < 2; half++) {
+            int16_t *M = Q + (size_t)half * N1 * ld;
+            int16_t old[8];
+            for (int i = 0; i < k; i++) old[i] = M[(size_t)sp.set[sp.src[i]] * ld + j];
+            for (int i = 0; i < k; i++) M[(size_t)sp.set[i] * ld + j] = old[i];
+        }
+        return;
     }
-}
-
-// Column part: thread per row.
-__global__ void k_perm_cols(int16_t *__restrict__ Q, int N1, int ld, SetPerm sp) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= 2 * N1) return;
-    int16_t *row = Q + (size_t)x * ld;
-    int16_t old[8];
-    for (int i = 0; i < sp.k; i++) old[i] = row[sp.set[sp.src[i]]];
-    for (int i = 0; i < sp.k; i++) row[sp.set[i]] = old[i];
-}
-
-__global__ void k_perm_map(int32_t *__restrict__ map, int32_t *__restrict__ pinv, int with_pinv, SetPerm sp) {
-    if (threadIdx.x != 0) return;
-    int old[8];
-    for (int i = 0; i < sp.k; i++) old[i] = map[sp.set[sp.src[i]]];
-    for (int i = 0; i < sp.k; i++) map[sp.set[i]] = old[i];
-    // the images are permuted among the set, so p stays injective
-    if (with_pinv)
-        for (int i = 0; i < sp.k; i++)
-            if (old[i] != 0) pinv[old[i]] = sp.set[i];
+    int64_t u = t - ld;
+    if (u < 2 * (int64_t)N1) {  // columns part
+        const int x = (int)u;
+        if (in_set(x >= N1 ? x - N1 : x)) return;
+        int16_t *row = Q + (size_t)x * ld;
+        int16_t old[8];
+        for (int i = 0; i < k; i++) old[i] = row[sp.set[sp.src[i]]];
+        for (int i = 0; i < k; i++) row[sp.set[i]] = old[i];
+        return;
+    }
+    u -= 2 * (int64_t)N1;
+    if (u < npatch) {  // translated G2 entries at the moved images
+        const int l0 = (int)u;
+        int i = 0;
+        while (l0 >= is.off[i + 1]) i++;
+        const int g = is.g[i], l = l0 - is.off[i];
+        const uint32_t hi = (uint32_t)is.y[i] << 16;
+        const int nin = g2iptr[g + 1] - g2iptr[g];
+        if (l < nin) {  // line s -> g: entry of s's out-list
+            const int j = g2o_at_in[g2iptr[g] + l];
+            g2oy[j] = hi | (g2o[j] & 0xFFFFu);
+        } else {        // line g -> v: entry of v's in-list
+            const int j = g2i_at_out[g2optr[g] + l - nin];
+            g2iy[j] = hi | (g2i[j] & 0xFFFFu);
+        }
+        return;
+    }
+    if (u == npatch) {  // the set x set block, the mapping
+        for (int half = 0; half < 2; half++) {
+            int16_t *M = Q + (size_t)half * N1 * ld;
+            int16_t old[8][8];
+            for (int i = 0; i < k; i++)
+                for (int j = 0; j < k; j++) old[i][j] = M[(size_t)sp.set[sp.src[i]] * ld + sp.set[sp.src[j]]];
+            for (int i = 0; i < k; i++)
+                for (int j = 0; j < k; j++) M[(size_t)sp.set[i] * ld + sp.set[j]] = old[i][j];
+        }
+        int img[8];
+        for (int i = 0; i < k; i++) img[i] = map[sp.set[sp.src[i]]];
+        for (int i = 0; i < k; i++) map[sp.set[i]] = img[i];
+        // the images are permuted among the set, so p stays injective
+        if (with_pinv)
+            for (int i = 0; i < k; i++)
+                if (img[i] != 0) pinv[img[i]] = sp.set[i];
+    }
 }
 
 // ---------------------------------------------------------------- helpers
@@ -511,36 +537,34 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
         sp.set[i] = set[i];
         sp.src[i] = src_of[i];
     }
-    BM_LAUNCH(c, "perm_rows",
-              hipLaunchKernelGGL(k_perm_rows, dim3((c->ld + 255) / 256), dim3(256), 0, c->stream, c->Q, c->N1, c->ld, sp));
-    BM_LAUNCH(c, "perm_cols", hipLaunchKernelGGL(k_perm_cols, dim3((2 * c->N1 + 255) / 256), dim3(256), 0, c->stream,
-                                                 c->Q, c->N1, c->ld, sp));
-    BM_LAUNCH(c, "perm_map", hipLaunchKernelGGL(k_perm_map, dim3(1), dim3(64), 0, c->stream, c->map, c->pinv, c->sparse_ok ? 1 : 0, sp));
     int32_t old[8];
     for (int i = 0; i < k; i++) old[i] = c->hmap[set[src_of[i]]];
     for (int i = 0; i < k; i++) c->hmap[set[i]] = old[i];
     if (c->sparse_ok)
         for (int i = 0; i < k; i++)
             if (old[i] != 0) c->hpinv[old[i]] = set[i];
-    if (c->sparse_ok && !c->g2y_dirty) {
-        // keep the translated G2 lists current: the lines at the moved images
-        ImageSet is;
-        memset(&is, 0, sizeof is);
+    // keep the translated G2 lists current: the lines at the moved images
+    ImageSet is;
+    memset(&is, 0, sizeof is);
+    const bool patch = c->sparse_ok && !c->g2y_dirty;
+    if (patch) {
         for (int i = 0; i < k; i++) {
             const int g = old[i];
             if (g == 0) continue;
             is.g[is.k] = g;
+            is.y[is.k] = set[i];
             is.off[is.k + 1] = is.off[is.k] + (c->hg2iptr[g + 1] - c->hg2iptr[g]) + (c->hg2optr[g + 1] - c->hg2optr[g]);
             is.k++;
         }
-        const int m = is.off[is.k];
-        if (m > 0)
-            BM_LAUNCH(c, "g2y_patch", hipLaunchKernelGGL(k_g2y_patch, dim3((m + 255) / 256), dim3(256), 0, c->stream,
-                                                         c->g2oy, c->g2iy, c->g2o, c->g2i, c->g2optr, c->g2iptr,
-                                                         c->g2o_at_in, c->g2i_at_out, c->pinv, is));
     } else {
         c->g2y_dirty = true;
     }
+    const int npatch = is.off[is.k];
+    const int64_t threads = (int64_t)c->ld + 2 * (int64_t)c->N1 + npatch + 1;
+    BM_LAUNCH(c, "apply_move",
+              hipLaunchKernelGGL(k_apply_move, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream, c->Q,
+                                 c->N1, c->ld, sp, c->map, c->pinv, c->sparse_ok ? 1 : 0, is, npatch, c->g2oy, c->g2iy,
+                                 c->g2o, c->g2i, c->g2optr, c->g2iptr, c->g2o_at_in, c->g2i_at_out));
     if (diag)
         BM_LAUNCH(c, "diag_set", hipLaunchKernelGGL(k_diag_set, dim3(1), dim3(64 * k), 0, c->stream, c->diag,
                                                     c->cptr, c->cr, c->cw, c->Q, c->ld, ts));
