@@ -297,29 +297,9 @@ struct TouchSet {
     int k;
     int v[8];
     int src[8];  // the old row / column that S[i] takes
+    int inv[8];  // the position that S[i]'s old row / column moves to
     int off[9];  // prefix of the comb-list lengths over the set
 };
-
-__global__ __launch_bounds__(256) void k_diag_delta(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
-                                                    const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
-                                                    const int16_t *__restrict__ Q, int N1, int ld, TouchSet ts) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ts.off[ts.k]) return;
-    int i = 0;
-    while (t >= ts.off[i + 1]) i++;
-    const int s = ts.v[i], s2 = ts.v[ts.src[i]];
-    if (s2 == s) return;
-    const int e = cptr[s] + (t - ts.off[i]);
-    const int r = cr[e], w = cw[e];
-    const bool out = r >= N1;
-    const int v = out ? r - N1 : r;
-    for (int q = 0; q < ts.k; q++)
-        if (ts.v[q] == v) return;  // a line inside S: v is recomputed in full
-    const int o = out ? (int)Q[(size_t)s * ld + v] : (int)Q[(size_t)v * ld + s];
-    const int nw = out ? (int)Q[(size_t)s2 * ld + v] : (int)Q[(size_t)v * ld + s2];
-    const int d = min(w, nw) - min(w, o);
-    if (d) atomicAdd(&diag[v], d);
-}
 
 __device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restrict__ cptr,
                                        const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
@@ -330,15 +310,39 @@ __device__ __forceinline__ int diag_of(int v, int lane, const int32_t *__restric
     return s;
 }
 
-// one wave per set vertex: its whole list, after the permutation
-__global__ __launch_bounds__(512) void k_diag_set(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
-                                                  const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
-                                                  const int16_t *__restrict__ Q, int ld, TouchSet ts) {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (w >= ts.k) return;
-    const int v = ts.v[w];
-    const int d = diag_of(v, lane, cptr, cr, cw, Q, ld);
-    if (lane == 0) diag[v] = d;
+// X(v,v) after a move, from the moved P and P^T, in one launch.
+//   blocks [0, nb): a thread per line between a set vertex s and a vertex v
+//     outside the set: X(v,v) changes by min(w, cell now) - min(w, cell before),
+//     the cell before being the one that now sits at s's new position inv;
+//   block nb: one wave per set vertex over its whole list.
+// The two parts write X(v,v) of disjoint vertices and read only P and P^T.
+__global__ __launch_bounds__(512) void k_diag_move(int32_t *__restrict__ diag, const int32_t *__restrict__ cptr,
+                                                   const int32_t *__restrict__ cr, const int32_t *__restrict__ cw,
+                                                   const int16_t *__restrict__ Q, int N1, int ld, TouchSet ts, int nb) {
+    if ((int)blockIdx.x == nb) {
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (w >= ts.k) return;
+        const int v = ts.v[w];
+        const int d = diag_of(v, lane, cptr, cr, cw, Q, ld);
+        if (lane == 0) diag[v] = d;
+        return;
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ts.off[ts.k]) return;
+    int i = 0;
+    while (t >= ts.off[i + 1]) i++;
+    if (ts.src[i] == i) return;  // s keeps its row and column
+    const int s = ts.v[i], s0 = ts.v[ts.inv[i]];
+    const int e = cptr[s] + (t - ts.off[i]);
+    const int r = cr[e], w = cw[e];
+    const bool out = r >= N1;
+    const int v = out ? r - N1 : r;
+    for (int q = 0; q < ts.k; q++)
+        if (ts.v[q] == v) return;  // a line inside S: v is recomputed in full
+    const int o = out ? (int)Q[(size_t)s0 * ld + v] : (int)Q[(size_t)v * ld + s0];
+    const int nw = out ? (int)Q[(size_t)s * ld + v] : (int)Q[(size_t)v * ld + s];
+    const int d = min(w, nw) - min(w, o);
+    if (d) atomicAdd(&diag[v], d);
 }
 
 // The translated G2 lists (target as p^-1(target), 0xFFFF when unmapped)
@@ -504,32 +508,15 @@ static void touch_set(const std::vector<int32_t> &deg, int k, const int32_t *set
     for (int i = 0; i < k; i++) {
         ts.v[i] = set[i];
         ts.src[i] = src_of[i];
+        ts.inv[src_of[i]] = i;
         ts.off[i + 1] = ts.off[i] + deg[set[i]];
     }
 }
 
 int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_of) {
     c->map_epoch++;
-    TouchSet ts;
-    const bool diag = c->diag_valid;  // keep X(v,v) current where it changes
-    if (diag) {
-        touch_set(c->hcdeg, k, set, src_of, ts);
-        const int m = ts.off[k];
-        if (m > 0)
-            BM_LAUNCH(c, "diag_delta", hipLaunchKernelGGL(k_diag_delta, dim3((m + 255) / 256), dim3(256), 0,
-                                                          c->stream, c->diag, c->cptr, c->cr, c->cw, c->Q, c->N1,
-                                                          c->ld, ts));
-    }
+    const bool diag = c->diag_valid;      // keep X(v,v) current where it changes
     const bool dfdiag = c->dfdiag_valid;  // the same for the dense-form lists
-    TouchSet tsd;
-    if (dfdiag) {
-        touch_set(c->hdfdeg, k, set, src_of, tsd);
-        const int m = tsd.off[k];
-        if (m > 0)
-            BM_LAUNCH(c, "diag_delta", hipLaunchKernelGGL(k_diag_delta, dim3((m + 255) / 256), dim3(256), 0,
-                                                          c->stream, c->dfdiag, c->dfptr, c->dfr, c->dfw, c->Q, c->N1,
-                                                          c->ld, tsd));
-    }
     SetPerm sp;
     memset(&sp, 0, sizeof sp);
     sp.k = k;
@@ -565,12 +552,19 @@ int permute_rows_cols(bm_ctx *c, int k, const int32_t *set, const int32_t *src_o
               hipLaunchKernelGGL(k_apply_move, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream, c->Q,
                                  c->N1, c->ld, sp, c->map, c->pinv, c->sparse_ok ? 1 : 0, is, npatch, c->g2oy, c->g2iy,
                                  c->g2o, c->g2i, c->g2optr, c->g2iptr, c->g2o_at_in, c->g2i_at_out));
-    if (diag)
-        BM_LAUNCH(c, "diag_set", hipLaunchKernelGGL(k_diag_set, dim3(1), dim3(64 * k), 0, c->stream, c->diag,
-                                                    c->cptr, c->cr, c->cw, c->Q, c->ld, ts));
-    if (dfdiag)
-        BM_LAUNCH(c, "diag_set", hipLaunchKernelGGL(k_diag_set, dim3(1), dim3(64 * k), 0, c->stream, c->dfdiag,
-                                                    c->dfptr, c->dfr, c->dfw, c->Q, c->ld, tsd));
+    TouchSet ts;
+    if (diag) {
+        touch_set(c->hcdeg, k, set, src_of, ts);
+        const int nb = (ts.off[k] + 511) / 512;
+        BM_LAUNCH(c, "diag_move", hipLaunchKernelGGL(k_diag_move, dim3(nb + 1), dim3(512), 0, c->stream, c->diag,
+                                                     c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, ts, nb));
+    }
+    if (dfdiag) {
+        touch_set(c->hdfdeg, k, set, src_of, ts);
+        const int nb = (ts.off[k] + 511) / 512;
+        BM_LAUNCH(c, "diag_move", hipLaunchKernelGGL(k_diag_move, dim3(nb + 1), dim3(512), 0, c->stream, c->dfdiag,
+                                                     c->dfptr, c->dfr, c->dfw, c->Q, c->N1, c->ld, ts, nb));
+    }
     return BM_OK;
 }
 
