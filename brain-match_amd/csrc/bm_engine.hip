@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "bm_internal.h"
+#include "../host/bm_private.h"
 #include "bm_wave.h"
 
 namespace bm {
@@ -1055,8 +1056,8 @@ int bm_score(bm_ctx *c, int64_t *score) {
 
 static int check_vertex(const bm_ctx *c, int32_t v) { return v >= 0 && v <= c->n; }
 
-int bm_apply_swap_scored(bm_ctx *c, int32_t a, int32_t b, int64_t *change) {
-    if (!c || !change) return set_error(BM_EINVAL, "NULL argument");
+int bm_apply_swap_deferred(bm_ctx *c, int32_t a, int32_t b) {
+    if (!c) return set_error(BM_EINVAL, "ctx is NULL");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
     if (!check_vertex(c, a) || !check_vertex(c, b) || a == b)
         return set_error(BM_EINVAL, "apply_swap(%d,%d): bad vertices", a, b);
@@ -1068,10 +1069,22 @@ int bm_apply_swap_scored(bm_ctx *c, int32_t a, int32_t b, int64_t *change) {
     int32_t set[2] = {a, b}, src[2] = {1, 0};
     int rc = permute_rows_cols(c, 2, set, src);
     if (rc) return rc;
-    BM_HIP(hipMemcpyAsync(c->hres + 2, slot, 8, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipMemcpyAsync(c->hres + 2, slot, 8, hipMemcpyDeviceToHost, c->stream));  // pinned
+    return BM_OK;
+}
+
+int bm_deferred_change(bm_ctx *c, int64_t *change) {
+    if (!c || !change) return set_error(BM_EINVAL, "NULL argument");
+    BM_HIP(hipSetDevice(c->device));
     BM_HIP(hipStreamSynchronize(c->stream));
     *change = (int64_t)(long long)c->hres[2];
     return BM_OK;
+}
+
+int bm_apply_swap_scored(bm_ctx *c, int32_t a, int32_t b, int64_t *change) {
+    if (!c || !change) return set_error(BM_EINVAL, "NULL argument");
+    const int rc = bm_apply_swap_deferred(c, a, b);
+    return rc ? rc : bm_deferred_change(c, change);
 }
 
 int bm_apply_swap(bm_ctx *c, int32_t a, int32_t b) {
@@ -1081,10 +1094,7 @@ int bm_apply_swap(bm_ctx *c, int32_t a, int32_t b) {
         return set_error(BM_EINVAL, "apply_swap(%d,%d): bad vertices", a, b);
     BM_HIP(hipSetDevice(c->device));
     int32_t set[2] = {a, b}, src[2] = {1, 0};
-    int rc = permute_rows_cols(c, 2, set, src);
-    if (rc) return rc;
-    BM_HIP(hipStreamSynchronize(c->stream));
-    return BM_OK;
+    return permute_rows_cols(c, 2, set, src);  // enqueued: every call that returns data waits for it
 }
 
 // triple.cu:903-934: new (p(m1),p(m2),p(m3)) = table[p] applied to (f1,f2,f3)
@@ -1098,10 +1108,7 @@ int bm_apply_rot3(bm_ctx *c, int32_t m1, int32_t m2, int32_t m3, int32_t p) {
     if (p == 0) return BM_OK;
     BM_HIP(hipSetDevice(c->device));
     int32_t set[3] = {m1, m2, m3}, src[3] = {tab[p][0], tab[p][1], tab[p][2]};
-    int rc = permute_rows_cols(c, 3, set, src);
-    if (rc) return rc;
-    BM_HIP(hipStreamSynchronize(c->stream));
-    return BM_OK;
+    return permute_rows_cols(c, 3, set, src);  // enqueued: every call that returns data waits for it
 }
 
 int bm_apply_perm(bm_ctx *c, int32_t k, const int32_t *set, const int32_t *perm) {
@@ -1118,10 +1125,7 @@ int bm_apply_perm(bm_ctx *c, int32_t k, const int32_t *set, const int32_t *perm)
         seen |= 1 << perm[i];
     }
     BM_HIP(hipSetDevice(c->device));
-    int rc = permute_rows_cols(c, k, set, perm);
-    if (rc) return rc;
-    BM_HIP(hipStreamSynchronize(c->stream));
-    return BM_OK;
+    return permute_rows_cols(c, k, set, perm);  // enqueued: every call that returns data waits for it
 }
 
 uint64_t bm_rng_next(uint64_t *state) {

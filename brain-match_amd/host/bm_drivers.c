@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include "bm_host.h"
+#include "bm_private.h"
 
 static void record(int32_t *acc, int64_t cap, int64_t i, int32_t a, int32_t b, int32_t d) {
     if (acc && i < cap) {
@@ -17,21 +18,42 @@ static void record(int32_t *acc, int64_t cap, int64_t i, int32_t a, int32_t b, i
     }
 }
 
-/* apply, advance the score by the delta, and verify it like
- * graph-alignment-optimized.c:570-576: the verified score (the exact score
- * change of the swap added to the previous score, which equals the full
- * rescore there) is trusted */
-static int apply_and_verify(bm_ctx *ctx, int32_t a, int32_t b, int32_t d, int verify, bm_drive_stats *st) {
-    const int64_t before = st->final_score;
-    int64_t change = 0;
-    int rc = verify ? bm_apply_swap_scored(ctx, a, b, &change) : bm_apply_swap(ctx, a, b);
+/* A verified swap whose exact score change has not been read yet. */
+typedef struct {
+    int pending;
+    int64_t before; /* the score before the swap */
+} pending_check;
+
+/* the check of graph-alignment-optimized.c:570-576 for the last verified
+ * swap: the verified score (the exact score change of the swap added to the
+ * previous score, which equals the full rescore there) is trusted.  Waits for
+ * the engine's stream if the change is not in yet; the drivers call it after
+ * their next evaluation, which has waited already. */
+static int settle(bm_ctx *ctx, bm_drive_stats *st, pending_check *pc) {
+    if (!pc->pending) return BM_OK;
+    int64_t change;
+    int rc = bm_deferred_change(ctx, &change);
     if (rc) return rc;
+    pc->pending = 0;
+    if (pc->before + change != st->final_score) {
+        st->mismatches++;
+        st->final_score = pc->before + change;
+    }
+    return BM_OK;
+}
+
+/* apply and advance the score by the delta; with verify, the exact change is
+ * enqueued and checked by the next settle() */
+static int apply_and_verify(bm_ctx *ctx, int32_t a, int32_t b, int32_t d, int verify, bm_drive_stats *st,
+                            pending_check *pc) {
+    int rc = verify ? bm_apply_swap_deferred(ctx, a, b) : bm_apply_swap(ctx, a, b);
+    if (rc) return rc;
+    if (verify) {
+        pc->pending = 1;
+        pc->before = st->final_score;
+    }
     st->final_score += d;
     st->accepted++;
-    if (verify && before + change != st->final_score) {
-        st->mismatches++;
-        st->final_score = before + change;
-    }
     return BM_OK;
 }
 
@@ -46,6 +68,7 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
     int rc = bm_score(ctx, &st.initial_score);
     if (rc) return rc;
     st.final_score = st.initial_score;
+    pending_check pc = {0, 0};
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
@@ -60,12 +83,14 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
             while (cursor <= n) {
                 int32_t b, d;
                 if ((rc = bm_eval_swap_row(ctx, a, cursor, n, BM_ROW_FIRST_POSITIVE, &b, &d))) return rc;
+                if ((rc = settle(ctx, &st, &pc))) return rc;
                 st.evals_rows++;
                 if (b == 0) break;
                 record(acc, acc_cap, st.accepted, a, b, d);
-                if ((rc = apply_and_verify(ctx, a, b, d, verify, &st))) return rc;
+                if ((rc = apply_and_verify(ctx, a, b, d, verify, &st, &pc))) return rc;
                 cursor = b + 1;
             }
+            if ((rc = settle(ctx, &st, &pc))) return rc;
         }
         st.sweeps++;
         if (st.accepted == before) break;
@@ -84,11 +109,13 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
     int rc = bm_score(ctx, &st.initial_score);
     if (rc) return rc;
     st.final_score = st.initial_score;
+    pending_check pc = {0, 0};
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
         for (int32_t a = 1; a <= n; a++) {
             st.rows++;
+            if (hook && (rc = settle(ctx, &st, &pc))) return rc;  /* the hook sees the verified score */
             if (hook && hook(user, st.rows, a, st.final_score, st.accepted)) {
                 stop = 1;
                 break;
@@ -96,12 +123,14 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
             if (a == n) continue;
             int32_t b, d;
             if ((rc = bm_eval_swap_row(ctx, a, a + 1, n, BM_ROW_ARGMAX, &b, &d))) return rc;
+            if ((rc = settle(ctx, &st, &pc))) return rc;
             st.evals_rows++;
             if (d > 0) {
                 record(acc, acc_cap, st.accepted, a, b, d);
-                if ((rc = apply_and_verify(ctx, a, b, d, verify, &st))) return rc;
+                if ((rc = apply_and_verify(ctx, a, b, d, verify, &st, &pc))) return rc;
             }
         }
+        if ((rc = settle(ctx, &st, &pc))) return rc;
         st.sweeps++;
         if (st.accepted == before) break;
     }
@@ -118,16 +147,19 @@ int bm_drive_global_best(bm_ctx *ctx, int32_t kind, int64_t max_steps, int verif
     int rc = bm_score(ctx, &st.initial_score);
     if (rc) return rc;
     st.final_score = st.initial_score;
+    pending_check pc = {0, 0};
     while (max_steps <= 0 || st.sweeps < max_steps) {
         int32_t a, b, d;
         rc = kind == BM_DELTA_DENSE ? bm_sweep_all_pairs_dense(ctx, &a, &b, &d, NULL, NULL, NULL)
                                     : bm_sweep_all_pairs(ctx, &a, &b, &d, NULL, NULL, NULL);
         if (rc) return rc;
+        if ((rc = settle(ctx, &st, &pc))) return rc;
         st.sweeps++;
         if (d <= 0) break;
         record(acc, acc_cap, st.accepted, a, b, d);
-        if ((rc = apply_and_verify(ctx, a, b, d, verify, &st))) return rc;
+        if ((rc = apply_and_verify(ctx, a, b, d, verify, &st, &pc))) return rc;
     }
+    if ((rc = settle(ctx, &st, &pc))) return rc;
     if (stats) *stats = st;
     return BM_OK;
 }
@@ -313,18 +345,19 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
                 zerocheck++;
                 st.zero_moves++;
             }
-            int64_t change;
-            if ((rc = bm_apply_swap_scored(ctx, m1, m2, &change))) goto out;
-            cur += change; /* :1248, the rescore after the swap */
+            if ((rc = bm_apply_swap_deferred(ctx, m1, m2))) goto out;
             st.moves++;
+            changed[0] = m1 < m2 ? m1 : m2;
+            changed[1] = m1 < m2 ? m2 : m1;
+            if ((rc = quick_eval(ctx, n, changed, 2, p->delta_kind, &q))) goto out;
+            int64_t change; /* in by now: the rows' copy back waited for the swap */
+            if ((rc = bm_deferred_change(ctx, &change))) goto out;
+            cur += change; /* :1248, the rescore after the swap */
             if (cur > st.best_score) {
                 st.best_score = cur;
                 if ((rc = bm_get_mapping(ctx, best))) goto out;
                 last = iter;
             }
-            changed[0] = m1 < m2 ? m1 : m2;
-            changed[1] = m1 < m2 ? m2 : m1;
-            if ((rc = quick_eval(ctx, n, changed, 2, p->delta_kind, &q))) goto out;
             iter++;
         }
     }

@@ -194,7 +194,8 @@ int bm_eval_swaps_hillclimb(bm_ctx *ctx, const int32_t *a, const int32_t *b, int
  * apply_rot3: apply_permutation, triple.cu:903-934 (p in 1..5);
  * apply_perm: apply_best_permutation,
  * graph-alignment-permset-parallel.c:508-518 (new p(set[i]) =
- * old p(set[perm[i]])). */
+ * old p(set[perm[i]])).  They return once the move is enqueued on the
+ * context's stream; every later call on the context sees it. */
 int bm_apply_swap(bm_ctx *ctx, int32_t a, int32_t b);
 /* bm_apply_swap that also returns the exact score change S(after) - S(before),
  * summed over the G1 lines with an end at a or b (each line once): what the
