@@ -227,51 +227,102 @@ out:
 /* Pairs quick.cu's kernel evaluates after a change (:151-196), in its
  * numbering: inside the sorted changed set (i < j), then each changed vertex
  * with every unchanged one ascending.  Scans them like :1172-1191 / :1280-1299:
- * the first strict maximum above 0, the positive pairs and the zero pairs in
- * scan order. */
+ * the first strict maximum above 0, and the number of positive and of zero
+ * pairs; the pick-th positive or zero pair in that order is found by a second
+ * scan when the search draws one (quick_pick), so the first scan keeps no
+ * lists and takes no data-dependent branch per pair. */
 typedef struct {
-    int32_t *rows;       /* nc * (n + 1) deltas from bm_eval_rows */
-    int32_t *pos, *zero; /* pairs (2 ints each) */
+    int32_t *rows;       /* nc * (n + 1) deltas from bm_eval_rows; changed columns cleared */
+    int32_t n, nc;
+    int32_t changed[8];
+    int32_t inside[28];  /* the deltas inside the changed set, in scan order */
     int64_t npos, nzero, pairs;
     int32_t best_a, best_b, max_delta;
-    uint8_t *in_set;
 } quick_scan;
 
 static int quick_eval(bm_ctx *ctx, int32_t n, const int32_t *changed, int nc, int32_t kind, quick_scan *q) {
     int rc = bm_eval_rows(ctx, changed, nc, kind, q->rows);
     if (rc) return rc;
     const size_t N1 = (size_t)n + 1;
+    q->n = n;
+    q->nc = nc;
+    memcpy(q->changed, changed, sizeof(int32_t) * (size_t)nc);
     q->pairs += (int64_t)nc * (nc - 1) / 2 + (int64_t)nc * (n - nc);
-    q->npos = q->nzero = 0;
-    q->max_delta = 0;
-    q->best_a = q->best_b = 0;
-    for (int i = 0; i < nc; i++) q->in_set[changed[i]] = 1;
-#define QUICK_PAIR(A, B, D)                     \
-    do {                                        \
-        const int32_t d_ = (D);                 \
-        if (d_ > q->max_delta) {                \
-            q->max_delta = d_;                  \
-            q->best_a = (A);                    \
-            q->best_b = (B);                    \
-        }                                       \
-        if (d_ > 0) {                           \
-            q->pos[2 * q->npos] = (A);          \
-            q->pos[2 * q->npos + 1] = (B);      \
-            q->npos++;                          \
-        } else if (d_ == 0) {                   \
-            q->zero[2 * q->nzero] = (A);        \
-            q->zero[2 * q->nzero + 1] = (B);    \
-            q->nzero++;                         \
-        }                                       \
-    } while (0)
+    int64_t npos = 0, nzero = 0;
+    int32_t maxd = 0, ba = 0, bb = 0;
+    int m = 0;
     for (int i = 0; i < nc; i++)
-        for (int j = i + 1; j < nc; j++) QUICK_PAIR(changed[i], changed[j], q->rows[i * N1 + changed[j]]);
+        for (int j = i + 1; j < nc; j++) {
+            const int32_t d = q->rows[i * N1 + changed[j]];
+            q->inside[m++] = d;
+            if (d > maxd) {
+                maxd = d;
+                ba = changed[i];
+                bb = changed[j];
+            }
+            npos += d > 0;
+            nzero += d == 0;
+        }
+    /* the changed columns are not unchanged vertices: neither positive nor zero */
     for (int i = 0; i < nc; i++)
-        for (int32_t v = 1; v <= n; v++)
-            if (!q->in_set[v]) QUICK_PAIR(changed[i], v, q->rows[i * N1 + v]);
-#undef QUICK_PAIR
-    for (int i = 0; i < nc; i++) q->in_set[changed[i]] = 0;
+        for (int j = 0; j < nc; j++) q->rows[i * N1 + changed[j]] = INT32_MIN;
+    for (int i = 0; i < nc; i++) {
+        const int32_t *r = q->rows + i * N1;
+        int64_t cp = 0, cz = 0;
+        int32_t rmax = INT32_MIN;
+        for (int32_t v = 1; v <= n; v++) {
+            const int32_t d = r[v];
+            cp += d > 0;
+            cz += d == 0;
+            rmax = d > rmax ? d : rmax;
+        }
+        npos += cp;
+        nzero += cz;
+        if (rmax > maxd) { /* the first v holding the row's maximum */
+            int32_t v = 1;
+            while (r[v] != rmax) v++;
+            maxd = rmax;
+            ba = changed[i];
+            bb = v;
+        }
+    }
+    q->npos = npos;
+    q->nzero = nzero;
+    q->max_delta = maxd;
+    q->best_a = ba;
+    q->best_b = bb;
     return BM_OK;
+}
+
+/* the pick-th positive (positive = 1) or zero pair of the last scan, in scan order */
+static void quick_pick(const quick_scan *q, int positive, int64_t pick, int32_t *a, int32_t *b) {
+    const size_t N1 = (size_t)q->n + 1;
+    int64_t k = 0;
+    int m = 0;
+    for (int i = 0; i < q->nc; i++)
+        for (int j = i + 1; j < q->nc; j++) {
+            const int32_t d = q->inside[m++];
+            if (positive ? d > 0 : d == 0) {
+                if (k++ == pick) {
+                    *a = q->changed[i];
+                    *b = q->changed[j];
+                    return;
+                }
+            }
+        }
+    for (int i = 0; i < q->nc; i++) {
+        const int32_t *r = q->rows + i * N1;
+        for (int32_t v = 1; v <= q->n; v++) {
+            const int32_t d = r[v];
+            k += positive ? d > 0 : d == 0;
+            if (k > pick) {
+                *a = q->changed[i];
+                *b = v;
+                return;
+            }
+        }
+    }
+    *a = *b = 0; /* not reached: pick < the count of the scan */
 }
 
 static void sort_small(int32_t *a, int m) {
@@ -293,16 +344,12 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
     bm_quick_stats st;
     memset(&st, 0, sizeof st);
     const size_t N1 = (size_t)n + 1;
-    const int64_t max_pairs = 8 * (int64_t)n;
     quick_scan q;
     memset(&q, 0, sizeof q);
     int32_t *best = malloc(sizeof(int32_t) * N1);
     q.rows = malloc(sizeof(int32_t) * 8 * N1);
-    q.pos = malloc(sizeof(int32_t) * 2 * (size_t)max_pairs);
-    q.zero = malloc(sizeof(int32_t) * 2 * (size_t)max_pairs);
-    q.in_set = calloc(N1, 1);
     int rc = BM_OK;
-    if (!best || !q.rows || !q.pos || !q.zero || !q.in_set) {
+    if (!best || !q.rows) {
         rc = BM_ENOMEM;
         goto out;
     }
@@ -330,8 +377,7 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
             if (q.max_delta > 0) {
                 if (bm_rng_next(&rng) % 2 == 0) {
                     const int64_t pick = (int64_t)(bm_rng_next(&rng) % (uint64_t)q.npos);
-                    m1 = q.pos[2 * pick];
-                    m2 = q.pos[2 * pick + 1];
+                    quick_pick(&q, 1, pick, &m1, &m2);
                 } else {
                     m1 = q.best_a;
                     m2 = q.best_b;
@@ -340,8 +386,7 @@ int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *p, int32_t *best_mapping,
                 st.positive_moves++;
             } else {
                 const int64_t pick = (int64_t)(bm_rng_next(&rng) % (uint64_t)q.nzero);
-                m1 = q.zero[2 * pick];
-                m2 = q.zero[2 * pick + 1];
+                quick_pick(&q, 0, pick, &m1, &m2);
                 zerocheck++;
                 st.zero_moves++;
             }
@@ -370,8 +415,5 @@ out:
     if (!rc && stats) *stats = st;
     free(best);
     free(q.rows);
-    free(q.pos);
-    free(q.zero);
-    free(q.in_set);
     return rc;
 }
