@@ -713,6 +713,7 @@ int bm_destroy(bm_ctx *c) {
     }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->hres) (void)hipHostFree(c->hres);
+    if (c->hrows) (void)hipHostFree(c->hrows);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BM_OK;

@@ -123,6 +123,8 @@ struct bm_ctx {
     unsigned long long *keys = nullptr;  // N1 + 8 reduction slots
     void *stage = nullptr;               // move-list staging (lazy, grows)
     unsigned long long *hres = nullptr;  // pinned host slots for small results (8 x 8 bytes)
+    void *hrows = nullptr;               // pinned host copy of bm_eval_rows' output (grown on demand)
+    size_t hrows_bytes = 0;
     size_t stage_bytes = 0;
     int64_t dev_bytes = 0;
 

@@ -1281,8 +1281,22 @@ extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32
             }
         }
     }
-    BM_HIP(hipMemcpyAsync(deltas, out, (size_t)count * N1 * 4, hipMemcpyDeviceToHost, c->stream));
+    // through a pinned buffer: a copy into pageable memory is staged by the
+    // runtime in pieces, several times slower for rows of this size
+    const size_t bytes = (size_t)count * N1 * 4;
+    if (c->hrows_bytes < bytes) {
+        BM_HIP(hipStreamSynchronize(c->stream));
+        std::lock_guard<std::mutex> g(sync_mutex());  // host allocations may wait for the device
+        if (c->hrows) (void)hipHostFree(c->hrows);
+        c->hrows = nullptr;
+        c->hrows_bytes = 0;
+        const size_t want = std::max(bytes, (size_t)8 * N1 * 4);
+        BM_HIP(hipHostMalloc(&c->hrows, want, hipHostMallocDefault));
+        c->hrows_bytes = want;
+    }
+    BM_HIP(hipMemcpyAsync(c->hrows, out, bytes, hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipStreamSynchronize(c->stream));
+    memcpy(deltas, c->hrows, bytes);
     return BM_OK;
 }
 
