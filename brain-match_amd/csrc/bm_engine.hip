@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_diag(int32_t *__restrict__ diag, const 
 }
 
 // Exact S_opt change of swapping the images of a and b, before the swap: the
-// G1 lines with an end at a or b, old term min(w, P[x][y]) from the current
+// G1 lines with an end at a or b and a source other than vertex 0, old term min(w, P[x][y]) from the current
 // P, new term min(w, W2[p'(x)][p'(y)]) at the swapped images.  Each line
 // once: the out entries of a and b, then their in entries whose source is
 // neither a nor b.  One workgroup; the total goes to *out.
@@ -258,6 +258,7 @@ __global__ __launch_bounds__(256) void k_swap_change(long long *__restrict__ out
             const bool out_entry = rw.x >= N1;  // v -> rw.x - N1, else rw.x -> v
             const int x = out_entry ? v : rw.x, y = out_entry ? rw.x - N1 : v;
             if (!out_entry && (x == a || x == b)) continue;  // counted as an out entry of x
+            if (x == 0) continue;                              // lines leaving vertex 0 are not scored
             const int px = x == a ? pb : x == b ? pa : map[x];
             const int py = y == a ? pb : y == b ? pa : map[y];
             s += min(rw.y, (int)W2[(size_t)px * ld + py]) - min(rw.y, (int)Q[(size_t)x * ld + y]);
@@ -269,13 +270,16 @@ __global__ __launch_bounds__(256) void k_swap_change(long long *__restrict__ out
     if (threadIdx.x == 0) *out = part[0] + part[1] + part[2] + part[3];
 }
 
-// S_opt = sum over G1 lines (x->y, w) of min(w, P[x][y]).
+// S_opt = sum over G1 lines (x->y, w), x >= 1, of min(w, P[x][y]): the flat
+// list is in source order, so the lines of vertex 0 are its first `lo`
+// entries, which calculate_alignment_score (graph-alignment-optimized.c:295)
+// never visits.
 __global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ out, const int32_t *__restrict__ es,
                                                const int32_t *__restrict__ ed, const int32_t *__restrict__ ew,
-                                               int64_t m, const int16_t *__restrict__ P, int ld) {
+                                               int64_t lo, int64_t m, const int16_t *__restrict__ P, int ld) {
     __shared__ long long part[4];
     long long s = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
         s += min(ew[i], (int)P[(size_t)es[i] * ld + ed[i]]);
     s = bm_wave::wave_sum64(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
@@ -480,10 +484,10 @@ int launch_diag(bm_ctx *c) {
 // S_opt into *out (which must hold 0 beforehand); no host synchronisation
 int launch_score(bm_ctx *c, unsigned long long *out) {
     int64_t m = c->e1;
-    int blocks = (int)std::min<int64_t>((m + 255) / 256, 4096);
+    int blocks = (int)std::min<int64_t>((m - c->e1_v0 + 255) / 256, 4096);
     if (blocks < 1) blocks = 1;
     BM_LAUNCH(c, "score", hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, c->stream, out, c->esrc, c->edst,
-                                             c->ew, m, c->Q, c->ld));
+                                             c->ew, c->e1_v0, m, c->Q, c->ld));
     return BM_OK;
 }
 
@@ -847,7 +851,9 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
                 cdw[e] = (int16_t)cw[e];
             }
     }
-    // flat out-edge list in CSR order (score, all-pairs corrections)
+    // flat out-edge list in CSR order (score, all-pairs corrections); the
+    // lines of vertex 0 come first and the score skips them
+    c->e1_v0 = cnt_out[0];
     std::vector<int32_t> es(e1), ed(e1), ewv(e1), edwv(e1);
     {
         int64_t k = 0;
@@ -952,10 +958,13 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = upload(c, &c->g2i_at_out, g2i_at_out))) goto fail;
     {
         // vertices on which every delta kind equals the exact score change:
-        // no self-loop line and no repeated (u,v) line touches them
+        // no self-loop line and no repeated (u,v) line touches them, and no
+        // line leaves vertex 0 for them (the deltas count such a line as an
+        // in-entry, S_opt does not score it)
         std::vector<uint8_t> simple(N1, 1);
         for (int64_t i = 0; i < e1; i++)
-            if (s1[i] == d1[i]) simple[s1[i]] = 0;
+            if (s1[i] == d1[i] || s1[i] == 0) simple[d1[i]] = 0;
+        simple[0] = 0;
         for (int v = 0; v < N1; v++)
             if (ddptr[v + 1] - ddptr[v] != cptr[v + 1] - cptr[v]) simple[v] = 0;
         if ((rc = upload(c, &c->vsimple, simple))) goto fail;
@@ -1045,10 +1054,10 @@ int bm_score(bm_ctx *c, int64_t *score) {
     unsigned long long *out = c->keys + c->N1 + 2;
     BM_HIP(hipMemsetAsync(out, 0, 8, c->stream));
     int64_t m = c->e1;
-    int blocks = (int)std::min<int64_t>((m + 255) / 256, 4096);
+    int blocks = (int)std::min<int64_t>((m - c->e1_v0 + 255) / 256, 4096);
     if (blocks < 1) blocks = 1;
     BM_LAUNCH(c, "score", hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, c->stream, out, c->esrc, c->edst,
-                                             c->ew, m, c->Q, c->ld));
+                                             c->ew, c->e1_v0, m, c->Q, c->ld));
     BM_HIP(hipMemcpyAsync(c->hres, out, 8, hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipStreamSynchronize(c->stream));
     *score = (int64_t)c->hres[0];

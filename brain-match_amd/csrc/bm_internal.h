@@ -50,6 +50,7 @@ struct bm_ctx {
     int32_t N1 = 0;   // n + 1
     int32_t ld = 0;   // row stride (elements) of W2 and Q, multiple of 64
     int64_t e1 = 0;   // G1 CSV lines
+    int64_t e1_v0 = 0; // of them, lines leaving vertex 0: the head of the flat out-line list, not scored
     int64_t ncomb = 0;
     int64_t ndd = 0;
     bool graphs = false;

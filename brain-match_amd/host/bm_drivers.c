@@ -18,6 +18,18 @@ static void record(int32_t *acc, int64_t cap, int64_t i, int32_t a, int32_t b, i
     }
 }
 
+static __thread bm_mismatch_hook mismatch_hook;
+static __thread void *mismatch_user;
+
+void bm_drive_on_mismatch(bm_mismatch_hook hook, void *user) {
+    mismatch_hook = hook;
+    mismatch_user = user;
+}
+
+static void report_mismatch(int64_t calculated, int64_t verified) {
+    if (mismatch_hook) mismatch_hook(mismatch_user, calculated, verified);
+}
+
 /* A verified swap whose exact score change has not been read yet. */
 typedef struct {
     int pending;
@@ -37,6 +49,7 @@ static int settle(bm_ctx *ctx, bm_drive_stats *st, pending_check *pc) {
     pc->pending = 0;
     if (pc->before + change != st->final_score) {
         st->mismatches++;
+        report_mismatch(st->final_score, pc->before + change);
         st->final_score = pc->before + change;
     }
     return BM_OK;
@@ -197,6 +210,7 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *
             st.accepted++;
             if (p->verify && verified != cur) { /* :652-656, the verified score is trusted */
                 st.mismatches++;
+                report_mismatch(cur, verified);
                 cur = verified;
             }
             if (cur > st.best_score) { /* :627-636 */

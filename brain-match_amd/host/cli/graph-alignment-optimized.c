@@ -9,10 +9,11 @@
  * (:682-738): first-improvement pair swaps (:513-581), a progress block
  * every 20 rows (:519-534), the mapping rewritten and re-read every 25 rows
  * (:536-540, :478-492), a device rescore after every accepted swap
- * (:570-576).  Differences, all deliberate:
+ * with the reference's per-swap mismatch line (:570-576).  Differences, all deliberate:
  *   - n is the largest vertex id in the inputs instead of NUM_NODES = 18524
- *     (set BM_ROWS_PER_SWEEP=18524 to reproduce the reference's row counter
- *     for smaller graphs; extra rows have no edges and never accept);
+ *     (set BM_ROWS_PER_SWEEP=18524 to reproduce the reference's vertex range
+ *     for smaller graphs: its rows, its row counter and the swaps with the
+ *     empty vertices above the data, which pay when G2 has lines at vertex 0);
  *   - the run ends after a sweep without an accepted swap (the reference
  *     loops forever, rewriting the same converged mapping);
  *   - the ordering files must exist, as in the reference, but are unused
@@ -90,8 +91,18 @@ static void save_intermediate(run_t *r, int64_t current_score) {
         bm_score(r->ctx, &written);
         bm_set_mapping(r->ctx, r->scratch);
         if (written != current_score)
-            LOG_ERROR("Score mismatch - internal: %lld, written: %lld", (long long)current_score, (long long)written);
+            fprintf(stderr, "[ERROR] save_intermediate_mapping:487: Score mismatch - internal: %lld, written: %lld\n",
+                    (long long)current_score, (long long)written);
     }
+}
+
+/* :570-576, logged at each accepted swap whose delta the rescore corrects
+ * (lines leaving vertex 0, self-loops, repeated lines); the message names
+ * the reference's function and line as its LOG_ERROR does */
+static void mismatch_hook(void *user, int64_t calculated, int64_t verified) {
+    (void)user;
+    fprintf(stderr, "[ERROR] optimize_mapping:573: Score mismatch after swap - calculated: %lld, verified: %lld\n",
+            (long long)calculated, (long long)verified);
 }
 
 static int row_hook(void *user, int64_t rows, int32_t row, int64_t score, int64_t improvements) {
@@ -154,6 +165,10 @@ int main(int argc, char *argv[]) {
     if (gm.max_id > n) n = gm.max_id;
     if (gf.max_id > n) n = gf.max_id;
     if (pairs.max_id > n) n = pairs.max_id;
+    /* BM_ROWS_PER_SWEEP: the reference's fixed vertex range (NUM_NODES, :15);
+     * its rows above the data are empty vertices that a swap can still reach */
+    const int64_t rows_env = env_i64("BM_ROWS_PER_SWEEP", 0);
+    if (rows_env > n && rows_env <= INT32_MAX - 1) n = (int32_t)rows_env;
     int32_t *mapping = calloc((size_t)n + 1, 4);
     bm_pairs_to_mapping(&pairs, n, mapping);
     bm_pairs_free(&pairs);
@@ -187,16 +202,13 @@ int main(int argc, char *argv[]) {
     r.reread = calloc((size_t)n + 1, 4);
     LOG_INFO("Starting optimization with initial score: %s", fmt_num(initial_score));
     bm_drive_stats st;
+    bm_drive_on_mismatch(mismatch_hook, NULL);
     rc = bm_drive_first_improvement(ctx, r.rows_per_sweep, env_i64("BM_MAX_SWEEPS", 0), (int)env_i64("BM_VERIFY", 1),
                                     row_hook, &r, NULL, 0, &st);
     if (rc) {
         LOG_ERROR("engine: %s (%s)", bm_strerror(rc), bm_last_error());
         return 1;
     }
-    if (st.mismatches)
-        LOG_ERROR("Score mismatch after swap: %lld accepted swaps disagreed with the rescore (self-loops or repeated "
-                  "lines); the rescored value was kept",
-                  (long long)st.mismatches);
     LOG_INFO("Optimization completed:");
     LOG_INFO("  - Final score: %s", fmt_num(st.final_score));
     LOG_INFO("  - Total improvements: %s", fmt_num(st.accepted));

@@ -116,6 +116,16 @@ int bm_gen_sets(uint64_t seed, int32_t n, int32_t k, int64_t count, int32_t *out
  * A non-zero return stops the driver. */
 typedef int (*bm_row_hook)(void *user, int64_t rows, int32_t row, int64_t score, int64_t improvements);
 
+/* Per-swap mismatch hook of the drivers: called when a verified swap's exact
+ * score change disagrees with its accepted delta, with the score the delta
+ * gave (calculated) and the verified score that replaces it, at the point
+ * where graph-alignment-optimized.c:570-576 logs "Score mismatch after swap -
+ * calculated: %d, verified: %d" (also alignment-omp.c:652-656).  Registered
+ * per calling thread for the drivers it calls afterwards; NULL turns it off.
+ * No counterpart function in the reference. */
+typedef void (*bm_mismatch_hook)(void *user, int64_t calculated, int64_t verified);
+void bm_drive_on_mismatch(bm_mismatch_hook hook, void *user);
+
 typedef struct {
     int64_t sweeps, rows, accepted, evals_rows;
     int64_t initial_score, final_score;

@@ -169,10 +169,13 @@ void orc_free(orc_graphs *g) {
 #define W1(g, x, y) ((int)(g)->w1d[(size_t)(x) * ((size_t)(g)->n + 1) + (size_t)(y)])
 #define W2(g, x, y) ((int)(g)->w2d[(size_t)(x) * ((size_t)(g)->n + 1) + (size_t)(y)])
 
-/* graph-alignment-optimized.c:292-307 */
+/* graph-alignment-optimized.c:292-307: the loop starts at src_m = 1 (:295),
+ * so lines leaving vertex 0 are never scored (Delta_B still counts them as
+ * in-entries of their target, which is why the reference's verify step at
+ * :570-576 can disagree with an accepted delta on such input) */
 int64_t orc_score_opt(const orc_graphs *g, const int32_t *map) {
     int64_t s = 0;
-    for (int u = 0; u <= g->n; u++)
+    for (int u = 1; u <= g->n; u++)
         for (int64_t i = g->out_ptr[u]; i < g->out_ptr[u + 1]; i++) {
             int v = g->out_dst[i];
             s += OMIN(g->out_w[i], W2(g, map[u], map[v]));
@@ -648,6 +651,100 @@ int64_t orc_omp_rowbest_rows(const orc_graphs *g, const int32_t *map, int row_lo
     }
     if (checksum) *checksum = cs;
     return evals;
+}
+
+/* ---- the same arithmetic over many rows / moves ------------------------ */
+
+static void set_threads(int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}
+
+void orc_row_best_rows(const orc_graphs *g, const int32_t *map, const int32_t *rows, int64_t count,
+                       int32_t *best_b, int32_t *best_d, int threads) {
+    set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < count; i++) {
+        int a = rows[i];
+        int32_t rb = INT32_MIN, rbb = 0;
+        for (int b = a + 1; b <= g->n; b++) {
+            int32_t d = orc_delta_b(g, map, a, b);
+            if (d > rb) {
+                rb = d;
+                rbb = b;
+            }
+        }
+        best_b[i] = rbb;
+        best_d[i] = rb;
+    }
+}
+
+void orc_delta_rows(const orc_graphs *g, const int32_t *map, const int32_t *rows, int64_t count, int32_t *out,
+                    int threads) {
+    const int64_t N1 = (int64_t)g->n + 1;
+    set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t t = 0; t < count * N1; t++) {
+        int64_t i = t / N1;
+        int b = (int)(t % N1), a = rows[i];
+        out[t] = (b == 0 || b == a) ? 0 : orc_delta_b(g, map, a, b);
+    }
+}
+
+int64_t orc_first_improvement_rows(const orc_graphs *g, int32_t *map, int max_rows, int32_t *acc, int64_t acc_cap,
+                                   int threads) {
+    const int n = g->n, BLOCK = 1024;
+    int32_t *d = xcalloc(BLOCK, sizeof(int32_t));
+    int64_t na = 0;
+    set_threads(threads);
+    for (int m1 = 1; m1 <= max_rows && m1 <= n; m1++) {
+        int cursor = 1;
+        while (cursor <= n) {
+            int hi = cursor + BLOCK - 1 > n ? n : cursor + BLOCK - 1;
+#pragma omp parallel for schedule(static)
+            for (int m2 = cursor; m2 <= hi; m2++) d[m2 - cursor] = m2 == m1 ? 0 : orc_delta_b(g, map, m1, m2);
+            int hit = 0;
+            for (int m2 = cursor; m2 <= hi; m2++)
+                if (d[m2 - cursor] > 0) {
+                    int32_t t = map[m1];
+                    map[m1] = map[m2];
+                    map[m2] = t;
+                    if (acc && na < acc_cap) {
+                        acc[3 * na] = m1;
+                        acc[3 * na + 1] = m2;
+                        acc[3 * na + 2] = d[m2 - cursor];
+                    }
+                    na++;
+                    cursor = m2 + 1;
+                    hit = 1;
+                    break;
+                }
+            if (!hit) cursor = hi + 1;
+        }
+    }
+    free(d);
+    return na;
+}
+
+void orc_best_g3_many(const orc_graphs *g, const int32_t *map, const int32_t *triples, int64_t count, int mode,
+                      int32_t *best_d, int32_t *best_p, int threads) {
+    set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < count; i++) {
+        int p;
+        best_d[i] = orc_best_g3(g, map, triples[3 * i], triples[3 * i + 1], triples[3 * i + 2], mode, &p);
+        best_p[i] = p;
+    }
+}
+
+void orc_best_perm_many(const orc_graphs *g, const int32_t *map, const int32_t *sets, int k, int64_t count,
+                        int32_t *best_d, int64_t *best_rank, int threads) {
+    set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < count; i++) best_d[i] = orc_best_perm(g, map, sets + (size_t)k * i, k, &best_rank[i]);
 }
 
 /* ------------------------------------------------------------------------

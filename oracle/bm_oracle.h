@@ -118,6 +118,32 @@ int32_t orc_all_pairs(const orc_graphs *g, const int32_t *map, int32_t *best_a, 
 int64_t orc_omp_rowbest_rows(const orc_graphs *g, const int32_t *map, int row_lo, int row_hi, int threads,
                              int64_t *checksum);
 
+/* --- the same arithmetic over many rows / moves (OpenMP, for the checks at
+ * the full-size configurations; threads <= 0 uses the OpenMP default) ----- */
+
+/* For each rows[i]: the arg-max over b > rows[i] of Delta_B (strict > from
+ * INT32_MIN, lowest b on ties), as orc_all_pairs' row_best_b/row_best_d. */
+void orc_row_best_rows(const orc_graphs *g, const int32_t *map, const int32_t *rows, int64_t count,
+                       int32_t *best_b, int32_t *best_d, int threads);
+/* Whole rows: out[i*(n+1) + b] = Delta_B(rows[i], b) for b in [1, n], b != rows[i];
+ * entries b = 0 and b = rows[i] are 0 (bm_eval_rows' layout). */
+void orc_delta_rows(const orc_graphs *g, const int32_t *map, const int32_t *rows, int64_t count, int32_t *out,
+                    int threads);
+/* The first max_rows rows of the first-improvement sweep of
+ * graph-alignment-optimized.c:513-581 (m2 = 1..n from the row's cursor,
+ * apply at the first Delta_B > 0, resume at m2+1), without the per-accept
+ * rescore: on graphs without self-loops, repeated lines or lines at vertex 0
+ * the delta is the exact change, so the rescore changes nothing.  Each
+ * cursor's candidates are evaluated in parallel blocks and scanned in order.
+ * Returns the number of accepted swaps, recorded as (a, b, delta) in acc. */
+int64_t orc_first_improvement_rows(const orc_graphs *g, int32_t *map, int max_rows, int32_t *acc, int64_t acc_cap,
+                                   int threads);
+/* orc_best_g3 / orc_best_perm over lists of triples / k-sets. */
+void orc_best_g3_many(const orc_graphs *g, const int32_t *map, const int32_t *triples, int64_t count, int mode,
+                      int32_t *best_d, int32_t *best_p, int threads);
+void orc_best_perm_many(const orc_graphs *g, const int32_t *map, const int32_t *sets, int k, int64_t count,
+                        int32_t *best_d, int64_t *best_rank, int threads);
+
 /* --- perturbation and the searches built on it --------------------------- */
 /* splitmix64 step (the generator the engine's perturbation draws from). */
 uint64_t orc_rng_next(uint64_t *state);

@@ -15,7 +15,11 @@ tests never need the reference.
                         improvements every 20 rows, :519-534) and the mapping
                         file it last wrote (:536-540).
 
-Usage: python tests/golden/make_golden.py
+  ga_opt_multi/        : the same on a pair with lines at vertex 0, self-loops
+                        and repeated lines, also recording the per-swap
+                        "Score mismatch after swap" lines (:570-576).
+
+Usage: python tests/golden/make_golden.py [score] [n100] [n240] [multi]
 """
 import json
 import os
@@ -91,15 +95,24 @@ def score_matching_cases(outdir):
 _NUM = re.compile(r"([\d,]+)")
 
 
-def run_ga_opt(name, inst_seed, n, m, outdir, budget_s=400):
+_MISMATCH = re.compile(r"Score mismatch after swap - calculated: (-?\d+), verified: (-?\d+)")
+_INITIAL = re.compile(r"Initial alignment score: (-?[\d,]+)")
+
+
+def run_ga_opt(name, inst_seed, n, m, outdir, budget_s=400, graphs=None):
+    """graphs: (g1, g2, p0, description) instead of the seeded power-law pair."""
     import bmatch
 
-    inst = bmatch.Instance.powerlaw(inst_seed, n, m)
+    if graphs is None:
+        inst = bmatch.Instance.powerlaw(inst_seed, n, m)
+        g1, g2, p0, desc = inst.g1, inst.g2, inst.p0, f"bmatch.Instance.powerlaw({inst_seed}, {n}, {m})"
+    else:
+        g1, g2, p0, desc = graphs
     d = os.path.join(outdir, f"ga_opt_{name}")
     os.makedirs(d, exist_ok=True)
-    write_edges(os.path.join(d, "male.csv"), *inst.g1)
-    write_edges(os.path.join(d, "female.csv"), *inst.g2)
-    write_map_mf(os.path.join(d, "init.csv"), inst.p0)
+    write_edges(os.path.join(d, "male.csv"), *g1)
+    write_edges(os.path.join(d, "female.csv"), *g2)
+    write_map_mf(os.path.join(d, "init.csv"), p0)
     work = os.path.join(d, "_run")
     os.makedirs(work, exist_ok=True)
     for o in ("order_m.csv", "order_f.csv"):
@@ -111,9 +124,17 @@ def run_ga_opt(name, inst_seed, n, m, outdir, budget_s=400):
            os.path.join(d, "init.csv"), outmap]
     proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, cwd=work)
     log, cur = [], {}
+    mismatches, initial = [], None
     t0 = time.time()
     converged_at = None
     for line in proc.stdout:
+        mm = _MISMATCH.search(line)
+        if mm:
+            mismatches.append((int(mm.group(1)), int(mm.group(2))))
+            continue
+        mi = _INITIAL.search(line)
+        if mi and initial is None:
+            initial = int(mi.group(1).replace(",", ""))
         if "Processed:" in line:
             cur = {"rows": int(line.split("Processed:")[1].split("/")[0].strip().replace(",", ""))}
         elif "Current score:" in line and "rows" in cur:
@@ -143,32 +164,71 @@ def run_ga_opt(name, inst_seed, n, m, outdir, budget_s=400):
     for ln in lines[1:]:
         a, b = ln.split(",")
         pairs.append((int(a[1:]), int(b[1:])))
-    final = np.zeros(n + 1, np.int32)
+    # with lines at G2's vertex 0 a swap with one of the empty rows above n
+    # (the program's vertices run to NUM_NODES) can pay, so the mapping file
+    # may name vertices past n; every image 1..n is still there once
+    vrange = NUM_NODES if graphs is not None else n
+    final = np.zeros(vrange + 1, np.int32)
     for a, b in pairs:
         final[a] = b
-    if sorted(final[1:].tolist()) != list(range(1, n + 1)):
+    if sorted(final[final > 0].tolist()) != list(range(1, n + 1)):
         raise RuntimeError(f"{name}: mapping file incomplete (caught mid-write?)")
     # keep the log entries where something changed, plus the last one
     compact = [log[0]] + [log[i] for i in range(1, len(log)) if log[i][1:] != log[i - 1][1:]] + [log[-1]]
     with open(os.path.join(d, "expected.json"), "w") as f:
-        json.dump({"program": "src/graph-alignment-optimized.c", "n": n, "m": m, "generator_seed": inst_seed,
+        json.dump({"program": "src/graph-alignment-optimized.c", "n": n, "m": m, "generator": desc,
                    "rows_per_sweep": NUM_NODES, "log_every": 20, "log_changes": compact,
                    "log_last_row": log[-1][0], "final_mapping": final.tolist(),
-                   "converged_at_row": converged_at}, f, separators=(",", ":"))
+                   "converged_at_row": converged_at, "initial_score": initial, "vertex_range": vrange,
+                   "mismatches": mismatches}, f, separators=(",", ":"))
     for x in os.listdir(work):
         os.remove(os.path.join(work, x))
     os.rmdir(work)
     print(f"ga_opt {name}: n={n} m={m} entries={len(log)} changes={len(compact)} final score={log[-1][1]} "
-          f"improvements={log[-1][2]} ({time.time() - t0:.0f}s)")
+          f"improvements={log[-1][2]} initial={initial} mismatches={len(mismatches)} ({time.time() - t0:.0f}s)")
+
+
+def multigraph_pair(seed, n, m):
+    """A pair with every input the reference's score and delta treat apart:
+    lines leaving and entering vertex 0 in both graphs (the score loop starts
+    at vertex 1, graph-alignment-optimized.c:295, while the delta counts a
+    line from 0 as an in-entry of its target, and W2's row 0 is read for
+    every line from 0), self-loops, and repeated lines with fresh weights
+    (the dense cell takes the last, :185)."""
+    from conftest import random_multigraph
+
+    rng = np.random.default_rng(seed)
+    g = []
+    for k in range(2):
+        s, d, w = random_multigraph(seed + 1 + k, n, m, dup_frac=0.1, selfloop_frac=0.03, wmax=30)
+        # a few lines at vertex 0 (no 0 -> 0), placed at random positions in the file
+        zs = rng.integers(1, n + 1, 12)
+        zsrc = np.where(np.arange(12) < 8, 0, zs).astype(np.int32)
+        zdst = np.where(np.arange(12) < 8, zs, 0).astype(np.int32)
+        zw = rng.integers(1, 31, 12).astype(np.int32)
+        s, d, w = np.concatenate([s, zsrc]), np.concatenate([d, zdst]), np.concatenate([w, zw])
+        perm = rng.permutation(len(s))
+        g.append((s[perm], d[perm], w[perm]))
+    p0 = np.zeros(n + 1, np.int32)
+    p0[1:] = rng.permutation(n) + 1
+    return g[0], g[1], p0, (f"tests/golden/make_golden.py multigraph_pair({seed}, {n}, {m}): random_multigraph "
+                            f"with dup_frac=0.1, selfloop_frac=0.03, wmax=30, plus 8 lines from and 4 into vertex 0 "
+                            f"per graph")
 
 
 def main():
     if not os.path.isdir("/root/reference"):
         sys.exit("the reference is not mounted here; fixtures are committed under tests/golden/")
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    score_matching_cases(HERE)
-    run_ga_opt("n100", 11, 100, 700, HERE)
-    run_ga_opt("n240", 12, 240, 2400, HERE)
+    only = sys.argv[1:]
+    if not only or "score" in only:
+        score_matching_cases(HERE)
+    if not only or "n100" in only:
+        run_ga_opt("n100", 11, 100, 700, HERE)
+    if not only or "n240" in only:
+        run_ga_opt("n240", 12, 240, 2400, HERE)
+    if not only or "multi" in only:
+        run_ga_opt("multi", 0, 90, 700, HERE, graphs=multigraph_pair(31, 90, 700))
 
 
 if __name__ == "__main__":

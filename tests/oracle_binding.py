@@ -56,6 +56,11 @@ def lib():
         "orc_row_best_pass": (C.c_int64, [vp, _i32p, _i32p, C.c_int64]),
         "orc_all_pairs": (C.c_int32, [vp, _i32p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp, vp, vp]),
         "orc_omp_rowbest_rows": (C.c_int64, [vp, _i32p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
+        "orc_row_best_rows": (None, [vp, _i32p, _i32p, C.c_int64, _i32p, _i32p, C.c_int]),
+        "orc_delta_rows": (None, [vp, _i32p, _i32p, C.c_int64, _i32p, C.c_int]),
+        "orc_first_improvement_rows": (C.c_int64, [vp, _i32p, C.c_int, _i32p, C.c_int64, C.c_int]),
+        "orc_best_g3_many": (None, [vp, _i32p, _i32p, C.c_int64, C.c_int, _i32p, _i32p, C.c_int]),
+        "orc_best_perm_many": (None, [vp, _i32p, _i32p, C.c_int, C.c_int64, _i32p, _i64p, C.c_int]),
         "orc_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
         "orc_vertex_scores": (None, [vp, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]),
         "orc_perturb": (C.c_int, [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_uint64), _i32p]),
@@ -190,6 +195,41 @@ class Oracle:
                                 rd.ctypes.data_as(C.c_void_p),
                                 dm.ctypes.data_as(C.c_void_p) if dm is not None else None)
         return {"best": (ba.value, bb.value, d), "row_best_b": rb, "row_best_delta": rd, "matrix": dm}
+
+    def row_best_rows(self, m, rows, threads=0):
+        rows = _i32(rows)
+        b = np.zeros(len(rows), np.int32)
+        d = np.zeros(len(rows), np.int32)
+        lib().orc_row_best_rows(self._h, _i32(m), rows, len(rows), b, d, threads)
+        return b, d
+
+    def delta_rows(self, m, rows, threads=0):
+        rows = _i32(rows)
+        out = np.zeros(len(rows) * (self.n + 1), np.int32)
+        lib().orc_delta_rows(self._h, _i32(m), rows, len(rows), out, threads)
+        return out.reshape(len(rows), self.n + 1)
+
+    def first_improvement_rows(self, m, max_rows, threads=0, acc_cap=1 << 20):
+        m = _i32(m).copy()
+        acc = np.zeros(3 * acc_cap, np.int32)
+        k = lib().orc_first_improvement_rows(self._h, m, max_rows, acc, acc_cap, threads)
+        return m, acc[: 3 * min(k, acc_cap)].reshape(-1, 3)
+
+    def best_g3_many(self, m, triples, mode, threads=0):
+        t = _i32(triples).reshape(-1)
+        c = len(t) // 3
+        d = np.zeros(c, np.int32)
+        p = np.zeros(c, np.int32)
+        lib().orc_best_g3_many(self._h, _i32(m), t, c, mode, d, p, threads)
+        return d, p
+
+    def best_perm_many(self, m, sets, threads=0):
+        s = _i32(sets)
+        c, k = s.shape
+        d = np.zeros(c, np.int32)
+        r = np.zeros(c, np.int64)
+        lib().orc_best_perm_many(self._h, _i32(m), np.ascontiguousarray(s.reshape(-1)), k, c, d, r, threads)
+        return d, r
 
     def omp_rowbest_rows(self, m, lo, hi, threads=0):
         cs = C.c_int64()

@@ -54,11 +54,15 @@ def test_first_improvement_matches_graph_alignment_optimized(name):
     d = os.path.join(GOLD, name)
     with open(os.path.join(d, "expected.json")) as f:
         exp = json.load(f)
-    n = exp["n"]
+    # vertex_range: the program's vertex range (NUM_NODES) where the run moved
+    # images onto the empty vertices above the data
+    n = exp.get("vertex_range", exp["n"])
     g1 = _load_edges(os.path.join(d, "male.csv"))
     g2 = _load_edges(os.path.join(d, "female.csv"))
     init = _load_mf(os.path.join(d, "init.csv"), n)
     o = ob.Oracle(n, g1, g2)
+    if "initial_score" in exp:  # "Initial alignment score" (:720), lines leaving vertex 0 not scored
+        assert o.score_opt(init) == exp["initial_score"]
     res = o.first_improvement(init, rows_per_sweep=exp["rows_per_sweep"], log_every=exp["log_every"])
     # the reference's progress samples, expanded from the stored change points
     changes = exp["log_changes"]

@@ -55,8 +55,15 @@ def test_graph_alignment_optimized_replays_reference(name, tmp_path):
             j += 1
         want.append((row, changes[j][1], changes[j][2]))
     assert got == want
-    np.testing.assert_array_equal(read_mf(out, exp["n"]), np.array(exp["final_mapping"], np.int32))
+    np.testing.assert_array_equal(read_mf(out, exp.get("vertex_range", exp["n"])),
+                                  np.array(exp["final_mapping"], np.int32))
     assert f"Final score: {changes[-1][1]:,}" in r.stdout
+    if "initial_score" in exp:
+        assert f"Initial alignment score: {exp['initial_score']:,}" in r.stdout
+        # the per-swap lines of :570-576, in order
+        mm = [(int(a), int(b)) for a, b in
+              re.findall(r"Score mismatch after swap - calculated: (-?\d+), verified: (-?\d+)", r.stderr)]
+        assert mm == [tuple(x) for x in exp["mismatches"]]
 
 
 def test_refine_matching_hillclimb_cli(tmp_path):

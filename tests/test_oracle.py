@@ -14,11 +14,13 @@ from conftest import random_mapping, random_multigraph
 
 
 def np_score(n, g1, g2, mp):
-    """S_opt by dense numpy: every G1 line, W2 dense with the last line winning, as short."""
+    """S_opt by dense numpy: every G1 line with a source >= 1 (the score loop
+    starts at vertex 1), W2 dense with the last line winning, as short."""
     W2 = np.zeros((n + 1, n + 1), np.int64)
     W2[g2[0], g2[1]] = g2[2]  # numpy fancy assignment keeps the last duplicate
     W2 = W2.astype(np.int16).astype(np.int64)
-    return int(np.minimum(g1[2], W2[mp[g1[0]], mp[g1[1]]]).sum())
+    keep = g1[0] >= 1
+    return int(np.minimum(g1[2][keep], W2[mp[g1[0][keep]], mp[g1[1][keep]]]).sum())
 
 
 def swapped(mp, a, b):
@@ -40,11 +42,23 @@ def simple_graph(seed, n, m, w=30):
 
 
 def test_score_opt_is_numpy_rescore():
-    for seed in range(4):
-        g1 = random_multigraph(seed, 30, 300, 0.2, 0.05)
-        g2 = random_multigraph(seed + 50, 30, 300, 0.2, 0.05)
+    for seed in range(6):
+        zero = seed >= 4  # lines at vertex 0 in both graphs
+        g1 = random_multigraph(seed, 30, 300, 0.2, 0.05, zero_vertex=zero)
+        g2 = random_multigraph(seed + 50, 30, 300, 0.2, 0.05, zero_vertex=zero)
         mp = random_mapping(seed, 30, 0.1, bijective=seed % 2 == 0)
+        if zero:
+            mp[0] = 7  # a mapped vertex 0 would make its lines count if they were scored
+            assert (g1[0] == 0).any()
         assert ob.Oracle(30, g1, g2).score_opt(mp) == np_score(30, g1, g2, mp)
+
+
+def test_score_opt_skips_lines_leaving_vertex_0():
+    """The three-line case: 0->1 (w 5) is not scored, 1->2 and 2->0 are."""
+    g1 = (np.array([0, 1, 2], np.int32), np.array([1, 2, 0], np.int32), np.array([5, 4, 3], np.int32))
+    g2 = (np.array([0, 1, 2], np.int32), np.array([1, 2, 0], np.int32), np.array([9, 9, 9], np.int32))
+    o = ob.Oracle(2, g1, g2)
+    assert o.score_opt(np.array([0, 1, 2], np.int32)) == 4 + 3
 
 
 def test_score_invariants():
