@@ -27,11 +27,13 @@ def closed_nbrs(n, g1):
 
 
 def simple_vertices(n, g1):
+    """Vertices no self-loop, repeated line or line from vertex 0 touches."""
     ok = np.ones(n + 1, bool)
+    ok[0] = False
     seen = set()
     for s, d in zip(g1[0].tolist(), g1[1].tolist()):
-        if s == d:
-            ok[s] = False
+        if s == d or s == 0:
+            ok[d] = False
         if (s, d) in seen:
             ok[s] = ok[d] = False
         seen.add((s, d))
