@@ -979,17 +979,23 @@ static int capture(bm_ctx *c) {
         rc = enqueue_step(c);
         e = hipStreamEndCapture(c->stream, &g);
         if (rc && g) (void)hipGraphDestroy(g);
+        if (!rc && e == hipSuccess) {
+            b->graph = g;
+            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                b->graph = nullptr;
+                b->exec = nullptr;
+                c->timing = timing;
+                c->g2y_dirty = dirty;
+                return hip_fail(e, "hipGraphInstantiate");
+            }
+        }
     }
     c->timing = timing;
     c->g2y_dirty = dirty;
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    b->graph = g;
-    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        drop_graph(b);
-        return hip_fail(e, "hipGraphInstantiate");
-    }
     b->captured_sparse = c->sparse_ok;
     return BM_OK;
 }
@@ -1170,7 +1176,15 @@ int refine_start(bm_ctx *c) {
 // Enqueue `steps` replays and an async copy of the state; returns at once.
 int refine_enqueue(bm_ctx *c, int steps) {
     bm_batch_state *b = c->batch;
-    for (int s = 0; s < steps; s++) BM_HIP(hipGraphLaunch(b->exec, c->stream));
+    {
+        // graph launches, like every other graph call (capture, instantiate,
+        // destroy), are serialised across the threads of a process: the
+        // serving loop launches from its scheduler thread while preparation
+        // threads capture and instantiate other jobs' graphs and destroy
+        // retired ones
+        std::lock_guard<std::mutex> lk(sync_mutex());
+        for (int s = 0; s < steps; s++) BM_HIP(hipGraphLaunch(b->exec, c->stream));
+    }
     BM_HIP(hipMemcpyAsync(b->hst, b->st, sizeof(RefineState), hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipEventRecord(b->done, c->stream));
     b->in_flight = true;
@@ -1291,9 +1305,10 @@ extern "C" int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t ma
     while (!live.empty()) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
-        BM_HIP(hipGraphCreate(&g, 0));
         std::vector<int> chunk(count, 0);
         int err = BM_OK;
+        std::unique_lock<std::mutex> glk(sync_mutex());  // graph calls are serialised (refine_enqueue)
+        BM_HIP(hipGraphCreate(&g, 0));
         for (int i : live) {
             bm_batch_state *b = ctxs[i]->batch;
             chunk[i] = (int)std::min<int64_t>(b->prm.steps_per_check, max_steps - done[i]);
@@ -1310,12 +1325,12 @@ extern "C" int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t ma
         if (!err && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
             err = set_error(BM_EHIP, "hipGraphInstantiate failed");
         if (!err && hipGraphLaunch(ge, stream) != hipSuccess) err = set_error(BM_EHIP, "hipGraphLaunch failed");
+        glk.unlock();
         if (!err && hipStreamSynchronize(stream) != hipSuccess) err = set_error(BM_EHIP, "pack failed");
-        {
-            std::lock_guard<std::mutex> lk(sync_mutex());
-            if (ge) (void)hipGraphExecDestroy(ge);
-            (void)hipGraphDestroy(g);
-        }
+        glk.lock();
+        if (ge) (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        glk.unlock();
         if (err) return err;
         std::vector<int> next;
         for (int i : live) {
