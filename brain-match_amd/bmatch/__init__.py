@@ -59,6 +59,7 @@ def lib():
         "bm_get_mapping": (C.c_int, [C.c_void_p, _i32p]),
         "bm_score": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
         "bm_eval_swaps": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
+        "bm_eval_swaps_noalloc": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
         "bm_eval_swap_row": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int,
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "bm_sweep_all_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -411,6 +412,13 @@ class Engine:
         a, b = _i32(a), _i32(b)
         out = np.zeros(len(a), np.int32)
         _check(self._L.bm_eval_swaps(self._h, a, b, len(a), out), "bm_eval_swaps")
+        return out
+
+    def eval_swaps_noalloc(self, a, b):
+        """bm_eval_swaps_noalloc: bm_eval_swaps staged in the library's fixed device array (no allocation)."""
+        a, b = _i32(a), _i32(b)
+        out = np.zeros(len(a), np.int32)
+        _check(self._L.bm_eval_swaps_noalloc(self._h, a, b, len(a), out), "bm_eval_swaps_noalloc")
         return out
 
     def eval_swaps_hillclimb(self, a, b):

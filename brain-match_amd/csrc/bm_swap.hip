@@ -1120,6 +1120,53 @@ extern "C" int bm_eval_swaps(bm_ctx *c, const int32_t *a, const int32_t *b, int6
     return BM_OK;
 }
 
+// ------------------------------------------------------------ no allocation
+
+// bm_eval_swaps without any allocation: the pairs and their deltas are staged
+// in a file-scope device array instead of the context's staging buffer, which
+// is allocated on first use.  Each device holds its own copy of the array at
+// its own address, taken once per device with hipGetSymbolAddress; host code
+// uses only that address (the array's name is a host-side stand-in).  Calls
+// on one device are serialised here: they share the array, and a call
+// returns after its deltas are copied out.
+namespace {
+constexpr int64_t kNoAllocPairs = 1 << 20;  // pairs per call, at most (12 MiB of scratch)
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) int32_t scratch[3 * kNoAllocPairs];
+std::mutex noalloc_mu[kNoAllocDevices];
+int32_t *noalloc_ptr[kNoAllocDevices];  // per device, under noalloc_mu[device]
+}  // namespace
+
+extern "C" int bm_eval_swaps_noalloc(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count,
+                                     int32_t *delta) {
+    if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (count < 0) return set_error(BM_EINVAL, "count < 0");
+    if (count > kNoAllocPairs)
+        return set_error(BM_EINVAL, "bm_eval_swaps_noalloc: %lld pairs exceed the scratch of %lld pairs per call",
+                         (long long)count, (long long)kNoAllocPairs);
+    if (c->device < 0 || c->device >= kNoAllocDevices) return set_error(BM_ENODEV, "device %d", c->device);
+    if (count == 0) return BM_OK;
+    int rc = check_pairs(c, a, b, count);
+    if (rc) return rc;
+    BM_HIP(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> lk(noalloc_mu[c->device]);
+    if (!noalloc_ptr[c->device]) {  // first call on this device, with the device current
+        void *p = nullptr;
+        BM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        noalloc_ptr[c->device] = (int32_t *)p;
+    }
+    int32_t *dA = noalloc_ptr[c->device], *dB = dA + kNoAllocPairs, *dD = dB + kNoAllocPairs;
+    BM_HIP(hipMemcpyAsync(dA, a, count * 4, hipMemcpyHostToDevice, c->stream));
+    BM_HIP(hipMemcpyAsync(dB, b, count * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_eval_pairs, dim3((unsigned)((count * 16 + 255) / 256)), dim3(256), 0, c->stream, dA, dB,
+                       count, dD, c->cptr, c->cr, c->cw, c->cdw, c->Q, c->N1, c->ld);
+    if ((rc = launch_check("eval_pairs (no allocation)"))) return rc;
+    BM_HIP(hipMemcpyAsync(delta, dD, count * 4, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
 // The dense-matrix delta of quick.cu:48-250 through the same pair kernel over
 // the dense-form lists (each entry owns its cell, so the dense-cell weight
 // array is the weight array).

@@ -100,6 +100,16 @@ int bm_eval_swaps(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count
 int bm_eval_swap_row(bm_ctx *ctx, int32_t a, int32_t b_lo, int32_t b_hi, int mode,
                      int32_t *best_b, int32_t *best_delta);
 
+/* bm_eval_swaps that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation).  No counterpart in the reference.  The pairs and
+ * deltas are staged in a fixed 12 MiB device array that every GPU holds from
+ * the first such call until the process ends, instead of the context's
+ * staging buffer; count <= 1,048,576 (BM_EINVAL above).  The context must
+ * have its graphs and mapping set.  Calls on one GPU share the array and are
+ * serialised (each returns after its deltas are copied out). */
+int bm_eval_swaps_noalloc(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta);
+
 /* The swap delta of the dense-matrix GPU kernels, calculateSwapDeltasKernel
  * and calculateSwapDeltasKernelQuick (quick.cu:48-250; the same arithmetic
  * in graph-re-alignment-gpu.cu:60-128): for every i in [1, n] other than a
