@@ -699,7 +699,7 @@ static void free_graph_buffers(bm_ctx *c) {
     c->graphs = c->mapped = c->diag_valid = c->dfdiag_valid = false;
     c->gain_cols = 0;
     c->sweep_kind = 0;
-    c->sparse_ok = false;
+    c->sparse_ok = c->all_mapped = false;
     c->rot3_tuned = false;
     c->permset_threads_k = 0;
 }
@@ -1029,6 +1029,9 @@ int bm_set_mapping(bm_ctx *c, const int32_t *mapping) {
         if (c->hpinv[g] >= 0) c->sparse_ok = false;
         else c->hpinv[g] = v;
     }
+    c->all_mapped = c->sparse_ok;
+    for (int g = 1; g < c->N1 && c->all_mapped; g++)
+        if (c->hpinv[g] < 0) c->all_mapped = false;
     if (c->sparse_ok)
         BM_HIP(hipMemcpyAsync(c->pinv, c->hpinv.data(), (size_t)c->N1 * 4, hipMemcpyHostToDevice, c->stream));
     c->g2y_dirty = true;

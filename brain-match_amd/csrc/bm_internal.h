@@ -101,6 +101,7 @@ struct bm_ctx {
     int32_t *g2i_at_out = nullptr;  // per out-list entry (line g->v): its index in the in-lists
     bool g2y_dirty = true;
     bool sparse_ok = false;   // p injective on mapped vertices and !g2_vertex0
+    bool all_mapped = false;  // sparse_ok and p a bijection of 1..n (moves keep it): every G2 target translates
     int sweep_kind = 0;       // all-pairs gain kernel: 0 = not tuned, 1 = dense, 2 = sparse
     float sparse_ms = 0.f;
     int32_t *diag = nullptr;

@@ -768,7 +768,7 @@ __global__ void k_clear_stale(RefineState *st) {
 struct bm_batch_state {
     bm_refine_params prm{};
     bool configured = false;
-    bool captured_sparse = false;
+    int captured_kind = -1;  // sparse_ok | all_mapped << 1 at the capture (they select the sweep kernels)
     int32_t *tri = nullptr, *tri_d = nullptr, *tri_p = nullptr;
     int32_t *sets = nullptr, *set_d = nullptr, *set_r = nullptr;
     int *ovf = nullptr;
@@ -996,7 +996,7 @@ static int capture(bm_ctx *c) {
     c->g2y_dirty = dirty;
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    b->captured_sparse = c->sparse_ok;
+    b->captured_kind = (c->sparse_ok ? 1 : 0) | (c->all_mapped ? 2 : 0);
     return BM_OK;
 }
 
@@ -1155,7 +1155,7 @@ int refine_set_log(bm_ctx *c, int32_t cap) {
 // New run: exact score of the current mapping, counters zeroed.
 int refine_start(bm_ctx *c) {
     bm_batch_state *b = c->batch;
-    if (b->captured_sparse != c->sparse_ok) {  // the mapping changed kind since the capture
+    if (b->captured_kind != ((c->sparse_ok ? 1 : 0) | (c->all_mapped ? 2 : 0))) {  // the mapping changed kind
         int rc = capture(c);
         if (rc) return rc;
     }

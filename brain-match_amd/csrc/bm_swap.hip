@@ -623,7 +623,10 @@ int bm::launch_refresh_g2y(bm_ctx *c) {
 // X(a,a) becomes diag[a] afterwards (k_diag_from_x).
 constexpr int kRowFirst = 1, kRowWhole = 2;
 
-template <bool kHubs, bool kNeg>
+// kAll: the mapping is a bijection of 1..n, so every list target translates
+// (no 0xFFFF word); the list words are read through a buffer resource with
+// 32-bit byte offsets.
+template <bool kHubs, bool kNeg, bool kAll>
 __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ plan, const int32_t *__restrict__ cptr,
                                                       const int2 *__restrict__ crw,
                                                       const int32_t *__restrict__ cdw,
@@ -723,11 +726,11 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                 }
                 uint32_t yv[kSparseUnroll];
 #pragma unroll
-                for (int u = 0; u < kSparseUnroll; u++) yv[u] = g2oy[at[u] + idx + 64 * u];
+                for (int u = 0; u < kSparseUnroll; u++) yv[u] = g2oy[(uint32_t)(at[u] + idx) + 64u * u];
 #pragma unroll
                 for (int u = 0; u < kSparseUnroll; u++) {
                     const int y = yv[u] >> 16;
-                    if (y != 0xFFFF) {
+                    if (kAll || y != 0xFFFF) {
                         const int w = wu[u];
                         // f(w, q) = min(w, q) - min(w, 0); kNeg: some G1 weight is negative
                         atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - (kNeg ? min(w, 0) : 0));
@@ -743,7 +746,7 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
                 }
                 const uint32_t yv = g2oy[cur.x + idx - s0];
                 const int y = yv >> 16;
-                if (y != 0xFFFF)
+                if (kAll || y != 0xFFFF)
                     atomicAdd(&acc[y], min(cur.y, (int)(int16_t)(yv & 0xffff)) - (kNeg ? min(cur.y, 0) : 0));
             }
         }
@@ -810,6 +813,12 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
 // The largest LDS row the sparse kernel may use (bytes).
 constexpr size_t kSparseLdsMax = 144 * 1024 - 32;
 
+// The sparse kernel's preconditions: the mapping (sparse_ok), the LDS row, and
+// list offsets that fit the 32-bit byte offsets of its buffer loads.
+static bool sparse_fits(const bm_ctx *c) {
+    return c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax && (uint64_t)c->ng2 * 8 < ((uint64_t)1 << 31);
+}
+
 // rows split over several plan items start from zero
 __global__ void k_zero_rows(int32_t *__restrict__ X, int64_t ldx, const int32_t *__restrict__ rows,
                             const int *__restrict__ run_flag) {
@@ -874,8 +883,14 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
     int rc = build_row_plan(c);
     if (rc) return rc;
     if (!c->sparse_attr) {  // once per context, so that captured launches make no attribute calls
-        const void *fns[4] = {(const void *)k_gain_sparse<true, true>, (const void *)k_gain_sparse<true, false>,
-                              (const void *)k_gain_sparse<false, true>, (const void *)k_gain_sparse<false, false>};
+        const void *fns[8] = {(const void *)k_gain_sparse<true, true, false>,
+                              (const void *)k_gain_sparse<true, false, false>,
+                              (const void *)k_gain_sparse<false, true, false>,
+                              (const void *)k_gain_sparse<false, false, false>,
+                              (const void *)k_gain_sparse<true, true, true>,
+                              (const void *)k_gain_sparse<true, false, true>,
+                              (const void *)k_gain_sparse<false, true, true>,
+                              (const void *)k_gain_sparse<false, false, true>};
         for (const void *f : fns)
             BM_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kSparseLdsMax + 32)));
         c->sparse_attr = true;
@@ -885,16 +900,24 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
                                                      c->ldx, c->split_rows, run_flag));
     // the long-list path costs registers and scheduling even when idle: only
     // the instantiation that needs it carries it; likewise the min(w, 0) term
-#define BM_GAIN_SPARSE(H, NEG)                                                                                      \
-    BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain_sparse<H, NEG>), dim3((unsigned)c->nplan), dim3(1024), lds,      \
+#define BM_GAIN_SPARSE(H, NEG, ALL)                                                                                 \
+    BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain_sparse<H, NEG, ALL>), dim3((unsigned)c->nplan), dim3(1024), lds, \
                                             c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg, c->map, \
                                             c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X, c->ldx, c->diag, \
                                             c->N1, c->dense_len, (int)c->ng2, run_flag))
     const bool hubs = c->g2_maxlen > c->dense_len, neg = !c->g1nonneg;
-    if (hubs && neg) BM_GAIN_SPARSE(true, true);
-    else if (hubs) BM_GAIN_SPARSE(true, false);
-    else if (neg) BM_GAIN_SPARSE(false, true);
-    else BM_GAIN_SPARSE(false, false);
+    // every target translated (the mapping is a bijection of 1..n): no list word is skipped
+    if (c->all_mapped) {
+        if (hubs && neg) BM_GAIN_SPARSE(true, true, true);
+        else if (hubs) BM_GAIN_SPARSE(true, false, true);
+        else if (neg) BM_GAIN_SPARSE(false, true, true);
+        else BM_GAIN_SPARSE(false, false, true);
+    } else {
+        if (hubs && neg) BM_GAIN_SPARSE(true, true, false);
+        else if (hubs) BM_GAIN_SPARSE(true, false, false);
+        else if (neg) BM_GAIN_SPARSE(false, true, false);
+        else BM_GAIN_SPARSE(false, false, false);
+    }
 #undef BM_GAIN_SPARSE
     if (c->nsplit > 0)
         BM_LAUNCH(c, "diag_from_x", hipLaunchKernelGGL(k_diag_from_x, dim3((c->nsplit + 255) / 256), dim3(256), 0,
@@ -924,7 +947,7 @@ static int choose_gain_timed(bm_ctx *c);
 
 static int choose_gain(bm_ctx *c) {
     if (c->gain_cols && (c->sweep_kind || !(c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax))) return BM_OK;
-    const bool can_sparse = c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
+    const bool can_sparse = sparse_fits(c);
     const bool forced = getenv("BM_GAIN_COLS") || getenv("BM_SWEEP_KERNEL");
     const auto key = std::make_tuple(ilog2(c->N1), ilog2(std::max<long long>(1, c->ncomb / c->N1)), can_sparse ? 1 : 0);
     if (!forced) {
@@ -946,7 +969,7 @@ static int choose_gain(bm_ctx *c) {
 }
 
 static int choose_gain_timed(bm_ctx *c) {
-    const bool can_sparse = c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
+    const bool can_sparse = sparse_fits(c);
     if (can_sparse && !getenv("BM_SWEEP_KERNEL") && !getenv("BM_GAIN_COLS")) {
         // By the work counts: the sparse kernel walks about sum_v deg1(v) deg2(p(v)) list
         // words, the dense ones ncomb * (n+1) cells.  A clear margin decides without timing
@@ -1369,9 +1392,7 @@ int bm::sweep_prepare(bm_ctx *c) {
 
 // Graph-capturable: kernels only (the choice of gain kernel and all
 // allocations are made by sweep_prepare).
-bool bm::sweep_is_sparse(const bm_ctx *c) {
-    return c->sweep_kind == 2 && c->sparse_ok && (size_t)c->N1 * 4 <= kSparseLdsMax;
-}
+bool bm::sweep_is_sparse(const bm_ctx *c) { return c->sweep_kind == 2 && sparse_fits(c); }
 
 int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag) {
     const int N1 = c->N1;
