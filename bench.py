@@ -3,25 +3,42 @@
 
 One step = one all-pairs Delta_B sweep (graph-alignment-optimized.c:321-408
 for every pair 1 <= a < b <= n, with per-row and global arg-max) over the
-synthetic n = 18,524 / 3.0M-edge graph pair, everything resident in HBM.
+synthetic n = 18,524 / 3.0M-line graph pair, everything resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
-                    [--cpu-seconds S | --no-cpu-baseline] [--calibrate-copy MiB]
+                    [--cpu-seconds S | --no-cpu-baseline] [--sweep-only]
+                    [--calibrate-copy MiB] [--calibrate-gather MiB]
                     [--workload NAME] [--window S] [--jobs J]
 
-Prints one JSON line: metric/value (Delta_B evaluations per second), the
-roofline of the dominant kernel (the all-pairs gain kernel the chooser
-picked) with its average launch time measured by HIP events on the engine's
-stream, and the CPU baseline (the oracle's restatement of the reference
-OpenMP row-best loop, alignment-omp.c:598-622, timed on this host on a
-bounded row sample).  --workload selects the other measurements: plain,
-plain-c2, rot3, permset, batched, batched-c1, batched-permset, batched-many, serve
-(bmatch/bench_workloads.py).
+Prints one JSON line:
+  value      Delta_B evaluations per second of the sweep;
+  roofline   the dominant kernel (the all-pairs gain kernel): its average
+             launch time from HIP events on the engine's stream, the bytes it
+             moves between L2 and memory per launch (rocprofv3 counters of the
+             same kernel and workload, stored under profiles/) over that time
+             against the 8 TB/s HBM peak, and beside it the algorithmic bytes
+             of the per-evaluation model (12 B per list term + 16 B);
+  served     the served stream (Poisson arrivals of refinement jobs): jobs/s
+             with p50/p99 time-to-result;
+  batched_vs_plain  exact score gain per second of the batched refinement
+             against plain first-improvement on the same pair and start;
+  cpu_baseline  the reference's own OpenMP first-improvement program
+             (graph-alignment-optimized-parallel, built from the reference
+             sources into oracle/_ref/) on this host's cores, from a mapping
+             no swap improves, so that its time goes to delta evaluations;
+             without that program, the oracle's restatement of the
+             alignment-omp.c:598-622 row-best loop.
+--sweep-only skips the served, batched and CPU legs (profiler runs).
+--workload selects the other measurements (bmatch/bench_workloads.py).
 """
 import argparse
 import json
 import os
+import re
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -30,7 +47,13 @@ sys.path.insert(0, os.path.join(ROOT, "brain-match_amd"))
 METRIC = "swap-delta evals/s & HBM GB/s at n=18.5k; refine jobs/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
 N, E, SEED = 18524, 3_000_000, 2
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "gain_allpairs_n18524_pmc.json")
+COUNTERS = os.path.join(ROOT, "profiles", "gain_allpairs_n18524_counters.json")
+REF_PARALLEL = os.path.join(ROOT, "oracle", "_ref", "graph-alignment-optimized-parallel")
+REF_NUM_NODES = 18524  # graph-alignment-optimized-parallel.c:16
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(n, two_e):
@@ -41,12 +64,113 @@ def algorithmic_bytes(n, two_e):
     return 12 * (n - 1) * two_e + 16 * n * (n - 1) // 2
 
 
-def cpu_baseline(inst, seconds):
+def host_cpu():
+    """(logical CPUs usable, physical cores among them, threads to use, CPU model)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        t = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(t + "physical_package_id") as f1, open(t + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    # one GPU's share of the host: the OpenMP thread count the environment sets, at most the physical cores
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(cores)
+    return len(cpus), len(cores), min(len(cores), share), model
+
+
+def cpu_baseline_reference(inst, mapping, seconds):
+    """Times graph-alignment-optimized-parallel (built unmodified from the reference
+    sources) on this pair: its rows/s between progress lines (every 100 rows,
+    graph-alignment-optimized-parallel.c:521-533), each row evaluating m2 = 1..18,524
+    (m2 != m1) with the OpenMP loop of :544-585."""
+    import bmatch
+
+    logical, physical, threads, model = host_cpu()
+    work = tempfile.mkdtemp(prefix="bm_cpu_", dir="/tmp")
+    files = {k: os.path.join(work, k + ".csv") for k in ("male", "female", "om", "of", "init", "out")}
+    bmatch.write_edges_csv(files["male"], inst.g1)
+    bmatch.write_edges_csv(files["female"], inst.g2)
+    bmatch.write_mapping_mf(files["init"], mapping)
+    for k in ("om", "of"):
+        with open(files[k], "w") as f:
+            f.write("node,order\n")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    # a pseudo-terminal keeps the program's stdout line-buffered, so each progress line is seen when printed
+    import pty
+
+    master, slave = pty.openpty()
+    proc = subprocess.Popen([REF_PARALLEL] + [files[k] for k in ("male", "female", "om", "of", "init", "out")],
+                            stdout=slave, stderr=subprocess.DEVNULL, stdin=subprocess.DEVNULL, cwd=work, env=env,
+                            start_new_session=True)
+    os.close(slave)
+    pat = re.compile(rb"Processed: ([\d,]+)/")
+    first = last = None
+    t_start = time.perf_counter()
+    buf = b""
+    try:
+        while True:
+            try:
+                chunk = os.read(master, 65536)
+            except OSError:
+                break
+            if not chunk:
+                break
+            now = time.perf_counter()
+            buf += chunk
+            *lines, buf = buf.split(b"\n")
+            for line in lines:
+                m = pat.search(line)
+                if m:
+                    rows = int(m.group(1).replace(b",", b""))
+                    if rows >= 200 and first is None:
+                        first = (rows, now)
+                    elif first is not None:
+                        last = (rows, now)
+            if last and last[1] - first[1] >= seconds:
+                break
+            if now - t_start > seconds + 120:
+                break
+    finally:
+        os.killpg(proc.pid, signal.SIGKILL)
+        proc.wait()
+        os.close(master)
+        for p in files.values():
+            if os.path.exists(p):
+                os.remove(p)
+        os.rmdir(work)
+    if not first or not last:
+        return None
+    rows_s = (last[0] - first[0]) / (last[1] - first[1])
+    evals_s = rows_s * (REF_NUM_NODES - 1)
+    return {"value": evals_s, "unit": "evals/s", "cores": threads, "kind": "reference",
+            "sample": f"rows {first[0]}..{last[0]} of graph-alignment-optimized-parallel (src/graph-alignment-"
+                      f"optimized-parallel.c, built from the reference sources) on this pair, {last[1] - first[1]:.1f} s, "
+                      f"from a mapping no swap improves (every row is {REF_NUM_NODES - 1} Delta_B evaluations, "
+                      f"graph-alignment-optimized.c:321-408, and the mapping dump every 25 rows)",
+            "rows_per_s": rows_s, "logical_cpus": logical, "physical_cores": physical, "cpu_model": model,
+            "omp": f"OMP_NUM_THREADS={threads} OMP_PROC_BIND=close OMP_PLACES=cores"}
+
+
+def cpu_baseline_port(inst, seconds):
+    """The oracle's restatement of alignment-omp.c:598-622 (OpenMP over m2, critical arg-max) on the
+    reference layout (EdgeMap lists, short** W2), rows timed with the mapping held fixed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_binding as ob
 
+    logical, physical, threads, model = host_cpu()
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     o = ob.Oracle(inst.n, inst.g1, inst.g2)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     evals, rows, t0 = 0, 0, time.perf_counter()
     step = 16
     while time.perf_counter() - t0 < seconds:
@@ -55,13 +179,14 @@ def cpu_baseline(inst, seconds):
         rows += step
     el = time.perf_counter() - t0
     return {"value": evals / el, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"rows 1..{rows} of the n={inst.n} row-best sweep ({evals} Delta_B evaluations, "
-                      f"{el:.1f} s): oracle restatement of alignment-omp.c:598-622 (OpenMP over m2, critical "
-                      f"arg-max) on the reference layout (EdgeMap lists, short** W2)"}
+            "sample": f"rows 1..{rows} of the n={inst.n} row-best sweep ({evals} Delta_B evaluations, {el:.1f} s): "
+                      f"oracle restatement of alignment-omp.c:598-622 on the reference layout",
+            "logical_cpus": logical, "physical_cores": physical, "cpu_model": model,
+            "omp": f"OMP_NUM_THREADS={threads} OMP_PROC_BIND=close"}
 
 
 def calibrate_copy(mib):
-    """A plain copy kernel of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    """A plain copy kernel of known size, for the memory counters."""
     import torch
 
     x = torch.ones(mib * (1 << 20) // 4, dtype=torch.int32, device="cuda")
@@ -72,19 +197,74 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_bytes": mib * (1 << 20), "launches": 3}))
 
 
+def calibrate_gather(mib, count_log2=24):
+    """Reference gathers for the memory read counters: count random 4-byte words
+    from a mib-MiB buffer (a miss per word), then the same count of consecutive
+    words; the difference of their read bytes is the cost of the misses."""
+    import torch
+
+    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.int32, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    k = 1 << count_log2
+    rnd = torch.randint(0, x.numel(), (k,), device="cuda", generator=g)
+    seq = torch.arange(k, device="cuda")
+    for idx in (rnd, seq, rnd, seq):
+        y = torch.index_select(x, 0, idx)
+    torch.cuda.synchronize()
+    lines = int(torch.unique(rnd // 32).numel())  # distinct 128-byte lines of the random gathers
+    print(json.dumps({"calibrate_gather_words": k, "buffer_bytes": mib * (1 << 20), "distinct_128B_lines": lines,
+                      "order": ["random", "sequential", "random", "sequential"], "out_sum": int(y.sum())}))
+
+
+def roofline(kname, kernel_ms, alg):
+    out = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+           "kernel": kname, "kernel_avg_ms": kernel_ms, "algorithmic_bytes_per_launch": alg,
+           "algorithmic_gbs": alg / (kernel_ms / 1000.0) / 1e9}
+    if os.path.exists(COUNTERS):
+        with open(COUNTERS) as f:
+            pmc = json.load(f)
+        if pmc.get("kernel") == kname:
+            traffic = pmc["read_bytes_per_launch"] + pmc["write_bytes_per_launch"]
+            gbs = traffic / (kernel_ms / 1000.0) / 1e9
+            out.update({"achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                        "traffic_source": f"profiles/{os.path.basename(COUNTERS)}: rocprofv3 --pmc "
+                                          "TCC_EA0_RDREQ_DRAM_32B_sum + TCC_EA0_WRREQ_WRITE_DRAM_32B_sum (x 32 B) of "
+                                          f"this kernel on this workload (kernel source {pmc.get('source', '?')})",
+                        "limiters": pmc.get("limiters")})
+    return out
+
+
+def served_leg(jobs, rate):
+    from bmatch import bench_workloads
+
+    class A:
+        pass
+
+    a = A()
+    a.jobs, a.rate = jobs, rate
+    r = bench_workloads.serve(a)
+    return {k: r[k] for k in ("value", "unit", "p50_s", "p99_s", "mean_queue_s", "mean_prepare_s",
+                              "mean_service_s", "score_gain_per_s")} | {"jobs": jobs, "arrival_rate": rate,
+                                                                         "workload": r["config"]["workload"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-only", action="store_true", help="only the sweep (profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB")
+    ap.add_argument("--calibrate-gather", type=int, default=0, metavar="MiB")
     ap.add_argument("--window", type=float, default=0.0, help="seconds per leg of the batched comparison")
     ap.add_argument("--jobs", type=int, default=0, help="jobs of the serve workload")
+    ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second of the serve workload")
     ap.add_argument("--workload", default="allpairs",
                     choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "batched-c1",
-                             "batched-permset", "batched-many", "quick", "serve"],
+                             "batched-permset", "batched-many", "quick", "serve", "serve-sweep"],
                     help="allpairs (default, the headline); the others measure the other configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -94,16 +274,18 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
+    if args.calibrate_gather:
+        calibrate_gather(args.calibrate_gather)
+        return
 
     import bmatch
+    from bmatch import bench_workloads
 
     if args.workload != "allpairs":
-        from bmatch import bench_workloads
-
         print(json.dumps(bench_workloads.run(args.workload, args)))
         return
 
-    print(f"[bench] generating n={N} E={E} (seed {SEED})", file=sys.stderr, flush=True)
+    log(f"generating n={N} E={E} (seed {SEED})")
     inst = bmatch.Instance.chunglu(SEED, N, E)
     eng = bmatch.Engine(0)
     eng.upload(inst.n, inst.g1, inst.g2)
@@ -113,7 +295,7 @@ def main():
     two_e = 2 * len(inst.g1[0])
     for _ in range(args.warmup):
         eng.sweep_all_pairs()
-    print(f"[bench] warmup done, timing {args.steps} sweeps", file=sys.stderr, flush=True)
+    log(f"warmup done, timing {args.steps} sweeps")
     eng.set_timing(True)
     eng.reset_timing()
     t0 = time.perf_counter()
@@ -127,13 +309,6 @@ def main():
     kname = "k_gain_sparse" if kind == 2 else f"k_gain<{cols}>"
     ms_step = (t1 - t0) * 1000.0 / args.steps
     gain_avg_ms = gain_ms / max(launches, 1)
-    alg = algorithmic_bytes(n, two_e)
-    achieved = alg / (gain_avg_ms / 1000.0) / 1e9
-    traffic = None
-    if os.path.exists(PMC_SUMMARY):
-        with open(PMC_SUMMARY) as f:
-            pmc = json.load(f)
-        traffic = pmc.get("hbm_bytes_per_launch") if pmc.get("kernel") == kname else None
     out = {
         "metric": METRIC,
         "value": pairs / (ms_step / 1000.0),
@@ -152,18 +327,40 @@ def main():
                    "weights": "round(LogNormal(1.0,1.2)) clipped 1..2000", "init": "pi with 30% shuffled",
                    "parallelism": "single GPU"},
         "best_move": {"a": best[0], "b": best[1], "delta": best[2]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "kernel_avg_ms": gain_avg_ms, "algorithmic_bytes_per_launch": alg,
-                     # the bytes the kernel really moves past L2 (PMC, profiles/), per second of this run
-                     "traffic_gbs": traffic / (gain_avg_ms / 1000.0) / 1e9 if traffic else None,
-                     "traffic_frac": traffic / (gain_avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS if traffic else None},
+        "roofline": roofline(kname, gain_avg_ms, algorithmic_bytes(n, two_e)),
     }
-    if not args.no_cpu_baseline:
-        print(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(inst, args.cpu_seconds)
-    else:
+    converged = None
+    if not args.sweep_only:
+        # batched refinement against plain first-improvement, same pair and start, on this engine; the
+        # batched run's end mapping (no improving swap left) is the CPU baseline's start
+        secs = args.window or 5.0
+        prm = bmatch.refine_params(pairs=1, triples=1024, sets=256, set_k=4, rounds=3, steps_per_check=8, seed=1)
+        log("batched vs plain")
+        eng.refine_configure(prm)
+        b = bench_workloads._refine_window(eng, secs, chunk=8)
+        converged = eng.mapping()
+        conv_best = eng.sweep_all_pairs()["best"][2]
+        eng.set_mapping(inst.p0)
+        p = bench_workloads._plain_window(eng, max(b["seconds"], 1.0))
+        out["batched_vs_plain"] = {
+            "ratio": b["gain_per_s"] / p["gain_per_s"] if p["gain_per_s"] > 0 else None,
+            "batched_gain_per_s": b["gain_per_s"], "plain_gain_per_s": p["gain_per_s"],
+            "batched": {k: b[k] for k in ("seconds", "steps", "moves", "initial_score", "final_score")},
+            "plain": {k: p[k] for k in ("seconds", "rows", "moves", "initial_score", "final_score")},
+            "converged": conv_best <= 0}
+    eng.close()
+    if not args.sweep_only:
+        log("served stream")
+        out["served"] = served_leg(args.jobs or 40, args.rate)
+    if args.no_cpu_baseline or args.sweep_only:
         out["cpu_baseline"] = None
+    else:
+        log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
+        cb = None
+        if os.path.exists(REF_PARALLEL) and n == REF_NUM_NODES and converged is not None:
+            cb = cpu_baseline_reference(inst, converged, args.cpu_seconds)
+        out["cpu_baseline"] = cb or cpu_baseline_port(inst, args.cpu_seconds)
+        out["cpu_baseline"]["effective_gbs"] = out["cpu_baseline"]["value"] * algorithmic_bytes(n, two_e) / pairs / 1e9
     print(json.dumps(out))
 
 

@@ -113,6 +113,7 @@ def lib():
         "bm_pairs_read_plain": (C.c_int, [C.c_char_p, C.c_void_p]),
         "bm_pairs_free": (None, [C.c_void_p]),
         "bm_mapping_write_mf": (C.c_int, [C.c_char_p, _i32p, C.c_int32]),
+        "bm_edges_write_csv": (C.c_int, [C.c_char_p, C.c_char_p, _i32p, _i32p, _i32p, C.c_int64]),
         "bm_drive_first_improvement": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
         "bm_drive_row_best": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -330,6 +331,12 @@ def read_pairs(path, fmt="optimized"):
         return arr(p.male), arr(p.female)
     finally:
         L.bm_pairs_free(C.byref(p))
+
+
+def write_edges_csv(path, g, header="Source,Target,Weight"):
+    """An edge list (src, dst, w) as the reference CLIs read it."""
+    s, d, w = (_i32(x) for x in g)
+    _check(lib().bm_edges_write_csv(os.fsencode(path), header.encode(), s, d, w, len(s)), "bm_edges_write_csv")
 
 
 def write_mapping_mf(path, mapping):

@@ -26,6 +26,9 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
             the serving loop; jobs/s and p50/p99 time-to-result (BM_SERVE_ACTIVE
             sets the server's active slots, default 8; BM_SERVE_PREP its
             preparation threads, default 8)
+  serve-sweep  the serve workload at rising arrival rates until the server
+            falls behind: the highest sustained rate and its steady-state
+            p50/p99
 """
 import os
 import time
@@ -306,7 +309,7 @@ def serve(args):
     import threading
 
     jobs = getattr(args, "jobs", 0) or 40
-    rate = 20.0
+    rate = getattr(args, "rate", 0) or 20.0
     rng = np.random.default_rng(2024)
     ns = rng.integers(500, 20001, jobs)
     gaps = rng.exponential(1.0 / rate, jobs)
@@ -382,7 +385,33 @@ def quick(args):
     return out
 
 
+def serve_sweep(args):
+    """The served stream at rising arrival rates: the highest rate whose queue stays bounded, with the
+    steady-state latency there (the last half of the jobs, after the queue has settled)."""
+    class A:
+        pass
+
+    rows = []
+    jobs = getattr(args, "jobs", 0) or 80
+    for rate in (2.0, 4.0, 6.0, 8.0, 10.0, 12.0, 16.0, 20.0):
+        a = A()
+        a.jobs, a.rate = jobs, rate
+        r = serve(a)
+        lat = sorted(j[1] for j in r["per_job"][jobs // 2:])
+        p50, p99 = lat[len(lat) // 2], lat[min(len(lat) - 1, int(0.99 * len(lat)))]
+        rows.append({"rate": rate, "jobs_per_s": r["value"], "p50_s": r["p50_s"], "p99_s": r["p99_s"],
+                     "steady_p50_s": p50, "steady_p99_s": p99, "mean_queue_s": r["mean_queue_s"]})
+        if r["value"] < 0.9 * rate:
+            break
+    sustained = [x for x in rows if x["jobs_per_s"] >= 0.9 * x["rate"]]
+    top = max(sustained, key=lambda x: x["rate"]) if sustained else None
+    out = _base("served refinement: highest sustained arrival rate", top["rate"] if top else 0.0, "jobs/s",
+                jobs, 0.0, {"workload": "serve at rising Poisson rates (jobs as the serve workload)", "jobs": jobs})
+    out.update({"rates": rows, "sustained": top})
+    return out
+
+
 def run(name, args):
     return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset, "batched": batched,
             "batched-c1": batched_c1, "batched-permset": batched_permset, "batched-many": batched_many,
-            "quick": quick, "serve": serve}[name](args)
+            "quick": quick, "serve": serve, "serve-sweep": serve_sweep}[name](args)
