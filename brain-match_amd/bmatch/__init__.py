@@ -122,6 +122,8 @@ def lib():
                                            C.c_void_p]),
         "bm_drive_row_best_restart": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "bm_drive_quick": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "bm_drive_q3": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "bm_q3_triples": (C.c_int64, [C.c_int32, _i32p, C.c_int32, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -195,6 +197,16 @@ class _QuickParams(C.Structure):
 class _QuickStats(C.Structure):
     _fields_ = [(f, C.c_int64) for f in ("evals", "perturbations", "moves", "positive_moves", "zero_moves",
                                          "initial_score", "final_score", "best_score", "pairs")]
+
+
+class _Q3Params(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("max_evals", C.c_int64), ("zero_limit", C.c_int32),
+                ("best_window", C.c_int32), ("gap", C.c_double), ("gap_norm", C.c_double)]
+
+
+class _Q3Stats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("evals", "perturbations", "moves", "positive_moves", "zero_moves",
+                                         "initial_score", "final_score", "best_score", "triples")]
 
 
 class _RefineStep(C.Structure):
@@ -337,6 +349,17 @@ def write_edges_csv(path, g, header="Source,Target,Weight"):
     """An edge list (src, dst, w) as the reference CLIs read it."""
     s, d, w = (_i32(x) for x in g)
     _check(lib().bm_edges_write_csv(os.fsencode(path), header.encode(), s, d, w, len(s)), "bm_edges_write_csv")
+
+
+def q3_triples(n, changed):
+    """bm_q3_triples: the triples q3.cu's kernel enumerates for the sorted changed vertices, (count, 3)."""
+    c = _i32(changed)
+    cnt = lib().bm_q3_triples(n, c, len(c), None)
+    if cnt < 0:
+        raise BMError(int(cnt), "bm_q3_triples", "bad arguments")
+    out = np.zeros(3 * cnt, np.int32)
+    lib().bm_q3_triples(n, c, len(c), out.ctypes.data_as(C.c_void_p))
+    return out.reshape(cnt, 3)
 
 
 def write_mapping_mf(path, mapping):
@@ -507,6 +530,15 @@ class Engine:
         _check(self._L.bm_drive_quick(self._h, C.byref(prm), best.ctypes.data_as(C.c_void_p), C.byref(st)),
                "bm_drive_quick")
         return best, {f: getattr(st, f) for f, _ in _QuickStats._fields_}
+
+    def q3(self, seed=1, max_evals=100, zero_limit=1000, best_window=1500, gap=0.002, gap_norm=5154247.0):
+        """bm_drive_q3, the changed-vertex triple search of q3.cu: (best mapping, stats)."""
+        prm = _Q3Params(seed, max_evals, zero_limit, best_window, gap, gap_norm)
+        best = np.zeros(self.n + 1, np.int32)
+        st = _Q3Stats()
+        _check(self._L.bm_drive_q3(self._h, C.byref(prm), best.ctypes.data_as(C.c_void_p), C.byref(st)),
+               "bm_drive_q3")
+        return best, {f: getattr(st, f) for f, _ in _Q3Stats._fields_}
 
     def apply_swap(self, a, b):
         _check(self._L.bm_apply_swap(self._h, a, b), "bm_apply_swap")

@@ -26,6 +26,10 @@ Each returns one dict in bench.py's JSON shape.  Workloads (seeded, synthetic):
             the serving loop; jobs/s and p50/p99 time-to-result (BM_SERVE_ACTIVE
             sets the server's active slots, default 8; BM_SERVE_PREP its
             preparation threads, default 8)
+  q3        the n=18,524 pair: the changed-vertex triple search of
+            q3.cu:1176-1440 (perturb 3-4 pairs, score every triple the
+            reference's kernel enumerates for the changed vertices, move, score
+            the moved triple's again) for a fixed number of evaluations
   serve-sweep  the serve workload at rising arrival rates until the server
             falls behind: the highest sustained rate and its steady-state
             p50/p99
@@ -411,7 +415,28 @@ def serve_sweep(args):
     return out
 
 
+def q3(args):
+    """The changed-vertex triple search of q3.cu on the n=18,524 pair: evaluations per second."""
+    inst = bmatch.Instance.chunglu(2, 18524, 3_000_000)
+    evals = max(args.steps, 1) * 10
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        s0 = e.score()
+        e.eval_rot3(bmatch.q3_triples(inst.n, [1, 2, 3]), bmatch.BM_ROT3_ALL)  # first-call set-up and form choice
+        t = time.perf_counter()
+        best, st = e.q3(seed=1, max_evals=evals)
+        el = time.perf_counter() - t
+    out = _base("changed-vertex triple search evaluations per second at n=18.5k", st["evals"] / el, "evals/s",
+                int(st["evals"]), el * 1000 / st["evals"],
+                {"workload": "Chung-Lu pair n=18524 E=3.0M, 30% shuffled start, q3.cu changed-vertex triple "
+                             f"search, {evals} evaluations", "n": inst.n})
+    out.update({"seconds": el, "initial_score": s0, "best_score": st["best_score"],
+                "triples_per_s": st["triples"] / el, "stats": st})
+    return out
+
+
 def run(name, args):
     return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset, "batched": batched,
             "batched-c1": batched_c1, "batched-permset": batched_permset, "batched-many": batched_many,
-            "quick": quick, "serve": serve, "serve-sweep": serve_sweep}[name](args)
+            "quick": quick, "q3": q3, "serve": serve, "serve-sweep": serve_sweep}[name](args)

@@ -431,3 +431,159 @@ out:
     free(q.rows);
     return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * q3.cu changed-vertex triple search */
+
+int64_t bm_q3_triples(int32_t n, const int32_t *chg, int32_t k, int32_t *out) {
+    if (n < 3 || k < 1 || k > n || !chg) return BM_EINVAL;
+    for (int32_t i = 0; i < k; i++)
+        if (chg[i] < 1 || chg[i] > n || (i > 0 && chg[i] <= chg[i - 1])) return BM_EINVAL;
+    const int64_t nr = (int64_t)n - k;
+    const int64_t count = (int64_t)k * (k - 1) * (k - 2) / 6 + (int64_t)k * (k - 1) / 2 * nr + (nr > 0 ? k * (nr - 1) : 0);
+    if (!out) return count;
+    int32_t *rest = malloc(sizeof(int32_t) * (size_t)(nr > 0 ? nr : 1));
+    if (!rest) return BM_ENOMEM;
+    for (int32_t v = 1, i = 0, r = 0; v <= n; v++) { /* fillArray: the unchanged vertices ascending */
+        if (i < k && chg[i] == v) i++;
+        else rest[r++] = v;
+    }
+    int64_t c = 0;
+    for (int32_t i = 0; i < k; i++) /* inside the changed set (:70-102) */
+        for (int32_t j = i + 1; j < k; j++)
+            for (int32_t l = j + 1; l < k; l++, c++) {
+                out[3 * c] = chg[i];
+                out[3 * c + 1] = chg[j];
+                out[3 * c + 2] = chg[l];
+            }
+    for (int64_t r = 0; r < nr; r++) /* a changed pair and one unchanged vertex (:105-131) */
+        for (int32_t i = 0; i < k; i++)
+            for (int32_t j = i + 1; j < k; j++, c++) {
+                out[3 * c] = chg[i];
+                out[3 * c + 1] = chg[j];
+                out[3 * c + 2] = rest[r];
+            }
+    for (int32_t i = 0; i < k; i++) /* a changed vertex and consecutive unchanged ones (:133-150) */
+        for (int64_t t = 0; t + 1 < nr; t++, c++) {
+            out[3 * c] = chg[i];
+            out[3 * c + 1] = rest[t];
+            out[3 * c + 2] = rest[t + 1];
+        }
+    free(rest);
+    return c;
+}
+
+typedef struct {
+    int64_t count, npos, nzero, best_idx, triples;
+    int32_t best_delta;
+    int32_t *tri, *d, *p;
+    int64_t *pos, *zero;
+} q3_scan;
+
+/* score the triples of the changed set; the first maximum from -1 and the
+ * positive and zero triples in index order (q3.cu:1264-1289) */
+static int q3_eval(bm_ctx *ctx, int32_t n, const int32_t *chg, int32_t k, q3_scan *q) {
+    q->count = bm_q3_triples(n, chg, k, q->tri);
+    if (q->count < 0) return (int)q->count;
+    int rc = bm_eval_rot3(ctx, q->tri, q->count, BM_ROT3_ALL, q->d, q->p);
+    if (rc) return rc;
+    q->best_delta = -1;
+    q->best_idx = -1;
+    q->npos = q->nzero = 0;
+    for (int64_t i = 0; i < q->count; i++) {
+        const int32_t d = q->d[i];
+        if (d > q->best_delta) {
+            q->best_delta = d;
+            q->best_idx = i;
+        }
+        if (d > 0) q->pos[q->npos++] = i;
+        else if (d == 0) q->zero[q->nzero++] = i;
+    }
+    q->triples += q->count;
+    return BM_OK;
+}
+
+/* q3.cu:1176-1440 */
+int bm_drive_q3(bm_ctx *ctx, const bm_q3_params *p, int32_t *best_mapping, bm_q3_stats *stats) {
+    if (!ctx || !p || p->max_evals < 1 || p->zero_limit < 1 || p->best_window < 0 || !(p->gap_norm > 0))
+        return BM_EINVAL;
+    const int32_t n = bm_num_vertices(ctx);
+    if (n < 8) return BM_EINVAL; /* 4 swaps need 8 distinct vertices */
+    const int32_t eight[8] = {1, 2, 3, 4, 5, 6, 7, 8};
+    const int64_t cap = bm_q3_triples(n, eight, 8, NULL); /* the most a perturbation can ask for */
+    const size_t N1 = (size_t)n + 1;
+    bm_q3_stats st;
+    memset(&st, 0, sizeof st);
+    q3_scan q;
+    memset(&q, 0, sizeof q);
+    int32_t *best = malloc(sizeof(int32_t) * N1);
+    q.tri = malloc(sizeof(int32_t) * 3 * (size_t)cap);
+    q.d = malloc(sizeof(int32_t) * (size_t)cap);
+    q.p = malloc(sizeof(int32_t) * (size_t)cap);
+    q.pos = malloc(sizeof(int64_t) * (size_t)cap);
+    q.zero = malloc(sizeof(int64_t) * (size_t)cap);
+    int rc = BM_OK;
+    if (!best || !q.tri || !q.d || !q.p || !q.pos || !q.zero) {
+        rc = BM_ENOMEM;
+        goto out;
+    }
+    uint64_t rng = p->seed;
+    int64_t cur, iter = 0;
+    const int64_t last = 0; /* q3.cu never advances it */
+    if ((rc = bm_get_mapping(ctx, best)) || (rc = bm_score(ctx, &cur))) goto out;
+    st.initial_score = st.best_score = cur;
+    while (iter < p->max_evals) {
+        /* :1186-1211 perturbation (best_delta <= 0 whenever the move loop ended) */
+        const int k = 3 + (int)(bm_rng_next(&rng) % 2);
+        if (iter - last < p->best_window || (double)(st.best_score - cur) / p->gap_norm > p->gap) {
+            if ((rc = bm_set_mapping(ctx, best))) goto out;
+        }
+        int32_t chg[8];
+        if ((rc = bm_perturb(ctx, k, BM_PERTURB_WEIGHTED, &rng, chg))) goto out;
+        sort_small(chg, 2 * k);
+        if ((rc = bm_score(ctx, &cur))) goto out;
+        st.perturbations++;
+        if ((rc = q3_eval(ctx, n, chg, 2 * k, &q))) goto out;
+        iter++;
+        int zerocheck = 0;
+        /* :1292-1434 */
+        while ((q.best_delta > 0 || q.nzero > 0) && zerocheck < p->zero_limit && iter < p->max_evals) {
+            int64_t t;
+            if (q.best_delta > 0) {
+                t = bm_rng_next(&rng) % 2 == 0 ? q.pos[bm_rng_next(&rng) % (uint64_t)q.npos] : q.best_idx;
+                if ((rc = bm_apply_rot3(ctx, q.tri[3 * t], q.tri[3 * t + 1], q.tri[3 * t + 2], q.p[t]))) goto out;
+                if ((rc = bm_score(ctx, &cur))) goto out; /* :1293 */
+                zerocheck = 0;
+                st.positive_moves++;
+            } else {
+                t = q.zero[bm_rng_next(&rng) % (uint64_t)q.nzero];
+                if ((rc = bm_apply_rot3(ctx, q.tri[3 * t], q.tri[3 * t + 1], q.tri[3 * t + 2], q.p[t]))) goto out;
+                zerocheck++;
+                st.zero_moves++;
+            }
+            st.moves++;
+            if (cur > st.best_score) {
+                st.best_score = cur;
+                if ((rc = bm_get_mapping(ctx, best))) goto out;
+            }
+            int32_t m3[3] = {q.tri[3 * t], q.tri[3 * t + 1], q.tri[3 * t + 2]};
+            sort_small(m3, 3);
+            if ((rc = q3_eval(ctx, n, m3, 3, &q))) goto out;
+            iter++;
+        }
+    }
+    st.evals = iter;
+    st.triples = q.triples;
+    st.final_score = cur;
+    rc = bm_set_mapping(ctx, best);
+out:
+    if (!rc && best_mapping) memcpy(best_mapping, best, sizeof(int32_t) * N1);
+    if (!rc && stats) *stats = st;
+    free(best);
+    free(q.tri);
+    free(q.d);
+    free(q.p);
+    free(q.pos);
+    free(q.zero);
+    return rc;
+}

@@ -225,6 +225,48 @@ typedef struct {
 
 int bm_drive_quick(bm_ctx *ctx, const bm_quick_params *params, int32_t *best_mapping, bm_quick_stats *stats);
 
+/* The triples q3.cu's calculateThreeNodeSwapKernel enumerates for the changed
+ * vertices chg[0..k) (ascending, distinct, in [1, n]) of an n-vertex graph,
+ * in its thread-index order (q3.cu:59-150): the triples inside the changed
+ * set (lexicographic), then for each unchanged vertex (ascending) every
+ * changed pair, then for each changed vertex every pair of consecutive
+ * unchanged vertices; the unchanged vertices are 1..n without chg, ascending
+ * (fillArray, :982-996).  Writes 3 ids per triple to out when non-NULL and
+ * returns the count (negative on bad arguments). */
+int64_t bm_q3_triples(int32_t n, const int32_t *chg, int32_t k, int32_t *out);
+
+/* The changed-vertex triple search of q3.cu:1176-1440 (one process, no
+ * best-file synchronisation): perturb by 3 + rand()%2 swaps
+ * (bm_perturb(BM_PERTURB_WEIGHTED)), from the best mapping while fewer than
+ * best_window evaluations ran (the reference never advances its `last`
+ * marker) or while the score trails the best by more than gap x gap_norm;
+ * score every triple of bm_q3_triples for the sorted changed vertices with
+ * Delta_G3 over p = 1..5 (bm_eval_rot3, BM_ROT3_ALL); then, while a triple
+ * has Delta > 0 or = 0 and fewer than zero_limit zero moves ran in a row:
+ * with Delta > 0 a random positive triple or the first best one (even
+ * odds), else a random zero triple; apply it (bm_apply_rot3); the score is
+ * rescored after positive moves only, as the reference does (:1293);
+ * evaluate the triples of the three moved vertices again.  Random choices
+ * come from a splitmix64 state seeded with seed, in the reference's order
+ * of rand() calls.  Stops after max_evals evaluations and leaves the
+ * context on the best mapping. */
+typedef struct {
+    uint64_t seed;
+    int64_t max_evals;
+    int32_t zero_limit;  /* 1000 */
+    int32_t best_window; /* 1500 */
+    double gap;          /* 0.002 */
+    double gap_norm;     /* 5154247 */
+} bm_q3_params;
+
+typedef struct {
+    int64_t evals, perturbations, moves, positive_moves, zero_moves;
+    int64_t initial_score, final_score, best_score;
+    int64_t triples; /* triples evaluated over all evaluations */
+} bm_q3_stats;
+
+int bm_drive_q3(bm_ctx *ctx, const bm_q3_params *params, int32_t *best_mapping, bm_q3_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

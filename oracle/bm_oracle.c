@@ -1045,3 +1045,146 @@ int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_eval
     free(q.zero);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * q3.cu changed-vertex triple search
+ */
+
+/* q3.cu:59-150 with fillArray :982-996 */
+int64_t orc_q3_triples(int n, const int32_t *chg, int k, int32_t *out) {
+    int32_t *rest = xcalloc((size_t)n + 1, sizeof(int32_t));
+    uint8_t *in = xcalloc((size_t)n + 2, 1);
+    for (int i = 0; i < k; i++) in[chg[i]] = 1;
+    int nr = 0;
+    for (int v = 1; v <= n; v++)
+        if (!in[v]) rest[nr++] = v;
+    int64_t c = 0;
+#define ORC_Q3_PUT(x, y, z)                  \
+    do {                                     \
+        if (out) {                           \
+            out[3 * c] = (x);                \
+            out[3 * c + 1] = (y);            \
+            out[3 * c + 2] = (z);            \
+        }                                    \
+        c++;                                 \
+    } while (0)
+    for (int i = 0; i < k; i++) /* first section */
+        for (int j = i + 1; j < k; j++)
+            for (int l = j + 1; l < k; l++) ORC_Q3_PUT(chg[i], chg[j], chg[l]);
+    for (int r = 0; r < nr; r++) /* second: idx / pairs = unchanged index, idx % pairs = the pair */
+        for (int i = 0; i < k; i++)
+            for (int j = i + 1; j < k; j++) ORC_Q3_PUT(chg[i], chg[j], rest[r]);
+    for (int i = 0; i < k; i++) /* third: idx / (nr - 1) = changed index, idx % (nr - 1) = t */
+        for (int t = 0; t + 1 < nr; t++) ORC_Q3_PUT(chg[i], rest[t], rest[t + 1]);
+#undef ORC_Q3_PUT
+    free(rest);
+    free(in);
+    return c;
+}
+
+typedef struct {
+    int64_t count, npos, nzero, best_idx, triples;
+    int32_t best_delta;
+    int32_t *tri, *d, *p;
+    int64_t *pos, *zero;
+} orc_q3scan;
+
+/* evaluate every enumerated triple; best = first maximum (from -1,
+ * :1268-1273), positives and zeros in index order (:1276-1288) */
+static void orc_q3_scan(const orc_graphs *g, const int32_t *map, const int32_t *chg, int k, orc_q3scan *q) {
+    q->count = orc_q3_triples(g->n, chg, k, q->tri);
+    q->best_delta = -1;
+    q->best_idx = -1;
+    q->npos = q->nzero = 0;
+    for (int64_t i = 0; i < q->count; i++) {
+        int bp;
+        int d = orc_best_g3(g, map, q->tri[3 * i], q->tri[3 * i + 1], q->tri[3 * i + 2], 1, &bp);
+        q->d[i] = d;
+        q->p[i] = bp;
+        if (d > q->best_delta) {
+            q->best_delta = d;
+            q->best_idx = i;
+        }
+        if (d > 0) q->pos[q->npos++] = i;
+        if (d == 0) q->zero[q->nzero++] = i;
+    }
+    q->triples += q->count;
+}
+
+/* q3.cu:1176-1440 */
+int orc_q3(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit, int best_window,
+           double gap, double gap_norm, int64_t *out) {
+    const int n = g->n;
+    if (n < 8) return -1;
+    const size_t bytes = ((size_t)n + 1) * sizeof(int32_t);
+    const int64_t cap = orc_q3_triples(n, (const int32_t[]){1, 2, 3, 4, 5, 6, 7, 8}, 8, NULL) + 8;
+    orc_q3scan q;
+    memset(&q, 0, sizeof q);
+    q.tri = xcalloc((size_t)cap * 3, sizeof(int32_t));
+    q.d = xcalloc((size_t)cap, sizeof(int32_t));
+    q.p = xcalloc((size_t)cap, sizeof(int32_t));
+    q.pos = xcalloc((size_t)cap, sizeof(int64_t));
+    q.zero = xcalloc((size_t)cap, sizeof(int64_t));
+    int32_t *best = xcalloc((size_t)n + 1, sizeof(int32_t));
+    uint64_t rng = seed;
+    memcpy(best, map, bytes);
+    int64_t cur = orc_score_opt(g, map), initial = cur, best_score = cur;
+    const int64_t last = 0; /* never updated in q3.cu */
+    int64_t iter = 0, perturbations = 0, moves = 0, pos_moves = 0, zero_moves = 0;
+    while (iter < max_evals) {
+        const int k = 3 + (int)(orc_rng_next(&rng) % 2); /* :1189 */
+        if (iter - last < best_window || (double)(best_score - cur) / gap_norm > gap) memcpy(map, best, bytes);
+        int32_t chg[8];
+        orc_perturb(g, map, k, 1, &rng, chg);
+        orc_sort_ints(chg, 2 * k);
+        cur = orc_score_opt(g, map);
+        perturbations++;
+        orc_q3_scan(g, map, chg, 2 * k, &q);
+        iter++;
+        int zerocheck = 0;
+        while ((q.best_delta > 0 || q.nzero > 0) && zerocheck < zero_limit && iter < max_evals) {
+            int64_t t;
+            if (q.best_delta > 0) {
+                if (orc_rng_next(&rng) % 2 == 0) t = q.pos[orc_rng_next(&rng) % (uint64_t)q.npos];
+                else t = q.best_idx;
+                orc_apply_g3(map, q.tri[3 * t], q.tri[3 * t + 1], q.tri[3 * t + 2], q.p[t]);
+                cur = orc_score_opt(g, map); /* :1293 */
+                zerocheck = 0;
+                pos_moves++;
+            } else {
+                t = q.zero[orc_rng_next(&rng) % (uint64_t)q.nzero];
+                orc_apply_g3(map, q.tri[3 * t], q.tri[3 * t + 1], q.tri[3 * t + 2], q.p[t]);
+                zerocheck++;
+                zero_moves++;
+            }
+            moves++;
+            if (cur > best_score) {
+                best_score = cur;
+                memcpy(best, map, bytes);
+            }
+            int32_t m3[3] = {q.tri[3 * t], q.tri[3 * t + 1], q.tri[3 * t + 2]};
+            orc_sort_ints(m3, 3);
+            orc_q3_scan(g, map, m3, 3, &q);
+            iter++;
+        }
+    }
+    if (out) {
+        out[0] = iter;
+        out[1] = perturbations;
+        out[2] = moves;
+        out[3] = pos_moves;
+        out[4] = zero_moves;
+        out[5] = initial;
+        out[6] = cur;
+        out[7] = best_score;
+        out[8] = q.triples;
+    }
+    memcpy(map, best, bytes);
+    free(best);
+    free(q.tri);
+    free(q.d);
+    free(q.p);
+    free(q.pos);
+    free(q.zero);
+    return 0;
+}

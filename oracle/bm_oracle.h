@@ -172,6 +172,31 @@ int orc_row_best_restart(const orc_graphs *g, int32_t *map, uint64_t seed, int i
 int orc_quick(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit,
               int best_window, double gap, double gap_norm, int kind, int64_t *out);
 
+/* q3.cu:59-150: the triples calculateThreeNodeSwapKernel enumerates for the
+ * changed vertices chg[0..k) (sorted ascending) of an n-vertex graph, in
+ * its index order: (1) triples inside the changed set, lexicographic;
+ * (2) for each unchanged vertex r (ascending), the changed pairs i < j
+ * lexicographic with r; (3) for each changed vertex c, the consecutive
+ * unchanged pairs (r_t, r_t+1).  The unchanged vertices are 1..n without
+ * chg, ascending (fillArray, :982-996).  Writes 3 ids per triple to out
+ * when non-NULL; returns the count. */
+int64_t orc_q3_triples(int n, const int32_t *chg, int k, int32_t *out);
+/* q3.cu:1176-1440 changed-vertex triple search for max_evals evaluations
+ * (one process, no best-file sync): perturbation by 3 + rand()%2 weighted
+ * swaps from the best mapping while iter < best_window (the reference never
+ * moves `last` from 0) or while the score trails the best by more than
+ * gap x gap_norm, else from the current one; every enumerated triple scored
+ * with Delta_G3 over p = 1..5; then, while a triple has Delta > 0 or = 0 and
+ * fewer than zero_limit zero moves ran in a row: with Delta > 0 a random
+ * positive triple or the first best one (even odds), else a random zero
+ * triple; apply_permutation; the score is rescored after positive moves only
+ * (:1293), as there; the moved triple is evaluated again.  rand() is
+ * orc_rng_next.  map ends on the best mapping; out[0..8] = evals,
+ * perturbations, moves, positive moves, zero moves, initial, final
+ * (current), best score, triples evaluated. */
+int orc_q3(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit, int best_window,
+           double gap, double gap_norm, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,8 @@ def lib():
         "orc_best_g3_many": (None, [vp, _i32p, _i32p, C.c_int64, C.c_int, _i32p, _i32p, C.c_int]),
         "orc_best_perm_many": (None, [vp, _i32p, _i32p, C.c_int, C.c_int64, _i32p, _i64p, C.c_int]),
         "orc_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
+        "orc_q3_triples": (C.c_int64, [C.c_int, _i32p, C.c_int, vp]),
+        "orc_q3": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, _i64p]),
         "orc_vertex_scores": (None, [vp, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]),
         "orc_perturb": (C.c_int, [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_uint64), _i32p]),
         "orc_row_best_restart": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int64, _i64p]),
@@ -164,6 +166,15 @@ class Oracle:
                 "best_score", "pairs")
         return mm, dict(zip(keys, (int(x) for x in out)))
 
+    def q3(self, m, seed=1, max_evals=100, zero_limit=1000, best_window=1500, gap=0.002, gap_norm=5154247.0):
+        mm = _i32(m).copy()
+        out = np.zeros(9, np.int64)
+        if lib().orc_q3(self._h, mm, seed, max_evals, zero_limit, best_window, gap, gap_norm, out):
+            raise ValueError("orc_q3 failed")
+        keys = ("evals", "perturbations", "moves", "positive_moves", "zero_moves", "initial_score", "final_score",
+                "best_score", "triples")
+        return mm, dict(zip(keys, (int(x) for x in out)))
+
     def delta_exact(self, m, a, b):
         return lib().orc_delta_swap_exact(self._h, _i32(m), a, b)
 
@@ -235,6 +246,14 @@ class Oracle:
         cs = C.c_int64()
         ev = lib().orc_omp_rowbest_rows(self._h, _i32(m), lo, hi, threads, C.byref(cs))
         return ev, cs.value
+
+
+def q3_triples(n, changed):
+    c = _i32(changed)
+    cnt = lib().orc_q3_triples(n, c, len(c), None)
+    out = np.zeros(3 * cnt, np.int32)
+    lib().orc_q3_triples(n, c, len(c), out.ctypes.data_as(C.c_void_p))
+    return out.reshape(cnt, 3)
 
 
 def unrank_perm(k, rank):
