@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "bm_device.h"
@@ -1272,12 +1273,14 @@ extern "C" int bm_refine_run(bm_ctx *c, int64_t max_steps, int64_t move_budget, 
     return BM_OK;
 }
 
-// Several graph pairs in one launch per chunk: a parent graph holds, for each
-// running context, its captured step graph as steps_per_check chained child
-// nodes followed by the copy of its counters; the contexts share no buffers, so
-// the branches are independent.  After each launch every context's stop rules
-// are applied exactly as bm_refine_run applies them, finished contexts leave
-// the pack and the parent graph is rebuilt for the rest.
+// Several graph pairs at once from one host thread: every context's step
+// graph is replayed on the context's own stream, steps_per_check steps per
+// chunk followed by the copy of its counters, so the pairs' kernels overlap on
+// the device (one parent graph holding every pair's steps as child nodes was
+// slower than one pair after another: ROCm runs the branches of a graph in
+// order).  When a pair's chunk has finished, its stop rules are applied
+// exactly as bm_refine_run applies them and its next chunk is enqueued at
+// once; the others keep running.
 extern "C" int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t max_steps, int64_t move_budget,
                                   int32_t patience, bm_refine_stats *stats) {
     if (!ctxs || !stats || count < 1) return set_error(BM_EINVAL, "NULL argument or count < 1");
@@ -1293,48 +1296,31 @@ extern "C" int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t ma
     }
     BM_HIP(hipSetDevice(ctxs[0]->device));
     int rc;
+    std::vector<int64_t> done(count, 0);
+    std::vector<int> chunk(count, 0);
+    auto enqueue = [&](int i) {
+        chunk[i] = (int)std::min<int64_t>(ctxs[i]->batch->prm.steps_per_check, max_steps - done[i]);
+        return refine_enqueue(ctxs[i], chunk[i]);
+    };
     for (int i = 0; i < count; i++) {
-        if ((rc = refine_start(ctxs[i]))) return rc;
-        BM_HIP(hipStreamSynchronize(ctxs[i]->stream));
         memset(&stats[i], 0, sizeof stats[i]);
+        if ((rc = refine_start(ctxs[i])) || (rc = enqueue(i))) return rc;
     }
     std::vector<int> live(count);
-    std::vector<int64_t> done(count, 0);
     for (int i = 0; i < count; i++) live[i] = i;
-    hipStream_t stream = ctxs[0]->stream;
+    int first_error = BM_OK;
     while (!live.empty()) {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        std::vector<int> chunk(count, 0);
-        int err = BM_OK;
-        std::unique_lock<std::mutex> glk(sync_mutex());  // graph calls are serialised (refine_enqueue)
-        BM_HIP(hipGraphCreate(&g, 0));
-        for (int i : live) {
-            bm_batch_state *b = ctxs[i]->batch;
-            chunk[i] = (int)std::min<int64_t>(b->prm.steps_per_check, max_steps - done[i]);
-            hipGraphNode_t prev = nullptr, node = nullptr;
-            for (int s = 0; s < chunk[i] && !err; s++) {
-                if (hipGraphAddChildGraphNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, b->graph) != hipSuccess)
-                    err = set_error(BM_EHIP, "hipGraphAddChildGraphNode failed");
-                prev = node;
-            }
-            if (!err && hipGraphAddMemcpyNode1D(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, b->hst, b->st,
-                                                sizeof(RefineState), hipMemcpyDeviceToHost) != hipSuccess)
-                err = set_error(BM_EHIP, "hipGraphAddMemcpyNode1D failed");
-        }
-        if (!err && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
-            err = set_error(BM_EHIP, "hipGraphInstantiate failed");
-        if (!err && hipGraphLaunch(ge, stream) != hipSuccess) err = set_error(BM_EHIP, "hipGraphLaunch failed");
-        glk.unlock();
-        if (!err && hipStreamSynchronize(stream) != hipSuccess) err = set_error(BM_EHIP, "pack failed");
-        glk.lock();
-        if (ge) (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
-        glk.unlock();
-        if (err) return err;
         std::vector<int> next;
+        bool progressed = false;
         for (int i : live) {
             bm_ctx *c = ctxs[i];
+            const int pr = refine_poll(c, false);
+            if (pr < 0) return pr;
+            if (pr == 0) {  // still running
+                next.push_back(i);
+                continue;
+            }
+            progressed = true;
             done[i] += chunk[i];
             bm_refine_stats &st = stats[i];
             refine_stats(c, &st);
@@ -1344,17 +1330,19 @@ extern "C" int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t ma
                 fin = true;
             }
             if (!fin) {
+                if ((rc = enqueue(i))) return rc;
                 next.push_back(i);
                 continue;
             }
             if ((rc = refine_finish(c))) return rc;
-            if (!st.verified)
-                return set_error(BM_ERANGE, "context %d: exact score check failed at step %lld", i,
-                                 (long long)st.bad_step);
+            if (!st.verified && !first_error)
+                first_error = set_error(BM_ERANGE, "context %d: exact score check failed at step %lld", i,
+                                        (long long)st.bad_step);
         }
         live.swap(next);
+        if (!progressed && !live.empty()) std::this_thread::yield();
     }
-    return BM_OK;
+    return first_error;
 }
 
 extern "C" int bm_refine_history(bm_ctx *c, int64_t first, int64_t count, bm_refine_step *out) {

@@ -291,9 +291,12 @@ int bm_refine_configure(bm_ctx *ctx, const bm_refine_params *params);
  * bm_get_mapping) then holds the result.  BM_ERANGE if an exact check failed. */
 int bm_refine_run(bm_ctx *ctx, int64_t max_steps, int64_t move_budget, int32_t patience, bm_refine_stats *stats);
 /* Runs `count` configured contexts (graph pairs of any sizes, one device) as
- * bm_refine_run would run each alone, with one graph launch per chunk for all
- * of them: each context's step graph, steps_per_check times, is a branch of one
- * parent graph.  stats[i] is context i's result (identical to its solo run). */
+ * bm_refine_run would run each alone, all at once from the calling thread:
+ * each context's step graph is replayed on its own stream, steps_per_check
+ * steps per chunk, and a context whose chunk has finished gets its stop rules
+ * applied and its next chunk enqueued while the others run.  stats[i] is
+ * context i's result (identical to its solo run); BM_ERANGE (the first
+ * failing context) when an exact check failed, after every context ended. */
 int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t max_steps, int64_t move_budget, int32_t patience,
                        bm_refine_stats *stats);
 /* Per-step records of the last run, steps [first, first+count) (the last 4096 are kept). */
