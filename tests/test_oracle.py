@@ -341,3 +341,37 @@ def test_omp_rowbest_checksum_matches_sequential(threads):
         want += md
     assert cs == want
     assert r["best"][2] == max(0, max(r["row_best_delta"][1:40]))
+
+
+def test_parallel_helpers_equal_the_single_routines():
+    """The OpenMP helpers the full-size checks use against the routines they batch."""
+    n = 40
+    g1 = random_multigraph(21, n, 600, 0.1, 0.05, zero_vertex=True)
+    g2 = random_multigraph(22, n, 600, 0.1, 0.05, zero_vertex=True)
+    mp = random_mapping(23, n)
+    o = ob.Oracle(n, g1, g2)
+    ap = o.all_pairs(mp)
+    rows = np.arange(1, n + 1, dtype=np.int32)
+    b, d = o.row_best_rows(mp, rows, 4)
+    np.testing.assert_array_equal(b, ap["row_best_b"][1:])
+    np.testing.assert_array_equal(d, ap["row_best_delta"][1:])
+    full = o.delta_rows(mp, np.array([3, 17, 40], np.int32), 4)
+    for i, a in enumerate((3, 17, 40)):
+        want = [0 if x in (0, a) else o.delta_b(mp, a, x) for x in range(n + 1)]
+        assert full[i].tolist() == want
+    # first-improvement rows: the sweep driver's first rows (a simple graph, where no rescore corrects a delta)
+    s = simple_graph(24, n, 300)
+    s2 = simple_graph(25, n, 320)
+    os_ = ob.Oracle(n, s, s2)
+    m0 = random_mapping(26, n)
+    full_run = os_.first_improvement(m0, max_sweeps=1)
+    rows_map, acc = os_.first_improvement_rows(m0, n, 4)
+    np.testing.assert_array_equal(acc, full_run["accepted"])
+    np.testing.assert_array_equal(rows_map, full_run["mapping"])
+    rng = np.random.default_rng(27)
+    tri = np.array([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(200)], np.int32)
+    dd, pp = o.best_g3_many(mp, tri, 1, 4)
+    assert [(int(x), int(y)) for x, y in zip(dd, pp)] == [o.best_g3(mp, *map(int, t), 1) for t in tri]
+    sets = np.array([rng.choice(np.arange(1, n + 1), 5, replace=False) for _ in range(20)], np.int32)
+    sd, sr = o.best_perm_many(mp, sets, 4)
+    assert [(int(x), int(y)) for x, y in zip(sd, sr)] == [o.best_perm(mp, s_) for s_ in sets]
