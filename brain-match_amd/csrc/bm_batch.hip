@@ -90,9 +90,9 @@ extern "C" int bm_moves_set(bm_ctx *c, int32_t k, const int32_t *verts, int64_t 
                 planes[i] = verts[2 * i];
                 planes[count + i] = verts[2 * i + 1];
             }
-            BM_HIP(hipMemcpy(c->mv_verts, planes.data(), planes.size() * 4, hipMemcpyHostToDevice));
+            if (int rc_ = copy_sync(c->mv_verts, planes.data(), planes.size() * 4, hipMemcpyHostToDevice)) return rc_;
         } else {
-            BM_HIP(hipMemcpy(c->mv_verts, verts, (size_t)count * k * 4, hipMemcpyHostToDevice));
+            if (int rc_ = copy_sync(c->mv_verts, verts, (size_t)count * k * 4, hipMemcpyHostToDevice)) return rc_;
         }
     }
     c->mv_k = k;
@@ -146,7 +146,9 @@ extern "C" int bm_moves_eval(bm_ctx *c, int mode, int64_t *best_index, int32_t *
         const int64_t i = (int64_t)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu));
         *best_index = i;
         *best_delta = (int32_t)((uint32_t)(key >> 32) ^ 0x80000000u);
-        if (c->mv_k >= 3) BM_HIP(hipMemcpy(best_aux, c->mv_aux + i, 4, hipMemcpyDeviceToHost));
+        if (c->mv_k >= 3) {
+            if (int rc_ = copy_sync(best_aux, c->mv_aux + i, 4, hipMemcpyDeviceToHost)) return rc_;
+        }
     }
     return BM_OK;
 }
@@ -156,10 +158,15 @@ extern "C" int bm_moves_results(bm_ctx *c, int32_t *delta, int32_t *aux) {
     if (!c->mv_k) return set_error(BM_ESTATE, "no resident move list");
     BM_HIP(hipSetDevice(c->device));
     const size_t m = (size_t)c->mv_count;
-    if (delta && m) BM_HIP(hipMemcpy(delta, c->mv_delta, m * 4, hipMemcpyDeviceToHost));
+    if (delta && m) {
+        if (int rc_ = copy_sync(delta, c->mv_delta, m * 4, hipMemcpyDeviceToHost)) return rc_;
+    }
     if (aux && m) {
-        if (c->mv_k >= 3) BM_HIP(hipMemcpy(aux, c->mv_aux, m * 4, hipMemcpyDeviceToHost));
-        else memset(aux, 0, m * 4);
+        if (c->mv_k >= 3) {
+            if (int rc_ = copy_sync(aux, c->mv_aux, m * 4, hipMemcpyDeviceToHost)) return rc_;
+        } else {
+            memset(aux, 0, m * 4);
+        }
     }
     return BM_OK;
 }

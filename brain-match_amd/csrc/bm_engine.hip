@@ -54,6 +54,12 @@ int dev_alloc(bm_ctx *c, void **p, size_t bytes) {
     return BM_OK;
 }
 
+int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    std::lock_guard<std::mutex> g(sync_mutex());
+    BM_HIP(hipMemcpy(dst, src, bytes, kind));
+    return BM_OK;
+}
+
 void dev_free(bm_ctx *c, void **p, size_t bytes) {
     if (*p) {
         std::lock_guard<std::mutex> g(sync_mutex());
@@ -579,10 +585,7 @@ template <typename T>
 static int upload(bm_ctx *c, T **dst, const std::vector<T> &src) {
     int rc = dev_alloc(c, (void **)dst, src.size() * sizeof(T));
     if (rc) return rc;
-    if (!src.empty()) {
-        std::lock_guard<std::mutex> g(sync_mutex());
-        BM_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-    }
+    if (!src.empty()) return copy_sync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
     return BM_OK;
 }
 
@@ -687,6 +690,13 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->pinv, N1 * 4);
     dev_free(c, (void **)&c->g2oy, (size_t)c->ng2 * 8);  // g2iy lives in the same allocation
     c->g2iy = nullptr;
+    dev_free(c, (void **)&c->g2q, (size_t)c->nquads * 16);
+    dev_free(c, (void **)&c->qoptr, (N1 + 1) * 4);
+    dev_free(c, (void **)&c->qiptr, (N1 + 1) * 4);
+    c->nquads = 0;
+    dev_free(c, (void **)&c->crws, (size_t)c->ncomb * 8);
+    dev_free(c, (void **)&c->cdws, (size_t)c->ncomb * 4);
+    dev_free(c, (void **)&c->pdg, N1 * 4);
     dev_free(c, (void **)&c->X, (N1 + 63) / 64 * 64 * (size_t)c->ldx * 4);
     dev_free(c, (void **)&c->keys, (N1 + 8) * 8);
     dev_free(c, (void **)&c->vsimple, N1);
@@ -754,7 +764,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     // ---- G2 (on a second host thread while this one builds the G1 lists):
     // last line per ordered pair, then its out- and in-lists packed (vertex << 16 | weight)
     std::vector<int32_t> tri;
-    bool g2nonneg = true, g2_vertex0 = false;
+    bool g2nonneg = true, g2_vertex0 = false, g2small = true;
     std::vector<int32_t> g2optr(N1 + 1, 0), g2iptr(N1 + 1, 0);
     std::vector<uint32_t> g2o, g2i;
     std::vector<int32_t> g2o_at_in, g2i_at_out;
@@ -778,6 +788,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
                     tri.push_back(od[e]);
                     tri.push_back(ow[e]);
                     if ((int16_t)ow[e] < 0) g2nonneg = false;
+                    if ((int16_t)ow[e] < 0 || (int16_t)ow[e] > 16383) g2small = false;
                 }
         m2 = (int64_t)tri.size() / 3;
         for (int64_t i = 0; i < m2; i++) {
@@ -903,6 +914,7 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
 
     if (g2_thread.joinable()) g2_thread.join();
     c->g2nonneg = g2nonneg;
+    c->g2q_small = g2small;
     c->g1nonneg = nonneg;
     c->nonneg = nonneg && g2nonneg;
     // every line its own dense cell: no self-loop, no repeated pair, no vertex 0, weights in [1, 32767]

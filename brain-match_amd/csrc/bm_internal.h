@@ -97,6 +97,17 @@ struct bm_ctx {
     int32_t *pinv = nullptr;  // pinv[g] = the vertex mapped to g (-1 if none); valid when sparse_ok
     // the G2 lists with targets as P coordinates (p^-1), 0xFFFF = none; one allocation, g2iy = g2oy + ng2
     uint32_t *g2oy = nullptr, *g2iy = nullptr;
+    // the sparse sweep's copy of the translated lists (lazy, rewritten before each sparse
+    // sweep): every list padded to whole 16-byte quads of 4 words, out-lists then
+    // in-lists, an untranslated or padding word replaced by N1 << 16 (a zero term
+    // aimed at the row accumulator's spare slot N1)
+    uint32_t *g2q = nullptr;
+    int32_t *qoptr = nullptr, *qiptr = nullptr;  // quad index of each out- and in-list in g2q
+    int64_t nquads = 0;
+    // crw and cdw with each row's entries in one global order (the sparse sweep's walk)
+    int32_t *crws = nullptr, *cdws = nullptr;
+    int32_t *pdg = nullptr;  // P[x][x] (the sparse sweep, written with g2q)
+    bool g2q_small = false;  // every list weight in [0, 16383]: word >> 14 = 4 * target
     int32_t *g2o_at_in = nullptr;   // per in-list entry (line u->g): its index in the out-lists
     int32_t *g2i_at_out = nullptr;  // per out-list entry (line g->v): its index in the in-lists
     bool g2y_dirty = true;
@@ -177,6 +188,9 @@ std::mutex &sync_mutex();
 
 int dev_alloc(bm_ctx *c, void **p, size_t bytes);
 void dev_free(bm_ctx *c, void **p, size_t bytes);
+// hipMemcpy under sync_mutex: a blocking copy runs on the null stream, which
+// waits for every blocking stream, a capturing one included
+int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 int ensure_stage(bm_ctx *c, size_t bytes);
 
 // kernel timing helpers (no-ops unless c->timing)

@@ -1357,8 +1357,9 @@ extern "C" int bm_refine_history(bm_ctx *c, int64_t first, int64_t count, bm_ref
     for (int64_t done = 0; done < count;) {
         const int64_t slot = (first + done) % kHist;
         const int64_t n = std::min<int64_t>(count - done, kHist - slot);
-        BM_HIP(hipMemcpy(out + done, c->batch->hist + slot, sizeof(bm_refine_step) * (size_t)n,
-                         hipMemcpyDeviceToHost));
+        if (int rc_ = copy_sync(out + done, c->batch->hist + slot, sizeof(bm_refine_step) * (size_t)n,
+                                hipMemcpyDeviceToHost))
+            return rc_;
         done += n;
     }
     return BM_OK;
@@ -1381,7 +1382,8 @@ extern "C" int bm_refine_moves(bm_ctx *c, int64_t *count, int32_t *recs, int64_t
     *count = b->hst->log_count;
     if (recs && capacity > 0 && held > 0) {
         BM_HIP(hipSetDevice(c->device));
-        BM_HIP(hipMemcpy(recs, b->log, (size_t)std::min(held, capacity) * sizeof(MoveRec), hipMemcpyDeviceToHost));
+        if (int rc_ = copy_sync(recs, b->log, (size_t)std::min(held, capacity) * sizeof(MoveRec), hipMemcpyDeviceToHost))
+            return rc_;
     }
     return BM_OK;
 }

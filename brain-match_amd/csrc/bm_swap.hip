@@ -579,13 +579,12 @@ static int tune_gain(bm_ctx *c) {
 // row a accumulates into an (n+1)-int LDS row.  The row's comb entries are
 // taken in chunks; per chunk the G2 lists they select are laid end to end
 // (prefix sums in LDS) and the 16 waves split that flat index space evenly,
-// lanes of a wave reading consecutive list entries (coalesced 4-byte
-// (target << 16 | weight) words) and ds_add-ing the contribution.  Work is the
-// sum over G1 lines of the G2 degree instead of 2E * n.  Exact when p is
-// injective on mapped vertices and no G2 line names vertex 0 (W2 row/column 0
-// empty); the caller checks that.
+// lanes of a wave reading consecutive 16-byte quads of 4 list words
+// ((target << 16 | weight) words, coalesced) and ds_add-ing each word's
+// contribution.  Work is the sum over G1 lines of the G2 degree instead of
+// 2E * n.  Exact when p is injective on mapped vertices and no G2 line names
+// vertex 0 (W2 row/column 0 empty); the caller checks that.
 constexpr int kSparseChunk = 512;
-constexpr int kSparseUnroll = 8;  // list entries in flight per lane
 constexpr int kSplitEntries = 2048;  // comb entries per block of the sparse kernel, at most
 // Long G2 lists streamed as whole Q rows, per chunk of entries.  Kept small:
 // two blocks per CU need the static LDS plus 4 (n+1) bytes under 80 KiB.
@@ -616,65 +615,126 @@ int bm::launch_refresh_g2y(bm_ctx *c) {
 }
 
 // One block per work item (a, lo, hi, flags) of the row plan: entries
-// [lo, hi) of comb(a).  A row with one item (flags & kRowWhole) is finished
-// here: X(a,a) into diag, the corrections, the row stored.  The items of a
-// split row add their partial rows into X (zeroed beforehand) with atomics;
-// the first of them (flags & kRowFirst) adds cneg and the corrections, and
-// X(a,a) becomes diag[a] afterwards (k_diag_from_x).
+// [lo, hi) of comb(a), with the pair corrections those entries carry.  A row
+// with one item (flags & kRowWhole) is finished here: X(a,a) into diag, the
+// row stored.  The items of a split row add their partial rows into X (zeroed
+// beforehand) with atomics; the first of them (flags & kRowFirst) adds cneg,
+// and X(a,a) becomes diag[a] afterwards (k_diag_from_x).
 constexpr int kRowFirst = 1, kRowWhole = 2;
 
-// kAll: the mapping is a bijection of 1..n, so every list target translates
-// (no 0xFFFF word); the list words are read through a buffer resource with
-// 32-bit byte offsets.
-template <bool kHubs, bool kNeg, bool kAll>
-__global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ plan, const int32_t *__restrict__ cptr,
-                                                      const int2 *__restrict__ crw,
-                                                      const int32_t *__restrict__ cdw,
-                                                      const int32_t *__restrict__ cnegv,
-                                                      const int32_t *__restrict__ map,
-                                                      const int32_t *__restrict__ g2optr,
-                                                      const uint32_t *__restrict__ g2oy,
-                                                      const int32_t *__restrict__ g2iptr,
-                                                      const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
-                                                      int ld, int32_t *__restrict__ X, int64_t ldx,
-                                                      int32_t *__restrict__ diag, int N1, int dense_len,
-                                                      int ng2, const int *__restrict__ run_flag) {
+// The translated lists regrouped for the sparse kernel: one wave per list
+// (out-lists 0..N1-1, then in-lists) copies its words to its quads; the words
+// that pad the last quad and the untranslated ones (0xFFFF) become N1 << 16,
+// a term min(w, 0) - min(w, 0) = 0 (or min(w, 0) = 0 when no G1 weight is
+// negative) added to the spare accumulator slot N1.  Also P's diagonal.
+__global__ __launch_bounds__(256) void k_pack_quads(uint32_t *__restrict__ g2q, const uint32_t *__restrict__ g2y,
+                                                    const int32_t *__restrict__ g2optr,
+                                                    const int32_t *__restrict__ g2iptr,
+                                                    const int32_t *__restrict__ qoptr, const int32_t *__restrict__ qiptr,
+                                                    const int16_t *__restrict__ Q, int ld, int32_t *__restrict__ pdg,
+                                                    int N1, int64_t ng2, const int *__restrict__ run_flag) {
+    if (run_flag && *run_flag == 0) return;
+    const int list = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (list >= 2 * N1) return;
+    const bool in = list >= N1;
+    const int g = in ? list - N1 : list;
+    if (!in && lane == 0) pdg[g] = Q[(size_t)g * ld + g];  // P[g][g], for the pair corrections
+    const int lo = in ? g2iptr[g] : g2optr[g], len = (in ? g2iptr[g + 1] : g2optr[g + 1]) - lo;
+    const uint32_t *src = g2y + (in ? ng2 : 0) + lo;
+    uint32_t *dst = g2q + 4 * (int64_t)(in ? qiptr[g] : qoptr[g]);
+    const uint32_t pad = (uint32_t)N1 << 16;
+    for (int i = lane; i < ((len + 3) & ~3); i += 64) {
+        uint32_t w = i < len ? src[i] : pad;
+        if ((w >> 16) == 0xFFFFu) w = pad;
+        dst[i] = w;
+    }
+}
+
+// The sparse row over the quad copy of the lists.  The flat index space of a
+// chunk counts quads and a lane loads one 16-byte quad (4 list words) per
+// step, so the entry bookkeeping (is the lane past its entry's end, the list
+// start) and the address are paid once per 4 words.  The row accumulator is
+// the first thing in LDS (byte address 0: the kernel has no static LDS), so
+// under kSmall (every list weight in [0, 16383], whose word then has zero bits
+// 14-15) the byte address of target y is the word shifted right by 14.  No
+// word is skipped: untranslated targets were made zero terms by k_pack_quads.
+// Each row walks its entries in increasing neighbour id (crws), one order for
+// all rows, so that the workgroups resident on an XCD read the same G2 lists
+// at about the same time (L2 hits 46 -> 53 %).  Scalar registers are capped:
+// above about 72 the GPU holds one 1024-thread workgroup per CU instead of two
+// (measured: 1.84 ms against 1.40 ms at n = 18,524), although the compiler's
+// occupancy estimate is 8 waves per SIMD for both.
+constexpr int kQuadUnroll = 2;  // quads in flight per lane (1: 1.61 ms, 2: 1.58, 3: 1.59, 4: 1.60 at C2)
+
+static size_t sparse_lds_bytes(int N1) {
+    // acc[N1 + 8 rounded to 4], pref[kSparseChunk + 2], lw[kSparseChunk] (int2),
+    // hubr[kSparseHubs], hubw[kSparseHubs], wsum[16], nhub (+3 spare ints)
+    return ((size_t)((N1 + 8 + 3) & ~3) + (kSparseChunk + 2) + 2 * kSparseChunk + 2 * kSparseHubs + 16 + 4) * 4;
+}
+
+template <bool kHubs, bool kNeg, bool kSmall>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(80))) void k_gain_sparse(
+    const int4 *__restrict__ plan, const int2 *__restrict__ crws, const int32_t *__restrict__ cdws,
+    const int32_t *__restrict__ pdg, const int32_t *__restrict__ cnegv, const int32_t *__restrict__ map,
+    const int32_t *__restrict__ qoptr, const int32_t *__restrict__ qiptr, const uint4 *__restrict__ g2q,
+    const int16_t *__restrict__ Q, int ld, int32_t *__restrict__ X, int64_t ldx, int32_t *__restrict__ diag, int N1,
+    int dense_len, const int *__restrict__ run_flag) {
     if (run_flag && *run_flag == 0) return;  // X already current (the batched step's incremental update)
-    extern __shared__ __align__(16) int acc[];
-    __shared__ int pref[kSparseChunk + 1];
-    __shared__ int2 lw[kSparseChunk];  // per entry: list start in g2oy (the in-lists follow) and weight
-    __shared__ int hubr[kHubs ? kSparseHubs : 1], hubw[kHubs ? kSparseHubs : 1];
-    __shared__ int wsum[16];
-    __shared__ int nhub;
+    extern __shared__ __align__(16) int smem[];
+    int *acc = smem;  // LDS byte address 0
+    int *pref = smem + ((N1 + 8 + 3) & ~3);
+    int2 *lw = (int2 *)(pref + kSparseChunk + 2);  // per entry: first quad of its list, weight
+    int *hubr = (int *)(lw + kSparseChunk), *hubw = hubr + kSparseHubs;
+    int *wsum = hubw + kSparseHubs;
+    int *nhub = wsum + 16;
     const int4 item = plan[blockIdx.x];
     const int a = item.x, tid = threadIdx.x, flags = item.w;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     for (int i = tid; i < N1; i += 1024) acc[i] = 0;
+    const int paa = (int)Q[(size_t)a * ld + a];
+    // one list word: f(w, q) = min(w, q) - min(w, 0) into the target's slot
+    auto add = [&](uint32_t yv, int w) {
+        const int v = min(w, (int)(int16_t)(yv & 0xffff)) - (kNeg ? min(w, 0) : 0);
+        if (kSmall)  // the LDS address itself: smem starts at byte 0
+            __hip_atomic_fetch_add((__attribute__((address_space(3))) int *)(size_t)(yv >> 14), v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            atomicAdd(&acc[yv >> 16], v);
+    };
     const int beg = item.y, end = item.z;
     for (int c0 = beg; c0 < end; c0 += kSparseChunk) {
         const int m = min(kSparseChunk, end - c0);
-        if (tid == 0) nhub = 0;
+        if (tid == 0) *nhub = 0;
         __syncthreads();
-        // per entry: which G2 list (start, length) and its weight; an entry
-        // whose list is longer than dense_len streams its Q row instead
-        int len = 0;
+        int len = 0;  // in quads
         if (tid < m) {
-            const int2 rw = crw[c0 + tid];
-            const bool out = rw.x >= N1;  // row of P^T (out entry a->d) or of P (in entry s->a)
-            const int g = map[out ? rw.x - N1 : rw.x];
-            const int lo = out ? g2iptr[g] : g2optr[g];
-            len = (out ? g2iptr[g + 1] : g2optr[g + 1]) - lo;
-            lw[tid] = make_int2(out ? ng2 + lo : lo, rw.y);  // into g2oy, or g2iy = g2oy + ng2
-            if (kHubs && len > dense_len) {
-                const int h = atomicAdd(&nhub, 1);
+            const int2 rw = crws[c0 + tid];
+            const bool out = rw.x >= N1;  // row of P^T (out entry a->d: the in-list of p(d)) or of P
+            const int b = out ? rw.x - N1 : rw.x;
+            // Corr(a, b) for b > a from this entry (each G1 line between a and b is one
+            // entry of comb(a)); P[b][a] read as P^T[a][b], so the reads stay in two rows.
+            // Its loads go first, so that they overlap the list lookups below.
+            int pab = 0, pba = 0, pbb = 0, dw = 0;
+            if (b > a) {
+                pab = (int)Q[(size_t)a * ld + b];
+                pba = (int)Q[(size_t)(N1 + a) * ld + b];
+                pbb = pdg[b];
+                dw = cdws[c0 + tid];
+            }
+            const int g = map[b];
+            const int lo = out ? qiptr[g] : qoptr[g];
+            len = (out ? qiptr[g + 1] : qoptr[g + 1]) - lo;
+            lw[tid] = make_int2(lo, rw.y);
+            if (kHubs && 4 * len > dense_len) {
+                const int h = atomicAdd(nhub, 1);
                 if (h < kSparseHubs) {  // beyond the cap the list is walked like the others
                     hubr[h] = rw.x;
                     hubw[h] = rw.y;
                     len = 0;
                 }
             }
+            if (b > a) atomicAdd(&acc[b], pair_corr(out, rw.y, dw, paa, pbb, pab, pba));
         }
-        // exclusive prefix sum of the list lengths (first 512 threads)
         int x = len;
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(x, o);
@@ -695,65 +755,60 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
         if (tid == 0) pref[0] = 0;
         __syncthreads();
         const int total = pref[m];
-        // the 16 waves split [0, total) into contiguous spans
         const int span = (total + 15) / 16;
         const int w0 = wave * span, w1 = min(total, w0 + span);
         if (w0 < w1) {
-            int lo_i = 0, hi_i = m - 1;  // entry holding flat index w0
+            int lo_i = 0, hi_i = m - 1;  // entry holding flat quad w0
             while (lo_i < hi_i) {
                 const int mid = (lo_i + hi_i + 1) >> 1;
                 if (pref[mid] <= w0) lo_i = mid;
                 else hi_i = mid - 1;
             }
-            // kSparseUnroll list entries in flight per lane: indices idx + 64u.  The lane's
-            // current entry i0 and its flat range [s0, s1) stay in registers; pref is read
-            // only when the lane moves on to the next entry.
             int i0 = lo_i, s0 = pref[i0], s1 = pref[i0 + 1];
-            int2 cur = lw[i0];  // the current entry's list start and weight
-            int idx = w0 + lane;
-            for (; idx + 64 * (kSparseUnroll - 1) < w1; idx += 64 * kSparseUnroll) {
-                int at[kSparseUnroll], wu[kSparseUnroll];
+            int2 cur = lw[i0];
+            int q = w0 + lane;
+            for (; q + 64 * (kQuadUnroll - 1) < w1; q += 64 * kQuadUnroll) {
+                int at[kQuadUnroll], wu[kQuadUnroll];
 #pragma unroll
-                for (int u = 0; u < kSparseUnroll; u++) {
-                    while (s1 <= idx + 64 * u) {
+                for (int u = 0; u < kQuadUnroll; u++) {
+                    while (s1 <= q + 64 * u) {
                         i0++;
                         s0 = s1;
                         s1 = pref[i0 + 1];
                         cur = lw[i0];
                     }
-                    at[u] = cur.x - s0;  // word idx + 64u of the flat range is g2oy[at + idx + 64u]
+                    at[u] = cur.x - s0;  // flat quad q + 64u is g2q[at + q + 64u]
                     wu[u] = cur.y;
                 }
-                uint32_t yv[kSparseUnroll];
+                uint4 v[kQuadUnroll];
 #pragma unroll
-                for (int u = 0; u < kSparseUnroll; u++) yv[u] = g2oy[(uint32_t)(at[u] + idx) + 64u * u];
+                for (int u = 0; u < kQuadUnroll; u++) v[u] = g2q[(uint32_t)(at[u] + q) + 64u * u];
 #pragma unroll
-                for (int u = 0; u < kSparseUnroll; u++) {
-                    const int y = yv[u] >> 16;
-                    if (kAll || y != 0xFFFF) {
-                        const int w = wu[u];
-                        // f(w, q) = min(w, q) - min(w, 0); kNeg: some G1 weight is negative
-                        atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - (kNeg ? min(w, 0) : 0));
-                    }
+                for (int u = 0; u < kQuadUnroll; u++) {
+                    add(v[u].x, wu[u]);
+                    add(v[u].y, wu[u]);
+                    add(v[u].z, wu[u]);
+                    add(v[u].w, wu[u]);
                 }
             }
-            for (; idx < w1; idx += 64) {
-                while (s1 <= idx) {
+            for (; q < w1; q += 64) {
+                while (s1 <= q) {
                     i0++;
                     s0 = s1;
                     s1 = pref[i0 + 1];
                     cur = lw[i0];
                 }
-                const uint32_t yv = g2oy[cur.x + idx - s0];
-                const int y = yv >> 16;
-                if (kAll || y != 0xFFFF)
-                    atomicAdd(&acc[y], min(cur.y, (int)(int16_t)(yv & 0xffff)) - (kNeg ? min(cur.y, 0) : 0));
+                const uint4 v = g2q[cur.x - s0 + q];
+                add(v.x, cur.y);
+                add(v.y, cur.y);
+                add(v.z, cur.y);
+                add(v.w, cur.y);
             }
         }
         __syncthreads();
-        // long lists: whole Q rows instead, up to four folded per pass; each
-        // lane owns an adjacent pair of columns (64-bit LDS accesses)
-        const int nh = kHubs ? min(nhub, kSparseHubs) : 0;
+        // long lists: whole Q rows, up to four folded per pass; each lane owns an adjacent
+        // pair of columns (64-bit LDS accesses)
+        const int nh = kHubs ? min(*nhub, kSparseHubs) : 0;
         for (int h0 = 0; h0 < nh; h0 += 4) {
             const int g = min(4, nh - h0);
             const int16_t *row[4];
@@ -768,43 +823,28 @@ __global__ __launch_bounds__(1024) void k_gain_sparse(const int4 *__restrict__ p
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     if (u < g) {
-                        const bm_s2 q = *(const bm_s2 *)(row[u] + y);
+                        const bm_s2 qq = *(const bm_s2 *)(row[u] + y);
                         const int base = -min(w[u], 0);
-                        v.x += min(w[u], (int)q[0]) + base;
-                        v.y += min(w[u], (int)q[1]) + base;
+                        v.x += min(w[u], (int)qq[0]) + base;
+                        v.y += min(w[u], (int)qq[1]) + base;
                     }
                 }
-                *(int2 *)&acc[y] = v;  // acc has N1 + 1 slots: the spare one absorbs y + 1 = N1
+                *(int2 *)&acc[y] = v;  // the spare slot N1 absorbs y + 1 = N1
             }
         }
         __syncthreads();
     }
     __syncthreads();  // also orders the zeroing of acc when the item has no entries
-    // X(a,a) before corrections; then Corr(a,b) for b > a from a's own entries
-    // (each G1 line between a and b is in comb(a) exactly once)
+    // X(a,a) (no correction lands on slot a)
     const int ca = cnegv[a];
     if ((flags & kRowWhole) && tid == 0) diag[a] = acc[a] + ca;
-    __syncthreads();
-    if (flags & kRowFirst) {
-        const int paa = (int)Q[(size_t)a * ld + a];
-        for (int e = cptr[a] + tid; e < cptr[a + 1]; e += 1024) {
-            const int2 rw = crw[e];
-            const bool out = rw.x >= N1;
-            const int b = out ? rw.x - N1 : rw.x;
-            if (b <= a) continue;
-            const int pbb = (int)Q[(size_t)b * ld + b], pab = (int)Q[(size_t)a * ld + b],
-                      pba = (int)Q[(size_t)b * ld + a];
-            atomicAdd(&acc[b], pair_corr(out, rw.y, cdw[e], paa, pbb, pab, pba));
-        }
-    }
-    __syncthreads();
     const int T = (int)(ldx >> 6);
     if (flags & kRowWhole) {
         for (int i = tid; i < ldx; i += 1024) __builtin_nontemporal_store(i < N1 ? acc[i] + ca : 0, X + xidx(a, i, T));
     } else {
-        const int add = (flags & kRowFirst) ? ca : 0;
+        const int add0 = (flags & kRowFirst) ? ca : 0;
         for (int i = tid; i < N1; i += 1024) {
-            const int v = acc[i] + add;
+            const int v = acc[i] + add0;
             if (v) atomicAdd(X + xidx(a, i, T), v);
         }
     }
@@ -840,7 +880,56 @@ __global__ void k_diag_from_x(const int32_t *__restrict__ X, int64_t ldx, const 
 // The row plan of the sparse kernel: rows with more than kSplitEntries comb
 // entries become ceil(deg / kSplitEntries) items; items ordered by entry count,
 // largest first, so that the longest blocks start first.
+static int build_quads(bm_ctx *c) {
+    if (c->g2q) return BM_OK;
+    const int N1 = c->N1;
+    std::vector<int32_t> qo(N1 + 1, 0), qi(N1 + 1, 0);
+    for (int g = 0; g < N1; g++) qo[g + 1] = qo[g] + (c->hg2optr[g + 1] - c->hg2optr[g] + 3) / 4;
+    qi[0] = qo[N1];
+    for (int g = 0; g < N1; g++) qi[g + 1] = qi[g] + (c->hg2iptr[g + 1] - c->hg2iptr[g] + 3) / 4;
+    int rc;
+    if (!c->qoptr && (rc = dev_alloc(c, (void **)&c->qoptr, (size_t)(N1 + 1) * 4))) return rc;
+    if (!c->qiptr && (rc = dev_alloc(c, (void **)&c->qiptr, (size_t)(N1 + 1) * 4))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->g2q, (size_t)qi[N1] * 16))) return rc;
+    c->nquads = qi[N1];
+    if ((rc = copy_sync(c->qoptr, qo.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice))) return rc;
+    if ((rc = copy_sync(c->qiptr, qi.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice))) return rc;
+    {
+        // each row's entries in increasing neighbour id, ties in list order: a
+        // counting pass by neighbour, then a stable one by row
+        const size_t nc = (size_t)c->ncomb;
+        std::vector<int2> rw(nc), rws(nc);
+        std::vector<int32_t> dw(nc), dws(nc), byx(nc), rowof(nc);
+        std::vector<int64_t> pos(N1 + 1, 0);
+        if ((rc = copy_sync(rw.data(), c->crw, nc * 8, hipMemcpyDeviceToHost))) return rc;
+        if ((rc = copy_sync(dw.data(), c->cdw, nc * 4, hipMemcpyDeviceToHost))) return rc;
+        auto nb = [&](size_t i) { return rw[i].x >= N1 ? rw[i].x - N1 : rw[i].x; };
+        for (size_t i = 0; i < nc; i++) pos[nb(i) + 1]++;
+        for (int v = 0; v < N1; v++) pos[v + 1] += pos[v];
+        for (size_t i = 0; i < nc; i++) byx[pos[nb(i)]++] = (int32_t)i;
+        pos[0] = 0;
+        for (int a = 0; a < N1; a++) {
+            pos[a + 1] = pos[a] + c->hcdeg[a];
+            for (int64_t k = pos[a]; k < pos[a + 1]; k++) rowof[k] = a;
+        }
+        for (size_t k = 0; k < nc; k++) {
+            const int32_t i = byx[k];
+            const int64_t j = pos[rowof[i]]++;
+            rws[j] = rw[i];
+            dws[j] = dw[i];
+        }
+        if (!c->crws && (rc = dev_alloc(c, (void **)&c->crws, nc * 8))) return rc;
+        if (!c->cdws && (rc = dev_alloc(c, (void **)&c->cdws, nc * 4))) return rc;
+        if (!c->pdg && (rc = dev_alloc(c, (void **)&c->pdg, (size_t)N1 * 4))) return rc;
+        if ((rc = copy_sync(c->crws, rws.data(), nc * 8, hipMemcpyHostToDevice))) return rc;
+        if ((rc = copy_sync(c->cdws, dws.data(), nc * 4, hipMemcpyHostToDevice))) return rc;
+    }
+    return BM_OK;
+}
+
 static int build_row_plan(bm_ctx *c) {
+    int rc0 = build_quads(c);
+    if (rc0) return rc0;
     if (c->plan) return BM_OK;
     int split = kSplitEntries;
     if (const char *e = getenv("BM_SPARSE_SPLIT")) split = std::max(4, atoi(e));
@@ -865,11 +954,11 @@ static int build_row_plan(bm_ctx *c) {
     int rc;
     if ((rc = dev_alloc(c, (void **)&c->plan, items.size() * sizeof(int4)))) return rc;
     c->nplan = (int64_t)items.size();
-    BM_HIP(hipMemcpy(c->plan, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if ((rc = copy_sync(c->plan, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice))) return rc;
     c->nsplit = (int)rows.size();
     if (!rows.empty()) {
         if ((rc = dev_alloc(c, (void **)&c->split_rows, rows.size() * 4))) return rc;
-        BM_HIP(hipMemcpy(c->split_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+        if ((rc = copy_sync(c->split_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice))) return rc;
     }
     return BM_OK;
 }
@@ -879,7 +968,6 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
         int rc = launch_refresh_g2y(c);
         if (rc) return rc;
     }
-    const size_t lds = ((size_t)c->N1 + 8) * 4;  // a spare slot past N1 for the paired column writes
     int rc = build_row_plan(c);
     if (rc) return rc;
     if (!c->sparse_attr) {  // once per context, so that captured launches make no attribute calls
@@ -892,22 +980,27 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
                               (const void *)k_gain_sparse<false, true, true>,
                               (const void *)k_gain_sparse<false, false, true>};
         for (const void *f : fns)
-            BM_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kSparseLdsMax + 32)));
+            BM_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sparse_lds_bytes((int)(kSparseLdsMax / 4))));
         c->sparse_attr = true;
     }
     if (c->nsplit > 0)
         BM_LAUNCH(c, "zero_rows", hipLaunchKernelGGL(k_zero_rows, dim3(c->nsplit), dim3(256), 0, c->stream, c->X,
                                                      c->ldx, c->split_rows, run_flag));
+    BM_LAUNCH(c, "pack_quads",
+              hipLaunchKernelGGL(k_pack_quads, dim3((unsigned)((2 * c->N1 + 3) / 4)), dim3(256), 0, c->stream, c->g2q,
+                                 c->g2oy, c->g2optr, c->g2iptr, c->qoptr, c->qiptr, c->Q, c->ld, c->pdg, c->N1, c->ng2,
+                                 run_flag));
     // the long-list path costs registers and scheduling even when idle: only
     // the instantiation that needs it carries it; likewise the min(w, 0) term
-#define BM_GAIN_SPARSE(H, NEG, ALL)                                                                                 \
-    BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain_sparse<H, NEG, ALL>), dim3((unsigned)c->nplan), dim3(1024), lds, \
-                                            c->stream, c->plan, c->cptr, (const int2 *)c->crw, c->cdw, c->cneg, c->map, \
-                                            c->g2optr, c->g2oy, c->g2iptr, c->g2iy, c->Q, c->ld, c->X, c->ldx, c->diag, \
-                                            c->N1, c->dense_len, (int)c->ng2, run_flag))
     const bool hubs = c->g2_maxlen > c->dense_len, neg = !c->g1nonneg;
-    // every target translated (the mapping is a bijection of 1..n): no list word is skipped
-    if (c->all_mapped) {
+    const size_t lds = sparse_lds_bytes(c->N1);
+#define BM_GAIN_SPARSE(H, NEG, SMALL)                                                                                  \
+    BM_LAUNCH(c, "gain", hipLaunchKernelGGL((k_gain_sparse<H, NEG, SMALL>), dim3((unsigned)c->nplan), dim3(1024), lds, \
+                                            c->stream, c->plan, (const int2 *)c->crws, c->cdws, c->pdg, c->cneg,       \
+                                            c->map, c->qoptr, c->qiptr, (const uint4 *)c->g2q, c->Q, c->ld, c->X,      \
+                                            c->ldx, c->diag, c->N1, c->dense_len, run_flag))
+    if (c->g2q_small) {
         if (hubs && neg) BM_GAIN_SPARSE(true, true, true);
         else if (hubs) BM_GAIN_SPARSE(true, false, true);
         else if (neg) BM_GAIN_SPARSE(false, true, true);
