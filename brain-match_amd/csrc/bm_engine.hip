@@ -693,6 +693,7 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->g2q, (size_t)c->nquads * 16);
     dev_free(c, (void **)&c->qoptr, (N1 + 1) * 4);
     dev_free(c, (void **)&c->qiptr, (N1 + 1) * 4);
+    dev_free(c, (void **)&c->g2qpos, (size_t)c->ng2 * 8);
     c->nquads = 0;
     dev_free(c, (void **)&c->crws, (size_t)c->ncomb * 8);
     dev_free(c, (void **)&c->cdws, (size_t)c->ncomb * 4);

@@ -103,6 +103,7 @@ struct bm_ctx {
     // aimed at the row accumulator's spare slot N1)
     uint32_t *g2q = nullptr;
     int32_t *qoptr = nullptr, *qiptr = nullptr;  // quad index of each out- and in-list in g2q
+    int32_t *g2qpos = nullptr;  // per word of g2oy / g2iy (2 ng2): its place in g2q
     int64_t nquads = 0;
     // crw and cdw with each row's entries in one global order (the sparse sweep's walk)
     int32_t *crws = nullptr, *cdws = nullptr;

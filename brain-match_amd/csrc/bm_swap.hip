@@ -622,32 +622,21 @@ int bm::launch_refresh_g2y(bm_ctx *c) {
 // and X(a,a) becomes diag[a] afterwards (k_diag_from_x).
 constexpr int kRowFirst = 1, kRowWhole = 2;
 
-// The translated lists regrouped for the sparse kernel: one wave per list
-// (out-lists 0..N1-1, then in-lists) copies its words to its quads; the words
-// that pad the last quad and the untranslated ones (0xFFFF) become N1 << 16,
-// a term min(w, 0) - min(w, 0) = 0 (or min(w, 0) = 0 when no G1 weight is
-// negative) added to the spare accumulator slot N1.  Also P's diagonal.
+// The translated lists copied into their quads (the padding words, N1 << 16,
+// are written once when the quads are laid out): one thread per list word,
+// an untranslated word (0xFFFF) becoming N1 << 16 too, a term
+// min(w, 0) - min(w, 0) = 0 (or min(w, 0) = 0 when no G1 weight is negative)
+// added to the spare accumulator slot N1.  Also P's diagonal.
 __global__ __launch_bounds__(256) void k_pack_quads(uint32_t *__restrict__ g2q, const uint32_t *__restrict__ g2y,
-                                                    const int32_t *__restrict__ g2optr,
-                                                    const int32_t *__restrict__ g2iptr,
-                                                    const int32_t *__restrict__ qoptr, const int32_t *__restrict__ qiptr,
+                                                    const int32_t *__restrict__ pos, int64_t m,
                                                     const int16_t *__restrict__ Q, int ld, int32_t *__restrict__ pdg,
-                                                    int N1, int64_t ng2, const int *__restrict__ run_flag) {
+                                                    int N1, const int *__restrict__ run_flag) {
     if (run_flag && *run_flag == 0) return;
-    const int list = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (list >= 2 * N1) return;
-    const bool in = list >= N1;
-    const int g = in ? list - N1 : list;
-    if (!in && lane == 0) pdg[g] = Q[(size_t)g * ld + g];  // P[g][g], for the pair corrections
-    const int lo = in ? g2iptr[g] : g2optr[g], len = (in ? g2iptr[g + 1] : g2optr[g + 1]) - lo;
-    const uint32_t *src = g2y + (in ? ng2 : 0) + lo;
-    uint32_t *dst = g2q + 4 * (int64_t)(in ? qiptr[g] : qoptr[g]);
-    const uint32_t pad = (uint32_t)N1 << 16;
-    for (int i = lane; i < ((len + 3) & ~3); i += 64) {
-        uint32_t w = i < len ? src[i] : pad;
-        if ((w >> 16) == 0xFFFFu) w = pad;
-        dst[i] = w;
-    }
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N1) pdg[i] = Q[i * ld + i];  // P[i][i], for the pair corrections
+    if (i >= m) return;
+    const uint32_t w = g2y[i];
+    g2q[pos[i]] = (w >> 16) == 0xFFFFu ? (uint32_t)N1 << 16 : w;
 }
 
 // The sparse row over the quad copy of the lists.  The flat index space of a
@@ -888,12 +877,23 @@ static int build_quads(bm_ctx *c) {
     qi[0] = qo[N1];
     for (int g = 0; g < N1; g++) qi[g + 1] = qi[g] + (c->hg2iptr[g + 1] - c->hg2iptr[g] + 3) / 4;
     int rc;
+    // each word's place: out-list words first (g2oy), then in-list words (g2iy = g2oy + ng2)
+    std::vector<int32_t> pos((size_t)c->ng2 * 2);
+    for (int g = 0; g < N1; g++) {
+        for (int k = c->hg2optr[g]; k < c->hg2optr[g + 1]; k++) pos[k] = 4 * qo[g] + (k - c->hg2optr[g]);
+        for (int k = c->hg2iptr[g]; k < c->hg2iptr[g + 1]; k++)
+            pos[c->ng2 + k] = 4 * qi[g] + (k - c->hg2iptr[g]);
+    }
+    const std::vector<uint32_t> pads((size_t)qi[N1] * 4, (uint32_t)N1 << 16);
     if (!c->qoptr && (rc = dev_alloc(c, (void **)&c->qoptr, (size_t)(N1 + 1) * 4))) return rc;
     if (!c->qiptr && (rc = dev_alloc(c, (void **)&c->qiptr, (size_t)(N1 + 1) * 4))) return rc;
-    if ((rc = dev_alloc(c, (void **)&c->g2q, (size_t)qi[N1] * 16))) return rc;
+    if (!c->g2qpos && (rc = dev_alloc(c, (void **)&c->g2qpos, pos.size() * 4 + 4))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->g2q, pads.size() * 4))) return rc;
     c->nquads = qi[N1];
     if ((rc = copy_sync(c->qoptr, qo.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice))) return rc;
     if ((rc = copy_sync(c->qiptr, qi.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice))) return rc;
+    if (!pos.empty() && (rc = copy_sync(c->g2qpos, pos.data(), pos.size() * 4, hipMemcpyHostToDevice))) return rc;
+    if ((rc = copy_sync(c->g2q, pads.data(), pads.size() * 4, hipMemcpyHostToDevice))) return rc;
     {
         // each row's entries in increasing neighbour id, ties in list order: a
         // counting pass by neighbour, then a stable one by row
@@ -987,10 +987,12 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
     if (c->nsplit > 0)
         BM_LAUNCH(c, "zero_rows", hipLaunchKernelGGL(k_zero_rows, dim3(c->nsplit), dim3(256), 0, c->stream, c->X,
                                                      c->ldx, c->split_rows, run_flag));
-    BM_LAUNCH(c, "pack_quads",
-              hipLaunchKernelGGL(k_pack_quads, dim3((unsigned)((2 * c->N1 + 3) / 4)), dim3(256), 0, c->stream, c->g2q,
-                                 c->g2oy, c->g2optr, c->g2iptr, c->qoptr, c->qiptr, c->Q, c->ld, c->pdg, c->N1, c->ng2,
-                                 run_flag));
+    {
+        const int64_t m = 2 * c->ng2, threads = std::max<int64_t>(m, c->N1);
+        BM_LAUNCH(c, "pack_quads",
+                  hipLaunchKernelGGL(k_pack_quads, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+                                     c->g2q, c->g2oy, c->g2qpos, m, c->Q, c->ld, c->pdg, c->N1, run_flag));
+    }
     // the long-list path costs registers and scheduling even when idle: only
     // the instantiation that needs it carries it; likewise the min(w, 0) term
     const bool hubs = c->g2_maxlen > c->dense_len, neg = !c->g1nonneg;
