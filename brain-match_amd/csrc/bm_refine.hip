@@ -283,9 +283,16 @@ __device__ __forceinline__ void unrank(int k, int rank, int *perm) {
 }
 
 // thread per winner: new p(v[q]) = old p(v[src[q]]); the sets are disjoint
+// With ulists set (the incremental update of X), each vertex whose image
+// changes is flagged in moved[] and gets its two G2 lists in ulists: entry
+// 2x the out-list, 2x + 1 the in-list (its words at ng2 + g2iptr in the
+// back-to-back translated lists), each as (start at the new image, length,
+// start at the old image, length).
 __global__ void k_apply(const int32_t *__restrict__ win, int max_win, CandView cv, int32_t *__restrict__ map,
                         int32_t *__restrict__ pinv, int with_pinv, RefineState *__restrict__ st,
-                        MoveRec *__restrict__ log, int log_cap) {
+                        MoveRec *__restrict__ log, int log_cap, uint8_t *__restrict__ moved,
+                        int4 *__restrict__ ulists, const int32_t *__restrict__ g2optr,
+                        const int32_t *__restrict__ g2iptr, int ng2) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= max_win || s >= (int)st->nwin) return;
     const long long i = win[s];
@@ -301,6 +308,17 @@ __global__ void k_apply(const int32_t *__restrict__ win, int max_win, CandView c
         unrank(k, cv.set_r[i - cv.N1 - cv.T], src);
     }
     for (int q = 0; q < k; q++) old[q] = map[v[src[q]]];
+    if (ulists) {
+        for (int q = 0; q < k; q++) {
+            const int gnew = old[q], gold = map[v[q]];  // map is still the image before the move
+            if (gnew == gold) continue;
+            moved[v[q]] = 1;
+            ulists[2 * v[q]] = make_int4(g2optr[gnew], g2optr[gnew + 1] - g2optr[gnew], g2optr[gold],
+                                         g2optr[gold + 1] - g2optr[gold]);
+            ulists[2 * v[q] + 1] = make_int4(ng2 + g2iptr[gnew], g2iptr[gnew + 1] - g2iptr[gnew],
+                                             ng2 + g2iptr[gold], g2iptr[gold + 1] - g2iptr[gold]);
+        }
+    }
     for (int q = 0; q < k; q++) {
         map[v[q]] = old[q];
         if (with_pinv && old[q] != 0) pinv[old[q]] = v[q];
@@ -489,12 +507,14 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
 
 
 // Corr of every G1 line at U into X(min, max), times sign; a line between two
-// vertices of U is taken from the smaller one's list only.
+// vertices of U is taken from the smaller one's list only.  With rowmark set
+// (the pass before the move), every neighbour u of a vertex of U is marked:
+// row u of X is the one whose list entry mirrors this one (k_xdelta_rows).
 __global__ void k_corr_inc(int sign, const RefineState *__restrict__ st, const int32_t *__restrict__ ulist,
                            const int32_t *__restrict__ ubase, const uint8_t *__restrict__ inU,
                            const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
                            const int32_t *__restrict__ cdw, const int16_t *__restrict__ Q, int ld, int N1,
-                           int32_t *__restrict__ X, int64_t ldx) {
+                           int32_t *__restrict__ X, int64_t ldx, uint8_t *__restrict__ rowmark) {
     if (!st->inc) return;
     const int nu = st->nu, total = st->npairs, T = (int)(ldx >> 6);
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
@@ -504,6 +524,7 @@ __global__ void k_corr_inc(int sign, const RefineState *__restrict__ st, const i
         const int2 rw = crw[e];
         const bool out = rw.x >= N1;
         const int u = out ? rw.x - N1 : rw.x;
+        if (rowmark) rowmark[u] = 1;
         if (u == x || (inU[u] && u < x)) continue;
         const int a = min(x, u), b = max(x, u);
         const bool out_a = x == a ? out : !out;
@@ -557,83 +578,148 @@ __global__ __launch_bounds__(256) void k_xdelta(const RefineState *__restrict__ 
     }
 }
 
-// The same row updates as k_xdelta, gathered per row: the workgroup of row u
-// finds the entries of comb(u) whose neighbour x moved (an in entry x -> u
-// reads the G2 out-list of x's image, an out entry u -> x its in-list), adds
-// f over the lists at the new image and subtracts it over the old one in an
-// (n+1)-int LDS row (columns outside U only), then adds the row's non-zero
-// entries to X with plain read-modify-writes: one workgroup owns the row, so
-// the scattered global atomics of the per-entry form become coalesced
-// accesses along the row.  Rows without a moved neighbour return after the
-// scan of their list.
-__global__ __launch_bounds__(1024) void k_xdelta_rows(const RefineState *__restrict__ st,
-                                                      const uint8_t *__restrict__ inU,
-                                                      const int32_t *__restrict__ uoldv,
-                                                      const int32_t *__restrict__ map,
-                                                      const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
-                                                      const int32_t *__restrict__ g2optr,
-                                                      const uint32_t *__restrict__ g2oy,
-                                                      const int32_t *__restrict__ g2iptr,
-                                                      const uint32_t *__restrict__ g2iy, int N1,
-                                                      int32_t *__restrict__ X, int64_t ldx) {
+// The same row updates as k_xdelta, gathered per row: the workgroup of a
+// marked row u (k_corr_inc) finds the entries of comb(u) whose neighbour x
+// moved (an in entry x -> u reads the G2 out-list of x's image, an out entry
+// u -> x its in-list), lays the two lists of each such entry end to end (the
+// list at the new image, added, and the one at the old image, subtracted) and
+// splits that flat word range evenly over its 16 waves, so that a row whose
+// one moved neighbour is a hub keeps every wave busy.  The words accumulate
+// f(w, q) in an (n+1)-int LDS row, whose non-zero entries are then added to X
+// with plain read-modify-writes: one workgroup owns the row, so the accesses
+// are coalesced along it.  Targets inside U are accumulated too: the columns
+// of U are rewritten whole afterwards (k_xcol_sum).  Entries are taken in
+// passes of at most kXdeltaQual: the whole list when it holds that few moved
+// neighbours, else windows of kXdeltaQual consecutive entries.
+constexpr int kXdeltaUnroll = 4;  // list words in flight per lane
+constexpr int kXdeltaRmw = 8;     // row entries per thread whose X reads are issued together
+
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(80))) void k_xdelta_rows(
+    const RefineState *__restrict__ st, const uint8_t *__restrict__ rowmark, const uint8_t *__restrict__ moved,
+    const int4 *__restrict__ ulists, const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+    const uint32_t *__restrict__ g2y, int N1, int32_t *__restrict__ X, int64_t ldx) {
     extern __shared__ __align__(16) int row[];
     __shared__ int nq;
-    __shared__ int qual[kXdeltaQual];  // the row's entries with a moved neighbour
+    __shared__ int2 qual[kXdeltaQual];        // (row, weight) of the entries of this pass
+    __shared__ int lstart[2 * kXdeltaQual];   // per list of the pass: its first word in g2y
+    __shared__ int pref[2 * kXdeltaQual + 1];  // flat word offset of each list
+    __shared__ int wsum[16];
     if (!st->inc) return;
     const int T = (int)(ldx >> 6);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int u = blockIdx.x; u < N1; u += gridDim.x) {
-        if (threadIdx.x == 0) nq = 0;
-        __syncthreads();
-        const int e0 = cptr[u], e1 = cptr[u + 1];
-        for (int e = e0 + threadIdx.x; e < e1; e += 1024) {
-            const int r = crw[e].x;
-            const int x = r >= N1 ? r - N1 : r;
-            if (inU[x] && uoldv[x] != map[x]) {
-                const int slot = atomicAdd(&nq, 1);
-                if (slot < kXdeltaQual) qual[slot] = e;
-            }
+    auto neighbour = [&](int r) { return r >= N1 ? r - N1 : r; };
+    // the entries qual[0, m): their lists into the LDS row
+    auto walk = [&](int m) {
+        int len = 0, lnew = 0;
+        if ((int)threadIdx.x < m) {
+            const int r = qual[threadIdx.x].x;
+            // an out entry u -> x reads the in-list of x's image, an in entry x -> u its out-list
+            const int4 L = ulists[2 * neighbour(r) + (r >= N1 ? 1 : 0)];
+            lstart[2 * threadIdx.x] = L.x;
+            lstart[2 * threadIdx.x + 1] = L.z;
+            lnew = L.y;
+            len = L.y + L.w;
         }
+        int total;
+        const int ex = block_scan_excl(len, wsum, total);
+        if ((int)threadIdx.x < m) {
+            pref[2 * threadIdx.x] = ex;
+            pref[2 * threadIdx.x + 1] = ex + lnew;
+        }
+        if (threadIdx.x == 0) pref[2 * m] = total;
         __syncthreads();
-        const int nqual = nq;
-        if (nqual == 0) continue;  // uniform
-        for (int y = threadIdx.x; y < N1; y += 1024) row[y] = 0;
-        __syncthreads();
-        // the recorded entries, or (more than kXdeltaQual of them) the whole list again
-        const bool listed = nqual <= kXdeltaQual;
-        for (int i = wave; i < (listed ? nqual : e1 - e0); i += 16) {  // one wave per entry
-            const int e = listed ? qual[i] : e0 + i;
-            const int2 rw = crw[e];
-            const bool out = rw.x >= N1;
-            const int x = out ? rw.x - N1 : rw.x;
-            if (!inU[x]) continue;
-            const int gold = uoldv[x], gnew = map[x];
-            if (gold == gnew) continue;
-            const int w = rw.y, wneg = min(w, 0);
-            const int32_t *ptr = out ? g2iptr : g2optr;
-            const uint32_t *L = out ? g2iy : g2oy;
-            for (int side = 0; side < 2; side++) {
-                const int g = side ? gold : gnew, sg = side ? -1 : 1;
-                const int j1 = ptr[g + 1];
-                for (int j = ptr[g] + lane; j < j1; j += 256) {  // four list words in flight per lane
-                    uint32_t yv[4];
+        const int span = (total + 15) / 16;
+        const int w0 = wave * span, w1 = min(total, w0 + span);
+        if (w0 < w1) {
+            int lo = 0, hi = 2 * m - 1;  // the list holding flat word w0
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pref[mid] <= w0) lo = mid;
+                else hi = mid - 1;
+            }
+            int l = lo, s0 = pref[l], s1 = pref[l + 1];
+            int at = lstart[l] - s0, w = qual[l >> 1].y, sg = (l & 1) ? -1 : 1;
+            for (int q0 = w0 + lane; q0 < w1; q0 += 64 * kXdeltaUnroll) {
+                int ad[kXdeltaUnroll], wu[kXdeltaUnroll], su[kXdeltaUnroll];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) yv[q] = j + 64 * q < j1 ? L[j + 64 * q] : 0xFFFF0000u;
-                    uint8_t skip[4];
+                for (int k = 0; k < kXdeltaUnroll; k++) {
+                    const int q = q0 + 64 * k;
+                    ad[k] = -1;
+                    wu[k] = 0;
+                    su[k] = 1;
+                    if (q < w1) {
+                        while (s1 <= q) {
+                            l++;
+                            s0 = s1;
+                            s1 = pref[l + 1];
+                            at = lstart[l] - s0;
+                            w = qual[l >> 1].y;
+                            sg = (l & 1) ? -1 : 1;
+                        }
+                        ad[k] = at + q;
+                        wu[k] = w;
+                        su[k] = sg;
+                    }
+                }
+                uint32_t yv[kXdeltaUnroll];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) skip[q] = (yv[q] >> 16) == 0xFFFF ? 1 : inU[yv[q] >> 16];
+                for (int k = 0; k < kXdeltaUnroll; k++) yv[k] = ad[k] >= 0 ? g2y[ad[k]] : 0xFFFF0000u;
 #pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        if (!skip[q]) atomicAdd(&row[yv[q] >> 16], sg * (min(w, (int)(int16_t)(yv[q] & 0xffff)) - wneg));
+                for (int k = 0; k < kXdeltaUnroll; k++) {
+                    if ((yv[k] >> 16) != 0xFFFFu)
+                        atomicAdd(&row[yv[k] >> 16], su[k] * (min(wu[k], (int)(int16_t)(yv[k] & 0xffff)) - min(wu[k], 0)));
                 }
             }
         }
         __syncthreads();
-        for (int y = threadIdx.x; y < N1; y += 1024) {
-            const int d = row[y];
-            if (d) {
-                int32_t *px = X + xidx(u, y, T);
-                *px += d;
+    };
+    for (int u = blockIdx.x; u < N1; u += gridDim.x) {
+        if (!rowmark[u]) continue;  // no neighbour of u moved (uniform)
+        if (threadIdx.x == 0) nq = 0;
+        for (int y = threadIdx.x; y < N1; y += 1024) row[y] = 0;
+        __syncthreads();
+        const int e0 = cptr[u], e1 = cptr[u + 1];
+        for (int e = e0 + threadIdx.x; e < e1; e += 1024) {
+            const int2 rw = crw[e];
+            if (moved[neighbour(rw.x)]) {
+                const int slot = atomicAdd(&nq, 1);
+                if (slot < kXdeltaQual) qual[slot] = rw;
+            }
+        }
+        __syncthreads();
+        const int nall = nq;
+        __syncthreads();  // every thread has read nq before it is reset
+        if (nall == 0) continue;  // the moved neighbours kept their images (uniform)
+        if (nall <= kXdeltaQual) {
+            walk(nall);
+        } else {
+            for (int c0 = e0; c0 < e1; c0 += kXdeltaQual) {
+                if (threadIdx.x == 0) nq = 0;
+                __syncthreads();
+                if ((int)threadIdx.x < min(kXdeltaQual, e1 - c0)) {
+                    const int2 rw = crw[c0 + threadIdx.x];
+                    if (moved[neighbour(rw.x)]) qual[atomicAdd(&nq, 1)] = rw;
+                }
+                __syncthreads();
+                walk(nq);
+            }
+        }
+        // kXdeltaRmw reads of X in flight per thread before their writes; X(u, y) is
+        // xr[(y >> 6) * 4096 + (y & 63)] in the tile-major layout
+        int32_t *xr = X + (size_t)(u >> 6) * T * 4096 + ((u & 63) << 6);
+        for (int y0 = threadIdx.x; y0 < N1; y0 += 1024 * kXdeltaRmw) {
+            int d[kXdeltaRmw], v[kXdeltaRmw];
+#pragma unroll
+            for (int k = 0; k < kXdeltaRmw; k++) d[k] = y0 + 1024 * k < N1 ? row[y0 + 1024 * k] : 0;
+#pragma unroll
+            for (int k = 0; k < kXdeltaRmw; k++) {
+                const int y = y0 + 1024 * k;
+                v[k] = d[k] ? xr[((y >> 6) << 12) + (y & 63)] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kXdeltaRmw; k++) {
+                const int y = y0 + 1024 * k;
+                if (d[k]) xr[((y >> 6) << 12) + (y & 63)] = v[k] + d[k];
             }
         }
         __syncthreads();
@@ -780,6 +866,7 @@ struct bm_batch_state {
     int32_t *xscratch = nullptr;  // chunk rows of the column recompute, xcap x (n+1)
     int xcap = 0;
     uint8_t *inU = nullptr;
+    int4 *ulists = nullptr;    // per moved vertex: its G2 out- and in-lists at the new and the old image
     bool incremental = false;  // the captured step updates X incrementally
     bool rot3_x = false;       // triples scored from the step's X (k_rot3_x) instead of their lists
     bool xdelta_rows = true;   // row updates gathered per row (k_xdelta_rows) instead of per entry
@@ -821,6 +908,11 @@ static CandView cand_view(bm_ctx *c) {
     return cv;
 }
 
+// the incremental update's byte flags share one allocation: inU, the row
+// marks, the moved flags, (N1 + 3) / 4 words each
+static uint8_t *rowmark(bm_batch_state *b, int N1) { return b->inU + (size_t)(N1 + 3) / 4 * 4; }
+static uint8_t *moved_flags(bm_batch_state *b, int N1) { return b->inU + (size_t)(N1 + 3) / 4 * 8; }
+
 static void drop_graph(bm_batch_state *b) {
     std::lock_guard<std::mutex> g(sync_mutex());
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -840,6 +932,7 @@ int bm::free_batch(bm_ctx *c) {
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
     fr(b->ckey), fr(b->vmax), fr(b->blocked), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
     fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU), fr(b->uoldv);
+    fr(b->ulists);
     if (b->hst) (void)hipHostFree(b->hst);
     if (b->done) (void)hipEventDestroy(b->done);
     c->dev_bytes -= (int64_t)b->bytes;
@@ -880,9 +973,10 @@ static int enqueue_step(bm_ctx *c) {
     // X is updated in place between steps only with the sparse sweep and swap candidates
     const bool inc = p.pairs && sweep_is_sparse(c);
     b->incremental = inc;
+    const long long mark_words = (long long)(N1 + 3) / 4 * 3;  // inU, the row marks, the moved flags
     if (inc)
-        BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((N1 / 4 + 256) / 256), dim3(256), 0, c->stream,
-                                                (uint32_t *)b->inU, (long long)(N1 + 3) / 4, 0u));
+        BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((unsigned)((mark_words + 255) / 256)), dim3(256), 0,
+                                                c->stream, (uint32_t *)b->inU, mark_words, 0u));
     if (p.pairs) {
         if ((rc = sweep_enqueue(c, nullptr, inc ? &b->st->xstale : nullptr))) return rc;
         if (inc) BM_LAUNCH(c, "clear_stale", hipLaunchKernelGGL(k_clear_stale, dim3(1), dim3(64), 0, c->stream, b->st));
@@ -920,12 +1014,13 @@ static int enqueue_step(bm_ctx *c) {
                                                    (int)(c->ncomb / std::max(1, c->N1))));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, -1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
-                                                    c->Q, c->ld, N1, c->X, c->ldx));
+                                                    c->Q, c->ld, N1, c->X, c->ldx, rowmark(b, N1)));
     }
     const int max_win = N1 / 2 + 1;
     BM_LAUNCH(c, "apply", hipLaunchKernelGGL(k_apply, dim3((max_win + 255) / 256), dim3(256), 0, c->stream, b->win,
                                              max_win, cv, c->map, c->pinv, c->sparse_ok ? 1 : 0, b->st, b->log,
-                                             b->log_cap));
+                                             b->log_cap, inc ? moved_flags(b, N1) : nullptr,
+                                             inc ? b->ulists : nullptr, c->g2optr, c->g2iptr, (int)c->ng2));
     if ((rc = inc ? launch_rebuild_p(c, &b->st->pinc, b->ulist, &b->st->nu) : launch_rebuild_p(c))) return rc;
     if (c->sparse_ok) {
         if ((rc = launch_refresh_g2y(c))) return rc;
@@ -933,9 +1028,9 @@ static int enqueue_step(bm_ctx *c) {
     if (inc) {
         if (b->xdelta_rows)
             BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta_rows, dim3((unsigned)std::min(N1, 2048)), dim3(1024),
-                                                      (size_t)N1 * 4, c->stream, b->st, b->inU, b->uoldv, c->map,
-                                                      c->cptr, (const int2 *)c->crw, c->g2optr, c->g2oy, c->g2iptr,
-                                                      c->g2iy, N1, c->X, c->ldx));
+                                                      (size_t)N1 * 4, c->stream, b->st, rowmark(b, N1),
+                                                      moved_flags(b, N1), b->ulists, c->cptr, (const int2 *)c->crw,
+                                                      c->g2oy, N1, c->X, c->ldx));
         else
             BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta, dim3(ublocks), dim3(256), 0, c->stream, b->st,
                                                       b->ulist, b->uold, b->ubase, b->inU, c->map, c->cptr,
@@ -951,7 +1046,7 @@ static int enqueue_step(bm_ctx *c) {
                                                     b->cbase, c->cneg, b->xscratch, N1, c->X, c->ldx));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, 1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
-                                                    c->Q, c->ld, N1, c->X, c->ldx));
+                                                    c->Q, c->ld, N1, c->X, c->ldx, nullptr));
         BM_LAUNCH(c, "diag_all", hipLaunchKernelGGL(k_diag_all, dim3((N1 + 255) / 256), dim3(256), 0, c->stream,
                                                     b->st, c->X, c->ldx, N1, c->diag));
     }
@@ -1070,7 +1165,8 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->ubase, (size_t)(N1 + 1) * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->cbase, (size_t)(N1 + 1) * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->uoldv, (size_t)N1 * 4))) return fail(rc);
-    if ((rc = balloc(c, (void **)&b->inU, (size_t)(N1 + 3) / 4 * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->inU, (size_t)(N1 + 3) / 4 * 12))) return fail(rc);  // inU, row marks, moved
+    if ((rc = balloc(c, (void **)&b->ulists, (size_t)N1 * 2 * sizeof(int4)))) return fail(rc);
     // steps moving at most inc_max vertices update X in place (about 3 |U| / n of a sweep)
     b->inc_max = std::max(64, c->N1 / 8);
     if (const char *e = getenv("BM_REFINE_INC_MAX")) b->inc_max = atoi(e);
