@@ -125,6 +125,7 @@ static int eval_resident(bm_ctx *c, int mode) {
 extern "C" int bm_moves_eval(bm_ctx *c, int mode, int64_t *best_index, int32_t *best_delta, int32_t *best_aux) {
     if (!c || !best_index || !best_delta || !best_aux) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (!c->mv_k) return set_error(BM_ESTATE, "no resident move list (bm_moves_set)");
     if (c->mv_k == 3 && mode != BM_ROT3_ROTATIONS && mode != BM_ROT3_ALL)
         return set_error(BM_EINVAL, "bad mode %d", mode);

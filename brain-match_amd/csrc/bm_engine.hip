@@ -213,21 +213,43 @@ __global__ __launch_bounds__(512) void k_gather_rows(int16_t *__restrict__ Q, co
 
 // The columns of a vertex list in every row of P and P^T:
 // P[x][v] = W2[p(x)][p(v)], P^T[x][v] = W2^T[p(x)][p(v)]; one wave per
-// output row, lanes over the list.  Skipped when *flag != want.
+// output row, lanes over the list, which each workgroup first stages in LDS
+// with the images p(v) (lists of up to kPatchStage vertices; longer ones are
+// read from memory).  Skipped when *flag != want.
+constexpr int kPatchStage = 1024;
+
 __global__ __launch_bounds__(256) void k_patch_cols(int16_t *__restrict__ Q, const int16_t *__restrict__ W2,
                                                     const int32_t *__restrict__ map, int N1, int ld,
                                                     const int32_t *__restrict__ cols, const int *__restrict__ ncols,
                                                     const int *__restrict__ flag, int want) {
+    __shared__ int lv[kPatchStage], lg[kPatchStage];
     if (*flag != want) return;
     const int n = *ncols, lane = threadIdx.x & 63;
+    const bool staged = n <= kPatchStage;
+    if (staged)
+        for (int i = threadIdx.x; i < n; i += 256) {
+            const int v = cols[i];
+            lv[i] = v;
+            lg[i] = map[v];
+        }
+    __syncthreads();
     const int nwaves = gridDim.x * 4;
     for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < 2 * N1; r += nwaves) {
         const int z = r >= N1, x = r - z * N1;
         const int16_t *src = W2 + ((size_t)z * N1 + map[x]) * ld;
         int16_t *dst = Q + (size_t)r * ld;
-        for (int i = lane; i < n; i += 64) {
-            const int v = cols[i];
-            dst[v] = src[map[v]];
+        for (int i0 = lane; i0 < n; i0 += 256) {  // four cells in flight per lane
+            int v[4];
+            int16_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int i = i0 + 64 * k;
+                v[k] = i < n ? (staged ? lv[i] : cols[i]) : -1;
+                w[k] = v[k] >= 0 ? src[staged ? lg[i] : map[v[k]]] : (int16_t)0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (v[k] >= 0) dst[v[k]] = w[k];
         }
     }
 }
@@ -280,13 +302,37 @@ __global__ __launch_bounds__(256) void k_swap_change(long long *__restrict__ out
 // list is in source order, so the lines of vertex 0 are its first `lo`
 // entries, which calculate_alignment_score (graph-alignment-optimized.c:295)
 // never visits.
+// With pmap the cells are read as W2[p(x)][p(y)] (P = W2), else as P[x][y].
 __global__ __launch_bounds__(256) void k_score(unsigned long long *__restrict__ out, const int32_t *__restrict__ es,
                                                const int32_t *__restrict__ ed, const int32_t *__restrict__ ew,
-                                               int64_t lo, int64_t m, const int16_t *__restrict__ P, int ld) {
+                                               int64_t lo, int64_t m, const int16_t *__restrict__ P, int ld,
+                                               const int32_t *__restrict__ pmap) {
     __shared__ long long part[4];
     long long s = 0;
-    for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-        s += min(ew[i], (int)P[(size_t)es[i] * ld + ed[i]]);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // four lines per thread per pass: their list loads, then their four cell gathers, in flight together
+    for (int64_t i0 = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < m; i0 += 4 * stride) {
+        int x[4], y[4], w[4], q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t i = i0 + k * stride;
+            const bool in = i < m;
+            x[k] = in ? es[i] : 0;
+            y[k] = in ? ed[i] : 0;
+            w[k] = in ? ew[i] : 0;
+        }
+        if (pmap) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                x[k] = pmap[x[k]];
+                y[k] = pmap[y[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) q[k] = i0 + k * stride < m ? (int)P[(size_t)x[k] * ld + y[k]] : 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) s += i0 + k * stride < m ? min(w[k], q[k]) : 0;
+    }
     s = bm_wave::wave_sum64(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -467,8 +513,10 @@ int launch_rebuild_p(bm_ctx *c, const int *patch_flag, const int32_t *verts, con
                                    (int)kGatherLdsMax));
         c->gather_attr = true;
     }
-    // whole matrices (unless *patch_flag says patch)
-    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_gather_rows, dim3(c->N1, both ? 1 : 2), dim3(512), lds, c->stream,
+    // whole matrices (unless *patch_flag says patch); grid-stride over the rows, so that a
+    // launch that the flag skips has few workgroups to retire
+    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_gather_rows, dim3(patch_flag ? std::min(c->N1, 2048) : c->N1,
+                                                                   both ? 1 : 2), dim3(512), lds, c->stream,
                                                c->Q, c->W2, c->map, c->N1, c->ld, both, nullptr, nullptr,
                                                patch_flag, 0));
     if (patch_flag) {  // or the rows and columns of the moved vertices only
@@ -481,6 +529,25 @@ int launch_rebuild_p(bm_ctx *c, const int *patch_flag, const int32_t *verts, con
     return BM_OK;
 }
 
+int launch_rebuild_p_if(bm_ctx *c, const int *flag) {
+    const int both = (size_t)c->ld * 4 <= kGatherLdsMax && !getenv("BM_GATHER_SPLIT") ? 1 : 0;
+    const size_t lds = (size_t)c->ld * 2 * (both ? 2 : 1);
+    if (!c->gather_attr) return set_error(BM_ESTATE, "P rebuild before the first mapping");
+    BM_LAUNCH(c, "build_p", hipLaunchKernelGGL(k_gather_rows, dim3(std::min(c->N1, 2048), both ? 1 : 2), dim3(512),
+                                               lds, c->stream, c->Q, c->W2, c->map, c->N1, c->ld, both, nullptr,
+                                               nullptr, flag, 1));
+    return BM_OK;
+}
+
+int ensure_p(bm_ctx *c) {
+    if (!c->p_stale) return BM_OK;
+    BM_HIP(hipSetDevice(c->device));
+    int rc = launch_rebuild_p(c);
+    if (rc) return rc;
+    c->p_stale = false;
+    return BM_OK;
+}
+
 int launch_diag(bm_ctx *c) {
     BM_LAUNCH(c, "diag", hipLaunchKernelGGL(k_diag, dim3((c->N1 + 3) / 4), dim3(256), 0, c->stream, c->diag, c->cptr,
                                             c->cr, c->cw, c->Q, c->N1, c->ld));
@@ -488,12 +555,12 @@ int launch_diag(bm_ctx *c) {
 }
 
 // S_opt into *out (which must hold 0 beforehand); no host synchronisation
-int launch_score(bm_ctx *c, unsigned long long *out) {
+int launch_score(bm_ctx *c, unsigned long long *out, bool via_map) {
     int64_t m = c->e1;
     int blocks = (int)std::min<int64_t>((m - c->e1_v0 + 255) / 256, 4096);
     if (blocks < 1) blocks = 1;
     BM_LAUNCH(c, "score", hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, c->stream, out, c->esrc, c->edst,
-                                             c->ew, c->e1_v0, m, c->Q, c->ld));
+                                             c->ew, c->e1_v0, m, via_map ? c->W2 : c->Q, c->ld, via_map ? c->map : nullptr));
     return BM_OK;
 }
 
@@ -1050,6 +1117,7 @@ int bm_set_mapping(bm_ctx *c, const int32_t *mapping) {
     c->g2y_dirty = true;
     int rc = launch_rebuild_p(c);
     if (rc) return rc;
+    c->p_stale = false;
     BM_HIP(hipStreamSynchronize(c->stream));
     c->mapped = true;
     c->diag_valid = c->dfdiag_valid = false;
@@ -1066,6 +1134,7 @@ int bm_get_mapping(bm_ctx *c, int32_t *mapping) {
 int bm_score(bm_ctx *c, int64_t *score) {
     if (!c || !score) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     BM_HIP(hipSetDevice(c->device));
     unsigned long long *out = c->keys + c->N1 + 2;
     BM_HIP(hipMemsetAsync(out, 0, 8, c->stream));
@@ -1073,7 +1142,7 @@ int bm_score(bm_ctx *c, int64_t *score) {
     int blocks = (int)std::min<int64_t>((m - c->e1_v0 + 255) / 256, 4096);
     if (blocks < 1) blocks = 1;
     BM_LAUNCH(c, "score", hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, c->stream, out, c->esrc, c->edst,
-                                             c->ew, c->e1_v0, m, c->Q, c->ld));
+                                             c->ew, c->e1_v0, m, c->Q, c->ld, nullptr));
     BM_HIP(hipMemcpyAsync(c->hres, out, 8, hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipStreamSynchronize(c->stream));
     *score = (int64_t)c->hres[0];
@@ -1085,6 +1154,7 @@ static int check_vertex(const bm_ctx *c, int32_t v) { return v >= 0 && v <= c->n
 int bm_apply_swap_deferred(bm_ctx *c, int32_t a, int32_t b) {
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (!check_vertex(c, a) || !check_vertex(c, b) || a == b)
         return set_error(BM_EINVAL, "apply_swap(%d,%d): bad vertices", a, b);
     BM_HIP(hipSetDevice(c->device));
@@ -1116,6 +1186,7 @@ int bm_apply_swap_scored(bm_ctx *c, int32_t a, int32_t b, int64_t *change) {
 int bm_apply_swap(bm_ctx *c, int32_t a, int32_t b) {
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (!check_vertex(c, a) || !check_vertex(c, b) || a == b)
         return set_error(BM_EINVAL, "apply_swap(%d,%d): bad vertices", a, b);
     BM_HIP(hipSetDevice(c->device));
@@ -1128,6 +1199,7 @@ int bm_apply_rot3(bm_ctx *c, int32_t m1, int32_t m2, int32_t m3, int32_t p) {
     static const int tab[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (!check_vertex(c, m1) || !check_vertex(c, m2) || !check_vertex(c, m3) || m1 == m2 || m1 == m3 || m2 == m3 ||
         p < 0 || p > 5)
         return set_error(BM_EINVAL, "apply_rot3: bad arguments");
@@ -1140,6 +1212,7 @@ int bm_apply_rot3(bm_ctx *c, int32_t m1, int32_t m2, int32_t m3, int32_t p) {
 int bm_apply_perm(bm_ctx *c, int32_t k, const int32_t *set, const int32_t *perm) {
     if (!c || !set || !perm) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (k < 1 || k > 8) return set_error(BM_EINVAL, "k=%d outside [1,8]", k);
     int seen = 0;
     for (int i = 0; i < k; i++) {
@@ -1297,6 +1370,7 @@ extern "C" {
 int bm_perturb(bm_ctx *c, int32_t k, int mode, uint64_t *rng, int32_t *selected) {
     if (!c || !rng) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (mode != BM_PERTURB_OMP && mode != BM_PERTURB_WEIGHTED) return set_error(BM_EINVAL, "bad mode %d", mode);
     const int n = c->n;
     if (k < 1 || 2 * (int64_t)k > n) return set_error(BM_EINVAL, "k=%d: need 1 <= k and 2k <= n=%d", k, n);

@@ -153,6 +153,9 @@ struct bm_ctx {
 
     // counts host-side mapping changes (set_mapping, apply_*, the end of a batched run)
     uint64_t map_epoch = 0;
+    // P / P^T lag the mapping: a batched run whose step reads W2 through the
+    // mapping does not keep them current; ensure_p rebuilds them on next use
+    bool p_stale = false;
 
     // host mirror of the mapping
     std::vector<int32_t> hmap;
@@ -212,7 +215,8 @@ int launch_check(const char *what);
 int launch_eval_pairs(bm_ctx *c, const int32_t *dA, const int32_t *dB, int64_t m, int32_t *dD);
 int launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP);
 // mult = 2: Delta_P as the reference (intra-set lines twice); 1: the exact score change
-int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult = 2);
+int launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult = 2,
+                   bool from_x = false, bool via_map = false);
 int free_moves(bm_ctx *c);
 // autotuner of the move kernels: times the variants on the given device list
 int tune_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int32_t *dD, int32_t *dP);
@@ -225,15 +229,21 @@ int launch_rot3_moves(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t
 // the dense-form lists and G1 cell matrix; Delta_G3 from the current X (needs
 // c->dense_equal, the cell matrix and X matching the mapping)
 int ensure_df_cells(bm_ctx *c);
-int launch_rot3_x(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP);
-int tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult);
+// via_map: P read as W2[p(x)][p(y)] (P / P^T may lag the mapping)
+int launch_rot3_x(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP, bool via_map = false);
+int tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult,
+                 bool from_x = false);
 int free_batch(bm_ctx *c);
 // P and P^T from W2 and the device mapping; with patch_flag, when *patch_flag
 // is 1 only the rows and columns of verts[0 .. *nverts) are rebuilt
 int launch_rebuild_p(bm_ctx *c, const int *patch_flag = nullptr, const int32_t *verts = nullptr,
                      const int *nverts = nullptr);
+// P and P^T whole when *flag is 1 (capturable)
+int launch_rebuild_p_if(bm_ctx *c, const int *flag);
 int launch_diag(bm_ctx *c);             // X(v,v) for every v
-int launch_score(bm_ctx *c, unsigned long long *out);
+// S_opt into *out; via_map reads the cells as W2[p(x)][p(y)] instead of P[x][y]
+int launch_score(bm_ctx *c, unsigned long long *out, bool via_map = false);
+int ensure_p(bm_ctx *c);                // rebuilds P / P^T when they lag the mapping
 int launch_refresh_g2y(bm_ctx *c);      // G2 lists through p^-1
 int sweep_prepare(bm_ctx *c);           // allocations and kernel choice of the all-pairs sweep
 // row-best keys into keys[0..n], global best into keys[n+1]; with run_flag the

@@ -223,22 +223,26 @@ __global__ __launch_bounds__(256) void k_table_pack(int2 *__restrict__ XP, const
 // except that X[min(a,b)][max(a,b)] also holds Corr(a,b) of the pair sweep
 // (pair_corr over the lines between a and b, each its own dense cell); that
 // is taken out again for the triple's pairs.
+// With pmap, Q is W2 and P[x][y] is read as W2[p(x)][p(y)].
 __global__ __launch_bounds__(256) void k_rot3_x(const int32_t *__restrict__ T, int64_t count, int mode,
                                                 int32_t *__restrict__ best_d, int32_t *__restrict__ best_p,
                                                 const int32_t *__restrict__ X, int tiles,
                                                 const int16_t *__restrict__ Q, const int16_t *__restrict__ W1,
-                                                int ld) {
+                                                int ld, const int32_t *__restrict__ pmap) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     int m[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) m[i] = T[3 * t + i];
+    int g[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) g[i] = pmap ? pmap[m[i]] : m[i];
     int p[3][3], x[3][3], c[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            p[i][j] = qv(Q, m[i], ld, m[j]);
+            p[i][j] = qv(Q, g[i], ld, g[j]);
             x[i][j] = X[xidx(m[i], m[j], tiles)];
             c[i][j] = i == j ? 0 : (int)W1[(size_t)m[i] * ld + m[j]];
         }
@@ -279,17 +283,26 @@ constexpr int kTuneMinSets = 256;
 
 // kFixedK > 0 compiles the kernel for that k (the loops over set positions
 // unroll and index the per-thread permutation statically); 0 takes k at run time.
+// With X (the batched step, when every G1 line is its own dense cell, W1 the
+// cell matrix) the table is read from the all-pairs gain matrix instead of
+// the lists: C[i][j] = X(S_i, S_j) - X(S_i, S_i) without the list entries
+// of S_i's lines to the set (they are the intra lines), and without the
+// Corr(S_i, S_j) that the pair sweep folds into X[min][max].
 template <int kThreads, int kFixedK>
 __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict__ sets, int64_t count, int k_arg,
                                                       int32_t *__restrict__ best_d, int32_t *__restrict__ best_rank,
                                                       const int32_t *__restrict__ cptr, const int32_t *__restrict__ cr,
                                                       const int32_t *__restrict__ cw, const int16_t *__restrict__ Q,
-                                                      int N1, int ld, int *__restrict__ overflow, int mult) {
+                                                      int N1, int ld, int *__restrict__ overflow, int mult,
+                                                      const int32_t *__restrict__ X, int tiles,
+                                                      const int16_t *__restrict__ W1, const int16_t *__restrict__ W2,
+                                                      const int32_t *__restrict__ pmap) {
     constexpr int kWaves = kThreads / 64;
     const int k = kFixedK ? kFixedK : k_arg;
     __shared__ int S[8];
     __shared__ int C[8][8];
     __shared__ int PS[8][8];
+    __shared__ int CW[8][8];  // G1 cells between set members (the table from X)
     __shared__ int ni;
     __shared__ int iw[kMaxIntra];
     __shared__ unsigned char ix[kMaxIntra], il[kMaxIntra];
@@ -304,10 +317,42 @@ __global__ __launch_bounds__(kThreads) void k_permset(const int32_t *__restrict_
         lbase = 0;
     }
     __syncthreads();
-    if (tid < k * k) PS[tid / k][tid % k] = qv(Q, S[tid / k], ld, S[tid % k]);
+    if (tid < k * k) {
+        // P among the set; with pmap (the table from X only) read as W2[p(x)][p(y)]
+        PS[tid / k][tid % k] = pmap ? qv(W2, pmap[S[tid / k]], ld, pmap[S[tid % k]]) : qv(Q, S[tid / k], ld, S[tid % k]);
+        if (X) CW[tid / k][tid % k] = tid / k == tid % k ? 0 : (int)W1[(size_t)S[tid / k] * ld + S[tid % k]];
+    }
+    if (X) {
+        __syncthreads();
+        if (tid < k * k) {
+            const int i = tid / k, j = tid % k;
+            if (i != j && CW[i][j] > 0) {  // the intra line S_i -> S_j (k * k <= kMaxIntra slots)
+                const int slot = atomicAdd(&ni, 1);
+                iw[slot] = CW[i][j];
+                ix[slot] = (unsigned char)i;
+                il[slot] = (unsigned char)j;
+            }
+            if (i != j) {
+                const int a = S[i], b = S[j];
+                int x = X[xidx(a, b, tiles)] - X[xidx(a, a, tiles)];
+                if (a < b) {  // Corr(a, b), folded into X[a][b]: the lines a -> b (out) and b -> a (in)
+                    if (CW[i][j] > 0) x -= pair_corr(true, CW[i][j], CW[i][j], PS[i][i], PS[j][j], PS[i][j], PS[j][i]);
+                    if (CW[j][i] > 0) x -= pair_corr(false, CW[j][i], CW[j][i], PS[i][i], PS[j][j], PS[i][j], PS[j][i]);
+                }
+                // the entries of S_i's lines to the set: out S_i -> S_l reads P[v][S_l], in S_l -> S_i reads P[S_l][v]
+                for (int l = 0; l < k; l++) {
+                    if (l == i) continue;
+                    const int wo = CW[i][l], wi = CW[l][i];
+                    if (wo > 0) x -= min(wo, PS[j][l]) - min(wo, PS[i][l]);
+                    if (wi > 0) x -= min(wi, PS[l][j]) - min(wi, PS[l][i]);
+                }
+                C[i][j] = x;
+            }
+        }
+    }
     // table C over all entries of the k lists: per-thread partial sums of row i
     // in registers, one wave reduction and one LDS atomic per (wave, j)
-    for (int i = 0; i < k; i++) {
+    for (int i = 0; i < (X ? 0 : k); i++) {
         const int v = S[i];
         int part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int e = cptr[v] + tid; e < cptr[v + 1]; e += kThreads) {
@@ -522,10 +567,11 @@ static int rot3_table_prepare(bm_ctx *c) {
     return BM_OK;
 }
 
-int bm::launch_rot3_x(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP) {
+int bm::launch_rot3_x(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP, bool via_map) {
     if (m <= 0) return BM_OK;
     BM_LAUNCH(c, "rot3_x", hipLaunchKernelGGL(k_rot3_x, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
-                                              dT, m, mode, dD, dP, c->X, (int)(c->ldx >> 6), c->Q, c->dfW1, c->ld));
+                                              dT, m, mode, dD, dP, c->X, (int)(c->ldx >> 6), via_map ? c->W2 : c->Q,
+                                              c->dfW1, c->ld, via_map ? c->map : nullptr));
     return BM_OK;
 }
 
@@ -536,11 +582,15 @@ static int launch_rot3_table(bm_ctx *c, const int32_t *dT, int64_t m, int mode, 
     return BM_OK;
 }
 
-int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult) {
+int bm::launch_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult,
+                       bool from_x, bool via_map) {
     if (m <= 0) return BM_OK;
+    const int32_t *xm = from_x ? c->X : nullptr;
+    const int32_t *pm = from_x && via_map ? c->map : nullptr;
 #define BM_PERMSET(T, K)                                                                                        \
     BM_LAUNCH(c, "permset", hipLaunchKernelGGL((k_permset<T, K>), dim3((unsigned)m), dim3(T), 0, c->stream, dS, m, \
-                                               k, dD, dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv, mult))
+                                               k, dD, dR, c->cptr, c->cr, c->cw, c->Q, c->N1, c->ld, dOv, mult, xm, \
+                                               (int)(c->ldx >> 6), c->dfW1, c->W2, pm))
     if (c->permset_threads != 64 && c->permset_threads != 256)
         return set_error(BM_EINVAL, "permset: %d threads per set", c->permset_threads);
     if (k == 8) {
@@ -685,11 +735,12 @@ int bm::launch_rot3_moves(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int
     return rc ? rc : launch_rot3_table(c, dT, m, mode, dD, dP);
 }
 
-int bm::tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult) {
+int bm::tune_permset(bm_ctx *c, int k, const int32_t *dS, int64_t m, int32_t *dD, int32_t *dR, int *dOv, int mult,
+                     bool from_x) {
     static const int ch[2] = {64, 256};
     const auto key = std::make_tuple(2, log2c(c->n), log2c(std::max<long long>(1, c->ncomb / c->N1)), k);
     int rc = tune_variant(c, "BM_PERMSET_THREADS", key, ch, 2, [&](int v) { c->permset_threads = v; },
-                          [&] { return launch_permset(c, k, dS, m, dD, dR, dOv, mult); });
+                          [&] { return launch_permset(c, k, dS, m, dD, dR, dOv, mult, from_x); });
     if (!rc) c->permset_threads_k = k;
     return rc;
 }
@@ -700,6 +751,7 @@ extern "C" int bm_eval_rot3(bm_ctx *c, const int32_t *triples, int64_t count, in
                             int32_t *best_perm) {
     if (!c || (count > 0 && (!triples || !best_delta || !best_perm))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (mode != BM_ROT3_ROTATIONS && mode != BM_ROT3_ALL) return set_error(BM_EINVAL, "bad mode %d", mode);
     if (count < 0) return set_error(BM_EINVAL, "count < 0");
     for (int64_t i = 0; i < count; i++) {
@@ -733,6 +785,7 @@ extern "C" int bm_eval_permset(bm_ctx *c, int32_t k, const int32_t *sets, int64_
                                int32_t *best_rank) {
     if (!c || (count > 0 && (!sets || !best_delta || !best_rank))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (k < 2 || k > 8) return set_error(BM_EINVAL, "k=%d outside [2,8]", k);
     if (count < 0) return set_error(BM_EINVAL, "count < 0");
     for (int64_t i = 0; i < count; i++) {
@@ -791,6 +844,7 @@ extern "C" int bm_eval_swaps_hillclimb(bm_ctx *c, const int32_t *a, const int32_
                                        int32_t *delta) {
     if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (count < 0) return set_error(BM_EINVAL, "count < 0");
     for (int64_t i = 0; i < count; i++)
         if (a[i] < 0 || a[i] > c->n || b[i] < 0 || b[i] > c->n || a[i] == b[i])

@@ -73,6 +73,9 @@ struct RefineState {
     int full;     // this step began with a whole sweep
     int xchunks;  // work items of the column recompute this step (prefix cbase)
     int xlines;   // G2 lines per work item of the column recompute
+    // steps that read W2 through the mapping leave P / P^T behind it
+    int pstale;    // P / P^T do not match the mapping
+    int prebuild;  // this step rebuilds P / P^T before its whole sweep (X and P both stale)
 };
 
 struct MoveRec {
@@ -376,7 +379,8 @@ __global__ void k_step_end(RefineState *st, bm_refine_step *hist) {
 // keep_x: the mapping is the one the previous run ended with, so X (current
 // unless that run's last step marked it stale) carries over.  Volatile reads
 // as in k_step_end.
-__global__ void k_state_init(RefineState *st, int keep_x) {
+// pstale: P / P^T lag the mapping at the start of the run.
+__global__ void k_state_init(RefineState *st, int keep_x, int pstale) {
     if (threadIdx.x != 0) return;
     const volatile RefineState *vs = st;
     const long long s = (long long)vs->score_slot;
@@ -385,7 +389,19 @@ __global__ void k_state_init(RefineState *st, int keep_x) {
     z.initial_score = s;
     z.gen_step = vs->gen_step;
     z.xstale = keep_x ? vs->xstale : 1;
+    z.pstale = pstale;
     *st = z;
+}
+
+// A step that reads W2 through the mapping needs P / P^T only for a whole
+// sweep (the sparse gain kernel's pair corrections, P's diagonal, long lists
+// streamed as rows): rebuilt first when both X and P are stale.
+__global__ void k_p_gate(RefineState *st) {
+    if (threadIdx.x != 0) return;
+    const volatile RefineState *vs = st;
+    const int rebuild = vs->xstale && vs->pstale ? 1 : 0;
+    st->prebuild = rebuild;
+    if (rebuild) st->pstale = 0;
 }
 
 // ---------------------------------------------------------------- incremental X
@@ -455,7 +471,7 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
                                                   int32_t *__restrict__ uold, int32_t *__restrict__ ubase,
                                                   uint8_t *__restrict__ inU, int32_t *__restrict__ uoldv,
                                                   RefineState *__restrict__ st, int inc_max, int pinc_max,
-                                                  long long inc_budget, int dbar) {
+                                                  long long inc_budget, int dbar, int pfree) {
     __shared__ int cnt, carry;
     __shared__ int wsum[16];
     __shared__ unsigned long long work;
@@ -500,6 +516,7 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
         st->npairs = carry;
         const bool inc = nu <= inc_max && work <= (unsigned long long)inc_budget;
         st->inc = inc ? 1 : 0;
+        if (pfree && nu > 0) st->pstale = 1;  // the step moves vertices and leaves P / P^T as they are
         st->pinc = nu <= pinc_max ? 1 : 0;
         if (!inc) st->xstale = 1;
     }
@@ -514,7 +531,8 @@ __global__ void k_corr_inc(int sign, const RefineState *__restrict__ st, const i
                            const int32_t *__restrict__ ubase, const uint8_t *__restrict__ inU,
                            const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
                            const int32_t *__restrict__ cdw, const int16_t *__restrict__ Q, int ld, int N1,
-                           int32_t *__restrict__ X, int64_t ldx, uint8_t *__restrict__ rowmark) {
+                           int32_t *__restrict__ X, int64_t ldx, uint8_t *__restrict__ rowmark,
+                           const int32_t *__restrict__ pmap) {
     if (!st->inc) return;
     const int nu = st->nu, total = st->npairs, T = (int)(ldx >> 6);
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
@@ -528,8 +546,10 @@ __global__ void k_corr_inc(int sign, const RefineState *__restrict__ st, const i
         if (u == x || (inU[u] && u < x)) continue;
         const int a = min(x, u), b = max(x, u);
         const bool out_a = x == a ? out : !out;
-        const int corr = pair_corr(out_a, rw.y, cdw[e], (int)Q[(size_t)a * ld + a], (int)Q[(size_t)b * ld + b],
-                                   (int)Q[(size_t)a * ld + b], (int)Q[(size_t)b * ld + a]);
+        // P[x][y], or W2[p(x)][p(y)] with pmap (Q is W2 then)
+        const int ga = pmap ? pmap[a] : a, gb = pmap ? pmap[b] : b;
+        const int corr = pair_corr(out_a, rw.y, cdw[e], (int)Q[(size_t)ga * ld + ga], (int)Q[(size_t)gb * ld + gb],
+                                   (int)Q[(size_t)ga * ld + gb], (int)Q[(size_t)gb * ld + ga]);
         if (corr) atomicAdd(X + xidx(a, b, T), sign * corr);
     }
 }
@@ -868,7 +888,8 @@ struct bm_batch_state {
     uint8_t *inU = nullptr;
     int4 *ulists = nullptr;    // per moved vertex: its G2 out- and in-lists at the new and the old image
     bool incremental = false;  // the captured step updates X incrementally
-    bool rot3_x = false;       // triples scored from the step's X (k_rot3_x) instead of their lists
+    bool rot3_x = false;       // triples and sets scored from the step's X (k_rot3_x, k_permset) instead of their lists
+    bool pfree = false;        // the captured step reads W2 through the mapping and leaves P / P^T behind it
     bool xdelta_rows = true;   // row updates gathered per row (k_xdelta_rows) instead of per entry
     int32_t *uoldv = nullptr;  // image before the step, indexed by moved vertex
     uint64_t x_epoch = ~0ull;  // map_epoch at the end of the last run (X then matched the mapping)
@@ -977,6 +998,17 @@ static int enqueue_step(bm_ctx *c) {
     if (inc)
         BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((unsigned)((mark_words + 255) / 256)), dim3(256), 0,
                                                 c->stream, (uint32_t *)b->inU, mark_words, 0u));
+    // every kernel of the step can read W2 through the mapping instead of P / P^T when X is kept
+    // current and the triples and sets are scored from it; P / P^T are then rebuilt only for a
+    // whole sweep, and otherwise on the next use outside the run (ensure_p)
+    const bool pfree = inc && (p.triples == 0 || b->rot3_x) && (p.sets == 0 || b->rot3_x);
+    b->pfree = pfree;
+    const int16_t *Pq = pfree ? c->W2 : c->Q;
+    const int32_t *pmap = pfree ? c->map : nullptr;
+    if (pfree) {
+        BM_LAUNCH(c, "p_gate", hipLaunchKernelGGL(k_p_gate, dim3(1), dim3(64), 0, c->stream, b->st));
+        if ((rc = launch_rebuild_p_if(c, &b->st->prebuild))) return rc;
+    }
     if (p.pairs) {
         if ((rc = sweep_enqueue(c, nullptr, inc ? &b->st->xstale : nullptr))) return rc;
         if (inc) BM_LAUNCH(c, "clear_stale", hipLaunchKernelGGL(k_clear_stale, dim3(1), dim3(64), 0, c->stream, b->st));
@@ -985,7 +1017,7 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "gen_triples",
                   hipLaunchKernelGGL(k_gen_triples, dim3((p.triples + 255) / 256), dim3(256), 0, c->stream, b->tri,
                                      p.triples, b->st, (unsigned long long)p.seed, c->cptr, c->cr, N1, c->n));
-        rc = b->rot3_x ? launch_rot3_x(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d, b->tri_p)
+        rc = b->rot3_x ? launch_rot3_x(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d, b->tri_p, pfree)
                        : launch_rot3(c, b->tri, p.triples, BM_ROT3_ALL, b->tri_d, b->tri_p);
         if (rc) return rc;
     }
@@ -993,7 +1025,8 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "gen_sets",
                   hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets, p.sets,
                                      p.set_k, p.set_gen, b->st, (unsigned long long)p.seed, c->cptr, c->cr, N1, c->n));
-        if ((rc = launch_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r, b->ovf, 1))) return rc;
+        if ((rc = launch_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r, b->ovf, 1, b->rot3_x, pfree)))
+            return rc;
     }
     const unsigned mb = (unsigned)((b->M + 255) / 256);
     BM_LAUNCH(c, "cand_keys",
@@ -1011,17 +1044,18 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "collect", hipLaunchKernelGGL(k_collect, dim3(1), dim3(1024), 0, c->stream, b->win, cv, c->map,
                                                    c->cptr, c->g2optr, c->g2iptr, b->ulist, b->uold, b->ubase, b->inU,
                                                    b->uoldv, b->st, b->inc_max, b->pinc_max, b->inc_budget,
-                                                   (int)(c->ncomb / std::max(1, c->N1))));
+                                                   (int)(c->ncomb / std::max(1, c->N1)), pfree ? 1 : 0));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, -1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
-                                                    c->Q, c->ld, N1, c->X, c->ldx, rowmark(b, N1)));
+                                                    Pq, c->ld, N1, c->X, c->ldx, rowmark(b, N1), pmap));
     }
     const int max_win = N1 / 2 + 1;
     BM_LAUNCH(c, "apply", hipLaunchKernelGGL(k_apply, dim3((max_win + 255) / 256), dim3(256), 0, c->stream, b->win,
                                              max_win, cv, c->map, c->pinv, c->sparse_ok ? 1 : 0, b->st, b->log,
                                              b->log_cap, inc ? moved_flags(b, N1) : nullptr,
                                              inc ? b->ulists : nullptr, c->g2optr, c->g2iptr, (int)c->ng2));
-    if ((rc = inc ? launch_rebuild_p(c, &b->st->pinc, b->ulist, &b->st->nu) : launch_rebuild_p(c))) return rc;
+    if (!pfree && (rc = inc ? launch_rebuild_p(c, &b->st->pinc, b->ulist, &b->st->nu) : launch_rebuild_p(c)))
+        return rc;
     if (c->sparse_ok) {
         if ((rc = launch_refresh_g2y(c))) return rc;
     }
@@ -1046,11 +1080,11 @@ static int enqueue_step(bm_ctx *c) {
                                                     b->cbase, c->cneg, b->xscratch, N1, c->X, c->ldx));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, 1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
-                                                    c->Q, c->ld, N1, c->X, c->ldx, nullptr));
+                                                    Pq, c->ld, N1, c->X, c->ldx, nullptr, pmap));
         BM_LAUNCH(c, "diag_all", hipLaunchKernelGGL(k_diag_all, dim3((N1 + 255) / 256), dim3(256), 0, c->stream,
                                                     b->st, c->X, c->ldx, N1, c->diag));
     }
-    if ((rc = launch_score(c, &b->st->score_slot))) return rc;
+    if ((rc = launch_score(c, &b->st->score_slot, pfree))) return rc;
     BM_LAUNCH(c, "refine_end", hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->stream, b->st, b->hist));
     return BM_OK;
 }
@@ -1115,7 +1149,7 @@ static int autotune(bm_ctx *c) {
                   hipLaunchKernelGGL(k_gen_sets, dim3((p.sets + 255) / 256), dim3(256), 0, c->stream, b->sets, p.sets,
                                      p.set_k, p.set_gen, b->st, (unsigned long long)p.seed, c->cptr, c->cr, c->N1,
                                      c->n));
-        if ((rc = tune_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r, b->ovf, 1))) return rc;
+        if ((rc = tune_permset(c, p.set_k, b->sets, p.sets, b->set_d, b->set_r, b->ovf, 1, b->rot3_x))) return rc;
     }
     BM_HIP(hipStreamSynchronize(c->stream));
     return BM_OK;
@@ -1126,6 +1160,7 @@ namespace bm {
 int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if (!c || !prm) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     bm_refine_params p = *prm;
     if (p.triples < 0 || p.sets < 0 || p.triples > (1 << 24) || p.sets > (1 << 20))
         return set_error(BM_EINVAL, "triples %d / sets %d outside [0, 2^24] / [0, 2^20]", p.triples, p.sets);
@@ -1206,7 +1241,7 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     // With pair moves the step keeps X current before the triples are scored; when every G1 line
     // is its own dense cell, the triples read their deltas from it (BM_REFINE_ROT3=direct: lists)
     b->rot3_x = false;
-    if (p.triples > 0 && p.pairs && c->dense_equal) {
+    if ((p.triples > 0 || p.sets > 0) && p.pairs && c->dense_equal) {
         const char *e = getenv("BM_REFINE_ROT3");
         if (!e || strcmp(e, "direct") != 0) {
             if ((rc = ensure_df_cells(c))) return fail(rc);
@@ -1260,11 +1295,16 @@ int refine_start(bm_ctx *c) {
         int rc = launch_refresh_g2y(c);
         if (rc) return rc;
     }
+    if (!b->pfree) {  // this step keeps P / P^T current from where they are
+        int rc = ensure_p(c);
+        if (rc) return rc;
+    }
     BM_HIP(hipMemsetAsync(&b->st->score_slot, 0, 8, c->stream));
-    int rc = launch_score(c, &b->st->score_slot);
+    int rc = launch_score(c, &b->st->score_slot, b->pfree);
     if (rc) return rc;
     const int keep_x = b->x_epoch == c->map_epoch && b->incremental ? 1 : 0;
-    BM_LAUNCH(c, "refine_init", hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, c->stream, b->st, keep_x));
+    BM_LAUNCH(c, "refine_init", hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, c->stream, b->st, keep_x,
+                                                   c->p_stale ? 1 : 0));
     b->run_steps = 0;
     b->in_flight = false;
     return BM_OK;
@@ -1326,6 +1366,7 @@ int refine_finish(bm_ctx *c) {
     }
     c->diag_valid = c->dfdiag_valid = false;
     c->g2y_dirty = !c->sparse_ok;
+    if (c->batch->pfree && c->batch->hst->moves > 0) c->p_stale = true;  // rebuilt on next use
     c->map_epoch++;  // the device moved the mapping; X matches it unless the last step left it stale
     c->batch->x_epoch = c->map_epoch;
     return BM_OK;

@@ -632,11 +632,14 @@ __global__ __launch_bounds__(256) void k_pack_quads(uint32_t *__restrict__ g2q, 
                                                     const int16_t *__restrict__ Q, int ld, int32_t *__restrict__ pdg,
                                                     int N1, const int *__restrict__ run_flag) {
     if (run_flag && *run_flag == 0) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N1) pdg[i] = Q[i * ld + i];  // P[i][i], for the pair corrections
-    if (i >= m) return;
-    const uint32_t w = g2y[i];
-    g2q[pos[i]] = (w >> 16) == 0xFFFFu ? (uint32_t)N1 << 16 : w;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < max(m, (int64_t)N1);
+         i += (int64_t)gridDim.x * blockDim.x) {  // grid-stride: few blocks to retire when X is current
+        if (i < N1) pdg[i] = Q[i * ld + i];  // P[i][i], for the pair corrections
+        if (i < m) {
+            const uint32_t w = g2y[i];
+            g2q[pos[i]] = (w >> 16) == 0xFFFFu ? (uint32_t)N1 << 16 : w;
+        }
+    }
 }
 
 // The sparse row over the quad copy of the lists.  The flat index space of a
@@ -990,7 +993,8 @@ static int launch_gain_sparse(bm_ctx *c, const int *run_flag = nullptr) {
     {
         const int64_t m = 2 * c->ng2, threads = std::max<int64_t>(m, c->N1);
         BM_LAUNCH(c, "pack_quads",
-                  hipLaunchKernelGGL(k_pack_quads, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+                  hipLaunchKernelGGL(k_pack_quads, dim3((unsigned)std::min<int64_t>((threads + 255) / 256, 2048)),
+                                     dim3(256), 0, c->stream,
                                      c->g2q, c->g2oy, c->g2qpos, m, c->Q, c->ld, c->pdg, c->N1, run_flag));
     }
     // the long-list path costs registers and scheduling even when idle: only
@@ -1151,20 +1155,31 @@ __global__ __launch_bounds__(256) void k_combine(const int32_t *__restrict__ X, 
     const int N1 = n + 1;
     if (tid < 64) dA[lane] = (a <= n) ? diag[a] : 0;
     unsigned long long kb = 0;
+    // the next tile pair's loads are issued before the current pair is scored
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    i4v va[4], vb[4];
+    auto load = [&](int J) {
+        const i4v *ta = reinterpret_cast<const i4v *>(X + ((size_t)I * T + J) * 4096);
+        const i4v *tb = reinterpret_cast<const i4v *>(X + ((size_t)J * T + I) * 4096);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            va[k] = __builtin_nontemporal_load(ta + tid + 256 * k);
+            vb[k] = __builtin_nontemporal_load(tb + tid + 256 * k);
+        }
+    };
+    load(J0);
     for (int J = J0; J < J1; J++) {
         const int b0 = J * 64;
-        const int4 *ta = reinterpret_cast<const int4 *>(X + ((size_t)I * T + J) * 4096);
-        const int4 *tb = reinterpret_cast<const int4 *>(X + ((size_t)J * T + I) * 4096);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int q = tid + 256 * k;  // int4 index in the 1024-int4 tile
-            const int4 va = ta[q], vb = tb[q];
             const int r = q >> 4, c = (q & 15) * 4;
-            XA[r][c] = va.x; XA[r][c + 1] = va.y; XA[r][c + 2] = va.z; XA[r][c + 3] = va.w;
-            XB[r][c] = vb.x; XB[r][c + 1] = vb.y; XB[r][c + 2] = vb.z; XB[r][c + 3] = vb.w;
+            XA[r][c] = va[k].x; XA[r][c + 1] = va[k].y; XA[r][c + 2] = va[k].z; XA[r][c + 3] = va[k].w;
+            XB[r][c] = vb[k].x; XB[r][c + 1] = vb[k].y; XB[r][c + 2] = vb[k].z; XB[r][c + 3] = vb[k].w;
         }
         if (tid < 64) dB[lane] = (b0 + lane <= n) ? diag[b0 + lane] : 0;
         __syncthreads();
+        if (J + 1 < J1) load(J + 1);
         if (a >= 1 && a <= n) {
             const int da = dA[lane];
 #pragma unroll 4
@@ -1219,6 +1234,7 @@ static int check_pairs(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t co
 extern "C" int bm_eval_swaps(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta) {
     if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (count < 0) return set_error(BM_EINVAL, "count < 0");
     if (count == 0) return BM_OK;
     int rc = check_pairs(c, a, b, count);
@@ -1259,6 +1275,7 @@ extern "C" int bm_eval_swaps_noalloc(bm_ctx *c, const int32_t *a, const int32_t 
                                      int32_t *delta) {
     if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (count < 0) return set_error(BM_EINVAL, "count < 0");
     if (count > kNoAllocPairs)
         return set_error(BM_EINVAL, "bm_eval_swaps_noalloc: %lld pairs exceed the scratch of %lld pairs per call",
@@ -1291,6 +1308,7 @@ extern "C" int bm_eval_swaps_noalloc(bm_ctx *c, const int32_t *a, const int32_t 
 extern "C" int bm_eval_swaps_dense(bm_ctx *c, const int32_t *a, const int32_t *b, int64_t count, int32_t *delta) {
     if (!c || (count > 0 && (!a || !b || !delta))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (count < 0) return set_error(BM_EINVAL, "count < 0");
     for (int64_t i = 0; i < count; i++)
         if (a[i] < 1 || a[i] > c->n || b[i] < 1 || b[i] > c->n || a[i] == b[i])
@@ -1344,6 +1362,7 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
                                 int32_t *best_delta) {
     if (!c || !best_b || !best_delta) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (mode != BM_ROW_FIRST_POSITIVE && mode != BM_ROW_ARGMAX) return set_error(BM_EINVAL, "bad mode %d", mode);
     if (a < 0 || a > c->n) return set_error(BM_EINVAL, "row %d outside [0,%d]", a, c->n);
     if (b_lo < 0) b_lo = 0;
@@ -1399,6 +1418,7 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
 extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32_t kind, int32_t *deltas) {
     if (!c || (count > 0 && (!rows || !deltas))) return set_error(BM_EINVAL, "NULL argument");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     if (count < 0 || count > 64) return set_error(BM_EINVAL, "count=%d outside [0,64]", count);
     if (kind != BM_DELTA_LISTS && kind != BM_DELTA_DENSE) return set_error(BM_EINVAL, "kind=%d unknown", kind);
     for (int i = 0; i < count; i++) {
@@ -1544,6 +1564,7 @@ extern "C" int bm_sweep_all_pairs_dense(bm_ctx *c, int32_t *best_a, int32_t *bes
                                         int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix) {
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     BM_HIP(hipSetDevice(c->device));
     const int N1 = c->N1;
     int rc;
@@ -1567,6 +1588,7 @@ extern "C" int bm_sweep_all_pairs(bm_ctx *c, int32_t *best_a, int32_t *best_b, i
                                   int32_t *row_best_b, int32_t *row_best_delta, int32_t *delta_matrix) {
     if (!c) return set_error(BM_EINVAL, "ctx is NULL");
     if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (int prc = ensure_p(c)) return prc;
     BM_HIP(hipSetDevice(c->device));
     const int N1 = c->N1;
     int rc;
