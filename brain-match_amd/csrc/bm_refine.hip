@@ -38,6 +38,7 @@
 
 #include "bm_device.h"
 #include "bm_internal.h"
+#include "bm_wave.h"
 
 using namespace bm;
 
@@ -76,6 +77,7 @@ struct RefineState {
     // steps that read W2 through the mapping leave P / P^T behind it
     int pstale;    // P / P^T do not match the mapping
     int prebuild;  // this step rebuilds P / P^T before its whole sweep (X and P both stale)
+    unsigned long long score_change;  // exact S_opt change of this step over the lines at U
 };
 
 struct MoveRec {
@@ -149,6 +151,7 @@ __global__ void k_step_begin(RefineState *st) {
         st->applied = 0;
         st->nwin = 0;
         st->score_slot = 0;
+        st->score_change = 0;
         st->nu = 0;
         st->full = 1;  // cleared by k_clear_stale when the step's sweep had nothing to do
     }
@@ -232,7 +235,10 @@ __global__ void k_claim(unsigned long long *__restrict__ ckey, long long M, Cand
     for (int q = 0; q < k; q++) atomicMax(&vmax[v[q]], key);
 }
 
-// one wave per candidate: wins when no vertex of N[V] carries a larger claim
+// one wave per candidate: wins when no vertex of N[V] carries a larger claim.
+// The candidate's own vertices are checked first (one lane each), then their
+// G1 neighbourhoods with four list entries in flight per lane; the wave stops
+// at the first larger claim it sees.
 __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__ ckey, long long M, CandView cv,
                                                 const unsigned long long *__restrict__ vmax,
                                                 uint32_t *__restrict__ blocked, int32_t *__restrict__ win,
@@ -245,17 +251,25 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
     if (!key) return;
     int v[8], delta, kind;
     const int k = cand_get(cv, i, v, delta, kind);
-    bool lose = false;
+    int mine = -1;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (q == lane && q < k) mine = v[q];
+    if (__ballot(mine >= 0 && vmax[mine] > key)) return;
     for (int q = 0; q < k; q++) {
-        const int x = v[q];
-        if (lane == 0 && vmax[x] > key) lose = true;
-        for (int e = cptr[x] + lane; e < cptr[x + 1]; e += 64) {
-            const int r = cr[e];
-            const int y = r >= N1 ? r - N1 : r;
-            if (vmax[y] > key) lose = true;
+        const int x = v[q], e1 = cptr[x + 1];
+        for (int eb = cptr[x]; eb < e1; eb += 256) {  // wave-uniform trip count: the ballot sees every lane
+            const int e = eb + lane;
+            int r[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) r[j] = e + 64 * j < e1 ? cr[e + 64 * j] : -1;
+            bool lose = false;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (r[j] >= 0) lose |= vmax[r[j] >= N1 ? r[j] - N1 : r[j]] > key;
+            if (__ballot(lose)) return;
         }
     }
-    if (__ballot(lose)) return;
     for (int q = 0; q < k; q++) {
         const int x = v[q];
         if (lane == 0) blocked[x] = 1;
@@ -351,10 +365,12 @@ __global__ void k_apply(const int32_t *__restrict__ win, int max_win, CandView c
 // may be issued through the scalar cache, and a vector store that overwrites
 // the word with a value that does not depend on the load can then land before
 // the load has returned.
-__global__ void k_step_end(RefineState *st, bm_refine_step *hist) {
+// by_change: the step's score is the previous one plus the exact change over
+// the lines at the moved vertices (k_score_change), else the whole rescore.
+__global__ void k_step_end(RefineState *st, bm_refine_step *hist, int by_change) {
     if (threadIdx.x != 0) return;
     const volatile RefineState *vs = st;
-    const long long now = (long long)vs->score_slot;
+    const long long now = by_change ? vs->score + (long long)vs->score_change : (long long)vs->score_slot;
     const long long gain = (long long)vs->gain;
     const long long prev = vs->score;
     const unsigned applied = vs->applied;
@@ -551,6 +567,75 @@ __global__ void k_corr_inc(int sign, const RefineState *__restrict__ st, const i
         const int corr = pair_corr(out_a, rw.y, cdw[e], (int)Q[(size_t)ga * ld + ga], (int)Q[(size_t)gb * ld + gb],
                                    (int)Q[(size_t)ga * ld + gb], (int)Q[(size_t)gb * ld + ga]);
         if (corr) atomicAdd(X + xidx(a, b, T), sign * corr);
+    }
+}
+
+// The exact S_opt change of the step, from W2 and the images before and after
+// it (not from the deltas or X): every G1 line with an end in U once (an in
+// entry whose source is in U is that source's out entry), lines leaving
+// vertex 0 skipped as the score skips them, min(w, W2[p'(s)][p'(d)]) -
+// min(w, W2[p(s)][p(d)]).  Runs after the apply; uoldv holds p on U.
+__global__ __launch_bounds__(256) void k_score_change(RefineState *__restrict__ st,
+                                                      const int32_t *__restrict__ ulist,
+                                                      const int32_t *__restrict__ ubase,
+                                                      const uint8_t *__restrict__ inU,
+                                                      const int32_t *__restrict__ uoldv,
+                                                      const int32_t *__restrict__ map,
+                                                      const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                      const int16_t *__restrict__ W2, int ld, int N1) {
+    __shared__ long long part[4];
+    const int nu = st->nu, total = st->npairs;
+    long long s = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        const int xi = find_base(ubase, nu, p);
+        const int x = ulist[xi];
+        const int2 rw = crw[cptr[x] + (p - ubase[xi])];
+        const bool out = rw.x >= N1;
+        const int u = out ? rw.x - N1 : rw.x;
+        if (!out && inU[u]) continue;  // counted from u's out entry
+        const int src = out ? x : u, dst = out ? u : x;
+        if (src == 0) continue;
+        const int ns = map[src], nd = map[dst];
+        const int os = inU[src] ? uoldv[src] : ns, od = inU[dst] ? uoldv[dst] : nd;
+        s += min(rw.y, (int)W2[(size_t)ns * ld + nd]) - min(rw.y, (int)W2[(size_t)os * ld + od]);
+    }
+    s = bm_wave::wave_sum64(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(&st->score_change, (unsigned long long)t);
+    }
+}
+
+// The translated G2 words at the images of U after the apply: every line at
+// the image g of a vertex x of U now translates g to x (the images are
+// permuted within U, so no other word changes).  One workgroup per vertex of
+// U: a line s -> g is an entry of s's out-list, g -> v one of v's in-list.
+__global__ __launch_bounds__(256) void k_g2y_patch(const RefineState *__restrict__ st,
+                                                   const int32_t *__restrict__ ulist,
+                                                   const int32_t *__restrict__ map, uint32_t *__restrict__ g2oy,
+                                                   uint32_t *__restrict__ g2iy, const uint32_t *__restrict__ g2o,
+                                                   const uint32_t *__restrict__ g2i,
+                                                   const int32_t *__restrict__ g2optr,
+                                                   const int32_t *__restrict__ g2iptr,
+                                                   const int32_t *__restrict__ g2o_at_in,
+                                                   const int32_t *__restrict__ g2i_at_out) {
+    const int nu = st->nu;
+    for (int i = blockIdx.x; i < nu; i += gridDim.x) {
+        const int x = ulist[i], g = map[x];
+        const uint32_t hi = (uint32_t)x << 16;
+        const int i0 = g2iptr[g], nin = g2iptr[g + 1] - i0;
+        const int o0 = g2optr[g], nout = g2optr[g + 1] - o0;
+        for (int l = threadIdx.x; l < nin + nout; l += 256) {
+            if (l < nin) {
+                const int j = g2o_at_in[i0 + l];
+                g2oy[j] = hi | (g2o[j] & 0xFFFFu);
+            } else {
+                const int j = g2i_at_out[o0 + l - nin];
+                g2iy[j] = hi | (g2i[j] & 0xFFFFu);
+            }
+        }
     }
 }
 
@@ -1057,7 +1142,12 @@ static int enqueue_step(bm_ctx *c) {
     if (!pfree && (rc = inc ? launch_rebuild_p(c, &b->st->pinc, b->ulist, &b->st->nu) : launch_rebuild_p(c)))
         return rc;
     if (c->sparse_ok) {
-        if ((rc = launch_refresh_g2y(c))) return rc;
+        if (inc)  // only the words at the moved images change
+            BM_LAUNCH(c, "g2y_patch", hipLaunchKernelGGL(k_g2y_patch, dim3(1024), dim3(256), 0, c->stream, b->st,
+                                                         b->ulist, c->map, c->g2oy, c->g2iy, c->g2o, c->g2i,
+                                                         c->g2optr, c->g2iptr, c->g2o_at_in, c->g2i_at_out));
+        else if ((rc = launch_refresh_g2y(c)))
+            return rc;
     }
     if (inc) {
         if (b->xdelta_rows)
@@ -1084,8 +1174,16 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "diag_all", hipLaunchKernelGGL(k_diag_all, dim3((N1 + 255) / 256), dim3(256), 0, c->stream,
                                                     b->st, c->X, c->ldx, N1, c->diag));
     }
-    if ((rc = launch_score(c, &b->st->score_slot, pfree))) return rc;
-    BM_LAUNCH(c, "refine_end", hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->stream, b->st, b->hist));
+    // the exact check: with U collected, the exact change over the lines at U (the whole score
+    // is recomputed at the end of the run); else the whole score
+    if (inc)
+        BM_LAUNCH(c, "score_change", hipLaunchKernelGGL(k_score_change, dim3(ublocks), dim3(256), 0, c->stream, b->st,
+                                                        b->ulist, b->ubase, b->inU, b->uoldv, c->map, c->cptr,
+                                                        (const int2 *)c->crw, c->W2, c->ld, N1));
+    else if ((rc = launch_score(c, &b->st->score_slot, pfree)))
+        return rc;
+    BM_LAUNCH(c, "refine_end", hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->stream, b->st, b->hist,
+                                                  inc ? 1 : 0));
     return BM_OK;
 }
 
@@ -1355,8 +1453,19 @@ void refine_stats(bm_ctx *c, bm_refine_stats *out) {
     out->bad_step = s.bad_step ? (int64_t)s.bad_step - 1 : -1;
 }
 
-// Copies the mapping back after a run (the device holds the truth).
+// Copies the mapping back after a run (the device holds the truth).  When the
+// steps checked their scores by the exact change over the lines at the moved
+// vertices, the whole score of the final mapping is recomputed and must equal
+// the run's running score.
 int refine_finish(bm_ctx *c) {
+    bm_batch_state *b = c->batch;
+    const bool rescore = b->incremental && b->hst->moves > 0 && b->hst->bad_step == 0;
+    if (rescore) {
+        BM_HIP(hipMemsetAsync(&b->st->score_slot, 0, 8, c->stream));
+        int rc = launch_score(c, &b->st->score_slot, b->pfree);
+        if (rc) return rc;
+        BM_HIP(hipMemcpyAsync(&b->hst->score_slot, &b->st->score_slot, 8, hipMemcpyDeviceToHost, c->stream));
+    }
     BM_HIP(hipMemcpyAsync(c->hmap.data(), c->map, (size_t)c->N1 * 4, hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipStreamSynchronize(c->stream));
     if (c->sparse_ok) {
@@ -1369,6 +1478,9 @@ int refine_finish(bm_ctx *c) {
     if (c->batch->pfree && c->batch->hst->moves > 0) c->p_stale = true;  // rebuilt on next use
     c->map_epoch++;  // the device moved the mapping; X matches it unless the last step left it stale
     c->batch->x_epoch = c->map_epoch;
+    if (rescore && (long long)b->hst->score_slot != b->hst->score)
+        return set_error(BM_ERANGE, "whole rescore %lld after the run differs from its running score %lld",
+                         (long long)b->hst->score_slot, b->hst->score);
     return BM_OK;
 }
 
