@@ -264,7 +264,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second of the serve workload")
     ap.add_argument("--workload", default="allpairs",
                     choices=["allpairs", "plain", "plain-c2", "rot3", "permset", "batched", "batched-c1",
-                             "batched-permset", "batched-many", "quick", "q3", "serve", "serve-sweep"],
+                             "batched-permset", "batched-many", "quick", "q3", "triples", "serve", "serve-sweep"],
                     help="allpairs (default, the headline); the others measure the other configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -124,6 +124,8 @@ def lib():
         "bm_drive_quick": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "bm_drive_q3": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "bm_q3_triples": (C.c_int64, [C.c_int32, _i32p, C.c_int32, C.c_void_p]),
+        "bm_lex_triples": (C.c_int64, [C.c_int32, C.c_int64, C.c_int64, C.c_void_p]),
+        "bm_drive_triples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -207,6 +209,15 @@ class _Q3Params(C.Structure):
 class _Q3Stats(C.Structure):
     _fields_ = [(f, C.c_int64) for f in ("evals", "perturbations", "moves", "positive_moves", "zero_moves",
                                          "initial_score", "final_score", "best_score", "triples")]
+
+
+class _TriplesParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("max_evals", C.c_int64), ("batch", C.c_int32)]
+
+
+class _TriplesStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("evals", "passes", "perturbations", "moves", "initial_score",
+                                         "final_score", "best_score", "triples")]
 
 
 class _RefineStep(C.Structure):
@@ -360,6 +371,16 @@ def q3_triples(n, changed):
     out = np.zeros(3 * cnt, np.int32)
     lib().bm_q3_triples(n, c, len(c), out.ctypes.data_as(C.c_void_p))
     return out.reshape(cnt, 3)
+
+
+def lex_triples(n, start, count):
+    """bm_lex_triples: lexicographic triples of 1..n from rank start, (count', 3)."""
+    cnt = lib().bm_lex_triples(n, start, count, None)
+    if cnt < 0:
+        raise BMError(int(cnt), "bm_lex_triples", "bad arguments")
+    out = np.zeros(3 * max(cnt, 1), np.int32)
+    lib().bm_lex_triples(n, start, count, out.ctypes.data_as(C.c_void_p))
+    return out[:3 * cnt].reshape(cnt, 3)
 
 
 def write_mapping_mf(path, mapping):
@@ -539,6 +560,15 @@ class Engine:
         _check(self._L.bm_drive_q3(self._h, C.byref(prm), best.ctypes.data_as(C.c_void_p), C.byref(st)),
                "bm_drive_q3")
         return best, {f: getattr(st, f) for f, _ in _Q3Stats._fields_}
+
+    def triples(self, seed=1, max_evals=100, batch=270000):
+        """bm_drive_triples, the batched lexicographic triple pass of triple.cu: (best mapping, stats)."""
+        prm = _TriplesParams(seed, max_evals, batch)
+        best = np.zeros(self.n + 1, np.int32)
+        st = _TriplesStats()
+        _check(self._L.bm_drive_triples(self._h, C.byref(prm), best.ctypes.data_as(C.c_void_p), C.byref(st)),
+               "bm_drive_triples")
+        return best, {f: getattr(st, f) for f, _ in _TriplesStats._fields_}
 
     def apply_swap(self, a, b):
         _check(self._L.bm_apply_swap(self._h, a, b), "bm_apply_swap")

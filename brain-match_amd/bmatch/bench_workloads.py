@@ -436,7 +436,27 @@ def q3(args):
     return out
 
 
+def triples(args):
+    """The batched lexicographic triple pass of triple.cu on the n=1,000 power-law pair: triples scored per
+    second over batches of 270,000 (bm_drive_triples)."""
+    inst = bmatch.Instance.powerlaw(1, 1000, 20000)
+    evals = max(args.steps, 1) * 20
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        e.set_mapping(inst.p0)
+        e.eval_rot3(bmatch.lex_triples(inst.n, 0, 1 << 18), bmatch.BM_ROT3_ALL)  # set-up and form choice
+        t = time.perf_counter()
+        best, st = e.triples(seed=1, max_evals=evals, batch=270000)
+        el = time.perf_counter() - t
+    out = _base("batched lexicographic triple pass: triples scored per second", st["triples"] / el, "triples/s",
+                int(st["evals"]), el * 1000 / st["evals"],
+                {"workload": "power-law pair n=1000 E=20000, random start, triple.cu batched pass, batches of "
+                             "270,000 lexicographic triples", "n": inst.n})
+    out.update({"stats": st, "seconds": el})
+    return out
+
+
 def run(name, args):
     return {"plain": plain, "plain-c2": plain_c2, "rot3": rot3, "permset": permset, "batched": batched,
             "batched-c1": batched_c1, "batched-permset": batched_permset, "batched-many": batched_many,
-            "quick": quick, "q3": q3, "serve": serve, "serve-sweep": serve_sweep}[name](args)
+            "quick": quick, "q3": q3, "triples": triples, "serve": serve, "serve-sweep": serve_sweep}[name](args)

@@ -587,3 +587,111 @@ out:
     free(q.zero);
     return rc;
 }
+
+/* lexicographic rank -> triple: with first element m1 there are C(n - m1, 2)
+ * triples, then with second element m2 there are n - m2 */
+int64_t bm_lex_triples(int32_t n, int64_t start, int64_t count, int32_t *out) {
+    if (n < 3 || start < 0 || count < 0) return -1;
+    const int64_t total = (int64_t)n * (n - 1) * (n - 2) / 6;
+    if (start >= total || count == 0) return 0;
+    if (count > total - start) count = total - start;
+    if (!out) return count;
+    int64_t t = start;
+    int32_t m1 = 1, m2, m3;
+    for (;;) {
+        const int64_t r = (int64_t)(n - m1) * (n - m1 - 1) / 2;
+        if (t < r) break;
+        t -= r;
+        m1++;
+    }
+    m2 = m1 + 1;
+    while (t >= n - m2) {
+        t -= n - m2;
+        m2++;
+    }
+    m3 = m2 + 1 + (int32_t)t;
+    for (int64_t i = 0; i < count; i++) {
+        out[3 * i] = m1;
+        out[3 * i + 1] = m2;
+        out[3 * i + 2] = m3;
+        if (++m3 > n) {
+            if (++m2 > n - 1) {
+                m1++;
+                m2 = m1 + 1;
+            }
+            m3 = m2 + 1;
+        }
+    }
+    return count;
+}
+
+/* triple.cu:941-1196 */
+int bm_drive_triples(bm_ctx *ctx, const bm_triples_params *p, int32_t *best_mapping, bm_triples_stats *stats) {
+    if (!ctx || !p || p->max_evals < 1 || p->batch < 1) return BM_EINVAL;
+    const int32_t n = bm_num_vertices(ctx);
+    if (n < 10) return BM_EINVAL; /* a perturbation of 5 swaps needs 10 distinct indices */
+    const int64_t total = (int64_t)n * (n - 1) * (n - 2) / 6;
+    const int64_t B = p->batch < total ? p->batch : total;
+    const size_t N1 = (size_t)n + 1;
+    bm_triples_stats st;
+    memset(&st, 0, sizeof st);
+    int32_t *best = malloc(sizeof(int32_t) * N1);
+    int32_t *tri = malloc(sizeof(int32_t) * 3 * (size_t)B);
+    int32_t *d = malloc(sizeof(int32_t) * (size_t)B);
+    int32_t *pp = malloc(sizeof(int32_t) * (size_t)B);
+    int rc = BM_OK;
+    if (!best || !tri || !d || !pp) {
+        rc = BM_ENOMEM;
+        goto out;
+    }
+    uint64_t rng = p->seed;
+    int64_t cur;
+    if ((rc = bm_get_mapping(ctx, best)) || (rc = bm_score(ctx, &cur))) goto out;
+    st.initial_score = st.best_score = cur;
+    while (st.evals < p->max_evals) {
+        int found = 0;
+        int64_t start = 0;
+        st.passes++;
+        for (; start < total && st.evals < p->max_evals; start += B) {
+            const int64_t m = bm_lex_triples(n, start, B, tri);
+            if ((rc = bm_eval_rot3(ctx, tri, m, BM_ROT3_ALL, d, pp))) goto out;
+            st.evals++;
+            st.triples += m;
+            int32_t bd = 0;
+            int64_t bi = -1;
+            for (int64_t i = 0; i < m; i++)
+                if (d[i] > bd) {
+                    bd = d[i];
+                    bi = i;
+                }
+            if (bd > 0) { /* :1148-1168 */
+                if ((rc = bm_apply_rot3(ctx, tri[3 * bi], tri[3 * bi + 1], tri[3 * bi + 2], pp[bi]))) goto out;
+                cur += bd;
+                st.moves++;
+                if (cur > st.best_score) {
+                    st.best_score = cur;
+                    if ((rc = bm_get_mapping(ctx, best))) goto out;
+                }
+                found = 1;
+                break;
+            }
+        }
+        if (!found && start >= total) { /* :1174-1181 */
+            if ((rc = bm_set_mapping(ctx, best))) goto out;
+            const int k = 3 + (int)(bm_rng_next(&rng) % 3);
+            if ((rc = bm_perturb(ctx, k, BM_PERTURB_OMP, &rng, NULL))) goto out;
+            if ((rc = bm_score(ctx, &cur))) goto out;
+            st.perturbations++;
+        }
+    }
+    st.final_score = cur;
+    rc = bm_set_mapping(ctx, best);
+out:
+    if (!rc && best_mapping) memcpy(best_mapping, best, sizeof(int32_t) * N1);
+    if (!rc && stats) *stats = st;
+    free(best);
+    free(tri);
+    free(d);
+    free(pp);
+    return rc;
+}

@@ -267,6 +267,43 @@ typedef struct {
 
 int bm_drive_q3(bm_ctx *ctx, const bm_q3_params *params, int32_t *best_mapping, bm_q3_stats *stats);
 
+/* The triples m1 < m2 < m3 of 1..n in lexicographic order, from index start
+ * (0-based): writes count triples (3 ids each) to out and returns how many
+ * there were (fewer at the end of the range; negative on bad arguments).
+ * The order triple.cu's calculateThreeNodeSwapKernel intends for its batch
+ * index (triple.cu:54-75). */
+int64_t bm_lex_triples(int32_t n, int64_t start, int64_t count, int32_t *out);
+
+/* The batched triple pass of triple.cu:941-1196 (one process, no best-file
+ * synchronisation): the lexicographic triples of 1..n in batches of `batch`
+ * from the first; each batch scored with Delta_G3 over p = 1..5
+ * (bm_eval_rot3, BM_ROT3_ALL); the first triple of the batch with the largest
+ * positive delta is applied (bm_apply_rot3), the running score raised by its
+ * delta, and the pass restarts from the first batch (:1148-1168); a batch
+ * without one moves on to the next.  After a pass in which no batch had a
+ * positive triple, the best mapping so far is perturbed by 3 + rand()%3
+ * swaps of random_swap_k_vertices (:822-858, bm_perturb with BM_PERTURB_OMP)
+ * and rescored (:1174-1181), and the next pass starts.  Random draws come
+ * from a splitmix64 state seeded with seed.  Stops after max_evals batch
+ * evaluations and leaves the context on the best mapping.  Deviation: the
+ * reference advances by batch + total/3 after a batch without a move, which
+ * takes its index map (whose blocks divide by 6 where the count of pairs
+ * needs 2) beyond m1 = n from the second batch on; here the batches follow
+ * one another through the whole range. */
+typedef struct {
+    uint64_t seed;
+    int64_t max_evals; /* batch evaluations */
+    int32_t batch;     /* 270000 in triple.cu */
+} bm_triples_params;
+
+typedef struct {
+    int64_t evals, passes, perturbations, moves;
+    int64_t initial_score, final_score, best_score;
+    int64_t triples; /* triples evaluated over all batches */
+} bm_triples_stats;
+
+int bm_drive_triples(bm_ctx *ctx, const bm_triples_params *params, int32_t *best_mapping, bm_triples_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

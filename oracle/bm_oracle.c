@@ -1188,3 +1188,106 @@ int orc_q3(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, 
     free(q.zero);
     return 0;
 }
+
+/* rank -> triple by counting (O(n) per triple; tests use small n) */
+int64_t orc_lex_triples(int n, int64_t start, int64_t count, int32_t *out) {
+    int64_t got = 0;
+    for (int64_t t = start; got < count; t++) {
+        int64_t r = t;
+        int m1 = 1, m2 = 0, m3 = 0, ok = 0;
+        for (; m1 <= n - 2 && !ok; m1++) {
+            const int64_t with_m1 = (int64_t)(n - m1) * (n - m1 - 1) / 2;
+            if (r >= with_m1) {
+                r -= with_m1;
+                continue;
+            }
+            for (m2 = m1 + 1; m2 <= n - 1; m2++) {
+                if (r >= n - m2) {
+                    r -= n - m2;
+                    continue;
+                }
+                m3 = m2 + 1 + (int)r;
+                ok = 1;
+                break;
+            }
+            if (ok) break;
+        }
+        if (!ok) break;
+        if (out) {
+            out[3 * got] = m1;
+            out[3 * got + 1] = m2;
+            out[3 * got + 2] = m3;
+        }
+        got++;
+    }
+    return got;
+}
+
+/* triple.cu:941-1196 */
+int orc_triples_pass(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int batch, int64_t *out) {
+    const int n = g->n;
+    if (n < 10 || batch < 1 || max_evals < 1) return -1;
+    const size_t bytes = ((size_t)n + 1) * sizeof(int32_t);
+    const int64_t total = (int64_t)n * (n - 1) * (n - 2) / 6;
+    const int64_t B = batch < total ? batch : total;
+    int32_t *tri = xcalloc((size_t)B * 3, sizeof(int32_t));
+    int32_t *best = xcalloc((size_t)n + 1, sizeof(int32_t));
+    uint64_t rng = seed;
+    memcpy(best, map, bytes);
+    int64_t cur = orc_score_opt(g, map), initial = cur, best_score = cur;
+    int64_t evals = 0, passes = 0, perturbations = 0, moves = 0, triples = 0;
+    while (evals < max_evals) {
+        int found = 0;
+        int64_t start = 0;
+        passes++;
+        for (; start < total && evals < max_evals; start += B) {
+            const int64_t m = orc_lex_triples(n, start, B, tri);
+            evals++;
+            triples += m;
+            int best_delta = 0, best_p = 0;
+            int64_t best_idx = -1;
+            for (int64_t i = 0; i < m; i++) { /* :1136-1143: the first index of the largest delta */
+                int p;
+                const int d = orc_best_g3(g, map, tri[3 * i], tri[3 * i + 1], tri[3 * i + 2], 1, &p);
+                if (d > best_delta) {
+                    best_delta = d;
+                    best_idx = i;
+                    best_p = p;
+                }
+            }
+            if (best_delta > 0) { /* :1148-1168 */
+                orc_apply_g3(map, tri[3 * best_idx], tri[3 * best_idx + 1], tri[3 * best_idx + 2], best_p);
+                cur += best_delta;
+                moves++;
+                if (cur > best_score) {
+                    memcpy(best, map, bytes);
+                    best_score = cur;
+                }
+                found = 1;
+                break;
+            }
+        }
+        if (!found && start >= total) { /* :1174-1181 */
+            memcpy(map, best, bytes);
+            const int k = 3 + (int)(orc_rng_next(&rng) % 3);
+            int32_t sel[16];
+            orc_perturb(g, map, k, 0, &rng, sel);
+            cur = orc_score_opt(g, map);
+            perturbations++;
+        }
+    }
+    if (out) {
+        out[0] = evals;
+        out[1] = passes;
+        out[2] = perturbations;
+        out[3] = moves;
+        out[4] = initial;
+        out[5] = cur;
+        out[6] = best_score;
+        out[7] = triples;
+    }
+    memcpy(map, best, bytes);
+    free(best);
+    free(tri);
+    return 0;
+}

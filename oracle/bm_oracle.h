@@ -196,6 +196,23 @@ int64_t orc_q3_triples(int n, const int32_t *chg, int k, int32_t *out);
  * (current), best score, triples evaluated. */
 int orc_q3(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int zero_limit, int best_window,
            double gap, double gap_norm, int64_t *out);
+/* The lexicographic triple of rank t (0-based) among the triples
+ * m1 < m2 < m3 of 1..n, restated from its definition: the ranks count the
+ * triples of smaller first element, then of smaller second element (the
+ * order triple.cu:54-75 intends).  out gets count triples from rank start;
+ * returns how many (fewer at the end of the range). */
+int64_t orc_lex_triples(int n, int64_t start, int64_t count, int32_t *out);
+/* triple.cu:941-1196 batched triple pass for max_evals batch evaluations:
+ * batches of `batch` lexicographic triples from the first; the first triple
+ * of the largest positive orc_best_g3 (p = 1..5) in a batch is applied, the
+ * score raised by its delta, and the pass restarts from the first batch; a
+ * pass without one perturbs the best mapping by 3 + rand()%3 uniform swaps
+ * (orc_perturb mode 0, random_swap_k_vertices :822-858) and rescores.
+ * Batches follow one another (the reference's stride of batch + total/3
+ * leaves its index range).  rand() is orc_rng_next.  map ends on the best
+ * mapping; out[0..7] = evals, passes, perturbations, moves, initial, final
+ * (current), best score, triples evaluated. */
+int orc_triples_pass(const orc_graphs *g, int32_t *map, uint64_t seed, int64_t max_evals, int batch, int64_t *out);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,8 @@ def lib():
         "orc_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
         "orc_q3_triples": (C.c_int64, [C.c_int, _i32p, C.c_int, vp]),
         "orc_q3": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, _i64p]),
+        "orc_lex_triples": (C.c_int64, [C.c_int, C.c_int64, C.c_int64, vp]),
+        "orc_triples_pass": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int64, C.c_int, _i64p]),
         "orc_vertex_scores": (None, [vp, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]),
         "orc_perturb": (C.c_int, [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_uint64), _i32p]),
         "orc_row_best_restart": (C.c_int, [vp, _i32p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int64, _i64p]),
@@ -175,6 +177,14 @@ class Oracle:
                 "best_score", "triples")
         return mm, dict(zip(keys, (int(x) for x in out)))
 
+    def triples_pass(self, m, seed=1, max_evals=100, batch=270000):
+        mm = _i32(m).copy()
+        out = np.zeros(8, np.int64)
+        if lib().orc_triples_pass(self._h, mm, seed, max_evals, batch, out):
+            raise ValueError("orc_triples_pass failed")
+        keys = ("evals", "passes", "perturbations", "moves", "initial_score", "final_score", "best_score", "triples")
+        return mm, dict(zip(keys, (int(x) for x in out)))
+
     def delta_exact(self, m, a, b):
         return lib().orc_delta_swap_exact(self._h, _i32(m), a, b)
 
@@ -254,6 +264,13 @@ def q3_triples(n, changed):
     out = np.zeros(3 * cnt, np.int32)
     lib().orc_q3_triples(n, c, len(c), out.ctypes.data_as(C.c_void_p))
     return out.reshape(cnt, 3)
+
+
+def lex_triples(n, start, count):
+    cnt = lib().orc_lex_triples(n, start, count, None)
+    out = np.zeros(3 * max(cnt, 1), np.int32)
+    lib().orc_lex_triples(n, start, count, out.ctypes.data_as(C.c_void_p))
+    return out[:3 * cnt].reshape(cnt, 3)
 
 
 def unrank_perm(k, rank):
