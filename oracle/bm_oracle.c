@@ -25,6 +25,10 @@
  *   orc_row_best_restart : equals plain row-best passes when no restart
  *       triggers; best score = rescoring of the returned mapping.
  *   orc_quick : counter identities and rescoring of the returned mapping.
+ *   orc_lex_triples : itertools.combinations order.
+ *   orc_triples_pass : a brute-force restatement that rescores every
+ *       permutation of every triple (simple graphs, where Delta_G3 is the
+ *       exact change), with the same generator draws.
  */
 #include "bm_oracle.h"
 
