@@ -204,9 +204,11 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
 // Q[n+1+d][a] != 0 for d = p^-1(g), g an out-neighbour; the comb entries
 // reading row s are the G1 out-lines s->b, those reading row n+1+d the G1
 // in-lines b->d.  k_row_expand gives each such list head kWaves waves
-// (comb(a) entries, then in- and out-neighbours of p(a)) and adds the
-// contributions for b in [lo, hi] into a device row buffer with atomics; its
-// block 0 also adds Corr(a,b) for adjacent b and seeds the result key.
+// (comb(a) entries, then in- and out-neighbours of p(a)) per slice of its
+// list (grid y: kRowSlices slices, so that a hub's long list is walked by
+// several workgroups at once) and adds the contributions for b in [lo, hi]
+// into a device row buffer with atomics; its block (0, 0) also adds Corr(a,b)
+// for adjacent b and seeds the result key.
 // k_row_final forms Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b),
 // reduces it to the first positive or arg-max key, and zeroes the buffer for
 // the next call.
@@ -227,7 +229,9 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
     const int pa = map[a];
     const int i0 = g2iptr[pa], nin = g2iptr[pa + 1] - i0;
     const int o0 = g2optr[pa], nout = g2optr[pa + 1] - o0;
+    const int sl = blockIdx.y, ns = gridDim.y;  // this block's slice of each list
     if (blockIdx.x == 0) {
+        if (sl != 0) return;
         if (threadIdx.x == 0) *result = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
         const int paa = (int)Q[(size_t)a * ld + a];
         for (int e = ca0 + threadIdx.x; e < ca0 + dega; e += 256) {
@@ -247,7 +251,8 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
         const bool out = rw.x >= N1;
         const int g = map[out ? rw.x - N1 : rw.x];
         const uint32_t *L = out ? g2iy : g2oy;
-        const int l0 = out ? g2iptr[g] : g2optr[g], l1 = out ? g2iptr[g + 1] : g2optr[g + 1];
+        const int b0 = out ? g2iptr[g] : g2optr[g], b1 = out ? g2iptr[g + 1] : g2optr[g + 1];
+        const int l0 = b0 + (int)((long long)(b1 - b0) * sl / ns), l1 = b0 + (int)((long long)(b1 - b0) * (sl + 1) / ns);
         const int w = rw.y, wneg = min(w, 0);
         int j = l0 + lane;
         for (; j + 3 * kLanes < l1; j += 4 * kLanes) {  // four list words in flight per lane
@@ -274,7 +279,8 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
         const int val = (int)(int16_t)(yv & 0xffff);
         // in-neighbour: rows s = v of P, read by the out-lines v->b (in entries of b);
         // out-neighbour: rows n+1+v of P^T, read by the in-lines b->v (out entries of b)
-        const int l0 = in ? cptr[v] : cmid[v], l1 = in ? cmid[v] : cptr[v + 1];
+        const int b0 = in ? cptr[v] : cmid[v], b1 = in ? cmid[v] : cptr[v + 1];
+        const int l0 = b0 + (int)((long long)(b1 - b0) * sl / ns), l1 = b0 + (int)((long long)(b1 - b0) * (sl + 1) / ns);
         int e = l0 + lane;
         for (; e + 3 * kLanes < l1; e += 4 * kLanes) {
             int2 rw[4];
@@ -1337,6 +1343,8 @@ extern "C" int bm_eval_swaps_dense(bm_ctx *c, const int32_t *a, const int32_t *b
 // per workgroup), 4 (the default) a whole workgroup, which spreads the long
 // lists of hub neighbours over 256 lanes.
 // dense: the dense-form lists in place of the comb lists.
+constexpr int kRowSlices = 4;  // workgroups per list head
+
 static int launch_row_expand(bm_ctx *c, bool dense, int a, int lo, int hi, int mode, unsigned long long *res) {
     static const int waves = [] {
         const char *e = getenv("BM_ROW_WAVES");
@@ -1349,7 +1357,8 @@ static int launch_row_expand(bm_ctx *c, bool dense, int a, int lo, int hi, int m
     const int32_t *ldw = dense ? c->dfw : c->cdw;
 #define BM_ROW_EXPAND(W)                                                                                         \
     BM_LAUNCH(c, "row_expand",                                                                                  \
-              hipLaunchKernelGGL(k_row_expand<W>, dim3(1 + (heads + per - 1) / per), dim3(256), 0, c->stream, a, lo, \
+              hipLaunchKernelGGL(k_row_expand<W>, dim3(1 + (heads + per - 1) / per, kRowSlices), dim3(256), 0,      \
+                                 c->stream, a, lo,                                                                  \
                                  hi, mode, c->N1, lptr, lmid, lrw, ldw, c->map, c->g2optr, c->g2oy, c->g2iptr,       \
                                  c->g2iy, c->Q, c->ld, c->rowacc, res))
     if (waves == 1) BM_ROW_EXPAND(1);
