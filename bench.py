@@ -7,7 +7,7 @@ synthetic n = 18,524 / 3.0M-line graph pair, everything resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
                     [--cpu-seconds S | --no-cpu-baseline] [--sweep-only]
-                    [--calibrate-copy MiB] [--calibrate-gather MiB]
+                    [--calibrate-copy MiB] [--calibrate-gather MiB] [--calibrate-stream MiB]
                     [--workload NAME] [--window S] [--jobs J]
 
 Prints one JSON line:
@@ -197,6 +197,32 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_bytes": mib * (1 << 20), "launches": 3}))
 
 
+def calibrate_stream(mib, reps=10):
+    """The read and read+write rates that torch's own streaming kernels (a
+    float sum, a copy) reach over a mib-MiB buffer, timed with events: library
+    kernels beside the 8 TB/s peak."""
+    import torch
+
+    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    xf = x.view(torch.float32)
+    out = {"calibrate_stream_bytes": mib * (1 << 20), "reps": reps}
+    for name, fn, moved in (("read_gbs", lambda: xf.sum(), 1), ("copy_gbs", lambda: y.copy_(x), 2)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = moved * mib * (1 << 20) / (ms * 1e-3) / 1e9
+        out[name.replace("_gbs", "_ms")] = ms
+    print(json.dumps(out))
+
+
 def calibrate_gather(mib, count_log2=24):
     """Reference gathers for the memory read counters: count random 4-byte words
     from a mib-MiB buffer (a miss per word), then the same count of consecutive
@@ -259,6 +285,7 @@ def main():
     ap.add_argument("--sweep-only", action="store_true", help="only the sweep (profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB")
     ap.add_argument("--calibrate-gather", type=int, default=0, metavar="MiB")
+    ap.add_argument("--calibrate-stream", type=int, default=0, metavar="MiB")
     ap.add_argument("--window", type=float, default=0.0, help="seconds per leg of the batched comparison")
     ap.add_argument("--jobs", type=int, default=0, help="jobs of the serve workload")
     ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second of the serve workload")
@@ -276,6 +303,9 @@ def main():
         return
     if args.calibrate_gather:
         calibrate_gather(args.calibrate_gather)
+        return
+    if args.calibrate_stream:
+        calibrate_stream(args.calibrate_stream)
         return
 
     import bmatch

@@ -145,21 +145,33 @@ __device__ __forceinline__ int cand_get(const CandView &cv, long long i, int *v,
     return cv.k;
 }
 
-__global__ void k_step_begin(RefineState *st) {
-    if (threadIdx.x == 0) {
+// The step's opening in one launch: the state fields reset (thread 0), with
+// the P rebuild gate when the step reads W2 through the mapping (pgate: P and
+// P^T are rebuilt before a whole sweep when both X and P are stale), and
+// three word ranges zeroed (the claim keys, the blocked flags, the byte flags
+// of the incremental update; a count of 0 skips one).  Volatile reads as in
+// k_step_end.
+__global__ void k_step_prologue(RefineState *st, int pgate, uint32_t *__restrict__ f0, long long n0,
+                                uint32_t *__restrict__ f1, long long n1, uint32_t *__restrict__ f2, long long n2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        const volatile RefineState *vs = st;
+        const int rebuild = pgate && vs->xstale && vs->pstale ? 1 : 0;
         st->gain = 0;
         st->applied = 0;
         st->nwin = 0;
         st->score_slot = 0;
         st->score_change = 0;
         st->nu = 0;
-        st->full = 1;  // cleared by k_clear_stale when the step's sweep had nothing to do
+        st->full = 1;  // cleared at the candidate keys when the step's sweep had nothing to do
+        if (pgate) {
+            st->prebuild = rebuild;
+            if (rebuild) st->pstale = 0;
+        }
     }
-}
-
-__global__ void k_fill32(uint32_t *__restrict__ p, long long count, uint32_t v) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = v;
+    if (i < n0) f0[i] = 0;
+    if (i < n1) f1[i] = 0;
+    if (i < n2) f2[i] = 0;
 }
 
 // 3-vertex candidates: u uniform, v a G1 neighbour of u, w a neighbour of v or u
@@ -205,9 +217,16 @@ __global__ void k_gen_sets(int32_t *__restrict__ sets, int S, int k, int uniform
     for (int q = 0; q < k; q++) sets[(long long)k * i + q] = s[q];
 }
 
+// With clear (the incremental update), thread 0 also records whether the
+// step's sweep ran whole (X was stale) and marks X current.
 __global__ void k_cand_keys(unsigned long long *__restrict__ ckey, long long M, CandView cv,
-                            const uint8_t *__restrict__ vsimple) {
+                            const uint8_t *__restrict__ vsimple, RefineState *__restrict__ clear) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear && i == 0) {
+        const int stale = ((const volatile RefineState *)clear)->xstale;  // as in k_step_end
+        clear->full = stale;  // the gain kernels ran whole iff X was stale
+        clear->xstale = 0;
+    }
     if (i >= M) return;
     int v[8], delta = 0, kind = 0;
     const int k = cand_get(cv, i, v, delta, kind);
@@ -409,17 +428,6 @@ __global__ void k_state_init(RefineState *st, int keep_x, int pstale) {
     *st = z;
 }
 
-// A step that reads W2 through the mapping needs P / P^T only for a whole
-// sweep (the sparse gain kernel's pair corrections, P's diagonal, long lists
-// streamed as rows): rebuilt first when both X and P are stale.
-__global__ void k_p_gate(RefineState *st) {
-    if (threadIdx.x != 0) return;
-    const volatile RefineState *vs = st;
-    const int rebuild = vs->xstale && vs->pstale ? 1 : 0;
-    st->prebuild = rebuild;
-    if (rebuild) st->pstale = 0;
-}
-
 // ---------------------------------------------------------------- incremental X
 //
 // X(u,v) = C_u + sum over e in comb(u) of the G2 list of e's neighbour's
@@ -435,7 +443,7 @@ __global__ void k_p_gate(RefineState *st) {
 //     their new images (k_xcol_plan, k_xcol, k_xcol_sum);
 //   - Corr changes on the lines at U: subtracted before the move with the old
 //     P, added after it with the new P (k_corr_inc);
-//   - X(v,v) is read back into diag (k_diag_all).
+//   - X(v,v) is read back into diag (k_score_change).
 // f(w, q) = min(w, q) - min(w, 0).  Costs about 3 |U| / n of a sweep; steps
 // that move more than inc_max vertices mark X stale instead.
 
@@ -571,7 +579,8 @@ __global__ void k_corr_inc(int sign, const RefineState *__restrict__ st, const i
 }
 
 // The exact S_opt change of the step, from W2 and the images before and after
-// it (not from the deltas or X): every G1 line with an end in U once (an in
+// it (not from the deltas or X), and X(v,v) read back into diag when X was
+// updated in place: every G1 line with an end in U once (an in
 // entry whose source is in U is that source's out entry), lines leaving
 // vertex 0 skipped as the score skips them, min(w, W2[p'(s)][p'(d)]) -
 // min(w, W2[p(s)][p(d)]).  Runs after the apply; uoldv holds p on U.
@@ -582,9 +591,14 @@ __global__ __launch_bounds__(256) void k_score_change(RefineState *__restrict__ 
                                                       const int32_t *__restrict__ uoldv,
                                                       const int32_t *__restrict__ map,
                                                       const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
-                                                      const int16_t *__restrict__ W2, int ld, int N1) {
+                                                      const int16_t *__restrict__ W2, int ld, int N1,
+                                                      const int32_t *__restrict__ X, int64_t ldx,
+                                                      int32_t *__restrict__ diag) {
     __shared__ long long part[4];
     const int nu = st->nu, total = st->npairs;
+    if (st->inc)  // X(v,v) back into diag after the incremental update
+        for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < N1; v += gridDim.x * blockDim.x)
+            diag[v] = X[xidx(v, v, (int)(ldx >> 6))];
     long long s = 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
         const int xi = find_base(ubase, nu, p);
@@ -940,21 +954,6 @@ __global__ __launch_bounds__(256) void k_xcol_sum(const RefineState *__restrict_
     }
 }
 
-__global__ void k_diag_all(const RefineState *__restrict__ st, const int32_t *__restrict__ X, int64_t ldx, int N1,
-                           int32_t *__restrict__ diag) {
-    if (!st->inc) return;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < N1) diag[v] = X[xidx(v, v, (int)(ldx >> 6))];
-}
-
-__global__ void k_clear_stale(RefineState *st) {
-    if (threadIdx.x == 0) {
-        const int stale = ((const volatile RefineState *)st)->xstale;  // as in k_step_end
-        st->full = stale;  // the gain kernels ran whole iff X was stale
-        st->xstale = 0;
-    }
-}
-
 }  // namespace
 
 struct bm_batch_state {
@@ -1070,19 +1069,9 @@ static int enqueue_step(bm_ctx *c) {
     const int N1 = c->N1;
     const CandView cv = cand_view(c);
     int rc;
-    BM_LAUNCH(c, "refine_begin", hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, c->stream, b->st));
-    const long long vm_words = (long long)p.rounds * N1 * 2;
-    BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((unsigned)((vm_words + 255) / 256)), dim3(256), 0,
-                                            c->stream, (uint32_t *)b->vmax, vm_words, 0u));
-    BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((N1 + 255) / 256), dim3(256), 0, c->stream, b->blocked,
-                                            (long long)N1, 0u));
     // X is updated in place between steps only with the sparse sweep and swap candidates
     const bool inc = p.pairs && sweep_is_sparse(c);
     b->incremental = inc;
-    const long long mark_words = (long long)(N1 + 3) / 4 * 3;  // inU, the row marks, the moved flags
-    if (inc)
-        BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill32, dim3((unsigned)((mark_words + 255) / 256)), dim3(256), 0,
-                                                c->stream, (uint32_t *)b->inU, mark_words, 0u));
     // every kernel of the step can read W2 through the mapping instead of P / P^T when X is kept
     // current and the triples and sets are scored from it; P / P^T are then rebuilt only for a
     // whole sweep, and otherwise on the next use outside the run (ensure_p)
@@ -1090,13 +1079,18 @@ static int enqueue_step(bm_ctx *c) {
     b->pfree = pfree;
     const int16_t *Pq = pfree ? c->W2 : c->Q;
     const int32_t *pmap = pfree ? c->map : nullptr;
-    if (pfree) {
-        BM_LAUNCH(c, "p_gate", hipLaunchKernelGGL(k_p_gate, dim3(1), dim3(64), 0, c->stream, b->st));
-        if ((rc = launch_rebuild_p_if(c, &b->st->prebuild))) return rc;
+    {
+        const long long vm_words = (long long)p.rounds * N1 * 2;
+        const long long mark_words = inc ? (long long)(N1 + 3) / 4 * 3 : 0;  // inU, the row marks, the moved flags
+        const long long most = std::max(std::max(vm_words, (long long)N1), mark_words);
+        BM_LAUNCH(c, "refine_begin",
+                  hipLaunchKernelGGL(k_step_prologue, dim3((unsigned)((most + 255) / 256)), dim3(256), 0, c->stream,
+                                     b->st, pfree ? 1 : 0, (uint32_t *)b->vmax, vm_words, b->blocked, (long long)N1,
+                                     (uint32_t *)b->inU, mark_words));
     }
+    if (pfree && (rc = launch_rebuild_p_if(c, &b->st->prebuild))) return rc;
     if (p.pairs) {
         if ((rc = sweep_enqueue(c, nullptr, inc ? &b->st->xstale : nullptr))) return rc;
-        if (inc) BM_LAUNCH(c, "clear_stale", hipLaunchKernelGGL(k_clear_stale, dim3(1), dim3(64), 0, c->stream, b->st));
     }
     if (p.triples > 0) {
         BM_LAUNCH(c, "gen_triples",
@@ -1115,7 +1109,8 @@ static int enqueue_step(bm_ctx *c) {
     }
     const unsigned mb = (unsigned)((b->M + 255) / 256);
     BM_LAUNCH(c, "cand_keys",
-              hipLaunchKernelGGL(k_cand_keys, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, c->vsimple));
+              hipLaunchKernelGGL(k_cand_keys, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, c->vsimple,
+                                 inc ? b->st : nullptr));
     for (int r = 0; r < p.rounds; r++) {
         unsigned long long *vm = b->vmax + (size_t)r * N1;
         BM_LAUNCH(c, "claim", hipLaunchKernelGGL(k_claim, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, vm,
@@ -1171,15 +1166,14 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, 1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
                                                     Pq, c->ld, N1, c->X, c->ldx, nullptr, pmap));
-        BM_LAUNCH(c, "diag_all", hipLaunchKernelGGL(k_diag_all, dim3((N1 + 255) / 256), dim3(256), 0, c->stream,
-                                                    b->st, c->X, c->ldx, N1, c->diag));
     }
     // the exact check: with U collected, the exact change over the lines at U (the whole score
     // is recomputed at the end of the run); else the whole score
     if (inc)
         BM_LAUNCH(c, "score_change", hipLaunchKernelGGL(k_score_change, dim3(ublocks), dim3(256), 0, c->stream, b->st,
                                                         b->ulist, b->ubase, b->inU, b->uoldv, c->map, c->cptr,
-                                                        (const int2 *)c->crw, c->W2, c->ld, N1));
+                                                        (const int2 *)c->crw, c->W2, c->ld, N1, c->X, c->ldx,
+                                                        c->diag));
     else if ((rc = launch_score(c, &b->st->score_slot, pfree)))
         return rc;
     BM_LAUNCH(c, "refine_end", hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->stream, b->st, b->hist,
