@@ -895,43 +895,93 @@ __global__ __launch_bounds__(1024) void k_xcol_plan(RefineState *__restrict__ st
     }
 }
 
-__global__ __launch_bounds__(1024) void k_xcol(const RefineState *__restrict__ st, const int32_t *__restrict__ ulist,
-                                               const int32_t *__restrict__ cbase, const int32_t *__restrict__ map,
-                                               const int32_t *__restrict__ cptr, const int32_t *__restrict__ cmid,
-                                               const int2 *__restrict__ crw, const int32_t *__restrict__ g2optr,
-                                               const uint32_t *__restrict__ g2oy, const int32_t *__restrict__ g2iptr,
-                                               const uint32_t *__restrict__ g2iy, int N1,
-                                               int32_t *__restrict__ scratch) {
+// A work item's G2 lines are taken kXcolHeads at a time; their G1 sub-lists
+// are laid end to end and the flat entry range is split evenly over the 16
+// waves (a hub preimage's long sub-list no longer falls to one group of lanes).
+constexpr int kXcolHeads = 512;
+
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(80))) void k_xcol(
+    const RefineState *__restrict__ st, const int32_t *__restrict__ ulist, const int32_t *__restrict__ cbase,
+    const int32_t *__restrict__ map, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cmid,
+    const int2 *__restrict__ crw, const int32_t *__restrict__ g2optr, const uint32_t *__restrict__ g2oy,
+    const int32_t *__restrict__ g2iptr, const uint32_t *__restrict__ g2iy, int N1, int32_t *__restrict__ scratch) {
     extern __shared__ __align__(16) int col[];
+    __shared__ int ls[kXcolHeads], lv[kXcolHeads], pref[kXcolHeads + 1], wsum[16];
     if (!st->inc) return;
     const int nu = st->nu, items = st->xchunks, L = st->xlines;
-    // 16 lanes per G2 line, so a wave has four lines' list reads in flight
-    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int it = blockIdx.x; it < items; it += gridDim.x) {
         const int vi = find_base(cbase, nu, it);
         const int g0 = map[ulist[vi]];
         for (int u = threadIdx.x; u < N1; u += 1024) col[u] = 0;
-        __syncthreads();
         const int i0 = g2iptr[g0], nin = g2iptr[g0 + 1] - i0;
         const int o0 = g2optr[g0], nout = g2optr[g0 + 1] - o0;
         const int h0 = (it - cbase[vi]) * L, h1 = min(h0 + L, nin + nout);
-        for (int h = h0 + grp; h < h1; h += 64) {
-            const bool in = h < nin;
-            const uint32_t yv = in ? g2iy[i0 + h] : g2oy[o0 + h - nin];
-            const int s = yv >> 16;
-            if (s == 0xFFFF) continue;
-            const int val = (int)(int16_t)(yv & 0xffff);
-            const int l0 = in ? cptr[s] : cmid[s], l1 = in ? cmid[s] : cptr[s + 1];
-            for (int e = l0 + gl; e < l1; e += 64) {  // four entries in flight per lane
-                int2 rw[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) rw[q] = e + 16 * q < l1 ? crw[e + 16 * q] : make_int2(-1, 0);
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (rw[q].x >= 0) atomicAdd(&col[in ? rw[q].x - N1 : rw[q].x], min(rw[q].y, val) - min(rw[q].y, 0));
+        for (int c0 = h0; c0 < h1; c0 += kXcolHeads) {
+            const int m = min(kXcolHeads, h1 - c0);
+            int len = 0;
+            if ((int)threadIdx.x < m) {
+                // in-line s -> p'(v): the out-lines s -> u of s (in entries of u); out-line p'(v) -> s: the
+                // in-lines u -> s (out entries of u)
+                const int h = c0 + threadIdx.x;
+                const bool in = h < nin;
+                const uint32_t yv = in ? g2iy[i0 + h] : g2oy[o0 + h - nin];
+                const int sv = yv >> 16;
+                if (sv != 0xFFFF) {
+                    const int l0 = in ? cptr[sv] : cmid[sv];
+                    len = (in ? cmid[sv] : cptr[sv + 1]) - l0;
+                    ls[threadIdx.x] = l0;
+                    lv[threadIdx.x] = (int)(int16_t)(yv & 0xffff) * 2 + (in ? 1 : 0);  // G2 weight, and the side
+                }
             }
+            int total;
+            const int ex = block_scan_excl(len, wsum, total);  // also orders the zeroing of col
+            if ((int)threadIdx.x < m) pref[threadIdx.x] = ex;
+            if (threadIdx.x == 0) pref[m] = total;
+            __syncthreads();
+            const int span = (total + 15) / 16;
+            const int w0 = wave * span, w1 = min(total, w0 + span);
+            if (w0 < w1) {
+                int lo = 0, hi = m - 1;  // the line holding flat entry w0
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (pref[mid] <= w0) lo = mid;
+                    else hi = mid - 1;
+                }
+                int l = lo, s0 = pref[l], s1 = pref[l + 1], at = ls[l] - s0, hv = lv[l];
+                for (int q0 = w0 + lane; q0 < w1; q0 += 256) {  // four entries in flight per lane
+                    int ad[4], vu[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int q = q0 + 64 * k;
+                        ad[k] = -1;
+                        vu[k] = 0;
+                        if (q < w1) {
+                            while (s1 <= q) {
+                                l++;
+                                s0 = s1;
+                                s1 = pref[l + 1];
+                                at = ls[l] - s0;
+                                hv = lv[l];
+                            }
+                            ad[k] = at + q;
+                            vu[k] = hv;
+                        }
+                    }
+                    int2 rw[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) rw[k] = ad[k] >= 0 ? crw[ad[k]] : make_int2(-1, 0);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (rw[k].x < 0) continue;
+                        const int val = vu[k] >> 1;
+                        atomicAdd(&col[(vu[k] & 1) ? rw[k].x - N1 : rw[k].x], min(rw[k].y, val) - min(rw[k].y, 0));
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        __syncthreads();  // an item without lines: the zeroing before the store
         int32_t *row = scratch + (size_t)it * N1;
         for (int u = threadIdx.x; u < N1; u += 1024) row[u] = col[u];
         __syncthreads();
