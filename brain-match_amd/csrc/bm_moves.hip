@@ -541,22 +541,31 @@ int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *
 // have changed), then one thread per triple.  The G1 cell matrix is built once
 // per graph pair.
 // The dense-form lists and their G1 cell matrix (once per graph pair).
+// The G1 cell matrix dfW1 of the dense-form lists.  When every G1 line is
+// its own dense cell those lists are the comb lists, which are on the device
+// already, so the cells come from them without building the dense-form lists.
 int bm::ensure_df_cells(bm_ctx *c) {
-    int rc = ensure_dense_lists(c);
-    if (rc || c->dfW1) return rc;
+    if (c->dfW1) return BM_OK;
+    int rc;
+    const int32_t *ptr = c->cptr, *mid = c->cmid;
+    const int2 *rw = (const int2 *)c->crw;
+    if (!c->dense_equal) {
+        if ((rc = ensure_dense_lists(c))) return rc;
+        ptr = c->dfptr, mid = c->dfmid, rw = (const int2 *)c->dfrw;
+    }
     const size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
     if ((rc = dev_alloc(c, (void **)&c->dfW1, N1 * ld * 2))) return rc;
     const int64_t cells = (int64_t)(N1 * ld);
     BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_zero16, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0,
                                                 c->stream, c->dfW1, cells));
     BM_LAUNCH(c, "df_cells", hipLaunchKernelGGL(k_df_cells, dim3((unsigned)((N1 + 3) / 4)), dim3(256), 0, c->stream,
-                                                c->dfW1, c->dfptr, c->dfmid, (const int2 *)c->dfrw, c->N1, c->ld));
+                                                c->dfW1, ptr, mid, rw, c->N1, c->ld));
     return BM_OK;
 }
 
 static int rot3_table_prepare(bm_ctx *c) {
-    int rc = ensure_df_cells(c);
-    if (rc) return rc;
+    int rc = ensure_dense_lists(c);
+    if (rc || (rc = ensure_df_cells(c))) return rc;
     const size_t N1 = (size_t)c->N1, ld = (size_t)c->ld;
     if (!c->dfX && (rc = dev_alloc(c, (void **)&c->dfX, (N1 + 63) / 64 * 64 * ld * 4))) return rc;
     if (!c->dfXP && (rc = dev_alloc(c, (void **)&c->dfXP, N1 * ld * 8))) return rc;

@@ -19,6 +19,8 @@
 #include <mutex>
 #include <tuple>
 
+#include <hipcub/hipcub.hpp>
+
 #include "bm_device.h"
 #include "bm_wave.h"
 #include "bm_internal.h"
@@ -878,6 +880,55 @@ __global__ void k_diag_from_x(const int32_t *__restrict__ X, int64_t ldx, const 
 // The row plan of the sparse kernel: rows with more than kSplitEntries comb
 // entries become ceil(deg / kSplitEntries) items; items ordered by entry count,
 // largest first, so that the longest blocks start first.
+// The quad copy's layout, built on the device: every G2 word's place (word k
+// of out-list g at 4 qo[g] + (k - optr[g]), in-list words after them), the
+// padding words (N1 << 16, zero terms at the spare slot), and the comb
+// entries of each row in increasing neighbour id (crws, cdws: a radix sort of
+// (row << 16 | neighbour) keys; ties in any order, the row sums do not depend
+// on it).
+__global__ void k_quad_pos(int32_t *__restrict__ pos, const int32_t *__restrict__ optr,
+                           const int32_t *__restrict__ iptr, const int32_t *__restrict__ qo,
+                           const int32_t *__restrict__ qi, int N1, int64_t ng2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * ng2) return;
+    const bool in = t >= ng2;
+    const int k = (int)(in ? t - ng2 : t);
+    const int32_t *ptr = in ? iptr : optr;
+    int lo = 0, hi = N1 - 1;  // the list holding word k: the last g with ptr[g] <= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ptr[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    pos[t] = 4 * (in ? qi[lo] : qo[lo]) + (k - ptr[lo]);
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, int64_t count, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// one wave per row: key (row << 16 | neighbour) and index of each comb entry
+__global__ void k_row_nb_keys(uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
+                              const int32_t *__restrict__ cptr, const int2 *__restrict__ crw, int N1) {
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= N1) return;
+    for (int e = cptr[a] + (threadIdx.x & 63); e < cptr[a + 1]; e += 64) {
+        const int r = crw[e].x;
+        keys[e] = ((uint32_t)a << 16) | (uint32_t)(r >= N1 ? r - N1 : r);
+        vals[e] = e;
+    }
+}
+
+__global__ void k_gather_entries(int2 *__restrict__ rws, int32_t *__restrict__ dws, const int2 *__restrict__ rw,
+                                 const int32_t *__restrict__ dw, const int32_t *__restrict__ idx, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int32_t j = idx[i];
+    rws[i] = rw[j];
+    dws[i] = dw[j];
+}
+
 static int build_quads(bm_ctx *c) {
     if (c->g2q) return BM_OK;
     const int N1 = c->N1;
@@ -886,52 +937,53 @@ static int build_quads(bm_ctx *c) {
     qi[0] = qo[N1];
     for (int g = 0; g < N1; g++) qi[g + 1] = qi[g] + (c->hg2iptr[g + 1] - c->hg2iptr[g] + 3) / 4;
     int rc;
-    // each word's place: out-list words first (g2oy), then in-list words (g2iy = g2oy + ng2)
-    std::vector<int32_t> pos((size_t)c->ng2 * 2);
-    for (int g = 0; g < N1; g++) {
-        for (int k = c->hg2optr[g]; k < c->hg2optr[g + 1]; k++) pos[k] = 4 * qo[g] + (k - c->hg2optr[g]);
-        for (int k = c->hg2iptr[g]; k < c->hg2iptr[g + 1]; k++)
-            pos[c->ng2 + k] = 4 * qi[g] + (k - c->hg2iptr[g]);
-    }
-    const std::vector<uint32_t> pads((size_t)qi[N1] * 4, (uint32_t)N1 << 16);
+    const int64_t nwords = 2 * c->ng2;
     if (!c->qoptr && (rc = dev_alloc(c, (void **)&c->qoptr, (size_t)(N1 + 1) * 4))) return rc;
     if (!c->qiptr && (rc = dev_alloc(c, (void **)&c->qiptr, (size_t)(N1 + 1) * 4))) return rc;
-    if (!c->g2qpos && (rc = dev_alloc(c, (void **)&c->g2qpos, pos.size() * 4 + 4))) return rc;
-    if ((rc = dev_alloc(c, (void **)&c->g2q, pads.size() * 4))) return rc;
+    if (!c->g2qpos && (rc = dev_alloc(c, (void **)&c->g2qpos, (size_t)nwords * 4 + 4))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->g2q, (size_t)qi[N1] * 16))) return rc;
     c->nquads = qi[N1];
-    if ((rc = copy_sync(c->qoptr, qo.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice))) return rc;
-    if ((rc = copy_sync(c->qiptr, qi.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice))) return rc;
-    if (!pos.empty() && (rc = copy_sync(c->g2qpos, pos.data(), pos.size() * 4, hipMemcpyHostToDevice))) return rc;
-    if ((rc = copy_sync(c->g2q, pads.data(), pads.size() * 4, hipMemcpyHostToDevice))) return rc;
-    {
-        // each row's entries in increasing neighbour id, ties in list order: a
-        // counting pass by neighbour, then a stable one by row
-        const size_t nc = (size_t)c->ncomb;
-        std::vector<int2> rw(nc), rws(nc);
-        std::vector<int32_t> dw(nc), dws(nc), byx(nc), rowof(nc);
-        std::vector<int64_t> pos(N1 + 1, 0);
-        if ((rc = copy_sync(rw.data(), c->crw, nc * 8, hipMemcpyDeviceToHost))) return rc;
-        if ((rc = copy_sync(dw.data(), c->cdw, nc * 4, hipMemcpyDeviceToHost))) return rc;
-        auto nb = [&](size_t i) { return rw[i].x >= N1 ? rw[i].x - N1 : rw[i].x; };
-        for (size_t i = 0; i < nc; i++) pos[nb(i) + 1]++;
-        for (int v = 0; v < N1; v++) pos[v + 1] += pos[v];
-        for (size_t i = 0; i < nc; i++) byx[pos[nb(i)]++] = (int32_t)i;
-        pos[0] = 0;
-        for (int a = 0; a < N1; a++) {
-            pos[a + 1] = pos[a] + c->hcdeg[a];
-            for (int64_t k = pos[a]; k < pos[a + 1]; k++) rowof[k] = a;
+    BM_HIP(hipMemcpyAsync(c->qoptr, qo.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    BM_HIP(hipMemcpyAsync(c->qiptr, qi.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    if (nwords > 0)
+        BM_LAUNCH(c, "quad_pos", hipLaunchKernelGGL(k_quad_pos, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0,
+                                                    c->stream, c->g2qpos, c->g2optr, c->g2iptr, c->qoptr, c->qiptr,
+                                                    N1, c->ng2));
+    if (qi[N1] > 0)
+        BM_LAUNCH(c, "quad_pad", hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, c->stream, c->g2q,
+                                                    (int64_t)qi[N1] * 4, (uint32_t)N1 << 16));
+    const size_t nc = (size_t)c->ncomb;
+    if (!c->crws && (rc = dev_alloc(c, (void **)&c->crws, nc * 8 + 8))) return rc;
+    if (!c->cdws && (rc = dev_alloc(c, (void **)&c->cdws, nc * 4 + 4))) return rc;
+    if (!c->pdg && (rc = dev_alloc(c, (void **)&c->pdg, (size_t)N1 * 4))) return rc;
+    if (nc > 0) {
+        uint32_t *k0 = nullptr, *k1 = nullptr;
+        int32_t *v0 = nullptr, *v1 = nullptr;
+        void *tmp = nullptr;
+        size_t tmp_bytes = 0;
+        int end_bit = 16;
+        while ((1 << (end_bit - 16)) < N1) end_bit++;
+        BM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, v0, v1, (int)nc, 0, end_bit,
+                                                  c->stream));
+        if ((rc = dev_alloc(c, (void **)&k0, nc * 4)) || (rc = dev_alloc(c, (void **)&k1, nc * 4)) ||
+            (rc = dev_alloc(c, (void **)&v0, nc * 4)) || (rc = dev_alloc(c, (void **)&v1, nc * 4)) ||
+            (rc = dev_alloc(c, &tmp, tmp_bytes))) {
+            dev_free(c, (void **)&k0, nc * 4), dev_free(c, (void **)&k1, nc * 4), dev_free(c, (void **)&v0, nc * 4),
+                dev_free(c, (void **)&v1, nc * 4), dev_free(c, &tmp, tmp_bytes);
+            return rc;
         }
-        for (size_t k = 0; k < nc; k++) {
-            const int32_t i = byx[k];
-            const int64_t j = pos[rowof[i]]++;
-            rws[j] = rw[i];
-            dws[j] = dw[i];
-        }
-        if (!c->crws && (rc = dev_alloc(c, (void **)&c->crws, nc * 8))) return rc;
-        if (!c->cdws && (rc = dev_alloc(c, (void **)&c->cdws, nc * 4))) return rc;
-        if (!c->pdg && (rc = dev_alloc(c, (void **)&c->pdg, (size_t)N1 * 4))) return rc;
-        if ((rc = copy_sync(c->crws, rws.data(), nc * 8, hipMemcpyHostToDevice))) return rc;
-        if ((rc = copy_sync(c->cdws, dws.data(), nc * 4, hipMemcpyHostToDevice))) return rc;
+        BM_LAUNCH(c, "row_nb_keys", hipLaunchKernelGGL(k_row_nb_keys, dim3((N1 + 3) / 4), dim3(256), 0, c->stream,
+                                                       k0, v0, c->cptr, (const int2 *)c->crw, N1));
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, v0, v1, (int)nc, 0, end_bit,
+                                                          c->stream);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(k_gather_entries, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream,
+                               (int2 *)c->crws, c->cdws, (const int2 *)c->crw, c->cdw, v1, (int64_t)nc);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // before the buffers are freed
+        dev_free(c, (void **)&k0, nc * 4), dev_free(c, (void **)&k1, nc * 4), dev_free(c, (void **)&v0, nc * 4),
+            dev_free(c, (void **)&v1, nc * 4), dev_free(c, &tmp, tmp_bytes);
+        if (e != hipSuccess) return hip_fail(e, "neighbour-ordered comb lists");
     }
     return BM_OK;
 }
