@@ -78,6 +78,7 @@ struct RefineState {
     int pstale;    // P / P^T do not match the mapping
     int prebuild;  // this step rebuilds P / P^T before its whole sweep (X and P both stale)
     unsigned long long score_change;  // exact S_opt change of this step over the lines at U
+    int nhash, ndense;  // the row updates by kind
 };
 
 struct MoveRec {
@@ -162,6 +163,8 @@ __global__ void k_step_prologue(RefineState *st, int pgate, uint32_t *__restrict
         st->nwin = 0;
         st->score_slot = 0;
         st->score_change = 0;
+        st->nhash = 0;
+        st->ndense = 0;
         st->nu = 0;
         st->full = 1;  // cleared at the candidate keys when the step's sweep had nothing to do
         if (pgate) {
@@ -626,6 +629,10 @@ __global__ __launch_bounds__(256) void k_score_change(RefineState *__restrict__ 
 // the image g of a vertex x of U now translates g to x (the images are
 // permuted within U, so no other word changes).  One workgroup per vertex of
 // U: a line s -> g is an entry of s's out-list, g -> v one of v's in-list.
+// With rowq set, each comb entry (x, u) of a moved x also counts, for row u
+// of X, one moved-neighbour entry in rowq[u] and the words of its two G2
+// lists in rowq[N1 + u] (u's mirrored entry: an out entry x -> u is the in
+// entry of u, which reads the out-lists, ulists[2x]).
 __global__ __launch_bounds__(256) void k_g2y_patch(const RefineState *__restrict__ st,
                                                    const int32_t *__restrict__ ulist,
                                                    const int32_t *__restrict__ map, uint32_t *__restrict__ g2oy,
@@ -634,10 +641,23 @@ __global__ __launch_bounds__(256) void k_g2y_patch(const RefineState *__restrict
                                                    const int32_t *__restrict__ g2optr,
                                                    const int32_t *__restrict__ g2iptr,
                                                    const int32_t *__restrict__ g2o_at_in,
-                                                   const int32_t *__restrict__ g2i_at_out) {
+                                                   const int32_t *__restrict__ g2i_at_out,
+                                                   const uint8_t *__restrict__ moved, const int4 *__restrict__ ulists,
+                                                   const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                   int N1, int32_t *__restrict__ rowq) {
     const int nu = st->nu;
     for (int i = blockIdx.x; i < nu; i += gridDim.x) {
         const int x = ulist[i], g = map[x];
+        if (rowq && moved[x]) {
+            const int4 Lo = ulists[2 * x], Li = ulists[2 * x + 1];
+            const int wo = Lo.y + Lo.w, wi = Li.y + Li.w;
+            for (int e = cptr[x] + threadIdx.x; e < cptr[x + 1]; e += 256) {
+                const int r = crw[e].x;
+                const int u = r >= N1 ? r - N1 : r;
+                atomicAdd(rowq + u, 1);
+                atomicAdd(rowq + N1 + u, r >= N1 ? wo : wi);
+            }
+        }
         const uint32_t hi = (uint32_t)x << 16;
         const int i0 = g2iptr[g], nin = g2iptr[g + 1] - i0;
         const int o0 = g2optr[g], nout = g2optr[g + 1] - o0;
@@ -697,8 +717,151 @@ __global__ __launch_bounds__(256) void k_xdelta(const RefineState *__restrict__ 
     }
 }
 
+// The row updates split by size.  k_g2y_patch counts, from the lists of the
+// moved vertices, each row's moved-neighbour entries and the words of their
+// lists (rowq, roww); k_xrow_class (a thread per row) lists the rows with at
+// most kHashQual such entries and kHashWords words, which k_xdelta_hash
+// accumulates in an LDS hash table of kHashSlots (column, sum) pairs (256
+// threads and about 22 KB of LDS per row, so seven rows are resident per CU,
+// and a row touches only its hashed columns), and the other rows, which keep
+// the (n+1)-int LDS row of k_xdelta_rows.
+constexpr int kHashQual = 256;
+constexpr int kHashWords = 1536;
+constexpr int kHashSlots = 2048;  // more than kHashWords: probing always finds a slot
+
+__global__ void k_xrow_class(RefineState *__restrict__ st, const int32_t *__restrict__ rowq,
+                             const int32_t *__restrict__ roww, int N1, int32_t *__restrict__ hrows,
+                             int32_t *__restrict__ drows) {
+    if (!st->inc) return;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = u < N1 ? rowq[u] : 0;
+    const bool small = q <= kHashQual && (unsigned)roww[u < N1 ? u : 0] <= (unsigned)kHashWords;
+    const bool h = q > 0 && small, d = q > 0 && !small;
+    // one atomic per wave and list on the two counters (thousands of rows a step)
+    const unsigned long long mh = __ballot(h), md = __ballot(d);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1;
+    int bh = 0, bd = 0;
+    if (lane == 0) {
+        if (mh) bh = atomicAdd(&st->nhash, __popcll(mh));
+        if (md) bd = atomicAdd(&st->ndense, __popcll(md));
+    }
+    bh = __shfl(bh, 0);
+    bd = __shfl(bd, 0);
+    if (h) hrows[bh + __popcll(mh & below)] = u;
+    if (d) drows[bd + __popcll(md & below)] = u;
+}
+
+__global__ __launch_bounds__(256) void k_xdelta_hash(const RefineState *__restrict__ st,
+                                                     const int32_t *__restrict__ hrows,
+                                                     const uint8_t *__restrict__ moved,
+                                                     const int4 *__restrict__ ulists, const int32_t *__restrict__ cptr,
+                                                     const int2 *__restrict__ crw, const uint32_t *__restrict__ g2y,
+                                                     int N1, int32_t *__restrict__ X, int64_t ldx) {
+    __shared__ int keys[kHashSlots], vals[kHashSlots];
+    __shared__ int2 qual[kHashQual];
+    __shared__ int lstart[2 * kHashQual], pref[2 * kHashQual + 1], wsum[4], nq;
+    if (!st->inc) return;
+    const int nrows = st->nhash, T = (int)(ldx >> 6);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int u = hrows[r];
+        for (int i = tid; i < kHashSlots; i += 256) {
+            keys[i] = -1;
+            vals[i] = 0;
+        }
+        if (tid == 0) nq = 0;
+        __syncthreads();
+        for (int e = cptr[u] + tid; e < cptr[u + 1]; e += 256) {  // the entries whose neighbour moved
+            const int2 rw = crw[e];
+            if (moved[rw.x >= N1 ? rw.x - N1 : rw.x]) {
+                const int slot = atomicAdd(&nq, 1);  // fewer than kHashQual + 1 (rowq)
+                if (slot < kHashQual) qual[slot] = rw;
+            }
+        }
+        __syncthreads();
+        const int m = min(nq, kHashQual);
+        int len = 0, lnew = 0;
+        if (tid < m) {
+            const int2 rw = qual[tid];
+            const int x = rw.x >= N1 ? rw.x - N1 : rw.x;
+            // an out entry u -> x reads the in-list of x's image, an in entry x -> u its out-list
+            const int4 L = ulists[2 * x + (rw.x >= N1 ? 1 : 0)];
+            lstart[2 * tid] = L.x;
+            lstart[2 * tid + 1] = L.z;
+            lnew = L.y;
+            len = L.y + L.w;
+        }
+        int x = len;  // inclusive scan over the 4 waves
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        const int before = (wave > 0 ? wsum[0] : 0) + (wave > 1 ? wsum[1] : 0) + (wave > 2 ? wsum[2] : 0);
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (tid < m) {
+            pref[2 * tid] = before + x - len;
+            pref[2 * tid + 1] = before + x - len + lnew;
+        }
+        if (tid == 0) pref[2 * m] = total;
+        __syncthreads();
+        const int span = (total + 3) / 4;
+        const int w0 = wave * span, w1 = min(total, w0 + span);
+        if (w0 < w1) {
+            int lo = 0, hi = 2 * m - 1;  // the list holding flat word w0
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pref[mid] <= w0) lo = mid;
+                else hi = mid - 1;
+            }
+            int l = lo, s0 = pref[l], s1 = pref[l + 1];
+            int at = lstart[l] - s0, w = qual[l >> 1].y, sg = (l & 1) ? -1 : 1;
+            for (int q = w0 + lane; q < w1; q += 64) {
+                while (s1 <= q) {
+                    l++;
+                    s0 = s1;
+                    s1 = pref[l + 1];
+                    at = lstart[l] - s0;
+                    w = qual[l >> 1].y;
+                    sg = (l & 1) ? -1 : 1;
+                }
+                const uint32_t yv = g2y[at + q];
+                const int y = yv >> 16;
+                if (y == 0xFFFF) continue;
+                const int v = sg * (min(w, (int)(int16_t)(yv & 0xffff)) - min(w, 0));
+                unsigned h = ((unsigned)y * 2654435761u) >> 21;  // 11 bits: kHashSlots
+                for (int probe = 0; probe < kHashSlots; probe++) {
+                    const int old = atomicCAS(&keys[h], -1, y);
+                    if (old == -1 || old == y) {
+                        atomicAdd(&vals[h], v);
+                        break;
+                    }
+                    h = (h + 1) & (kHashSlots - 1);
+                }
+            }
+        }
+        __syncthreads();
+        // X(u, y) += the sums: the reads of a thread's eight slots issued together
+        int32_t *xr = X + (size_t)(u >> 6) * T * 4096 + ((u & 63) << 6);
+        int ky[kHashSlots / 256], dv[kHashSlots / 256], old[kHashSlots / 256];
+#pragma unroll
+        for (int k = 0; k < kHashSlots / 256; k++) {
+            ky[k] = keys[tid + 256 * k];
+            dv[k] = ky[k] < 0 ? 0 : vals[tid + 256 * k];
+        }
+#pragma unroll
+        for (int k = 0; k < kHashSlots / 256; k++) old[k] = dv[k] ? xr[((ky[k] >> 6) << 12) + (ky[k] & 63)] : 0;
+#pragma unroll
+        for (int k = 0; k < kHashSlots / 256; k++)
+            if (dv[k]) xr[((ky[k] >> 6) << 12) + (ky[k] & 63)] = old[k] + dv[k];
+        __syncthreads();
+    }
+}
+
 // The same row updates as k_xdelta, gathered per row: the workgroup of a
-// marked row u (k_corr_inc) finds the entries of comb(u) whose neighbour x
+// row u of the dense kind (k_xrow_class) finds the entries of comb(u) whose neighbour x
 // moved (an in entry x -> u reads the G2 out-list of x's image, an out entry
 // u -> x its in-list), lays the two lists of each such entry end to end (the
 // list at the new image, added, and the one at the old image, subtracted) and
@@ -714,7 +877,7 @@ constexpr int kXdeltaUnroll = 4;  // list words in flight per lane
 constexpr int kXdeltaRmw = 8;     // row entries per thread whose X reads are issued together
 
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(80))) void k_xdelta_rows(
-    const RefineState *__restrict__ st, const uint8_t *__restrict__ rowmark, const uint8_t *__restrict__ moved,
+    const RefineState *__restrict__ st, const int32_t *__restrict__ drows, const uint8_t *__restrict__ moved,
     const int4 *__restrict__ ulists, const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
     const uint32_t *__restrict__ g2y, int N1, int32_t *__restrict__ X, int64_t ldx) {
     extern __shared__ __align__(16) int row[];
@@ -792,8 +955,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(80))) void k_x
         }
         __syncthreads();
     };
-    for (int u = blockIdx.x; u < N1; u += gridDim.x) {
-        if (!rowmark[u]) continue;  // no neighbour of u moved (uniform)
+    const int nd = st->ndense;
+    for (int i = blockIdx.x; i < nd; i += gridDim.x) {
+        const int u = drows[i];
         if (threadIdx.x == 0) nq = 0;
         for (int y = threadIdx.x; y < N1; y += 1024) row[y] = 0;
         __syncthreads();
@@ -1021,6 +1185,8 @@ struct bm_batch_state {
     int xcap = 0;
     uint8_t *inU = nullptr;
     int4 *ulists = nullptr;    // per moved vertex: its G2 out- and in-lists at the new and the old image
+    int32_t *hrows = nullptr;  // rows updated through the LDS hash table
+    int32_t *drows = nullptr;  // rows updated through the (n+1)-int LDS row
     bool incremental = false;  // the captured step updates X incrementally
     bool rot3_x = false;       // triples and sets scored from the step's X (k_rot3_x, k_permset) instead of their lists
     bool pfree = false;        // the captured step reads W2 through the mapping and leaves P / P^T behind it
@@ -1067,6 +1233,8 @@ static CandView cand_view(bm_ctx *c) {
 // marks, the moved flags, (N1 + 3) / 4 words each
 static uint8_t *rowmark(bm_batch_state *b, int N1) { return b->inU + (size_t)(N1 + 3) / 4 * 4; }
 static uint8_t *moved_flags(bm_batch_state *b, int N1) { return b->inU + (size_t)(N1 + 3) / 4 * 8; }
+// then the per-row counts of the row updates: rowq (entries), roww (words), N1 ints each
+static int32_t *row_q(bm_batch_state *b, int N1) { return (int32_t *)(b->inU + (size_t)(N1 + 3) / 4 * 12); }
 
 static void drop_graph(bm_batch_state *b) {
     std::lock_guard<std::mutex> g(sync_mutex());
@@ -1087,7 +1255,7 @@ int bm::free_batch(bm_ctx *c) {
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
     fr(b->ckey), fr(b->vmax), fr(b->blocked), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
     fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU), fr(b->uoldv);
-    fr(b->ulists);
+    fr(b->ulists), fr(b->hrows), fr(b->drows);
     if (b->hst) (void)hipHostFree(b->hst);
     if (b->done) (void)hipEventDestroy(b->done);
     c->dev_bytes -= (int64_t)b->bytes;
@@ -1131,7 +1299,8 @@ static int enqueue_step(bm_ctx *c) {
     const int32_t *pmap = pfree ? c->map : nullptr;
     {
         const long long vm_words = (long long)p.rounds * N1 * 2;
-        const long long mark_words = inc ? (long long)(N1 + 3) / 4 * 3 : 0;  // inU, the row marks, the moved flags
+        // inU, the row marks, the moved flags, the row counts
+        const long long mark_words = inc ? (long long)(N1 + 3) / 4 * 3 + 2LL * N1 : 0;
         const long long most = std::max(std::max(vm_words, (long long)N1), mark_words);
         BM_LAUNCH(c, "refine_begin",
                   hipLaunchKernelGGL(k_step_prologue, dim3((unsigned)((most + 255) / 256)), dim3(256), 0, c->stream,
@@ -1190,16 +1359,27 @@ static int enqueue_step(bm_ctx *c) {
         if (inc)  // only the words at the moved images change
             BM_LAUNCH(c, "g2y_patch", hipLaunchKernelGGL(k_g2y_patch, dim3(1024), dim3(256), 0, c->stream, b->st,
                                                          b->ulist, c->map, c->g2oy, c->g2iy, c->g2o, c->g2i,
-                                                         c->g2optr, c->g2iptr, c->g2o_at_in, c->g2i_at_out));
+                                                         c->g2optr, c->g2iptr, c->g2o_at_in, c->g2i_at_out,
+                                                         moved_flags(b, N1), b->ulists, c->cptr,
+                                                         (const int2 *)c->crw, N1,
+                                                         b->xdelta_rows ? row_q(b, N1) : nullptr));
         else if ((rc = launch_refresh_g2y(c)))
             return rc;
     }
     if (inc) {
-        if (b->xdelta_rows)
+        if (b->xdelta_rows) {
+            BM_LAUNCH(c, "xrow_class",
+                      hipLaunchKernelGGL(k_xrow_class, dim3((N1 + 255) / 256), dim3(256), 0, c->stream, b->st,
+                                         row_q(b, N1), row_q(b, N1) + N1, N1, b->hrows, b->drows));
+            BM_LAUNCH(c, "xdelta_hash",
+                      hipLaunchKernelGGL(k_xdelta_hash, dim3((unsigned)std::min(N1, 4096)), dim3(256), 0, c->stream,
+                                         b->st, b->hrows, moved_flags(b, N1), b->ulists, c->cptr,
+                                         (const int2 *)c->crw, c->g2oy, N1, c->X, c->ldx));
             BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta_rows, dim3((unsigned)std::min(N1, 2048)), dim3(1024),
-                                                      (size_t)N1 * 4, c->stream, b->st, rowmark(b, N1),
+                                                      (size_t)N1 * 4, c->stream, b->st, b->drows,
                                                       moved_flags(b, N1), b->ulists, c->cptr, (const int2 *)c->crw,
                                                       c->g2oy, N1, c->X, c->ldx));
+        }
         else
             BM_LAUNCH(c, "xdelta", hipLaunchKernelGGL(k_xdelta, dim3(ublocks), dim3(256), 0, c->stream, b->st,
                                                       b->ulist, b->uold, b->ubase, b->inU, c->map, c->cptr,
@@ -1342,8 +1522,10 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->ubase, (size_t)(N1 + 1) * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->cbase, (size_t)(N1 + 1) * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->uoldv, (size_t)N1 * 4))) return fail(rc);
-    if ((rc = balloc(c, (void **)&b->inU, (size_t)(N1 + 3) / 4 * 12))) return fail(rc);  // inU, row marks, moved
+    if ((rc = balloc(c, (void **)&b->inU, (size_t)(N1 + 3) / 4 * 12 + (size_t)N1 * 8))) return fail(rc);  // + rowq, roww
     if ((rc = balloc(c, (void **)&b->ulists, (size_t)N1 * 2 * sizeof(int4)))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->hrows, (size_t)N1 * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->drows, (size_t)N1 * 4))) return fail(rc);
     // steps moving at most inc_max vertices update X in place (about 3 |U| / n of a sweep)
     b->inc_max = std::max(64, c->N1 / 8);
     if (const char *e = getenv("BM_REFINE_INC_MAX")) b->inc_max = atoi(e);
