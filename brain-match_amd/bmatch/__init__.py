@@ -630,12 +630,12 @@ class Engine:
         _check(self._L.bm_refine_log(self._h, capacity), "bm_refine_log")
 
     def refine_moves(self, capacity=1 << 20):
-        """Applied-move records of the last run: int32 array (count, 20) =
-        step, kind, k, delta, vertices[8], new images[8]."""
+        """Applied-move records of the last run: int32 array (count, 22) =
+        step, kind, k, delta, round, base delta, vertices[8], new images[8]."""
         cnt = C.c_int64()
         _check(self._L.bm_refine_moves(self._h, C.byref(cnt), None, 0), "bm_refine_moves")
         m = min(cnt.value, capacity)
-        rec = np.zeros((max(m, 1), 20), np.int32)
+        rec = np.zeros((max(m, 1), 22), np.int32)
         _check(self._L.bm_refine_moves(self._h, C.byref(cnt), rec.ctypes.data_as(C.c_void_p), m), "bm_refine_moves")
         return rec[:m], cnt.value
 

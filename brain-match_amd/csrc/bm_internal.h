@@ -248,7 +248,8 @@ int launch_refresh_g2y(bm_ctx *c);      // G2 lists through p^-1
 int sweep_prepare(bm_ctx *c);           // allocations and kernel choice of the all-pairs sweep
 // row-best keys into keys[0..n], global best into keys[n+1]; with run_flag the
 // sparse gain kernels run only when *run_flag != 0 (X stale), the combine always
-int sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag = nullptr);
+// skip_flag (device): when it reads nonzero, the row-best keys are left as they are
+int sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag = nullptr, const int *skip_flag = nullptr);
 bool sweep_is_sparse(const bm_ctx *c);  // the sweep runs the sparse gain kernel
 
 // batched refinement, split for the serving loop: start a run, enqueue a chunk

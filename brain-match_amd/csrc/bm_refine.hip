@@ -12,10 +12,13 @@
 //      delta kind equals the exact change of S_opt) gets the priority key
 //      delta << 32 | ~index;
 //   3. selection, `rounds` times: each live candidate claims its vertices
-//      (atomicMax of its key), then wins if no vertex of its closed G1
-//      neighbourhood N[V] was claimed by a larger key; winners block N[V],
-//      which retires every candidate that touches it.  Two winners are never
-//      equal or G1-adjacent, so the selected deltas add up exactly;
+//      (atomicMax of its key); a candidate whose closed G1 neighbourhood N[V]
+//      holds no larger claim is scored exactly given the winners of the
+//      earlier rounds (its lines to the vertices they moved corrected) and
+//      wins if that change is positive; winners take their vertices, which
+//      retires every candidate sharing one.  Two winners of one round are
+//      never equal or G1-adjacent, so each round adds up exactly on top of
+//      the earlier ones;
 //   4. apply: every winner permutes the images of its vertices (and p^-1);
 //      P and P^T are rebuilt from W2 and the new mapping, the G2 lists
 //      re-expressed through p^-1;
@@ -79,10 +82,12 @@ struct RefineState {
     int prebuild;  // this step rebuilds P / P^T before its whole sweep (X and P both stale)
     unsigned long long score_change;  // exact S_opt change of this step over the lines at U
     int nhash, ndense;  // the row updates by kind
+    int keep_keys;      // the previous step of this run moved nothing: the row-best keys still hold
+    int live[32];       // per selection round: some candidate claimed (later rounds have nothing to do without)
 };
 
 struct MoveRec {
-    int32_t step, kind, k, delta;
+    int32_t step, kind, k, delta, round, base;
     int32_t v[8];
     int32_t img[8];  // images after the move
 };
@@ -149,7 +154,7 @@ __device__ __forceinline__ int cand_get(const CandView &cv, long long i, int *v,
 // The step's opening in one launch: the state fields reset (thread 0), with
 // the P rebuild gate when the step reads W2 through the mapping (pgate: P and
 // P^T are rebuilt before a whole sweep when both X and P are stale), and
-// three word ranges zeroed (the claim keys, the blocked flags, the byte flags
+// three word ranges zeroed (the claim keys, the taken words, the byte flags
 // of the incremental update; a count of 0 skips one).  Volatile reads as in
 // k_step_end.
 __global__ void k_step_prologue(RefineState *st, int pgate, uint32_t *__restrict__ f0, long long n0,
@@ -158,6 +163,8 @@ __global__ void k_step_prologue(RefineState *st, int pgate, uint32_t *__restrict
     if (i == 0) {
         const volatile RefineState *vs = st;
         const int rebuild = pgate && vs->xstale && vs->pstale ? 1 : 0;
+        st->keep_keys = vs->step > 0 && vs->applied == 0 ? 1 : 0;
+        for (int r = 0; r < 32; r++) st->live[r] = 0;
         st->gain = 0;
         st->applied = 0;
         st->nwin = 0;
@@ -241,71 +248,6 @@ __global__ void k_cand_keys(unsigned long long *__restrict__ ckey, long long M, 
     ckey[i] = ok ? ((unsigned long long)(unsigned)delta << 32) | (0xFFFFFFFFu - (unsigned)i) : 0ull;
 }
 
-__global__ void k_claim(unsigned long long *__restrict__ ckey, long long M, CandView cv,
-                        unsigned long long *__restrict__ vmax, const uint32_t *__restrict__ blocked) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const unsigned long long key = ckey[i];
-    if (!key) return;
-    int v[8], delta, kind;
-    const int k = cand_get(cv, i, v, delta, kind);
-    for (int q = 0; q < k; q++)
-        if (blocked[v[q]]) {
-            ckey[i] = 0;  // touches the closed neighbourhood of an earlier winner
-            return;
-        }
-    for (int q = 0; q < k; q++) atomicMax(&vmax[v[q]], key);
-}
-
-// one wave per candidate: wins when no vertex of N[V] carries a larger claim.
-// The candidate's own vertices are checked first (one lane each), then their
-// G1 neighbourhoods with four list entries in flight per lane; the wave stops
-// at the first larger claim it sees.
-__global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__ ckey, long long M, CandView cv,
-                                                const unsigned long long *__restrict__ vmax,
-                                                uint32_t *__restrict__ blocked, int32_t *__restrict__ win,
-                                                RefineState *__restrict__ st, const int32_t *__restrict__ cptr,
-                                                const int32_t *__restrict__ cr, int N1) {
-    const long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (i >= M) return;
-    const unsigned long long key = ckey[i];
-    if (!key) return;
-    int v[8], delta, kind;
-    const int k = cand_get(cv, i, v, delta, kind);
-    int mine = -1;
-#pragma unroll
-    for (int q = 0; q < 8; q++)
-        if (q == lane && q < k) mine = v[q];
-    if (__ballot(mine >= 0 && vmax[mine] > key)) return;
-    for (int q = 0; q < k; q++) {
-        const int x = v[q], e1 = cptr[x + 1];
-        for (int eb = cptr[x]; eb < e1; eb += 256) {  // wave-uniform trip count: the ballot sees every lane
-            const int e = eb + lane;
-            int r[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) r[j] = e + 64 * j < e1 ? cr[e + 64 * j] : -1;
-            bool lose = false;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (r[j] >= 0) lose |= vmax[r[j] >= N1 ? r[j] - N1 : r[j]] > key;
-            if (__ballot(lose)) return;
-        }
-    }
-    for (int q = 0; q < k; q++) {
-        const int x = v[q];
-        if (lane == 0) blocked[x] = 1;
-        for (int e = cptr[x] + lane; e < cptr[x + 1]; e += 64) {
-            const int r = cr[e];
-            blocked[r >= N1 ? r - N1 : r] = 1;
-        }
-    }
-    if (lane == 0) {
-        ckey[i] = 0;
-        win[atomicAdd(&st->nwin, 1u)] = (int32_t)i;
-    }
-}
-
 __device__ __forceinline__ void unrank(int k, int rank, int *perm) {
     int pool[8];
     for (int q = 0; q < k; q++) pool[q] = q;
@@ -321,7 +263,132 @@ __device__ __forceinline__ void unrank(int k, int rank, int *perm) {
     }
 }
 
-// thread per winner: new p(v[q]) = old p(v[src[q]]); the sets are disjoint
+// the move of candidate i: new p(v[q]) = old p(v[src[q]])
+__device__ __forceinline__ void move_src(const CandView &cv, long long i, int kind, int k, int *src) {
+    if (kind == 0) {
+        src[0] = 1;
+        src[1] = 0;
+    } else if (kind == 1) {
+        const int p = cv.tri_p[i - cv.N1];
+        for (int q = 0; q < 3; q++) src[q] = kTab3[p][q];
+    } else {
+        unrank(k, cv.set_r[i - cv.N1 - cv.T], src);
+    }
+}
+
+__global__ void k_claim(unsigned long long *__restrict__ ckey, long long M, CandView cv,
+                        unsigned long long *__restrict__ vmax, const int2 *__restrict__ taken,
+                        RefineState *__restrict__ st, int round) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M || (round > 0 && !st->live[round - 1])) return;
+    const unsigned long long key = ckey[i];
+    if (!key) return;
+    int v[8], delta, kind;
+    const int k = cand_get(cv, i, v, delta, kind);
+    for (int q = 0; q < k; q++)
+        if (taken[v[q]].x) {
+            ckey[i] = 0;  // shares a vertex with an earlier winner
+            return;
+        }
+    for (int q = 0; q < k; q++) atomicMax(&vmax[v[q]], key);
+    st->live[round] = 1;
+}
+
+// one wave per candidate (grid-stride): its claim holds when no vertex of
+// N[V] carries a larger claim.  The candidate's own vertices are checked
+// first (one lane each), then their G1 lists in one pass with kDecideUnroll
+// entries in flight per lane; the wave stops at the first larger claim it
+// sees.  The same pass scores the candidate given the winners of the earlier
+// rounds of the step: its delta assumed every other vertex at its image
+// before the step, so each G1 line from V to a vertex y an earlier winner
+// moved (taken[y] = (y's new image + 1, its old image); never a vertex of V,
+// whose candidate k_claim retired) changes it by
+//   t(p'x, p'y) - t(px, p'y) - t(p'x, py) + t(px, py),
+// t(a, b) = min(w, W2[a][b]) for a line x -> y, min(w, W2[b][a]) for y -> x.
+// It wins when that exact change is positive: its vertices are taken with
+// their new images, and (candidate, change, round) join the winner list
+// (win, win + max_win, win + 2 max_win).  Two winners of one round are
+// never equal or G1-adjacent (the smaller key would have seen the larger
+// claim), so the winners of a round add up exactly on top of the earlier
+// rounds, and the step's gain is the sum of the changes.
+constexpr int kDecideUnroll = 8;  // list entries in flight per lane in k_decide's scan
+
+__global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__ ckey, long long M, CandView cv,
+                                                const unsigned long long *__restrict__ vmax,
+                                                int2 *__restrict__ taken, int32_t *__restrict__ win, int max_win,
+                                                int round, RefineState *__restrict__ st,
+                                                const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
+                                                const int32_t *__restrict__ map, const int16_t *__restrict__ W2,
+                                                int ld, int N1) {
+    if (!st->live[round]) return;
+    const int lane = threadIdx.x & 63;
+    constexpr int kSpan = 64 * kDecideUnroll;
+    for (long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); i < M; i += (long long)gridDim.x * 4) {
+        const unsigned long long key = ckey[i];
+        if (!key) continue;
+        int v[8], delta, kind;
+        const int k = cand_get(cv, i, v, delta, kind);
+        int mine = -1;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q == lane && q < k) mine = v[q];
+        if (__ballot(mine >= 0 && vmax[mine] > key)) continue;
+        int src[8];
+        move_src(cv, i, kind, k, src);
+        // one pass over N(V): the claims, and the lines to vertices moved in earlier rounds
+        int corr = 0;
+        bool lost = false;
+        for (int q = 0; q < k && !lost; q++) {
+            const int x = v[q], px = map[x], pn = map[v[src[q]]], e1 = cptr[x + 1];
+            for (int eb = cptr[x]; eb < e1; eb += kSpan) {  // wave-uniform trip count: the ballot sees every lane
+                const int e = eb + lane;
+                int2 rw[kDecideUnroll];
+#pragma unroll
+                for (int j = 0; j < kDecideUnroll; j++) rw[j] = e + 64 * j < e1 ? crw[e + 64 * j] : make_int2(-1, 0);
+                bool lose = false;
+                int2 t[kDecideUnroll];
+#pragma unroll
+                for (int j = 0; j < kDecideUnroll; j++) {
+                    t[j] = make_int2(0, 0);
+                    if (rw[j].x < 0) continue;
+                    const int y = rw[j].x >= N1 ? rw[j].x - N1 : rw[j].x;
+                    lose |= vmax[y] > key;
+                    t[j] = taken[y];
+                }
+                if (__ballot(lose)) {
+                    lost = true;
+                    break;
+                }
+#pragma unroll
+                for (int j = 0; j < kDecideUnroll; j++) {
+                    if (!t[j].x) continue;
+                    const bool out = rw[j].x >= N1;
+                    const int yn = t[j].x - 1, py = t[j].y, w = rw[j].y;
+                    auto tw = [&](int a, int b) {
+                        return min(w, (int)(out ? W2[(size_t)a * ld + b] : W2[(size_t)b * ld + a]));
+                    };
+                    corr += tw(pn, yn) - tw(px, yn) - tw(pn, py) + tw(px, py);
+                }
+            }
+        }
+        if (lost) continue;
+        const int change = delta + bm_wave::wave_sum(corr);
+        if (lane == 0) {
+            ckey[i] = 0;
+            if (change > 0) {
+                for (int q = 0; q < k; q++) taken[v[q]] = make_int2(map[v[src[q]]] + 1, map[v[q]]);
+                const unsigned s = atomicAdd(&st->nwin, 1u);
+                win[s] = (int32_t)i;
+                win[max_win + s] = change;
+                win[2 * max_win + s] = round;
+            }
+        }
+    }
+}
+
+
+// thread per winner: new p(v[q]) = old p(v[src[q]]); the sets are disjoint.
+// The step's gain adds each winner's exact change (not its candidate delta).
 // With ulists set (the incremental update of X), each vertex whose image
 // changes is flagged in moved[] and gets its two G2 lists in ulists: entry
 // 2x the out-list, 2x + 1 the in-list (its words at ng2 + g2iptr in the
@@ -335,17 +402,10 @@ __global__ void k_apply(const int32_t *__restrict__ win, int max_win, CandView c
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= max_win || s >= (int)st->nwin) return;
     const long long i = win[s];
-    int v[8], src[8], old[8], delta = 0, kind = 0;
-    const int k = cand_get(cv, i, v, delta, kind);
-    if (kind == 0) {
-        src[0] = 1;
-        src[1] = 0;
-    } else if (kind == 1) {
-        const int p = cv.tri_p[i - cv.N1];
-        for (int q = 0; q < 3; q++) src[q] = kTab3[p][q];
-    } else {
-        unrank(k, cv.set_r[i - cv.N1 - cv.T], src);
-    }
+    int v[8], src[8], old[8], base = 0, kind = 0;
+    const int k = cand_get(cv, i, v, base, kind);
+    const int delta = win[max_win + s];  // the exact change given the earlier rounds' winners
+    move_src(cv, i, kind, k, src);
     for (int q = 0; q < k; q++) old[q] = map[v[src[q]]];
     if (ulists) {
         for (int q = 0; q < k; q++) {
@@ -373,6 +433,8 @@ __global__ void k_apply(const int32_t *__restrict__ win, int max_win, CandView c
             r.kind = kind;
             r.k = k;
             r.delta = delta;
+            r.round = win[2 * max_win + s];
+            r.base = base;
             for (int q = 0; q < 8; q++) {
                 r.v[q] = q < k ? v[q] : 0;
                 r.img[q] = q < k ? old[q] : 0;
@@ -1178,7 +1240,7 @@ struct bm_batch_state {
     int32_t *sets = nullptr, *set_d = nullptr, *set_r = nullptr;
     int *ovf = nullptr;
     unsigned long long *ckey = nullptr, *vmax = nullptr;
-    uint32_t *blocked = nullptr;
+    int2 *taken = nullptr;  // per vertex: 0, or (new image + 1, old image) once a winner of this step moves it
     int32_t *win = nullptr;
     int32_t *ulist = nullptr, *uold = nullptr, *ubase = nullptr, *cbase = nullptr;
     int32_t *xscratch = nullptr;  // chunk rows of the column recompute, xcap x (n+1)
@@ -1253,7 +1315,7 @@ int bm::free_batch(bm_ctx *c) {
         if (p) (void)hipFree(p);
     };
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
-    fr(b->ckey), fr(b->vmax), fr(b->blocked), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
+    fr(b->ckey), fr(b->vmax), fr(b->taken), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
     fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU), fr(b->uoldv);
     fr(b->ulists), fr(b->hrows), fr(b->drows);
     if (b->hst) (void)hipHostFree(b->hst);
@@ -1301,15 +1363,16 @@ static int enqueue_step(bm_ctx *c) {
         const long long vm_words = (long long)p.rounds * N1 * 2;
         // inU, the row marks, the moved flags, the row counts
         const long long mark_words = inc ? (long long)(N1 + 3) / 4 * 3 + 2LL * N1 : 0;
-        const long long most = std::max(std::max(vm_words, (long long)N1), mark_words);
+        const long long most = std::max(std::max(vm_words, 2LL * N1), mark_words);
         BM_LAUNCH(c, "refine_begin",
                   hipLaunchKernelGGL(k_step_prologue, dim3((unsigned)((most + 255) / 256)), dim3(256), 0, c->stream,
-                                     b->st, pfree ? 1 : 0, (uint32_t *)b->vmax, vm_words, b->blocked, (long long)N1,
+                                     b->st, pfree ? 1 : 0, (uint32_t *)b->vmax, vm_words, (uint32_t *)b->taken, 2LL * N1,
                                      (uint32_t *)b->inU, mark_words));
     }
     if (pfree && (rc = launch_rebuild_p_if(c, &b->st->prebuild))) return rc;
     if (p.pairs) {
-        if ((rc = sweep_enqueue(c, nullptr, inc ? &b->st->xstale : nullptr))) return rc;
+        // the sweep's fill and combine skipped after a step that moved nothing (same mapping, same keys)
+        if ((rc = sweep_enqueue(c, nullptr, inc ? &b->st->xstale : nullptr, &b->st->keep_keys))) return rc;
     }
     if (p.triples > 0) {
         BM_LAUNCH(c, "gen_triples",
@@ -1333,10 +1396,12 @@ static int enqueue_step(bm_ctx *c) {
     for (int r = 0; r < p.rounds; r++) {
         unsigned long long *vm = b->vmax + (size_t)r * N1;
         BM_LAUNCH(c, "claim", hipLaunchKernelGGL(k_claim, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, vm,
-                                                 b->blocked));
+                                                 b->taken, b->st, r));
         BM_LAUNCH(c, "decide",
-                  hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->M + 3) / 4)), dim3(256), 0, c->stream, b->ckey,
-                                     b->M, cv, vm, b->blocked, b->win, b->st, c->cptr, c->cr, N1));
+                  hipLaunchKernelGGL(k_decide, dim3((unsigned)std::min<long long>((b->M + 3) / 4, 1024)), dim3(256),
+                                     0, c->stream, b->ckey,
+                                     b->M, cv, vm, b->taken, b->win, N1 / 2 + 1, r, b->st, c->cptr,
+                                     (const int2 *)c->crw, c->map, c->W2, c->ld, N1));
     }
     const unsigned ublocks = 1024;  // grid-stride kernels of the incremental update
     if (inc) {
@@ -1488,7 +1553,7 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
         return set_error(BM_EINVAL, "triples %d / sets %d outside [0, 2^24] / [0, 2^20]", p.triples, p.sets);
     if (p.sets > 0 && (p.set_k < 2 || p.set_k > 8)) return set_error(BM_EINVAL, "set_k=%d outside [2,8]", p.set_k);
     if (p.set_gen != 0 && p.set_gen != 1) return set_error(BM_EINVAL, "set_gen=%d not 0 or 1", p.set_gen);
-    if (p.rounds < 1 || p.rounds > 8) return set_error(BM_EINVAL, "rounds=%d outside [1,8]", p.rounds);
+    if (p.rounds < 1 || p.rounds > 32) return set_error(BM_EINVAL, "rounds=%d outside [1,32]", p.rounds);
     if (p.steps_per_check < 1) return set_error(BM_EINVAL, "steps_per_check=%d < 1", p.steps_per_check);
     if (!p.pairs && !p.triples && !p.sets) return set_error(BM_EINVAL, "no move kind enabled");
     if (c->n < 3) return set_error(BM_ERANGE, "n=%d too small for batched refinement", c->n);
@@ -1515,8 +1580,8 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->ovf, 64))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->ckey, (size_t)b->M * 8))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->vmax, (size_t)p.rounds * N1 * 8))) return fail(rc);
-    if ((rc = balloc(c, (void **)&b->blocked, (size_t)N1 * 4))) return fail(rc);
-    if ((rc = balloc(c, (void **)&b->win, (size_t)(N1 / 2 + 1) * 4))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->taken, (size_t)N1 * 8))) return fail(rc);
+    if ((rc = balloc(c, (void **)&b->win, (size_t)(N1 / 2 + 1) * 12))) return fail(rc);  // index, change, round
     if ((rc = balloc(c, (void **)&b->ulist, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->uold, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->ubase, (size_t)(N1 + 1) * 4))) return fail(rc);

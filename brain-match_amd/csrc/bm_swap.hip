@@ -1200,7 +1200,8 @@ __global__ __launch_bounds__(256) void k_combine(const int32_t *__restrict__ X, 
                                                  const int32_t *__restrict__ diag, int n,
                                                  unsigned long long *__restrict__ rowkey,
                                                  unsigned long long *__restrict__ gkey,
-                                                 int32_t *__restrict__ dmat) {
+                                                 int32_t *__restrict__ dmat, const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int I = blockIdx.x;
     const int T = (int)(ldx >> 6), Tn = (n + 64) / 64;
     const int J0 = I + blockIdx.y * kCombChunk, J1 = min(Tn, J0 + kCombChunk);
@@ -1546,7 +1547,8 @@ extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32
     return BM_OK;
 }
 
-__global__ void k_fill_keys(unsigned long long *__restrict__ p, int64_t count) {
+__global__ void k_fill_keys(unsigned long long *__restrict__ p, int64_t count, const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = 0ull;
 }
@@ -1570,11 +1572,11 @@ int bm::sweep_prepare(bm_ctx *c) {
 // allocations are made by sweep_prepare).
 bool bm::sweep_is_sparse(const bm_ctx *c) { return c->sweep_kind == 2 && sparse_fits(c); }
 
-int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag) {
+int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag, const int *skip_flag) {
     const int N1 = c->N1;
     int rc;
     BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill_keys, dim3((N1 + 256) / 256), dim3(256), 0, c->stream, c->keys,
-                                            (int64_t)N1 + 1));
+                                            (int64_t)N1 + 1, skip_flag));
     const bool sparse = c->sweep_kind == 2 && c->sparse_ok && (size_t)N1 * 4 <= kSparseLdsMax;
     if (sparse) {
         if ((rc = launch_gain_sparse(c, run_flag))) return rc;  // also writes X(a,a) and folds Corr in
@@ -1589,7 +1591,8 @@ int bm::sweep_enqueue(bm_ctx *c, int32_t *dmat, const int *run_flag) {
     c->diag_valid = true;
     const int T = (N1 + 63) / 64;
     BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, (T + kCombChunk - 1) / kCombChunk), dim3(256), 0,
-                                               c->stream, c->X, c->ldx, c->diag, c->n, c->keys, c->keys + N1, dmat));
+                                               c->stream, c->X, c->ldx, c->diag, c->n, c->keys, c->keys + N1, dmat,
+                                               skip_flag));
     return BM_OK;
 }
 
@@ -1606,7 +1609,7 @@ static int sweep_dense_enqueue(bm_ctx *c, int32_t *dmat) {
     if (rc) return rc;
     if (!c->dfX && (rc = dev_alloc(c, (void **)&c->dfX, (size_t)(N1 + 63) / 64 * 64 * c->ld * 4))) return rc;
     BM_LAUNCH(c, "fill", hipLaunchKernelGGL(k_fill_keys, dim3((N1 + 256) / 256), dim3(256), 0, c->stream, c->keys,
-                                            (int64_t)N1 + 1));
+                                            (int64_t)N1 + 1, nullptr));
     if ((rc = launch_list_diag(c, c->dfdiag, c->dfptr, c->dfr, c->dfw))) return rc;
     c->dfdiag_valid = true;
     if ((rc = launch_list_gain(c, c->dfptr, c->dfrw, c->dfX, c->dfmaxw))) return rc;
@@ -1617,7 +1620,7 @@ static int sweep_dense_enqueue(bm_ctx *c, int32_t *dmat) {
     const int T = (N1 + 63) / 64;
     BM_LAUNCH(c, "combine", hipLaunchKernelGGL(k_combine, dim3(T, (T + kCombChunk - 1) / kCombChunk), dim3(256), 0,
                                                c->stream, c->dfX, (int64_t)c->ld, c->dfdiag, c->n, c->keys,
-                                               c->keys + N1, dmat));
+                                               c->keys + N1, dmat, nullptr));
     return BM_OK;
 }
 

@@ -239,12 +239,14 @@ int bm_moves_results(bm_ctx *ctx, int32_t *delta, int32_t *aux);
  * of the all-pairs Delta_B sweep, `triples` 3-vertex moves drawn from 2-hop
  * G1 neighbourhoods and scored with Delta_G3 over p = 1..5, `sets` k-sets
  * grown along G1 lines and scored over all k! permutations with intra-set
- * lines counted once); a conflict-free subset of the improving ones is
- * selected (largest delta first, lowest candidate index on ties; no two
- * selected moves share a vertex or a G1 line, so their deltas add up
- * exactly); they are applied, P / P^T are rebuilt, and S_opt is recomputed
- * from scratch and checked to equal the previous score plus the applied
- * deltas.  Only vertices without self-loops or repeated lines move, where
+ * lines counted once); moves are selected in `rounds` rounds (largest
+ * delta first, lowest candidate index on ties): a round's winners share no
+ * vertex and no G1 line with each other and no vertex with earlier winners,
+ * and each is scored exactly given the earlier rounds' winners (its lines to
+ * the vertices they moved re-scored) and kept only if that change is
+ * positive, so the changes add up exactly; they are applied, the step's
+ * state is brought up to date, and S_opt is checked to equal the previous
+ * score plus the applied changes.  Only vertices without self-loops or repeated lines move, where
  * every delta kind is the exact score change.  Results are deterministic for
  * a given instance, mapping and parameters.
  *
@@ -258,7 +260,7 @@ typedef struct bm_refine_params {
     int32_t sets;            /* k-set candidates generated per step, 0..2^20 */
     int32_t set_k;           /* k of the sets, 2..8 */
     int32_t set_gen;         /* 0: sets grown along G1 lines from a random vertex; 1: uniform random sets */
-    int32_t rounds;          /* selection rounds per step, 1..8 */
+    int32_t rounds;          /* selection rounds per step, 1..32 */
     int32_t steps_per_check; /* steps enqueued between host checks of the stop rules */
     uint64_t seed;           /* candidate generator seed */
 } bm_refine_params;
@@ -302,8 +304,10 @@ int bm_refine_run_many(bm_ctx *const *ctxs, int32_t count, int64_t max_steps, in
 /* Per-step records of the last run, steps [first, first+count) (the last 4096 are kept). */
 int bm_refine_history(bm_ctx *ctx, int64_t first, int64_t count, bm_refine_step *out);
 /* Keep a log of up to `capacity` applied moves per run (0: off).  Each record
- * is 20 int32: step, kind (0 pair, 1 triple, 2 set), k, delta, vertices[8],
- * their new images[8].  bm_refine_moves copies min(count, capacity) records. */
+ * is 22 int32: step, kind (0 pair, 1 triple, 2 set), k, delta (the exact
+ * change given the earlier rounds' winners of the step), round, base (the
+ * candidate's delta at the step's starting mapping), vertices[8], their new
+ * images[8].  bm_refine_moves copies min(count, capacity) records. */
 int bm_refine_log(bm_ctx *ctx, int32_t capacity);
 int bm_refine_moves(bm_ctx *ctx, int64_t *count, int32_t *records, int64_t capacity);
 /* Autotuned variants in use: lanes per triple (8/16/32/64), threads per set (64/256). */

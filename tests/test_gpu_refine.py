@@ -1,9 +1,11 @@
 """Batched refinement on the GPU: every applied step is checked on the CPU.
 
-For each step of a run the log of applied moves is replayed on the oracle:
-the moves of one step share no vertex and no G1 line, each move's logged
-delta equals the oracle's exact score change when it is applied alone to the
-step's starting mapping (and, for swaps, Delta_B of
+For each step of a run the log of applied moves is replayed on the oracle,
+round by round: the moves of one round share no vertex and no G1 line, no
+move shares a vertex with a move of an earlier round of the step, each
+move's logged delta equals the oracle's exact score change when it is
+applied to the mapping the earlier rounds left (its logged base delta is the
+change at the step's starting mapping: for swaps Delta_B of
 graph-alignment-optimized.c:321-408 and the arg-max of its row over b > a),
 and the step's moves together raise S_opt by exactly their sum, which is the
 score the device recorded for that step.
@@ -41,7 +43,8 @@ def simple_vertices(n, g1):
 
 
 def sorted_log(recs):
-    order = np.lexsort((recs[:, 4], recs[:, 0]))
+    """Log records (step, kind, k, delta, round, base, vertices[8], images[8]) by step, round, first vertex."""
+    order = np.lexsort((recs[:, 6], recs[:, 4], recs[:, 0]))
     return recs[order]
 
 
@@ -55,37 +58,47 @@ def replay(n, g1, o, mp0, recs, stats, hist, check_rows=True):
     kinds = [0, 0, 0]
     for step in range(stats["steps"]):
         ms = recs[recs[:, 0] == step]
-        used = set()
-        nxt = cur.copy()
-        gain = 0
-        for r in ms:
-            kind, k, delta = int(r[1]), int(r[2]), int(r[3])
-            vs = [int(x) for x in r[4:4 + k]]
-            img = [int(x) for x in r[12:12 + k]]
-            kinds[kind] += 1
-            assert delta > 0
-            assert all(1 <= v <= n and simple[v] for v in vs)
-            assert sorted(img) == sorted(int(cur[v]) for v in vs)
-            closed = set().union(*(nb[v] for v in vs))
-            assert not (closed & used), f"step {step}: move {vs} conflicts with an earlier one"
-            used.update(vs)
-            alone = cur.copy()
-            alone[vs] = img
-            assert o.score_opt(alone) - score == delta, (step, kind, vs)
-            if kind == 0:
-                a, b = vs
-                assert a < b and o.delta_b(cur, a, b) == delta
-                if check_rows:
-                    row = [o.delta_b(cur, a, x) for x in range(a + 1, n + 1)]
-                    assert max(row) == delta and row.index(delta) == b - a - 1
-            nxt[vs] = img
-            gain += delta
-        new = o.score_opt(nxt)
-        assert new == score + gain
+        taken = set()
+        nxt, now, gain = cur.copy(), score, 0
+        for rnd in sorted(set(ms[:, 4].tolist())):
+            before, before_score = nxt.copy(), now
+            in_round = set()
+            for r in ms[ms[:, 4] == rnd]:
+                kind, k, delta, base = int(r[1]), int(r[2]), int(r[3]), int(r[5])
+                vs = [int(x) for x in r[6:6 + k]]
+                img = [int(x) for x in r[14:14 + k]]
+                kinds[kind] += 1
+                assert delta > 0
+                assert all(1 <= v <= n and simple[v] for v in vs)
+                assert not (set(vs) & taken), f"step {step}: move {vs} reuses a vertex of an earlier round"
+                assert sorted(img) == sorted(int(cur[v]) for v in vs)
+                closed = set().union(*(nb[v] for v in vs))
+                assert not (closed & in_round), f"step {step} round {rnd}: move {vs} conflicts with an earlier one"
+                in_round.update(vs)
+                # the exact change on top of the earlier rounds, and the candidate's delta at the step's start
+                moved = before.copy()
+                moved[vs] = img
+                assert o.score_opt(moved) - before_score == delta, (step, rnd, kind, vs)
+                alone = cur.copy()
+                alone[vs] = img
+                assert o.score_opt(alone) - score == base, (step, rnd, kind, vs)
+                if not taken:
+                    assert delta == base
+                if kind == 0:
+                    a, b = vs
+                    assert a < b and o.delta_b(cur, a, b) == base
+                    if check_rows:
+                        row = [o.delta_b(cur, a, x) for x in range(a + 1, n + 1)]
+                        assert max(row) == base and row.index(base) == b - a - 1
+                nxt[vs] = img
+                gain += delta
+            taken |= in_round
+            now = o.score_opt(nxt)
+            assert now == before_score + sum(int(r[3]) for r in ms[ms[:, 4] == rnd])
         h = hist[step]
-        assert h[:4] == (new, gain, len(ms), 1), (step, h)
+        assert h[:4] == (now, gain, len(ms), 1), (step, h)
         assert h[4] in (0, sum(int(r[2]) for r in ms))  # moved vertices (counted with the incremental update)
-        cur, score = nxt, new
+        cur, score = nxt, now
     assert score == stats["final_score"]
     assert kinds == [stats["pair_moves"], stats["triple_moves"], stats["set_moves"]]
     return cur
@@ -118,6 +131,30 @@ def test_pairs_only_converges_exactly():
         assert e.score() == st["final_score"] == o.score_opt(fin)
         # converged: no improving swap is left (the generator makes every vertex simple)
         assert e.sweep_all_pairs()["best"][2] <= 0
+
+
+def test_later_rounds_score_on_top_of_earlier_winners():
+    # power-law pair: the first rounds' winners hold hubs whose neighbourhoods cover most vertices, so
+    # later rounds take moves next to them, each scored given what the earlier rounds moved
+    inst, o = instance(41, 300, 6000)
+    prm = bmatch.refine_params(pairs=1, triples=1000, sets=100, set_k=4, rounds=12, steps_per_check=2, seed=3)
+    with bmatch.Engine(0) as e:
+        e.upload(inst.n, inst.g1, inst.g2)
+        st, recs, hist = run(e, inst.p0, prm, max_steps=6, patience=6)
+        assert st["verified"] == 1 and st["moves"] > 0
+        fin = replay(inst.n, inst.g1, o, inst.p0, recs, st, hist)
+        np.testing.assert_array_equal(e.mapping(), fin)
+        nb = closed_nbrs(inst.n, inst.g1)
+        # some move of a later round touches a vertex an earlier round of its step moved
+        adjacent = 0
+        for step in range(st["steps"]):
+            ms = recs[recs[:, 0] == step]
+            for r in ms:
+                earlier = ms[ms[:, 4] < r[4]]
+                near = set().union(*(nb[int(v)] for v in r[6:6 + int(r[2])]))
+                adjacent += any(int(v) in near for q in earlier for v in q[6:6 + int(q[2])])
+        assert adjacent > 0
+        assert (recs[:, 3] != recs[:, 5]).any()
 
 
 @pytest.mark.parametrize("k", [3, 4, 8])
