@@ -227,15 +227,18 @@ __global__ void k_gen_sets(int32_t *__restrict__ sets, int S, int k, int uniform
     for (int q = 0; q < k; q++) sets[(long long)k * i + q] = s[q];
 }
 
-// With clear (the incremental update), thread 0 also records whether the
-// step's sweep ran whole (X was stale) and marks X current.
+// Each live candidate also makes its first-round claim: the atomicMax of its
+// key on each of its vertices (vmax0).  With inc (the incremental update),
+// thread 0 also records whether the step's sweep ran whole (X was stale) and
+// marks X current.
 __global__ void k_cand_keys(unsigned long long *__restrict__ ckey, long long M, CandView cv,
-                            const uint8_t *__restrict__ vsimple, RefineState *__restrict__ clear) {
+                            const uint8_t *__restrict__ vsimple, RefineState *__restrict__ st, int inc,
+                            unsigned long long *__restrict__ vmax0) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (clear && i == 0) {
-        const int stale = ((const volatile RefineState *)clear)->xstale;  // as in k_step_end
-        clear->full = stale;  // the gain kernels ran whole iff X was stale
-        clear->xstale = 0;
+    if (inc && i == 0) {
+        const int stale = ((const volatile RefineState *)st)->xstale;  // as in k_step_end
+        st->full = stale;  // the gain kernels ran whole iff X was stale
+        st->xstale = 0;
     }
     if (i >= M) return;
     int v[8], delta = 0, kind = 0;
@@ -245,7 +248,11 @@ __global__ void k_cand_keys(unsigned long long *__restrict__ ckey, long long M, 
         ok = v[q] >= 1 && v[q] <= cv.n && vsimple[v[q]];
         for (int r = 0; r < q && ok; r++) ok = v[r] != v[q];
     }
-    ckey[i] = ok ? ((unsigned long long)(unsigned)delta << 32) | (0xFFFFFFFFu - (unsigned)i) : 0ull;
+    const unsigned long long key = ok ? ((unsigned long long)(unsigned)delta << 32) | (0xFFFFFFFFu - (unsigned)i) : 0ull;
+    ckey[i] = key;
+    if (!ok) return;
+    for (int q = 0; q < k; q++) atomicMax(&vmax0[v[q]], key);
+    st->live[0] = 1;
 }
 
 __device__ __forceinline__ void unrank(int k, int rank, int *perm) {
@@ -276,33 +283,22 @@ __device__ __forceinline__ void move_src(const CandView &cv, long long i, int ki
     }
 }
 
-__global__ void k_claim(unsigned long long *__restrict__ ckey, long long M, CandView cv,
-                        unsigned long long *__restrict__ vmax, const int2 *__restrict__ taken,
-                        RefineState *__restrict__ st, int round) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M || (round > 0 && !st->live[round - 1])) return;
-    const unsigned long long key = ckey[i];
-    if (!key) return;
-    int v[8], delta, kind;
-    const int k = cand_get(cv, i, v, delta, kind);
-    for (int q = 0; q < k; q++)
-        if (taken[v[q]].x) {
-            ckey[i] = 0;  // shares a vertex with an earlier winner
-            return;
-        }
-    for (int q = 0; q < k; q++) atomicMax(&vmax[v[q]], key);
-    st->live[round] = 1;
-}
-
-// one wave per candidate (grid-stride): its claim holds when no vertex of
-// N[V] carries a larger claim.  The candidate's own vertices are checked
-// first (one lane each), then their G1 lists in one pass with kDecideUnroll
-// entries in flight per lane; the wave stops at the first larger claim it
-// sees.  The same pass scores the candidate given the winners of the earlier
-// rounds of the step: its delta assumed every other vertex at its image
-// before the step, so each G1 line from V to a vertex y an earlier winner
-// moved (taken[y] = (y's new image + 1, its old image); never a vertex of V,
-// whose candidate k_claim retired) changes it by
+// Selection round `round`, one wave per live candidate (grid-stride); every
+// live candidate has claimed its vertices in vm (the round's claims, made at
+// the candidate keys for the first round and by the losers of the previous
+// round for the others).  A candidate sharing a vertex with an earlier
+// winner (taken) retires.  Its claim holds when no vertex of N[V] carries a
+// larger claim: the own vertices are checked first (one lane each), then
+// their G1 lists in one pass with kDecideUnroll entries in flight per lane;
+// the wave stops at the first larger claim it sees, and the loser claims its
+// vertices for the next round in vmn.  (A claim made there by a candidate
+// that the next round retires may make a neighbour lose that round too:
+// losing is always safe, the neighbour claims again.)  The same pass scores
+// the candidate given the winners of the earlier rounds of the step: its
+// delta assumed every other vertex at its image before the step, so each G1
+// line from V to a vertex y an earlier winner moved (taken[y] = (y's new
+// image + 1, its old image | the round << 20); never a vertex of V) changes
+// it by
 //   t(p'x, p'y) - t(px, p'y) - t(p'x, py) + t(px, py),
 // t(a, b) = min(w, W2[a][b]) for a line x -> y, min(w, W2[b][a]) for y -> x.
 // It wins when that exact change is positive: its vertices are taken with
@@ -315,7 +311,7 @@ constexpr int kDecideUnroll = 8;  // list entries in flight per lane in k_decide
 
 __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__ ckey, long long M, CandView cv,
                                                 const unsigned long long *__restrict__ vmax,
-                                                int2 *__restrict__ taken, int32_t *__restrict__ win, int max_win,
+                                                unsigned long long *__restrict__ vmn, int2 *__restrict__ taken, int32_t *__restrict__ win, int max_win,
                                                 int round, RefineState *__restrict__ st,
                                                 const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
                                                 const int32_t *__restrict__ map, const int16_t *__restrict__ W2,
@@ -332,7 +328,20 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
 #pragma unroll
         for (int q = 0; q < 8; q++)
             if (q == lane && q < k) mine = v[q];
-        if (__ballot(mine >= 0 && vmax[mine] > key)) continue;
+        // shares a vertex with a winner of an earlier round (this round's winners are left to the claims,
+        // so that what a wave sees does not depend on timing)
+        if (__ballot(mine >= 0 && taken[mine].x && (taken[mine].y >> 20) < round)) {
+            if (lane == 0) ckey[i] = 0;
+            continue;
+        }
+        auto claim_next = [&]() {
+            if (vmn && mine >= 0) atomicMax(&vmn[mine], key);
+            if (vmn && lane == 0) st->live[round + 1] = 1;
+        };
+        if (__ballot(mine >= 0 && vmax[mine] > key)) {
+            claim_next();
+            continue;
+        }
         int src[8];
         move_src(cv, i, kind, k, src);
         // one pass over N(V): the claims, and the lines to vertices moved in earlier rounds
@@ -354,6 +363,7 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
                     const int y = rw[j].x >= N1 ? rw[j].x - N1 : rw[j].x;
                     lose |= vmax[y] > key;
                     t[j] = taken[y];
+                    if ((t[j].y >> 20) >= round) t[j].x = 0;  // only earlier rounds
                 }
                 if (__ballot(lose)) {
                     lost = true;
@@ -363,7 +373,7 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
                 for (int j = 0; j < kDecideUnroll; j++) {
                     if (!t[j].x) continue;
                     const bool out = rw[j].x >= N1;
-                    const int yn = t[j].x - 1, py = t[j].y, w = rw[j].y;
+                    const int yn = t[j].x - 1, py = t[j].y & 0xFFFFF, w = rw[j].y;
                     auto tw = [&](int a, int b) {
                         return min(w, (int)(out ? W2[(size_t)a * ld + b] : W2[(size_t)b * ld + a]));
                     };
@@ -371,12 +381,15 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
                 }
             }
         }
-        if (lost) continue;
+        if (lost) {
+            claim_next();
+            continue;
+        }
         const int change = delta + bm_wave::wave_sum(corr);
         if (lane == 0) {
             ckey[i] = 0;
             if (change > 0) {
-                for (int q = 0; q < k; q++) taken[v[q]] = make_int2(map[v[src[q]]] + 1, map[v[q]]);
+                for (int q = 0; q < k; q++) taken[v[q]] = make_int2(map[v[src[q]]] + 1, map[v[q]] | (round << 20));
                 const unsigned s = atomicAdd(&st->nwin, 1u);
                 win[s] = (int32_t)i;
                 win[max_win + s] = change;
@@ -1240,7 +1253,7 @@ struct bm_batch_state {
     int32_t *sets = nullptr, *set_d = nullptr, *set_r = nullptr;
     int *ovf = nullptr;
     unsigned long long *ckey = nullptr, *vmax = nullptr;
-    int2 *taken = nullptr;  // per vertex: 0, or (new image + 1, old image) once a winner of this step moves it
+    int2 *taken = nullptr;  // per vertex: 0, or (new image + 1, old image | round << 20) once a winner of this step moves it
     int32_t *win = nullptr;
     int32_t *ulist = nullptr, *uold = nullptr, *ubase = nullptr, *cbase = nullptr;
     int32_t *xscratch = nullptr;  // chunk rows of the column recompute, xcap x (n+1)
@@ -1392,15 +1405,13 @@ static int enqueue_step(bm_ctx *c) {
     const unsigned mb = (unsigned)((b->M + 255) / 256);
     BM_LAUNCH(c, "cand_keys",
               hipLaunchKernelGGL(k_cand_keys, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, c->vsimple,
-                                 inc ? b->st : nullptr));
+                                 b->st, inc ? 1 : 0, b->vmax));
     for (int r = 0; r < p.rounds; r++) {
         unsigned long long *vm = b->vmax + (size_t)r * N1;
-        BM_LAUNCH(c, "claim", hipLaunchKernelGGL(k_claim, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, vm,
-                                                 b->taken, b->st, r));
+        unsigned long long *vmn = r + 1 < p.rounds ? vm + N1 : nullptr;
         BM_LAUNCH(c, "decide",
                   hipLaunchKernelGGL(k_decide, dim3((unsigned)std::min<long long>((b->M + 3) / 4, 1024)), dim3(256),
-                                     0, c->stream, b->ckey,
-                                     b->M, cv, vm, b->taken, b->win, N1 / 2 + 1, r, b->st, c->cptr,
+                                     0, c->stream, b->ckey, b->M, cv, vm, vmn, b->taken, b->win, N1 / 2 + 1, r, b->st, c->cptr,
                                      (const int2 *)c->crw, c->map, c->W2, c->ld, N1));
     }
     const unsigned ublocks = 1024;  // grid-stride kernels of the incremental update
