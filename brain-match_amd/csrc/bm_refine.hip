@@ -83,7 +83,7 @@ struct RefineState {
     unsigned long long score_change;  // exact S_opt change of this step over the lines at U
     int nhash, ndense;  // the row updates by kind
     int keep_keys;      // the previous step of this run moved nothing: the row-best keys still hold
-    int live[32];       // per selection round: some candidate claimed (later rounds have nothing to do without)
+    int nlive[32];      // per selection round: candidates in its list (each has claimed for the round)
 };
 
 struct MoveRec {
@@ -164,7 +164,7 @@ __global__ void k_step_prologue(RefineState *st, int pgate, uint32_t *__restrict
         const volatile RefineState *vs = st;
         const int rebuild = pgate && vs->xstale && vs->pstale ? 1 : 0;
         st->keep_keys = vs->step > 0 && vs->applied == 0 ? 1 : 0;
-        for (int r = 0; r < 32; r++) st->live[r] = 0;
+        for (int r = 0; r < 32; r++) st->nlive[r] = 0;
         st->gain = 0;
         st->applied = 0;
         st->nwin = 0;
@@ -233,26 +233,31 @@ __global__ void k_gen_sets(int32_t *__restrict__ sets, int S, int k, int uniform
 // marks X current.
 __global__ void k_cand_keys(unsigned long long *__restrict__ ckey, long long M, CandView cv,
                             const uint8_t *__restrict__ vsimple, RefineState *__restrict__ st, int inc,
-                            unsigned long long *__restrict__ vmax0) {
+                            unsigned long long *__restrict__ vmax0, int32_t *__restrict__ list0) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (inc && i == 0) {
         const int stale = ((const volatile RefineState *)st)->xstale;  // as in k_step_end
         st->full = stale;  // the gain kernels ran whole iff X was stale
         st->xstale = 0;
     }
-    if (i >= M) return;
     int v[8], delta = 0, kind = 0;
-    const int k = cand_get(cv, i, v, delta, kind);
+    const int k = i < M ? cand_get(cv, i, v, delta, kind) : 0;
     bool ok = k > 0 && delta > 0;
     for (int q = 0; q < k && ok; q++) {
         ok = v[q] >= 1 && v[q] <= cv.n && vsimple[v[q]];
         for (int r = 0; r < q && ok; r++) ok = v[r] != v[q];
     }
     const unsigned long long key = ok ? ((unsigned long long)(unsigned)delta << 32) | (0xFFFFFFFFu - (unsigned)i) : 0ull;
-    ckey[i] = key;
+    if (i < M) ckey[i] = key;
+    // the first round's list: one atomic per wave
+    const unsigned long long m = __ballot(ok);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(&st->nlive[0], __popcll(m));
+    base = __shfl(base, 0);
     if (!ok) return;
+    list0[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)i;
     for (int q = 0; q < k; q++) atomicMax(&vmax0[v[q]], key);
-    st->live[0] = 1;
 }
 
 __device__ __forceinline__ void unrank(int k, int rank, int *perm) {
@@ -307,21 +312,36 @@ __device__ __forceinline__ void move_src(const CandView &cv, long long i, int ki
 // never equal or G1-adjacent (the smaller key would have seen the larger
 // claim), so the winners of a round add up exactly on top of the earlier
 // rounds, and the step's gain is the sum of the changes.
-constexpr int kDecideUnroll = 8;  // list entries in flight per lane in k_decide's scan
+constexpr int kDecideUnroll = 8;       // list entries in flight per lane in k_decide's scan
+constexpr int kDecideLoserCap = 256;   // losers a k_decide block holds in LDS before it appends them
 
 __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__ ckey, long long M, CandView cv,
                                                 const unsigned long long *__restrict__ vmax,
-                                                unsigned long long *__restrict__ vmn, int2 *__restrict__ taken, int32_t *__restrict__ win, int max_win,
-                                                int round, RefineState *__restrict__ st,
+                                                unsigned long long *__restrict__ vmn, int2 *__restrict__ taken,
+                                                int32_t *__restrict__ win, int max_win, const int32_t *__restrict__ list,
+                                                int32_t *__restrict__ next, int round, RefineState *__restrict__ st,
                                                 const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
                                                 const int32_t *__restrict__ map, const int16_t *__restrict__ W2,
                                                 int ld, int N1) {
-    if (!st->live[round]) return;
+    // the losers of the round are gathered in LDS and appended to the next
+    // round's list with one atomic per block and flush
+    __shared__ int32_t lq[kDecideLoserCap];
+    __shared__ int lqn, lqbase;
     const int lane = threadIdx.x & 63;
     constexpr int kSpan = 64 * kDecideUnroll;
-    for (long long i = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); i < M; i += (long long)gridDim.x * 4) {
+    auto flush = [&]() {  // block-uniform
+        __syncthreads();
+        const int cnt = lqn;
+        if (threadIdx.x == 0 && cnt) lqbase = atomicAdd(&st->nlive[round + 1], cnt);
+        __syncthreads();
+        for (int j = threadIdx.x; j < cnt; j += 256) next[lqbase + j] = lq[j];
+        __syncthreads();
+        if (threadIdx.x == 0) lqn = 0;
+        __syncthreads();
+    };
+    auto decide = [&](int t) {
+        const long long i = list[t];
         const unsigned long long key = ckey[i];
-        if (!key) continue;
         int v[8], delta, kind;
         const int k = cand_get(cv, i, v, delta, kind);
         int mine = -1;
@@ -332,15 +352,15 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
         // so that what a wave sees does not depend on timing)
         if (__ballot(mine >= 0 && taken[mine].x && (taken[mine].y >> 20) < round)) {
             if (lane == 0) ckey[i] = 0;
-            continue;
+            return;
         }
         auto claim_next = [&]() {
             if (vmn && mine >= 0) atomicMax(&vmn[mine], key);
-            if (vmn && lane == 0) st->live[round + 1] = 1;
+            if (vmn && lane == 0) lq[atomicAdd(&lqn, 1)] = (int32_t)i;
         };
         if (__ballot(mine >= 0 && vmax[mine] > key)) {
             claim_next();
-            continue;
+            return;
         }
         int src[8];
         move_src(cv, i, kind, k, src);
@@ -383,7 +403,7 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
         }
         if (lost) {
             claim_next();
-            continue;
+            return;
         }
         const int change = delta + bm_wave::wave_sum(corr);
         if (lane == 0) {
@@ -396,7 +416,16 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
                 win[2 * max_win + s] = round;
             }
         }
+    };
+    if (threadIdx.x == 0) lqn = 0;
+    __syncthreads();
+    const int nl = st->nlive[round];
+    for (int t0 = blockIdx.x * 4; t0 < nl; t0 += gridDim.x * 4) {  // block-uniform trip count
+        if (t0 + (int)(threadIdx.x >> 6) < nl) decide(t0 + (threadIdx.x >> 6));
+        __syncthreads();
+        if (vmn && lqn > kDecideLoserCap - 4) flush();
     }
+    if (vmn) flush();
 }
 
 
@@ -1255,6 +1284,7 @@ struct bm_batch_state {
     unsigned long long *ckey = nullptr, *vmax = nullptr;
     int2 *taken = nullptr;  // per vertex: 0, or (new image + 1, old image | round << 20) once a winner of this step moves it
     int32_t *win = nullptr;
+    int32_t *clist = nullptr;  // the live candidates of a selection round, two lists of M (rounds alternate)
     int32_t *ulist = nullptr, *uold = nullptr, *ubase = nullptr, *cbase = nullptr;
     int32_t *xscratch = nullptr;  // chunk rows of the column recompute, xcap x (n+1)
     int xcap = 0;
@@ -1328,7 +1358,7 @@ int bm::free_batch(bm_ctx *c) {
         if (p) (void)hipFree(p);
     };
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
-    fr(b->ckey), fr(b->vmax), fr(b->taken), fr(b->win), fr(b->st), fr(b->hist), fr(b->log);
+    fr(b->ckey), fr(b->vmax), fr(b->taken), fr(b->win), fr(b->clist), fr(b->st), fr(b->hist), fr(b->log);
     fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU), fr(b->uoldv);
     fr(b->ulists), fr(b->hrows), fr(b->drows);
     if (b->hst) (void)hipHostFree(b->hst);
@@ -1405,13 +1435,15 @@ static int enqueue_step(bm_ctx *c) {
     const unsigned mb = (unsigned)((b->M + 255) / 256);
     BM_LAUNCH(c, "cand_keys",
               hipLaunchKernelGGL(k_cand_keys, dim3(mb), dim3(256), 0, c->stream, b->ckey, b->M, cv, c->vsimple,
-                                 b->st, inc ? 1 : 0, b->vmax));
+                                 b->st, inc ? 1 : 0, b->vmax, b->clist));
     for (int r = 0; r < p.rounds; r++) {
         unsigned long long *vm = b->vmax + (size_t)r * N1;
         unsigned long long *vmn = r + 1 < p.rounds ? vm + N1 : nullptr;
         BM_LAUNCH(c, "decide",
-                  hipLaunchKernelGGL(k_decide, dim3((unsigned)std::min<long long>((b->M + 3) / 4, 1024)), dim3(256),
-                                     0, c->stream, b->ckey, b->M, cv, vm, vmn, b->taken, b->win, N1 / 2 + 1, r, b->st, c->cptr,
+                  hipLaunchKernelGGL(k_decide, dim3((unsigned)std::min<long long>((b->M + 3) / 4, 2048)), dim3(256),
+                                     0, c->stream, b->ckey, b->M, cv, vm, vmn, b->taken, b->win, N1 / 2 + 1,
+                                     b->clist + (size_t)(r & 1) * b->M, b->clist + (size_t)((r + 1) & 1) * b->M, r,
+                                     b->st, c->cptr,
                                      (const int2 *)c->crw, c->map, c->W2, c->ld, N1));
     }
     const unsigned ublocks = 1024;  // grid-stride kernels of the incremental update
@@ -1593,6 +1625,7 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
     if ((rc = balloc(c, (void **)&b->vmax, (size_t)p.rounds * N1 * 8))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->taken, (size_t)N1 * 8))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->win, (size_t)(N1 / 2 + 1) * 12))) return fail(rc);  // index, change, round
+    if ((rc = balloc(c, (void **)&b->clist, (size_t)b->M * 8))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->ulist, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->uold, (size_t)N1 * 4))) return fail(rc);
     if ((rc = balloc(c, (void **)&b->ubase, (size_t)(N1 + 1) * 4))) return fail(rc);
