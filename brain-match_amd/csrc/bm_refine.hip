@@ -314,21 +314,136 @@ __device__ __forceinline__ void move_src(const CandView &cv, long long i, int ki
 // rounds, and the step's gain is the sum of the changes.
 constexpr int kDecideUnroll = 8;       // list entries in flight per lane in k_decide's scan
 constexpr int kDecideLoserCap = 256;   // losers a k_decide block holds in LDS before it appends them
+constexpr int kDecideWave = 1024;      // list entries a wave scans; a candidate with more takes the workgroup
 
-__global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__ ckey, long long M, CandView cv,
-                                                const unsigned long long *__restrict__ vmax,
-                                                unsigned long long *__restrict__ vmn, int2 *__restrict__ taken,
-                                                int32_t *__restrict__ win, int max_win, const int32_t *__restrict__ list,
-                                                int32_t *__restrict__ next, int round, RefineState *__restrict__ st,
-                                                const int32_t *__restrict__ cptr, const int2 *__restrict__ crw,
-                                                const int32_t *__restrict__ map, const int16_t *__restrict__ W2,
-                                                int ld, int N1) {
+struct DecideArgs {
+    unsigned long long *ckey;
+    CandView cv;
+    const unsigned long long *vmax;
+    unsigned long long *vmn;
+    int2 *taken;
+    int32_t *win;
+    int max_win, round;
+    RefineState *st;
+    const int32_t *cptr;
+    const int2 *crw;
+    const int32_t *map;
+    const int16_t *W2;
+    int ld, N1;
+};
+
+// One candidate decided by a group of kG threads (a wave, 64, or the
+// workgroup, 256): votes by ballot or __syncthreads_or, the correction
+// summed over the group.  A wave hands a candidate with more than
+// kDecideWave list entries to its workgroup (bigq) instead.  Losers are
+// queued in lq for the next round.
+template <int kG>
+__device__ __forceinline__ void decide_one(const DecideArgs &A, long long i, int g, int *lq, int *lqn, long long *bigq,
+                                           int *nbig, int *red) {
+    const bool block = kG == 256;
+    auto vote = [&](bool x) { return block ? __syncthreads_or(x) != 0 : __ballot(x) != 0ull; };
+    const unsigned long long key = A.ckey[i];
+    int v[8], delta, kind;
+    const int k = cand_get(A.cv, i, v, delta, kind);
+    if (!block) {
+        int ent = 0;
+        for (int q = 0; q < k; q++) ent += A.cptr[v[q] + 1] - A.cptr[v[q]];
+        if (ent > kDecideWave) {
+            if (g == 0) bigq[atomicAdd(nbig, 1)] = i;
+            return;
+        }
+    }
+    int mine = -1;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (q == g && q < k) mine = v[q];
+    // shares a vertex with a winner of an earlier round (this round's winners are left to the claims,
+    // so that what a group sees does not depend on timing)
+    if (vote(mine >= 0 && A.taken[mine].x && (A.taken[mine].y >> 20) < A.round)) {
+        if (g == 0) A.ckey[i] = 0;
+        return;
+    }
+    auto claim_next = [&]() {
+        if (A.vmn && mine >= 0) atomicMax(&A.vmn[mine], key);
+        if (A.vmn && g == 0) lq[atomicAdd(lqn, 1)] = (int32_t)i;
+    };
+    if (vote(mine >= 0 && A.vmax[mine] > key)) {
+        claim_next();
+        return;
+    }
+    int src[8];
+    move_src(A.cv, i, kind, k, src);
+    // one pass over N(V): the claims, and the lines to vertices moved in earlier rounds
+    int corr = 0;
+    bool lost = false;
+    const int N1 = A.N1;
+    for (int q = 0; q < k && !lost; q++) {
+        const int x = v[q], px = A.map[x], pn = A.map[v[src[q]]], e1 = A.cptr[x + 1];
+        for (int eb = A.cptr[x]; eb < e1; eb += kG * kDecideUnroll) {  // uniform trip count: the vote sees the group
+            const int e = eb + g;
+            int2 rw[kDecideUnroll];
+#pragma unroll
+            for (int j = 0; j < kDecideUnroll; j++) rw[j] = e + kG * j < e1 ? A.crw[e + kG * j] : make_int2(-1, 0);
+            bool lose = false;
+            int2 t[kDecideUnroll];
+#pragma unroll
+            for (int j = 0; j < kDecideUnroll; j++) {
+                t[j] = make_int2(0, 0);
+                if (rw[j].x < 0) continue;
+                const int y = rw[j].x >= N1 ? rw[j].x - N1 : rw[j].x;
+                lose |= A.vmax[y] > key;
+                t[j] = A.taken[y];
+                if ((t[j].y >> 20) >= A.round) t[j].x = 0;  // only earlier rounds
+            }
+            if (vote(lose)) {
+                lost = true;
+                break;
+            }
+#pragma unroll
+            for (int j = 0; j < kDecideUnroll; j++) {
+                if (!t[j].x) continue;
+                const bool out = rw[j].x >= N1;
+                const int yn = t[j].x - 1, py = t[j].y & 0xFFFFF, w = rw[j].y;
+                auto tw = [&](int a, int b) {
+                    return min(w, (int)(out ? A.W2[(size_t)a * A.ld + b] : A.W2[(size_t)b * A.ld + a]));
+                };
+                corr += tw(pn, yn) - tw(px, yn) - tw(pn, py) + tw(px, py);
+            }
+        }
+    }
+    if (lost) {
+        claim_next();
+        return;
+    }
+    corr = bm_wave::wave_sum(corr);
+    if (block) {
+        if ((g & 63) == 0) red[g >> 6] = corr;
+        __syncthreads();
+        corr = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+    const int change = delta + corr;
+    if (g == 0) {
+        A.ckey[i] = 0;
+        if (change > 0) {
+            for (int q = 0; q < k; q++) A.taken[v[q]] = make_int2(A.map[v[src[q]]] + 1, A.map[v[q]] | (A.round << 20));
+            const unsigned s = atomicAdd(&A.st->nwin, 1u);
+            A.win[s] = (int32_t)i;
+            A.win[A.max_win + s] = change;
+            A.win[2 * A.max_win + s] = A.round;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_decide(DecideArgs A, const int32_t *__restrict__ list,
+                                                int32_t *__restrict__ next) {
     // the losers of the round are gathered in LDS and appended to the next
     // round's list with one atomic per block and flush
     __shared__ int32_t lq[kDecideLoserCap];
-    __shared__ int lqn, lqbase;
-    const int lane = threadIdx.x & 63;
-    constexpr int kSpan = 64 * kDecideUnroll;
+    __shared__ int lqn, lqbase, nbig, red[4];
+    __shared__ long long bigq[4];
+    RefineState *st = A.st;
+    const int round = A.round;
     auto flush = [&]() {  // block-uniform
         __syncthreads();
         const int cnt = lqn;
@@ -339,93 +454,21 @@ __global__ __launch_bounds__(256) void k_decide(unsigned long long *__restrict__
         if (threadIdx.x == 0) lqn = 0;
         __syncthreads();
     };
-    auto decide = [&](int t) {
-        const long long i = list[t];
-        const unsigned long long key = ckey[i];
-        int v[8], delta, kind;
-        const int k = cand_get(cv, i, v, delta, kind);
-        int mine = -1;
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (q == lane && q < k) mine = v[q];
-        // shares a vertex with a winner of an earlier round (this round's winners are left to the claims,
-        // so that what a wave sees does not depend on timing)
-        if (__ballot(mine >= 0 && taken[mine].x && (taken[mine].y >> 20) < round)) {
-            if (lane == 0) ckey[i] = 0;
-            return;
-        }
-        auto claim_next = [&]() {
-            if (vmn && mine >= 0) atomicMax(&vmn[mine], key);
-            if (vmn && lane == 0) lq[atomicAdd(&lqn, 1)] = (int32_t)i;
-        };
-        if (__ballot(mine >= 0 && vmax[mine] > key)) {
-            claim_next();
-            return;
-        }
-        int src[8];
-        move_src(cv, i, kind, k, src);
-        // one pass over N(V): the claims, and the lines to vertices moved in earlier rounds
-        int corr = 0;
-        bool lost = false;
-        for (int q = 0; q < k && !lost; q++) {
-            const int x = v[q], px = map[x], pn = map[v[src[q]]], e1 = cptr[x + 1];
-            for (int eb = cptr[x]; eb < e1; eb += kSpan) {  // wave-uniform trip count: the ballot sees every lane
-                const int e = eb + lane;
-                int2 rw[kDecideUnroll];
-#pragma unroll
-                for (int j = 0; j < kDecideUnroll; j++) rw[j] = e + 64 * j < e1 ? crw[e + 64 * j] : make_int2(-1, 0);
-                bool lose = false;
-                int2 t[kDecideUnroll];
-#pragma unroll
-                for (int j = 0; j < kDecideUnroll; j++) {
-                    t[j] = make_int2(0, 0);
-                    if (rw[j].x < 0) continue;
-                    const int y = rw[j].x >= N1 ? rw[j].x - N1 : rw[j].x;
-                    lose |= vmax[y] > key;
-                    t[j] = taken[y];
-                    if ((t[j].y >> 20) >= round) t[j].x = 0;  // only earlier rounds
-                }
-                if (__ballot(lose)) {
-                    lost = true;
-                    break;
-                }
-#pragma unroll
-                for (int j = 0; j < kDecideUnroll; j++) {
-                    if (!t[j].x) continue;
-                    const bool out = rw[j].x >= N1;
-                    const int yn = t[j].x - 1, py = t[j].y & 0xFFFFF, w = rw[j].y;
-                    auto tw = [&](int a, int b) {
-                        return min(w, (int)(out ? W2[(size_t)a * ld + b] : W2[(size_t)b * ld + a]));
-                    };
-                    corr += tw(pn, yn) - tw(px, yn) - tw(pn, py) + tw(px, py);
-                }
-            }
-        }
-        if (lost) {
-            claim_next();
-            return;
-        }
-        const int change = delta + bm_wave::wave_sum(corr);
-        if (lane == 0) {
-            ckey[i] = 0;
-            if (change > 0) {
-                for (int q = 0; q < k; q++) taken[v[q]] = make_int2(map[v[src[q]]] + 1, map[v[q]] | (round << 20));
-                const unsigned s = atomicAdd(&st->nwin, 1u);
-                win[s] = (int32_t)i;
-                win[max_win + s] = change;
-                win[2 * max_win + s] = round;
-            }
-        }
-    };
-    if (threadIdx.x == 0) lqn = 0;
+    if (threadIdx.x == 0) lqn = nbig = 0;
     __syncthreads();
     const int nl = st->nlive[round];
     for (int t0 = blockIdx.x * 4; t0 < nl; t0 += gridDim.x * 4) {  // block-uniform trip count
-        if (t0 + (int)(threadIdx.x >> 6) < nl) decide(t0 + (threadIdx.x >> 6));
+        const int t = t0 + (int)(threadIdx.x >> 6);
+        if (t < nl) decide_one<64>(A, list[t], threadIdx.x & 63, lq, &lqn, bigq, &nbig, red);
         __syncthreads();
-        if (vmn && lqn > kDecideLoserCap - 4) flush();
+        const int nb = nbig;
+        for (int b = 0; b < nb; b++) decide_one<256>(A, bigq[b], threadIdx.x, lq, &lqn, bigq, &nbig, red);
+        __syncthreads();  // every thread has read nbig
+        if (threadIdx.x == 0) nbig = 0;
+        __syncthreads();
+        if (A.vmn && lqn > kDecideLoserCap - 8) flush();
     }
-    if (vmn) flush();
+    if (A.vmn) flush();
 }
 
 
@@ -1439,12 +1482,12 @@ static int enqueue_step(bm_ctx *c) {
     for (int r = 0; r < p.rounds; r++) {
         unsigned long long *vm = b->vmax + (size_t)r * N1;
         unsigned long long *vmn = r + 1 < p.rounds ? vm + N1 : nullptr;
+        DecideArgs da{b->ckey, cv, vm, vmn, b->taken, b->win, N1 / 2 + 1, r, b->st, c->cptr,
+                      (const int2 *)c->crw, c->map, c->W2, c->ld, N1};
         BM_LAUNCH(c, "decide",
                   hipLaunchKernelGGL(k_decide, dim3((unsigned)std::min<long long>((b->M + 3) / 4, 2048)), dim3(256),
-                                     0, c->stream, b->ckey, b->M, cv, vm, vmn, b->taken, b->win, N1 / 2 + 1,
-                                     b->clist + (size_t)(r & 1) * b->M, b->clist + (size_t)((r + 1) & 1) * b->M, r,
-                                     b->st, c->cptr,
-                                     (const int2 *)c->crw, c->map, c->W2, c->ld, N1));
+                                     0, c->stream, da, b->clist + (size_t)(r & 1) * b->M,
+                                     b->clist + (size_t)((r + 1) & 1) * b->M));
     }
     const unsigned ublocks = 1024;  // grid-stride kernels of the incremental update
     if (inc) {
