@@ -150,6 +150,36 @@ __global__ void k_scatter_w2(int16_t *__restrict__ W2, int N1, int ld, const int
     W2[((size_t)N1 + v) * ld + u] = (int16_t)w;
 }
 
+// (neighbour, weight) per comb entry, from the two arrays
+__global__ void k_interleave(const int32_t *__restrict__ a, const int32_t *__restrict__ b, int64_t m,
+                             int2 *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = make_int2(a[i], b[i]);
+}
+
+// The flat out-line list in CSR order from the comb lists: out entry e of
+// vertex v (cptr[v] <= e < cmid[v]) is line outptr[v] + e - cptr[v].
+__global__ void k_flat_out(const int32_t *__restrict__ cptr, const int32_t *__restrict__ cmid,
+                           const int32_t *__restrict__ outptr, const int32_t *__restrict__ cr,
+                           const int32_t *__restrict__ cw, const int32_t *__restrict__ cdw, int N1, int64_t nc,
+                           int32_t *__restrict__ es, int32_t *__restrict__ ed, int32_t *__restrict__ ew,
+                           int32_t *__restrict__ edw) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nc) return;
+    int lo = 0, hi = N1 - 1;  // the vertex whose entries hold e: last v with cptr[v] <= e
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cptr[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    if (e >= cmid[lo]) return;  // an in entry
+    const int64_t k = outptr[lo] + (e - cptr[lo]);
+    es[k] = lo;
+    ed[k] = cr[e] - N1;
+    ew[k] = cw[e];
+    edw[k] = cdw[e];
+}
+
 typedef int16_t bm_h8 __attribute__((ext_vector_type(8)));
 // LDS limit of the row-gather kernel (160 KiB per CU on gfx950); one row of
 // ld <= 65536 int16 always fits
@@ -743,6 +773,8 @@ static void free_graph_buffers(bm_ctx *c) {
     c->dfmaxw = 0;
     c->dfdiag_valid = false;
     c->hdfdeg.clear();
+    c->hcptr.clear(), c->hcmid.clear(), c->hcr.clear(), c->hcw.clear(), c->hddptr.clear(), c->hddmid.clear();
+    c->hlast.clear();
     dev_free(c, (void **)&c->esrc, (size_t)c->e1 * 4);
     dev_free(c, (void **)&c->edst, (size_t)c->e1 * 4);
     dev_free(c, (void **)&c->ew, (size_t)c->e1 * 4);
@@ -915,62 +947,40 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         cw[pi] = w1[i];
         nb[pi] = s;
     }
-    // owner of the dense G1 cell: the last line of each ordered pair
-    std::vector<int32_t> stamp(N1, -1);
+    // owner of the dense G1 cell: the last line of each ordered pair; the same
+    // entries make the de-duplicated view (last weight wins), out part then
+    // in part per vertex, whose lists are built on first use (host_dd)
+    std::vector<int32_t> stamp(N1, -1), ddptr(N1 + 1, 0), ddmid(N1, 0);
+    std::vector<uint8_t> last(nc, 0);
+    int64_t ndd = 0;
     for (int v = 0; v < N1; v++) {
+        ddptr[v] = (int32_t)ndd;
         int mid = cptr[v] + cnt_out[v];
         for (int e = mid - 1; e >= cptr[v]; e--)
             if (stamp[nb[e]] != 2 * v) {
                 stamp[nb[e]] = 2 * v;
                 cdw[e] = (int16_t)cw[e];
+                last[e] = 1;
+                ndd++;
             }
+        ddmid[v] = (int32_t)ndd;
         for (int e = cptr[v + 1] - 1; e >= mid; e--)
             if (stamp[nb[e]] != 2 * v + 1) {
                 stamp[nb[e]] = 2 * v + 1;
                 cdw[e] = (int16_t)cw[e];
+                last[e] = 1;
+                ndd++;
             }
     }
-    // flat out-edge list in CSR order (score, all-pairs corrections); the
-    // lines of vertex 0 come first and the score skips them
+    ddptr[N1] = (int32_t)ndd;
+    // the flat out-line list in CSR order (score, all-pairs corrections) is
+    // built on the device from the comb lists; the lines of vertex 0 come
+    // first and the score skips them
     c->e1_v0 = cnt_out[0];
-    std::vector<int32_t> es(e1), ed(e1), ewv(e1), edwv(e1);
-    {
-        int64_t k = 0;
-        for (int v = 0; v < N1; v++)
-            for (int e = cptr[v]; e < cptr[v] + cnt_out[v]; e++) {
-                es[k] = v;
-                ed[k] = nb[e];
-                ewv[k] = cw[e];
-                edwv[k] = cdw[e];
-                k++;
-            }
-    }
-    // de-duplicated view (last weight wins), out part then in part per vertex
-    std::vector<int32_t> ddptr(N1 + 1, 0), ddmid(N1, 0), ddr, ddw;
-    ddr.reserve(nc);
-    ddw.reserve(nc);
-    std::fill(stamp.begin(), stamp.end(), -1);
-    for (int v = 0; v < N1; v++) {
-        ddptr[v] = (int32_t)ddr.size();
-        int mid = cptr[v] + cnt_out[v];
-        // keep the last occurrence of each pair (list order is immaterial to the sums)
-        for (int e = mid - 1; e >= cptr[v]; e--)
-            if (stamp[nb[e]] != 2 * v) {
-                stamp[nb[e]] = 2 * v;
-                ddr.push_back(cr[e]);
-                ddw.push_back(cw[e]);
-            }
-        ddmid[v] = (int32_t)ddr.size();
-        for (int e = cptr[v + 1] - 1; e >= mid; e--)
-            if (stamp[nb[e]] != 2 * v + 1) {
-                stamp[nb[e]] = 2 * v + 1;
-                ddr.push_back(cr[e]);
-                ddw.push_back(cw[e]);
-            }
-    }
-    ddptr[N1] = (int32_t)ddr.size();
+    std::vector<int32_t> outptr(N1 + 1, 0);
+    for (int v = 0; v < N1; v++) outptr[v + 1] = outptr[v] + cnt_out[v];
     c->ncomb = nc;
-    c->ndd = (int64_t)ddr.size();
+    c->ndd = ndd;
 
     // weight facts used by the packed fast paths
     bool nonneg = true;
@@ -1006,12 +1016,6 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     if ((rc = upload(c, &c->cw, cw))) goto fail;
     if ((rc = upload(c, &c->cdw, cdw))) goto fail;
     {
-        std::vector<int32_t> rw(2 * (size_t)nc);
-        for (int64_t i = 0; i < nc; i++) {
-            rw[2 * i] = cr[i];
-            rw[2 * i + 1] = cw[i];
-        }
-        if ((rc = upload(c, &c->crw, rw))) goto fail;
         std::vector<int32_t> mid(N1), neg(N1, 0);
         for (int v = 0; v < N1; v++) {
             mid[v] = cptr[v] + cnt_out[v];
@@ -1021,15 +1025,27 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         if ((rc = upload(c, &c->cneg, neg))) goto fail;
         std::vector<int32_t> zeros(N1, 0);
         if ((rc = upload(c, &c->rowacc, zeros))) goto fail;
+        // (neighbour, weight) pairs and the flat out-line list, from what is on the device now
+        if ((rc = dev_alloc(c, (void **)&c->crw, (size_t)nc * 8))) goto fail;
+        if ((rc = dev_alloc(c, (void **)&c->esrc, (size_t)e1 * 4))) goto fail;
+        if ((rc = dev_alloc(c, (void **)&c->edst, (size_t)e1 * 4))) goto fail;
+        if ((rc = dev_alloc(c, (void **)&c->ew, (size_t)e1 * 4))) goto fail;
+        if ((rc = dev_alloc(c, (void **)&c->edw, (size_t)e1 * 4))) goto fail;
+        if ((rc = ensure_stage(c, (size_t)(N1 + 1) * 4))) goto fail;
+        hipError_t e = hipMemcpyAsync(c->stage, outptr.data(), (size_t)(N1 + 1) * 4, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "copy outptr"); goto fail; }
+        if (nc > 0) {
+            hipLaunchKernelGGL(k_interleave, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, c->cr, c->cw,
+                               nc, (int2 *)c->crw);
+            hipLaunchKernelGGL(k_flat_out, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, c->cptr,
+                               c->cmid, (const int32_t *)c->stage, c->cr, c->cw, c->cdw, N1, nc, c->esrc, c->edst,
+                               c->ew, c->edw);
+            if ((rc = launch_check("flat_out"))) goto fail;
+        }
+        // the stage buffer is reused by the W2 scatter below: wait for these kernels first
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "flat_out sync"); goto fail; }
     }
-    if ((rc = upload(c, &c->ddptr, ddptr))) goto fail;
-    if ((rc = upload(c, &c->ddmid, ddmid))) goto fail;
-    if ((rc = upload(c, &c->ddr, ddr))) goto fail;
-    if ((rc = upload(c, &c->ddw, ddw))) goto fail;
-    if ((rc = upload(c, &c->esrc, es))) goto fail;
-    if ((rc = upload(c, &c->edst, ed))) goto fail;
-    if ((rc = upload(c, &c->ew, ewv))) goto fail;
-    if ((rc = upload(c, &c->edw, edwv))) goto fail;
     if ((rc = upload(c, &c->g2optr, g2optr))) goto fail;
     if ((rc = upload(c, &c->g2iptr, g2iptr))) goto fail;
     if ((rc = upload(c, &c->g2o, g2o))) goto fail;
@@ -1081,6 +1097,17 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         c->hcdeg[v] = cptr[v + 1] - cptr[v];
         c->hg2deg[v] = (g2optr[v + 1] - g2optr[v]) + (g2iptr[v + 1] - g2iptr[v]);
     }
+    {
+        std::vector<int32_t> mid(N1);
+        for (int v = 0; v < N1; v++) mid[v] = cptr[v] + cnt_out[v];
+        c->hcmid.swap(mid);
+    }
+    c->hcptr.swap(cptr);
+    c->hcr.swap(cr);
+    c->hcw.swap(cw);
+    c->hlast.swap(last);
+    c->hddptr.swap(ddptr);
+    c->hddmid.swap(ddmid);
     c->graphs = true;
     c->mapped = false;
     return BM_OK;
@@ -1245,14 +1272,9 @@ uint64_t bm_rng_next(uint64_t *state) {
 static int vertex_scores(bm_ctx *c) {
     if (!c->hvscore.empty()) return BM_OK;
     const int N1 = c->N1;
-    std::vector<int32_t> ptr(N1 + 1), mid(N1), r((size_t)c->ndd), w((size_t)c->ndd);
-    BM_HIP(hipMemcpyAsync(ptr.data(), c->ddptr, (size_t)(N1 + 1) * 4, hipMemcpyDeviceToHost, c->stream));
-    BM_HIP(hipMemcpyAsync(mid.data(), c->ddmid, (size_t)N1 * 4, hipMemcpyDeviceToHost, c->stream));
-    if (c->ndd > 0) {
-        BM_HIP(hipMemcpyAsync(r.data(), c->ddr, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
-        BM_HIP(hipMemcpyAsync(w.data(), c->ddw, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
-    }
-    BM_HIP(hipStreamSynchronize(c->stream));
+    const std::vector<int32_t> &ptr = c->hddptr, &mid = c->hddmid;
+    std::vector<int32_t> r, w;
+    host_dd(c, r, w);
     std::vector<float> score(N1, 0.f);
     std::vector<std::pair<int, int>> cells;
     auto sum_sorted = [&](int e0, int e1, bool out) {
@@ -1284,16 +1306,36 @@ static int vertex_scores(bm_ctx *c) {
 // a and b.  Those are the de-duplicated entries whose int16 cell is > 0,
 // minus self-loops and vertex 0; with each entry owning its cell (dense-cell
 // weight = weight) the list-form kernels compute exactly that delta.
+void bm::host_dd(const bm_ctx *c, std::vector<int32_t> &r, std::vector<int32_t> &w) {
+    // the last entry of each (direction, neighbour) pair, taken from the back of each part as upload marked them
+    r.resize((size_t)c->ndd);
+    w.resize((size_t)c->ndd);
+    int64_t k = 0;
+    for (int v = 0; v < c->N1; v++) {
+        for (int e = c->hcmid[v] - 1; e >= c->hcptr[v]; e--)
+            if (c->hlast[e]) r[k] = c->hcr[e], w[k++] = c->hcw[e];
+        for (int e = c->hcptr[v + 1] - 1; e >= c->hcmid[v]; e--)
+            if (c->hlast[e]) r[k] = c->hcr[e], w[k++] = c->hcw[e];
+    }
+}
+
+int bm::ensure_dd(bm_ctx *c) {
+    if (c->ddptr) return BM_OK;
+    std::vector<int32_t> r, w;
+    host_dd(c, r, w);
+    int rc;
+    if ((rc = upload(c, &c->ddptr, c->hddptr)) || (rc = upload(c, &c->ddmid, c->hddmid)) ||
+        (rc = upload(c, &c->ddr, r)) || (rc = upload(c, &c->ddw, w)))
+        return rc;
+    return BM_OK;
+}
+
 int bm::ensure_dense_lists(bm_ctx *c) {
     if (c->dfptr) return BM_OK;
     const int N1 = c->N1;
-    std::vector<int32_t> ptr(N1 + 1), r((size_t)c->ndd), w((size_t)c->ndd);
-    BM_HIP(hipMemcpyAsync(ptr.data(), c->ddptr, (size_t)(N1 + 1) * 4, hipMemcpyDeviceToHost, c->stream));
-    if (c->ndd > 0) {
-        BM_HIP(hipMemcpyAsync(r.data(), c->ddr, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
-        BM_HIP(hipMemcpyAsync(w.data(), c->ddw, (size_t)c->ndd * 4, hipMemcpyDeviceToHost, c->stream));
-    }
-    BM_HIP(hipStreamSynchronize(c->stream));
+    const std::vector<int32_t> &ptr = c->hddptr;
+    std::vector<int32_t> r, w;
+    host_dd(c, r, w);
     // out entries (r >= N1) stay ahead of in entries, as in the lists they come from
     std::vector<int32_t> fptr(N1 + 1, 0), fmid(N1, 0), fr, fw, frw, deg(N1, 0), es, ed, ew;
     fr.reserve((size_t)c->ndd);

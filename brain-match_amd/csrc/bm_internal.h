@@ -162,6 +162,11 @@ struct bm_ctx {
     std::vector<int32_t> hpinv;
     std::vector<int32_t> hcdeg;   // comb-list length per vertex
     std::vector<int32_t> hdfdeg;  // dense-form list length per vertex (with the lists)
+    // host copies behind the de-duplicated view: comb offsets, out/in split, neighbour + weight per entry,
+    // whether the entry is the last of its (direction, neighbour) pair, and the view's own offsets; its
+    // device copy (ddptr ...) is made on first use (ensure_dd)
+    std::vector<int32_t> hcptr, hcmid, hcr, hcw, hddptr, hddmid;
+    std::vector<uint8_t> hlast;
     std::vector<int32_t> hg2deg;  // G2 in- plus out-degree per G2 vertex (de-duplicated)
     std::vector<float> hvscore;   // perturbation weights per G1 vertex (computed on first use)
     std::vector<int32_t> hg2optr, hg2iptr;  // host copies of the G2 list offsets
@@ -203,6 +208,9 @@ void timer_end(bm_ctx *c, int token);
 
 int ensure_diag(bm_ctx *c);
 int ensure_dense_lists(bm_ctx *c);
+// the de-duplicated view (last line of each (direction, neighbour) pair per vertex) on the host, and on the device
+void host_dd(const bm_ctx *c, std::vector<int32_t> &r, std::vector<int32_t> &w);
+int ensure_dd(bm_ctx *c);
 // out[v] = sum over the list of v of min(w, Q[r][v]), for every v
 int launch_list_diag(bm_ctx *c, int32_t *out, const int32_t *ptr, const int32_t *r, const int32_t *w);
 // X[u][v] = sum over the list of u of min(w, Q[r][v]) for all u, v (tile-major,

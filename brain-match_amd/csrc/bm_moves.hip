@@ -522,6 +522,9 @@ __global__ __launch_bounds__(256) void k_hillclimb(const int32_t *__restrict__ A
 
 int bm::launch_rot3(bm_ctx *c, const int32_t *dT, int64_t m, int mode, int32_t *dD, int32_t *dP) {
     if (m <= 0) return BM_OK;
+    // the de-duplicated lists on the device (a no-op after the first call; a batched step's
+    // configuration makes them before its capture)
+    if (int rc = ensure_dd(c)) return rc;
     const int lanes = c->rot3_lanes;
     const unsigned blocks = (unsigned)((m * lanes + 255) / 256);
 #define BM_ROT3(L)                                                                                                 \
@@ -860,6 +863,7 @@ extern "C" int bm_eval_swaps_hillclimb(bm_ctx *c, const int32_t *a, const int32_
             return set_error(BM_EINVAL, "pair %lld: (%d,%d) invalid", (long long)i, a[i], b[i]);
     if (count == 0) return BM_OK;
     BM_HIP(hipSetDevice(c->device));
+    if (int drc = ensure_dd(c)) return drc;
     const int64_t chunk = 1 << 22;
     int rc = ensure_stage(c, (size_t)chunk * 12);
     if (rc) return rc;

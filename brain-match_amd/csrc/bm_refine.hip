@@ -1724,6 +1724,8 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
             b->rot3_x = true;
         }
     }
+    // the direct triple kernel reads the de-duplicated lists: on the device before the capture
+    if (p.triples > 0 && !b->rot3_x && (rc = ensure_dd(c))) return fail(rc);
     if ((rc = autotune(c))) return fail(rc);
     if (p.pairs && c->sweep_kind == 2 && c->sparse_ok && !c->sparse_attr) {
         // first sparse launch outside the capture sets the kernel's LDS attribute
