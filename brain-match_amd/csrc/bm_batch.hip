@@ -44,6 +44,7 @@ __global__ __launch_bounds__(256) void k_argmax_moves(const int32_t *__restrict_
 
 int bm::free_moves(bm_ctx *c) {
     const size_t cap = (size_t)c->mv_cap;
+    if (c->mv_verts && c->stream) (void)hipStreamSynchronize(c->stream);  // queued work may still read them
     dev_free(c, (void **)&c->mv_verts, cap * 8 * 4);
     dev_free(c, (void **)&c->mv_delta, cap * 4);
     dev_free(c, (void **)&c->mv_aux, cap * 4);

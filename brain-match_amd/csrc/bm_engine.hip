@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -39,13 +40,117 @@ std::mutex &sync_mutex() {
     return mu;
 }
 
-int dev_alloc(bm_ctx *c, void **p, size_t bytes) {
-    if (bytes == 0) bytes = 16;
+// Device memory is taken through a process-wide cache: a freed block goes
+// back to its device's free list instead of to hipFree, and an allocation
+// takes the smallest cached block of at least its size and at most a quarter
+// more.  hipFree waits for the device, and under the serving loop every
+// retired job freed some forty buffers while other jobs ran; the cache also
+// spares the next job most of its hipMalloc calls.  Sizes are rounded to
+// 256 B below 1 MiB and to 2 MiB above.  The caller frees a block only when
+// its context's stream has no work left that uses it.  The free lists hold at
+// most kCacheMax bytes per device (the largest blocks go back first), and a
+// failed hipMalloc returns every cached block of the device and tries again.
+namespace {
+constexpr int kCacheDevices = 64;
+constexpr size_t kCacheMax = (size_t)48 << 30;
+struct BlockCache {
+    std::mutex mu;
+    std::map<void *, std::pair<int, size_t>> live;  // block -> (device, bytes)
+    std::multimap<size_t, void *> idle[kCacheDevices];
+    size_t idle_bytes[kCacheDevices] = {};
+};
+BlockCache &block_cache() {
+    static BlockCache *bc = new BlockCache;  // never destroyed: frees may come during process exit
+    return *bc;
+}
+size_t block_size(size_t bytes) {
+    if (bytes < ((size_t)1 << 20)) return (bytes + 255) & ~(size_t)255;
+    return (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+}
+void hip_free_all(const std::vector<void *> &blocks) {
+    if (blocks.empty()) return;
+    std::lock_guard<std::mutex> g(sync_mutex());
+    for (void *p : blocks) (void)hipFree(p);
+}
+}  // namespace
+
+hipError_t block_alloc(int device, void **p, size_t bytes) {
+    BlockCache &bc = block_cache();
+    const size_t want = block_size(bytes ? bytes : 16);
+    const int d = device >= 0 && device < kCacheDevices ? device : 0;
+    {
+        std::lock_guard<std::mutex> g(bc.mu);
+        auto it = bc.idle[d].lower_bound(want);
+        if (it != bc.idle[d].end() && it->first <= want + want / 4) {
+            *p = it->second;
+            bc.live[*p] = {d, it->first};
+            bc.idle_bytes[d] -= it->first;
+            bc.idle[d].erase(it);
+            return hipSuccess;
+        }
+    }
     hipError_t e;
     {
         std::lock_guard<std::mutex> g(sync_mutex());
-        e = hipMalloc(p, bytes);
+        e = hipMalloc(p, want);
     }
+    if (e != hipSuccess) {  // give this device's cached blocks back and try once more
+        (void)hipGetLastError();
+        std::vector<void *> drop;
+        {
+            std::lock_guard<std::mutex> g(bc.mu);
+            for (auto &kv : bc.idle[d]) drop.push_back(kv.second);
+            bc.idle[d].clear();
+            bc.idle_bytes[d] = 0;
+        }
+        if (drop.empty()) {
+            *p = nullptr;
+            return e;
+        }
+        hip_free_all(drop);
+        {
+            std::lock_guard<std::mutex> g(sync_mutex());
+            e = hipMalloc(p, want);
+        }
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return e;
+        }
+    }
+    std::lock_guard<std::mutex> g(bc.mu);
+    bc.live[*p] = {d, want};
+    return hipSuccess;
+}
+
+void block_free(void *p) {
+    if (!p) return;
+    BlockCache &bc = block_cache();
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> g(bc.mu);
+        auto it = bc.live.find(p);
+        if (it == bc.live.end()) {
+            drop.push_back(p);  // not from block_alloc
+        } else {
+            const int d = it->second.first;
+            const size_t sz = it->second.second;
+            bc.live.erase(it);
+            bc.idle[d].emplace(sz, p);
+            bc.idle_bytes[d] += sz;
+            while (bc.idle_bytes[d] > kCacheMax && !bc.idle[d].empty()) {
+                auto big = std::prev(bc.idle[d].end());
+                bc.idle_bytes[d] -= big->first;
+                drop.push_back(big->second);
+                bc.idle[d].erase(big);
+            }
+        }
+    }
+    hip_free_all(drop);
+}
+
+int dev_alloc(bm_ctx *c, void **p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    const hipError_t e = block_alloc(c->device, p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
         return hip_fail(e, "hipMalloc");
@@ -62,8 +167,7 @@ int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
 
 void dev_free(bm_ctx *c, void **p, size_t bytes) {
     if (*p) {
-        std::lock_guard<std::mutex> g(sync_mutex());
-        (void)hipFree(*p);
+        block_free(*p);
         c->dev_bytes -= (int64_t)(bytes ? bytes : 16);
     }
     *p = nullptr;
@@ -72,8 +176,8 @@ void dev_free(bm_ctx *c, void **p, size_t bytes) {
 int ensure_stage(bm_ctx *c, size_t bytes) {
     if (c->stage_bytes >= bytes) return BM_OK;
     if (c->stage) {
-        std::lock_guard<std::mutex> g(sync_mutex());
-        (void)hipFree(c->stage);
+        BM_HIP(hipStreamSynchronize(c->stream));  // the stream's last use of the buffer is over
+        block_free(c->stage);
         c->dev_bytes -= (int64_t)c->stage_bytes;
         c->stage = nullptr;
         c->stage_bytes = 0;
@@ -819,8 +923,8 @@ int bm_destroy(bm_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph_buffers(c);
+    block_free(c->stage);
     std::lock_guard<std::mutex> g(sync_mutex());
-    if (c->stage) (void)hipFree(c->stage);
     for (auto &r : c->pending) {
         (void)hipEventDestroy(r.start);
         (void)hipEventDestroy(r.stop);

@@ -195,6 +195,9 @@ int hip_fail(hipError_t e, const char *what);
 // capture in progress.
 std::mutex &sync_mutex();
 
+// device blocks through the process-wide cache (free only blocks no queued work still uses)
+hipError_t block_alloc(int device, void **p, size_t bytes);
+void block_free(void *p);
 int dev_alloc(bm_ctx *c, void **p, size_t bytes);
 void dev_free(bm_ctx *c, void **p, size_t bytes);
 // hipMemcpy under sync_mutex: a blocking copy runs on the null stream, which

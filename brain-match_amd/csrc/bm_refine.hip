@@ -1396,16 +1396,17 @@ int bm::free_batch(bm_ctx *c) {
     bm_batch_state *b = c->batch;
     if (!b) return BM_OK;
     drop_graph(b);
-    std::lock_guard<std::mutex> g(sync_mutex());
-    auto fr = [&](void *p) {
-        if (p) (void)hipFree(p);
-    };
+    (void)hipStreamSynchronize(c->stream);  // no step of this state is still running
+    auto fr = [&](void *p) { block_free(p); };
     fr(b->tri), fr(b->tri_d), fr(b->tri_p), fr(b->sets), fr(b->set_d), fr(b->set_r), fr(b->ovf);
     fr(b->ckey), fr(b->vmax), fr(b->taken), fr(b->win), fr(b->clist), fr(b->st), fr(b->hist), fr(b->log);
     fr(b->ulist), fr(b->uold), fr(b->ubase), fr(b->cbase), fr(b->xscratch), fr(b->inU), fr(b->uoldv);
     fr(b->ulists), fr(b->hrows), fr(b->drows);
-    if (b->hst) (void)hipHostFree(b->hst);
-    if (b->done) (void)hipEventDestroy(b->done);
+    {
+        std::lock_guard<std::mutex> g(sync_mutex());
+        if (b->hst) (void)hipHostFree(b->hst);
+        if (b->done) (void)hipEventDestroy(b->done);
+    }
     c->dev_bytes -= (int64_t)b->bytes;
     delete b;
     c->batch = nullptr;
@@ -1414,11 +1415,7 @@ int bm::free_batch(bm_ctx *c) {
 
 static int balloc(bm_ctx *c, void **p, size_t bytes) {
     if (bytes == 0) bytes = 16;
-    hipError_t e;
-    {
-        std::lock_guard<std::mutex> g(sync_mutex());
-        e = hipMalloc(p, bytes);
-    }
+    const hipError_t e = block_alloc(c->device, p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
         return set_error(BM_ENOMEM, "device allocation of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
@@ -1747,8 +1744,8 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
 int refine_set_log(bm_ctx *c, int32_t cap) {
     bm_batch_state *b = c->batch;
     if (b->log) {
-        std::lock_guard<std::mutex> g(sync_mutex());
-        (void)hipFree(b->log);
+        BM_HIP(hipStreamSynchronize(c->stream));
+        block_free(b->log);
         b->bytes -= (size_t)b->log_cap * sizeof(MoveRec);
         c->dev_bytes -= (int64_t)b->log_cap * (int64_t)sizeof(MoveRec);
         b->log = nullptr;
