@@ -397,7 +397,7 @@ def serve_sweep(args):
 
     rows = []
     jobs = getattr(args, "jobs", 0) or 80
-    for rate in (2.0, 4.0, 6.0, 8.0, 10.0, 12.0, 16.0, 20.0, 24.0, 28.0, 32.0):
+    for rate in (2.0, 4.0, 8.0, 12.0, 16.0, 20.0, 24.0, 32.0, 40.0, 48.0, 64.0):
         a = A()
         a.jobs, a.rate = jobs, rate
         r = serve(a)
