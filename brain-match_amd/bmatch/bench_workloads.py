@@ -234,7 +234,7 @@ def batched_c1(args):
 
 def batched_permset(args):
     inst = bmatch.Instance.powerlaw(1, 1000, 20000)
-    prm = bmatch.refine_params(pairs=0, triples=0, sets=4096, set_k=8, set_gen=1, rounds=8, steps_per_check=4,
+    prm = bmatch.refine_params(pairs=0, triples=0, sets=4096, set_k=8, set_gen=1, rounds=3, steps_per_check=4,
                                seed=1)
     with bmatch.Engine(0) as e:
         e.upload(inst.n, inst.g1, inst.g2)

@@ -645,7 +645,7 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
                                                   int32_t *__restrict__ uold, int32_t *__restrict__ ubase,
                                                   uint8_t *__restrict__ inU, int32_t *__restrict__ uoldv,
                                                   RefineState *__restrict__ st, int inc_max, int pinc_max,
-                                                  long long inc_budget, int dbar, int pfree) {
+                                                  long long inc_budget, int dbar, int pfree, int N1) {
     __shared__ int cnt, carry;
     __shared__ int wsum[16];
     __shared__ unsigned long long work;
@@ -659,12 +659,10 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
     for (int w = threadIdx.x; w < nwin; w += 1024) {
         int v[8], d, kind;
         const int k = cand_get(cv, win[w], v, d, kind);
-        const int b = atomicAdd(&cnt, k);
+        atomicAdd(&cnt, k);
         unsigned long long wk = 0;
         for (int q = 0; q < k; q++) {
             const int g = map[v[q]];
-            ulist[b + q] = v[q];
-            uold[b + q] = g;
             uoldv[v[q]] = g;
             inU[v[q]] = 1;
             const unsigned long long d2 = (g2iptr[g + 1] - g2iptr[g]) + (g2optr[g + 1] - g2optr[g]);
@@ -674,6 +672,22 @@ __global__ __launch_bounds__(1024) void k_collect(const int32_t *__restrict__ wi
     }
     __syncthreads();
     const int nu = cnt;
+    {
+        // U in increasing vertex order, from the flags: each thread a contiguous range of vertices
+        // (the column pass then writes neighbouring columns of a row together)
+        const int per = (N1 + 1023) / 1024, v0 = min(N1, (int)threadIdx.x * per), v1 = min(N1, v0 + per);
+        int mine = 0;
+        for (int v = v0; v < v1; v++) mine += inU[v];
+        int total;
+        int k = block_scan_excl(mine, wsum, total);
+        for (int v = v0; v < v1; v++)
+            if (inU[v]) {
+                ulist[k] = v;
+                uold[k] = uoldv[v];
+                k++;
+            }
+        __syncthreads();
+    }
     for (int c0 = 0; c0 < nu; c0 += 1024) {
         const int i = c0 + threadIdx.x;
         const int deg = i < nu ? cptr[ulist[i] + 1] - cptr[ulist[i]] : 0;
@@ -1299,19 +1313,45 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(80))) void k_x
     }
 }
 
+// Column v of X for the moved set, summed and written a 64 x 64 block at a
+// time: a block takes a tile row I (64 rows u) and 64 consecutive entries of
+// U (kept in increasing vertex order), sums C_u and each column's chunk rows
+// into an LDS tile (lanes along u: coalesced chunk-row reads), then writes it
+// with lanes along the columns, so the columns of U that share a 64-column
+// tile land in the same 256-byte piece of row u together.
 __global__ __launch_bounds__(256) void k_xcol_sum(const RefineState *__restrict__ st,
                                                   const int32_t *__restrict__ ulist, const int32_t *__restrict__ cbase,
                                                   const int32_t *__restrict__ cneg,
                                                   const int32_t *__restrict__ scratch, int N1,
                                                   int32_t *__restrict__ X, int64_t ldx) {
+    __shared__ int tile[64][65];
+    __shared__ int vcol[64];
     if (!st->inc) return;
     const int nu = st->nu, T = (int)(ldx >> 6);
-    const int64_t total = (int64_t)nu * N1;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int vi = (int)(i / N1), u = (int)(i - (int64_t)vi * N1);
-        int x = cneg[u];
-        for (int it = cbase[vi]; it < cbase[vi + 1]; it++) x += scratch[(size_t)it * N1 + u];
-        X[xidx(u, ulist[vi], T)] = x;
+    const int rows = (N1 + 63) / 64, groups = (nu + 63) / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int w = blockIdx.x; w < rows * groups; w += gridDim.x) {
+        const int I = w % rows, c0 = (w / rows) * 64, nc = min(64, nu - c0);
+        const int u = I * 64 + lane;
+        for (int c = wave; c < nc; c += 4) {
+            const int vi = c0 + c;
+            int x = 0;
+            if (u < N1) {
+                x = cneg[u];
+                for (int it = cbase[vi]; it < cbase[vi + 1]; it++) x += scratch[(size_t)it * N1 + u];
+            }
+            tile[c][lane] = x;
+        }
+        if ((int)threadIdx.x < nc) vcol[threadIdx.x] = ulist[c0 + threadIdx.x];
+        __syncthreads();
+        if (lane < nc) {
+            const int v = vcol[lane];
+            for (int r = wave; r < 64; r += 4) {
+                const int ur = I * 64 + r;
+                if (ur < N1) X[xidx(ur, v, T)] = tile[lane][r];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1491,7 +1531,7 @@ static int enqueue_step(bm_ctx *c) {
         BM_LAUNCH(c, "collect", hipLaunchKernelGGL(k_collect, dim3(1), dim3(1024), 0, c->stream, b->win, cv, c->map,
                                                    c->cptr, c->g2optr, c->g2iptr, b->ulist, b->uold, b->ubase, b->inU,
                                                    b->uoldv, b->st, b->inc_max, b->pinc_max, b->inc_budget,
-                                                   (int)(c->ncomb / std::max(1, c->N1)), pfree ? 1 : 0));
+                                                   (int)(c->ncomb / std::max(1, c->N1)), pfree ? 1 : 0, N1));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, -1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
                                                     Pq, c->ld, N1, c->X, c->ldx, rowmark(b, N1), pmap));
@@ -1539,7 +1579,7 @@ static int enqueue_step(bm_ctx *c) {
                                                 b->st, b->ulist, b->cbase, c->map, c->cptr, c->cmid,
                                                 (const int2 *)c->crw, c->g2optr, c->g2oy, c->g2iptr, c->g2iy, N1,
                                                 b->xscratch));
-        BM_LAUNCH(c, "xcol_sum", hipLaunchKernelGGL(k_xcol_sum, dim3(512), dim3(256), 0, c->stream, b->st, b->ulist,
+        BM_LAUNCH(c, "xcol_sum", hipLaunchKernelGGL(k_xcol_sum, dim3(2048), dim3(256), 0, c->stream, b->st, b->ulist,
                                                     b->cbase, c->cneg, b->xscratch, N1, c->X, c->ldx));
         BM_LAUNCH(c, "corr_inc", hipLaunchKernelGGL(k_corr_inc, dim3(ublocks), dim3(256), 0, c->stream, 1, b->st,
                                                     b->ulist, b->ubase, b->inU, c->cptr, (const int2 *)c->crw, c->cdw,
