@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <map>
 #include <thread>
+#include <functional>
 #include <vector>
 
 #include "bm_internal.h"
@@ -965,6 +966,36 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     c->ld = (N1 + 63) / 64 * 64;
     c->e1 = e1;
 
+    // The host list builds run on a few threads each (stable counting sorts
+    // over contiguous chunks of the input: a chunk's entries keep their file
+    // order, and the chunks are placed one after another, so the lists come
+    // out exactly as a single pass would lay them down).
+    auto threads_for = [](int64_t work) { return (int)std::max<int64_t>(1, std::min<int64_t>(4, work / 200000)); };
+    auto parallel = [](int T, const std::function<void(int)> &fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < T; t++) {
+            try {
+                th.emplace_back(fn, t);
+            } catch (...) {
+                fn(t);  // no thread: run the part here
+            }
+        }
+        fn(0);
+        for (auto &x : th) x.join();
+    };
+    // per-thread stable placement: counts[t][v] (of chunk t) become the
+    // chunk's first position for key v, given each key's base position
+    auto chunk_offsets = [](std::vector<int32_t> &counts, int T, int K, const std::function<int32_t(int)> &base) {
+        for (int v = 0; v < K; v++) {
+            int32_t o = base(v);
+            for (int t = 0; t < T; t++) {
+                const int32_t c = counts[(size_t)t * K + v];
+                counts[(size_t)t * K + v] = o;
+                o += c;
+            }
+        }
+    };
+
     // ---- G2 (on a second host thread while this one builds the G1 lists):
     // last line per ordered pair, then its out- and in-lists packed (vertex << 16 | weight)
     std::vector<int32_t> tri;
@@ -974,50 +1005,98 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     std::vector<int32_t> g2o_at_in, g2i_at_out;
     int64_t m2 = 0;
     auto build_g2 = [&]() {
-        std::vector<int32_t> p2(N1 + 1, 0), seen(N1, -1);
-        for (int64_t i = 0; i < e2; i++) p2[s2[i] + 1]++;
-        for (int v = 0; v < N1; v++) p2[v + 1] += p2[v];
-        std::vector<int32_t> pos(p2.begin(), p2.end() - 1), od(e2), ow(e2);
-        for (int64_t i = 0; i < e2; i++) {
-            int p = pos[s2[i]]++;
-            od[p] = d2[i];
-            ow[p] = w2[i];
-        }
-        tri.reserve(3 * (size_t)e2);
-        for (int u = 0; u < N1; u++)
-            for (int e = p2[u + 1] - 1; e >= p2[u]; e--)
-                if (seen[od[e]] != u) {
-                    seen[od[e]] = u;
-                    tri.push_back(u);
-                    tri.push_back(od[e]);
-                    tri.push_back(ow[e]);
-                    if ((int16_t)ow[e] < 0) g2nonneg = false;
-                    if ((int16_t)ow[e] < 0 || (int16_t)ow[e] > 16383) g2small = false;
-                }
-        m2 = (int64_t)tri.size() / 3;
-        for (int64_t i = 0; i < m2; i++) {
-            g2optr[tri[3 * i] + 1]++;
-            g2iptr[tri[3 * i + 1] + 1]++;
-            if (tri[3 * i] == 0 || tri[3 * i + 1] == 0) g2_vertex0 = true;
-        }
+        const int T = threads_for(e2);
+        auto lo = [&](int64_t m, int t) { return m * t / T; };
+        // lines by source, file order within a source
+        std::vector<int32_t> p2(N1 + 1, 0), cnt((size_t)T * N1, 0);
+        parallel(T, [&](int t) {
+            int32_t *c = cnt.data() + (size_t)t * N1;
+            for (int64_t i = lo(e2, t); i < lo(e2, t + 1); i++) c[s2[i]]++;
+        });
         for (int v = 0; v < N1; v++) {
-            g2optr[v + 1] += g2optr[v];
-            g2iptr[v + 1] += g2iptr[v];
+            int32_t sum = 0;
+            for (int t = 0; t < T; t++) sum += cnt[(size_t)t * N1 + v];
+            p2[v + 1] = p2[v] + sum;
         }
-        std::vector<int32_t> po(g2optr.begin(), g2optr.end() - 1), pi(g2iptr.begin(), g2iptr.end() - 1);
+        chunk_offsets(cnt, T, N1, [&](int v) { return p2[v]; });
+        std::vector<int32_t> od(e2), ow(e2);
+        parallel(T, [&](int t) {
+            int32_t *pos = cnt.data() + (size_t)t * N1;
+            for (int64_t i = lo(e2, t); i < lo(e2, t + 1); i++) {
+                const int p = pos[s2[i]]++;
+                od[p] = d2[i];
+                ow[p] = w2[i];
+            }
+        });
+        // the last line of each (source, target) pair, sources in increasing order: each thread a range of
+        // sources (split by line count) into its own part, the parts then laid end to end
+        std::vector<int> vs(T + 1, N1);
+        vs[0] = 0;
+        for (int t = 1; t < T; t++)
+            vs[t] = (int)(std::upper_bound(p2.begin(), p2.end(), (int32_t)lo(e2, t)) - p2.begin()) - 1;
+        for (int t = 1; t <= T; t++) vs[t] = std::max(vs[t], vs[t - 1]);
+        std::vector<std::vector<int32_t>> part(T);
+        std::vector<char> neg(T, 0), big(T, 0), v0(T, 0);
+        parallel(T, [&](int t) {
+            std::vector<int32_t> seen(N1, -1);
+            auto &out = part[t];
+            out.reserve(3 * (size_t)(p2[vs[t + 1]] - p2[vs[t]]));
+            for (int u = vs[t]; u < vs[t + 1]; u++)
+                for (int e = p2[u + 1] - 1; e >= p2[u]; e--)
+                    if (seen[od[e]] != u) {
+                        seen[od[e]] = u;
+                        out.push_back(u);
+                        out.push_back(od[e]);
+                        out.push_back(ow[e]);
+                        if ((int16_t)ow[e] < 0) neg[t] = 1;
+                        if ((int16_t)ow[e] < 0 || (int16_t)ow[e] > 16383) big[t] = 1;
+                        if (u == 0 || od[e] == 0) v0[t] = 1;
+                    }
+        });
+        size_t all = 0;
+        for (int t = 0; t < T; t++) all += part[t].size();
+        tri.reserve(all);
+        for (int t = 0; t < T; t++) {
+            tri.insert(tri.end(), part[t].begin(), part[t].end());
+            std::vector<int32_t>().swap(part[t]);
+            if (neg[t]) g2nonneg = false;
+            if (big[t]) g2small = false;
+            if (v0[t]) g2_vertex0 = true;
+        }
+        m2 = (int64_t)tri.size() / 3;
+        // the out- and in-lists of the de-duplicated lines, in their order
+        std::vector<int32_t> co((size_t)T * N1, 0), ci((size_t)T * N1, 0);
+        parallel(T, [&](int t) {
+            int32_t *a = co.data() + (size_t)t * N1, *b = ci.data() + (size_t)t * N1;
+            for (int64_t i = lo(m2, t); i < lo(m2, t + 1); i++) {
+                a[tri[3 * i]]++;
+                b[tri[3 * i + 1]]++;
+            }
+        });
+        for (int v = 0; v < N1; v++) {
+            int32_t so = 0, si = 0;
+            for (int t = 0; t < T; t++) so += co[(size_t)t * N1 + v], si += ci[(size_t)t * N1 + v];
+            g2optr[v + 1] = g2optr[v] + so;
+            g2iptr[v + 1] = g2iptr[v] + si;
+        }
+        chunk_offsets(co, T, N1, [&](int v) { return g2optr[v]; });
+        chunk_offsets(ci, T, N1, [&](int v) { return g2iptr[v]; });
         g2o.resize(m2);
         g2i.resize(m2);
         g2o_at_in.resize(m2);
         g2i_at_out.resize(m2);
-        for (int64_t i = 0; i < m2; i++) {
-            const uint32_t u = (uint32_t)tri[3 * i], v = (uint32_t)tri[3 * i + 1];
-            const uint32_t wv = (uint16_t)(int16_t)tri[3 * i + 2];
-            const int jo = po[u]++, ji = pi[v]++;
-            g2o[jo] = (v << 16) | wv;
-            g2i[ji] = (u << 16) | wv;
-            g2o_at_in[ji] = jo;  // the same line in the other list
-            g2i_at_out[jo] = ji;
-        }
+        parallel(T, [&](int t) {
+            int32_t *po = co.data() + (size_t)t * N1, *pi = ci.data() + (size_t)t * N1;
+            for (int64_t i = lo(m2, t); i < lo(m2, t + 1); i++) {
+                const uint32_t u = (uint32_t)tri[3 * i], v = (uint32_t)tri[3 * i + 1];
+                const uint32_t wv = (uint16_t)(int16_t)tri[3 * i + 2];
+                const int jo = po[u]++, ji = pi[v]++;
+                g2o[jo] = (v << 16) | wv;
+                g2i[ji] = (u << 16) | wv;
+                g2o_at_in[ji] = jo;  // the same line in the other list
+                g2i_at_out[jo] = ji;
+            }
+        });
     };
     std::thread g2_thread;
     try {
@@ -1027,54 +1106,77 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
     }
 
     // ---- G1 comb lists: out entries (file order) then in entries (file order)
-    std::vector<int32_t> cnt_out(N1, 0), cnt_in(N1, 0);
-    for (int64_t i = 0; i < e1; i++) {
-        cnt_out[s1[i]]++;
-        cnt_in[d1[i]]++;
-    }
-    std::vector<int32_t> cptr(N1 + 1, 0), pos_out(N1), pos_in(N1);
+    const int T1 = threads_for(e1);
+    auto lo1 = [&](int64_t m, int t) { return m * t / T1; };
+    std::vector<int32_t> cnt_out(N1, 0), cnt_in(N1, 0), po_t((size_t)T1 * N1, 0), pi_t((size_t)T1 * N1, 0);
+    parallel(T1, [&](int t) {
+        int32_t *a = po_t.data() + (size_t)t * N1, *b = pi_t.data() + (size_t)t * N1;
+        for (int64_t i = lo1(e1, t); i < lo1(e1, t + 1); i++) {
+            a[s1[i]]++;
+            b[d1[i]]++;
+        }
+    });
+    for (int v = 0; v < N1; v++)
+        for (int t = 0; t < T1; t++) cnt_out[v] += po_t[(size_t)t * N1 + v], cnt_in[v] += pi_t[(size_t)t * N1 + v];
+    std::vector<int32_t> cptr(N1 + 1, 0);
     for (int v = 0; v < N1; v++) cptr[v + 1] = cptr[v] + cnt_out[v] + cnt_in[v];
-    for (int v = 0; v < N1; v++) {
-        pos_out[v] = cptr[v];
-        pos_in[v] = cptr[v] + cnt_out[v];
-    }
+    chunk_offsets(po_t, T1, N1, [&](int v) { return cptr[v]; });
+    chunk_offsets(pi_t, T1, N1, [&](int v) { return cptr[v] + cnt_out[v]; });
     const int64_t nc = 2 * e1;
     std::vector<int32_t> cr(nc), cw(nc), cdw(nc, 0), nb(nc);
-    for (int64_t i = 0; i < e1; i++) {
-        int s = s1[i], d = d1[i];
-        int po = pos_out[s]++;
-        cr[po] = N1 + d;
-        cw[po] = w1[i];
-        nb[po] = d;
-        int pi = pos_in[d]++;
-        cr[pi] = s;
-        cw[pi] = w1[i];
-        nb[pi] = s;
-    }
+    parallel(T1, [&](int t) {
+        int32_t *pos_out = po_t.data() + (size_t)t * N1, *pos_in = pi_t.data() + (size_t)t * N1;
+        for (int64_t i = lo1(e1, t); i < lo1(e1, t + 1); i++) {
+            const int s = s1[i], d = d1[i];
+            const int po = pos_out[s]++;
+            cr[po] = N1 + d;
+            cw[po] = w1[i];
+            nb[po] = d;
+            const int pi = pos_in[d]++;
+            cr[pi] = s;
+            cw[pi] = w1[i];
+            nb[pi] = s;
+        }
+    });
+    std::vector<int32_t>().swap(po_t);
+    std::vector<int32_t>().swap(pi_t);
     // owner of the dense G1 cell: the last line of each ordered pair; the same
     // entries make the de-duplicated view (last weight wins), out part then
     // in part per vertex, whose lists are built on first use (host_dd)
-    std::vector<int32_t> stamp(N1, -1), ddptr(N1 + 1, 0), ddmid(N1, 0);
+    std::vector<int32_t> ddptr(N1 + 1, 0), ddmid(N1, 0), ddo(N1, 0), ddi(N1, 0);
     std::vector<uint8_t> last(nc, 0);
+    {
+        std::vector<int> vs(T1 + 1, N1);
+        vs[0] = 0;
+        for (int t = 1; t < T1; t++)
+            vs[t] = (int)(std::upper_bound(cptr.begin(), cptr.end(), (int32_t)lo1(nc, t)) - cptr.begin()) - 1;
+        for (int t = 1; t <= T1; t++) vs[t] = std::max(vs[t], vs[t - 1]);
+        parallel(T1, [&](int t) {
+            std::vector<int32_t> stamp(N1, -1);
+            for (int v = vs[t]; v < vs[t + 1]; v++) {
+                const int mid = cptr[v] + cnt_out[v];
+                for (int e = mid - 1; e >= cptr[v]; e--)
+                    if (stamp[nb[e]] != 2 * v) {
+                        stamp[nb[e]] = 2 * v;
+                        cdw[e] = (int16_t)cw[e];
+                        last[e] = 1;
+                        ddo[v]++;
+                    }
+                for (int e = cptr[v + 1] - 1; e >= mid; e--)
+                    if (stamp[nb[e]] != 2 * v + 1) {
+                        stamp[nb[e]] = 2 * v + 1;
+                        cdw[e] = (int16_t)cw[e];
+                        last[e] = 1;
+                        ddi[v]++;
+                    }
+            }
+        });
+    }
     int64_t ndd = 0;
     for (int v = 0; v < N1; v++) {
         ddptr[v] = (int32_t)ndd;
-        int mid = cptr[v] + cnt_out[v];
-        for (int e = mid - 1; e >= cptr[v]; e--)
-            if (stamp[nb[e]] != 2 * v) {
-                stamp[nb[e]] = 2 * v;
-                cdw[e] = (int16_t)cw[e];
-                last[e] = 1;
-                ndd++;
-            }
-        ddmid[v] = (int32_t)ndd;
-        for (int e = cptr[v + 1] - 1; e >= mid; e--)
-            if (stamp[nb[e]] != 2 * v + 1) {
-                stamp[nb[e]] = 2 * v + 1;
-                cdw[e] = (int16_t)cw[e];
-                last[e] = 1;
-                ndd++;
-            }
+        ddmid[v] = (int32_t)(ndd + ddo[v]);
+        ndd += ddo[v] + ddi[v];
     }
     ddptr[N1] = (int32_t)ndd;
     // the flat out-line list in CSR order (score, all-pairs corrections) is
