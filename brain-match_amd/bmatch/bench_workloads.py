@@ -213,7 +213,7 @@ def batched(args):
 
 def batched_c1(args):
     inst = bmatch.Instance.powerlaw(1, 1000, 20000)
-    prm = bmatch.refine_params(pairs=1, triples=1 << 14, sets=512, set_k=4, rounds=32, steps_per_check=8, seed=1)
+    prm = bmatch.refine_params(pairs=1, triples=1 << 14, sets=512, set_k=4, rounds=12, steps_per_check=8, seed=1)
     with bmatch.Engine(0) as e:
         e.upload(inst.n, inst.g1, inst.g2)
         e.set_mapping(inst.p0)
