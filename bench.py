@@ -364,7 +364,7 @@ def main():
         # batched refinement against plain first-improvement, same pair and start, on this engine; the
         # batched run's end mapping (no improving swap left) is the CPU baseline's start
         secs = args.window or 5.0
-        prm = bmatch.refine_params(pairs=1, triples=1024, sets=256, set_k=4, rounds=32, steps_per_check=8, seed=1)
+        prm = bmatch.refine_params(pairs=1, triples=1024, sets=256, set_k=4, rounds=0, steps_per_check=8, seed=1)
         log("batched vs plain")
         eng.refine_configure(prm)
         b = bench_workloads._refine_window(eng, secs, chunk=8)

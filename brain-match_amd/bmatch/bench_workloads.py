@@ -204,7 +204,7 @@ def _batched_compare(inst, prm, seconds, label, chunk):
 
 def batched(args):
     inst = bmatch.Instance.chunglu(2, 18524, 3_000_000)
-    prm = bmatch.refine_params(pairs=1, triples=1024, sets=256, set_k=4, rounds=32, steps_per_check=8, seed=1)
+    prm = bmatch.refine_params(pairs=1, triples=1024, sets=256, set_k=4, rounds=0, steps_per_check=8, seed=1)
     secs = getattr(args, "window", 0) or 10.0
     return _batched_compare(inst, prm, secs, {"workload": "Chung-Lu pair n=18524 E=3.0M, 30% shuffled start, "
                                                           f"{secs:.0f} s window, batched vs plain", "n": inst.n},
@@ -263,7 +263,7 @@ def batched_many(args):
     with cf.ThreadPoolExecutor(8) as ex:
         insts = list(ex.map(lambda i: bmatch.Instance.powerlaw(900 + i, ns[i], 160 * ns[i], shuffle_frac=0.3),
                             range(len(ns))))
-    prm = bmatch.refine_params(pairs=1, triples=1024, sets=0, rounds=32, steps_per_check=8, seed=11)
+    prm = bmatch.refine_params(pairs=1, triples=1024, sets=0, rounds=0, steps_per_check=8, seed=11)
 
     def engines():
         es = []
@@ -317,7 +317,7 @@ def serve(args):
     rng = np.random.default_rng(2024)
     ns = rng.integers(500, 20001, jobs)
     gaps = rng.exponential(1.0 / rate, jobs)
-    prm = bmatch.refine_params(pairs=1, triples=1024, sets=0, rounds=32, steps_per_check=8, seed=11)
+    prm = bmatch.refine_params(pairs=1, triples=1024, sets=0, rounds=0, steps_per_check=8, seed=11)
     t = time.perf_counter()
     with cf.ThreadPoolExecutor(8) as ex:
         insts = list(ex.map(lambda i: bmatch.Instance.powerlaw(500 + i, int(ns[i]), 160 * int(ns[i]),

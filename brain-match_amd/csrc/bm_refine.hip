@@ -48,6 +48,7 @@ using namespace bm;
 namespace {
 
 constexpr int kHist = 4096;  // per-step history ring
+constexpr int kMaxRounds = 64;  // selection rounds per step
 constexpr int kXcolBlocks = 512;  // workgroups of the column recompute (grid-stride over its chunks)
 constexpr int kXcolLines = 256;   // G2 lines of a column's new image per chunk (at least)
 constexpr int kXcolSpare = 512;   // chunk slots beyond one per moved vertex
@@ -83,7 +84,7 @@ struct RefineState {
     unsigned long long score_change;  // exact S_opt change of this step over the lines at U
     int nhash, ndense;  // the row updates by kind
     int keep_keys;      // the previous step of this run moved nothing: the row-best keys still hold
-    int nlive[32];      // per selection round: candidates in its list (each has claimed for the round)
+    int nlive[kMaxRounds];  // per selection round: candidates in its list (each has claimed for the round)
 };
 
 struct MoveRec {
@@ -164,7 +165,7 @@ __global__ void k_step_prologue(RefineState *st, int pgate, uint32_t *__restrict
         const volatile RefineState *vs = st;
         const int rebuild = pgate && vs->xstale && vs->pstale ? 1 : 0;
         st->keep_keys = vs->step > 0 && vs->applied == 0 ? 1 : 0;
-        for (int r = 0; r < 32; r++) st->nlive[r] = 0;
+        for (int r = 0; r < kMaxRounds; r++) st->nlive[r] = 0;
         st->gain = 0;
         st->applied = 0;
         st->nwin = 0;
@@ -1676,7 +1677,11 @@ int refine_configure(bm_ctx *c, const bm_refine_params *prm) {
         return set_error(BM_EINVAL, "triples %d / sets %d outside [0, 2^24] / [0, 2^20]", p.triples, p.sets);
     if (p.sets > 0 && (p.set_k < 2 || p.set_k > 8)) return set_error(BM_EINVAL, "set_k=%d outside [2,8]", p.set_k);
     if (p.set_gen != 0 && p.set_gen != 1) return set_error(BM_EINVAL, "set_gen=%d not 0 or 1", p.set_gen);
-    if (p.rounds < 1 || p.rounds > 32) return set_error(BM_EINVAL, "rounds=%d outside [1,32]", p.rounds);
+    // rounds 0: by size (a round's launch outweighs its work on small pairs; on large ones the hub
+    // moves settle one per round, and more rounds per step mean fewer steps)
+    if (p.rounds == 0) p.rounds = c->n >= 8000 ? kMaxRounds : 32;
+    if (p.rounds < 1 || p.rounds > kMaxRounds)
+        return set_error(BM_EINVAL, "rounds=%d outside [0,%d]", p.rounds, kMaxRounds);
     if (p.steps_per_check < 1) return set_error(BM_EINVAL, "steps_per_check=%d < 1", p.steps_per_check);
     if (!p.pairs && !p.triples && !p.sets) return set_error(BM_EINVAL, "no move kind enabled");
     if (c->n < 3) return set_error(BM_ERANGE, "n=%d too small for batched refinement", c->n);

@@ -87,7 +87,7 @@ extern "C" int64_t bm_job_footprint(int32_t n, int64_t e1, int64_t e2, const bm_
     if (p) {
         const int64_t M = N1 + p->triples + p->sets;
         b += (int64_t)p->triples * 20 + (int64_t)p->sets * (4 * std::max(p->set_k, 1) + 8) + M * 16 +
-             (int64_t)p->rounds * N1 * 8 + N1 * 6 + 24 * 4096 + 4096;
+             (int64_t)(p->rounds > 0 ? p->rounds : (n >= 8000 ? 64 : 32)) * N1 * 8 + N1 * 6 + 24 * 4096 + 4096;
         if (p->triples > 0 && p->pairs)  // dense-form lists and G1 cell matrix of the triple deltas
             b += N1 * ld * 2 + 2 * e1 * 16 + N1 * 24;
     }

@@ -260,7 +260,7 @@ typedef struct bm_refine_params {
     int32_t sets;            /* k-set candidates generated per step, 0..2^20 */
     int32_t set_k;           /* k of the sets, 2..8 */
     int32_t set_gen;         /* 0: sets grown along G1 lines from a random vertex; 1: uniform random sets */
-    int32_t rounds;          /* selection rounds per step, 1..32 */
+    int32_t rounds;          /* selection rounds per step, 1..64; 0: 64 when n >= 8,000, else 32 */
     int32_t steps_per_check; /* steps enqueued between host checks of the stop rules */
     uint64_t seed;           /* candidate generator seed */
 } bm_refine_params;

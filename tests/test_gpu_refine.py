@@ -359,4 +359,7 @@ def test_refine_errors():
         with pytest.raises(bmatch.BMError):
             e.refine_configure(bmatch.refine_params(sets=10, set_k=9))
         with pytest.raises(bmatch.BMError):
-            e.refine_configure(bmatch.refine_params(rounds=0))
+            e.refine_configure(bmatch.refine_params(rounds=-1))
+        with pytest.raises(bmatch.BMError):
+            e.refine_configure(bmatch.refine_params(rounds=65))
+        e.refine_configure(bmatch.refine_params(rounds=0))  # 0: chosen by size
