@@ -275,6 +275,33 @@ def served_leg(jobs, rate):
                                                                          "workload": r["config"]["workload"]}
 
 
+def config_legs(steps):
+    """Short runs of the n=1,000 configurations: plain first-improvement to convergence, 2^24 two-hop
+    3-rotations per launch, 4,096 8-sets x 40,320 permutations per launch, and the batched 8-set step
+    with its exact score check."""
+    from bmatch import bench_workloads
+
+    class A:
+        pass
+
+    a = A()
+    a.steps = steps
+    keep = ("value", "unit", "ms_per_step", "steps")
+    out = {}
+    r = bench_workloads.plain(a)
+    out["c1_plain"] = {k: r[k] for k in keep} | {k: r[k] for k in ("seconds", "accepted", "initial_score",
+                                                                   "final_score")}
+    r = bench_workloads.rot3(a)
+    out["c3_rot3"] = {k: r[k] for k in keep} | {"form": r["form"], "kernel_avg_ms": r["kernel_avg_ms"],
+                                                "best": r["best"]}
+    r = bench_workloads.permset(a)
+    out["c4_permset"] = {k: r[k] for k in keep} | {"kernel_avg_ms": r["kernel_avg_ms"], "best": r["best"]}
+    r = bench_workloads.batched_permset(a)
+    out["c4_batched_permset"] = {k: r[k] for k in keep} | {"sets_applied": r["sets_applied"],
+                                                           "score_gain": r["score_gain"], "verified": True}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +409,8 @@ def main():
     if not args.sweep_only:
         log("served stream")
         out["served"] = served_leg(args.jobs or 40, args.rate)
+        log("n=1,000 configurations (plain, 3-rotations, 8-sets, batched 8-sets)")
+        out["configs_1000"] = config_legs(args.steps)
     if args.no_cpu_baseline or args.sweep_only:
         out["cpu_baseline"] = None
     else:
