@@ -397,6 +397,16 @@ def main():
         b = bench_workloads._refine_window(eng, secs, chunk=8)
         converged = eng.mapping()
         conv_best = eng.sweep_all_pairs()["best"][2]
+        # the drop-in first-improvement sweep on that mapping (every row searched, none accepts when
+        # no swap improves): the GPU side of the CPU baseline below, which runs on the same mapping
+        log("plain sweep from the converged mapping")
+        t = time.perf_counter()
+        _, st_c = eng.first_improvement(verify=False, max_sweeps=1)
+        el_c = time.perf_counter() - t
+        out["plain_converged_sweep"] = {"rows_per_s": st_c["rows"] / el_c,
+                                        "evals_per_s": st_c["rows"] * (n - 1) / el_c, "rows": st_c["rows"],
+                                        "accepted": st_c["accepted"], "seconds": el_c,
+                                        "row_searches": st_c["evals_rows"]}
         eng.set_mapping(inst.p0)
         p = bench_workloads._plain_window(eng, max(b["seconds"], 1.0))
         out["batched_vs_plain"] = {

@@ -20,6 +20,7 @@ BIN_DIR = os.path.join(_PKG, "bin")
 BM_OK, BM_EINVAL, BM_ENOMEM, BM_EHIP, BM_ESTATE, BM_ENODEV, BM_ERANGE, BM_EIO = 0, -1, -2, -3, -4, -5, -6, -7
 BM_ROW_FIRST_POSITIVE = 0
 BM_ROW_ARGMAX = 1
+BM_ROW_GROUP_MAX = 16
 BM_ROT3_ROTATIONS = 0
 BM_ROT3_ALL = 1
 BM_PERTURB_OMP = 0
@@ -62,6 +63,8 @@ def lib():
         "bm_eval_swaps_noalloc": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i32p]),
         "bm_eval_swap_row": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int,
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "bm_eval_swap_rows": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int,
+                                        _i32p, _i32p]),
         "bm_sweep_all_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]),
         "bm_sweep_all_pairs_dense": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -482,6 +485,13 @@ class Engine:
         b, d = C.c_int32(), C.c_int32()
         _check(self._L.bm_eval_swap_row(self._h, a, lo, hi, mode, C.byref(b), C.byref(d)), "bm_eval_swap_row")
         return b.value, d.value
+
+    def eval_swap_rows(self, a, count, lo, lo_next, mode=BM_ROW_FIRST_POSITIVE):
+        """bm_eval_swap_rows: rows a..a+count-1 under one mapping -> (b[count], delta[count])."""
+        b = np.zeros(count, np.int32)
+        d = np.zeros(count, np.int32)
+        _check(self._L.bm_eval_swap_rows(self._h, a, count, lo, lo_next, mode, b, d), "bm_eval_swap_rows")
+        return b, d
 
     def sweep_all_pairs(self, rows=False, matrix=False, kind=BM_DELTA_LISTS):
         """bm_sweep_all_pairs (Delta_B) or, with kind=BM_DELTA_DENSE, bm_sweep_all_pairs_dense."""

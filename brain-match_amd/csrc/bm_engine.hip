@@ -856,6 +856,8 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->cmid, N1 * 4);
     dev_free(c, (void **)&c->cneg, N1 * 4);
     dev_free(c, (void **)&c->rowacc, N1 * 4);
+    dev_free(c, (void **)&c->grpacc, (size_t)BM_ROW_GROUP_MAX * N1 * 4);
+    dev_free(c, (void **)&c->grpres, (size_t)BM_ROW_GROUP_MAX * 8);
     dev_free(c, (void **)&c->ddptr, (N1 + 1) * 4);
     dev_free(c, (void **)&c->ddmid, N1 * 4);
     dev_free(c, (void **)&c->ddr, (size_t)c->ndd * 4);
@@ -933,6 +935,7 @@ int bm_destroy(bm_ctx *c) {
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->hres) (void)hipHostFree(c->hres);
     if (c->hrows) (void)hipHostFree(c->hrows);
+    if (c->hgrpres) (void)hipHostFree(c->hgrpres);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BM_OK;

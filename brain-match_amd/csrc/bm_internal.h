@@ -71,6 +71,9 @@ struct bm_ctx {
     int32_t *cmid = nullptr;    // cmid[v]: first in entry of comb(v)
     int32_t *cneg = nullptr;    // cneg[v] = sum over comb(v) of min(w, 0)
     int32_t *rowacc = nullptr;  // (n+1)-int accumulator of the sparse row kernel, kept zeroed
+    int32_t *grpacc = nullptr;  // kRowGroupMax accumulators of (n+1) ints for row groups (lazy), kept zeroed
+    unsigned long long *grpres = nullptr;   // kRowGroupMax keys of a row group (lazy)
+    unsigned long long *hgrpres = nullptr;  // pinned host copy of those keys (lazy)
     int32_t *ddptr = nullptr, *ddmid = nullptr, *ddr = nullptr, *ddw = nullptr;
     // dense-form lists (lazy): the de-duplicated entries whose int16 G1 cell is
     // positive, without self-loops and vertex 0; weights are the int16 cells

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
 // reduces it to the first positive or arg-max key, and zeroes the buffer for
 // the next call.
 template <int kWaves>
-__global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int mode, int N1,
+__global__ __launch_bounds__(256) void k_row_expand(int a0, int lo0, int lo_next, int hi, int mode, int N1,
                                                     const int32_t *__restrict__ cptr,
                                                     const int32_t *__restrict__ cmid, const int2 *__restrict__ crw,
                                                     const int32_t *__restrict__ cdw, const int32_t *__restrict__ map,
@@ -223,9 +223,15 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
                                                     const uint32_t *__restrict__ g2oy,
                                                     const int32_t *__restrict__ g2iptr,
                                                     const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
-                                                    int ld, int *__restrict__ acc,
+                                                    int ld, int *__restrict__ acc, int64_t acc_stride,
                                                     unsigned long long *__restrict__ result) {
     constexpr int kLanes = 64 * kWaves, kHeads = 4 / kWaves;  // lanes per head, heads per block
+    // grid z: row a0 + z over [lo, hi], lo = lo0 for the first row and lo_next (-1: the
+    // row's own index + 1) for the others; each row has its own accumulator and key
+    const int a = a0 + (int)blockIdx.z;
+    const int lo = blockIdx.z == 0 ? lo0 : (lo_next < 0 ? a + 1 : lo_next);
+    acc += (int64_t)blockIdx.z * acc_stride;
+    result += blockIdx.z;
     const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes), lane = threadIdx.x % kLanes;
     const int ca0 = cptr[a], dega = cptr[a + 1] - ca0;
     const int pa = map[a];
@@ -302,15 +308,22 @@ __global__ __launch_bounds__(256) void k_row_expand(int a, int lo, int hi, int m
     }
 }
 
-__global__ __launch_bounds__(256) void k_row_final(int a, int lo, int hi, int mode, int *__restrict__ acc,
+__global__ __launch_bounds__(256) void k_row_final(int a0, int lo0, int lo_next, int lo_base, int hi, int mode,
+                                                   int *__restrict__ acc, int64_t acc_stride,
                                                    const int32_t *__restrict__ cneg,
                                                    const int32_t *__restrict__ diag,
                                                    unsigned long long *__restrict__ result,
                                                    int32_t *__restrict__ store) {
     __shared__ unsigned long long red[4];
-    const int b = lo + blockIdx.x * 256 + threadIdx.x;
-    if (store) {  // whole row: every delta, no reduction
-        if (b <= hi) {
+    // grid y: the rows of k_row_expand's grid z; the blocks cover [lo_base, hi], lo_base
+    // the lowest of the rows' lower ends
+    const int a = a0 + (int)blockIdx.y;
+    const int lo = blockIdx.y == 0 ? lo0 : (lo_next < 0 ? a + 1 : lo_next);
+    acc += (int64_t)blockIdx.y * acc_stride;
+    result += blockIdx.y;
+    const int b = lo_base + blockIdx.x * 256 + threadIdx.x;
+    if (store) {  // whole row (one row per launch): every delta, no reduction
+        if (b >= lo && b <= hi) {
             const int x = acc[b];
             acc[b] = 0;
             store[b] = b == a || b == 0 ? 0 : x + cneg[a] + cneg[b] - diag[a] - diag[b];
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(256) void k_row_final(int a, int lo, int hi, int mo
         return;
     }
     unsigned long long key = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
-    if (b <= hi) {
+    if (b >= lo && b <= hi) {
         const int x = acc[b];
         acc[b] = 0;
         if (b != a) {
@@ -1398,27 +1411,33 @@ extern "C" int bm_eval_swaps_dense(bm_ctx *c, const int32_t *a, const int32_t *b
 // dense: the dense-form lists in place of the comb lists.
 constexpr int kRowSlices = 4;  // workgroups per list head
 
-static int launch_row_expand(bm_ctx *c, bool dense, int a, int lo, int hi, int mode, unsigned long long *res) {
+// rows a .. a + rows - 1 (see k_row_expand), accumulators acc + i * N1
+static int launch_row_expand(bm_ctx *c, bool dense, int a, int lo, int lo_next, int hi, int mode, int32_t *acc,
+                             unsigned long long *res, int rows = 1) {
     static const int waves = [] {
         const char *e = getenv("BM_ROW_WAVES");
         return e && atoi(e) == 1 ? 1 : 4;
     }();
-    const int heads = (dense ? c->hdfdeg[a] : c->hcdeg[a]) + c->hg2deg[c->hmap[a]];
+    int heads = 0;
+    for (int i = 0; i < rows; i++)
+        heads = std::max(heads, (dense ? c->hdfdeg[a + i] : c->hcdeg[a + i]) + c->hg2deg[c->hmap[a + i]]);
     const int per = 4 / waves;
     const int32_t *lptr = dense ? c->dfptr : c->cptr, *lmid = dense ? c->dfmid : c->cmid;
     const int2 *lrw = (const int2 *)(dense ? c->dfrw : c->crw);
     const int32_t *ldw = dense ? c->dfw : c->cdw;
 #define BM_ROW_EXPAND(W)                                                                                         \
     BM_LAUNCH(c, "row_expand",                                                                                  \
-              hipLaunchKernelGGL(k_row_expand<W>, dim3(1 + (heads + per - 1) / per, kRowSlices), dim3(256), 0,      \
-                                 c->stream, a, lo,                                                                  \
+              hipLaunchKernelGGL(k_row_expand<W>, dim3(1 + (heads + per - 1) / per, kRowSlices, rows), dim3(256),   \
+                                 0, c->stream, a, lo, lo_next,                                                      \
                                  hi, mode, c->N1, lptr, lmid, lrw, ldw, c->map, c->g2optr, c->g2oy, c->g2iptr,       \
-                                 c->g2iy, c->Q, c->ld, c->rowacc, res))
+                                 c->g2iy, c->Q, c->ld, acc, (int64_t)c->N1, res))
     if (waves == 1) BM_ROW_EXPAND(1);
     else BM_ROW_EXPAND(4);
 #undef BM_ROW_EXPAND
     return BM_OK;
 }
+
+static void decode_row_key(unsigned long long k, int mode, int32_t *best_b, int32_t *best_delta);
 
 extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi, int mode, int32_t *best_b,
                                 int32_t *best_delta) {
@@ -1440,10 +1459,10 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
         if (c->g2y_dirty) {
             if ((rc = launch_refresh_g2y(c))) return rc;
         }
-        if ((rc = launch_row_expand(c, false, a, b_lo, b_hi, mode, res))) return rc;
+        if ((rc = launch_row_expand(c, false, a, b_lo, -1, b_hi, mode, c->rowacc, res))) return rc;
         BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((b_hi - b_lo + 256) / 256), dim3(256), 0,
-                                                     c->stream, a, b_lo, b_hi, mode, c->rowacc, c->cneg, c->diag, res,
-                                                     nullptr));
+                                                     c->stream, a, b_lo, -1, b_lo, b_hi, mode, c->rowacc,
+                                                     (int64_t)c->N1, c->cneg, c->diag, res, nullptr));
     } else {
         BM_HIP(hipMemsetAsync(res, mode == BM_ROW_FIRST_POSITIVE ? 0xFF : 0x00, 8, c->stream));
         const int blocks = (b_hi - b_lo + kRowCh) / kRowCh;
@@ -1460,7 +1479,13 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
     }
     BM_HIP(hipMemcpyAsync(c->hres, res, 8, hipMemcpyDeviceToHost, c->stream));  // pinned: no staging copy
     BM_HIP(hipStreamSynchronize(c->stream));
-    const unsigned long long k = c->hres[0];
+    decode_row_key(c->hres[0], mode, best_b, best_delta);
+    return BM_OK;
+}
+
+static void decode_row_key(unsigned long long k, int mode, int32_t *best_b, int32_t *best_delta) {
+    *best_b = 0;
+    *best_delta = mode == BM_ROW_ARGMAX ? INT32_MIN : 0;
     if (mode == BM_ROW_FIRST_POSITIVE) {
         if (k != ~0ull) {
             *best_b = (int32_t)(k >> 32);
@@ -1470,6 +1495,55 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
         *best_delta = (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
         *best_b = (int32_t)(0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFu));
     }
+}
+
+// Consecutive rows in one launch of each row kernel (grid z / y: the row), each
+// with its own zeroed accumulator and key; without the sparse preconditions the
+// rows go one by one through bm_eval_swap_row.
+extern "C" int bm_eval_swap_rows(bm_ctx *c, int32_t a, int32_t count, int32_t b_lo, int32_t b_lo_next, int mode,
+                                 int32_t *best_b, int32_t *best_delta) {
+    if (!c || !best_b || !best_delta) return set_error(BM_EINVAL, "NULL argument");
+    if (!c->mapped) return set_error(BM_ESTATE, "mapping not set");
+    if (mode != BM_ROW_FIRST_POSITIVE && mode != BM_ROW_ARGMAX) return set_error(BM_EINVAL, "bad mode %d", mode);
+    if (count < 1 || count > BM_ROW_GROUP_MAX)
+        return set_error(BM_EINVAL, "count=%d outside [1,%d]", count, BM_ROW_GROUP_MAX);
+    if (a < 0 || a + count - 1 > c->n)
+        return set_error(BM_EINVAL, "rows %d..%d outside [0,%d]", a, a + count - 1, c->n);
+    const int n = c->n;
+    auto lo_of = [&](int i) { return std::max(0, i == 0 ? b_lo : (b_lo_next < 0 ? a + i + 1 : b_lo_next)); };
+    if (!c->sparse_ok) {
+        for (int i = 0; i < count; i++)
+            if (int rc = bm_eval_swap_row(c, a + i, lo_of(i), n, mode, best_b + i, best_delta + i)) return rc;
+        return BM_OK;
+    }
+    if (int prc = ensure_p(c)) return prc;
+    BM_HIP(hipSetDevice(c->device));
+    int rc = ensure_diag(c);
+    if (rc) return rc;
+    const size_t N1 = (size_t)c->N1;
+    if (!c->grpacc) {
+        if ((rc = dev_alloc(c, (void **)&c->grpacc, (size_t)BM_ROW_GROUP_MAX * N1 * 4))) return rc;
+        BM_HIP(hipMemsetAsync(c->grpacc, 0, (size_t)BM_ROW_GROUP_MAX * N1 * 4, c->stream));
+    }
+    if (!c->grpres && (rc = dev_alloc(c, (void **)&c->grpres, (size_t)BM_ROW_GROUP_MAX * 8))) return rc;
+    if (!c->hgrpres) {
+        std::lock_guard<std::mutex> g(sync_mutex());  // host allocations may wait for the device
+        BM_HIP(hipHostMalloc((void **)&c->hgrpres, (size_t)BM_ROW_GROUP_MAX * 8, hipHostMallocDefault));
+    }
+    if (c->g2y_dirty && (rc = launch_refresh_g2y(c))) return rc;
+    // rows whose range is empty keep their neutral key (set by the expand kernel) and
+    // add nothing: k_row_expand and k_row_final test b against the row's own range
+    int lo_base = n + 1;
+    for (int i = 0; i < count; i++) lo_base = std::min(lo_base, lo_of(i));
+    const int first_lo = lo_of(0), next_lo = b_lo_next < 0 ? -1 : std::max(0, b_lo_next);
+    if ((rc = launch_row_expand(c, false, a, first_lo, next_lo, n, mode, c->grpacc, c->grpres, count))) return rc;
+    if (lo_base <= n)
+        BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((n - lo_base + 256) / 256, count), dim3(256),
+                                                     0, c->stream, a, first_lo, next_lo, lo_base, n, mode, c->grpacc,
+                                                     (int64_t)N1, c->cneg, c->diag, c->grpres, nullptr));
+    BM_HIP(hipMemcpyAsync(c->hgrpres, c->grpres, (size_t)count * 8, hipMemcpyDeviceToHost, c->stream));
+    BM_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < count; i++) decode_row_key(c->hgrpres[i], mode, best_b + i, best_delta + i);
     return BM_OK;
 }
 
@@ -1511,9 +1585,10 @@ extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32
         const int a = rows[i];
         int32_t *row = out + (size_t)i * N1;
         if (c->sparse_ok) {
-            if ((rc = launch_row_expand(c, dense, a, 0, n, BM_ROW_ARGMAX, res))) return rc;
+            if ((rc = launch_row_expand(c, dense, a, 0, -1, n, BM_ROW_ARGMAX, c->rowacc, res))) return rc;
             BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_final, dim3((n + 256) / 256), dim3(256), 0, c->stream,
-                                                         a, 0, n, BM_ROW_ARGMAX, c->rowacc, lneg, ldiag, res, row));
+                                                         a, 0, -1, 0, n, BM_ROW_ARGMAX, c->rowacc, (int64_t)c->N1,
+                                                         lneg, ldiag, res, row));
         } else {
             const int blocks = (n + kRowCh) / kRowCh;
             const size_t lds = (size_t)2 * c->N1 * sizeof(int16_t);

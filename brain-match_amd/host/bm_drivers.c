@@ -70,6 +70,48 @@ static int apply_and_verify(bm_ctx *ctx, int32_t a, int32_t b, int32_t d, int ve
     return BM_OK;
 }
 
+/* Rows evaluated ahead of the current one (bm_eval_swap_rows): their results
+ * hold while the mapping stays as it was, so any applied swap drops them.  A
+ * row that ends without an accept doubles the rows taken per launch (up to
+ * BM_ROW_GROUP_MAX), an accept sets it back to one: while most rows accept,
+ * nothing is evaluated in vain; once they stop accepting, one launch covers
+ * up to BM_ROW_GROUP_MAX rows. */
+typedef struct {
+    int32_t first, count, width;
+    int32_t b[BM_ROW_GROUP_MAX], d[BM_ROW_GROUP_MAX];
+} rows_ahead;
+
+static void ahead_drop(rows_ahead *ra) {
+    ra->count = 0;
+    ra->width = 1;
+}
+
+static void ahead_grow(rows_ahead *ra) {
+    ra->width = ra->width * 2 > BM_ROW_GROUP_MAX ? BM_ROW_GROUP_MAX : ra->width * 2;
+}
+
+/* row a over [lo, n]; fresh: lo is the row's first lower end, the one the rows
+ * ahead were evaluated with (lo_next, -1 for a + 1) */
+static int eval_row_ahead(bm_ctx *ctx, rows_ahead *ra, int32_t n, int32_t a, int32_t lo, int32_t lo_next, int mode,
+                          int fresh, int32_t *b, int32_t *d) {
+    if (fresh && ra->count > 0 && a >= ra->first && a < ra->first + ra->count) {
+        *b = ra->b[a - ra->first];
+        *d = ra->d[a - ra->first];
+        return BM_OK;
+    }
+    int32_t cnt = ra->width < n - a + 1 ? ra->width : n - a + 1;
+    int32_t bb[BM_ROW_GROUP_MAX], dd[BM_ROW_GROUP_MAX];
+    int rc = bm_eval_swap_rows(ctx, a, cnt, lo, lo_next, mode, bb, dd);
+    if (rc) return rc;
+    ra->first = a + 1;
+    ra->count = cnt - 1;
+    memcpy(ra->b, bb + 1, sizeof(int32_t) * (size_t)(cnt - 1));
+    memcpy(ra->d, dd + 1, sizeof(int32_t) * (size_t)(cnt - 1));
+    *b = bb[0];
+    *d = dd[0];
+    return BM_OK;
+}
+
 /* graph-alignment-optimized.c:513-581 */
 int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_sweeps, int verify,
                                bm_row_hook hook, void *user, int32_t *acc, int64_t acc_cap,
@@ -82,6 +124,8 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
     if (rc) return rc;
     st.final_score = st.initial_score;
     pending_check pc = {0, 0};
+    rows_ahead ra;
+    ahead_drop(&ra);
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
@@ -95,11 +139,17 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
             int32_t cursor = 1;
             while (cursor <= n) {
                 int32_t b, d;
-                if ((rc = bm_eval_swap_row(ctx, a, cursor, n, BM_ROW_FIRST_POSITIVE, &b, &d))) return rc;
+                /* the rows after a from 1, as their own first evaluation will be */
+                if ((rc = eval_row_ahead(ctx, &ra, n, a, cursor, 1, BM_ROW_FIRST_POSITIVE, cursor == 1, &b, &d)))
+                    return rc;
                 if ((rc = settle(ctx, &st, &pc))) return rc;
                 st.evals_rows++;
-                if (b == 0) break;
+                if (b == 0) {
+                    ahead_grow(&ra);
+                    break;
+                }
                 record(acc, acc_cap, st.accepted, a, b, d);
+                ahead_drop(&ra);
                 if ((rc = apply_and_verify(ctx, a, b, d, verify, &st, &pc))) return rc;
                 cursor = b + 1;
             }
@@ -123,6 +173,8 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
     if (rc) return rc;
     st.final_score = st.initial_score;
     pending_check pc = {0, 0};
+    rows_ahead ra;
+    ahead_drop(&ra);
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
@@ -135,12 +187,15 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
             }
             if (a == n) continue;
             int32_t b, d;
-            if ((rc = bm_eval_swap_row(ctx, a, a + 1, n, BM_ROW_ARGMAX, &b, &d))) return rc;
+            if ((rc = eval_row_ahead(ctx, &ra, n, a, a + 1, -1, BM_ROW_ARGMAX, 1, &b, &d))) return rc;
             if ((rc = settle(ctx, &st, &pc))) return rc;
             st.evals_rows++;
             if (d > 0) {
                 record(acc, acc_cap, st.accepted, a, b, d);
+                ahead_drop(&ra);
                 if ((rc = apply_and_verify(ctx, a, b, d, verify, &st, &pc))) return rc;
+            } else {
+                ahead_grow(&ra);
             }
         }
         if ((rc = settle(ctx, &st, &pc))) return rc;
@@ -196,12 +251,18 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *
     if (p->initial_swaps > 0 && (rc = bm_perturb(ctx, p->initial_swaps, BM_PERTURB_OMP, &rng, NULL))) goto out;
     if ((rc = bm_score(ctx, &cur))) goto out;
     int64_t pass = 0, improvements = 0, last = 0;
+    rows_ahead ra;
     while (st.passes < p->max_passes) {
+        ahead_drop(&ra); /* the mapping may have been reset and perturbed */
         for (int32_t a = 1; a < n; a++) { /* :571-655; row n has no b > a */
             int32_t b, d;
-            if ((rc = bm_eval_swap_row(ctx, a, a + 1, n, BM_ROW_ARGMAX, &b, &d))) goto out;
+            if ((rc = eval_row_ahead(ctx, &ra, n, a, a + 1, -1, BM_ROW_ARGMAX, 1, &b, &d))) goto out;
             st.evals_rows++;
-            if (d <= 0) continue;
+            if (d <= 0) {
+                ahead_grow(&ra);
+                continue;
+            }
+            ahead_drop(&ra);
             int64_t change = 0;
             if ((rc = p->verify ? bm_apply_swap_scored(ctx, a, b, &change) : bm_apply_swap(ctx, a, b))) goto out;
             const int64_t verified = cur + change;

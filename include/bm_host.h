@@ -113,7 +113,10 @@ int bm_gen_sets(uint64_t seed, int32_t n, int32_t k, int64_t count, int32_t *out
  * its moves are searched (like the progress/dump blocks of
  * graph-alignment-optimized.c:519-540).  rows = rows processed including
  * this one, score = current score, improvements = accepted moves so far.
- * A non-zero return stops the driver. */
+ * A non-zero return stops the driver.  The hook may read the context (the
+ * mapping, the score) but must not change it: the drivers evaluate up to
+ * BM_ROW_GROUP_MAX rows ahead and use those results for as long as they apply
+ * no swap themselves. */
 typedef int (*bm_row_hook)(void *user, int64_t rows, int32_t row, int64_t score, int64_t improvements);
 
 /* Per-swap mismatch hook of the drivers: called when a verified swap's exact

@@ -100,6 +100,20 @@ int bm_eval_swaps(bm_ctx *ctx, const int32_t *a, const int32_t *b, int64_t count
 int bm_eval_swap_row(bm_ctx *ctx, int32_t a, int32_t b_lo, int32_t b_hi, int mode,
                      int32_t *best_b, int32_t *best_delta);
 
+/* bm_eval_swap_row for the consecutive rows a, a + 1, ..., a + count - 1
+ * (count <= BM_ROW_GROUP_MAX, a + count - 1 <= n), all under the current
+ * mapping and in one launch: row a over [b_lo, n], each later row r over
+ * [b_lo_next, n], or [r + 1, n] when b_lo_next < 0.  best_b[i], best_delta[i]
+ * are row a + i's result as bm_eval_swap_row gives it.  Replaces the next
+ * count evaluations of the m2 loops of graph-alignment-optimized.c:542-579
+ * (b_lo_next = 1) and alignment-omp.c:601-622 (b_lo_next = -1) for as long
+ * as no swap is accepted: a row's result holds only until a swap is applied,
+ * so the drivers use the later rows' results while the earlier rows accept
+ * nothing, which keeps the reference's move sequence. */
+#define BM_ROW_GROUP_MAX 16
+int bm_eval_swap_rows(bm_ctx *ctx, int32_t a, int32_t count, int32_t b_lo, int32_t b_lo_next, int mode,
+                      int32_t *best_b, int32_t *best_delta);
+
 /* bm_eval_swaps that allocates nothing, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, hosts
  * that forbid allocation).  No counterpart in the reference.  The pairs and

@@ -122,6 +122,38 @@ def test_apply_swap_patches_state():
             np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_row_group_kernel(case):
+    """bm_eval_swap_rows: rows a..a+count-1 under one mapping, the first over [lo, n], the
+    others over [lo_next, n] ([r+1, n] for lo_next < 0), each as the oracle's row search."""
+    n, g1, g2, mp = make_case(case)
+    o = ob.Oracle(n, g1, g2)
+    rng = np.random.default_rng(5)
+    imin = np.iinfo(np.int32).min
+    with engine_for(n, g1, g2, mp) as e:
+        groups = [(1, 16), (n - 15, 16), (n - 2, 3), (n, 1), (0, 5)]
+        groups += [(int(rng.integers(0, n - 6)), int(rng.integers(1, 8))) for _ in range(6)]
+        for a, count in groups:
+            lo = int(rng.integers(0, n + 2))
+            for mode in (bmatch.BM_ROW_FIRST_POSITIVE, bmatch.BM_ROW_ARGMAX):
+                for lo_next in (1, -1, 0):
+                    bs, ds_ = e.eval_swap_rows(a, count, lo, lo_next, mode)
+                    for i in range(count):
+                        r = a + i
+                        rlo = lo if i == 0 else (r + 1 if lo_next < 0 else lo_next)
+                        ds = {b: o.delta_b(mp, r, b) for b in range(max(rlo, 0), n + 1) if b != r}
+                        if mode == bmatch.BM_ROW_FIRST_POSITIVE:
+                            pos = [b for b in sorted(ds) if ds[b] > 0]
+                            want = (pos[0], ds[pos[0]]) if pos else (0, 0)
+                        else:
+                            bb = max(ds, key=lambda b: (ds[b], -b)) if ds else 0
+                            want = (bb, ds[bb]) if ds else (0, imin)
+                        assert (int(bs[i]), int(ds_[i])) == want, (a, count, lo, lo_next, mode, i)
+        for bad in ((1, 0), (1, bmatch.BM_ROW_GROUP_MAX + 1), (n - 2, 4), (-1, 2)):
+            with pytest.raises(bmatch.BMError):
+                e.eval_swap_rows(bad[0], bad[1], 1, 1)
+
+
 def test_first_improvement_trace_matches_oracle():
     inst = bmatch.Instance.powerlaw(5, 120, 1500)
     o = ob.Oracle(inst.n, inst.g1, inst.g2)
