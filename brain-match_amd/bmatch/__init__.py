@@ -163,7 +163,8 @@ class _Instance(C.Structure):
 
 class _DriveStats(C.Structure):
     _fields_ = [("sweeps", C.c_int64), ("rows", C.c_int64), ("accepted", C.c_int64), ("evals_rows", C.c_int64),
-                ("initial_score", C.c_int64), ("final_score", C.c_int64), ("mismatches", C.c_int64)]
+                ("initial_score", C.c_int64), ("final_score", C.c_int64), ("mismatches", C.c_int64),
+                ("rows_cleared", C.c_int64)]
 
 
 _ROW_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_int64)

@@ -112,6 +112,39 @@ static int eval_row_ahead(bm_ctx *ctx, rows_ahead *ra, int32_t n, int32_t a, int
     return BM_OK;
 }
 
+/* Rows since the last accepted swap after which a driver asks an all-pairs
+ * sweep (about 1.8 ms at n = 18,524, against about 25 us per row searched)
+ * whether any swap improves the current mapping.  If none does, no row can
+ * accept before the mapping changes, and the mapping changes only by an
+ * accept, so the rest of the pass is searched in vain: the reference finds
+ * nothing in those rows, and the driver only counts them.  Asked once per
+ * stretch without accepts. */
+#define BM_QUIET_ROWS 48
+
+typedef struct {
+    int64_t quiet; /* rows searched since the last accept */
+    int asked;     /* the sweep was asked in this stretch */
+    int cleared;   /* it found no improving swap: the pass's remaining rows are skipped */
+} quiet_state;
+
+static void quiet_accept(quiet_state *q) {
+    q->quiet = 0;
+    q->asked = 0;
+    q->cleared = 0;
+}
+
+/* called before searching a row: 1 if the row cannot accept */
+static int quiet_cleared(bm_ctx *ctx, quiet_state *q, int *rc) {
+    *rc = BM_OK;
+    if (q->cleared) return 1;
+    if (q->asked || q->quiet < BM_QUIET_ROWS) return 0;
+    q->asked = 1;
+    int32_t a, b, d;
+    if ((*rc = bm_sweep_all_pairs(ctx, &a, &b, &d, NULL, NULL, NULL))) return 0;
+    q->cleared = d <= 0;
+    return q->cleared;
+}
+
 /* graph-alignment-optimized.c:513-581 */
 int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_sweeps, int verify,
                                bm_row_hook hook, void *user, int32_t *acc, int64_t acc_cap,
@@ -126,6 +159,8 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
     pending_check pc = {0, 0};
     rows_ahead ra;
     ahead_drop(&ra);
+    quiet_state qs;
+    quiet_accept(&qs);
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
@@ -136,6 +171,12 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
                 break;
             }
             if (a > n) continue; /* padding rows: no edges, never accept */
+            if (quiet_cleared(ctx, &qs, &rc)) {
+                st.rows_cleared++;
+                continue;
+            }
+            if (rc || (rc = settle(ctx, &st, &pc))) return rc;
+            qs.quiet++;
             int32_t cursor = 1;
             while (cursor <= n) {
                 int32_t b, d;
@@ -150,14 +191,16 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
                 }
                 record(acc, acc_cap, st.accepted, a, b, d);
                 ahead_drop(&ra);
+                quiet_accept(&qs);
                 if ((rc = apply_and_verify(ctx, a, b, d, verify, &st, &pc))) return rc;
                 cursor = b + 1;
             }
             if ((rc = settle(ctx, &st, &pc))) return rc;
         }
-        st.sweeps++;
+        st.sweeps++; /* a clearance holds into the next pass: the mapping is unchanged */
         if (st.accepted == before) break;
     }
+    if ((rc = settle(ctx, &st, &pc))) return rc;
     if (stats) *stats = st;
     return BM_OK;
 }
@@ -175,6 +218,8 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
     pending_check pc = {0, 0};
     rows_ahead ra;
     ahead_drop(&ra);
+    quiet_state qs;
+    quiet_accept(&qs);
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
@@ -186,6 +231,12 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
                 break;
             }
             if (a == n) continue;
+            if (quiet_cleared(ctx, &qs, &rc)) {
+                st.rows_cleared++;
+                continue;
+            }
+            if (rc) return rc;
+            qs.quiet++;
             int32_t b, d;
             if ((rc = eval_row_ahead(ctx, &ra, n, a, a + 1, -1, BM_ROW_ARGMAX, 1, &b, &d))) return rc;
             if ((rc = settle(ctx, &st, &pc))) return rc;
@@ -193,6 +244,7 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
             if (d > 0) {
                 record(acc, acc_cap, st.accepted, a, b, d);
                 ahead_drop(&ra);
+                quiet_accept(&qs);
                 if ((rc = apply_and_verify(ctx, a, b, d, verify, &st, &pc))) return rc;
             } else {
                 ahead_grow(&ra);
@@ -252,10 +304,15 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *
     if ((rc = bm_score(ctx, &cur))) goto out;
     int64_t pass = 0, improvements = 0, last = 0;
     rows_ahead ra;
+    quiet_state qs;
+    quiet_accept(&qs);
     while (st.passes < p->max_passes) {
         ahead_drop(&ra); /* the mapping may have been reset and perturbed */
         for (int32_t a = 1; a < n; a++) { /* :571-655; row n has no b > a */
             int32_t b, d;
+            if (quiet_cleared(ctx, &qs, &rc)) continue;
+            if (rc) goto out;
+            qs.quiet++;
             if ((rc = eval_row_ahead(ctx, &ra, n, a, a + 1, -1, BM_ROW_ARGMAX, 1, &b, &d))) goto out;
             st.evals_rows++;
             if (d <= 0) {
@@ -263,6 +320,7 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *
                 continue;
             }
             ahead_drop(&ra);
+            quiet_accept(&qs);
             int64_t change = 0;
             if ((rc = p->verify ? bm_apply_swap_scored(ctx, a, b, &change) : bm_apply_swap(ctx, a, b))) goto out;
             const int64_t verified = cur + change;
@@ -287,6 +345,7 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *
             if ((rc = bm_score(ctx, &cur))) goto out;
             improvements = last = pass = 0;
             st.restarts++;
+            quiet_accept(&qs); /* a new mapping */
         }
         if (last < improvements) last = improvements; /* :674-676 */
     }

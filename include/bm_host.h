@@ -133,6 +133,9 @@ typedef struct {
     int64_t sweeps, rows, accepted, evals_rows;
     int64_t initial_score, final_score;
     int64_t mismatches; /* accepted deltas that disagreed with the rescore */
+    /* rows not searched one by one because an all-pairs sweep of the current
+     * mapping found no improving swap at all (counted in rows, not in evals_rows) */
+    int64_t rows_cleared;
 } bm_drive_stats;
 
 /* First-improvement sweep of graph-alignment-optimized.c:513-581 on the

@@ -57,6 +57,11 @@ def test_c1_first_improvement_to_convergence(c1):
         assert st["final_score"] == want["final_score"] == e.score() == o.score_opt(want["mapping"])
         assert st["mismatches"] == 0 and st["sweeps"] == want["sweeps"]
         assert len(acc) > 1000
+        # the last pass accepts nothing: once rows stop accepting (in that pass or at the end of the
+        # one before) an all-pairs sweep finds no improving swap, and the rest is counted without
+        # being searched
+        assert 0 < st["rows_cleared"] < 2 * inst.n
+        assert st["rows"] == st["sweeps"] * inst.n
 
 
 def test_c2_all_pairs_sweep(c2):
