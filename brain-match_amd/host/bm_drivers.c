@@ -6,6 +6,7 @@
 #include <limits.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "bm_host.h"
 #include "bm_private.h"
@@ -112,25 +113,41 @@ static int eval_row_ahead(bm_ctx *ctx, rows_ahead *ra, int32_t n, int32_t a, int
     return BM_OK;
 }
 
-/* Rows since the last accepted swap after which a driver asks an all-pairs
- * sweep (about 1.8 ms at n = 18,524, against about 25 us per row searched)
- * whether any swap improves the current mapping.  If none does, no row can
- * accept before the mapping changes, and the mapping changes only by an
- * accept, so the rest of the pass is searched in vain: the reference finds
- * nothing in those rows, and the driver only counts them.  Asked once per
- * stretch without accepts. */
+/* A stretch of rows without an accept asks one all-pairs sweep whether any
+ * swap improves the current mapping.  If none does, no row can accept before
+ * the mapping changes, and the mapping changes only by an accept, so the rest
+ * of the pass is searched in vain: the reference finds nothing in those rows,
+ * and the driver only counts them.  The sweep is asked once per stretch,
+ * after at least BM_QUIET_ROWS rows and once the stretch has taken as long as
+ * the last sweep did (about 1.8 ms at n = 18,524 on the sparse kernel, 15-26
+ * ms on the dense one, against about 25 us per row searched): a sweep that
+ * finds an improving swap then costs at most as much as the rows before it. */
 #define BM_QUIET_ROWS 48
 
 typedef struct {
-    int64_t quiet; /* rows searched since the last accept */
-    int asked;     /* the sweep was asked in this stretch */
-    int cleared;   /* it found no improving swap: the pass's remaining rows are skipped */
+    int64_t quiet;  /* rows searched since the last accept */
+    double t0;      /* when the stretch began */
+    double sweep_s; /* the last sweep's wall time */
+    int asked;      /* the sweep was asked in this stretch */
+    int cleared;    /* it found no improving swap: the pass's remaining rows are skipped */
 } quiet_state;
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 static void quiet_accept(quiet_state *q) {
     q->quiet = 0;
+    q->t0 = now_s();
     q->asked = 0;
     q->cleared = 0;
+}
+
+static void quiet_init(quiet_state *q) {
+    q->sweep_s = 0.0;
+    quiet_accept(q);
 }
 
 /* called before searching a row: 1 if the row cannot accept */
@@ -138,9 +155,12 @@ static int quiet_cleared(bm_ctx *ctx, quiet_state *q, int *rc) {
     *rc = BM_OK;
     if (q->cleared) return 1;
     if (q->asked || q->quiet < BM_QUIET_ROWS) return 0;
+    const double t = now_s();
+    if (t - q->t0 < q->sweep_s) return 0;
     q->asked = 1;
     int32_t a, b, d;
     if ((*rc = bm_sweep_all_pairs(ctx, &a, &b, &d, NULL, NULL, NULL))) return 0;
+    q->sweep_s = now_s() - t;
     q->cleared = d <= 0;
     return q->cleared;
 }
@@ -160,7 +180,7 @@ int bm_drive_first_improvement(bm_ctx *ctx, int32_t rows_per_sweep, int64_t max_
     rows_ahead ra;
     ahead_drop(&ra);
     quiet_state qs;
-    quiet_accept(&qs);
+    quiet_init(&qs);
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
@@ -219,7 +239,7 @@ int bm_drive_row_best(bm_ctx *ctx, int64_t max_sweeps, int verify, bm_row_hook h
     rows_ahead ra;
     ahead_drop(&ra);
     quiet_state qs;
-    quiet_accept(&qs);
+    quiet_init(&qs);
     int stop = 0;
     while (!stop && (max_sweeps <= 0 || st.sweeps < max_sweeps)) {
         int64_t before = st.accepted;
@@ -305,7 +325,7 @@ int bm_drive_row_best_restart(bm_ctx *ctx, const bm_restart_params *p, int32_t *
     int64_t pass = 0, improvements = 0, last = 0;
     rows_ahead ra;
     quiet_state qs;
-    quiet_accept(&qs);
+    quiet_init(&qs);
     while (st.passes < p->max_passes) {
         ahead_drop(&ra); /* the mapping may have been reset and perturbed */
         for (int32_t a = 1; a < n; a++) { /* :571-655; row n has no b > a */

@@ -178,6 +178,26 @@ def test_first_improvement_with_loops_and_dups():
         assert st["final_score"] == ref["final_score"]
 
 
+@pytest.mark.parametrize("unmapped,bijective", [(0.2, True), (0.1, False)], ids=["unmapped", "nonbijective"])
+def test_first_improvement_dense_route_to_convergence(unmapped, bijective):
+    """Mappings the sparse kernels do not take (unmapped or colliding images): rows and the
+    quiet-stretch sweep go through the dense kernels; the moves stay the oracle's."""
+    n = 160
+    g1 = random_multigraph(31, n, 1600, 0.1, 0.05, 30, False)
+    g2 = random_multigraph(32, n, 1600, 0.1, 0.05, 30, False)
+    mp = random_mapping(33, n, unmapped, bijective)
+    o = ob.Oracle(n, g1, g2)
+    ref = o.first_improvement(mp)
+    with engine_for(n, g1, g2, mp) as e:
+        acc, st = e.first_improvement(verify=True)
+        np.testing.assert_array_equal(acc, ref["accepted"])
+        np.testing.assert_array_equal(e.mapping(), ref["mapping"])
+        assert st["final_score"] == ref["final_score"] and st["sweeps"] == ref["sweeps"]
+        assert st["rows_cleared"] > 0 and st["rows"] == st["sweeps"] * n
+        if not bijective:
+            assert e.sweep_config()[0] != 2  # colliding images: not the sparse kernel
+
+
 def test_row_best_pass_matches_oracle():
     inst = bmatch.Instance.powerlaw(6, 150, 2000)
     o = ob.Oracle(inst.n, inst.g1, inst.g2)
