@@ -258,6 +258,14 @@ def roofline(kname, kernel_ms, alg):
                                           "TCC_EA0_RDREQ_DRAM_32B_sum + TCC_EA0_WRREQ_WRITE_DRAM_32B_sum (x 32 B) of "
                                           f"this kernel on this workload (kernel source {pmc.get('source', '?')})",
                         "limiters": pmc.get("limiters")})
+        lim = pmc.get("limiters") or {}
+        if lim:
+            out["limited_by"] = (
+                "no unit saturated: the traffic past L2 (MALL hits included, so an upper bound on HBM bytes) at "
+                f"{gbs / HBM_PEAK_GBS:.0%} of the HBM peak, LDS active {lim.get('lds_active_frac', 0):.0%} of the "
+                f"time ({lim.get('lds_bank_conflict_over_active', 0):.0%} of that bank conflicts from the "
+                f"scattered row adds), VALU about {lim.get('valu_busy_est', 0):.0%} busy, L2 hit rate "
+                f"{lim.get('l2_hit_rate', 0):.0%}; the per-word list walk (load, min, LDS add) sets the time")
     return out
 
 
