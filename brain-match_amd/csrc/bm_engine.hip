@@ -855,8 +855,8 @@ static void free_graph_buffers(bm_ctx *c) {
     dev_free(c, (void **)&c->crw, (size_t)c->ncomb * 8);
     dev_free(c, (void **)&c->cmid, N1 * 4);
     dev_free(c, (void **)&c->cneg, N1 * 4);
-    dev_free(c, (void **)&c->rowacc, N1 * 4);
-    dev_free(c, (void **)&c->grpacc, (size_t)BM_ROW_GROUP_MAX * N1 * 4);
+    dev_free(c, (void **)&c->rowpart, c->rowpart_bytes);
+    c->rowpart_bytes = 0;
     dev_free(c, (void **)&c->grpres, (size_t)BM_ROW_GROUP_MAX * 8);
     dev_free(c, (void **)&c->ddptr, (N1 + 1) * 4);
     dev_free(c, (void **)&c->ddmid, N1 * 4);
@@ -1232,8 +1232,6 @@ int bm_upload_graphs(bm_ctx *c, int32_t n, const int32_t *s1, const int32_t *d1,
         }
         if ((rc = upload(c, &c->cmid, mid))) goto fail;
         if ((rc = upload(c, &c->cneg, neg))) goto fail;
-        std::vector<int32_t> zeros(N1, 0);
-        if ((rc = upload(c, &c->rowacc, zeros))) goto fail;
         // (neighbour, weight) pairs and the flat out-line list, from what is on the device now
         if ((rc = dev_alloc(c, (void **)&c->crw, (size_t)nc * 8))) goto fail;
         if ((rc = dev_alloc(c, (void **)&c->esrc, (size_t)e1 * 4))) goto fail;

@@ -70,8 +70,8 @@ struct bm_ctx {
     int32_t *crw = nullptr;     // (r, w) pairs of the comb lists, 8 bytes per entry
     int32_t *cmid = nullptr;    // cmid[v]: first in entry of comb(v)
     int32_t *cneg = nullptr;    // cneg[v] = sum over comb(v) of min(w, 0)
-    int32_t *rowacc = nullptr;  // (n+1)-int accumulator of the sparse row kernel, kept zeroed
-    int32_t *grpacc = nullptr;  // kRowGroupMax accumulators of (n+1) ints for row groups (lazy), kept zeroed
+    int32_t *rowpart = nullptr;  // part rows of the sparse row kernel: rows x parts x (n+1) ints (lazy, grows)
+    size_t rowpart_bytes = 0;
     unsigned long long *grpres = nullptr;   // kRowGroupMax keys of a row group (lazy)
     unsigned long long *hgrpres = nullptr;  // pinned host copy of those keys (lazy)
     int32_t *ddptr = nullptr, *ddmid = nullptr, *ddr = nullptr, *ddw = nullptr;

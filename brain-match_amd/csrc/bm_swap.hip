@@ -205,17 +205,23 @@ __global__ __launch_bounds__(256) void k_row_eval(int a, int lo, int hi, int mod
 // through p^-1: Q[s][a] != 0 for s = p^-1(g), g an in-neighbour of p(a), and
 // Q[n+1+d][a] != 0 for d = p^-1(g), g an out-neighbour; the comb entries
 // reading row s are the G1 out-lines s->b, those reading row n+1+d the G1
-// in-lines b->d.  k_row_expand gives each such list head kWaves waves
-// (comb(a) entries, then in- and out-neighbours of p(a)) per slice of its
-// list (grid y: kRowSlices slices, so that a hub's long list is walked by
-// several workgroups at once) and adds the contributions for b in [lo, hi]
-// into a device row buffer with atomics; its block (0, 0) also adds Corr(a,b)
-// for adjacent b and seeds the result key.
-// k_row_final forms Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a) - X(b,b),
-// reduces it to the first positive or arg-max key, and zeroes the buffer for
-// the next call.
-template <int kWaves>
-__global__ __launch_bounds__(256) void k_row_expand(int a0, int lo0, int lo_next, int hi, int mode, int N1,
+// in-lines b->d.  k_row_final forms Delta_B(a,b) = X(a,b) + X(b,a) - X(a,a)
+// - X(b,b) and reduces it to the first positive or arg-max key.
+//
+// Deltas of rows a0 + y (grid y), the first over [lo0, hi], the others over
+// [lo_next, hi] ([r + 1, hi] for lo_next < 0).  A row's list heads -- the comb
+// entries of a (their G2 lists give X(a, .)) and the G2 neighbours of p(a)
+// (the G1 lists of their preimages give X(., a)) -- go round-robin to the 16
+// waves of each of the row's `parts` workgroups (grid x = parts x column
+// windows); a wave walks its heads' lists a word per lane and adds into the
+// workgroup's LDS row, which is then stored whole as one part row.  A C2 row
+// has about 2 x 10^5 list words: as device-scope atomics into one shared row
+// they bound the row at about 24 us, in LDS they cost a few.  Rows wider than
+// the LDS budget are cut into column windows, each walking all heads and
+// keeping the words that land in it.  k_row_final sums the parts.
+constexpr int kRowWin = 36 * 1024;  // ints of one column window (144 KB of LDS)
+
+__global__ __launch_bounds__(1024) void k_row_parts(int a0, int lo0, int lo_next, int hi, int mode, int N1,
                                                     const int32_t *__restrict__ cptr,
                                                     const int32_t *__restrict__ cmid, const int2 *__restrict__ crw,
                                                     const int32_t *__restrict__ cdw, const int32_t *__restrict__ map,
@@ -223,117 +229,121 @@ __global__ __launch_bounds__(256) void k_row_expand(int a0, int lo0, int lo_next
                                                     const uint32_t *__restrict__ g2oy,
                                                     const int32_t *__restrict__ g2iptr,
                                                     const uint32_t *__restrict__ g2iy, const int16_t *__restrict__ Q,
-                                                    int ld, int *__restrict__ acc, int64_t acc_stride,
+                                                    int ld, int parts, int win, int32_t *__restrict__ part,
                                                     unsigned long long *__restrict__ result) {
-    constexpr int kLanes = 64 * kWaves, kHeads = 4 / kWaves;  // lanes per head, heads per block
-    // grid z: row a0 + z over [lo, hi], lo = lo0 for the first row and lo_next (-1: the
-    // row's own index + 1) for the others; each row has its own accumulator and key
-    const int a = a0 + (int)blockIdx.z;
-    const int lo = blockIdx.z == 0 ? lo0 : (lo_next < 0 ? a + 1 : lo_next);
-    acc += (int64_t)blockIdx.z * acc_stride;
-    result += blockIdx.z;
-    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes), lane = threadIdx.x % kLanes;
+    extern __shared__ __align__(16) int acc[];
+    const int x = blockIdx.x % parts, wlo = (blockIdx.x / parts) * win, whi = min(N1, wlo + win);
+    const int row = blockIdx.y, a = a0 + row;
+    const int lo = max(row == 0 ? lo0 : (lo_next < 0 ? a + 1 : lo_next), wlo), top = min(hi, whi - 1);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int i = tid; i < whi - wlo; i += 1024) acc[i] = 0;
+    if (blockIdx.x == 0 && tid == 0) result[row] = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
+    __syncthreads();
     const int ca0 = cptr[a], dega = cptr[a + 1] - ca0;
     const int pa = map[a];
     const int i0 = g2iptr[pa], nin = g2iptr[pa + 1] - i0;
     const int o0 = g2optr[pa], nout = g2optr[pa + 1] - o0;
-    const int sl = blockIdx.y, ns = gridDim.y;  // this block's slice of each list
-    if (blockIdx.x == 0) {
-        if (sl != 0) return;
-        if (threadIdx.x == 0) *result = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
+    const int H = dega + nin + nout;
+    if (x == 0) {  // Corr(a, b) of the lines between a and b, one comb entry each
         const int paa = (int)Q[(size_t)a * ld + a];
-        for (int e = ca0 + threadIdx.x; e < ca0 + dega; e += 256) {
+        for (int e = ca0 + tid; e < ca0 + dega; e += 1024) {
             const int2 rw = crw[e];
             const bool out = rw.x >= N1;
             const int b = out ? rw.x - N1 : rw.x;
-            if (b == a || b < lo || b > hi) continue;
+            if (b == a || b < lo || b > top) continue;
             const int pbb = (int)Q[(size_t)b * ld + b], pab = (int)Q[(size_t)a * ld + b],
                       pba = (int)Q[(size_t)b * ld + a];
-            atomicAdd(&acc[b], pair_corr(out, rw.y, cdw[e], paa, pbb, pab, pba));
-        }
-        return;
-    }
-    const int h = (blockIdx.x - 1) * kHeads + slot;
-    if (h < dega) {  // X(a, b): comb entry of a, its G2 list
-        const int2 rw = crw[ca0 + h];
-        const bool out = rw.x >= N1;
-        const int g = map[out ? rw.x - N1 : rw.x];
-        const uint32_t *L = out ? g2iy : g2oy;
-        const int b0 = out ? g2iptr[g] : g2optr[g], b1 = out ? g2iptr[g + 1] : g2optr[g + 1];
-        const int l0 = b0 + (int)((long long)(b1 - b0) * sl / ns), l1 = b0 + (int)((long long)(b1 - b0) * (sl + 1) / ns);
-        const int w = rw.y, wneg = min(w, 0);
-        int j = l0 + lane;
-        for (; j + 3 * kLanes < l1; j += 4 * kLanes) {  // four list words in flight per lane
-            uint32_t yv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) yv[u] = L[j + kLanes * u];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int y = yv[u] >> 16;
-                if (y != 0xFFFF && y >= lo && y <= hi)
-                    atomicAdd(&acc[y], min(w, (int)(int16_t)(yv[u] & 0xffff)) - wneg);
-            }
-        }
-        for (; j < l1; j += kLanes) {
-            const uint32_t yv = L[j];
-            const int y = yv >> 16;
-            if (y != 0xFFFF && y >= lo && y <= hi) atomicAdd(&acc[y], min(w, (int)(int16_t)(yv & 0xffff)) - wneg);
-        }
-    } else if (h < dega + nin + nout) {  // X(b, a): a G2 neighbour of p(a), the G1 lines of its preimage
-        const bool in = h < dega + nin;
-        const uint32_t yv = in ? g2iy[i0 + h - dega] : g2oy[o0 + h - dega - nin];
-        const int v = yv >> 16;
-        if (v == 0xFFFF) return;
-        const int val = (int)(int16_t)(yv & 0xffff);
-        // in-neighbour: rows s = v of P, read by the out-lines v->b (in entries of b);
-        // out-neighbour: rows n+1+v of P^T, read by the in-lines b->v (out entries of b)
-        const int b0 = in ? cptr[v] : cmid[v], b1 = in ? cmid[v] : cptr[v + 1];
-        const int l0 = b0 + (int)((long long)(b1 - b0) * sl / ns), l1 = b0 + (int)((long long)(b1 - b0) * (sl + 1) / ns);
-        int e = l0 + lane;
-        for (; e + 3 * kLanes < l1; e += 4 * kLanes) {
-            int2 rw[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) rw[u] = crw[e + kLanes * u];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int b = in ? rw[u].x - N1 : rw[u].x;
-                if (b >= lo && b <= hi) atomicAdd(&acc[b], min(rw[u].y, val) - min(rw[u].y, 0));
-            }
-        }
-        for (; e < l1; e += kLanes) {
-            const int2 rw = crw[e];
-            const int b = in ? rw.x - N1 : rw.x;
-            if (b >= lo && b <= hi) atomicAdd(&acc[b], min(rw.y, val) - min(rw.y, 0));
+            atomicAdd(&acc[b - wlo], pair_corr(out, rw.y, cdw[e], paa, pbb, pab, pba));
         }
     }
+    for (int h = x * 16 + wave; h < H; h += parts * 16) {
+        if (h < dega) {  // X(a, b): comb entry of a, its G2 list
+            const int2 rw = crw[ca0 + h];
+            const bool out = rw.x >= N1;
+            const int g = map[out ? rw.x - N1 : rw.x];
+            const uint32_t *L = out ? g2iy : g2oy;
+            const int b0 = out ? g2iptr[g] : g2optr[g], b1 = out ? g2iptr[g + 1] : g2optr[g + 1];
+            const int w = rw.y, wneg = min(w, 0);
+            int j = b0 + lane;
+            for (; j + 192 < b1; j += 256) {  // four list words in flight per lane
+                uint32_t yv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) yv[u] = L[j + 64 * u];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int y = yv[u] >> 16;
+                    if (y != 0xFFFF && y >= lo && y <= top)
+                        atomicAdd(&acc[y - wlo], min(w, (int)(int16_t)(yv[u] & 0xffff)) - wneg);
+                }
+            }
+            for (; j < b1; j += 64) {
+                const uint32_t yv = L[j];
+                const int y = yv >> 16;
+                if (y != 0xFFFF && y >= lo && y <= top)
+                    atomicAdd(&acc[y - wlo], min(w, (int)(int16_t)(yv & 0xffff)) - wneg);
+            }
+        } else {  // X(b, a): a G2 neighbour of p(a), the G1 lines of its preimage
+            const bool in = h < dega + nin;
+            const uint32_t yv = in ? g2iy[i0 + h - dega] : g2oy[o0 + h - dega - nin];
+            const int v = yv >> 16;
+            if (v == 0xFFFF) continue;
+            const int val = (int)(int16_t)(yv & 0xffff);
+            // in-neighbour: rows s = v of P, read by the out-lines v->b (in entries of b);
+            // out-neighbour: rows n+1+v of P^T, read by the in-lines b->v (out entries of b)
+            const int b0 = in ? cptr[v] : cmid[v], b1 = in ? cmid[v] : cptr[v + 1];
+            int e = b0 + lane;
+            for (; e + 192 < b1; e += 256) {
+                int2 rw[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) rw[u] = crw[e + 64 * u];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int b = in ? rw[u].x - N1 : rw[u].x;
+                    if (b >= lo && b <= top) atomicAdd(&acc[b - wlo], min(rw[u].y, val) - min(rw[u].y, 0));
+                }
+            }
+            for (; e < b1; e += 64) {
+                const int2 rw = crw[e];
+                const int b = in ? rw.x - N1 : rw.x;
+                if (b >= lo && b <= top) atomicAdd(&acc[b - wlo], min(rw.y, val) - min(rw.y, 0));
+            }
+        }
+    }
+    __syncthreads();
+    int32_t *dst = part + ((int64_t)row * parts + x) * N1;
+    for (int i = wlo + tid; i < whi; i += 1024) dst[i] = acc[i - wlo];
 }
 
 __global__ __launch_bounds__(256) void k_row_final(int a0, int lo0, int lo_next, int lo_base, int hi, int mode,
-                                                   int *__restrict__ acc, int64_t acc_stride,
+                                                   const int32_t *__restrict__ part, int parts, int N1,
                                                    const int32_t *__restrict__ cneg,
                                                    const int32_t *__restrict__ diag,
                                                    unsigned long long *__restrict__ result,
                                                    int32_t *__restrict__ store) {
     __shared__ unsigned long long red[4];
-    // grid y: the rows of k_row_expand's grid z; the blocks cover [lo_base, hi], lo_base
+    // grid y: the rows of k_row_parts' grid y; the blocks cover [lo_base, hi], lo_base
     // the lowest of the rows' lower ends
     const int a = a0 + (int)blockIdx.y;
     const int lo = blockIdx.y == 0 ? lo0 : (lo_next < 0 ? a + 1 : lo_next);
-    acc += (int64_t)blockIdx.y * acc_stride;
+    part += (int64_t)blockIdx.y * parts * N1;  // this row's part rows
     result += blockIdx.y;
     const int b = lo_base + blockIdx.x * 256 + threadIdx.x;
+    auto sum_parts = [&]() {
+        int x = 0;
+        for (int p = 0; p < parts; p++) x += part[(int64_t)p * N1 + b];
+        return x;
+    };
     if (store) {  // whole row (one row per launch): every delta, no reduction
         if (b >= lo && b <= hi) {
-            const int x = acc[b];
-            acc[b] = 0;
+            const int x = sum_parts();
             store[b] = b == a || b == 0 ? 0 : x + cneg[a] + cneg[b] - diag[a] - diag[b];
         }
         return;
     }
     unsigned long long key = mode == BM_ROW_FIRST_POSITIVE ? ~0ull : 0ull;
     if (b >= lo && b <= hi) {
-        const int x = acc[b];
-        acc[b] = 0;
+        const int x = sum_parts();
         if (b != a) {
             const int d = x + cneg[a] + cneg[b] - diag[a] - diag[b];
             if (mode == BM_ROW_FIRST_POSITIVE) {
@@ -1405,35 +1415,33 @@ extern "C" int bm_eval_swaps_dense(bm_ctx *c, const int32_t *a, const int32_t *b
     return BM_OK;
 }
 
-// One row's list heads: BM_ROW_WAVES = 1 gives each head one wave (four heads
-// per workgroup), 4 (the default) a whole workgroup, which spreads the long
-// lists of hub neighbours over 256 lanes.
 // dense: the dense-form lists in place of the comb lists.
-constexpr int kRowSlices = 4;  // workgroups per list head
 
-// rows a .. a + rows - 1 (see k_row_expand), accumulators acc + i * N1
-static int launch_row_expand(bm_ctx *c, bool dense, int a, int lo, int lo_next, int hi, int mode, int32_t *acc,
-                             unsigned long long *res, int rows = 1) {
-    static const int waves = [] {
-        const char *e = getenv("BM_ROW_WAVES");
-        return e && atoi(e) == 1 ? 1 : 4;
-    }();
+// rows a .. a + rows - 1 of k_row_parts; *parts_out = part rows per row, in c->rowpart
+static int launch_row_parts(bm_ctx *c, bool dense, int a, int lo, int lo_next, int hi, int mode,
+                            unsigned long long *res, int rows, int *parts_out) {
     int heads = 0;
     for (int i = 0; i < rows; i++)
         heads = std::max(heads, (dense ? c->hdfdeg[a + i] : c->hcdeg[a + i]) + c->hg2deg[c->hmap[a + i]]);
-    const int per = 4 / waves;
+    const int N1 = c->N1;
+    // about one head per wave, at most 64 workgroups per row and window
+    const int parts = std::min(64, std::max(1, (heads + 15) / 16));
+    const int win = std::min(N1, kRowWin), wins = (N1 + win - 1) / win;
+    const size_t need = (size_t)rows * parts * N1 * 4;
+    if (c->rowpart_bytes < need) {
+        dev_free(c, (void **)&c->rowpart, c->rowpart_bytes);
+        c->rowpart_bytes = 0;
+        if (int rc = dev_alloc(c, (void **)&c->rowpart, need)) return rc;
+        c->rowpart_bytes = need;
+    }
     const int32_t *lptr = dense ? c->dfptr : c->cptr, *lmid = dense ? c->dfmid : c->cmid;
     const int2 *lrw = (const int2 *)(dense ? c->dfrw : c->crw);
     const int32_t *ldw = dense ? c->dfw : c->cdw;
-#define BM_ROW_EXPAND(W)                                                                                         \
-    BM_LAUNCH(c, "row_expand",                                                                                  \
-              hipLaunchKernelGGL(k_row_expand<W>, dim3(1 + (heads + per - 1) / per, kRowSlices, rows), dim3(256),   \
-                                 0, c->stream, a, lo, lo_next,                                                      \
-                                 hi, mode, c->N1, lptr, lmid, lrw, ldw, c->map, c->g2optr, c->g2oy, c->g2iptr,       \
-                                 c->g2iy, c->Q, c->ld, acc, (int64_t)c->N1, res))
-    if (waves == 1) BM_ROW_EXPAND(1);
-    else BM_ROW_EXPAND(4);
-#undef BM_ROW_EXPAND
+    BM_LAUNCH(c, "row_expand",
+              hipLaunchKernelGGL(k_row_parts, dim3(parts * wins, rows), dim3(1024), (size_t)win * 4, c->stream, a, lo,
+                                 lo_next, hi, mode, N1, lptr, lmid, lrw, ldw, c->map, c->g2optr, c->g2oy, c->g2iptr,
+                                 c->g2iy, c->Q, c->ld, parts, win, c->rowpart, res));
+    *parts_out = parts;
     return BM_OK;
 }
 
@@ -1459,10 +1467,11 @@ extern "C" int bm_eval_swap_row(bm_ctx *c, int32_t a, int32_t b_lo, int32_t b_hi
         if (c->g2y_dirty) {
             if ((rc = launch_refresh_g2y(c))) return rc;
         }
-        if ((rc = launch_row_expand(c, false, a, b_lo, -1, b_hi, mode, c->rowacc, res))) return rc;
+        int parts;
+        if ((rc = launch_row_parts(c, false, a, b_lo, -1, b_hi, mode, res, 1, &parts))) return rc;
         BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((b_hi - b_lo + 256) / 256), dim3(256), 0,
-                                                     c->stream, a, b_lo, -1, b_lo, b_hi, mode, c->rowacc,
-                                                     (int64_t)c->N1, c->cneg, c->diag, res, nullptr));
+                                                     c->stream, a, b_lo, -1, b_lo, b_hi, mode, c->rowpart, parts,
+                                                     c->N1, c->cneg, c->diag, res, nullptr));
     } else {
         BM_HIP(hipMemsetAsync(res, mode == BM_ROW_FIRST_POSITIVE ? 0xFF : 0x00, 8, c->stream));
         const int blocks = (b_hi - b_lo + kRowCh) / kRowCh;
@@ -1520,27 +1529,23 @@ extern "C" int bm_eval_swap_rows(bm_ctx *c, int32_t a, int32_t count, int32_t b_
     BM_HIP(hipSetDevice(c->device));
     int rc = ensure_diag(c);
     if (rc) return rc;
-    const size_t N1 = (size_t)c->N1;
-    if (!c->grpacc) {
-        if ((rc = dev_alloc(c, (void **)&c->grpacc, (size_t)BM_ROW_GROUP_MAX * N1 * 4))) return rc;
-        BM_HIP(hipMemsetAsync(c->grpacc, 0, (size_t)BM_ROW_GROUP_MAX * N1 * 4, c->stream));
-    }
     if (!c->grpres && (rc = dev_alloc(c, (void **)&c->grpres, (size_t)BM_ROW_GROUP_MAX * 8))) return rc;
     if (!c->hgrpres) {
         std::lock_guard<std::mutex> g(sync_mutex());  // host allocations may wait for the device
         BM_HIP(hipHostMalloc((void **)&c->hgrpres, (size_t)BM_ROW_GROUP_MAX * 8, hipHostMallocDefault));
     }
     if (c->g2y_dirty && (rc = launch_refresh_g2y(c))) return rc;
-    // rows whose range is empty keep their neutral key (set by the expand kernel) and
-    // add nothing: k_row_expand and k_row_final test b against the row's own range
+    // rows whose range is empty keep their neutral key (set by k_row_parts) and
+    // add nothing: k_row_parts and k_row_final test b against the row's own range
     int lo_base = n + 1;
     for (int i = 0; i < count; i++) lo_base = std::min(lo_base, lo_of(i));
     const int first_lo = lo_of(0), next_lo = b_lo_next < 0 ? -1 : std::max(0, b_lo_next);
-    if ((rc = launch_row_expand(c, false, a, first_lo, next_lo, n, mode, c->grpacc, c->grpres, count))) return rc;
+    int parts;
+    if ((rc = launch_row_parts(c, false, a, first_lo, next_lo, n, mode, c->grpres, count, &parts))) return rc;
     if (lo_base <= n)
         BM_LAUNCH(c, "row_final", hipLaunchKernelGGL(k_row_final, dim3((n - lo_base + 256) / 256, count), dim3(256),
-                                                     0, c->stream, a, first_lo, next_lo, lo_base, n, mode, c->grpacc,
-                                                     (int64_t)N1, c->cneg, c->diag, c->grpres, nullptr));
+                                                     0, c->stream, a, first_lo, next_lo, lo_base, n, mode, c->rowpart,
+                                                     parts, c->N1, c->cneg, c->diag, c->grpres, nullptr));
     BM_HIP(hipMemcpyAsync(c->hgrpres, c->grpres, (size_t)count * 8, hipMemcpyDeviceToHost, c->stream));
     BM_HIP(hipStreamSynchronize(c->stream));
     for (int i = 0; i < count; i++) decode_row_key(c->hgrpres[i], mode, best_b + i, best_delta + i);
@@ -1570,7 +1575,7 @@ extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32
     const size_t N1 = (size_t)c->N1;
     if ((rc = ensure_stage(c, (size_t)count * N1 * 4))) return rc;
     int32_t *out = (int32_t *)c->stage;
-    unsigned long long *res = c->keys + c->N1 + 1;  // written by the expand kernel, not read here
+    unsigned long long *res = c->keys + c->N1 + 1;  // written by k_row_parts, not read here
     if (c->sparse_ok && c->g2y_dirty && (rc = launch_refresh_g2y(c))) return rc;
     // the dense form: X(v,v) over its own lists, from the current P (kept current by later moves)
     if (dense && !c->dfdiag_valid) {
@@ -1585,9 +1590,10 @@ extern "C" int bm_eval_rows(bm_ctx *c, const int32_t *rows, int32_t count, int32
         const int a = rows[i];
         int32_t *row = out + (size_t)i * N1;
         if (c->sparse_ok) {
-            if ((rc = launch_row_expand(c, dense, a, 0, -1, n, BM_ROW_ARGMAX, c->rowacc, res))) return rc;
+            int parts;
+            if ((rc = launch_row_parts(c, dense, a, 0, -1, n, BM_ROW_ARGMAX, res, 1, &parts))) return rc;
             BM_LAUNCH(c, "row_store", hipLaunchKernelGGL(k_row_final, dim3((n + 256) / 256), dim3(256), 0, c->stream,
-                                                         a, 0, -1, 0, n, BM_ROW_ARGMAX, c->rowacc, (int64_t)c->N1,
+                                                         a, 0, -1, 0, n, BM_ROW_ARGMAX, c->rowpart, parts, c->N1,
                                                          lneg, ldiag, res, row));
         } else {
             const int blocks = (n + kRowCh) / kRowCh;
