@@ -329,10 +329,15 @@ __global__ __launch_bounds__(256) void k_row_final(int a0, int lo0, int lo_next,
     part += (int64_t)blockIdx.y * parts * N1;  // this row's part rows
     result += blockIdx.y;
     const int b = lo_base + blockIdx.x * 256 + threadIdx.x;
-    auto sum_parts = [&]() {
-        int x = 0;
-        for (int p = 0; p < parts; p++) x += part[(int64_t)p * N1 + b];
-        return x;
+    auto sum_parts = [&]() {  // eight part rows' loads in flight
+        int x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int p = 0;
+        for (; p + 8 <= parts; p += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] += part[(int64_t)(p + u) * N1 + b];
+        }
+        for (; p < parts; p++) x[0] += part[(int64_t)p * N1 + b];
+        return ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
     };
     if (store) {  // whole row (one row per launch): every delta, no reduction
         if (b >= lo && b <= hi) {
@@ -1424,8 +1429,8 @@ static int launch_row_parts(bm_ctx *c, bool dense, int a, int lo, int lo_next, i
     for (int i = 0; i < rows; i++)
         heads = std::max(heads, (dense ? c->hdfdeg[a + i] : c->hcdeg[a + i]) + c->hg2deg[c->hmap[a + i]]);
     const int N1 = c->N1;
-    // about one head per wave, at most 64 workgroups per row and window
-    const int parts = std::min(64, std::max(1, (heads + 15) / 16));
+    // about two heads per wave, at most 64 workgroups per row and window
+    const int parts = std::min(64, std::max(1, (heads + 31) / 32));
     const int win = std::min(N1, kRowWin), wins = (N1 + win - 1) / win;
     const size_t need = (size_t)rows * parts * N1 * 4;
     if (c->rowpart_bytes < need) {
